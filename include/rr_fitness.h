@@ -1,0 +1,170 @@
+/* rr_fitness.h -- C ABI of the MI355X candidate-fitness library (librr_hip.so).
+ *
+ * The reference has no internal seam here: its hot path is a private chain of
+ * member functions in one translation unit,
+ *     local_search (rils_rols_cpp/rils_rols_cpp.cpp:601-643)
+ *       -> tune_constants (445-518) -> fitness (520-541) -> R2/RMSE (40-49)
+ *       -> node::evaluate_all (rils_rols_cpp/node.cpp:5-95)
+ *       -> Eigen ColPivHouseholderQR::solve (rils_rols_cpp.cpp:484).
+ * This header is the boundary that replaces that chain with one batched call
+ * per local-search sweep.  The host side (tree rewrites, factor extraction,
+ * OLS-tree construction, acceptance) stays in C++ above it; everything that
+ * touches the n samples happens on the GPU below it.
+ *
+ * Rules: plain C types only; caller-owned host buffers; library-owned device
+ * memory keyed by handles; every function returns 0 on success and a negative
+ * RR_E_* code on failure, with a message from rr_last_error() (per thread).
+ * A context is used by one host thread at a time.
+ */
+#ifndef RR_FITNESS_H
+#define RR_FITNESS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes */
+#define RR_OK 0
+#define RR_E_ARG (-1)     /* invalid argument (sizes, handles, program text) */
+#define RR_E_HIP (-2)     /* HIP runtime error (no device, launch failure, ...) */
+#define RR_E_NOMEM (-3)   /* device allocation failed */
+#define RR_E_LIMIT (-4)   /* a candidate exceeds a compiled limit (columns, stack) */
+
+/* ---- candidate programs ------------------------------------------------
+ * Each expression tree is compiled on the host to postfix bytecode for an
+ * accumulator machine with a small operand stack.  One 32-bit instruction:
+ *     bits 0..7   opcode
+ *     bits 8..31  operand: feature index (VAR forms) or constant-pool index
+ * A binary opcode is RR_OP_BIN + 8*bop + mode, where bop is one of RR_BOP_*
+ * and mode says where the two operands come from (acc = accumulator):
+ *     RR_M_S  : left = pop(), right = acc        RR_M_SR : left = acc, right = pop()
+ *     RR_M_V  : left = acc, right = X[arg]       RR_M_VR : left = X[arg], right = acc
+ *     RR_M_C  : left = acc, right = K[arg]       RR_M_CR : left = K[arg], right = acc
+ * Operators and their per-sample semantics are those of node::evaluate_inner
+ * (rils_rols_cpp/node.cpp:23-95).  POW is not representable: the reference
+ * never generates it (setup_nodes, rils_rols_cpp.cpp:121-148). */
+#define RR_OP_LDV 1u    /* acc = X[arg]       (node.cpp:30-32) */
+#define RR_OP_LDC 2u    /* acc = K[arg]       (node.cpp:25-29) */
+#define RR_OP_PUSH 3u   /* push(acc) */
+#define RR_OP_SIN 4u    /* node.cpp:41-52 */
+#define RR_OP_COS 5u
+#define RR_OP_LN 6u
+#define RR_OP_EXP 7u
+#define RR_OP_SQRT 8u
+#define RR_OP_SQR 9u
+#define RR_OP_BIN 16u
+
+#define RR_BOP_ADD 0u   /* node.cpp:33-40 */
+#define RR_BOP_SUB 1u
+#define RR_BOP_MUL 2u
+#define RR_BOP_DIV 3u
+#define RR_BOP_LT 4u    /* node.cpp:55-78: 1.0 / 0.0 */
+#define RR_BOP_GT 5u
+#define RR_BOP_EQ 6u
+#define RR_BOP_NEQ 7u
+#define RR_BOP_MIN 8u   /* node.cpp:79-90 */
+#define RR_BOP_MAX 9u
+
+#define RR_M_S 0u
+#define RR_M_SR 1u
+#define RR_M_V 2u
+#define RR_M_VR 3u
+#define RR_M_C 4u
+#define RR_M_CR 5u
+
+#define RR_MAX_STACK 8      /* operand-stack depth a program may use */
+#define RR_MAX_COLS 512     /* OLS columns per candidate (factors + free term) */
+
+/* A batch of candidates.  Candidate c owns programs
+ * [cand_start[c], cand_start[c+1]); program p owns instructions
+ * [prog_start[p], prog_start[p+1]) of `code`; operands of LDC and *_C forms
+ * index `consts`. */
+typedef struct rr_batch {
+  int32_t n_cands;
+  const int32_t* cand_start;  /* n_cands + 1 */
+  const int32_t* prog_start;  /* cand_start[n_cands] + 1 */
+  const uint32_t* code;
+  const double* consts;
+  int32_t n_consts;
+} rr_batch;
+
+/* mode of rr_fitness_batch */
+#define RR_MODE_OLS 0   /* tune_constants + fitness (rils_rols_cpp.cpp:445-541):
+                           programs are the additive factors of the rewritten
+                           candidate; the free term is implicit and last. */
+#define RR_MODE_RAW 1   /* fitness only (rils_rols_cpp.cpp:520-541): exactly
+                           one program per candidate, the whole tree. */
+
+/* precision of rr_fitness_batch */
+#define RR_FP64 0        /* plain form: bit-for-bit the canonical FP64 algorithm */
+#define RR_FP32_REFINED 1 /* FP32 evaluation + compensated Gram sums, FP64 solve
+                             with one refinement step; candidates flagged
+                             RR_F_NEEDS64 must be re-scored in FP64 before they
+                             decide anything */
+
+/* flags of rr_fit */
+#define RR_F_SENTINEL 1  /* R2 or RMSE was NaN: fitness is (1000, 1000, 1000) */
+#define RR_F_NEEDS64 2   /* FP32 result not trustworthy to its stated bound */
+
+typedef struct rr_fit {
+  double e0;      /* 1 - R2 (rils_rols_cpp.cpp:538) */
+  double e1;      /* RMSE */
+  double err;     /* FP32 form: bound on |fitness_value error| relative; 0 in FP64 */
+  int32_t rank;   /* nonzero pivots of the column-pivoted QR (OLS mode) */
+  int32_t flags;  /* RR_F_* */
+} rr_fit;
+
+/* ---- context ----------------------------------------------------------- */
+typedef struct rr_ctx rr_ctx;
+
+/* Create a context on HIP device `device` (its own non-blocking stream). */
+int rr_create(int device, rr_ctx** out);
+int rr_destroy(rr_ctx* ctx);
+/* Message of the last error on this thread ("" if none). */
+const char* rr_last_error(void);
+
+/* ---- datasets -----------------------------------------------------------
+ * Replaces the feature-major vector<ArrayXd> X and ArrayXd y that fit_inner
+ * builds (rils_rols_cpp.cpp:675-715, 772-795).  X_fm holds d columns of n
+ * samples (column j at X_fm + j*n), already sampled/shuffled by the caller.
+ * The data stay resident in HBM until rr_dataset_free. */
+int rr_dataset_upload(rr_ctx* ctx, const double* X_fm, const double* y, int64_t n, int32_t d, int32_t* handle);
+int rr_dataset_free(rr_ctx* ctx, int32_t handle);
+/* mean(y) and sum((y - mean)^2) in the canonical order (R2, rils_rols_cpp.cpp:40-45) */
+int rr_dataset_stats(rr_ctx* ctx, int32_t handle, double* ybar, double* sst);
+
+/* ---- the hot path -------------------------------------------------------
+ * One call scores every candidate of `batch` against dataset `handle`.
+ * OLS mode: coefs_out receives k_c = (#programs of c) + 1 coefficients per
+ * candidate, at offset cand_start[c] + c, in factor order with the free term
+ * last (the vector `coefs` of rils_rols_cpp.cpp:484); NaN coefficients are
+ * written as the x86 default NaN.  fit_out[c] receives 1-R2 and RMSE of the
+ * OLS tree that tune_constants builds from those coefficients (the size
+ * component of the fitness tuple is the caller's).  RAW mode: coefs_out may
+ * be NULL. */
+int rr_fitness_batch(rr_ctx* ctx, int32_t handle, const rr_batch* batch, int32_t mode, int32_t precision,
+                     double* coefs_out, rr_fit* fit_out);
+
+/* Evaluate one program over all samples of the dataset (node::evaluate_all,
+ * used by rils_rols::predict, rils_rols_cpp.cpp:730-750). */
+int rr_predict(rr_ctx* ctx, int32_t handle, const uint32_t* code, int32_t len, const double* consts,
+               int32_t n_consts, double* out);
+
+/* ---- tuning ---------------------------------------------------------------
+ * Named integer options: "slab_budget_mb" (device memory the FP64 OLS path
+ * may use for its per-candidate work columns), "fp32_block_waves",
+ * "fp32_rows_per_tile".  Unknown names return RR_E_ARG. */
+int rr_set_option(rr_ctx* ctx, const char* name, int64_t value);
+int rr_get_option(rr_ctx* ctx, const char* name, int64_t* value);
+
+/* Kernel time of the last rr_fitness_batch / rr_predict call on the context's
+ * stream, in milliseconds (HIP events around the launches). */
+int rr_last_kernel_ms(rr_ctx* ctx, double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RR_FITNESS_H */

@@ -1,0 +1,450 @@
+// rr_capi.hip -- implementation of the C ABI declared in include/rr_fitness.h.
+//
+// Host-side responsibilities: validate every program of a batch before it
+// reaches the GPU (opcodes, operand ranges, stack discipline), stage the
+// batch in pinned memory, move it to the device, split FP64 OLS launches so
+// their work slabs fit the configured device-memory budget, launch, and copy
+// the per-candidate results back.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/rr_fitness.h"
+#include "rr_kernels.h"
+#include "rr_kernels_fp32.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define RR_HIP(call)                                                                         \
+  do {                                                                                       \
+    hipError_t e_ = (call);                                                                  \
+    if (e_ != hipSuccess) return fail(RR_E_HIP, std::string(#call ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  int ensure(size_t bytes) {
+    if (bytes <= cap) return RR_OK;
+    if (p) { (void)hipFree(p); p = nullptr; cap = 0; }
+    size_t want = std::max<size_t>(bytes + bytes / 2, 4096);
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) {
+      e = hipMalloc(&p, bytes);
+      if (e != hipSuccess) { p = nullptr; return fail(RR_E_NOMEM, "hipMalloc failed for work buffer"); }
+      want = bytes;
+    }
+    cap = want;
+    return RR_OK;
+  }
+  void release() { if (p) (void)hipFree(p); p = nullptr; cap = 0; }
+};
+
+struct HostBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  int ensure(size_t bytes) {
+    if (bytes <= cap) return RR_OK;
+    if (p) { (void)hipHostFree(p); p = nullptr; cap = 0; }
+    size_t want = std::max<size_t>(bytes + bytes / 2, 4096);
+    if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess) { p = nullptr; return fail(RR_E_NOMEM, "hipHostMalloc failed"); }
+    cap = want;
+    return RR_OK;
+  }
+  void release() { if (p) (void)hipHostFree(p); p = nullptr; cap = 0; }
+};
+
+struct Dataset {
+  bool live = false;
+  int64_t n = 0, ldx = 0;
+  int32_t d = 0;
+  double* X = nullptr;    // d columns, stride ldx
+  double* y = nullptr;
+  float* X32 = nullptr;   // FP32 copies for the precision form
+  float* y32 = nullptr;
+  double ybar = 0, sst = 0;
+};
+
+}  // namespace
+
+struct rr_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::vector<Dataset> ds;
+  DevBuf d_cand, d_prog, d_code, d_K, d_slab, d_slab_off, d_coefs, d_fit, d_misc, d_fp32_work;
+  HostBuf h_stage, h_out;
+  int64_t slab_budget_mb = 32768;
+  int64_t fp32_block_waves = 4;
+  int64_t fp32_rows_per_tile = 256;
+  double last_ms = 0.0;
+};
+
+namespace {
+
+int set_device(rr_ctx* ctx) {
+  if (!ctx) return fail(RR_E_ARG, "null context");
+  RR_HIP(hipSetDevice(ctx->device));
+  return RR_OK;
+}
+
+Dataset* get_ds(rr_ctx* ctx, int32_t h) {
+  if (h < 0 || h >= (int32_t)ctx->ds.size() || !ctx->ds[h].live) return nullptr;
+  return &ctx->ds[h];
+}
+
+// Validate one program: opcodes, operands and the stack discipline.
+int check_program(const uint32_t* code, int32_t len, int32_t d, int32_t n_consts, int* max_depth) {
+  if (len <= 0) return fail(RR_E_ARG, "empty program");
+  int sp = 0, depth = 0;
+  bool acc = false;
+  for (int32_t pc = 0; pc < len; ++pc) {
+    const uint32_t ins = code[pc], op = ins & 0xffu, arg = ins >> 8;
+    if (op >= RR_OP_BIN) {
+      const uint32_t b = op - RR_OP_BIN, bop = b >> 3, mode = b & 7u;
+      if (bop > RR_BOP_MAX || mode > RR_M_CR) return fail(RR_E_ARG, "bad binary opcode");
+      if (!acc) return fail(RR_E_ARG, "binary op with empty accumulator");
+      if (mode == RR_M_S || mode == RR_M_SR) { if (sp == 0) return fail(RR_E_ARG, "stack underflow"); --sp; }
+      if ((mode == RR_M_V || mode == RR_M_VR) && (int64_t)arg >= d) return fail(RR_E_ARG, "variable index out of range");
+      if ((mode == RR_M_C || mode == RR_M_CR) && (int64_t)arg >= n_consts) return fail(RR_E_ARG, "constant index out of range");
+      continue;
+    }
+    switch (op) {
+      case RR_OP_LDV: if ((int64_t)arg >= d) return fail(RR_E_ARG, "variable index out of range"); acc = true; break;
+      case RR_OP_LDC: if ((int64_t)arg >= n_consts) return fail(RR_E_ARG, "constant index out of range"); acc = true; break;
+      case RR_OP_PUSH:
+        if (!acc) return fail(RR_E_ARG, "push of empty accumulator");
+        if (++sp > RR_MAX_STACK) return fail(RR_E_LIMIT, "program needs more than RR_MAX_STACK stack slots");
+        depth = std::max(depth, sp);
+        acc = false;
+        break;
+      case RR_OP_SIN: case RR_OP_COS: case RR_OP_LN: case RR_OP_EXP: case RR_OP_SQRT: case RR_OP_SQR:
+        if (!acc) return fail(RR_E_ARG, "unary op with empty accumulator");
+        break;
+      default: return fail(RR_E_ARG, "bad opcode");
+    }
+  }
+  if (sp != 0 || !acc) return fail(RR_E_ARG, "program leaves the stack unbalanced");
+  if (max_depth) *max_depth = std::max(*max_depth, depth);
+  return RR_OK;
+}
+
+int check_batch(const rr_batch* b, int32_t mode, int32_t d) {
+  if (!b || b->n_cands < 0) return fail(RR_E_ARG, "bad batch");
+  if (b->n_cands == 0) return RR_OK;
+  if (!b->cand_start || !b->prog_start || !b->code || (b->n_consts > 0 && !b->consts)) return fail(RR_E_ARG, "null batch array");
+  if (b->cand_start[0] != 0) return fail(RR_E_ARG, "cand_start[0] must be 0");
+  for (int32_t c = 0; c < b->n_cands; ++c) {
+    const int32_t np = b->cand_start[c + 1] - b->cand_start[c];
+    if (np < 0) return fail(RR_E_ARG, "cand_start not monotone");
+    if (mode == RR_MODE_RAW && np != 1) return fail(RR_E_ARG, "RAW mode takes exactly one program per candidate");
+    if (mode == RR_MODE_OLS && np + 1 > RR_MAX_COLS) return fail(RR_E_LIMIT, "candidate has more than RR_MAX_COLS OLS columns");
+  }
+  const int32_t n_progs = b->cand_start[b->n_cands];
+  if (b->prog_start[0] != 0) return fail(RR_E_ARG, "prog_start[0] must be 0");
+  for (int32_t p = 0; p < n_progs; ++p) {
+    const int32_t len = b->prog_start[p + 1] - b->prog_start[p];
+    int rc = check_program(b->code + b->prog_start[p], len, d, b->n_consts, nullptr);
+    if (rc != RR_OK) {
+      char m[64];
+      snprintf(m, sizeof m, " (program %d)", p);
+      g_err += m;
+      return rc;
+    }
+  }
+  return RR_OK;
+}
+
+// Copies `bytes` from host `src` into device buffer `dst` through pinned staging.
+int upload(rr_ctx* ctx, DevBuf& dst, const void* src, size_t bytes, size_t stage_off) {
+  if (dst.ensure(bytes) != RR_OK) return RR_E_NOMEM;
+  if (bytes == 0) return RR_OK;
+  memcpy((char*)ctx->h_stage.p + stage_off, src, bytes);
+  RR_HIP(hipMemcpyAsync(dst.p, (char*)ctx->h_stage.p + stage_off, bytes, hipMemcpyHostToDevice, ctx->stream));
+  return RR_OK;
+}
+
+size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
+
+}  // namespace
+
+extern "C" {
+
+const char* rr_last_error(void) { return g_err.c_str(); }
+
+int rr_create(int device, rr_ctx** out) {
+  if (!out) return fail(RR_E_ARG, "null output pointer");
+  *out = nullptr;
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count == 0) return fail(RR_E_HIP, "no HIP device available");
+  if (device < 0 || device >= count) return fail(RR_E_ARG, "device index out of range");
+  RR_HIP(hipSetDevice(device));
+  rr_ctx* c = new rr_ctx;
+  c->device = device;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+    delete c;
+    return fail(RR_E_HIP, "stream/event creation failed");
+  }
+  *out = c;
+  return RR_OK;
+}
+
+int rr_destroy(rr_ctx* ctx) {
+  if (!ctx) return RR_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  for (Dataset& d : ctx->ds) {
+    if (!d.live) continue;
+    (void)hipFree(d.X); (void)hipFree(d.y); (void)hipFree(d.X32); (void)hipFree(d.y32);
+  }
+  DevBuf* bufs[] = {&ctx->d_cand, &ctx->d_prog, &ctx->d_code, &ctx->d_K, &ctx->d_slab, &ctx->d_slab_off,
+                    &ctx->d_coefs, &ctx->d_fit, &ctx->d_misc, &ctx->d_fp32_work};
+  for (DevBuf* b : bufs) b->release();
+  ctx->h_stage.release();
+  ctx->h_out.release();
+  (void)hipEventDestroy(ctx->ev0);
+  (void)hipEventDestroy(ctx->ev1);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return RR_OK;
+}
+
+int rr_dataset_upload(rr_ctx* ctx, const double* X_fm, const double* y, int64_t n, int32_t d, int32_t* handle) {
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  if (!handle || !y || n <= 0 || d < 0 || (d > 0 && !X_fm)) return fail(RR_E_ARG, "bad dataset arguments");
+  if (n >= ((int64_t)1 << 40)) return fail(RR_E_ARG, "too many samples");
+  Dataset D;
+  D.n = n; D.d = d;
+  D.ldx = (n + 63) & ~int64_t(63);
+  const size_t xb = (size_t)D.ldx * (size_t)std::max(d, 1) * sizeof(double);
+  if (hipMalloc(&D.X, xb) != hipSuccess) return fail(RR_E_NOMEM, "hipMalloc for X failed");
+  if (hipMalloc(&D.y, (size_t)D.ldx * sizeof(double)) != hipSuccess) { (void)hipFree(D.X); return fail(RR_E_NOMEM, "hipMalloc for y failed"); }
+  if (hipMalloc(&D.X32, (size_t)D.ldx * std::max(d, 1) * sizeof(float)) != hipSuccess ||
+      hipMalloc(&D.y32, (size_t)D.ldx * sizeof(float)) != hipSuccess) {
+    (void)hipFree(D.X); (void)hipFree(D.y); (void)hipFree(D.X32);
+    return fail(RR_E_NOMEM, "hipMalloc for FP32 copies failed");
+  }
+  RR_HIP(hipMemsetAsync(D.X, 0, xb, ctx->stream));
+  RR_HIP(hipMemsetAsync(D.y, 0, (size_t)D.ldx * sizeof(double), ctx->stream));
+  if (d > 0)
+    RR_HIP(hipMemcpy2DAsync(D.X, (size_t)D.ldx * sizeof(double), X_fm, (size_t)n * sizeof(double),
+                            (size_t)n * sizeof(double), (size_t)d, hipMemcpyHostToDevice, ctx->stream));
+  RR_HIP(hipMemcpyAsync(D.y, y, (size_t)n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  RR_HIP(rr::launch_to_fp32(D.X, D.X32, (int64_t)D.ldx * std::max(d, 1), ctx->stream));
+  RR_HIP(rr::launch_to_fp32(D.y, D.y32, D.ldx, ctx->stream));
+  if (ctx->d_misc.ensure(2 * sizeof(double)) != RR_OK) return RR_E_NOMEM;
+  RR_HIP(rr::launch_stats_fp64(D.y, n, (double*)ctx->d_misc.p, ctx->stream));
+  double st[2];
+  RR_HIP(hipMemcpyAsync(st, ctx->d_misc.p, sizeof st, hipMemcpyDeviceToHost, ctx->stream));
+  RR_HIP(hipStreamSynchronize(ctx->stream));
+  D.ybar = st[0];
+  D.sst = st[1];
+  D.live = true;
+  ctx->ds.push_back(D);
+  *handle = (int32_t)ctx->ds.size() - 1;
+  return RR_OK;
+}
+
+int rr_dataset_free(rr_ctx* ctx, int32_t handle) {
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  Dataset* D = get_ds(ctx, handle);
+  if (!D) return fail(RR_E_ARG, "bad dataset handle");
+  RR_HIP(hipStreamSynchronize(ctx->stream));
+  (void)hipFree(D->X); (void)hipFree(D->y); (void)hipFree(D->X32); (void)hipFree(D->y32);
+  *D = Dataset();
+  return RR_OK;
+}
+
+int rr_dataset_stats(rr_ctx* ctx, int32_t handle, double* ybar, double* sst) {
+  if (!ctx) return fail(RR_E_ARG, "null context");
+  Dataset* D = get_ds(ctx, handle);
+  if (!D) return fail(RR_E_ARG, "bad dataset handle");
+  if (ybar) *ybar = D->ybar;
+  if (sst) *sst = D->sst;
+  return RR_OK;
+}
+
+int rr_fitness_batch(rr_ctx* ctx, int32_t handle, const rr_batch* b, int32_t mode, int32_t precision,
+                     double* coefs_out, rr_fit* fit_out) {
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  Dataset* D = get_ds(ctx, handle);
+  if (!D) return fail(RR_E_ARG, "bad dataset handle");
+  if (mode != RR_MODE_OLS && mode != RR_MODE_RAW) return fail(RR_E_ARG, "bad mode");
+  if (precision != RR_FP64 && precision != RR_FP32_REFINED) return fail(RR_E_ARG, "bad precision");
+  rc = check_batch(b, mode, D->d);
+  if (rc) return rc;
+  if (!fit_out) return fail(RR_E_ARG, "null fit_out");
+  if (mode == RR_MODE_OLS && !coefs_out) return fail(RR_E_ARG, "null coefs_out");
+  const int32_t nc = b->n_cands;
+  if (nc == 0) { ctx->last_ms = 0.0; return RR_OK; }
+  const int32_t np = b->cand_start[nc];
+  const int32_t ncode = b->prog_start[np];
+  const int64_t ncoef = (int64_t)np + nc;
+
+  // stage inputs
+  const size_t s_cand = align_up((size_t)(nc + 1) * 4), s_prog = align_up((size_t)(np + 1) * 4);
+  const size_t s_code = align_up((size_t)ncode * 4), s_K = align_up((size_t)std::max(b->n_consts, 1) * 8);
+  const size_t s_off = align_up((size_t)nc * 8);
+  if (ctx->h_stage.ensure(s_cand + s_prog + s_code + s_K + s_off) != RR_OK) return RR_E_NOMEM;
+  size_t so = 0;
+  if ((rc = upload(ctx, ctx->d_cand, b->cand_start, (size_t)(nc + 1) * 4, so))) return rc;
+  so += s_cand;
+  if ((rc = upload(ctx, ctx->d_prog, b->prog_start, (size_t)(np + 1) * 4, so))) return rc;
+  so += s_prog;
+  if ((rc = upload(ctx, ctx->d_code, b->code, (size_t)ncode * 4, so))) return rc;
+  so += s_code;
+  if (b->n_consts > 0) { if ((rc = upload(ctx, ctx->d_K, b->consts, (size_t)b->n_consts * 8, so))) return rc; }
+  else if (ctx->d_K.ensure(8) != RR_OK) return RR_E_NOMEM;
+  so += s_K;
+  if (ctx->d_fit.ensure((size_t)nc * sizeof(rr_fit)) != RR_OK) return RR_E_NOMEM;
+  if (ctx->d_coefs.ensure((size_t)std::max<int64_t>(ncoef, 1) * 8) != RR_OK) return RR_E_NOMEM;
+
+  const int32_t* dc = (const int32_t*)ctx->d_cand.p;
+  const int32_t* dp = (const int32_t*)ctx->d_prog.p;
+  const uint32_t* dcode = (const uint32_t*)ctx->d_code.p;
+  const double* dK = (const double*)ctx->d_K.p;
+  rr_fit* dfit = (rr_fit*)ctx->d_fit.p;
+  double* dcoef = (double*)ctx->d_coefs.p;
+
+  if (mode == RR_MODE_RAW) {
+    RR_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    if (precision == RR_FP64) {
+      rr::RawLaunch L{dc, dp, dcode, dK, D->X, D->y, D->n, D->ldx, D->sst, dfit, nc};
+      RR_HIP(rr::launch_raw_fp64(L, ctx->stream));
+    } else {
+      rr::Fp32Launch L{};
+      L.cand_start = dc; L.prog_start = dp; L.code = dcode; L.K = dK; L.X32 = D->X32; L.y32 = D->y32; L.X = D->X;
+      L.y = D->y; L.n = D->n; L.ldx = D->ldx; L.ybar = D->ybar; L.sst = D->sst; L.coefs = dcoef; L.fit = dfit;
+      L.n_cands = nc; L.raw = 1; L.block_waves = (int)ctx->fp32_block_waves;
+      RR_HIP(rr::launch_fp32(L, ctx->stream));
+    }
+    RR_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+  } else if (precision == RR_FP64) {
+    // chunk so that each launch's slab fits the budget
+    std::vector<int64_t> off(nc);
+    const int64_t budget = std::max<int64_t>(ctx->slab_budget_mb, 1) << 20;
+    struct Chunk { int c0, c1, maxk; int64_t words; };
+    std::vector<Chunk> chunks;
+    int64_t max_words = 0;
+    for (int c = 0; c < nc;) {
+      int64_t used = 0;
+      int c1 = c, maxk = 0;
+      while (c1 < nc) {
+        const int k = b->cand_start[c1 + 1] - b->cand_start[c1] + 1;
+        const int64_t need = (int64_t)(k + 1) * D->ldx;
+        if (c1 > c && (used + need) * 8 > budget) break;
+        off[c1] = used;
+        used += need;
+        maxk = std::max(maxk, k);
+        ++c1;
+      }
+      chunks.push_back({c, c1, maxk, used});
+      max_words = std::max(max_words, used);
+      c = c1;
+    }
+    if (ctx->d_slab.ensure((size_t)max_words * 8) != RR_OK) return RR_E_NOMEM;
+    if ((rc = upload(ctx, ctx->d_slab_off, off.data(), (size_t)nc * 8, so))) return rc;
+    RR_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    for (const Chunk& ch : chunks) {
+      rr::OlsLaunch L;
+      L.cand_start = dc; L.prog_start = dp; L.code = dcode; L.K = dK; L.X = D->X; L.y = D->y;
+      L.n = D->n; L.ldx = D->ldx; L.sst = D->sst; L.slab = (double*)ctx->d_slab.p;
+      L.slab_off = (const int64_t*)ctx->d_slab_off.p + ch.c0; L.coefs = dcoef; L.fit = dfit;
+      L.cand_base = ch.c0; L.n_cands = ch.c1 - ch.c0; L.max_cols = ch.maxk;
+      RR_HIP(rr::launch_ols_fp64(L, ctx->stream));
+    }
+    RR_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+  } else {
+    rr::Fp32Launch L{};
+    L.cand_start = dc; L.prog_start = dp; L.code = dcode; L.K = dK; L.X32 = D->X32; L.y32 = D->y32; L.X = D->X;
+    L.y = D->y; L.n = D->n; L.ldx = D->ldx; L.ybar = D->ybar; L.sst = D->sst; L.coefs = dcoef; L.fit = dfit;
+    L.n_cands = nc; L.raw = 0; L.block_waves = (int)ctx->fp32_block_waves;
+    RR_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    RR_HIP(rr::launch_fp32(L, ctx->stream));
+    RR_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+  }
+
+  RR_HIP(hipMemcpyAsync(fit_out, dfit, (size_t)nc * sizeof(rr_fit), hipMemcpyDeviceToHost, ctx->stream));
+  if (mode == RR_MODE_OLS)
+    RR_HIP(hipMemcpyAsync(coefs_out, dcoef, (size_t)ncoef * 8, hipMemcpyDeviceToHost, ctx->stream));
+  RR_HIP(hipStreamSynchronize(ctx->stream));
+  float ms = 0.f;
+  RR_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  ctx->last_ms = ms;
+  return RR_OK;
+}
+
+int rr_predict(rr_ctx* ctx, int32_t handle, const uint32_t* code, int32_t len, const double* consts, int32_t n_consts,
+               double* out) {
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  Dataset* D = get_ds(ctx, handle);
+  if (!D) return fail(RR_E_ARG, "bad dataset handle");
+  if (!code || !out || n_consts < 0 || (n_consts > 0 && !consts)) return fail(RR_E_ARG, "bad predict arguments");
+  if ((rc = check_program(code, len, D->d, n_consts, nullptr))) return rc;
+  const size_t s_code = align_up((size_t)len * 4);
+  if (ctx->h_stage.ensure(s_code + align_up((size_t)std::max(n_consts, 1) * 8)) != RR_OK) return RR_E_NOMEM;
+  if ((rc = upload(ctx, ctx->d_code, code, (size_t)len * 4, 0))) return rc;
+  if (n_consts > 0) { if ((rc = upload(ctx, ctx->d_K, consts, (size_t)n_consts * 8, s_code))) return rc; }
+  else if (ctx->d_K.ensure(8) != RR_OK) return RR_E_NOMEM;
+  if (ctx->d_misc.ensure((size_t)D->n * 8) != RR_OK) return RR_E_NOMEM;
+  RR_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+  RR_HIP(rr::launch_predict_fp64((const uint32_t*)ctx->d_code.p, len, (const double*)ctx->d_K.p, D->X, D->n, D->ldx,
+                                 (double*)ctx->d_misc.p, ctx->stream));
+  RR_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+  RR_HIP(hipMemcpyAsync(out, ctx->d_misc.p, (size_t)D->n * 8, hipMemcpyDeviceToHost, ctx->stream));
+  RR_HIP(hipStreamSynchronize(ctx->stream));
+  float ms = 0.f;
+  RR_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  ctx->last_ms = ms;
+  return RR_OK;
+}
+
+int rr_set_option(rr_ctx* ctx, const char* name, int64_t value) {
+  if (!ctx || !name) return fail(RR_E_ARG, "null argument");
+  const std::string s(name);
+  if (s == "slab_budget_mb") { if (value < 1) return fail(RR_E_ARG, "slab_budget_mb must be >= 1"); ctx->slab_budget_mb = value; }
+  else if (s == "fp32_block_waves") {
+    if (value != 1 && value != 2 && value != 4 && value != 8) return fail(RR_E_ARG, "fp32_block_waves must be 1, 2, 4 or 8");
+    ctx->fp32_block_waves = value;
+  } else if (s == "fp32_rows_per_tile") {
+    if (value < 64 || value > 4096 || (value & 63)) return fail(RR_E_ARG, "fp32_rows_per_tile must be a multiple of 64 in [64, 4096]");
+    ctx->fp32_rows_per_tile = value;
+  } else return fail(RR_E_ARG, "unknown option " + s);
+  return RR_OK;
+}
+
+int rr_get_option(rr_ctx* ctx, const char* name, int64_t* value) {
+  if (!ctx || !name || !value) return fail(RR_E_ARG, "null argument");
+  const std::string s(name);
+  if (s == "slab_budget_mb") *value = ctx->slab_budget_mb;
+  else if (s == "fp32_block_waves") *value = ctx->fp32_block_waves;
+  else if (s == "fp32_rows_per_tile") *value = ctx->fp32_rows_per_tile;
+  else return fail(RR_E_ARG, "unknown option " + s);
+  return RR_OK;
+}
+
+int rr_last_kernel_ms(rr_ctx* ctx, double* ms) {
+  if (!ctx || !ms) return fail(RR_E_ARG, "null argument");
+  *ms = ctx->last_ms;
+  return RR_OK;
+}
+
+}  // extern "C"

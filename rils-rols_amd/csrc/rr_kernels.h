@@ -1,0 +1,48 @@
+// rr_kernels.h -- launch descriptors of the fitness kernels (host side of
+// the kernels, used by the C-ABI implementation).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/rr_fitness.h"
+
+namespace rr {
+
+struct OlsLaunch {
+  const int32_t* cand_start;  // device
+  const int32_t* prog_start;
+  const uint32_t* code;
+  const double* K;
+  const double* X;            // feature-major, column stride ldx
+  const double* y;
+  int64_t n, ldx;
+  double sst;
+  double* slab;               // work columns
+  const int64_t* slab_off;    // per launched candidate, in doubles
+  double* coefs;
+  rr_fit* fit;
+  int cand_base;              // first candidate of this launch
+  int n_cands;                // candidates in this launch
+  int max_cols;               // largest k among them
+};
+
+struct RawLaunch {
+  const int32_t* cand_start;
+  const int32_t* prog_start;
+  const uint32_t* code;
+  const double* K;
+  const double* X;
+  const double* y;
+  int64_t n, ldx;
+  double sst;
+  rr_fit* fit;
+  int n_cands;
+};
+
+hipError_t launch_ols_fp64(const OlsLaunch& L, hipStream_t s);
+hipError_t launch_raw_fp64(const RawLaunch& L, hipStream_t s);
+hipError_t launch_predict_fp64(const uint32_t* code, int len, const double* K, const double* X, int64_t n,
+                               int64_t ldx, double* out, hipStream_t s);
+hipError_t launch_stats_fp64(const double* y, int64_t n, double* out2, hipStream_t s);
+
+}  // namespace rr

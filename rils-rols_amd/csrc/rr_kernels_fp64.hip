@@ -1,0 +1,399 @@
+// rr_kernels_fp64.hip -- the plain FP64 form of the candidate-fitness path.
+//
+// One wavefront owns one candidate and walks all n samples; lane l handles
+// rows l, l+64, l+128, ...  Every sum over samples follows the canonical
+// order of rr_device.h, and the column-pivoted Householder QR follows
+// Eigen's ColPivHouseholderQR (eigen/Eigen/src/QR/ColPivHouseholderQR.h:
+// 482-607, Householder.h:67-135) operation for operation, so the results are
+// the same bits as the CPU oracle's.
+//
+// Per candidate the kernel
+//   1. evaluates each factor program and writes its column to a work slab in
+//      HBM (tune_constants, rils_rols_cpp.cpp:477-482), accumulating the
+//      column norms on the way;
+//   2. runs the pivoted QR on the slab: per step one pass computing the
+//      Householder dot products and one pass applying the reflector, which
+//      also accumulates the tail norms the next step and the LAWN-176 norm
+//      recomputation need; the reflectors are applied to a copy of y as they
+//      are made (Q^T b of _solve_impl); the factorisation stops at the first
+//      step that falls under Eigen's rank threshold, after which nothing
+//      changes the solution;
+//   3. back-substitutes on the leading r x r block and scatters the
+//      coefficients through the column permutation (zeros for the rest);
+//   4. re-evaluates the factors to form the prediction of the OLS tree that
+//      tune_constants builds (rils_rols_cpp.cpp:488-517): left-deep sum of
+//      c*f (or f when |c-1| < 1e-12), skipping |c| < 1e-12, free term last,
+//      and reduces SSR into 1-R2 and RMSE (fitness, 520-541).
+#include "rr_device.h"
+#include "rr_kernels.h"
+
+namespace rr {
+
+namespace {
+
+constexpr double EPS12 = 1e-12;                 // node.h:13-14
+constexpr double DBL_EPS = 2.220446049250313e-16;
+constexpr double DBL_MIN_ = 2.2250738585072014e-308;
+constexpr int G = 8;                            // columns per pass
+
+struct Fp64Args {
+  const int32_t* cand_start;
+  const int32_t* prog_start;
+  const uint32_t* code;
+  const double* K;
+  const double* X;
+  const double* y;
+  int64_t n, ldx;
+  double sst;
+  double* slab;
+  const int64_t* slab_off;
+  double* coefs;
+  rr_fit* fit;
+  int cand_base;
+};
+
+__device__ inline void write_fit(rr_fit* f, double ssr, double sst, int64_t n, int rank) {
+  const double r2 = 1 - ssr / sst;
+  const double rmse = __builtin_sqrt(ssr / (double)n);
+  rr_fit o;
+  if (r2 != r2 || rmse != rmse) {
+    o.e0 = 1000; o.e1 = 1000; o.flags = RR_F_SENTINEL;
+  } else {
+    o.e0 = 1 - r2; o.e1 = rmse; o.flags = 0;
+  }
+  o.err = 0.0;
+  o.rank = rank;
+  *f = o;
+}
+
+template <int MAXK>
+__global__ __launch_bounds__(64) void k_ols_fp64(Fp64Args a) {
+  __shared__ double normU[MAXK], normD[MAXK], tsq[MAXK], cc[MAXK];
+  __shared__ int phys[MAXK];
+
+  const int cand = a.cand_base + (int)blockIdx.x;
+  const int lane = (int)threadIdx.x;
+  const int pbeg = a.cand_start[cand];
+  const int nf = a.cand_start[cand + 1] - pbeg;
+  const int k = nf + 1;
+  const int64_t n = a.n, ldx = a.ldx;
+  double* const S = a.slab + a.slab_off[blockIdx.x];   // columns 0..k-1: A, column k: Q^T y
+  const double* const X = a.X;
+
+  // ---- 1. factor columns, free term, copy of y; norms
+  for (int j = 0; j < nf; ++j) {
+    const uint32_t* code = a.code + a.prog_start[pbeg + j];
+    const int len = a.prog_start[pbeg + j + 1] - a.prog_start[pbeg + j];
+    double* col = S + (int64_t)j * ldx;
+    double s_all = 0.0, s_1 = 0.0;
+    for (int64_t i = lane; i < n; i += WAVE) {
+      const double v = eval_f64(code, len, a.K, X, ldx, i);
+      col[i] = v;
+      s_all = s_all + v * v;
+      if (i >= 1) s_1 = s_1 + v * v;
+    }
+    s_all = wave_sum(s_all);
+    s_1 = wave_sum(s_1);
+    if (lane == 0) { normD[j] = __builtin_sqrt(s_all); tsq[j] = s_1; }
+  }
+  {
+    double* col = S + (int64_t)nf * ldx;
+    double* cy = S + (int64_t)k * ldx;
+    for (int64_t i = lane; i < n; i += WAVE) { col[i] = 1.0; cy[i] = a.y[i]; }
+    if (lane == 0) { normD[nf] = __builtin_sqrt((double)n); tsq[nf] = (double)(n - 1); }
+  }
+  for (int j = lane; j < k; j += WAVE) phys[j] = j;
+  __syncthreads();
+  for (int j = lane; j < k; j += WAVE) normU[j] = normD[j];
+  __syncthreads();
+
+  // ---- 2. column-pivoted Householder QR
+  const int size = (int)((n < (int64_t)k) ? n : (int64_t)k);
+  double mx = normU[0];
+  for (int j = 1; j < k; ++j) { const double v = normU[j]; mx = (mx < v) ? v : mx; }
+  const double thr = ((mx * DBL_EPS) * (mx * DBL_EPS)) / (double)n;
+  const double dd_thr = __builtin_sqrt(DBL_EPS);
+  int nonzero = size;
+  double* const cy = S + (int64_t)k * ldx;
+
+  for (int p = 0; p < size; ++p) {
+    int bi = p;
+    double best = normU[p];
+    for (int j = p + 1; j < k; ++j) { const double v = normU[j]; if (v > best) { best = v; bi = j; } }
+    if (best * best < thr * (double)(n - p)) { nonzero = p; break; }
+    __syncthreads();
+    if (lane == 0 && bi != p) {
+      int ti = phys[p]; phys[p] = phys[bi]; phys[bi] = ti;
+      double t;
+      t = normU[p]; normU[p] = normU[bi]; normU[bi] = t;
+      t = normD[p]; normD[p] = normD[bi]; normD[bi] = t;
+      t = tsq[p]; tsq[p] = tsq[bi]; tsq[bi] = t;
+    }
+    __syncthreads();
+
+    double* const v = S + (int64_t)phys[p] * ldx;
+    const double c0 = v[p];
+    const double tailSq = (n - p == 1) ? 0.0 : tsq[p];
+    double tau, beta;
+    if (tailSq <= DBL_MIN_) {
+      tau = 0.0;
+      beta = c0;
+      for (int64_t i = first_row(p + 1, lane); i < n; i += WAVE) v[i] = 0.0;
+    } else {
+      beta = __builtin_sqrt(c0 * c0 + tailSq);
+      if (c0 >= 0.0) beta = -beta;
+      tau = (beta - c0) / beta;
+    }
+    const double den = c0 - beta;
+    const bool apply = tau != 0.0;         // applyHouseholderOnTheLeft does nothing for tau == 0
+    __syncthreads();   // every lane has read c0 before row p is overwritten
+    if (lane == 0) v[p] = beta;
+
+    const int nrem = k - 1 - p;            // remaining A columns: positions p+1..k-1
+    const int ntarget = nrem + 1;          // plus the y column (last target)
+    if (n - p == 1) {
+      // applyHouseholderOnTheLeft with a single row: scale by (1 - tau)
+      if (lane == 0) {
+        for (int t = 0; t < nrem; ++t) { double* col = S + (int64_t)phys[p + 1 + t] * ldx; col[p] *= (1.0 - tau); }
+        cy[p] *= (1.0 - tau);
+      }
+      __syncthreads();
+      for (int t = 0; t < nrem; ++t) if (lane == 0) tsq[p + 1 + t] = 0.0;
+      // the LAWN-176 recomputation below needs sums over rows p+1.. : empty
+      for (int t = 0; t < nrem; ++t) {
+        const int j = p + 1 + t;
+        if (normU[j] != 0.0) {
+          const double apj = S[(int64_t)phys[j] * ldx + p];
+          double temp = __builtin_fabs(apj) / normU[j];
+          temp = (1.0 + temp) * (1.0 - temp);
+          temp = temp < 0.0 ? 0.0 : temp;
+          const double ratio = normU[j] / normD[j];
+          const double temp2 = temp * (ratio * ratio);
+          __syncthreads();
+          if (lane == 0) {
+            if (temp2 <= dd_thr) { normD[j] = 0.0; normU[j] = 0.0; }
+            else normU[j] *= __builtin_sqrt(temp);
+          }
+          __syncthreads();
+        }
+      }
+      continue;
+    }
+
+    for (int g0 = 0; g0 < ntarget; g0 += G) {
+      const int gn = (ntarget - g0 < G) ? (ntarget - g0) : G;
+      double* cols[G];
+#pragma unroll
+      for (int t = 0; t < G; ++t) {
+        const int tg = g0 + t;
+        cols[t] = (t < gn) ? ((tg == nrem) ? cy : S + (int64_t)phys[p + 1 + tg] * ldx) : cy;
+      }
+      double tdot[G];
+#pragma unroll
+      for (int t = 0; t < G; ++t) tdot[t] = 0.0;
+      if (apply) {
+        // Householder dot products  t = ess . a[p+1..] (+ a[p])
+        double acc[G];
+#pragma unroll
+        for (int t = 0; t < G; ++t) acc[t] = 0.0;
+        for (int64_t i = first_row(p + 1, lane); i < n; i += WAVE) {
+          double e;
+          if (g0 == 0) { e = v[i] / den; v[i] = e; } else { e = v[i]; }
+#pragma unroll
+          for (int t = 0; t < G; ++t) if (t < gn) acc[t] = acc[t] + e * cols[t][i];
+        }
+#pragma unroll
+        for (int t = 0; t < G; ++t) {
+          if (t < gn) {
+            double s = wave_sum(acc[t]);
+            const double ap = cols[t][p];
+            s = s + ap;
+            tdot[t] = s;
+          }
+        }
+        __syncthreads();
+        if (lane == 0) {
+#pragma unroll
+          for (int t = 0; t < G; ++t) if (t < gn) cols[t][p] = cols[t][p] - tau * tdot[t];
+        }
+      }
+      // apply the reflector and accumulate tail norms (rows >= p+1 and >= p+2)
+      double s1[G], s2[G];
+#pragma unroll
+      for (int t = 0; t < G; ++t) { s1[t] = 0.0; s2[t] = 0.0; }
+      for (int64_t i = first_row(p + 1, lane); i < n; i += WAVE) {
+        const double te = apply ? tau * v[i] : 0.0;
+#pragma unroll
+        for (int t = 0; t < G; ++t) {
+          if (t < gn) {
+            double x = cols[t][i];
+            if (apply) { x = x - te * tdot[t]; cols[t][i] = x; }
+            s1[t] = s1[t] + x * x;
+            if (i >= p + 2) s2[t] = s2[t] + x * x;
+          }
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < G; ++t) {
+        if (t < gn) {
+          const double a1 = wave_sum(s1[t]);
+          const double a2 = wave_sum(s2[t]);
+          const int tg = g0 + t;
+          if (tg < nrem && lane == 0) { cc[p + 1 + tg] = a1; tsq[p + 1 + tg] = a2; }
+        }
+      }
+      __syncthreads();
+    }
+
+    // LAWN-176 norm downdate (cc[] holds the recomputed tail norms^2)
+    for (int j = p + 1; j < k; ++j) {
+      const double nu = normU[j];
+      if (nu != 0.0) {
+        const double apj = S[(int64_t)phys[j] * ldx + p];
+        double temp = __builtin_fabs(apj) / nu;
+        temp = (1.0 + temp) * (1.0 - temp);
+        temp = temp < 0.0 ? 0.0 : temp;
+        const double ratio = nu / normD[j];
+        const double temp2 = temp * (ratio * ratio);
+        const double rec = __builtin_sqrt(cc[j]);
+        __syncthreads();
+        if (lane == 0) {
+          if (temp2 <= dd_thr) { normD[j] = rec; normU[j] = rec; }
+          else normU[j] = nu * __builtin_sqrt(temp);
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  // ---- 3. back substitution on the leading nonzero x nonzero block
+  const int r = nonzero;
+  for (int i = lane; i < r; i += WAVE) cc[i] = cy[i];
+  __syncthreads();
+  for (int i = r - 1; i >= 0; --i) {
+    const double* coli = S + (int64_t)phys[i] * ldx;
+    const double ci = cc[i] / coli[i];
+    __syncthreads();
+    for (int l = lane; l < i; l += WAVE) cc[l] = cc[l] - ci * coli[l];
+    if (lane == 0) cc[i] = ci;
+    __syncthreads();
+  }
+  double* const co = a.coefs + (int64_t)pbeg + cand;
+  for (int i = lane; i < k; i += WAVE) {
+    double cv = (i < r) ? cc[i] : 0.0;
+    if (cv != cv) cv = rrm::qnan();
+    co[phys[i]] = cv;
+  }
+  __syncthreads();
+
+  // ---- 4. prediction of the OLS tree and fitness
+  double cf_free = co[nf];
+  double ssr = 0.0;
+  for (int64_t i = lane; i < n; i += WAVE) {
+    double yp = 0.0;
+    bool any = false;
+    for (int j = 0; j < nf; ++j) {
+      const double cj = co[j];
+      if (__builtin_fabs(cj) < EPS12) continue;
+      const uint32_t* code = a.code + a.prog_start[pbeg + j];
+      const int len = a.prog_start[pbeg + j + 1] - a.prog_start[pbeg + j];
+      const double f = eval_f64(code, len, a.K, X, ldx, i);
+      const double term = (__builtin_fabs(cj - 1) < EPS12) ? f : cj * f;
+      yp = any ? yp + term : term;
+      any = true;
+    }
+    if (!(__builtin_fabs(cf_free) < EPS12)) {
+      const double term = cf_free * 1.0;
+      yp = any ? yp + term : term;
+    }
+    const double d = a.y[i] - yp;
+    ssr = ssr + d * d;
+  }
+  ssr = wave_sum(ssr);
+  if (lane == 0) write_fit(a.fit + cand, ssr, a.sst, n, r);
+}
+
+struct RawArgs {
+  const int32_t* cand_start;
+  const int32_t* prog_start;
+  const uint32_t* code;
+  const double* K;
+  const double* X;
+  const double* y;
+  int64_t n, ldx;
+  double sst;
+  rr_fit* fit;
+};
+
+// fitness() of a tree as given (no OLS): main-loop perturbations and the
+// re-scores after acceptance (rils_rols_cpp.cpp:631, 800, 828, 842).
+__global__ __launch_bounds__(64) void k_raw_fp64(RawArgs a) {
+  const int cand = (int)blockIdx.x;
+  const int lane = (int)threadIdx.x;
+  const int p = a.cand_start[cand];
+  const uint32_t* code = a.code + a.prog_start[p];
+  const int len = a.prog_start[p + 1] - a.prog_start[p];
+  double ssr = 0.0;
+  for (int64_t i = lane; i < a.n; i += WAVE) {
+    const double d = a.y[i] - eval_f64(code, len, a.K, a.X, a.ldx, i);
+    ssr = ssr + d * d;
+  }
+  ssr = wave_sum(ssr);
+  if (lane == 0) write_fit(a.fit + cand, ssr, a.sst, a.n, 0);
+}
+
+// node::evaluate_all over all samples; many waves, any order (no reduction)
+__global__ __launch_bounds__(256) void k_predict_fp64(const uint32_t* code, int len, const double* K, const double* X,
+                                                      int64_t n, int64_t ldx, double* out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = eval_f64(code, len, K, X, ldx, i);
+}
+
+// mean(y) and sum((y - mean)^2), canonical order (R2, rils_rols_cpp.cpp:40-45)
+__global__ __launch_bounds__(64) void k_stats_fp64(const double* y, int64_t n, double* out2) {
+  const int lane = (int)threadIdx.x;
+  double s = 0.0;
+  for (int64_t i = lane; i < n; i += WAVE) s = s + y[i];
+  s = wave_sum(s);
+  const double ybar = s / (double)n;
+  double q = 0.0;
+  for (int64_t i = lane; i < n; i += WAVE) q = q + (y[i] - ybar) * (y[i] - ybar);
+  q = wave_sum(q);
+  if (lane == 0) { out2[0] = ybar; out2[1] = q; }
+}
+
+}  // namespace
+
+hipError_t launch_ols_fp64(const OlsLaunch& L, hipStream_t s) {
+  Fp64Args a;
+  a.cand_start = L.cand_start; a.prog_start = L.prog_start; a.code = L.code; a.K = L.K;
+  a.X = L.X; a.y = L.y; a.n = L.n; a.ldx = L.ldx; a.sst = L.sst;
+  a.slab = L.slab; a.slab_off = L.slab_off; a.coefs = L.coefs; a.fit = L.fit; a.cand_base = L.cand_base;
+  const dim3 grid(L.n_cands), block(64);
+  if (L.max_cols <= 16) hipLaunchKernelGGL(k_ols_fp64<16>, grid, block, 0, s, a);
+  else if (L.max_cols <= 64) hipLaunchKernelGGL(k_ols_fp64<64>, grid, block, 0, s, a);
+  else hipLaunchKernelGGL(k_ols_fp64<RR_MAX_COLS>, grid, block, 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_raw_fp64(const RawLaunch& L, hipStream_t s) {
+  RawArgs a;
+  a.cand_start = L.cand_start; a.prog_start = L.prog_start; a.code = L.code; a.K = L.K;
+  a.X = L.X; a.y = L.y; a.n = L.n; a.ldx = L.ldx; a.sst = L.sst; a.fit = L.fit;
+  hipLaunchKernelGGL(k_raw_fp64, dim3(L.n_cands), dim3(64), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_predict_fp64(const uint32_t* code, int len, const double* K, const double* X, int64_t n,
+                               int64_t ldx, double* out, hipStream_t s) {
+  const int64_t blocks = (n + 255) / 256;
+  hipLaunchKernelGGL(k_predict_fp64, dim3((unsigned)blocks), dim3(256), 0, s, code, len, K, X, n, ldx, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_stats_fp64(const double* y, int64_t n, double* out2, hipStream_t s) {
+  hipLaunchKernelGGL(k_stats_fp64, dim3(1), dim3(64), 0, s, y, n, out2);
+  return hipGetLastError();
+}
+
+}  // namespace rr

@@ -1,0 +1,125 @@
+"""FP64 GPU path vs the CPU oracle: bit-for-bit.
+
+The GPU kernels and the oracle implement the same canonical arithmetic
+(rr_math functions, 64-lane canonical sums, Eigen's ColPivHouseholderQR
+operation order), so coefficients, 1-R2 and RMSE must be identical bits; the
+only allowance is the sign of a NaN produced by a hardware operation (x86 and
+AMDGPU default NaNs differ), which is compared as "both NaN".
+"""
+import numpy as np
+import pytest
+
+import capi
+import oracle_lib as O
+import treegen
+
+pytestmark = pytest.mark.gpu
+
+
+def same_bits(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    both_nan = np.isnan(a) & np.isnan(b)
+    eq = a.view(np.uint64) == b.view(np.uint64)
+    return bool(np.all(eq | both_nan))
+
+
+def make_data(seed, n, d, lo=-5.0, hi=5.0):
+    rng = np.random.default_rng(seed)
+    X = rng.uniform(lo, hi, size=(n, d))
+    y = np.sin(X[:, 0]) * 3.0 + X[:, min(1, d - 1)] ** 2 - 0.5 * X[:, 0] + rng.normal(0, 0.01, n)
+    return X, y
+
+
+SPECIAL = np.array([0.0, -0.0, 1.0, -1.0, 0.5, 2.0, np.pi, 1e-300, 5e-324, 1e300, -1e300, np.inf, -np.inf, np.nan,
+                    709.78, 709.79, -745.1, -745.2, 1e8, 2.0 ** 28, 2.0 ** 28 + 1, 1e22, 6.38e189, 1e308,
+                    3.0 * np.pi / 2, 1e-8, 0.7853981633974483, 710.0, 1e16])
+
+
+def test_elementwise_ops_bitexact(ctx):
+    print("case: elementwise ops over special + random inputs", flush=True)
+    rng = np.random.default_rng(7)
+    x = np.concatenate([SPECIAL, -SPECIAL, rng.uniform(-10, 10, 3000), rng.uniform(-1e6, 1e6, 500),
+                        10.0 ** rng.uniform(-300, 300, 500), rng.normal(0, 1, 1000)])
+    y2 = rng.normal(0, 2, x.size)
+    X = np.stack([x, y2], axis=1)
+    h = ctx.upload(X, np.zeros(x.size))
+    od = O.Dataset(X, np.zeros(x.size))
+    for pre in ["sin x0", "cos x0", "ln x0", "exp x0", "sqrt x0", "sqr x0", "+ x0 x1", "- x0 x1", "* x0 x1",
+                "/ x0 x1", "< x0 x1", "> x0 x1", "== x0 x0", "!= x0 x1", "min x0 x1", "max x0 x1",
+                "/ x0 c0x0p+0", "- c0x1.8p+1 x0"]:
+        g = ctx.predict(h, pre, x.size)
+        o = od.eval(pre)
+        assert same_bits(g, o), pre
+    ctx.free(h)
+
+
+def test_dataset_stats_match(ctx):
+    X, y = make_data(3, 1000, 3)
+    h = ctx.upload(X, y)
+    assert ctx.stats(h) == O.Dataset(X, y).stats()
+    ctx.free(h)
+
+
+@pytest.mark.parametrize("n,d,seed", [(150, 2, 1), (1000, 5, 2), (4099, 10, 3)])
+def test_raw_fitness_bitexact(ctx, n, d, seed):
+    print("case: raw n=%d d=%d seed=%d" % (n, d, seed), flush=True)
+    X, y = make_data(seed, n, d)
+    trees = treegen.random_trees(seed, 300, 1, 40, d)
+    h = ctx.upload(X, y)
+    od = O.Dataset(X, y)
+    arrs = capi.build_batch([[t] for t in trees])
+    _, fit = ctx.fitness(h, arrs, mode="raw")
+    for t, f in zip(trees, fit):
+        e0, e1, _ = od.score(t)
+        assert same_bits([f["e0"], f["e1"]], [e0, e1]), t
+    ctx.free(h)
+
+
+@pytest.mark.parametrize("n,d,seed", [(150, 2, 11), (777, 4, 12), (5000, 10, 13), (64, 3, 14), (7, 3, 15)])
+def test_ols_bitexact(ctx, n, d, seed):
+    print("case: ols n=%d d=%d seed=%d" % (n, d, seed), flush=True)
+    X, y = make_data(seed, n, d)
+    trees = treegen.random_trees(seed, 250, 3, 40, d)
+    trees += ["+ + * c0x1p+1 x0 sin x1 c0x1.8p+1", "+ sin c0x1p+0 x0", "+ x0 x0", "- * x0 c0x1p+1 + x0 x0",
+              "+ + x0 sqr x1 * x0 x1", "ln - x0 x0", "+ x0 ln - x0 x0", "sin c0x1p+0"]
+    od = O.Dataset(X, y)
+    tuned = [od.tune(t) for t in trees]
+    arrs = capi.build_batch([tu["factors"] for tu in tuned])
+    h = ctx.upload(X, y)
+    coefs, fit = ctx.fitness(h, arrs, mode="ols")
+    for t, tu, c, f in zip(trees, tuned, coefs, fit):
+        assert len(c) == len(tu["coefs"]), t
+        assert same_bits(c, tu["coefs"]), (t, c, tu["coefs"])
+        assert f["rank"] == tu["rank"], t
+        assert same_bits([f["e0"], f["e1"]], [tu["fit"][0], tu["fit"][1]]), (t, f, tu["fit"])
+    ctx.free(h)
+
+
+def test_ols_chunked_slab_same_bits(ctx):
+    """A tiny slab budget forces many launches; results must not change."""
+    X, y = make_data(21, 3000, 4)
+    trees = treegen.random_trees(21, 120, 3, 30, 4)
+    od = O.Dataset(X, y)
+    arrs = capi.build_batch([od.tune(t)["factors"] for t in trees])
+    h = ctx.upload(X, y)
+    c1, f1 = ctx.fitness(h, arrs)
+    ctx.set_option("slab_budget_mb", 1)
+    c2, f2 = ctx.fitness(h, arrs)
+    ctx.set_option("slab_budget_mb", 32768)
+    assert all(same_bits(a, b) for a, b in zip(c1, c2))
+    assert same_bits(f1["e0"], f2["e0"]) and same_bits(f1["e1"], f2["e1"])
+    ctx.free(h)
+
+
+def test_bad_program_rejected_before_launch(ctx):
+    X, y = make_data(1, 100, 2)
+    h = ctx.upload(X, y)
+    arrs = capi.build_batch([["x5"]])          # variable index out of range
+    with pytest.raises(RuntimeError, match="out of range"):
+        ctx.fitness(h, arrs, mode="raw")
+    arrs = capi.build_batch([["x0"]])
+    arrs["code"] = np.array([capi.D["RR_OP_BIN"]], dtype=np.uint32)  # binary op on empty accumulator
+    with pytest.raises(RuntimeError):
+        ctx.fitness(h, arrs, mode="raw")
+    ctx.free(h)
