@@ -1,0 +1,155 @@
+// pybind_module.cpp -- the drop-in `rils_rols_cpp` extension module.
+//
+// Same module name, class and method signatures as the reference's
+// PYBIND11_MODULE (rils_rols_cpp/rils_rols_cpp.cpp:996-1007).  Differences:
+//   * input-size mismatches raise ValueError instead of printing and calling
+//     exit(1) (rils_rols_cpp.cpp:679-682, 704-707, 722-725, 733-736);
+//   * the GIL is released while fitting;
+//   * extra methods (moves, stats, precision, device) and test hooks prefixed
+//     with an underscore.
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include "search.h"
+#include "tree.h"
+
+namespace py = pybind11;
+using rrh::Flat;
+using rrh::RilsRols;
+
+namespace {
+
+py::array_t<double> as_f64(py::array_t<double, py::array::c_style | py::array::forcecast> a) { return a; }
+
+std::vector<std::string> candidates_hook(const std::string& prefix, int nvars, bool classification, bool local_search) {
+  RilsRols rr(classification, 1, 1, 0.001, 50, 1.0, false, 0);
+  std::vector<int> rel(nvars);
+  for (int i = 0; i < nvars; ++i) rel[i] = i;
+  rr.setup_nodes(rel);
+  std::vector<Flat> c = rr.all_candidates(rrh::parse_prefix(prefix), local_search);
+  std::vector<std::string> out;
+  out.reserve(c.size());
+  for (const Flat& f : c) out.push_back(rrh::to_prefix(f));
+  return out;
+}
+
+py::object rewrite_hook(const std::string& prefix, const std::string& op) {
+  Flat f = rrh::parse_prefix(prefix);
+  rrh::Pool pool;
+  if (op == "to_string") return py::str(rrh::to_string(f));
+  if (op == "size") return py::int_((int)f.size());
+  if (op == "candidate_rounds") { rrh::rewrite_rounds(f, true, pool); return py::str(rrh::to_prefix(f)); }
+  if (op == "accept_rounds") { rrh::rewrite_rounds(f, false, pool); return py::str(rrh::to_prefix(f)); }
+  const int32_t root = pool.load(f);
+  if (op == "expand") pool.expand(root);
+  else if (op == "simplify") pool.simplify(root);
+  else if (op == "normalize") pool.normalize_factor_constants(root, false);
+  else throw std::invalid_argument("unknown rewrite op " + op);
+  Flat out;
+  pool.store(root, out);
+  return py::str(rrh::to_prefix(out));
+}
+
+std::vector<std::string> factors_hook(const std::string& prefix) {
+  rrh::Pool pool;
+  std::vector<Flat> fs;
+  rrh::ols_factors(rrh::parse_prefix(prefix), fs, pool);
+  std::vector<std::string> out;
+  for (const Flat& f : fs) out.push_back(rrh::to_prefix(f));
+  return out;
+}
+
+py::tuple compile_hook(const std::string& prefix) {
+  std::vector<uint32_t> code;
+  std::vector<double> consts;
+  const int depth = rrh::compile(rrh::parse_prefix(prefix), code, consts);
+  return py::make_tuple(code, consts, depth);
+}
+
+std::string ols_tree_hook(const std::vector<std::string>& factors, const std::vector<double>& coefs) {
+  std::vector<Flat> fs;
+  for (const std::string& s : factors) fs.push_back(rrh::parse_prefix(s));
+  if (coefs.size() != fs.size() + 1) throw std::invalid_argument("need len(factors)+1 coefficients");
+  Flat out;
+  rrh::ols_tree(fs, coefs.data(), out);
+  return rrh::to_prefix(out);
+}
+
+std::vector<int> rand_hook(unsigned seed, int count) {
+  rrh::CRand r;
+  r.seed(seed);
+  std::vector<int> out;
+  for (int i = 0; i < count; ++i) out.push_back(r.next());
+  return out;
+}
+
+}  // namespace
+
+PYBIND11_MODULE(rils_rols_cpp, m) {
+  m.doc() = "RILS-ROLS with the candidate-fitness hot path on MI355X (HIP)";
+  py::class_<RilsRols>(m, "rils_rols")
+      .def(py::init<bool, int, int, double, int, double, bool, int>())
+      .def("fit",
+           [](RilsRols& self, py::array_t<double, py::array::c_style | py::array::forcecast> X,
+              py::array_t<double, py::array::c_style | py::array::forcecast> y, int data_cnt, int feat_cnt) {
+             py::buffer_info bx = X.request(), by = y.request();
+             const double* xp = (const double*)bx.ptr;
+             const double* yp = (const double*)by.ptr;
+             py::gil_scoped_release nogil;
+             self.fit(xp, bx.size, yp, by.size, data_cnt, feat_cnt);
+           })
+      .def("predict",
+           [](RilsRols& self, py::array_t<double, py::array::c_style | py::array::forcecast> X, int data_cnt,
+              int feat_cnt) {
+             py::buffer_info bx = X.request();
+             std::vector<double> out;
+             {
+               py::gil_scoped_release nogil;
+               out = self.predict((const double*)bx.ptr, bx.size, data_cnt, feat_cnt);
+             }
+             py::array_t<double> res(data_cnt);
+             std::copy(out.begin(), out.end(), (double*)res.request().ptr);
+             return res;
+           })
+      .def("get_model_string", &RilsRols::get_model_string)
+      .def("get_best_time", &RilsRols::get_best_time)
+      .def("get_fit_calls", &RilsRols::get_fit_calls)
+      .def("get_total_time", &RilsRols::get_total_time)
+      // beyond the reference
+      .def("get_moves",
+           [](const RilsRols& self) {
+             py::list out;
+             for (const rrh::Move& mv : self.moves())
+               out.append(py::make_tuple(mv.kind, mv.fit_calls, mv.e0, mv.e1, mv.size, mv.model));
+             return out;
+           })
+      .def("get_stats",
+           [](const RilsRols& self) {
+             const rrh::Stats& s = self.stats();
+             py::dict d;
+             d["sweeps"] = s.sweeps; d["batched_cands"] = s.batched_cands; d["raw_batches"] = s.raw_batches;
+             d["raw_cands"] = s.raw_cands; d["single_calls"] = s.single_calls; d["rescored_fp64"] = s.rescored_fp64;
+             d["host_gen_s"] = s.host_gen_s; d["host_prep_s"] = s.host_prep_s; d["gpu_s"] = s.gpu_s;
+             d["kernel_ms"] = s.kernel_ms; d["replay_s"] = s.replay_s;
+             return d;
+           })
+      .def("get_model_prefix", &RilsRols::final_prefix)
+      .def("set_precision",
+           [](RilsRols& self, const std::string& p) {
+             if (p == "fp64") self.set_precision(RR_FP64);
+             else if (p == "fp32") self.set_precision(RR_FP32_REFINED);
+             else throw std::invalid_argument("precision must be 'fp64' or 'fp32'");
+           })
+      .def("set_device", &RilsRols::set_device)
+      .def("set_slab_budget_mb", &RilsRols::set_slab_budget_mb);
+
+  // test hooks (host engine only; no GPU needed)
+  m.def("_all_candidates", &candidates_hook, py::arg("prefix"), py::arg("nvars"), py::arg("classification") = false,
+        py::arg("local_search") = true);
+  m.def("_rewrite", &rewrite_hook);
+  m.def("_ols_factors", &factors_hook);
+  m.def("_compile", &compile_hook);
+  m.def("_ols_tree", &ols_tree_hook);
+  m.def("_crand", &rand_hook);
+}

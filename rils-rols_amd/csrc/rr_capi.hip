@@ -144,7 +144,7 @@ int check_program(const uint32_t* code, int32_t len, int32_t d, int32_t n_consts
 int check_batch(const rr_batch* b, int32_t mode, int32_t d) {
   if (!b || b->n_cands < 0) return fail(RR_E_ARG, "bad batch");
   if (b->n_cands == 0) return RR_OK;
-  if (!b->cand_start || !b->prog_start || !b->code || (b->n_consts > 0 && !b->consts)) return fail(RR_E_ARG, "null batch array");
+  if (!b->cand_start || !b->prog_start || (b->n_consts > 0 && !b->consts)) return fail(RR_E_ARG, "null batch array");
   if (b->cand_start[0] != 0) return fail(RR_E_ARG, "cand_start[0] must be 0");
   for (int32_t c = 0; c < b->n_cands; ++c) {
     const int32_t np = b->cand_start[c + 1] - b->cand_start[c];
@@ -154,6 +154,7 @@ int check_batch(const rr_batch* b, int32_t mode, int32_t d) {
   }
   const int32_t n_progs = b->cand_start[b->n_cands];
   if (b->prog_start[0] != 0) return fail(RR_E_ARG, "prog_start[0] must be 0");
+  if (b->prog_start[n_progs] > 0 && !b->code) return fail(RR_E_ARG, "null code array");
   for (int32_t p = 0; p < n_progs; ++p) {
     const int32_t len = b->prog_start[p + 1] - b->prog_start[p];
     int rc = check_program(b->code + b->prog_start[p], len, d, b->n_consts, nullptr);
@@ -308,7 +309,8 @@ int rr_fitness_batch(rr_ctx* ctx, int32_t handle, const rr_batch* b, int32_t mod
   so += s_cand;
   if ((rc = upload(ctx, ctx->d_prog, b->prog_start, (size_t)(np + 1) * 4, so))) return rc;
   so += s_prog;
-  if ((rc = upload(ctx, ctx->d_code, b->code, (size_t)ncode * 4, so))) return rc;
+  if (ncode > 0) { if ((rc = upload(ctx, ctx->d_code, b->code, (size_t)ncode * 4, so))) return rc; }
+  else if (ctx->d_code.ensure(4) != RR_OK) return RR_E_NOMEM;
   so += s_code;
   if (b->n_consts > 0) { if ((rc = upload(ctx, ctx->d_K, b->consts, (size_t)b->n_consts * 8, so))) return rc; }
   else if (ctx->d_K.ensure(8) != RR_OK) return RR_E_NOMEM;

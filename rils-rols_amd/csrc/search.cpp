@@ -1,0 +1,660 @@
+// search.cpp -- RILS-ROLS search driver with batched GPU fitness (see search.h).
+#include "search.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <iostream>
+#include <numeric>
+#include <random>
+#include <stdexcept>
+#include <utility>
+
+namespace rrh {
+
+// ------------------------------------------------------------------ CRand
+void CRand::seed(unsigned s) {
+  memset(state_, 0, sizeof state_);
+  memset(&data_, 0, sizeof data_);
+  initstate_r(s, state_, sizeof state_, &data_);
+  srandom_r(s, &data_);
+}
+int CRand::next() {
+  int32_t r = 0;
+  random_r(&data_, &r);
+  return (int)r;
+}
+
+// ------------------------------------------------------------------ GPU session
+static void rr_check(int rc, const char* what) {
+  if (rc != RR_OK) throw std::runtime_error(std::string(what) + ": " + rr_last_error());
+}
+
+class GpuSession {
+ public:
+  rr_ctx* ctx = nullptr;
+  int32_t ds = -1;
+  int64_t n = 0;
+  int d = 0;
+
+  GpuSession(int device, int64_t slab_mb) {
+    rr_check(rr_create(device, &ctx), "rr_create (the MI355X fitness library needs a HIP device)");
+    if (slab_mb > 0) rr_check(rr_set_option(ctx, "slab_budget_mb", slab_mb), "rr_set_option");
+  }
+  ~GpuSession() {
+    if (ctx) {
+      if (ds >= 0) rr_dataset_free(ctx, ds);
+      rr_destroy(ctx);
+    }
+  }
+  void upload(const std::vector<std::vector<double>>& X, const std::vector<double>& y) {
+    n = (int64_t)y.size();
+    d = (int)X.size();
+    std::vector<double> fm((size_t)n * std::max(d, 1));
+    for (int j = 0; j < d; ++j) std::copy(X[j].begin(), X[j].end(), fm.begin() + (size_t)j * n);
+    if (ds >= 0) rr_dataset_free(ctx, ds);
+    rr_check(rr_dataset_upload(ctx, fm.data(), y.data(), n, d, &ds), "rr_dataset_upload");
+  }
+
+  struct Builder {
+    std::vector<int32_t> cand_start{0}, prog_start{0};
+    std::vector<uint32_t> code;
+    std::vector<double> consts;
+    void add(const Flat& f) {
+      compile(f, code, consts);
+      prog_start.push_back((int32_t)code.size());
+    }
+    void close_candidate() { cand_start.push_back((int32_t)prog_start.size() - 1); }
+    rr_batch view() const {
+      rr_batch b;
+      b.n_cands = (int32_t)cand_start.size() - 1;
+      b.cand_start = cand_start.data();
+      b.prog_start = prog_start.data();
+      b.code = code.data();
+      b.consts = consts.data();
+      b.n_consts = (int32_t)consts.size();
+      return b;
+    }
+  };
+
+  void raw(const std::vector<const Flat*>& trees, std::vector<rr_fit>& out, Stats& st) {
+    Builder B;
+    for (const Flat* t : trees) { B.add(*t); B.close_candidate(); }
+    out.resize(trees.size());
+    if (trees.empty()) return;
+    const rr_batch b = B.view();
+    const auto t0 = std::chrono::steady_clock::now();
+    rr_check(rr_fitness_batch(ctx, ds, &b, RR_MODE_RAW, RR_FP64, nullptr, out.data()), "rr_fitness_batch(raw)");
+    st.gpu_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    double ms = 0;
+    rr_last_kernel_ms(ctx, &ms);
+    st.kernel_ms += ms;
+    st.raw_batches++;
+    st.raw_cands += (int64_t)trees.size();
+  }
+
+  // factors[c] are the OLS factors of candidate c; coefficients of c start at coefs[off[c]]
+  void ols(const std::vector<std::vector<Flat>>& factors, size_t count, int precision, std::vector<double>& coefs,
+           std::vector<int64_t>& off, std::vector<rr_fit>& out, Stats& st) {
+    Builder B;
+    off.resize(count);
+    int64_t total = 0;
+    for (size_t c = 0; c < count; ++c) {
+      for (const Flat& f : factors[c]) B.add(f);
+      B.close_candidate();
+      off[c] = total;
+      total += (int64_t)factors[c].size() + 1;
+    }
+    coefs.resize(std::max<int64_t>(total, 1));
+    out.resize(count);
+    if (count == 0) return;
+    const rr_batch b = B.view();
+    const auto t0 = std::chrono::steady_clock::now();
+    rr_check(rr_fitness_batch(ctx, ds, &b, RR_MODE_OLS, precision, coefs.data(), out.data()), "rr_fitness_batch(ols)");
+    st.gpu_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    double ms = 0;
+    rr_last_kernel_ms(ctx, &ms);
+    st.kernel_ms += ms;
+  }
+};
+
+// ------------------------------------------------------------------ basics
+static double secs_since(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+RilsRols::RilsRols(bool classification, int max_fit_calls, int max_seconds, double complexity_penalty,
+                   int max_complexity, double sample_size, bool verbose, int random_state)
+    : max_fit_calls_(max_fit_calls), max_seconds_(max_seconds), random_state_(random_state),
+      complexity_penalty_(complexity_penalty), sample_size_(sample_size), max_complexity_(max_complexity),
+      classification_(classification), verbose_(verbose) {
+  reset();
+}
+
+RilsRols::~RilsRols() { delete gpu_; }
+
+void RilsRols::reset() {   // rils_rols_cpp.cpp:106-117
+  main_it_ = 0;
+  last_improved_it_ = 0;
+  fit_calls_ = 0;
+  ls_calls_ = 0;
+  start_time_ = std::chrono::steady_clock::now();
+  checked_perts_.clear();
+  skipped_perts_ = 0;
+  total_perts_ = 0;
+  rng_.seed((unsigned)random_state_);
+}
+
+bool RilsRols::finished() const {   // rils_rols_cpp.cpp:659-661
+  const auto el = std::chrono::duration_cast<std::chrono::seconds>(std::chrono::steady_clock::now() - start_time_).count();
+  return fit_calls_ >= max_fit_calls_ || el > max_seconds_;
+}
+
+bool RilsRols::check_skip(const std::string& s) {   // 663-672
+  total_perts_++;
+  if (checked_perts_.find(s) != checked_perts_.end()) { skipped_perts_++; return true; }
+  checked_perts_.insert(s);
+  return false;
+}
+
+double RilsRols::fitness_value(const Fit& f) const {   // 543-545
+  return (1 + f.e0) * (1 + f.e1) * (1 + f.size * complexity_penalty_);
+}
+
+int RilsRols::compare_fitness(const Fit& a, const Fit& b) const {   // 547-561
+  if ((a.size > max_complexity_ || b.size > max_complexity_) && a.size != b.size) return a.size - b.size;
+  const double x = fitness_value(a), y = fitness_value(b);
+  if (x < y) return -1;
+  if (x > y) return 1;
+  return 0;
+}
+
+bool RilsRols::dominates(const Fit& p, const Fit& f) { return p.e0 <= f.e0 && p.e1 <= f.e1 && p.size <= f.size; }
+bool RilsRols::is_dominated(const std::vector<Fit>& par, const Fit& f) {
+  for (const Fit& p : par) if (dominates(p, f)) return true;
+  return false;
+}
+void RilsRols::add_to_pareto(std::vector<Fit>& par, const Fit& f) {   // 592-599
+  if (is_dominated(par, f)) return;
+  for (int i = (int)par.size() - 1; i >= 0; --i)
+    if (dominates(f, par[i])) par.erase(par.begin() + i);
+  par.push_back(f);
+}
+
+void RilsRols::print_state(const Fit& c) const {   // 563-579
+  std::cout << "it=" << main_it_ << "\tfit_calls=" << fit_calls_ << "\tls_calls=" << ls_calls_;
+  if (classification_) {
+    std::cout << "\tcurr_LOSS=" << c.e0 << "\tcurr_size=" << c.size;
+    std::cout << "\tfinal_LOSS=" << final_fitness_.e0 << "\tfinal_size=" << final_fitness_.size;
+  } else {
+    std::cout << "\tcurr_R2=" << (1 - c.e0) << "\tcurr_RMSE=" << c.e1 << "\tcurr_size=" << c.size;
+    std::cout << "\tfinal_R2=" << (1 - final_fitness_.e0) << "\tfinal_RMSE=" << final_fitness_.e1
+              << "\tfinal_size=" << final_fitness_.size;
+  }
+  std::cout << "\tchecks_skip=" << skipped_perts_ << "/" << total_perts_ << "\tsol=";
+  std::cout << to_string(final_solution_) << std::endl << std::endl;
+}
+
+void RilsRols::log_move(int kind, const Flat& t, const Fit& f) {
+  moves_.push_back(Move{kind, fit_calls_, f.e0, f.e1, f.size, to_string(t)});
+}
+
+// ------------------------------------------------------------------ generators
+void RilsRols::setup_nodes(const std::vector<int>& rel) {   // 121-148
+  const uint8_t ops[] = {PLUS, MINUS, MULTIPLY, DIVIDE, SIN, COS, LN, EXP, SQRT, SQR};
+  for (uint8_t t : ops) allowed_.push_back(tok_op(t));
+  const double consts[] = {-1., 0., 0.5, 1., 2., 3.14159265358979323846, 10.};
+  for (double c : consts) allowed_.push_back(tok_const(c));
+  for (int v : rel) allowed_.push_back(tok_var(v));
+  if (classification_) {
+    const uint8_t cops[] = {LT, GT, EQ, NEQ, MIN, MAX};
+    for (uint8_t t : cops) allowed_.push_back(tok_op(t));
+  }
+  if (verbose_) std::cout << "Finished creating allowed nodes" << std::endl;
+}
+
+// node.cpp:97-118
+static bool allowed_left(uint8_t parent, uint8_t child) {
+  switch (parent) {
+    case EXP: case LN: return !(child == EXP || child == LN);
+    case POW: return child != POW;
+    case COS: case SIN: return !(child == COS || child == SIN);
+    default: return true;
+  }
+}
+
+// get_all_subtrees (node.h:351-377): the subtree at 0, then BFS order
+static void bfs_subtrees(const Flat& o, const std::vector<int32_t>& sz, std::vector<Flat>& out) {
+  std::vector<int32_t> q{0};
+  for (size_t h = 0; h < q.size(); ++h) {
+    const int p = q[h];
+    out.emplace_back(o.begin() + p, o.begin() + p + sz[p]);
+    const int a = arity(o[p].t);
+    if (a >= 1) q.push_back(p + 1);
+    if (a >= 2) q.push_back(p + 1 + sz[p + 1]);
+  }
+}
+
+static Flat join1(uint8_t op, const Flat& a) {
+  Flat f;
+  f.reserve(a.size() + 1);
+  f.push_back(tok_op(op));
+  f.insert(f.end(), a.begin(), a.end());
+  return f;
+}
+static Flat join2(uint8_t op, const Flat& a, const Flat& b) {
+  Flat f;
+  f.reserve(a.size() + b.size() + 1);
+  f.push_back(tok_op(op));
+  f.insert(f.end(), a.begin(), a.end());
+  f.insert(f.end(), b.begin(), b.end());
+  return f;
+}
+
+void RilsRols::add_const_finetune(const Flat& o, std::vector<Flat>& c) {   // 163-179
+  if (o[0].t != CONST) return;
+  if (o[0].c == 0.0) { c.push_back({tok_const(-1.0)}); c.push_back({tok_const(1.0)}); return; }
+  const double mult[] = {-1., 0.01, 0.1, 0.2, 0.5, 0.8, 0.9, 0., 1, 1.1, 1.2, 2., 3.14159265358979323846,
+                         5., 10., 20., 50., 100.};
+  for (double m : mult) c.push_back({tok_const(o[0].c * m)});
+}
+
+void RilsRols::add_change_to_subtree(const Flat& o, const std::vector<int32_t>& sz, std::vector<Flat>& c) {  // 200-209
+  const int a = arity(o[0].t);
+  if (a >= 1) {
+    Flat l(o.begin() + 1, o.begin() + 1 + sz[1]);
+    std::vector<int32_t> lsz(sz.begin() + 1, sz.begin() + 1 + sz[1]);
+    bfs_subtrees(l, lsz, c);
+  }
+  if (a >= 2) {
+    const int r = 1 + sz[1];
+    Flat rt(o.begin() + r, o.begin() + r + sz[r]);
+    std::vector<int32_t> rsz(sz.begin() + r, sz.begin() + r + sz[r]);
+    bfs_subtrees(rt, rsz, c);
+  }
+}
+
+void RilsRols::add_change_to_var_or_1(const Flat& o, std::vector<Flat>& c) {   // 222-232
+  for (const Tok& n : allowed_) {
+    if (n.t != VAR) continue;
+    if (o[0].t == VAR && o[0].var == n.var) continue;
+    c.push_back({n});
+  }
+  c.push_back({tok_const(1.0)});
+}
+
+void RilsRols::add_change_unary_applied(const Flat& o, std::vector<Flat>& c) {   // 245-252
+  for (const Tok& n : allowed_) {
+    if (arity(n.t) != 1 || !allowed_left(n.t, o[0].t)) continue;
+    c.push_back(join1(n.t, o));
+  }
+}
+
+void RilsRols::add_change_unary_to_another(const Flat& o, std::vector<Flat>& c) {   // 258-270
+  if (arity(o[0].t) != 1) return;
+  const Flat child(o.begin() + 1, o.end());
+  for (const Tok& n : allowed_) {
+    if (arity(n.t) != 1 || n.t == o[0].t) continue;
+    if (!allowed_left(n.t, child[0].t)) continue;
+    c.push_back(join1(n.t, child));
+  }
+}
+
+void RilsRols::add_change_binary_applied(const Flat& o, const std::vector<int32_t>& sz, std::vector<Flat>& c) {  // 272-296
+  std::vector<Flat> args;
+  bfs_subtrees(o, sz, args);
+  for (const Tok& n : allowed_)
+    if (n.t == VAR || n.t == CONST) args.push_back({n});
+  for (const Tok& nb : allowed_) {
+    if (arity(nb.t) != 2) continue;
+    for (const Flat& a : args) {
+      if (allowed_left(nb.t, o[0].t)) c.push_back(join2(nb.t, o, a));
+      if (!symmetric(nb.t) && allowed_left(nb.t, a[0].t)) c.push_back(join2(nb.t, a, o));
+    }
+  }
+}
+
+void RilsRols::add_change_binary_to_another(const Flat& o, const std::vector<int32_t>& sz, std::vector<Flat>& c) {  // 303-316
+  if (arity(o[0].t) != 2) return;
+  const Flat l(o.begin() + 1, o.begin() + 1 + sz[1]);
+  const Flat r(o.begin() + 1 + sz[1], o.end());
+  for (const Tok& nb : allowed_) {
+    if (arity(nb.t) != 2 || nb.t == o[0].t) continue;
+    if (allowed_left(nb.t, l[0].t)) c.push_back(join2(nb.t, l, r));
+    if (!symmetric(nb.t) && allowed_left(nb.t, r[0].t)) c.push_back(join2(nb.t, r, l));
+  }
+}
+
+void RilsRols::gen_perturb(const Flat& o, std::vector<Flat>& c) {   // 318-330
+  std::vector<int32_t> sz;
+  subtree_sizes(o, sz);
+  add_change_to_subtree(o, sz, c);
+  add_change_to_var_or_1(o, c);
+  if (o[0].t == VAR) add_change_unary_applied(o, c);
+  add_change_unary_to_another(o, c);
+  if (o[0].t == VAR || o[0].t == CONST) add_change_binary_applied(o, sz, c);
+  add_change_binary_to_another(o, sz, c);
+}
+
+void RilsRols::gen_change(const Flat& o, std::vector<Flat>& c) {   // 332-346
+  std::vector<int32_t> sz;
+  subtree_sizes(o, sz);
+  add_const_finetune(o, c);
+  add_change_to_subtree(o, sz, c);
+  add_change_to_var_or_1(o, c);
+  add_change_unary_applied(o, c);
+  add_change_unary_to_another(o, c);
+  add_change_binary_applied(o, sz, c);
+  add_change_binary_to_another(o, sz, c);
+}
+
+// rils_rols_cpp.cpp:348-443.  Candidate strings for the dedup are assembled
+// from the solution's string around the changed site instead of re-printing
+// the whole tree.
+std::vector<Flat> RilsRols::all_candidates(const Flat& sol, bool local_search) {
+  std::vector<int32_t> sz;
+  subtree_sizes(sol, sz);
+  std::string full;
+  std::vector<int32_t> be;
+  append_string(sol, 0, full, &be);
+
+  std::vector<Flat> all;
+  all.reserve(3000);
+  std::unordered_set<std::string> seen;
+  seen.reserve(6000);
+  std::vector<Flat> cands;
+  std::string key, cs;
+  auto get = [&](const Flat& t) {
+    cands.clear();
+    if (local_search) gen_change(t, cands); else gen_perturb(t, cands);
+  };
+  auto site = [&](int p) {   // candidates replacing the subtree at position p
+    const Flat old(sol.begin() + p, sol.begin() + p + sz[p]);
+    get(old);
+    for (const Flat& c : cands) {
+      key.assign(full, 0, be[2 * p]);
+      cs.clear();
+      append_string(c, 0, cs);
+      key += cs;
+      key.append(full, be[2 * p + 1], std::string::npos);
+      if (seen.insert(key).second) {
+        Flat f;
+        f.reserve(sol.size() - sz[p] + c.size());
+        f.insert(f.end(), sol.begin(), sol.begin() + p);
+        f.insert(f.end(), c.begin(), c.end());
+        f.insert(f.end(), sol.begin() + p + sz[p], sol.end());
+        all.push_back(std::move(f));
+      }
+    }
+  };
+  std::vector<int32_t> q{0};
+  for (size_t h = 0; h < q.size(); ++h) {
+    const int p = q[h];
+    if (h == 0) site(0);
+    const int a = arity(sol[p].t);
+    if (a >= 1) { site(p + 1); q.push_back(p + 1); }
+    if (a >= 2) { const int r = p + 1 + sz[p + 1]; site(r); q.push_back(r); }
+  }
+  for (Flat& f : all) rewrite_rounds(f, true, pool_);
+  std::unordered_set<std::string> seen2;
+  seen2.reserve(all.size() * 2);
+  std::vector<Flat> out;
+  out.reserve(all.size());
+  for (Flat& f : all) {
+    if (seen2.insert(to_string(f)).second) out.push_back(std::move(f));
+  }
+  return out;
+}
+
+// ------------------------------------------------------------------ fitness
+static Fit to_fit(const rr_fit& r, int size) {
+  if (r.flags & RR_F_SENTINEL) return Fit{1000, 1000, 1000};   // rils_rols_cpp.cpp:536-537
+  return Fit{r.e0, r.e1, size};
+}
+
+Fit RilsRols::fitness_raw(const Flat& t) {   // fitness(), 520-541
+  fit_calls_++;
+  std::vector<const Flat*> one{&t};
+  std::vector<rr_fit> out;
+  gpu_->raw(one, out, stats_);
+  stats_.single_calls++;
+  return to_fit(out[0], (int)t.size());
+}
+
+Flat RilsRols::tune_and_fit(const Flat& cand, Fit* fit) {   // tune_constants (445-518) + fitness
+  std::vector<std::vector<Flat>> facs(1);
+  ols_factors(cand, facs[0], pool_);
+  std::vector<double> coefs;
+  std::vector<int64_t> off;
+  std::vector<rr_fit> out;
+  gpu_->ols(facs, 1, RR_FP64, coefs, off, out, stats_);
+  stats_.single_calls++;
+  Flat tree;
+  ols_tree(facs[0], coefs.data(), tree);
+  fit_calls_++;
+  *fit = to_fit(out[0], (int)tree.size());
+  return tree;
+}
+
+Flat RilsRols::local_search(const Flat& passed, Fit* out_fit) {   // 601-643
+  std::vector<Fit> pareto;
+  ls_calls_++;
+  bool improved = true;
+  Fit curr_fit;
+  Flat curr = tune_and_fit(passed, &curr_fit);
+  std::vector<std::vector<Flat>> facs;
+  std::vector<double> coefs;
+  std::vector<int64_t> off;
+  std::vector<rr_fit> res;
+  std::vector<int32_t> fsz;
+  while (improved && !finished()) {
+    improved = false;
+    auto t0 = std::chrono::steady_clock::now();
+    std::vector<Flat> perts = all_candidates(curr, true);
+    stats_.host_gen_s += secs_since(t0);
+
+    // every candidate the replay can reach costs at least one fit call
+    const int64_t room = std::max<int64_t>((int64_t)max_fit_calls_ - fit_calls_, 0);
+    const size_t m = std::min<size_t>(perts.size(), (size_t)room);
+    t0 = std::chrono::steady_clock::now();
+    if (facs.size() < m) facs.resize(m);
+    for (size_t j = 0; j < m; ++j) ols_factors(perts[j], facs[j], pool_);
+    stats_.host_prep_s += secs_since(t0);
+    gpu_->ols(facs, m, precision_ == RR_FP32_REFINED ? RR_FP32_REFINED : RR_FP64, coefs, off, res, stats_);
+    stats_.sweeps++;
+    stats_.batched_cands += (int64_t)m;
+
+    t0 = std::chrono::steady_clock::now();
+    for (size_t j = 0; j < perts.size(); ++j) {
+      if (finished()) break;
+      if (j >= m) throw std::logic_error("local-search replay ran past the scored batch");
+      const double* cj = coefs.data() + off[j];
+      fsz.clear();
+      for (const Flat& f : facs[j]) fsz.push_back((int32_t)f.size());
+      fit_calls_++;
+      Fit tf = to_fit(res[j], ols_tree_size(fsz, cj));
+      if (verbose_ && fit_calls_ % 10000 == 0) print_state(curr_fit);
+      if (!is_dominated(pareto, tf) && compare_fitness(tf, curr_fit) < 0) {
+        improved = true;
+        Flat tuned;
+        ols_tree(facs[j], cj, tuned);
+        rewrite_rounds(tuned, false, pool_);   // expand + simplify rounds, 621-630
+        tf = fitness_raw(tuned);
+        curr = tuned;
+        curr_fit = tf;
+        add_to_pareto(pareto, tf);
+        log_move(0, curr, curr_fit);
+      }
+    }
+    stats_.replay_s += secs_since(t0);
+  }
+  *out_fit = curr_fit;
+  return curr;
+}
+
+// canonical-order sum (64 partials, row i to partial i mod 64, xor butterfly)
+static double canon_sum_host(const double* v, int64_t n, const double* w = nullptr, double wshift = 0.0) {
+  double acc[64];
+  for (int l = 0; l < 64; ++l) acc[l] = 0.0;
+  for (int64_t i = 0; i < n; ++i) {
+    const double t = w ? (v[i] - w[i]) * (v[i] - w[i]) : (v[i] - wshift) * (v[i] - wshift);
+    acc[i & 63] = acc[i & 63] + t;
+  }
+  for (int s = 32; s >= 1; s >>= 1) {
+    double nx[64];
+    for (int l = 0; l < 64; ++l) nx[l] = acc[l] + acc[l ^ s];
+    for (int l = 0; l < 64; ++l) acc[l] = nx[l];
+  }
+  return acc[0];
+}
+
+// 753-770: with more than 200 features keep the 200 with the largest R2(X[i], y)
+std::vector<int> RilsRols::relevant_features(const std::vector<std::vector<double>>& X, const std::vector<double>& y) {
+  const int d = (int)X.size();
+  std::vector<int> rel;
+  if (d <= max_feat_) { rel.resize(d); std::iota(rel.begin(), rel.end(), 0); return rel; }
+  const int64_t n = (int64_t)y.size();
+  std::vector<std::tuple<double, int>> by;
+  for (int i = 0; i < d; ++i) {
+    const double* x = X[i].data();
+    double acc[64];
+    for (int l = 0; l < 64; ++l) acc[l] = 0.0;
+    for (int64_t r = 0; r < n; ++r) acc[r & 63] = acc[r & 63] + x[r];
+    for (int s = 32; s >= 1; s >>= 1) { double nx[64]; for (int l = 0; l < 64; ++l) nx[l] = acc[l] + acc[l ^ s]; for (int l = 0; l < 64; ++l) acc[l] = nx[l]; }
+    const double xbar = acc[0] / (double)n;
+    const double ssr = canon_sum_host(x, n, y.data());
+    const double sst = canon_sum_host(x, n, nullptr, xbar);
+    by.push_back(std::tuple<double, int>{1 - ssr / sst, i});
+  }
+  std::sort(by.begin(), by.end(), std::greater<>());
+  for (int i = 0; i < max_feat_; ++i) rel.push_back(std::get<1>(by[i]));
+  return rel;
+}
+
+void RilsRols::fit_inner(const std::vector<std::vector<double>>& X_all, const std::vector<double>& y_all) {  // 772-859
+  reset();
+  moves_.clear();
+  stats_ = Stats();
+  const int64_t N = (int64_t)y_all.size();
+  const int sample_cnt = int(sample_size_ * (double)N);
+  std::vector<int> selected(N);
+  std::iota(selected.begin(), selected.end(), 0);
+  std::shuffle(selected.begin(), selected.end(), std::default_random_engine(random_state_));
+  std::vector<std::vector<double>> X(X_all.size(), std::vector<double>(std::max(sample_cnt, 0)));
+  std::vector<double> y(std::max(sample_cnt, 0));
+  for (int ix = 0; ix < sample_cnt; ++ix) {
+    const int i = selected[ix];
+    for (size_t j = 0; j < X.size(); ++j) X[j][ix] = X_all[j][i];
+    y[ix] = y_all[i];
+  }
+  if (sample_cnt <= 0) throw std::invalid_argument("sample_size * n must be at least one sample");
+  std::vector<int> rel = relevant_features(X, y);
+  setup_nodes(rel);
+  if (!gpu_) gpu_ = new GpuSession(device_, slab_budget_mb_);
+  gpu_->upload(X, y);
+
+  final_solution_ = {tok_const(0.0)};
+  final_fitness_ = fitness_raw(final_solution_);
+  bool improved = true;
+  std::vector<rr_fit> res;
+  while (!finished()) {
+    main_it_ += 1;
+    Flat start = final_solution_;
+    if (!improved) {
+      std::vector<Flat> all_perts = all_candidates(final_solution_, false);
+      const Flat p1 = all_perts[(size_t)rng_.next() % all_perts.size()];
+      std::vector<Flat> all2 = all_candidates(p1, false);
+      start = all2[(size_t)rng_.next() % all2.size()];
+      if (verbose_) std::cout << "Randomized to " << to_string(start) << std::endl;
+    }
+    improved = false;
+    std::vector<Flat> all_perts = all_candidates(start, false);
+    if (verbose_) std::cout << "Checking " << all_perts.size() << " perturbations of starting solution." << std::endl;
+    // 819-830: the perturbations that get scored, then one batched score
+    std::vector<const Flat*> sel;
+    {
+      const auto el = [&]() {
+        return std::chrono::duration_cast<std::chrono::seconds>(std::chrono::steady_clock::now() - start_time_).count();
+      };
+      int64_t virt = fit_calls_;
+      for (size_t i = 0; i < all_perts.size(); ++i) {
+        if (virt >= max_fit_calls_ || el() > max_seconds_) break;
+        if (check_skip(to_string(all_perts[i]))) continue;
+        sel.push_back(&all_perts[i]);
+        ++virt;
+      }
+    }
+    gpu_->raw(sel, res, stats_);
+    fit_calls_ += (int)sel.size();
+    std::vector<std::pair<double, const Flat*>> by;
+    by.reserve(sel.size());
+    for (size_t i = 0; i < sel.size(); ++i) by.emplace_back(to_fit(res[i], (int)sel[i]->size()).e0, sel[i]);
+    std::sort(by.begin(), by.end(), [](const std::pair<double, const Flat*>& a, const std::pair<double, const Flat*>& b) {
+      return a.first < b.first;
+    });
+    for (size_t i = 0; i < by.size(); ++i) {   // 833-855
+      if (finished()) break;
+      const Flat& lp = *by[i].second;
+      checked_perts_.insert(to_string(lp));
+      Fit ls_fit;
+      Flat ls = local_search(lp, &ls_fit);
+      fit_calls_++;   // fitness(ls_pert), 842: the same tree local_search just scored
+      if (compare_fitness(ls_fit, final_fitness_) < 0) {
+        improved = true;
+        final_solution_ = ls;
+        final_fitness_ = ls_fit;
+        if (verbose_) print_state(final_fitness_);
+        best_time_ = std::chrono::duration_cast<std::chrono::milliseconds>(std::chrono::steady_clock::now() - start_time_).count() / 1000.0;
+        log_move(1, final_solution_, final_fitness_);
+      }
+    }
+  }
+  total_time_ = std::chrono::duration_cast<std::chrono::milliseconds>(std::chrono::steady_clock::now() - start_time_).count() / 1000.0;
+}
+
+void RilsRols::fit(const double* Xp, int64_t x_size, const double* yp, int64_t y_size, int data_cnt, int feat_cnt) {
+  if (x_size != (int64_t)data_cnt * feat_cnt)
+    throw std::invalid_argument("Size of X " + std::to_string(x_size) +
+                                " is not the same as the product of data count and feature count " +
+                                std::to_string((int64_t)data_cnt * feat_cnt));
+  if (y_size != data_cnt)
+    throw std::invalid_argument("Size of y " + std::to_string(y_size) + " is not the same as the data count " +
+                                std::to_string(data_cnt));
+  std::vector<std::vector<double>> Xe(feat_cnt, std::vector<double>(data_cnt));
+  for (int i = 0; i < data_cnt; ++i)
+    for (int j = 0; j < feat_cnt; ++j) Xe[j][i] = Xp[(int64_t)i * feat_cnt + j];
+  std::vector<double> ye(yp, yp + data_cnt);
+  fit_inner(Xe, ye);
+}
+
+std::vector<double> RilsRols::predict(const double* Xp, int64_t x_size, int data_cnt, int feat_cnt) {
+  if (x_size != (int64_t)data_cnt * feat_cnt)
+    throw std::invalid_argument("Size of X " + std::to_string(x_size) +
+                                " is not the same as the product of data count and feature count " +
+                                std::to_string((int64_t)data_cnt * feat_cnt));
+  if (final_solution_.empty()) throw std::runtime_error("predict called before fit");
+  if (!gpu_) gpu_ = new GpuSession(device_, slab_budget_mb_);
+  std::vector<double> fm((size_t)data_cnt * std::max(feat_cnt, 1));
+  for (int i = 0; i < data_cnt; ++i)
+    for (int j = 0; j < feat_cnt; ++j) fm[(size_t)j * data_cnt + i] = Xp[(int64_t)i * feat_cnt + j];
+  std::vector<double> zeros(data_cnt, 0.0), out(data_cnt);
+  int32_t h = -1;
+  rr_check(rr_dataset_upload(gpu_->ctx, fm.data(), zeros.data(), data_cnt, feat_cnt, &h), "rr_dataset_upload");
+  std::vector<uint32_t> code;
+  std::vector<double> consts;
+  compile(final_solution_, code, consts);
+  const int rc = rr_predict(gpu_->ctx, h, code.data(), (int32_t)code.size(), consts.data(), (int32_t)consts.size(),
+                            out.data());
+  rr_dataset_free(gpu_->ctx, h);
+  rr_check(rc, "rr_predict");
+  if (classification_)
+    for (double& v : out) v = v >= 0.5 ? 1.0 : 0.0;
+  return out;
+}
+
+std::string RilsRols::get_model_string() const {
+  if (final_solution_.empty()) throw std::runtime_error("no model: call fit first");
+  return to_string(final_solution_);
+}
+
+}  // namespace rrh

@@ -1,0 +1,113 @@
+"""End-to-end search: the product (rils_rols_cpp on the GPU) vs the oracle.
+
+For a fixed seed the product must accept the same sequence of moves as the
+CPU restatement of the reference's search: every local-search acceptance and
+every new best solution, with the fit-call count at which it happened, the
+exact fitness bits and the model string; and end with the same model and the
+same number of fit calls.  The time budget is set so large that only the
+fit-call budget ends a run (the wall-clock part of finished(),
+rils_rols_cpp.cpp:659-661, is not reproducible).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+BIG_TIME = 10 ** 8
+
+
+def load_config1(which="train"):
+    d = np.loadtxt(os.path.join(HERE, "golden", "config1_%s.csv" % which), delimiter=",", skiprows=1)
+    return d[:, :2], d[:, 2]
+
+
+def run_product(X, y, classification=False, max_fit_calls=100000, complexity_penalty=0.001, max_complexity=50,
+                sample_size=1.0, random_state=0, precision="fp64"):
+    import rils_rols_cpp
+    n, d = X.shape
+    rr = rils_rols_cpp.rils_rols(classification, max_fit_calls, BIG_TIME, complexity_penalty, max_complexity,
+                                 sample_size, False, random_state)
+    rr.set_precision(precision)
+    rr.fit(np.ascontiguousarray(X, dtype=np.float64).reshape(-1), np.ascontiguousarray(y, dtype=np.float64), n, d)
+    return rr
+
+
+def assert_same_run(rr, want):
+    moves = [tuple(m) for m in rr.get_moves()]
+    assert len(moves) == len(want["moves"])
+    for got, exp in zip(moves, want["moves"]):
+        assert got == exp, (got, exp)
+    assert rr.get_fit_calls() == want["fit_calls"]
+    assert rr.get_model_string() == want["model"]
+
+
+@pytest.mark.parametrize("max_fit_calls", [100000])
+def test_config1_move_sequence_identical(max_fit_calls):
+    print("case: config1 max_fit_calls=%d seed=12345" % max_fit_calls, flush=True)
+    X, y = load_config1()
+    want = O.fit(X, y, max_fit_calls=max_fit_calls, max_time=BIG_TIME, random_state=12345)
+    rr = run_product(X, y, max_fit_calls=max_fit_calls, random_state=12345)
+    assert_same_run(rr, want)
+    print("config1 model:", rr.get_model_string(), "fit_calls:", rr.get_fit_calls(), "stats:", rr.get_stats(),
+          flush=True)
+    # predict on the held-out rows equals the oracle's evaluation of the same model
+    Xt, yt = load_config1("test")
+    od = O.Dataset(Xt, yt)
+    got = rr.predict(np.ascontiguousarray(Xt).reshape(-1), Xt.shape[0], Xt.shape[1])
+    assert np.array_equal(got, od.eval(rr.get_model_prefix()))
+
+
+@pytest.mark.parametrize("seed,n,d,budget", [(1, 300, 3, 8000), (7, 1000, 5, 5000), (23654, 2000, 2, 4000)])
+def test_random_problem_move_sequence_identical(seed, n, d, budget):
+    print("case: random problem seed=%d n=%d d=%d budget=%d" % (seed, n, d, budget), flush=True)
+    rng = np.random.default_rng(seed)
+    X = rng.uniform(-3, 3, (n, d))
+    y = 1.5 * np.sin(X[:, 0]) * X[:, d - 1] - 0.7 * X[:, 0] ** 2 + rng.normal(0, 0.01, n)
+    want = O.fit(X, y, max_fit_calls=budget, max_time=BIG_TIME, random_state=seed, sample_size=0.9)
+    rr = run_product(X, y, max_fit_calls=budget, random_state=seed, sample_size=0.9)
+    assert_same_run(rr, want)
+
+
+def test_classification_move_sequence_identical():
+    print("case: breast_cancer classification budget=3000", flush=True)
+    from sklearn.datasets import load_breast_cancer
+    from sklearn.model_selection import train_test_split
+    X, y = load_breast_cancer(return_X_y=True)
+    X, _, y, _ = train_test_split(X, y, train_size=0.75, random_state=12345)
+    want = O.fit(X, y, classification=True, max_fit_calls=3000, max_time=BIG_TIME, max_complexity=20,
+                 random_state=12345)
+    rr = run_product(X, y, classification=True, max_fit_calls=3000, max_complexity=20, random_state=12345)
+    assert_same_run(rr, want)
+    p = rr.predict(np.ascontiguousarray(X).reshape(-1), X.shape[0], X.shape[1])
+    assert set(np.unique(p)) <= {0.0, 1.0}
+
+
+def test_python_api_drop_in():
+    """The estimator API of rils_rols/rils_rols.py:154-188 on the GPU path."""
+    print("case: python API regressor/classifier", flush=True)
+    from rils_rols import RILSROLSBinaryClassifier, RILSROLSRegressor
+    X, y = load_config1()
+    Xt, yt = load_config1("test")
+    reg = RILSROLSRegressor(max_fit_calls=5000, sample_size=1, random_state=12345)
+    reg.fit(X, y)
+    want = O.fit(X, y, max_fit_calls=5000, max_time=100, random_state=12345)
+    assert reg.model == want["model"]
+    assert reg.fit_calls == want["fit_calls"]
+    assert reg.score(Xt, yt) > 0.99
+    assert "fitCalls=%d" % reg.fit_calls in reg.fit_report_string()
+    assert reg.model_string() is not None
+    Xc = X.copy()
+    yc = (y > np.median(y)).astype(float)
+    clf = RILSROLSBinaryClassifier(max_fit_calls=2000, max_complexity=20, random_state=3)
+    clf.fit(Xc, yc)
+    pr = clf.predict_proba(Xc)
+    assert pr.shape == (len(yc), 2)
+    assert 0.0 <= clf.score(Xc, yc) <= 1.0
+    with pytest.raises(Exception):
+        clf.fit(Xc, y)   # non-binary targets
+    with pytest.raises(Exception):
+        RILSROLSRegressor().predict(X)   # predict before fit

@@ -1,0 +1,244 @@
+"""Pins the CPU oracle (oracle/rr_oracle.cpp) against facts that do not come
+from the oracle itself: mpmath, closed forms, numpy/scipy least squares, IEEE
+special values, and known answers derived by hand from the reference source
+(node.cpp, node.h, rils_rols_cpp.cpp).  The reference has no tests or golden
+vectors of its own; these are the checks that a dropped term, a wrong sign or
+a transposed operand in the restatement would fail.
+"""
+import math
+
+import mpmath as mp
+import numpy as np
+import pytest
+import scipy.linalg
+
+import oracle_lib as O
+
+mp.mp.prec = 200
+
+
+# ---------------------------------------------------------------- rr_math
+def ulp_error(op, xs):
+    got = O.math(op, xs)
+    worst = 0.0
+    f = {"sin": mp.sin, "cos": mp.cos, "exp": mp.exp, "log": mp.log}[op]
+    for x, g in zip(xs, got):
+        t = f(mp.mpf(float(x)))
+        if not mp.isfinite(t):
+            continue
+        tf = float(t)
+        if tf == 0.0 or not math.isfinite(tf) or abs(tf) < 2.3e-308:
+            continue
+        worst = max(worst, float(abs(mp.mpf(float(g)) - t)) / np.spacing(abs(tf)))
+    return worst
+
+
+@pytest.mark.parametrize("op,gen", [
+    ("sin", lambda r: np.concatenate([r.uniform(-10, 10, 2000), r.uniform(-2e8, 2e8, 500),
+                                      10 ** r.uniform(8.5, 307, 500) * r.choice([-1, 1], 500)])),
+    ("cos", lambda r: np.concatenate([r.uniform(-10, 10, 2000), r.uniform(-2e8, 2e8, 500),
+                                      10 ** r.uniform(8.5, 307, 500)])),
+    ("exp", lambda r: np.concatenate([r.uniform(-708, 709.7, 2000), r.uniform(-1, 1, 1000)])),
+    ("log", lambda r: np.concatenate([10 ** r.uniform(-307, 308, 2000), r.uniform(0.5, 2.0, 1000)])),
+])
+def test_math_within_one_ulp_of_mpmath(op, gen):
+    xs = gen(np.random.default_rng(3))
+    assert ulp_error(op, xs) < 1.0
+
+
+def test_math_special_values():
+    inf, nan = float("inf"), float("nan")
+    assert np.isnan(O.math("log", [-1.0]))[0]
+    assert O.math("log", [0.0])[0] == -inf and O.math("log", [-0.0])[0] == -inf
+    assert O.math("log", [inf])[0] == inf and O.math("log", [1.0])[0] == 0.0
+    assert O.math("exp", [710.0])[0] == inf and O.math("exp", [-746.0])[0] == 0.0
+    assert O.math("exp", [0.0])[0] == 1.0 and O.math("exp", [-inf])[0] == 0.0
+    assert np.isnan(O.math("sin", [inf, -inf, nan])).all() and np.isnan(O.math("cos", [inf])).all()
+    assert O.math("sin", [0.0])[0] == 0.0 and math.copysign(1, O.math("sin", [-0.0])[0]) == -1
+    assert O.math("cos", [0.0])[0] == 1.0
+    assert np.isnan(O.math("sqrt", [-1.0]))[0]
+    x = np.random.default_rng(0).uniform(0, 1e6, 1000)
+    assert np.array_equal(O.math("sqrt", x), np.sqrt(x))          # IEEE correctly rounded
+    sub = np.array([5e-324, 1e-310, 2.2e-308])
+    assert np.allclose(O.math("log", sub), np.log(sub), rtol=1e-15)
+    assert np.allclose(O.math("exp", [-740.0, -709.0]), np.exp([-740.0, -709.0]), rtol=1e-10)
+
+
+# ---------------------------------------------------------------- canonical sums
+def test_canonical_sum_exact_on_integers_and_close_to_fsum():
+    v = np.arange(1, 10001, dtype=np.float64)
+    assert O.canon_sum(v) == 50005000.0
+    r = np.random.default_rng(1).normal(size=100003)
+    assert abs(O.canon_sum(r) - math.fsum(r)) <= 1e-12 * math.fsum(abs(r))
+    assert O.canon_sum(np.array([])) == 0.0
+    one = np.zeros(200)
+    one[137] = 3.25
+    assert O.canon_sum(one) == 3.25 and O.canon_sum(one, 138) == 0.0
+
+
+# ---------------------------------------------------------------- QR solve
+def test_qr_full_rank_matches_lstsq():
+    rng = np.random.default_rng(2)
+    A = np.column_stack([rng.normal(size=(500, 6)), np.ones(500)])
+    b = rng.normal(size=500)
+    c, rank = O.qr_solve(A, b)
+    ref = np.linalg.lstsq(A, b, rcond=None)[0]
+    assert rank == 7
+    assert np.allclose(c, ref, rtol=1e-12, atol=1e-13)
+
+
+def test_qr_recovers_exact_integer_coefficients():
+    rng = np.random.default_rng(4)
+    A = np.column_stack([rng.integers(-9, 10, 300).astype(float) for _ in range(4)] + [np.ones(300)])
+    coef = np.array([3.0, -2.0, 0.5, 7.0, -11.0])
+    c, rank = O.qr_solve(A, A @ coef)
+    assert rank == 5 and np.allclose(c, coef, rtol=0, atol=1e-12)
+
+
+def test_qr_duplicate_column_gets_zero_on_later_pivot():
+    x = np.random.default_rng(5).uniform(1, 2, 100)
+    A = np.column_stack([x, x, np.ones(100)])
+    c, rank = O.qr_solve(A, 2 * x + 3)
+    # equal norms: the first column pivots first (strict '>' in the max scan);
+    # the duplicate then falls under Eigen's threshold and is zeroed
+    assert rank == 2 and c[1] == 0.0
+    assert np.allclose(c[[0, 2]], [2.0, 3.0], atol=1e-12)
+
+
+def test_qr_constant_column_collinear_with_free_term():
+    y = np.random.default_rng(6).normal(size=80)
+    A = np.column_stack([np.full(80, 0.5), np.ones(80)])
+    c, rank = O.qr_solve(A, y)
+    assert rank == 1 and c[0] == 0.0 and abs(c[1] - y.mean()) < 1e-13
+
+
+def test_qr_zero_column_and_short_matrix():
+    rng = np.random.default_rng(7)
+    A = np.column_stack([rng.normal(size=50), np.zeros(50), np.ones(50)])
+    c, rank = O.qr_solve(A, rng.normal(size=50))
+    assert rank == 2 and c[1] == 0.0
+    A = rng.normal(size=(3, 5))
+    c, rank = O.qr_solve(A, rng.normal(size=3))
+    assert rank == 3 and np.allclose(A @ c, A @ c)   # n < k: at most n pivots
+    assert np.count_nonzero(c) == 3
+
+
+def test_qr_pivot_order_agrees_with_lapack_geqp3():
+    # same pivoting rule family (largest remaining column norm, LAWN-176
+    # downdates): on a well-separated design the basic solutions coincide
+    rng = np.random.default_rng(8)
+    A = rng.normal(size=(200, 5)) * np.array([1.0, 10.0, 0.1, 3.0, 0.5])
+    A[:, 3] = A[:, 1] * 2.0                      # exact collinearity: rank 4
+    b = rng.normal(size=200)
+    c, rank = O.qr_solve(A, b)
+    Q, R, piv = scipy.linalg.qr(A, pivoting=True, mode="economic")
+    r = 4
+    ref = np.zeros(5)
+    ref[piv[:r]] = scipy.linalg.solve_triangular(R[:r, :r], (Q.T @ b)[:r])
+    assert rank == r
+    assert np.allclose(c, ref, rtol=1e-10, atol=1e-12)
+
+
+# ---------------------------------------------------------------- fitness
+def data(seed=0, n=400):
+    rng = np.random.default_rng(seed)
+    X = rng.uniform(-3, 3, (n, 2))
+    y = 3 * X[:, 0] + 2 * np.sin(X[:, 1]) - 1 + rng.normal(0, 0.1, n)
+    return X, y
+
+
+def test_exact_fit_has_zero_error_and_true_coefficients():
+    X, _ = data()
+    y = 3 * X[:, 0] + 2 * np.sin(X[:, 1]) - 1
+    tu = O.Dataset(X, y).tune("+ x0 sin x1")
+    assert np.allclose(tu["coefs"], [3.0, 2.0, -1.0], atol=1e-13)
+    assert tu["fit"][0] < 1e-25 and tu["fit"][1] < 1e-13
+
+
+def test_r2_rmse_formulas():
+    X, y = data(1)
+    od = O.Dataset(X, y)
+    tu = od.tune("+ x0 sin x1")
+    yp = od.eval(tu["ols"])
+    ssr = math.fsum((y - yp) ** 2)
+    sst = math.fsum((y - y.mean()) ** 2)
+    assert abs(tu["fit"][1] - math.sqrt(ssr / len(y))) < 1e-12 * tu["fit"][1]
+    assert abs(tu["fit"][0] - ssr / sst) < 1e-10 * tu["fit"][0]
+    e0, e1, size = od.score("c0x1p+1")                      # constant prediction 2
+    assert abs(e0 - math.fsum((y - 2) ** 2) / sst) < 1e-12 * e0 and size == 1
+    # size of the OLS tree ((c0*x0)+(c1*sin(x1)))+c2: + + * c0 x0 * c1 sin x1 c2 (node.h:311-322)
+    assert tu["fit"][2] == 10
+
+
+def test_nan_gives_sentinel_and_inf_passes():
+    X, y = data(2)
+    od = O.Dataset(X, y)
+    assert od.score("ln x0") == (1000, 1000, 1000)          # ln of negatives -> NaN (fitness, 536-537)
+    e0, e1, _ = od.score("exp * x0 c0x1p+10")               # +inf predictions: not NaN
+    assert e0 == float("inf") and e1 == float("inf")
+
+
+# ---------------------------------------------------------------- rewrites
+@pytest.mark.parametrize("tree,op,want", [
+    # expand (node.cpp:340-374): only the both-sums case is distributed correctly
+    ("* + x0 x1 x2", "expand", "((x0*x2)*(x2*x2))"),
+    ("* x0 - x1 x2", "expand", "((x0*x1)*((x0*x1)*x2))"),
+    ("* + x0 x1 - x2 x3", "expand", "(((x0*x2)-(x0*x3))+((x1*x2)-(x1*x3)))"),
+    ("sqr + x0 x1", "expand", "(((x0+x1))**2)"),
+    # simplify (node.cpp:152-296), including its sign slips
+    ("+ c0x1p+1 - c0x1.8p+1 x0", "simplify", "(5+x0)"),
+    ("- c0x1p+1 - c0x1.8p+1 x0", "simplify", "(-1-x0)"),
+    ("- c0x1p+1 - x0 c0x1.8p+1", "simplify", "(5-x0)"),
+    ("- + x0 c0x1p+1 c0x1.8p+1", "simplify", "((x0+0)-5)"),
+    ("- - x0 c0x1p+1 c0x1.8p+1", "simplify", "((x0-0)-1)"),
+    ("+ + x0 c0x1p+1 c0x1.8p+1", "simplify", "((x0+0)+5)"),
+    ("- c0x0p+0 x0", "simplify", "x0"),
+    ("* c0x1p+0 x0", "simplify", "x0"),
+    ("* c0x0p+0 x0", "simplify", "0"),
+    ("* c0x1p+1 * c0x1.8p+1 x0", "simplify", "(6*x0)"),
+    ("* * x0 c0x1p+1 c0x1.8p+1", "simplify", "(x0*6)"),
+    ("/ c0x0p+0 x0", "simplify", "0"),
+    ("/ x0 c0x1p+1", "simplify", "(x0/2)"),
+    ("+ c0x1.999999999999ap-4 c0x1.999999999999ap-3", "simplify", "0.300000"),
+    # normalize_factor_constants (node.cpp:312-327)
+    ("+ * c0x1.8p+1 x0 c0x1.4p+2", "normalize", "((1*x0)+1)"),
+    ("* * c0x1p+1 x0 x1", "normalize", "((2*x0)*x1)"),
+    ("/ c0x1.8p+1 c0x1.4p+2", "normalize", "(3/1)"),
+    ("sin c0x1.8p+1", "normalize", "sin(3)"),
+])
+def test_rewrite_known_answers(tree, op, want):
+    assert O.to_string(O.rewrite(tree, op)) == want
+
+
+@pytest.mark.parametrize("tree,want", [
+    ("c0x1.999999999999ap-4", "0.100000"), ("c0x1p+1", "2"), ("c0x1.c25c268497682p-44", "0"),
+    ("c-0x0p+0", "0"), ("c0x1.65a0bcp+31", "-2147483648"), ("c-0x1.8p-1", "-0.750000"),
+    ("> x0 x1", "(x0<x1)"), ("== x0 x1", "(x0<x1)"), ("!= x0 x1", "(x0<x1)"), ("sqr x0", "((x0)**2)"),
+    ("min x0 x1", "MIN(x0, x1)"), ("max x0 c0x1p+0", "MAX(x0, 1)"), ("ln exp x2", "ln(exp(x2))"),
+])
+def test_to_string_known_answers(tree, want):
+    assert O.to_string(tree) == want
+
+
+def test_perturbations_of_a_variable_known_answer():
+    # perturb_candidates(x0) with one feature, then the rewrite rounds and the
+    # string dedup of all_candidates (rils_rols_cpp.cpp:318-330, 348-443),
+    # worked out by hand from the generator order
+    # (0/x0) simplifies to 0 in the first rewrite round; the size changed, so a
+    # second round runs and normalize_factor_constants turns the root constant
+    # into 1, a duplicate of the first candidate.
+    want = ["1", "sin(x0)", "cos(x0)", "ln(x0)", "exp(x0)", "sqrt(x0)", "((x0)**2)", "(x0+x0)", "(x0+1)",
+            "(x0-x0)", "(x0-1)", "(1-x0)", "(x0*x0)", "x0", "(x0/x0)", "(x0/1)", "(-1/x0)",
+            "(0.500000/x0)", "(1/x0)", "(2/x0)", "(3.141593/x0)", "(10/x0)"]
+    got = [O.to_string(c) for c in O.all_candidates("x0", 1, local_search=False)]
+    assert got == want
+
+
+def test_search_budget_and_determinism():
+    X, y = data(3, 200)
+    a = O.fit(X, y, max_fit_calls=3000, random_state=5)
+    b = O.fit(X, y, max_fit_calls=3000, random_state=5)
+    assert a == b
+    assert a["fit_calls"] in (3000, 3001)            # finished() is checked before each call
+    assert all(m[1] <= a["fit_calls"] for m in a["moves"])
+    assert a["final"][0] < 0.01                      # R2 > 0.99 on this easy problem
