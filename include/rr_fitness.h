@@ -107,6 +107,9 @@ typedef struct rr_batch {
 /* flags of rr_fit */
 #define RR_F_SENTINEL 1  /* R2 or RMSE was NaN: fitness is (1000, 1000, 1000) */
 #define RR_F_NEEDS64 2   /* FP32 result not trustworthy to its stated bound */
+/* FP64 OLS form: bits 16..31 of flags hold the number of Householder steps
+ * that ran passes over the samples (for traffic accounting) */
+#define RR_F_STEPS(flags) (((uint32_t)(flags)) >> 16)
 
 typedef struct rr_fit {
   double e0;      /* 1 - R2 (rils_rols_cpp.cpp:538) */

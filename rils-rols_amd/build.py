@@ -23,8 +23,13 @@ HOST_SOURCES = ["tree.cpp", "search.cpp", "pybind_module.cpp"]
 # -ffp-contract=off: no fused multiply-add unless written as fma(); the FP64
 # path must produce the same bits as host code compiled the same way.
 # -packed-fp32-ops off: packed FP32 VALU forms are not used on this target.
+# -disable-promote-alloca-to-vector: the interpreters keep operand-stack slots
+# 0-1 in registers and the rarely used deeper slots in private memory; left
+# to itself the compiler turns the deep slots into registers too and the
+# kernels lose half their occupancy.
 HIP_FLAGS = ["-O3", "--offload-arch=gfx950", "-std=c++17", "-fPIC", "-ffp-contract=off",
-             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall", "-Wno-unused-function"]
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-mllvm", "-disable-promote-alloca-to-vector", "-Wall", "-Wno-unused-function"]
 HOST_FLAGS = ["-O3", "-std=c++17", "-march=x86-64-v3", "-ffp-contract=off", "-fPIC", "-Wall",
               "-Wno-unused-function"]
 

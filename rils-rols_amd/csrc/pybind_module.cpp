@@ -84,6 +84,130 @@ std::vector<int> rand_hook(unsigned seed, int count) {
   return out;
 }
 
+// A resident dataset plus a prepared candidate batch: the unit the benchmark
+// and the serving loop drive.  prepare() runs the host half of
+// tune_constants (expand, simplify, factor split; rils_rols_cpp.cpp:448-475)
+// and compiles the factors; run() is one rr_fitness_batch call.
+class BatchRunner {
+ public:
+  BatchRunner(py::array_t<double, py::array::c_style | py::array::forcecast> X_fm,
+              py::array_t<double, py::array::c_style | py::array::forcecast> y, int device) {
+    py::buffer_info bx = X_fm.request(), by = y.request();
+    if (bx.ndim != 2 || by.ndim != 1 || bx.shape[1] != by.shape[0]) throw std::invalid_argument("X_fm must be (d, n), y (n,)");
+    n_ = by.shape[0];
+    d_ = (int)bx.shape[0];
+    check(rr_create(device, &ctx_), "rr_create");
+    check(rr_dataset_upload(ctx_, (const double*)bx.ptr, (const double*)by.ptr, n_, d_, &h_), "rr_dataset_upload");
+  }
+  ~BatchRunner() { if (ctx_) rr_destroy(ctx_); }
+
+  // OLS mode: returns the number of factors per candidate
+  std::vector<int> prepare_ols(const std::vector<std::string>& trees) {
+    reset();
+    mode_ = RR_MODE_OLS;
+    rrh::Pool pool;
+    std::vector<Flat> fs;
+    std::vector<int> nf;
+    for (const std::string& t : trees) {
+      rrh::ols_factors(rrh::parse_prefix(t), fs, pool);
+      for (const Flat& f : fs) { rrh::compile(f, code_, consts_); prog_.push_back((int32_t)code_.size()); }
+      cand_.push_back((int32_t)prog_.size() - 1);
+      nf.push_back((int)fs.size());
+    }
+    return nf;
+  }
+  // OLS mode from ready factor lists (no rewrites)
+  void prepare_factors(const std::vector<std::vector<std::string>>& cands) {
+    reset();
+    mode_ = RR_MODE_OLS;
+    for (const auto& c : cands) {
+      for (const std::string& f : c) { rrh::compile(rrh::parse_prefix(f), code_, consts_); prog_.push_back((int32_t)code_.size()); }
+      cand_.push_back((int32_t)prog_.size() - 1);
+    }
+  }
+  void prepare_raw(const std::vector<std::string>& trees) {
+    reset();
+    mode_ = RR_MODE_RAW;
+    for (const std::string& t : trees) {
+      rrh::compile(rrh::parse_prefix(t), code_, consts_);
+      prog_.push_back((int32_t)code_.size());
+      cand_.push_back((int32_t)prog_.size() - 1);
+    }
+  }
+  py::tuple run(const std::string& precision) {
+    const int pr = precision == "fp32" ? RR_FP32_REFINED : RR_FP64;
+    const int nc = (int)cand_.size() - 1;
+    const int64_t ncoef = (int64_t)cand_.back() + nc;
+    py::array_t<double> coefs(std::max<int64_t>(ncoef, 1));
+    py::array_t<double> fit({(py::ssize_t)std::max(nc, 1), (py::ssize_t)5});
+    std::vector<rr_fit> out(std::max(nc, 1));
+    rr_batch b{nc, cand_.data(), prog_.data(), code_.data(), consts_.data(), (int32_t)consts_.size()};
+    double* cp = (double*)coefs.request().ptr;
+    {
+      py::gil_scoped_release nogil;
+      check(rr_fitness_batch(ctx_, h_, &b, mode_, pr, cp, out.data()), "rr_fitness_batch");
+    }
+    double* fp = (double*)fit.request().ptr;
+    for (int c = 0; c < nc; ++c) {
+      fp[5 * c] = out[c].e0; fp[5 * c + 1] = out[c].e1; fp[5 * c + 2] = out[c].err;
+      fp[5 * c + 3] = out[c].rank; fp[5 * c + 4] = out[c].flags;
+    }
+    return py::make_tuple(coefs, fit);
+  }
+  // Sub-batch of already prepared candidates (same constant pool): used to
+  // re-score in FP64 the candidates the FP32 form flags.
+  void select(const std::vector<int64_t>& idx) {
+    scand_.assign(1, 0);
+    sprog_.assign(1, 0);
+    scode_.clear();
+    for (int64_t c : idx) {
+      if (c < 0 || c + 1 >= (int64_t)cand_.size()) throw std::out_of_range("candidate index");
+      for (int32_t p = cand_[c]; p < cand_[c + 1]; ++p) {
+        scode_.insert(scode_.end(), code_.begin() + prog_[p], code_.begin() + prog_[p + 1]);
+        sprog_.push_back((int32_t)scode_.size());
+      }
+      scand_.push_back((int32_t)sprog_.size() - 1);
+    }
+  }
+  py::tuple run_selected(const std::string& precision) {
+    const int pr = precision == "fp32" ? RR_FP32_REFINED : RR_FP64;
+    const int nc = (int)scand_.size() - 1;
+    const int64_t ncoef = (int64_t)scand_.back() + nc;
+    py::array_t<double> coefs(std::max<int64_t>(ncoef, 1));
+    py::array_t<double> fit({(py::ssize_t)std::max(nc, 1), (py::ssize_t)5});
+    std::vector<rr_fit> out(std::max(nc, 1));
+    rr_batch b{nc, scand_.data(), sprog_.data(), scode_.data(), consts_.data(), (int32_t)consts_.size()};
+    double* cp = (double*)coefs.request().ptr;
+    {
+      py::gil_scoped_release nogil;
+      check(rr_fitness_batch(ctx_, h_, &b, mode_, pr, cp, out.data()), "rr_fitness_batch");
+    }
+    double* fp = (double*)fit.request().ptr;
+    for (int c = 0; c < nc; ++c) {
+      fp[5 * c] = out[c].e0; fp[5 * c + 1] = out[c].e1; fp[5 * c + 2] = out[c].err;
+      fp[5 * c + 3] = out[c].rank; fp[5 * c + 4] = out[c].flags;
+    }
+    return py::make_tuple(coefs, fit);
+  }
+  double last_kernel_ms() { double ms = 0; rr_last_kernel_ms(ctx_, &ms); return ms; }
+  void set_option(const std::string& k, int64_t v) { check(rr_set_option(ctx_, k.c_str(), v), "rr_set_option"); }
+  int64_t get_option(const std::string& k) { int64_t v = 0; check(rr_get_option(ctx_, k.c_str(), &v), "rr_get_option"); return v; }
+  std::vector<int> cand_start() const { return std::vector<int>(cand_.begin(), cand_.end()); }
+  int64_t code_words() const { return (int64_t)code_.size(); }
+
+ private:
+  static void check(int rc, const char* what) { if (rc) throw std::runtime_error(std::string(what) + ": " + rr_last_error()); }
+  void reset() { cand_.assign(1, 0); prog_.assign(1, 0); code_.clear(); consts_.clear(); }
+  rr_ctx* ctx_ = nullptr;
+  int32_t h_ = -1;
+  int64_t n_ = 0;
+  int d_ = 0;
+  int mode_ = RR_MODE_OLS;
+  std::vector<int32_t> cand_{0}, prog_{0}, scand_{0}, sprog_{0};
+  std::vector<uint32_t> code_, scode_;
+  std::vector<double> consts_;
+};
+
 }  // namespace
 
 PYBIND11_MODULE(rils_rols_cpp, m) {
@@ -143,6 +267,22 @@ PYBIND11_MODULE(rils_rols_cpp, m) {
            })
       .def("set_device", &RilsRols::set_device)
       .def("set_slab_budget_mb", &RilsRols::set_slab_budget_mb);
+
+  py::class_<BatchRunner>(m, "BatchRunner")
+      .def(py::init<py::array_t<double, py::array::c_style | py::array::forcecast>,
+                    py::array_t<double, py::array::c_style | py::array::forcecast>, int>(),
+           py::arg("X_fm"), py::arg("y"), py::arg("device") = 0)
+      .def("prepare_ols", &BatchRunner::prepare_ols)
+      .def("prepare_factors", &BatchRunner::prepare_factors)
+      .def("prepare_raw", &BatchRunner::prepare_raw)
+      .def("run", &BatchRunner::run, py::arg("precision") = "fp64")
+      .def("select", &BatchRunner::select)
+      .def("run_selected", &BatchRunner::run_selected, py::arg("precision") = "fp64")
+      .def("last_kernel_ms", &BatchRunner::last_kernel_ms)
+      .def("set_option", &BatchRunner::set_option)
+      .def("get_option", &BatchRunner::get_option)
+      .def("cand_start", &BatchRunner::cand_start)
+      .def("code_words", &BatchRunner::code_words);
 
   // test hooks (host engine only; no GPU needed)
   m.def("_all_candidates", &candidates_hook, py::arg("prefix"), py::arg("nvars"), py::arg("classification") = false,

@@ -17,6 +17,7 @@
 #include "../../include/rr_fitness.h"
 #include "rr_kernels.h"
 #include "rr_kernels_fp32.h"
+#include "rr_device.h"
 
 namespace {
 
@@ -74,7 +75,7 @@ struct Dataset {
   double* y = nullptr;
   float* X32 = nullptr;   // FP32 copies for the precision form
   float* y32 = nullptr;
-  double ybar = 0, sst = 0;
+  double ybar = 0, sst = 0, ysum = 0;
 };
 
 }  // namespace
@@ -89,6 +90,7 @@ struct rr_ctx {
   int64_t slab_budget_mb = 32768;
   int64_t fp32_block_waves = 4;
   int64_t fp32_rows_per_tile = 256;
+  int64_t fp64_rows = 2;
   double last_ms = 0.0;
 };
 
@@ -231,7 +233,7 @@ int rr_dataset_upload(rr_ctx* ctx, const double* X_fm, const double* y, int64_t 
   if (n >= ((int64_t)1 << 40)) return fail(RR_E_ARG, "too many samples");
   Dataset D;
   D.n = n; D.d = d;
-  D.ldx = (n + 63) & ~int64_t(63);
+  D.ldx = (n + rr::RR_ROW_PAD - 1) / rr::RR_ROW_PAD * rr::RR_ROW_PAD;   // whole tiles, never read past a column
   const size_t xb = (size_t)D.ldx * (size_t)std::max(d, 1) * sizeof(double);
   if (hipMalloc(&D.X, xb) != hipSuccess) return fail(RR_E_NOMEM, "hipMalloc for X failed");
   if (hipMalloc(&D.y, (size_t)D.ldx * sizeof(double)) != hipSuccess) { (void)hipFree(D.X); return fail(RR_E_NOMEM, "hipMalloc for y failed"); }
@@ -248,13 +250,14 @@ int rr_dataset_upload(rr_ctx* ctx, const double* X_fm, const double* y, int64_t 
   RR_HIP(hipMemcpyAsync(D.y, y, (size_t)n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
   RR_HIP(rr::launch_to_fp32(D.X, D.X32, (int64_t)D.ldx * std::max(d, 1), ctx->stream));
   RR_HIP(rr::launch_to_fp32(D.y, D.y32, D.ldx, ctx->stream));
-  if (ctx->d_misc.ensure(2 * sizeof(double)) != RR_OK) return RR_E_NOMEM;
+  if (ctx->d_misc.ensure(3 * sizeof(double)) != RR_OK) return RR_E_NOMEM;
   RR_HIP(rr::launch_stats_fp64(D.y, n, (double*)ctx->d_misc.p, ctx->stream));
-  double st[2];
+  double st[3];
   RR_HIP(hipMemcpyAsync(st, ctx->d_misc.p, sizeof st, hipMemcpyDeviceToHost, ctx->stream));
   RR_HIP(hipStreamSynchronize(ctx->stream));
   D.ybar = st[0];
   D.sst = st[1];
+  D.ysum = st[2];
   D.live = true;
   ctx->ds.push_back(D);
   *handle = (int32_t)ctx->ds.size() - 1;
@@ -328,14 +331,18 @@ int rr_fitness_batch(rr_ctx* ctx, int32_t handle, const rr_batch* b, int32_t mod
   if (mode == RR_MODE_RAW) {
     RR_HIP(hipEventRecord(ctx->ev0, ctx->stream));
     if (precision == RR_FP64) {
-      rr::RawLaunch L{dc, dp, dcode, dK, D->X, D->y, D->n, D->ldx, D->sst, dfit, nc};
+      rr::RawLaunch L{dc, dp, dcode, dK, D->X, D->y, D->n, D->ldx, D->sst, dfit, nc, (int)ctx->fp64_rows};
       RR_HIP(rr::launch_raw_fp64(L, ctx->stream));
     } else {
+      std::vector<int32_t> order(nc);
+      for (int32_t c = 0; c < nc; ++c) order[c] = c;
+      if ((rc = upload(ctx, ctx->d_slab_off, order.data(), (size_t)nc * 4, so))) return rc;
       rr::Fp32Launch L{};
       L.cand_start = dc; L.prog_start = dp; L.code = dcode; L.K = dK; L.X32 = D->X32; L.y32 = D->y32; L.X = D->X;
-      L.y = D->y; L.n = D->n; L.ldx = D->ldx; L.ybar = D->ybar; L.sst = D->sst; L.coefs = dcoef; L.fit = dfit;
-      L.n_cands = nc; L.raw = 1; L.block_waves = (int)ctx->fp32_block_waves;
-      RR_HIP(rr::launch_fp32(L, ctx->stream));
+      L.y = D->y; L.n = D->n; L.ldx = D->ldx; L.ybar = D->ybar; L.sst = D->sst; L.ysum = D->ysum; L.coefs = dcoef;
+      L.fit = dfit; L.n_cands = nc; L.raw = 1; L.block_waves = (int)ctx->fp32_block_waves;
+      const int counts[4] = {nc, 0, 0, 0};
+      RR_HIP(rr::launch_fp32(L, (const int32_t*)ctx->d_slab_off.p, counts, ctx->stream));
     }
     RR_HIP(hipEventRecord(ctx->ev1, ctx->stream));
   } else if (precision == RR_FP64) {
@@ -350,7 +357,7 @@ int rr_fitness_batch(rr_ctx* ctx, int32_t handle, const rr_batch* b, int32_t mod
       int c1 = c, maxk = 0;
       while (c1 < nc) {
         const int k = b->cand_start[c1 + 1] - b->cand_start[c1] + 1;
-        const int64_t need = (int64_t)(k + 1) * D->ldx;
+        const int64_t need = (int64_t)(2 * k) * D->ldx;   // A, Q^T y, kept factor values
         if (c1 > c && (used + need) * 8 > budget) break;
         off[c1] = used;
         used += need;
@@ -369,17 +376,30 @@ int rr_fitness_batch(rr_ctx* ctx, int32_t handle, const rr_batch* b, int32_t mod
       L.cand_start = dc; L.prog_start = dp; L.code = dcode; L.K = dK; L.X = D->X; L.y = D->y;
       L.n = D->n; L.ldx = D->ldx; L.sst = D->sst; L.slab = (double*)ctx->d_slab.p;
       L.slab_off = (const int64_t*)ctx->d_slab_off.p + ch.c0; L.coefs = dcoef; L.fit = dfit;
-      L.cand_base = ch.c0; L.n_cands = ch.c1 - ch.c0; L.max_cols = ch.maxk;
+      L.cand_base = ch.c0; L.n_cands = ch.c1 - ch.c0; L.max_cols = ch.maxk; L.rows = (int)ctx->fp64_rows;
       RR_HIP(rr::launch_ols_fp64(L, ctx->stream));
     }
     RR_HIP(hipEventRecord(ctx->ev1, ctx->stream));
   } else {
+    // group candidates by column count so each kernel instance keeps only
+    // the accumulators its bucket needs
+    std::vector<int32_t> order;
+    order.reserve(nc);
+    int counts[4] = {0, 0, 0, 0};
+    const int lim[3] = {2, 5, 9};
+    for (int bkt = 0; bkt < 4; ++bkt)
+      for (int32_t c = 0; c < nc; ++c) {
+        const int k = b->cand_start[c + 1] - b->cand_start[c] + 1;
+        const int kb = k <= lim[0] ? 0 : k <= lim[1] ? 1 : k <= lim[2] ? 2 : 3;
+        if (kb == bkt) { order.push_back(c); ++counts[bkt]; }
+      }
+    if ((rc = upload(ctx, ctx->d_slab_off, order.data(), (size_t)nc * 4, so))) return rc;
     rr::Fp32Launch L{};
     L.cand_start = dc; L.prog_start = dp; L.code = dcode; L.K = dK; L.X32 = D->X32; L.y32 = D->y32; L.X = D->X;
-    L.y = D->y; L.n = D->n; L.ldx = D->ldx; L.ybar = D->ybar; L.sst = D->sst; L.coefs = dcoef; L.fit = dfit;
-    L.n_cands = nc; L.raw = 0; L.block_waves = (int)ctx->fp32_block_waves;
+    L.y = D->y; L.n = D->n; L.ldx = D->ldx; L.ybar = D->ybar; L.sst = D->sst; L.ysum = D->ysum; L.coefs = dcoef;
+    L.fit = dfit; L.n_cands = nc; L.raw = 0; L.block_waves = (int)ctx->fp32_block_waves;
     RR_HIP(hipEventRecord(ctx->ev0, ctx->stream));
-    RR_HIP(rr::launch_fp32(L, ctx->stream));
+    RR_HIP(rr::launch_fp32(L, (const int32_t*)ctx->d_slab_off.p, counts, ctx->stream));
     RR_HIP(hipEventRecord(ctx->ev1, ctx->stream));
   }
 
@@ -426,6 +446,9 @@ int rr_set_option(rr_ctx* ctx, const char* name, int64_t value) {
   else if (s == "fp32_block_waves") {
     if (value != 1 && value != 2 && value != 4 && value != 8) return fail(RR_E_ARG, "fp32_block_waves must be 1, 2, 4 or 8");
     ctx->fp32_block_waves = value;
+  } else if (s == "fp64_rows") {
+    if (value != 1 && value != 2 && value != 4) return fail(RR_E_ARG, "fp64_rows must be 1, 2 or 4");
+    ctx->fp64_rows = value;
   } else if (s == "fp32_rows_per_tile") {
     if (value < 64 || value > 4096 || (value & 63)) return fail(RR_E_ARG, "fp32_rows_per_tile must be a multiple of 64 in [64, 4096]");
     ctx->fp32_rows_per_tile = value;
@@ -439,6 +462,7 @@ int rr_get_option(rr_ctx* ctx, const char* name, int64_t* value) {
   if (s == "slab_budget_mb") *value = ctx->slab_budget_mb;
   else if (s == "fp32_block_waves") *value = ctx->fp32_block_waves;
   else if (s == "fp32_rows_per_tile") *value = ctx->fp32_rows_per_tile;
+  else if (s == "fp64_rows") *value = ctx->fp64_rows;
   else return fail(RR_E_ARG, "unknown option " + s);
   return RR_OK;
 }

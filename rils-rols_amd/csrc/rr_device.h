@@ -51,34 +51,24 @@ __device__ __noinline__ double call_cos(double x) { return rrm::cos(x); }
 __device__ __noinline__ double call_log(double x) { return rrm::log(x); }
 __device__ __noinline__ double call_exp(double x) { return rrm::exp(x); }
 
-// Operand stack held in named registers; the stack pointer is wave-uniform,
-// so the switch is a scalar branch and no scratch memory is used.
-static_assert(RR_MAX_STACK == 8, "Stack64 is written out for 8 slots");
-struct Stack64 {
-  double s[RR_MAX_STACK];
-  __device__ inline void set(int i, double v) {
-    switch (i) {
-#define RR_SET(k) case k: s[k] = v; break;
-      RR_SET(0) RR_SET(1) RR_SET(2) RR_SET(3) RR_SET(4) RR_SET(5) RR_SET(6) default: s[7] = v; break;
-#undef RR_SET
-    }
-  }
-  __device__ inline double get(int i) const {
-    switch (i) {
-#define RR_GET(k) case k: return s[k];
-      RR_GET(0) RR_GET(1) RR_GET(2) RR_GET(3) RR_GET(4) RR_GET(5) RR_GET(6) default: return s[7];
-#undef RR_GET
-    }
-  }
-};
+// Rows are padded so that a tile of 64*RR_TILE_ROWS rows never reads past
+// the end of a column: the row stride of every dataset and work column is a
+// multiple of RR_ROW_PAD.
+constexpr int RR_TILE_ROWS = 8;
+constexpr int64_t RR_ROW_PAD = 64 * RR_TILE_ROWS;
 
-// Evaluate program code[0..len) at sample `row`.  X is feature-major with
-// column stride ldx.  Per-sample semantics: node::evaluate_inner
-// (rils_rols_cpp/node.cpp:23-95) with rr_math functions.
-__device__ inline double eval_f64(const uint32_t* __restrict__ code, int len, const double* __restrict__ K,
-                                  const double* __restrict__ X, int64_t ldx, int64_t row) {
-  double acc = 0.0;
-  Stack64 st;
+// Evaluate program code[0..len) at R samples: row0, row0+64, ...,
+// row0+64(R-1).  X is feature-major with column stride ldx.  Per-sample
+// semantics: node::evaluate_inner (rils_rols_cpp/node.cpp:23-95) with
+// rr_math functions.  The opcode is decoded once per R samples.
+template <int R>
+__device__ inline void eval_f64(const uint32_t* __restrict__ code, int len, const double* __restrict__ K,
+                                const double* __restrict__ X, int64_t ldx, int64_t row0, double (&acc)[R]) {
+  // stack slots 0-1 in registers; deeper slots (rare with Sethi-Ullman
+  // ordering) in a private array
+  double st0[R], st1[R], stx[RR_MAX_STACK - 2][R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) acc[r] = 0.0;
   int sp = 0;
   for (int pc = 0; pc < len; ++pc) {
     const uint32_t ins = code[pc];
@@ -87,31 +77,106 @@ __device__ inline double eval_f64(const uint32_t* __restrict__ code, int len, co
     if (op >= RR_OP_BIN) {
       const uint32_t b = op - RR_OP_BIN;
       const uint32_t bop = b >> 3, mode = b & 7u;
-      double l, r;
+      double l[R], rr[R];
       switch (mode) {
-        case RR_M_S: --sp; l = st.get(sp); r = acc; break;
-        case RR_M_SR: --sp; l = acc; r = st.get(sp); break;
-        case RR_M_V: l = acc; r = X[(int64_t)arg * ldx + row]; break;
-        case RR_M_VR: l = X[(int64_t)arg * ldx + row]; r = acc; break;
-        case RR_M_C: l = acc; r = K[arg]; break;
-        default: l = K[arg]; r = acc; break;
+        case RR_M_S:
+        case RR_M_SR: {
+          --sp;
+          double s[R];
+          if (sp == 0) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) s[r] = st0[r];
+          } else if (sp == 1) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) s[r] = st1[r];
+          } else {
+#pragma unroll
+            for (int r = 0; r < R; ++r) s[r] = stx[sp - 2][r];
+          }
+#pragma unroll
+          for (int r = 0; r < R; ++r) { l[r] = mode == RR_M_S ? s[r] : acc[r]; rr[r] = mode == RR_M_S ? acc[r] : s[r]; }
+          break;
+        }
+        case RR_M_V:
+#pragma unroll
+          for (int r = 0; r < R; ++r) { l[r] = acc[r]; rr[r] = X[(int64_t)arg * ldx + row0 + 64 * r]; }
+          break;
+        case RR_M_VR:
+#pragma unroll
+          for (int r = 0; r < R; ++r) { l[r] = X[(int64_t)arg * ldx + row0 + 64 * r]; rr[r] = acc[r]; }
+          break;
+        case RR_M_C: {
+          const double k = K[arg];
+#pragma unroll
+          for (int r = 0; r < R; ++r) { l[r] = acc[r]; rr[r] = k; }
+          break;
+        }
+        default: {
+          const double k = K[arg];
+#pragma unroll
+          for (int r = 0; r < R; ++r) { l[r] = k; rr[r] = acc[r]; }
+          break;
+        }
       }
-      acc = bin_f64(bop, l, r);
+      switch (bop) {   // one uniform branch per instruction, not per row
+#define RR_BIN(B) case B: _Pragma("unroll") for (int r = 0; r < R; ++r) acc[r] = bin_f64(B, l[r], rr[r]); break;
+        RR_BIN(RR_BOP_ADD) RR_BIN(RR_BOP_SUB) RR_BIN(RR_BOP_MUL) RR_BIN(RR_BOP_DIV) RR_BIN(RR_BOP_LT)
+        RR_BIN(RR_BOP_GT) RR_BIN(RR_BOP_EQ) RR_BIN(RR_BOP_NEQ) RR_BIN(RR_BOP_MIN)
+        default: _Pragma("unroll") for (int r = 0; r < R; ++r) acc[r] = bin_f64(RR_BOP_MAX, l[r], rr[r]); break;
+#undef RR_BIN
+      }
     } else {
       switch (op) {
-        case RR_OP_LDV: acc = X[(int64_t)arg * ldx + row]; break;
-        case RR_OP_LDC: acc = K[arg]; break;
-        case RR_OP_PUSH: st.set(sp, acc); ++sp; break;
-        case RR_OP_SIN: acc = call_sin(acc); break;
-        case RR_OP_COS: acc = call_cos(acc); break;
-        case RR_OP_LN: acc = call_log(acc); break;
-        case RR_OP_EXP: acc = call_exp(acc); break;
-        case RR_OP_SQRT: acc = rrm::sqrt(acc); break;
-        default: acc = acc * acc; break;  // RR_OP_SQR
+        case RR_OP_LDV:
+#pragma unroll
+          for (int r = 0; r < R; ++r) acc[r] = X[(int64_t)arg * ldx + row0 + 64 * r];
+          break;
+        case RR_OP_LDC: {
+          const double k = K[arg];
+#pragma unroll
+          for (int r = 0; r < R; ++r) acc[r] = k;
+          break;
+        }
+        case RR_OP_PUSH:
+          if (sp == 0) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) st0[r] = acc[r];
+          } else if (sp == 1) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) st1[r] = acc[r];
+          } else {
+#pragma unroll
+            for (int r = 0; r < R; ++r) stx[sp - 2][r] = acc[r];
+          }
+          ++sp;
+          break;
+        case RR_OP_SIN:
+#pragma unroll
+          for (int r = 0; r < R; ++r) acc[r] = call_sin(acc[r]);
+          break;
+        case RR_OP_COS:
+#pragma unroll
+          for (int r = 0; r < R; ++r) acc[r] = call_cos(acc[r]);
+          break;
+        case RR_OP_LN:
+#pragma unroll
+          for (int r = 0; r < R; ++r) acc[r] = call_log(acc[r]);
+          break;
+        case RR_OP_EXP:
+#pragma unroll
+          for (int r = 0; r < R; ++r) acc[r] = call_exp(acc[r]);
+          break;
+        case RR_OP_SQRT:
+#pragma unroll
+          for (int r = 0; r < R; ++r) acc[r] = rrm::sqrt(acc[r]);
+          break;
+        default:  // RR_OP_SQR
+#pragma unroll
+          for (int r = 0; r < R; ++r) acc[r] = acc[r] * acc[r];
+          break;
       }
     }
   }
-  return acc;
 }
 
 }  // namespace rr

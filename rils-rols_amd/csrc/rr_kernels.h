@@ -24,6 +24,7 @@ struct OlsLaunch {
   int cand_base;              // first candidate of this launch
   int n_cands;                // candidates in this launch
   int max_cols;               // largest k among them
+  int rows;                   // rows per lane per tile (1, 2 or 4)
 };
 
 struct RawLaunch {
@@ -37,12 +38,14 @@ struct RawLaunch {
   double sst;
   rr_fit* fit;
   int n_cands;
+  int rows;
 };
 
 hipError_t launch_ols_fp64(const OlsLaunch& L, hipStream_t s);
 hipError_t launch_raw_fp64(const RawLaunch& L, hipStream_t s);
 hipError_t launch_predict_fp64(const uint32_t* code, int len, const double* K, const double* X, int64_t n,
                                int64_t ldx, double* out, hipStream_t s);
-hipError_t launch_stats_fp64(const double* y, int64_t n, double* out2, hipStream_t s);
+// out3 = {mean(y), sum((y - mean)^2), sum(y)}
+hipError_t launch_stats_fp64(const double* y, int64_t n, double* out3, hipStream_t s);
 
 }  // namespace rr

@@ -17,7 +17,7 @@ struct Fp32Launch {
   const double* X;    // FP64 originals (bound computation)
   const double* y;
   int64_t n, ldx;
-  double ybar, sst;
+  double ybar, sst, ysum;
   double* coefs;
   rr_fit* fit;
   int n_cands;
@@ -25,7 +25,10 @@ struct Fp32Launch {
   int block_waves;    // waves (candidates) per workgroup
 };
 
-hipError_t launch_fp32(const Fp32Launch& L, hipStream_t s);
+// `order` lists the candidates grouped by column count: bucket_count[0]
+// with k <= 2, [1] with k <= 5, [2] with k <= 9, [3] larger (flagged
+// RR_F_NEEDS64: no FP32 form).  RAW mode uses order as is.
+hipError_t launch_fp32(const Fp32Launch& L, const int32_t* order, const int* bucket_count, hipStream_t s);
 hipError_t launch_to_fp32(const double* src, float* dst, int64_t count, hipStream_t s);
 
 }  // namespace rr

@@ -1,11 +1,12 @@
 // rr_kernels_fp64.hip -- the plain FP64 form of the candidate-fitness path.
 //
-// One wavefront owns one candidate and walks all n samples; lane l handles
-// rows l, l+64, l+128, ...  Every sum over samples follows the canonical
-// order of rr_device.h, and the column-pivoted Householder QR follows
-// Eigen's ColPivHouseholderQR (eigen/Eigen/src/QR/ColPivHouseholderQR.h:
-// 482-607, Householder.h:67-135) operation for operation, so the results are
-// the same bits as the CPU oracle's.
+// One wavefront owns one candidate and walks all n samples in tiles of 64*R
+// rows: lane l handles rows l, l+64, l+128, ..., R of them per tile, in
+// increasing order.  Every sum over samples therefore follows the canonical
+// order of rr_device.h whatever R is, and the column-pivoted Householder QR
+// follows Eigen's ColPivHouseholderQR (eigen/Eigen/src/QR/
+// ColPivHouseholderQR.h:482-607, Householder.h:67-135) operation for
+// operation, so the results are the same bits as the CPU oracle's.
 //
 // Per candidate the kernel
 //   1. evaluates each factor program and writes its column to a work slab in
@@ -15,15 +16,19 @@
 //      Householder dot products and one pass applying the reflector, which
 //      also accumulates the tail norms the next step and the LAWN-176 norm
 //      recomputation need; the reflectors are applied to a copy of y as they
-//      are made (Q^T b of _solve_impl); the factorisation stops at the first
-//      step that falls under Eigen's rank threshold, after which nothing
-//      changes the solution;
+//      are made (Q^T b of _solve_impl).  The factorisation stops at the first
+//      step under Eigen's rank threshold, after which nothing changes the
+//      solution, and at the first reflector whose beta is NaN, after which
+//      the outcome is fixed (see nan_exit below);
 //   3. back-substitutes on the leading r x r block and scatters the
 //      coefficients through the column permutation (zeros for the rest);
-//   4. re-evaluates the factors to form the prediction of the OLS tree that
-//      tune_constants builds (rils_rols_cpp.cpp:488-517): left-deep sum of
+//   4. reads the factor values kept from step 1 (a copy the QR does not
+//      touch) to form the prediction of the OLS tree that tune_constants
+//      builds (rils_rols_cpp.cpp:488-517): left-deep sum of
 //      c*f (or f when |c-1| < 1e-12), skipping |c| < 1e-12, free term last,
-//      and reduces SSR into 1-R2 and RMSE (fitness, 520-541).
+//      and reduces SSR into 1-R2 and RMSE (fitness, 520-541).  A kept NaN
+//      coefficient makes every prediction NaN, so that case is written as
+//      the sentinel without the pass.
 #include "rr_device.h"
 #include "rr_kernels.h"
 
@@ -34,7 +39,6 @@ namespace {
 constexpr double EPS12 = 1e-12;                 // node.h:13-14
 constexpr double DBL_EPS = 2.220446049250313e-16;
 constexpr double DBL_MIN_ = 2.2250738585072014e-308;
-constexpr int G = 8;                            // columns per pass
 
 struct Fp64Args {
   const int32_t* cand_start;
@@ -52,7 +56,7 @@ struct Fp64Args {
   int cand_base;
 };
 
-__device__ inline void write_fit(rr_fit* f, double ssr, double sst, int64_t n, int rank) {
+__device__ inline void write_fit(rr_fit* f, double ssr, double sst, int64_t n, int rank, int steps = 0) {
   const double r2 = 1 - ssr / sst;
   const double rmse = __builtin_sqrt(ssr / (double)n);
   rr_fit o;
@@ -63,11 +67,15 @@ __device__ inline void write_fit(rr_fit* f, double ssr, double sst, int64_t n, i
   }
   o.err = 0.0;
   o.rank = rank;
+  o.flags |= steps << 16;
   *f = o;
 }
 
-template <int MAXK>
+__device__ inline int64_t tile_start(int64_t lo, int R) { return (lo / (64 * R)) * (64 * R); }
+
+template <int MAXK, int R>
 __global__ __launch_bounds__(64) void k_ols_fp64(Fp64Args a) {
+  constexpr int G = (R == 1) ? 8 : (R == 2) ? 4 : 2;   // columns per QR pass
   __shared__ double normU[MAXK], normD[MAXK], tsq[MAXK], cc[MAXK];
   __shared__ int phys[MAXK];
 
@@ -77,28 +85,55 @@ __global__ __launch_bounds__(64) void k_ols_fp64(Fp64Args a) {
   const int nf = a.cand_start[cand + 1] - pbeg;
   const int k = nf + 1;
   const int64_t n = a.n, ldx = a.ldx;
-  double* const S = a.slab + a.slab_off[blockIdx.x];   // columns 0..k-1: A, column k: Q^T y
+  // slab columns 0..k-1: A (factors, free term), k: Q^T y, k+1..k+nf: the
+  // factor values as evaluated (kept for the prediction of step 4)
+  double* const S = a.slab + a.slab_off[blockIdx.x];
   const double* const X = a.X;
+  double* const co = a.coefs + (int64_t)pbeg + cand;
 
   // ---- 1. factor columns, free term, copy of y; norms
+  // Factor 0 NaN at a row >= 1 fixes the outcome: its norm is NaN, so the
+  // rank threshold is NaN (no cut), factor 0 pivots first (nothing compares
+  // greater than NaN), its tail norm and hence its reflector are NaN, and
+  // with the threshold NaN no later column stops the factorisation: every
+  // coefficient is NaN, rank = min(n, k).  Evaluation stops at the first tile
+  // that shows it.
   for (int j = 0; j < nf; ++j) {
     const uint32_t* code = a.code + a.prog_start[pbeg + j];
     const int len = a.prog_start[pbeg + j + 1] - a.prog_start[pbeg + j];
     double* col = S + (int64_t)j * ldx;
+    double* keep = S + (int64_t)(k + 1 + j) * ldx;
     double s_all = 0.0, s_1 = 0.0;
-    for (int64_t i = lane; i < n; i += WAVE) {
-      const double v = eval_f64(code, len, a.K, X, ldx, i);
-      col[i] = v;
-      s_all = s_all + v * v;
-      if (i >= 1) s_1 = s_1 + v * v;
+    for (int64_t base = 0; base < n; base += 64 * R) {
+      const int64_t row0 = base + lane;
+      double v[R];
+      eval_f64<R>(code, len, a.K, X, ldx, row0, v);
+      bool nan_tail = false;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int64_t i = row0 + 64 * r;
+        col[i] = v[r];
+        keep[i] = v[r];
+        if (i < n) {
+          s_all = s_all + v[r] * v[r];
+          if (i >= 1) s_1 = s_1 + v[r] * v[r];
+          if (i >= 1 && v[r] != v[r]) nan_tail = true;
+        }
+      }
+      if (j == 0 && __any((int)nan_tail)) {
+        const int size0 = (int)((n < (int64_t)k) ? n : (int64_t)k);
+        for (int i = lane; i < k; i += WAVE) co[i] = (i < size0) ? rrm::qnan() : 0.0;
+        if (lane == 0) write_fit(a.fit + cand, rrm::qnan(), a.sst, n, size0, 0);
+        return;
+      }
     }
     s_all = wave_sum(s_all);
     s_1 = wave_sum(s_1);
     if (lane == 0) { normD[j] = __builtin_sqrt(s_all); tsq[j] = s_1; }
   }
+  double* const cy = S + (int64_t)k * ldx;
   {
     double* col = S + (int64_t)nf * ldx;
-    double* cy = S + (int64_t)k * ldx;
     for (int64_t i = lane; i < n; i += WAVE) { col[i] = 1.0; cy[i] = a.y[i]; }
     if (lane == 0) { normD[nf] = __builtin_sqrt((double)n); tsq[nf] = (double)(n - 1); }
   }
@@ -114,7 +149,8 @@ __global__ __launch_bounds__(64) void k_ols_fp64(Fp64Args a) {
   const double thr = ((mx * DBL_EPS) * (mx * DBL_EPS)) / (double)n;
   const double dd_thr = __builtin_sqrt(DBL_EPS);
   int nonzero = size;
-  double* const cy = S + (int64_t)k * ldx;
+  int nan_exit = -1;   // step whose reflector had a NaN beta
+  int steps = 0;       // steps that made passes over the samples
 
   for (int p = 0; p < size; ++p) {
     int bi = p;
@@ -135,7 +171,8 @@ __global__ __launch_bounds__(64) void k_ols_fp64(Fp64Args a) {
     const double c0 = v[p];
     const double tailSq = (n - p == 1) ? 0.0 : tsq[p];
     double tau, beta;
-    if (tailSq <= DBL_MIN_) {
+    const bool zero_tail = tailSq <= DBL_MIN_;
+    if (zero_tail) {
       tau = 0.0;
       beta = c0;
       for (int64_t i = first_row(p + 1, lane); i < n; i += WAVE) v[i] = 0.0;
@@ -144,6 +181,19 @@ __global__ __launch_bounds__(64) void k_ols_fp64(Fp64Args a) {
       if (c0 >= 0.0) beta = -beta;
       tau = (beta - c0) / beta;
     }
+    if (!zero_tail && beta != beta && n - p > 1) {
+      // A NaN reflector turns every remaining column and Q^T y into NaN, so
+      // R(p,p) = NaN and every coefficient up to the rank is NaN.  The norm
+      // downdate leaves columns with norm 0 at 0 and makes all others NaN;
+      // later steps pivot in place, and the first column at 0 stops the
+      // factorisation if the threshold is positive.
+      nan_exit = p;
+      int r = size;
+      if (thr > 0.0)
+        for (int j = p + 1; j < size; ++j) if (normU[j] == 0.0) { r = j; break; }
+      nonzero = r;
+      break;
+    }
     const double den = c0 - beta;
     const bool apply = tau != 0.0;         // applyHouseholderOnTheLeft does nothing for tau == 0
     __syncthreads();   // every lane has read c0 before row p is overwritten
@@ -151,6 +201,7 @@ __global__ __launch_bounds__(64) void k_ols_fp64(Fp64Args a) {
 
     const int nrem = k - 1 - p;            // remaining A columns: positions p+1..k-1
     const int ntarget = nrem + 1;          // plus the y column (last target)
+    ++steps;
     if (n - p == 1) {
       // applyHouseholderOnTheLeft with a single row: scale by (1 - tau)
       if (lane == 0) {
@@ -180,6 +231,7 @@ __global__ __launch_bounds__(64) void k_ols_fp64(Fp64Args a) {
       continue;
     }
 
+    const int64_t t0 = tile_start(p + 1, R);
     for (int g0 = 0; g0 < ntarget; g0 += G) {
       const int gn = (ntarget - g0 < G) ? (ntarget - g0) : G;
       double* cols[G];
@@ -196,11 +248,25 @@ __global__ __launch_bounds__(64) void k_ols_fp64(Fp64Args a) {
         double acc[G];
 #pragma unroll
         for (int t = 0; t < G; ++t) acc[t] = 0.0;
-        for (int64_t i = first_row(p + 1, lane); i < n; i += WAVE) {
-          double e;
-          if (g0 == 0) { e = v[i] / den; v[i] = e; } else { e = v[i]; }
+        for (int64_t base = t0; base < n; base += 64 * R) {
+          double e[R];
+          bool ok[R];
 #pragma unroll
-          for (int t = 0; t < G; ++t) if (t < gn) acc[t] = acc[t] + e * cols[t][i];
+          for (int r = 0; r < R; ++r) {
+            const int64_t i = base + lane + 64 * r;
+            ok[r] = i > p && i < n;
+            if (g0 == 0) { e[r] = v[i] / den; if (ok[r]) v[i] = e[r]; } else { e[r] = v[i]; }
+          }
+#pragma unroll
+          for (int t = 0; t < G; ++t) {
+            if (t < gn) {
+#pragma unroll
+              for (int r = 0; r < R; ++r) {
+                const double x = cols[t][base + lane + 64 * r];
+                if (ok[r]) acc[t] = acc[t] + e[r] * x;
+              }
+            }
+          }
         }
 #pragma unroll
         for (int t = 0; t < G; ++t) {
@@ -221,15 +287,28 @@ __global__ __launch_bounds__(64) void k_ols_fp64(Fp64Args a) {
       double s1[G], s2[G];
 #pragma unroll
       for (int t = 0; t < G; ++t) { s1[t] = 0.0; s2[t] = 0.0; }
-      for (int64_t i = first_row(p + 1, lane); i < n; i += WAVE) {
-        const double te = apply ? tau * v[i] : 0.0;
+      for (int64_t base = t0; base < n; base += 64 * R) {
+        double te[R];
+        bool ok[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int64_t i = base + lane + 64 * r;
+          ok[r] = i > p && i < n;
+          te[r] = apply ? tau * v[i] : 0.0;
+        }
 #pragma unroll
         for (int t = 0; t < G; ++t) {
           if (t < gn) {
-            double x = cols[t][i];
-            if (apply) { x = x - te * tdot[t]; cols[t][i] = x; }
-            s1[t] = s1[t] + x * x;
-            if (i >= p + 2) s2[t] = s2[t] + x * x;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+              const int64_t i = base + lane + 64 * r;
+              double x = cols[t][i];
+              if (ok[r]) {
+                if (apply) { x = x - te[r] * tdot[t]; cols[t][i] = x; }
+                s1[t] = s1[t] + x * x;
+                if (i >= p + 2) s2[t] = s2[t] + x * x;
+              }
+            }
           }
         }
       }
@@ -266,8 +345,14 @@ __global__ __launch_bounds__(64) void k_ols_fp64(Fp64Args a) {
     __syncthreads();
   }
 
-  // ---- 3. back substitution on the leading nonzero x nonzero block
   const int r = nonzero;
+  if (nan_exit >= 0) {
+    for (int i = lane; i < k; i += WAVE) co[phys[i]] = (i < r) ? rrm::qnan() : 0.0;
+    if (lane == 0) write_fit(a.fit + cand, rrm::qnan(), a.sst, n, r, steps);
+    return;
+  }
+
+  // ---- 3. back substitution on the leading nonzero x nonzero block
   for (int i = lane; i < r; i += WAVE) cc[i] = cy[i];
   __syncthreads();
   for (int i = r - 1; i >= 0; --i) {
@@ -278,39 +363,62 @@ __global__ __launch_bounds__(64) void k_ols_fp64(Fp64Args a) {
     if (lane == 0) cc[i] = ci;
     __syncthreads();
   }
-  double* const co = a.coefs + (int64_t)pbeg + cand;
+  bool any_nan = false;
+  for (int i = 0; i < k; ++i) {
+    const double cv = (i < r) ? cc[i] : 0.0;
+    if (cv != cv) any_nan = true;
+  }
   for (int i = lane; i < k; i += WAVE) {
     double cv = (i < r) ? cc[i] : 0.0;
     if (cv != cv) cv = rrm::qnan();
     co[phys[i]] = cv;
   }
   __syncthreads();
+  if (any_nan) {   // a kept NaN coefficient: every prediction is NaN
+    if (lane == 0) write_fit(a.fit + cand, rrm::qnan(), a.sst, n, r, steps);
+    return;
+  }
 
   // ---- 4. prediction of the OLS tree and fitness
-  double cf_free = co[nf];
+  const double cf_free = co[nf];
   double ssr = 0.0;
-  for (int64_t i = lane; i < n; i += WAVE) {
-    double yp = 0.0;
+  for (int64_t base = 0; base < n; base += 64 * R) {
+    const int64_t row0 = base + lane;
+    double yp[R];
     bool any = false;
+#pragma unroll
+    for (int q = 0; q < R; ++q) yp[q] = 0.0;
     for (int j = 0; j < nf; ++j) {
       const double cj = co[j];
       if (__builtin_fabs(cj) < EPS12) continue;
-      const uint32_t* code = a.code + a.prog_start[pbeg + j];
-      const int len = a.prog_start[pbeg + j + 1] - a.prog_start[pbeg + j];
-      const double f = eval_f64(code, len, a.K, X, ldx, i);
-      const double term = (__builtin_fabs(cj - 1) < EPS12) ? f : cj * f;
-      yp = any ? yp + term : term;
+      const double* keep = S + (int64_t)(k + 1 + j) * ldx;
+      double f[R];
+#pragma unroll
+      for (int q = 0; q < R; ++q) f[q] = keep[row0 + 64 * q];
+      const bool bare = __builtin_fabs(cj - 1) < EPS12;
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        const double term = bare ? f[q] : cj * f[q];
+        yp[q] = any ? yp[q] + term : term;
+      }
       any = true;
     }
     if (!(__builtin_fabs(cf_free) < EPS12)) {
       const double term = cf_free * 1.0;
-      yp = any ? yp + term : term;
+#pragma unroll
+      for (int q = 0; q < R; ++q) yp[q] = any ? yp[q] + term : term;
     }
-    const double d = a.y[i] - yp;
-    ssr = ssr + d * d;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int64_t i = row0 + 64 * q;
+      if (i < n) {
+        const double d = a.y[i] - yp[q];
+        ssr = ssr + d * d;
+      }
+    }
   }
   ssr = wave_sum(ssr);
-  if (lane == 0) write_fit(a.fit + cand, ssr, a.sst, n, r);
+  if (lane == 0) write_fit(a.fit + cand, ssr, a.sst, n, r, steps);
 }
 
 struct RawArgs {
@@ -327,6 +435,7 @@ struct RawArgs {
 
 // fitness() of a tree as given (no OLS): main-loop perturbations and the
 // re-scores after acceptance (rils_rols_cpp.cpp:631, 800, 828, 842).
+template <int R>
 __global__ __launch_bounds__(64) void k_raw_fp64(RawArgs a) {
   const int cand = (int)blockIdx.x;
   const int lane = (int)threadIdx.x;
@@ -334,9 +443,18 @@ __global__ __launch_bounds__(64) void k_raw_fp64(RawArgs a) {
   const uint32_t* code = a.code + a.prog_start[p];
   const int len = a.prog_start[p + 1] - a.prog_start[p];
   double ssr = 0.0;
-  for (int64_t i = lane; i < a.n; i += WAVE) {
-    const double d = a.y[i] - eval_f64(code, len, a.K, a.X, a.ldx, i);
-    ssr = ssr + d * d;
+  for (int64_t base = 0; base < a.n; base += 64 * R) {
+    const int64_t row0 = base + lane;
+    double yp[R];
+    eval_f64<R>(code, len, a.K, a.X, a.ldx, row0, yp);
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int64_t i = row0 + 64 * q;
+      if (i < a.n) {
+        const double d = a.y[i] - yp[q];
+        ssr = ssr + d * d;
+      }
+    }
   }
   ssr = wave_sum(ssr);
   if (lane == 0) write_fit(a.fit + cand, ssr, a.sst, a.n, 0);
@@ -346,11 +464,13 @@ __global__ __launch_bounds__(64) void k_raw_fp64(RawArgs a) {
 __global__ __launch_bounds__(256) void k_predict_fp64(const uint32_t* code, int len, const double* K, const double* X,
                                                       int64_t n, int64_t ldx, double* out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = eval_f64(code, len, K, X, ldx, i);
+  double v[1];
+  eval_f64<1>(code, len, K, X, ldx, i < n ? i : 0, v);
+  if (i < n) out[i] = v[0];
 }
 
 // mean(y) and sum((y - mean)^2), canonical order (R2, rils_rols_cpp.cpp:40-45)
-__global__ __launch_bounds__(64) void k_stats_fp64(const double* y, int64_t n, double* out2) {
+__global__ __launch_bounds__(64) void k_stats_fp64(const double* y, int64_t n, double* out3) {
   const int lane = (int)threadIdx.x;
   double s = 0.0;
   for (int64_t i = lane; i < n; i += WAVE) s = s + y[i];
@@ -359,7 +479,7 @@ __global__ __launch_bounds__(64) void k_stats_fp64(const double* y, int64_t n, d
   double q = 0.0;
   for (int64_t i = lane; i < n; i += WAVE) q = q + (y[i] - ybar) * (y[i] - ybar);
   q = wave_sum(q);
-  if (lane == 0) { out2[0] = ybar; out2[1] = q; }
+  if (lane == 0) { out3[0] = ybar; out3[1] = q; out3[2] = s; }
 }
 
 }  // namespace
@@ -370,9 +490,16 @@ hipError_t launch_ols_fp64(const OlsLaunch& L, hipStream_t s) {
   a.X = L.X; a.y = L.y; a.n = L.n; a.ldx = L.ldx; a.sst = L.sst;
   a.slab = L.slab; a.slab_off = L.slab_off; a.coefs = L.coefs; a.fit = L.fit; a.cand_base = L.cand_base;
   const dim3 grid(L.n_cands), block(64);
-  if (L.max_cols <= 16) hipLaunchKernelGGL(k_ols_fp64<16>, grid, block, 0, s, a);
-  else if (L.max_cols <= 64) hipLaunchKernelGGL(k_ols_fp64<64>, grid, block, 0, s, a);
-  else hipLaunchKernelGGL(k_ols_fp64<RR_MAX_COLS>, grid, block, 0, s, a);
+  const int R = L.rows;
+  if (L.max_cols <= 16) {
+    if (R == 1) hipLaunchKernelGGL((k_ols_fp64<16, 1>), grid, block, 0, s, a);
+    else if (R == 2) hipLaunchKernelGGL((k_ols_fp64<16, 2>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((k_ols_fp64<16, 4>), grid, block, 0, s, a);
+  } else if (L.max_cols <= 64) {
+    hipLaunchKernelGGL((k_ols_fp64<64, 2>), grid, block, 0, s, a);
+  } else {
+    hipLaunchKernelGGL((k_ols_fp64<RR_MAX_COLS, 2>), grid, block, 0, s, a);
+  }
   return hipGetLastError();
 }
 
@@ -380,7 +507,9 @@ hipError_t launch_raw_fp64(const RawLaunch& L, hipStream_t s) {
   RawArgs a;
   a.cand_start = L.cand_start; a.prog_start = L.prog_start; a.code = L.code; a.K = L.K;
   a.X = L.X; a.y = L.y; a.n = L.n; a.ldx = L.ldx; a.sst = L.sst; a.fit = L.fit;
-  hipLaunchKernelGGL(k_raw_fp64, dim3(L.n_cands), dim3(64), 0, s, a);
+  if (L.rows == 1) hipLaunchKernelGGL(k_raw_fp64<1>, dim3(L.n_cands), dim3(64), 0, s, a);
+  else if (L.rows == 2) hipLaunchKernelGGL(k_raw_fp64<2>, dim3(L.n_cands), dim3(64), 0, s, a);
+  else hipLaunchKernelGGL(k_raw_fp64<4>, dim3(L.n_cands), dim3(64), 0, s, a);
   return hipGetLastError();
 }
 
@@ -391,8 +520,8 @@ hipError_t launch_predict_fp64(const uint32_t* code, int len, const double* K, c
   return hipGetLastError();
 }
 
-hipError_t launch_stats_fp64(const double* y, int64_t n, double* out2, hipStream_t s) {
-  hipLaunchKernelGGL(k_stats_fp64, dim3(1), dim3(64), 0, s, y, n, out2);
+hipError_t launch_stats_fp64(const double* y, int64_t n, double* out3, hipStream_t s) {
+  hipLaunchKernelGGL(k_stats_fp64, dim3(1), dim3(64), 0, s, y, n, out3);
   return hipGetLastError();
 }
 
