@@ -168,6 +168,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--precision", default="both", choices=["both", "fp64", "fp32"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--fp64-rows", type=int, default=None, help="samples per lane per tile, FP64 kernels (1, 2, 4)")
+    ap.add_argument("--fp32-rows", type=int, default=None, help="samples per lane per tile, FP32 kernels (1, 2)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: collectives and multi-GPU sharding are not part of this build "
@@ -179,6 +181,10 @@ def main():
     X, y, target, trees, factors = workload()
     t_gen = time.perf_counter() - t0
     runner = rils_rols_cpp.BatchRunner(np.ascontiguousarray(X.T), y, 0)
+    if args.fp64_rows:
+        runner.set_option("fp64_rows", args.fp64_rows)
+    if args.fp32_rows:
+        runner.set_option("fp32_rows", args.fp32_rows)
     t0 = time.perf_counter()
     nf = runner.prepare_ols(trees)
     t_prep = time.perf_counter() - t0

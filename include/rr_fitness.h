@@ -99,8 +99,10 @@ typedef struct rr_batch {
 
 /* precision of rr_fitness_batch */
 #define RR_FP64 0        /* plain form: bit-for-bit the canonical FP64 algorithm */
-#define RR_FP32_REFINED 1 /* FP32 evaluation + compensated Gram sums, FP64 solve
-                             with one refinement step; candidates flagged
+#define RR_FP32_REFINED 1 /* single pass: FP32 evaluation with per-value error
+                             bounds, Gram sums of the FP32 values in FP64 (exact
+                             products), FP64 solve with one refinement step, and
+                             a certified bound rr_fit.err; candidates flagged
                              RR_F_NEEDS64 must be re-scored in FP64 before they
                              decide anything */
 
@@ -115,7 +117,12 @@ typedef struct rr_fit {
   double e0;      /* 1 - R2 (rils_rols_cpp.cpp:538) */
   double e1;      /* RMSE */
   double err;     /* FP32 form: bound on |fitness_value error| relative; 0 in FP64 */
-  int32_t rank;   /* nonzero pivots of the column-pivoted QR (OLS mode) */
+  int32_t rank;   /* nonzero pivots of the column-pivoted QR (OLS mode); FP32
+                     form: -reason when flagged RR_F_NEEDS64 (1 uncertain
+                     value, 2 fewer samples than columns, 3 NaN column with
+                     ill-conditioned finite columns, 4 ill-conditioned, 5 a
+                     coefficient threshold inside the bound, 6 bound too wide,
+                     7 more than 8 factors) */
   int32_t flags;  /* RR_F_* */
 } rr_fit;
 
@@ -155,10 +162,21 @@ int rr_fitness_batch(rr_ctx* ctx, int32_t handle, const rr_batch* batch, int32_t
 int rr_predict(rr_ctx* ctx, int32_t handle, const uint32_t* code, int32_t len, const double* consts,
                int32_t n_consts, double* out);
 
+/* FP32 evaluation of one program with a per-sample bound: |out_v[i] - v64[i]|
+ * <= out_e[i], where v64 is what rr_predict returns.  out_e[i] = +inf where
+ * FP32 cannot vouch for the sample (overflow, a domain edge inside the
+ * bound); out_v[i] = NaN with out_e[i] = 0 where v64[i] is certainly NaN.
+ * No counterpart in the reference: it exposes the evaluator of the
+ * RR_FP32_REFINED precision form, whose bounds the certificates rest on. */
+int rr_predict_fp32(rr_ctx* ctx, int32_t handle, const uint32_t* code, int32_t len, const double* consts,
+                    int32_t n_consts, float* out_v, float* out_e);
+
 /* ---- tuning ---------------------------------------------------------------
  * Named integer options: "slab_budget_mb" (device memory the FP64 OLS path
- * may use for its per-candidate work columns), "fp32_block_waves",
- * "fp32_rows_per_tile".  Unknown names return RR_E_ARG. */
+ * may use for its per-candidate work columns, default 32768), "fp64_rows"
+ * (samples per lane per tile of the FP64 kernels: 1, 2 or 4; default 2),
+ * "fp32_rows" (the same for the FP32 kernels: 1 or 2; default 1).
+ * Unknown names return RR_E_ARG. */
 int rr_set_option(rr_ctx* ctx, const char* name, int64_t value);
 int rr_get_option(rr_ctx* ctx, const char* name, int64_t* value);
 

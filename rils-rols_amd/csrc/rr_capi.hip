@@ -73,9 +73,8 @@ struct Dataset {
   int32_t d = 0;
   double* X = nullptr;    // d columns, stride ldx
   double* y = nullptr;
-  float* X32 = nullptr;   // FP32 copies for the precision form
-  float* y32 = nullptr;
-  double ybar = 0, sst = 0, ysum = 0;
+  float* X32 = nullptr;   // FP32 copy for the precision form
+  double ybar = 0, sst = 0, ysum = 0, yy = 0;
 };
 
 }  // namespace
@@ -84,13 +83,16 @@ struct rr_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // side streams (default priority) for the concurrent FP32 bucket kernels,
+  // forked from and joined back into `stream` with events
+  hipStream_t side[rr::FP32_BUCKETS - 1] = {};
+  hipEvent_t fork = nullptr, join[rr::FP32_BUCKETS - 1] = {};
   std::vector<Dataset> ds;
-  DevBuf d_cand, d_prog, d_code, d_K, d_slab, d_slab_off, d_coefs, d_fit, d_misc, d_fp32_work;
+  DevBuf d_cand, d_prog, d_code, d_K, d_slab, d_slab_off, d_coefs, d_fit, d_misc, d_pred_e;
   HostBuf h_stage, h_out;
   int64_t slab_budget_mb = 32768;
-  int64_t fp32_block_waves = 4;
-  int64_t fp32_rows_per_tile = 256;
   int64_t fp64_rows = 2;
+  int64_t fp32_rows = 1;
   double last_ms = 0.0;
 };
 
@@ -197,8 +199,13 @@ int rr_create(int device, rr_ctx** out) {
   RR_HIP(hipSetDevice(device));
   rr_ctx* c = new rr_ctx;
   c->device = device;
-  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+  bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess &&
+            hipEventCreate(&c->ev0) == hipSuccess && hipEventCreate(&c->ev1) == hipSuccess &&
+            hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) == hipSuccess;
+  for (int i = 0; ok && i < rr::FP32_BUCKETS - 1; ++i)
+    ok = hipStreamCreateWithFlags(&c->side[i], hipStreamNonBlocking) == hipSuccess &&
+         hipEventCreateWithFlags(&c->join[i], hipEventDisableTiming) == hipSuccess;
+  if (!ok) {
     delete c;
     return fail(RR_E_HIP, "stream/event creation failed");
   }
@@ -212,15 +219,21 @@ int rr_destroy(rr_ctx* ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   for (Dataset& d : ctx->ds) {
     if (!d.live) continue;
-    (void)hipFree(d.X); (void)hipFree(d.y); (void)hipFree(d.X32); (void)hipFree(d.y32);
+    (void)hipFree(d.X); (void)hipFree(d.y); (void)hipFree(d.X32);
   }
   DevBuf* bufs[] = {&ctx->d_cand, &ctx->d_prog, &ctx->d_code, &ctx->d_K, &ctx->d_slab, &ctx->d_slab_off,
-                    &ctx->d_coefs, &ctx->d_fit, &ctx->d_misc, &ctx->d_fp32_work};
+                    &ctx->d_coefs, &ctx->d_fit, &ctx->d_misc, &ctx->d_pred_e};
   for (DevBuf* b : bufs) b->release();
   ctx->h_stage.release();
   ctx->h_out.release();
   (void)hipEventDestroy(ctx->ev0);
   (void)hipEventDestroy(ctx->ev1);
+  (void)hipEventDestroy(ctx->fork);
+  for (int i = 0; i < rr::FP32_BUCKETS - 1; ++i) {
+    (void)hipStreamSynchronize(ctx->side[i]);
+    (void)hipStreamDestroy(ctx->side[i]);
+    (void)hipEventDestroy(ctx->join[i]);
+  }
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return RR_OK;
@@ -237,10 +250,9 @@ int rr_dataset_upload(rr_ctx* ctx, const double* X_fm, const double* y, int64_t 
   const size_t xb = (size_t)D.ldx * (size_t)std::max(d, 1) * sizeof(double);
   if (hipMalloc(&D.X, xb) != hipSuccess) return fail(RR_E_NOMEM, "hipMalloc for X failed");
   if (hipMalloc(&D.y, (size_t)D.ldx * sizeof(double)) != hipSuccess) { (void)hipFree(D.X); return fail(RR_E_NOMEM, "hipMalloc for y failed"); }
-  if (hipMalloc(&D.X32, (size_t)D.ldx * std::max(d, 1) * sizeof(float)) != hipSuccess ||
-      hipMalloc(&D.y32, (size_t)D.ldx * sizeof(float)) != hipSuccess) {
-    (void)hipFree(D.X); (void)hipFree(D.y); (void)hipFree(D.X32);
-    return fail(RR_E_NOMEM, "hipMalloc for FP32 copies failed");
+  if (hipMalloc(&D.X32, (size_t)D.ldx * std::max(d, 1) * sizeof(float)) != hipSuccess) {
+    (void)hipFree(D.X); (void)hipFree(D.y);
+    return fail(RR_E_NOMEM, "hipMalloc for the FP32 copy failed");
   }
   RR_HIP(hipMemsetAsync(D.X, 0, xb, ctx->stream));
   RR_HIP(hipMemsetAsync(D.y, 0, (size_t)D.ldx * sizeof(double), ctx->stream));
@@ -249,15 +261,15 @@ int rr_dataset_upload(rr_ctx* ctx, const double* X_fm, const double* y, int64_t 
                             (size_t)n * sizeof(double), (size_t)d, hipMemcpyHostToDevice, ctx->stream));
   RR_HIP(hipMemcpyAsync(D.y, y, (size_t)n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
   RR_HIP(rr::launch_to_fp32(D.X, D.X32, (int64_t)D.ldx * std::max(d, 1), ctx->stream));
-  RR_HIP(rr::launch_to_fp32(D.y, D.y32, D.ldx, ctx->stream));
-  if (ctx->d_misc.ensure(3 * sizeof(double)) != RR_OK) return RR_E_NOMEM;
+  if (ctx->d_misc.ensure(4 * sizeof(double)) != RR_OK) return RR_E_NOMEM;
   RR_HIP(rr::launch_stats_fp64(D.y, n, (double*)ctx->d_misc.p, ctx->stream));
-  double st[3];
+  double st[4];
   RR_HIP(hipMemcpyAsync(st, ctx->d_misc.p, sizeof st, hipMemcpyDeviceToHost, ctx->stream));
   RR_HIP(hipStreamSynchronize(ctx->stream));
   D.ybar = st[0];
   D.sst = st[1];
   D.ysum = st[2];
+  D.yy = st[3];
   D.live = true;
   ctx->ds.push_back(D);
   *handle = (int32_t)ctx->ds.size() - 1;
@@ -270,7 +282,7 @@ int rr_dataset_free(rr_ctx* ctx, int32_t handle) {
   Dataset* D = get_ds(ctx, handle);
   if (!D) return fail(RR_E_ARG, "bad dataset handle");
   RR_HIP(hipStreamSynchronize(ctx->stream));
-  (void)hipFree(D->X); (void)hipFree(D->y); (void)hipFree(D->X32); (void)hipFree(D->y32);
+  (void)hipFree(D->X); (void)hipFree(D->y); (void)hipFree(D->X32);
   *D = Dataset();
   return RR_OK;
 }
@@ -338,11 +350,12 @@ int rr_fitness_batch(rr_ctx* ctx, int32_t handle, const rr_batch* b, int32_t mod
       for (int32_t c = 0; c < nc; ++c) order[c] = c;
       if ((rc = upload(ctx, ctx->d_slab_off, order.data(), (size_t)nc * 4, so))) return rc;
       rr::Fp32Launch L{};
-      L.cand_start = dc; L.prog_start = dp; L.code = dcode; L.K = dK; L.X32 = D->X32; L.y32 = D->y32; L.X = D->X;
-      L.y = D->y; L.n = D->n; L.ldx = D->ldx; L.ybar = D->ybar; L.sst = D->sst; L.ysum = D->ysum; L.coefs = dcoef;
-      L.fit = dfit; L.n_cands = nc; L.raw = 1; L.block_waves = (int)ctx->fp32_block_waves;
-      const int counts[4] = {nc, 0, 0, 0};
-      RR_HIP(rr::launch_fp32(L, (const int32_t*)ctx->d_slab_off.p, counts, ctx->stream));
+      L.cand_start = dc; L.prog_start = dp; L.code = dcode; L.K = dK; L.X32 = D->X32; L.y = D->y; L.n = D->n;
+      L.ldx = D->ldx; L.sst = D->sst; L.ysum = D->ysum; L.yy = D->yy; L.coefs = dcoef; L.fit = dfit;
+      L.n_cands = nc; L.raw = 1; L.rows = (int)ctx->fp32_rows;
+      const int counts[rr::FP32_BUCKETS] = {nc, 0, 0, 0, 0};
+      const hipStream_t st[rr::FP32_BUCKETS] = {ctx->stream, ctx->stream, ctx->stream, ctx->stream, ctx->stream};
+      RR_HIP(rr::launch_fp32(L, (const int32_t*)ctx->d_slab_off.p, counts, st));
     }
     RR_HIP(hipEventRecord(ctx->ev1, ctx->stream));
   } else if (precision == RR_FP64) {
@@ -383,23 +396,36 @@ int rr_fitness_batch(rr_ctx* ctx, int32_t handle, const rr_batch* b, int32_t mod
   } else {
     // group candidates by column count so each kernel instance keeps only
     // the accumulators its bucket needs
-    std::vector<int32_t> order;
-    order.reserve(nc);
-    int counts[4] = {0, 0, 0, 0};
-    const int lim[3] = {2, 5, 9};
-    for (int bkt = 0; bkt < 4; ++bkt)
-      for (int32_t c = 0; c < nc; ++c) {
-        const int k = b->cand_start[c + 1] - b->cand_start[c] + 1;
-        const int kb = k <= lim[0] ? 0 : k <= lim[1] ? 1 : k <= lim[2] ? 2 : 3;
-        if (kb == bkt) { order.push_back(c); ++counts[bkt]; }
-      }
+    std::vector<int32_t> order(nc);
+    int counts[rr::FP32_BUCKETS] = {0, 0, 0, 0, 0};
+    std::vector<int8_t> bucket(nc);
+    for (int32_t c = 0; c < nc; ++c) {
+      const int nf = b->cand_start[c + 1] - b->cand_start[c];
+      int kb = 0;
+      while (kb < rr::FP32_BUCKETS - 1 && nf > rr::FP32_BUCKET_FACTORS[kb]) ++kb;
+      bucket[c] = (int8_t)kb;
+      ++counts[kb];
+    }
+    int start[rr::FP32_BUCKETS];
+    for (int kb = 0, acc = 0; kb < rr::FP32_BUCKETS; ++kb) { start[kb] = acc; acc += counts[kb]; }
+    for (int32_t c = 0; c < nc; ++c) order[start[bucket[c]]++] = c;
     if ((rc = upload(ctx, ctx->d_slab_off, order.data(), (size_t)nc * 4, so))) return rc;
     rr::Fp32Launch L{};
-    L.cand_start = dc; L.prog_start = dp; L.code = dcode; L.K = dK; L.X32 = D->X32; L.y32 = D->y32; L.X = D->X;
-    L.y = D->y; L.n = D->n; L.ldx = D->ldx; L.ybar = D->ybar; L.sst = D->sst; L.ysum = D->ysum; L.coefs = dcoef;
-    L.fit = dfit; L.n_cands = nc; L.raw = 0; L.block_waves = (int)ctx->fp32_block_waves;
+    L.cand_start = dc; L.prog_start = dp; L.code = dcode; L.K = dK; L.X32 = D->X32; L.y = D->y; L.n = D->n;
+    L.ldx = D->ldx; L.sst = D->sst; L.ysum = D->ysum; L.yy = D->yy; L.coefs = dcoef; L.fit = dfit;
+    L.n_cands = nc; L.raw = 0; L.rows = (int)ctx->fp32_rows;
+    // bucket 0 (the bulk) on the main stream, the others on side streams
+    hipStream_t st[rr::FP32_BUCKETS];
+    st[0] = ctx->stream;
+    for (int i = 1; i < rr::FP32_BUCKETS; ++i) st[i] = ctx->side[i - 1];
     RR_HIP(hipEventRecord(ctx->ev0, ctx->stream));
-    RR_HIP(rr::launch_fp32(L, (const int32_t*)ctx->d_slab_off.p, counts, ctx->stream));
+    RR_HIP(hipEventRecord(ctx->fork, ctx->stream));
+    for (int i = 0; i < rr::FP32_BUCKETS - 1; ++i) RR_HIP(hipStreamWaitEvent(ctx->side[i], ctx->fork, 0));
+    RR_HIP(rr::launch_fp32(L, (const int32_t*)ctx->d_slab_off.p, counts, st));
+    for (int i = 0; i < rr::FP32_BUCKETS - 1; ++i) {
+      RR_HIP(hipEventRecord(ctx->join[i], ctx->side[i]));
+      RR_HIP(hipStreamWaitEvent(ctx->stream, ctx->join[i], 0));
+    }
     RR_HIP(hipEventRecord(ctx->ev1, ctx->stream));
   }
 
@@ -439,19 +465,43 @@ int rr_predict(rr_ctx* ctx, int32_t handle, const uint32_t* code, int32_t len, c
   return RR_OK;
 }
 
+int rr_predict_fp32(rr_ctx* ctx, int32_t handle, const uint32_t* code, int32_t len, const double* consts,
+                    int32_t n_consts, float* out_v, float* out_e) {
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  Dataset* D = get_ds(ctx, handle);
+  if (!D) return fail(RR_E_ARG, "bad dataset handle");
+  if (!code || !out_v || !out_e || n_consts < 0 || (n_consts > 0 && !consts)) return fail(RR_E_ARG, "bad predict arguments");
+  if ((rc = check_program(code, len, D->d, n_consts, nullptr))) return rc;
+  const size_t s_code = align_up((size_t)len * 4);
+  if (ctx->h_stage.ensure(s_code + align_up((size_t)std::max(n_consts, 1) * 8)) != RR_OK) return RR_E_NOMEM;
+  if ((rc = upload(ctx, ctx->d_code, code, (size_t)len * 4, 0))) return rc;
+  if (n_consts > 0) { if ((rc = upload(ctx, ctx->d_K, consts, (size_t)n_consts * 8, s_code))) return rc; }
+  else if (ctx->d_K.ensure(8) != RR_OK) return RR_E_NOMEM;
+  if (ctx->d_misc.ensure((size_t)D->n * 4) != RR_OK || ctx->d_pred_e.ensure((size_t)D->n * 4) != RR_OK) return RR_E_NOMEM;
+  RR_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+  RR_HIP(rr::launch_predict_fp32((const uint32_t*)ctx->d_code.p, len, (const double*)ctx->d_K.p, D->X32, D->n, D->ldx,
+                                 (float*)ctx->d_misc.p, (float*)ctx->d_pred_e.p, ctx->stream));
+  RR_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+  RR_HIP(hipMemcpyAsync(out_v, ctx->d_misc.p, (size_t)D->n * 4, hipMemcpyDeviceToHost, ctx->stream));
+  RR_HIP(hipMemcpyAsync(out_e, ctx->d_pred_e.p, (size_t)D->n * 4, hipMemcpyDeviceToHost, ctx->stream));
+  RR_HIP(hipStreamSynchronize(ctx->stream));
+  float ms = 0.f;
+  RR_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  ctx->last_ms = ms;
+  return RR_OK;
+}
+
 int rr_set_option(rr_ctx* ctx, const char* name, int64_t value) {
   if (!ctx || !name) return fail(RR_E_ARG, "null argument");
   const std::string s(name);
   if (s == "slab_budget_mb") { if (value < 1) return fail(RR_E_ARG, "slab_budget_mb must be >= 1"); ctx->slab_budget_mb = value; }
-  else if (s == "fp32_block_waves") {
-    if (value != 1 && value != 2 && value != 4 && value != 8) return fail(RR_E_ARG, "fp32_block_waves must be 1, 2, 4 or 8");
-    ctx->fp32_block_waves = value;
-  } else if (s == "fp64_rows") {
+  else if (s == "fp64_rows") {
     if (value != 1 && value != 2 && value != 4) return fail(RR_E_ARG, "fp64_rows must be 1, 2 or 4");
     ctx->fp64_rows = value;
-  } else if (s == "fp32_rows_per_tile") {
-    if (value < 64 || value > 4096 || (value & 63)) return fail(RR_E_ARG, "fp32_rows_per_tile must be a multiple of 64 in [64, 4096]");
-    ctx->fp32_rows_per_tile = value;
+  } else if (s == "fp32_rows") {
+    if (value != 1 && value != 2) return fail(RR_E_ARG, "fp32_rows must be 1 or 2");
+    ctx->fp32_rows = value;
   } else return fail(RR_E_ARG, "unknown option " + s);
   return RR_OK;
 }
@@ -460,9 +510,8 @@ int rr_get_option(rr_ctx* ctx, const char* name, int64_t* value) {
   if (!ctx || !name || !value) return fail(RR_E_ARG, "null argument");
   const std::string s(name);
   if (s == "slab_budget_mb") *value = ctx->slab_budget_mb;
-  else if (s == "fp32_block_waves") *value = ctx->fp32_block_waves;
-  else if (s == "fp32_rows_per_tile") *value = ctx->fp32_rows_per_tile;
   else if (s == "fp64_rows") *value = ctx->fp64_rows;
+  else if (s == "fp32_rows") *value = ctx->fp32_rows;
   else return fail(RR_E_ARG, "unknown option " + s);
   return RR_OK;
 }

@@ -45,7 +45,7 @@ hipError_t launch_ols_fp64(const OlsLaunch& L, hipStream_t s);
 hipError_t launch_raw_fp64(const RawLaunch& L, hipStream_t s);
 hipError_t launch_predict_fp64(const uint32_t* code, int len, const double* K, const double* X, int64_t n,
                                int64_t ldx, double* out, hipStream_t s);
-// out3 = {mean(y), sum((y - mean)^2), sum(y)}
-hipError_t launch_stats_fp64(const double* y, int64_t n, double* out3, hipStream_t s);
+// out4 = {mean(y), sum((y - mean)^2), sum(y), sum(y^2)}
+hipError_t launch_stats_fp64(const double* y, int64_t n, double* out4, hipStream_t s);
 
 }  // namespace rr

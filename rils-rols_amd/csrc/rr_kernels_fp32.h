@@ -13,22 +13,29 @@ struct Fp32Launch {
   const uint32_t* code;
   const double* K;
   const float* X32;   // FP32 copy of X, same layout
-  const float* y32;
-  const double* X;    // FP64 originals (bound computation)
   const double* y;
   int64_t n, ldx;
-  double ybar, sst, ysum;
+  double sst, ysum, yy;   // sum (y - ybar)^2, sum y, sum y^2
   double* coefs;
   rr_fit* fit;
   int n_cands;
   int raw;            // 1: score the tree as given, 0: OLS
-  int block_waves;    // waves (candidates) per workgroup
+  int rows;           // samples per lane per tile: 1 or 2
 };
 
-// `order` lists the candidates grouped by column count: bucket_count[0]
-// with k <= 2, [1] with k <= 5, [2] with k <= 9, [3] larger (flagged
-// RR_F_NEEDS64: no FP32 form).  RAW mode uses order as is.
-hipError_t launch_fp32(const Fp32Launch& L, const int32_t* order, const int* bucket_count, hipStream_t s);
+// Number of factor buckets of the FP32 OLS kernels: candidates with at most
+// 1, 2, 4 and 8 factors; a fifth group (more factors) is flagged
+// RR_F_NEEDS64 without evaluation.
+constexpr int FP32_BUCKETS = 5;
+constexpr int FP32_BUCKET_FACTORS[FP32_BUCKETS - 1] = {1, 2, 4, 8};
+
+// `order` lists the candidates grouped by bucket, bucket_count[b] of each;
+// bucket b is launched on streams[b] (the caller orders the streams).  RAW
+// mode uses order as is, on streams[0].
+hipError_t launch_fp32(const Fp32Launch& L, const int32_t* order, const int* bucket_count, const hipStream_t* streams);
 hipError_t launch_to_fp32(const double* src, float* dst, int64_t count, hipStream_t s);
+// FP32 prediction with per-sample bounds (ldx a multiple of 256)
+hipError_t launch_predict_fp32(const uint32_t* code, int len, const double* K, const float* X, int64_t n, int64_t ldx,
+                               float* out_v, float* out_e, hipStream_t s);
 
 }  // namespace rr

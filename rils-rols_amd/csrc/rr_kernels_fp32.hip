@@ -1,42 +1,61 @@
 // rr_kernels_fp32.hip -- FP32 precision form of the candidate-fitness path.
 //
-// One wavefront owns one candidate (like the FP64 form) and makes two passes
-// over the samples, each evaluating the candidate's factor programs in FP32
-// for R rows per lane:
-//   pass 1  accumulates the Gram matrix of [factors | 1] and A^T y in FP64
-//           (a product of two FP32 numbers is exact in FP64, so only the
-//           accumulation rounds);
-//   solve   FP64 pivoted Cholesky of the Gram matrix on the wave, with the
-//           pivot rule of ColPivHouseholderQR (largest remaining column
-//           norm, first index on ties);
-//   pass 2  residual r = y - A c in FP64, A^T r for one refinement step, SSR,
-//           and the error terms of the bound below;
-//   refine  c += G^-1 A^T r, SSR -= dc . A^T r.
+// On this hardware plain FP32 and FP64 vector arithmetic issue at the same
+// rate; FP32 is cheaper only through the hardware transcendental, reciprocal
+// and square-root instructions and through moving half the bytes.  The form
+// is built around that: one wavefront owns one candidate and makes ONE pass
+// over the samples, evaluating the factor programs in FP32 with the hardware
+// instructions for R rows per lane, while every value carries a running bound
+// e >= |v32 - v64| on its distance from the FP64 form's value of the same
+// expression.  The pass accumulates, per candidate,
+//     G  = A^T A   (factors; FP64 -- a product of two FP32 numbers is exact)
+//     S1 = column sums, SY = A^T y, Bd = sum_i e_ij^2       (FP64)
+// and then, on one lane:
+//   solve   FP64 pivoted Cholesky of the Gram matrix of [factors | 1] (the
+//           pivot rule of ColPivHouseholderQR: largest remaining column, first
+//           index on ties), one refinement step on the normal equations;
+//   SSR     yy - 2 c.g + c^T G c (its rounding counted in the bound);
+//   bound   |sqrt(SSR32) - sqrt(SSR64)| <= D,
+//             D = sum_j |c_j| sqrt(Bd_j)                    (||dA |c|||)
+//               + sum_j dc_j sqrt(Bd_j)                     (coefficient shift)
+//               + sum_j 1e-12 sqrt(G_jj) (1 + |c_j|)        (OLS-tree thresholds)
+//               + 2 sqrt(dcr^T G dcr)                       (solve residue)
+//           dc = 2 |G^-1| (|dA|^T |r| + |A|^T |dA| |c|) + ...   (first-order
+//           least-squares perturbation, doubled), with the products bounded by
+//           Cauchy-Schwarz from G, Bd and y.y:
+//             (|dA|^T|r|)_j    <= sqrt(Bd_j) (sqrt(yy) + sum_l |c_l| sqrt(G_ll))
+//             (|A|^T|dA||c|)_j <= sqrt(G_jj) sum_l |c_l| sqrt(Bd_l)
+// Candidates that cannot be certified are flagged RR_F_NEEDS64 (reason in
+// rr_fit.rank, negated) and re-scored by the FP64 form: a value whose
+// finiteness FP32 and FP64 can disagree on (overflow, a domain boundary
+// inside the bound); a rank decision (Schur complement below TAU_C of its
+// column); a coefficient whose |c| < 1e-12 or |c - 1| < 1e-12 decision lies
+// inside its bound; more than 8 factors; n < k.
 //
-// Every FP32 value carries a running error bound e >= |v32 - v| where v is
-// the FP64 value of the same expression (u = 2^-23, twice the FP32 unit
-// roundoff, absorbs the rounding of the bound arithmetic and the FP64 path's
-// own error).  A value is "uncertain" (e = +inf) when FP32 and FP64 can
-// disagree on finiteness (overflow, a domain boundary inside the error bound);
-// a NaN with e == 0 is a "certain NaN": FP64 produces NaN at that sample too.
+// NaN: +, -, *, / and the unary operators give NaN for a NaN argument, so a
+// domain violation that is certain (sqrt or ln of an argument certainly
+// negative) makes the FP64 value NaN at that sample; the row state carries
+// that.  The QR outcome with certain-NaN columns is decided as in the FP64
+// form (see k_ols_fp32).  MIN and MAX can absorb a NaN: under them such a row
+// becomes uncertain; comparisons give 0 or 1 either way (bound 1).
 //
-// Certification, per candidate:
-//   * any uncertain value                                   -> RR_F_NEEDS64
-//   * a certain-NaN factor column: the FP64 QR reaches that column's
-//     reflector (no rank cut before it, guaranteed when the finite columns
-//     are well conditioned, or when it is factor 0 and the threshold is NaN)
-//     and then every coefficient is NaN                      -> sentinel, exact
-//   * Schur complement of a pivot below TAU_C of its column, or n < k
-//                                                            -> RR_F_NEEDS64
-//   * otherwise |sqrt(SSR32) - sqrt(SSR64)| <= D with
-//       D = ||dA |c|||  +  sum_j (dc_j + |refinement_j|) ||dA_j||
-//           + sum_j 1e-12 ||A_j|| (1 + |c_j|)          (OLS-tree thresholds)
-//     where dA is bounded elementwise by the running errors and
-//       dc = 2 |G^-1| (|dA|^T |r| + |A|^T |dA| |c|)     (first-order LS
-//     perturbation, doubled); a coefficient whose |c| vs 1e-12 or |c - 1|
-//     vs 1e-12 decision lies inside dc                    -> RR_F_NEEDS64.
-//   The bound on SSR gives bounds on 1-R2 and RMSE, and rr_fit.err is the
-//   relative bound on the fitness value (1+e0)(1+e1)(1+size*penalty).
+// Operator bounds (u = 2^-23, twice the FP32 unit roundoff; T = 2^-125 covers
+// a result that underflows, flushed or not):
+//   + -   e = ea + eb + |v| u
+//   *     e = (|a| + ea) eb + |b| ea + |v| u + T
+//   /     e = (|v| eb + ea) / (|b| - eb) (1 + 8u) + 2 |v| u + T
+//                                              (needs |b| > eb, |b| < 2^100)
+//   sqr   e = (2|a| + ea) ea + |v| u + T
+//   sqrt  e = ea / (sqrt(a) + sqrt(a - ea)) (1 + 4u) + 2 |v| u   (a > ea or exact 0)
+//   ln    e = ea / (a - ea) (1 + 4u) + H_LOG (1 + |v|)            (a > ea)
+//   exp   e = v (ea (1 + ea) + 2|a| u + H_EXP) + 2^-124           (ea <= 1)
+//   sin, cos  e = min(2|a| u + ea, 2) + H_TRIG   (|a| < 1600; beyond: v = 0, e = 1)
+// The 2|a|u terms are the rounding of the argument scaling (x / 2pi, x log2 e)
+// the hardware instructions need.  H_* bound the instructions' own errors
+// against the FP64 functions; the GPU test suite measures them.  The bound
+// arithmetic itself rounds (relative 3 * 2^-24 per operation at most), which
+// the final D absorbs through a factor 1 + 2^-8 (programs of up to ~20000
+// operations).
 #include "rr_device.h"
 #include "rr_kernels_fp32.h"
 
@@ -46,112 +65,130 @@ namespace {
 
 constexpr float U32 = 0x1p-23f;
 constexpr float FINF = __builtin_huge_valf();
-constexpr double TAU_C = 1e-4;       // smallest accepted Schur complement / column norm^2
-constexpr double TAU_S = 1e-20;      // smallest accepted Schur complement / (max norm)^2
+constexpr float TINY = 0x1p-125f;
+constexpr float H_EXP = 0x1p-20f;     // relative
+constexpr float H_LOG = 0x1p-20f;     // absolute, scaled by (1 + |v|)
+constexpr float H_TRIG = 0x1p-19f;    // absolute
+constexpr float TRIG_RANGE = 1600.f;  // hardware sin/cos take |x / 2pi| <= 256
+constexpr double TAU_C = 1e-4;        // smallest accepted Schur complement / column norm^2
+constexpr double TAU_S = 1e-20;       // smallest accepted Schur complement / (max norm)^2
 constexpr double EPS12 = 1e-12;
-constexpr int MAXF = 8;              // factors per candidate in this form (k <= 9)
+constexpr double D_ROUND = 1.0 + 0x1p-8;
+constexpr float INV_2PI = 0.15915494309189535f;
+constexpr float LOG2E = 1.4426950408889634f;
+constexpr float LN2 = 0.6931471805599453f;
+
+// row state bits
+constexpr uint32_t S_NAN = 1u;   // the FP64 value is NaN at this sample
+constexpr uint32_t S_UNC = 2u;   // FP32 cannot vouch for this sample
 
 struct VE { float v, e; };
 
-__device__ inline bool is_certain_nan(float v, float e) { return v != v && e == 0.f; }
-__device__ inline float fix_e(float e) { return (e == e) ? e : FINF; }
+__device__ inline float hw_exp(float x) { return __builtin_amdgcn_exp2f(x * LOG2E); }
+__device__ inline float hw_log(float x) { return __builtin_amdgcn_logf(x) * LN2; }
+__device__ inline float hw_sin(float x) { return __builtin_amdgcn_sinf(x * INV_2PI); }
+__device__ inline float hw_cos(float x) { return __builtin_amdgcn_cosf(x * INV_2PI); }
+__device__ inline float hw_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ inline float hw_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
-// Transcendentals: HIP's FP32 functions, with the error bounds assumed (and
-// checked on the GPU against the FP64 functions by the test suite): sinf/cosf
-// 2^-21 absolute, expf/logf 4 ulp relative, sqrtf 1 ulp relative.
-__device__ __noinline__ VE f_sin(VE a) {
-  if (is_certain_nan(a.v, a.e)) return {a.v, 0.f};
-  const float v = sinf(a.v);
-  return {v, fix_e(fminf(a.e, 2.f) + 8.f * 0x1p-24f)};
-}
-__device__ __noinline__ VE f_cos(VE a) {
-  if (is_certain_nan(a.v, a.e)) return {a.v, 0.f};
-  const float v = cosf(a.v);
-  return {v, fix_e(fminf(a.e, 2.f) + 8.f * 0x1p-24f)};
-}
-__device__ __noinline__ VE f_exp(VE a) {
-  if (is_certain_nan(a.v, a.e)) return {a.v, 0.f};
-  const float v = expf(a.v);
-  if (!(v < FINF)) return {v, FINF};
-  // |e^(a+d) - e^a| <= e^a (e^|d| - 1), and e^x - 1 <= x (1 + x) for 0 <= x <= 1
-  const float g = (a.e <= 1.f) ? a.e * (1.f + a.e) : expf(a.e) - 1.f;
-  return {v, fix_e(v * g * (1.f + 4.f * U32) + v * 4.f * U32 + 0x1p-126f)};
-}
-__device__ __noinline__ VE f_log(VE a) {
-  if (is_certain_nan(a.v, a.e)) return {a.v, 0.f};
-  const float lo = a.v - a.e * (1.f + U32);
-  if (lo > 0.f) {
-    const float v = logf(a.v);
-    return {v, fix_e(a.e / lo * (1.f + 2.f * U32) + fabsf(v) * 4.f * U32)};
-  }
-  if (a.v + a.e * (1.f + U32) < 0.f) return {__builtin_nanf(""), 0.f};   // certainly negative
-  return {logf(a.v), FINF};
-}
-__device__ inline VE f_sqrt(VE a) {
-  if (is_certain_nan(a.v, a.e)) return {a.v, 0.f};
-  const float lo = a.v - a.e * (1.f + U32);
-  if (lo >= 0.f) {
-    const float v = sqrtf(a.v);
-    const float den = v + sqrtf(lo);
-    const float e = den > 0.f ? a.e / den * (1.f + 2.f * U32) : sqrtf(a.e);
-    return {v, fix_e(e + v * U32)};
-  }
-  if (a.v + a.e * (1.f + U32) < 0.f) return {__builtin_nanf(""), 0.f};
-  return {sqrtf(a.v), FINF};
-}
-
-__device__ inline VE f_bin(uint32_t bop, VE a, VE b) {
-  const bool an = is_certain_nan(a.v, a.e), bn = is_certain_nan(b.v, b.e);
+__device__ inline VE op_bin(uint32_t bop, VE a, VE b, uint32_t& st) {
   switch (bop) {
-    case RR_BOP_ADD: case RR_BOP_SUB: case RR_BOP_MUL: case RR_BOP_DIV: {
-      float v;
-      if (bop == RR_BOP_ADD) v = a.v + b.v;
-      else if (bop == RR_BOP_SUB) v = a.v - b.v;
-      else if (bop == RR_BOP_MUL) v = a.v * b.v;
-      else v = a.v / b.v;
-      if (an || bn) return {__builtin_nanf(""), 0.f};   // NaN op anything = NaN in both precisions
-      if (!(fabsf(v) < FINF)) return {v, FINF};
-      float e;
-      if (bop <= RR_BOP_SUB) e = a.e + b.e;
-      else if (bop == RR_BOP_MUL) e = fabsf(a.v) * b.e + fabsf(b.v) * a.e + a.e * b.e;
-      else {
-        const float bl = fabsf(b.v) - b.e * (1.f + U32);
-        if (!(bl > 0.f)) return {v, FINF};
-        e = (fabsf(a.v) * b.e + fabsf(b.v) * a.e) / (fabsf(b.v) * bl);
-      }
-      return {v, fix_e(e * (1.f + 4.f * U32) + fabsf(v) * U32)};
+    case RR_BOP_ADD: { const float v = a.v + b.v; return {v, fmaf(fabsf(v), U32, a.e + b.e)}; }
+    case RR_BOP_SUB: { const float v = a.v - b.v; return {v, fmaf(fabsf(v), U32, a.e + b.e)}; }
+    case RR_BOP_MUL: {
+      const float v = a.v * b.v;
+      return {v, fmaf(fabsf(a.v) + a.e, b.e, fmaf(fabsf(b.v), a.e, fmaf(fabsf(v), U32, TINY)))};
     }
-    case RR_BOP_LT: case RR_BOP_GT: case RR_BOP_EQ: case RR_BOP_NEQ: {
+    case RR_BOP_DIV: {
+      // |a/b - a'/b'| <= (|a| eb + |b| ea) / (|b| (|b| - eb)) = (|a/b| eb + ea) / (|b| - eb);
+      // this form cannot overflow where the bound itself does not
+      const float v = a.v * hw_rcp(b.v);
+      const float ab = fabsf(b.v);
+      const float bl = ab - b.e * (1.f + U32);
+      const float e = fmaf(fmaf(fabsf(v), b.e, a.e) * hw_rcp(bl), 1.f + 8.f * U32, fmaf(fabsf(v), 2.f * U32, TINY));
+      return {v, (bl > 0.f && ab < 0x1p100f) ? e : FINF};
+    }
+    case RR_BOP_MIN: case RR_BOP_MAX: {
+      if (st & S_NAN) st = (st & ~S_NAN) | S_UNC;   // min(NaN, b) = b, min(a, NaN) = NaN
+      const bool ta = (bop == RR_BOP_MIN) ? (a.v < b.v) : (a.v > b.v);
+      const VE r = ta ? a : b;
+      const float d = fabsf(a.v - b.v);
+      const bool sep = d > (a.e + b.e) * (1.f + 4.f * U32);
+      return {r.v, sep ? r.e : fmaxf(a.e, b.e) + d};
+    }
+    default: {   // comparisons: 0 or 1 in both precisions, whatever the arguments
+      st &= ~S_NAN;
       float v;
       if (bop == RR_BOP_LT) v = a.v < b.v ? 1.f : 0.f;
       else if (bop == RR_BOP_GT) v = a.v > b.v ? 1.f : 0.f;
       else if (bop == RR_BOP_EQ) v = a.v == b.v ? 1.f : 0.f;
       else v = a.v != b.v ? 1.f : 0.f;
-      if (an || bn) return {v, 0.f};   // a comparison with NaN is false (!= true) in both precisions
-      const bool sep = fabsf(a.v - b.v) > (a.e + b.e) * (1.f + 4.f * U32) && bop != RR_BOP_EQ && bop != RR_BOP_NEQ;
-      const bool exact = a.e == 0.f && b.e == 0.f;
-      return {v, (sep || exact) ? 0.f : 1.f};
-    }
-    default: {   // MIN / MAX: a<b ? a : b  /  a>b ? a : b
-      const bool take_a = (bop == RR_BOP_MIN) ? (a.v < b.v) : (a.v > b.v);
-      const VE r = take_a ? a : b;
-      if (a.e == FINF || b.e == FINF) return {r.v, FINF};
-      if (an || bn) return (a.e == 0.f && b.e == 0.f) ? r : VE{r.v, FINF};
       const bool sep = fabsf(a.v - b.v) > (a.e + b.e) * (1.f + 4.f * U32);
-      return {r.v, sep ? r.e : fix_e(fmaxf(a.e, b.e) + fabsf(a.v - b.v))};
+      const bool exact = a.e == 0.f && b.e == 0.f && !(st & S_UNC);
+      return {v, (sep || exact) && !(st & S_UNC) ? 0.f : 1.f};
     }
   }
 }
 
-// FP32 evaluation of one program at R samples (rows row0 + 64 r).
+__device__ inline VE op_un(uint32_t op, VE a, uint32_t& st) {
+  switch (op) {
+    case RR_OP_SIN: case RR_OP_COS: {
+      const float aa = fabsf(a.v);
+      const bool in = aa < TRIG_RANGE;
+      const float v = in ? ((op == RR_OP_SIN) ? hw_sin(a.v) : hw_cos(a.v)) : 0.f;
+      const float e = in ? fminf(fmaf(aa, 2.f * U32, a.e), 2.f) + H_TRIG : 1.f;
+      return {v, a.e < FINF ? e : FINF};
+    }
+    case RR_OP_EXP: {
+      const float v = hw_exp(a.v);
+      const float e = fmaf(v, fmaf(a.e, 1.f + a.e, fmaf(fabsf(a.v), 2.f * U32, H_EXP)), 0x1p-124f);
+      return {v, a.e <= 1.f ? e : FINF};
+    }
+    case RR_OP_LN: {
+      const float slack = a.e * (1.f + U32);
+      const float lo = a.v - slack;
+      if (a.e < FINF && a.v + slack < 0.f) st |= S_NAN;   // argument certainly negative
+      const float v = hw_log(a.v);
+      const float e = fmaf(a.e * hw_rcp(lo), 1.f + 4.f * U32, H_LOG * (1.f + fabsf(v)));
+      return {v, lo > 0.f ? e : FINF};
+    }
+    case RR_OP_SQRT: {
+      const float slack = a.e * (1.f + U32);
+      const float lo = a.v - slack;
+      if (a.e < FINF && a.v + slack < 0.f) st |= S_NAN;
+      const float v = hw_sqrt(a.v);
+      const float den = v + hw_sqrt(fmaxf(lo, 0.f));
+      const float e = fmaf(a.e * hw_rcp(den), 1.f + 4.f * U32, 2.f * v * U32);
+      return {v, lo > 0.f ? e : ((a.v == 0.f && a.e == 0.f) ? 0.f : FINF)};
+    }
+    default: {  // SQR
+      const float v = a.v * a.v;
+      return {v, fmaf(2.f * fabsf(a.v) + a.e, a.e, fmaf(v, U32, TINY))};
+    }
+  }
+}
+
+// A constant of the program as an FP32 value with its bound (a nonzero
+// constant below 2^-100 in magnitude is left uncertain: its conversion can
+// underflow).
+__device__ inline VE const_ve(double kd) {
+  const float kf = (float)kd;
+  const double ad = __builtin_fabs(kd);
+  const bool ok = ad == 0.0 || ad >= 0x1p-100;
+  return {kf, ok ? fabsf(kf) * U32 : FINF};
+}
+
+// FP32 evaluation of one program at R samples (rows row0 + 64 r); st[r]
+// receives the row state.  X holds the FP32 copy of the features (tiny
+// nonzero and non-finite entries stored as NaN, see k_to_fp32).
 template <int R>
 __device__ inline void eval_f32(const uint32_t* __restrict__ code, int len, const double* __restrict__ K,
-                                const float* __restrict__ X, int64_t ldx, int64_t row0, VE (&acc)[R]) {
-  // slots 0-1 in registers; deeper slots (rare with Sethi-Ullman ordering)
-  // in a private array
-  VE st0[R], st1[R], stx[RR_MAX_STACK - 2][R];
+                                const float* __restrict__ X, int64_t ldx, int64_t row0, VE (&acc)[R],
+                                uint32_t (&st)[R]) {
+  VE s0[R], s1[R], sx[RR_MAX_STACK - 2][R];
+  uint32_t t0[R], t1[R], tx[RR_MAX_STACK - 2][R];
 #pragma unroll
-  for (int r = 0; r < R; ++r) acc[r] = {0.f, 0.f};
+  for (int r = 0; r < R; ++r) { acc[r] = {0.f, 0.f}; st[r] = 0; }
   int sp = 0;
   for (int pc = 0; pc < len; ++pc) {
     const uint32_t ins = code[pc];
@@ -164,18 +201,23 @@ __device__ inline void eval_f32(const uint32_t* __restrict__ code, int len, cons
       if (mode <= RR_M_SR) {
         --sp;
         VE s[R];
+        uint32_t t[R];
         if (sp == 0) {
 #pragma unroll
-          for (int r = 0; r < R; ++r) s[r] = st0[r];
+          for (int r = 0; r < R; ++r) { s[r] = s0[r]; t[r] = t0[r]; }
         } else if (sp == 1) {
 #pragma unroll
-          for (int r = 0; r < R; ++r) s[r] = st1[r];
+          for (int r = 0; r < R; ++r) { s[r] = s1[r]; t[r] = t1[r]; }
         } else {
 #pragma unroll
-          for (int r = 0; r < R; ++r) s[r] = stx[sp - 2][r];
+          for (int r = 0; r < R; ++r) { s[r] = sx[sp - 2][r]; t[r] = tx[sp - 2][r]; }
         }
 #pragma unroll
-        for (int r = 0; r < R; ++r) { l[r] = mode == RR_M_S ? s[r] : acc[r]; rr[r] = mode == RR_M_S ? acc[r] : s[r]; }
+        for (int r = 0; r < R; ++r) {
+          l[r] = mode == RR_M_S ? s[r] : acc[r];
+          rr[r] = mode == RR_M_S ? acc[r] : s[r];
+          st[r] |= t[r];
+        }
       } else if (mode <= RR_M_VR) {
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -185,17 +227,15 @@ __device__ inline void eval_f32(const uint32_t* __restrict__ code, int len, cons
           rr[r] = mode == RR_M_V ? xv : acc[r];
         }
       } else {
-        const double kd = K[arg];
-        const float kf = (float)kd;
-        const VE kv{kf, (fabsf(kf) < FINF) ? fabsf(kf) * U32 : FINF};
+        const VE kv = const_ve(K[arg]);
 #pragma unroll
         for (int r = 0; r < R; ++r) { l[r] = mode == RR_M_C ? acc[r] : kv; rr[r] = mode == RR_M_C ? kv : acc[r]; }
       }
       switch (bop) {   // one uniform branch per instruction, not per row
-#define RR_BIN(B) case B: _Pragma("unroll") for (int r = 0; r < R; ++r) acc[r] = f_bin(B, l[r], rr[r]); break;
+#define RR_BIN(B) case B: _Pragma("unroll") for (int r = 0; r < R; ++r) acc[r] = op_bin(B, l[r], rr[r], st[r]); break;
         RR_BIN(RR_BOP_ADD) RR_BIN(RR_BOP_SUB) RR_BIN(RR_BOP_MUL) RR_BIN(RR_BOP_DIV) RR_BIN(RR_BOP_LT)
         RR_BIN(RR_BOP_GT) RR_BIN(RR_BOP_EQ) RR_BIN(RR_BOP_NEQ) RR_BIN(RR_BOP_MIN)
-        default: _Pragma("unroll") for (int r = 0; r < R; ++r) acc[r] = f_bin(RR_BOP_MAX, l[r], rr[r]); break;
+        default: _Pragma("unroll") for (int r = 0; r < R; ++r) acc[r] = op_bin(RR_BOP_MAX, l[r], rr[r], st[r]); break;
 #undef RR_BIN
       }
     } else {
@@ -205,60 +245,43 @@ __device__ inline void eval_f32(const uint32_t* __restrict__ code, int len, cons
           for (int r = 0; r < R; ++r) {
             const float x = X[(int64_t)arg * ldx + row0 + 64 * r];
             acc[r] = {x, fabsf(x) * U32};
+            st[r] = 0;
           }
           break;
         case RR_OP_LDC: {
-          const float kf = (float)K[arg];
+          const VE kv = const_ve(K[arg]);
 #pragma unroll
-          for (int r = 0; r < R; ++r) acc[r] = {kf, (fabsf(kf) < FINF) ? fabsf(kf) * U32 : FINF};
+          for (int r = 0; r < R; ++r) { acc[r] = kv; st[r] = 0; }
           break;
         }
         case RR_OP_PUSH:
           if (sp == 0) {
 #pragma unroll
-            for (int r = 0; r < R; ++r) st0[r] = acc[r];
+            for (int r = 0; r < R; ++r) { s0[r] = acc[r]; t0[r] = st[r]; }
           } else if (sp == 1) {
 #pragma unroll
-            for (int r = 0; r < R; ++r) st1[r] = acc[r];
+            for (int r = 0; r < R; ++r) { s1[r] = acc[r]; t1[r] = st[r]; }
           } else {
 #pragma unroll
-            for (int r = 0; r < R; ++r) stx[sp - 2][r] = acc[r];
+            for (int r = 0; r < R; ++r) { sx[sp - 2][r] = acc[r]; tx[sp - 2][r] = st[r]; }
           }
           ++sp;
           break;
-        case RR_OP_SIN:
+#define RR_UN(O) case O: _Pragma("unroll") for (int r = 0; r < R; ++r) acc[r] = op_un(O, acc[r], st[r]); break;
+        RR_UN(RR_OP_SIN) RR_UN(RR_OP_COS) RR_UN(RR_OP_LN) RR_UN(RR_OP_EXP) RR_UN(RR_OP_SQRT)
+#undef RR_UN
+        default:
 #pragma unroll
-          for (int r = 0; r < R; ++r) acc[r] = f_sin(acc[r]);
+          for (int r = 0; r < R; ++r) acc[r] = op_un(RR_OP_SQR, acc[r], st[r]);
           break;
-        case RR_OP_COS:
-#pragma unroll
-          for (int r = 0; r < R; ++r) acc[r] = f_cos(acc[r]);
-          break;
-        case RR_OP_LN:
-#pragma unroll
-          for (int r = 0; r < R; ++r) acc[r] = f_log(acc[r]);
-          break;
-        case RR_OP_EXP:
-#pragma unroll
-          for (int r = 0; r < R; ++r) acc[r] = f_exp(acc[r]);
-          break;
-        case RR_OP_SQRT:
-#pragma unroll
-          for (int r = 0; r < R; ++r) acc[r] = f_sqrt(acc[r]);
-          break;
-        default: {  // SQR
-#pragma unroll
-          for (int r = 0; r < R; ++r) {
-            const VE a = acc[r];
-            if (is_certain_nan(a.v, a.e)) { acc[r] = {a.v, 0.f}; continue; }
-            const float v = a.v * a.v;
-            if (!(v < FINF)) { acc[r] = {v, FINF}; continue; }
-            acc[r] = {v, fix_e((2.f * fabsf(a.v) * a.e + a.e * a.e) * (1.f + 4.f * U32) + v * U32)};
-          }
-          break;
-        }
       }
     }
+  }
+  // final state: NaN (certain), uncertain, or certified value
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    if (st[r] & S_UNC) st[r] = S_UNC;
+    else if (!(st[r] & S_NAN) && !(acc[r].e < FINF && acc[r].v - acc[r].v == 0.f)) st[r] = S_UNC;
   }
 }
 
@@ -270,7 +293,7 @@ struct Fp32Args {
   const float* X32;
   const double* y;
   int64_t n, ldx;
-  double sst, ysum;
+  double sst, ysum, yy;
   double* coefs;
   rr_fit* fit;
   const int32_t* order;   // candidates of this launch (indices into the batch)
@@ -285,63 +308,77 @@ __device__ inline void write_needs64(rr_fit* f, int reason) {
   *f = o;
 }
 
+__device__ inline void write_sentinel(rr_fit* f, int rank) {
+  rr_fit o;
+  o.e0 = 1000; o.e1 = 1000; o.err = 0.0; o.rank = rank; o.flags = RR_F_SENTINEL;
+  *f = o;
+}
+
 // LDS scratch of the per-candidate solve
 template <int KB>
 struct Solve {
-  double M[KB][KB];    // Gram matrix, then Cholesky factor (lower)
-  double Ginv[KB][KB];
-  double g[KB], c[KB], w[KB];
+  double M[KB][KB];    // Gram matrix of [factors | 1]
+  double L[KB][KB];    // pivoted Cholesky factor (lower, permuted indices)
+  double g[KB], bd[KB];
   int perm[KB];
+  int ok;
 };
 
-// Pivoted Cholesky of the k x k matrix in S.M (ColPivHouseholderQR pivot
-// rule); returns false when a pivot fails the conditioning checks.  Only
-// columns with use[j] take part (the others are left out entirely).
+// Pivoted Cholesky of the m x m matrix T (pivot: largest remaining diagonal,
+// first index on ties) into S.L / S.perm; false when a pivot fails the
+// conditioning checks.
 template <int KB>
-__device__ inline bool chol(Solve<KB>& S, int k, double mx2) {
-  double diag0[KB];
-  for (int j = 0; j < k; ++j) { S.perm[j] = j; diag0[j] = S.M[j][j]; }
-  for (int p = 0; p < k; ++p) {
+__device__ inline bool chol(Solve<KB>& S, const double (&T)[KB][KB], int m) {
+  double diag0[KB], mx2 = 0.0;
+  for (int j = 0; j < m; ++j) {
+    S.perm[j] = j;
+    diag0[j] = T[j][j];
+    mx2 = mx2 < T[j][j] ? T[j][j] : mx2;
+    for (int i = 0; i < m; ++i) S.L[j][i] = T[j][i];
+  }
+  for (int p = 0; p < m; ++p) {
     int bi = p;
-    double best = S.M[S.perm[p]][S.perm[p]];
-    for (int j = p + 1; j < k; ++j) { const double v = S.M[S.perm[j]][S.perm[j]]; if (v > best) { best = v; bi = j; } }
+    double best = S.L[S.perm[p]][S.perm[p]];
+    for (int j = p + 1; j < m; ++j) { const double v = S.L[S.perm[j]][S.perm[j]]; if (v > best) { best = v; bi = j; } }
     const int t = S.perm[p]; S.perm[p] = S.perm[bi]; S.perm[bi] = t;
     const int q = S.perm[p];
     if (!(best > TAU_C * diag0[q]) || !(best > TAU_S * mx2)) return false;
     const double l = __builtin_sqrt(best);
-    S.M[q][q] = l;
-    for (int j = p + 1; j < k; ++j) { const int a = S.perm[j]; S.M[a][q] = S.M[a][q] / l; }
-    for (int j = p + 1; j < k; ++j) {
+    S.L[q][q] = l;
+    for (int j = p + 1; j < m; ++j) { const int a = S.perm[j]; S.L[a][q] = S.L[a][q] / l; }
+    for (int j = p + 1; j < m; ++j) {
       const int a = S.perm[j];
-      for (int i = p + 1; i <= j; ++i) { const int b = S.perm[i]; S.M[a][b] -= S.M[a][q] * S.M[b][q]; }
-      for (int i = p + 1; i <= j; ++i) { const int b = S.perm[i]; S.M[b][a] = S.M[a][b]; }
+      for (int i = p + 1; i <= j; ++i) { const int b = S.perm[i]; S.L[a][b] -= S.L[a][q] * S.L[b][q]; }
+      for (int i = p + 1; i <= j; ++i) { const int b = S.perm[i]; S.L[b][a] = S.L[a][b]; }
     }
   }
   return true;
 }
 
-// x = G^-1 b using the pivoted factor (lower L in S.M over permuted indices)
+// x = G^-1 b with the pivoted factor
 template <int KB>
 __device__ inline void chol_solve(const Solve<KB>& S, int k, const double* b, double* x) {
   double z[KB];
-  for (int p = 0; p < k; ++p) {            // L z = P^T b
+  for (int p = 0; p < k; ++p) {
     const int q = S.perm[p];
     double s = b[q];
-    for (int i = 0; i < p; ++i) s -= S.M[q][S.perm[i]] * z[i];
-    z[p] = s / S.M[q][q];
+    for (int i = 0; i < p; ++i) s -= S.L[q][S.perm[i]] * z[i];
+    z[p] = s / S.L[q][q];
   }
-  for (int p = k - 1; p >= 0; --p) {       // L^T w = z
+  for (int p = k - 1; p >= 0; --p) {
     const int q = S.perm[p];
     double s = z[p];
-    for (int i = p + 1; i < k; ++i) s -= S.M[S.perm[i]][q] * z[i];
-    z[p] = s / S.M[q][q];
+    for (int i = p + 1; i < k; ++i) s -= S.L[S.perm[i]][q] * z[i];
+    z[p] = s / S.L[q][q];
   }
   for (int p = 0; p < k; ++p) x[S.perm[p]] = z[p];
 }
 
-template <int KB, int R>
+// Single-pass OLS fitness of the candidates with at most NF factors.
+template <int NF, int R>
 __global__ __launch_bounds__(64) void k_ols_fp32(Fp32Args a) {
-  constexpr int NF = KB - 1;
+  constexpr int KB = NF + 1;
+  constexpr int NG = NF * (NF + 1) / 2;
   __shared__ Solve<KB> S;
   const int cand = a.order[blockIdx.x];
   const int lane = (int)threadIdx.x;
@@ -352,75 +389,61 @@ __global__ __launch_bounds__(64) void k_ols_fp32(Fp32Args a) {
   double* const co = a.coefs + (int64_t)pbeg + cand;
   if (n < k) { if (lane == 0) write_needs64(a.fit + cand, NR_SHORT); return; }
 
-  const uint32_t* codes[NF];
-  int lens[NF];
+  double G[NG], S1[NF], SY[NF], BD[NF];
 #pragma unroll
-  for (int j = 0; j < NF; ++j) {
-    if (j < nf) {
-      codes[j] = a.code + a.prog_start[pbeg + j];
-      lens[j] = a.prog_start[pbeg + j + 1] - a.prog_start[pbeg + j];
-    } else {
-      codes[j] = a.code;
-      lens[j] = 0;
-    }
-  }
-
-  // ---- pass 1: Gram of the factors, factor sums, A^T y; uncertain / NaN columns
-  double G[NF * (NF + 1) / 2], S1[NF], SY[NF];
+  for (int j = 0; j < NG; ++j) G[j] = 0.0;
 #pragma unroll
-  for (int j = 0; j < NF * (NF + 1) / 2; ++j) G[j] = 0.0;
-#pragma unroll
-  for (int j = 0; j < NF; ++j) { S1[j] = 0.0; SY[j] = 0.0; }
+  for (int j = 0; j < NF; ++j) { S1[j] = 0.0; SY[j] = 0.0; BD[j] = 0.0; }
   uint32_t unc = 0, nanc = 0;
-  bool nan0_tail = false;
   for (int64_t base = 0; base < n; base += 64 * R) {
     const int64_t row0 = base + lane;
     double v[NF][R];
+    float e[NF][R];
 #pragma unroll
-    for (int j = 0; j < NF; ++j) {
-      if (j < nf) {
-        VE f[R];
-        eval_f32<R>(codes[j], lens[j], a.K, a.X32, ldx, row0, f);
+    for (int j = 0; j < NF; ++j)
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-          const bool ok = row0 + 64 * r < n;
-          if (ok && f[r].e == FINF) unc |= 1u << j;
-          if (ok && is_certain_nan(f[r].v, f[r].e)) {
-            nanc |= 1u << j;
-            if (j == 0 && row0 + 64 * r >= 1) nan0_tail = true;
-          }
-          v[j][r] = (double)f[r].v;
-        }
+      for (int r = 0; r < R; ++r) { v[j][r] = 0.0; e[j][r] = 0.f; }
+    bool nan0_tail = false;
+    // one interpreter instance for all factors; the results go to their
+    // registers by a uniform index
+#pragma nounroll
+    for (int j = 0; j < nf; ++j) {
+      const int ps = a.prog_start[pbeg + j];
+      VE f[R];
+      uint32_t st[R];
+      eval_f32<R>(a.code + ps, a.prog_start[pbeg + j + 1] - ps, a.K, a.X32, ldx, row0, f, st);
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int64_t i = row0 + 64 * r;
+        const bool in = i < n;
+        if (in && st[r] == S_UNC) unc |= 1u << j;
+        if (in && st[r] == S_NAN) { nanc |= 1u << j; if (j == 0 && i >= 1) nan0_tail = true; }
+        const double fv = in ? (double)f[r].v : 0.0;
+        const float fe = in ? f[r].e : 0.f;
+#pragma unroll
+        for (int jj = 0; jj < NF; ++jj) if (jj == j) { v[jj][r] = fv; e[jj][r] = fe; }
       }
     }
-    // A certain NaN of factor 0 at a row >= 1 fixes the FP64 outcome: its
-    // column norm is NaN, so Eigen's threshold is NaN (no rank cut), factor 0
-    // pivots first, its tail norm is NaN and so is its reflector; every
-    // coefficient is then NaN.  The rest of the pass is not needed.
+    // Factor 0 certainly NaN at a row >= 1 decides the FP64 outcome: its
+    // column norm is NaN, so the rank threshold is NaN (no cut), factor 0
+    // pivots first and its reflector is NaN -- every coefficient is NaN.
     if (__any((int)nan0_tail)) {
       for (int i = lane; i < k; i += WAVE) co[i] = rrm::qnan();
-      if (lane == 0) {
-        rr_fit o;
-        o.e0 = 1000; o.e1 = 1000; o.err = 0.0; o.rank = k; o.flags = RR_F_SENTINEL;
-        a.fit[cand] = o;
-      }
+      if (lane == 0) write_sentinel(a.fit + cand, k);
       return;
     }
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      const int64_t i = row0 + 64 * r;
-      if (i < n) {
-        const double yi = a.y[i];
-        int g = 0;
+      const double yi = a.y[row0 + 64 * r] ;   // zero past n (padded column)
+      int g = 0;
 #pragma unroll
-        for (int j = 0; j < NF; ++j) {
+      for (int j = 0; j < NF; ++j) {
 #pragma unroll
-          for (int l = 0; l <= j; ++l) {
-            if (j < nf) G[g] = G[g] + v[j][r] * v[l][r];
-            ++g;
-          }
-          if (j < nf) { S1[j] = S1[j] + v[j][r]; SY[j] = SY[j] + v[j][r] * yi; }
-        }
+        for (int l = 0; l <= j; ++l) { G[g] = __builtin_fma(v[j][r], v[l][r], G[g]); ++g; }
+        S1[j] = S1[j] + v[j][r];
+        SY[j] = __builtin_fma(v[j][r], yi, SY[j]);
+        const double ej = (double)e[j][r];
+        BD[j] = __builtin_fma(ej, ej, BD[j]);
       }
     }
   }
@@ -428,177 +451,123 @@ __global__ __launch_bounds__(64) void k_ols_fp32(Fp32Args a) {
   for (int s = 32; s >= 1; s >>= 1) { unc |= __shfl_xor(unc, s); nanc |= __shfl_xor(nanc, s); }
   if (unc) { if (lane == 0) write_needs64(a.fit + cand, NR_UNCERTAIN); return; }
 #pragma unroll
-  for (int j = 0; j < NF * (NF + 1) / 2; ++j) G[j] = wave_sum(G[j]);
+  for (int j = 0; j < NG; ++j) G[j] = wave_sum(G[j]);
 #pragma unroll
-  for (int j = 0; j < NF; ++j) { S1[j] = wave_sum(S1[j]); SY[j] = wave_sum(SY[j]); }
+  for (int j = 0; j < NF; ++j) { S1[j] = wave_sum(S1[j]); SY[j] = wave_sum(SY[j]); BD[j] = wave_sum(BD[j]); }
+  if (lane != 0) return;
 
-  // full Gram of [factors | 1] in LDS (every lane holds the same values)
-  if (lane == 0) {
+  // ---- the rest runs on one lane
+  {
     int g = 0;
 #pragma unroll
     for (int j = 0; j < NF; ++j) {
 #pragma unroll
       for (int l = 0; l <= j; ++l) { if (j < nf) { S.M[j][l] = G[g]; S.M[l][j] = G[g]; } ++g; }
-      if (j < nf) { S.M[j][nf] = S1[j]; S.M[nf][j] = S1[j]; S.g[j] = SY[j]; }
+      if (j < nf) { S.M[j][nf] = S1[j]; S.M[nf][j] = S1[j]; S.g[j] = SY[j]; S.bd[j] = BD[j]; }
     }
     S.M[nf][nf] = (double)n;
     S.g[nf] = a.ysum;
+    S.bd[nf] = 0.0;
   }
-  __syncthreads();
-
   if (nanc) {
-    // The FP64 QR pivots the first NaN factor column j0 at step j0 (a NaN norm
-    // never wins the pivot search and stays in place); a cut before that step
-    // needs a near-dependent finite column.  With factor 0 NaN the threshold
-    // is NaN and no cut can happen at all.
+    // The FP64 QR pivots the first certain-NaN column at the step it sits at
+    // (a NaN norm never wins the pivot search), unless a finite column falls
+    // below its rank threshold first; with factor 0 NaN the threshold itself
+    // is NaN and nothing is cut.  Reaching a NaN reflector makes every
+    // coefficient NaN (the sentinel).  Certified when the finite columns are
+    // well conditioned.
     bool certain = (nanc & 1u) != 0;
-    if (!certain && lane == 0) {
-      // Gram of the finite columns only
+    if (!certain) {
       int idx[KB];
       int m = 0;
       for (int j = 0; j < k; ++j) if (j == nf || !((nanc >> j) & 1u)) idx[m++] = j;
       double T[KB][KB];
       for (int i = 0; i < m; ++i) for (int j = 0; j < m; ++j) T[i][j] = S.M[idx[i]][idx[j]];
-      for (int i = 0; i < m; ++i) for (int j = 0; j < m; ++j) S.M[i][j] = T[i][j];
-      double mx2 = 0.0;
-      for (int i = 0; i < m; ++i) mx2 = mx2 < S.M[i][i] ? S.M[i][i] : mx2;
-      S.w[0] = chol<KB>(S, m, mx2) ? 1.0 : 0.0;
+      certain = chol<KB>(S, T, m);
     }
-    __syncthreads();
-    if (!certain) certain = S.w[0] == 1.0;
-    if (!certain) { if (lane == 0) write_needs64(a.fit + cand, NR_NAN_COND); return; }
-    for (int i = lane; i < k; i += WAVE) co[i] = rrm::qnan();
-    if (lane == 0) {
-      rr_fit o;
-      o.e0 = 1000; o.e1 = 1000; o.err = 0.0; o.rank = k; o.flags = RR_F_SENTINEL;
-      a.fit[cand] = o;
-    }
+    if (!certain) { write_needs64(a.fit + cand, NR_NAN_COND); return; }
+    for (int i = 0; i < k; ++i) co[i] = rrm::qnan();
+    write_sentinel(a.fit + cand, k);
     return;
   }
+  if (!chol<KB>(S, S.M, k)) { write_needs64(a.fit + cand, NR_COND); return; }
 
-  // ---- solve G c = g
-  if (lane == 0) {
-    double mx2 = 0.0;
-    for (int j = 0; j < k; ++j) mx2 = mx2 < S.M[j][j] ? S.M[j][j] : mx2;
-    double Gs[KB][KB];
-    for (int i = 0; i < k; ++i) for (int j = 0; j < k; ++j) Gs[i][j] = S.M[i][j];
-    const bool ok = chol<KB>(S, k, mx2);
-    S.w[0] = ok ? 1.0 : 0.0;
-    if (ok) {
-      chol_solve<KB>(S, k, S.g, S.c);
-      double e[KB], col[KB];
-      for (int j = 0; j < k; ++j) {          // explicit inverse for the error bound
-        for (int i = 0; i < k; ++i) e[i] = (i == j) ? 1.0 : 0.0;
-        chol_solve<KB>(S, k, e, col);
-        for (int i = 0; i < k; ++i) S.Ginv[i][j] = col[i];
-      }
-    }
-    for (int i = 0; i < k; ++i) for (int j = 0; j < k; ++j) S.M[i][j] = Gs[i][j];
+  // solve, one refinement step on the normal equations
+  double c[KB], r1[KB], dcr[KB];
+  chol_solve<KB>(S, k, S.g, c);
+  for (int i = 0; i < k; ++i) {
+    double s = S.g[i];
+    for (int j = 0; j < k; ++j) s -= S.M[i][j] * c[j];
+    r1[i] = s;
   }
-  __syncthreads();
-  if (S.w[0] != 1.0) { if (lane == 0) write_needs64(a.fit + cand, NR_COND); return; }
-  double c[KB];
-#pragma unroll
-  for (int j = 0; j < KB; ++j) c[j] = (j < k) ? S.c[j] : 0.0;
-  const double cfree = S.c[nf];
+  chol_solve<KB>(S, k, r1, dcr);
+  for (int i = 0; i < k; ++i) c[i] += dcr[i];
 
-  // ---- pass 2: residual, A^T r, SSR and the bound terms
-  double ssr = 0.0, atr_free = 0.0, d2 = 0.0, aad_free = 0.0;
-  double ATR[NF], BER[NF], AAD[NF], BE2[NF];
-#pragma unroll
-  for (int j = 0; j < NF; ++j) { ATR[j] = 0.0; BER[j] = 0.0; AAD[j] = 0.0; BE2[j] = 0.0; }
-  for (int64_t base = 0; base < n; base += 64 * R) {
-    const int64_t row0 = base + lane;
-    double v[NF][R];
-    float e[NF][R];
-#pragma unroll
-    for (int j = 0; j < NF; ++j) {
-      if (j < nf) {
-        VE f[R];
-        eval_f32<R>(codes[j], lens[j], a.K, a.X32, ldx, row0, f);
-#pragma unroll
-        for (int r = 0; r < R; ++r) { v[j][r] = (double)f[r].v; e[j][r] = f[r].e; }
-      }
-    }
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const int64_t i = row0 + 64 * r;
-      if (i < n) {
-        double yp = cfree, dl = 0.0;
-#pragma unroll
-        for (int j = 0; j < NF; ++j)
-          if (j < nf) { yp = yp + c[j] * v[j][r]; dl = dl + __builtin_fabs(c[j]) * (double)e[j][r]; }
-        const double res = a.y[i] - yp;
-        ssr = ssr + res * res;
-        atr_free = atr_free + res;
-        d2 = d2 + dl * dl;
-        aad_free = aad_free + dl;
-        const double ar = __builtin_fabs(res);
-#pragma unroll
-        for (int j = 0; j < NF; ++j) {
-          if (j < nf) {
-            const double ej = (double)e[j][r];
-            ATR[j] = ATR[j] + v[j][r] * res;
-            BER[j] = BER[j] + ej * ar;
-            AAD[j] = AAD[j] + __builtin_fabs(v[j][r]) * dl;
-            BE2[j] = BE2[j] + ej * ej;
-          }
-        }
-      }
-    }
+  // SSR by the normal-equation identity; its rounding and that of the
+  // accumulations, bounded through Cauchy-Schwarz (|sum a b| <= sqrt(G_jj yy))
+  double cg = 0.0, cGc = 0.0, sab = 0.0, sbd = 0.0, dGd = 0.0;
+  double absc[KB], nrm[KB], sbdj[KB];
+  for (int i = 0; i < k; ++i) {
+    absc[i] = __builtin_fabs(c[i]);
+    nrm[i] = __builtin_sqrt(S.M[i][i]);
+    sbdj[i] = __builtin_sqrt(S.bd[i] * (1.0 + (double)(n / 64 + 64) * 0x1p-52));
+    cg += c[i] * S.g[i];
+    double s = 0.0, sd = 0.0;
+    for (int j = 0; j < k; ++j) { s += S.M[i][j] * c[j]; sd += S.M[i][j] * dcr[j]; }
+    cGc += c[i] * s;
+    dGd += dcr[i] * sd;
+    sab += absc[i] * nrm[i];
+    sbd += absc[i] * sbdj[i];
   }
-  ssr = wave_sum(ssr); atr_free = wave_sum(atr_free); d2 = wave_sum(d2); aad_free = wave_sum(aad_free);
-#pragma unroll
-  for (int j = 0; j < NF; ++j) { ATR[j] = wave_sum(ATR[j]); BER[j] = wave_sum(BER[j]); AAD[j] = wave_sum(AAD[j]); BE2[j] = wave_sum(BE2[j]); }
+  const double ny = __builtin_sqrt(a.yy);
+  const double mag = (ny + sab) * (ny + sab);
+  const double acc_err = 2.0 * ((double)(n / 64 + 128 + 8 * k) * 0x1p-52) * mag;
+  double ssr = a.yy - 2.0 * cg + cGc;
+  if (ssr < 0.0) ssr = 0.0;
 
-  if (lane == 0) {
-    double atr[KB], vv[KB], dc[KB], cr[KB], be[KB];
+  // coefficient bound dc and the bound D on |sqrt(SSR32) - sqrt(SSR64)|
+  double Ginv[KB][KB];
+  {
+    double ev[KB], col[KB];
     for (int j = 0; j < k; ++j) {
-      atr[j] = (j == nf) ? atr_free : ATR[j];
-      vv[j] = (j == nf) ? aad_free : BER[j] + AAD[j];
-      be[j] = (j == nf) ? 0.0 : __builtin_sqrt(BE2[j]);
+      for (int i = 0; i < k; ++i) ev[i] = (i == j) ? 1.0 : 0.0;
+      chol_solve<KB>(S, k, ev, col);
+      for (int i = 0; i < k; ++i) Ginv[i][j] = col[i];
     }
-    // refinement step
-    double ref[KB];
-    for (int i = 0; i < k; ++i) { double s = 0.0; for (int j = 0; j < k; ++j) s += S.Ginv[i][j] * atr[j]; ref[i] = s; }
-    double dssr = 0.0;
-    for (int i = 0; i < k; ++i) { cr[i] = c[i] + ref[i]; dssr += ref[i] * atr[i]; }
-    double ssr2 = ssr - dssr;
-    if (ssr2 < 0.0) ssr2 = 0.0;
-    // coefficient bound and the bound on sqrt(SSR)
-    double D = __builtin_sqrt(d2);
-    bool ambiguous = false;
-    for (int i = 0; i < k; ++i) {
-      double s = 0.0;
-      for (int j = 0; j < k; ++j) s += __builtin_fabs(S.Ginv[i][j]) * vv[j];
-      dc[i] = 2.0 * s + 1e-12 * __builtin_fabs(cr[i]) + 2.0 * __builtin_fabs(ref[i]);
-      D += (dc[i] + __builtin_fabs(ref[i])) * be[i] + EPS12 * __builtin_sqrt(S.M[i][i]) * (1.0 + __builtin_fabs(cr[i]));
-      const double ac = __builtin_fabs(cr[i]);
-      if (!(ac - dc[i] >= EPS12 || ac + dc[i] < EPS12)) ambiguous = true;
-      const double a1 = __builtin_fabs(cr[i] - 1.0);
-      if (i != nf && !(a1 - dc[i] >= EPS12 || a1 + dc[i] < EPS12)) ambiguous = true;
-    }
-    for (int i = 0; i < k; ++i) co[i] = cr[i];
-    rr_fit o;
-    const double rs = __builtin_sqrt(ssr2);
-    const double slack = 1e-10;
-    const double hi = (rs + D) * (rs + D) * (1.0 + slack) + 1e-300;
-    const double lo = (rs > D) ? (rs - D) * (rs - D) * (1.0 - slack) : 0.0;
-    const double r2 = 1 - ssr2 / a.sst;
-    o.e0 = 1 - r2;
-    o.e1 = __builtin_sqrt(ssr2 / (double)n);
-    const double e0h = hi / a.sst, e0l = lo / a.sst;
-    const double e1h = __builtin_sqrt(hi / (double)n), e1l = __builtin_sqrt(lo / (double)n);
-    const double fv = (1 + o.e0) * (1 + o.e1);
-    const double up = (1 + e0h) * (1 + e1h) / fv - 1.0, dn = 1.0 - (1 + e0l) * (1 + e1l) / fv;
-    o.err = (up > dn ? up : dn) + 1e-12;
-    o.rank = k;
-    o.flags = 0;
-    int reason = ambiguous ? NR_THRESHOLD : 0;
-    if (r2 != r2 || o.e1 != o.e1 || !(o.err < 1.0)) reason = NR_BOUND;
-    if (reason) { o.flags = RR_F_NEEDS64; o.err = __builtin_huge_val(); o.rank = -reason; }
-    a.fit[cand] = o;
   }
+  double vv[KB];
+  for (int j = 0; j < k; ++j) vv[j] = sbdj[j] * (ny + sab) + nrm[j] * sbd;
+  double D = sbd + 2.0 * __builtin_sqrt(__builtin_fabs(dGd));
+  bool ambiguous = false;
+  for (int i = 0; i < k; ++i) {
+    double s = 0.0;
+    for (int j = 0; j < k; ++j) s += __builtin_fabs(Ginv[i][j]) * vv[j];
+    const double dc = 2.0 * s + 1e-12 * absc[i] + 2.0 * __builtin_fabs(dcr[i]);
+    D += dc * sbdj[i] + EPS12 * nrm[i] * (1.0 + absc[i]);
+    if (!(absc[i] - dc >= EPS12 || absc[i] + dc < EPS12)) ambiguous = true;
+    const double a1 = __builtin_fabs(c[i] - 1.0);
+    if (i != nf && !(a1 - dc >= EPS12 || a1 + dc < EPS12)) ambiguous = true;
+  }
+  D *= D_ROUND;
+  for (int i = 0; i < k; ++i) co[i] = c[i];
+
+  const double rhi = __builtin_sqrt(ssr + acc_err) + D;
+  const double rlo = __builtin_sqrt(ssr > acc_err ? ssr - acc_err : 0.0) - D;
+  const double hi = rhi * rhi * (1.0 + 1e-12), lo = rlo > 0.0 ? rlo * rlo * (1.0 - 1e-12) : 0.0;
+  rr_fit o;
+  const double r2 = 1 - ssr / a.sst;
+  o.e0 = 1 - r2;
+  o.e1 = __builtin_sqrt(ssr / (double)n);
+  const double fv = (1 + o.e0) * (1 + o.e1);
+  const double up = (1 + hi / a.sst) * (1 + __builtin_sqrt(hi / (double)n)) / fv - 1.0;
+  const double dn = 1.0 - (1 + lo / a.sst) * (1 + __builtin_sqrt(lo / (double)n)) / fv;
+  o.err = (up > dn ? up : dn) + 1e-12;
+  o.rank = k;
+  o.flags = 0;
+  int reason = ambiguous ? NR_THRESHOLD : 0;
+  if (r2 != r2 || o.e1 != o.e1 || !(o.err < 1.0)) reason = NR_BOUND;
+  if (reason) { o.flags = RR_F_NEEDS64; o.err = __builtin_huge_val(); o.rank = -reason; }
+  a.fit[cand] = o;
 }
 
 // raw scoring of one tree in FP32 with the same certification
@@ -614,45 +583,75 @@ __global__ __launch_bounds__(64) void k_raw_fp32(Fp32Args a) {
   for (int64_t base = 0; base < a.n; base += 64 * R) {
     const int64_t row0 = base + lane;
     VE f[R];
-    eval_f32<R>(code, len, a.K, a.X32, a.ldx, row0, f);
+    uint32_t st[R];
+    eval_f32<R>(code, len, a.K, a.X32, a.ldx, row0, f, st);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int64_t i = row0 + 64 * r;
       if (i < a.n) {
-        if (f[r].e == FINF) unc = 1;
-        if (is_certain_nan(f[r].v, f[r].e)) nanc = 1;
-        const double d = a.y[i] - (double)f[r].v;
-        ssr = ssr + d * d;
-        d2 = d2 + (double)f[r].e * (double)f[r].e;
+        if (st[r] == S_NAN) nanc = 1;
+        else if (st[r] == S_UNC) unc = 1;
+        else {
+          const double d = a.y[i] - (double)f[r].v;
+          ssr = __builtin_fma(d, d, ssr);
+          d2 = __builtin_fma((double)f[r].e, (double)f[r].e, d2);
+        }
       }
     }
+    if (__any((int)nanc)) break;   // a certain NaN prediction: the fitness is the sentinel
   }
 #pragma unroll
   for (int s = 32; s >= 1; s >>= 1) { unc |= __shfl_xor(unc, s); nanc |= __shfl_xor(nanc, s); }
   ssr = wave_sum(ssr);
   d2 = wave_sum(d2);
   if (lane != 0) return;
+  if (nanc) { write_sentinel(a.fit + cand, 0); return; }
+  if (unc) { write_needs64(a.fit + cand, NR_UNCERTAIN); return; }
   rr_fit o;
   o.rank = 0;
-  if (unc) { write_needs64(a.fit + cand, NR_UNCERTAIN); return; }
-  if (nanc) { o.e0 = 1000; o.e1 = 1000; o.err = 0.0; o.flags = RR_F_SENTINEL; a.fit[cand] = o; return; }
   const double r2 = 1 - ssr / a.sst;
   o.e0 = 1 - r2;
   o.e1 = __builtin_sqrt(ssr / (double)a.n);
-  const double rs = __builtin_sqrt(ssr), D = __builtin_sqrt(d2);
-  const double hi = (rs + D) * (rs + D) * (1.0 + 1e-10), lo = rs > D ? (rs - D) * (rs - D) * (1.0 - 1e-10) : 0.0;
+  // ||y - p32|| carries its accumulation rounding; ||p32 - p64|| <= sqrt(d2)
+  const double acc = (double)(a.n / 64 + 128) * 0x1p-52;
+  const double rs = __builtin_sqrt(ssr);
+  const double D = __builtin_sqrt(d2 * (1.0 + acc)) * D_ROUND + rs * acc;
+  const double hi = (rs + D) * (rs + D) * (1.0 + 1e-12), lo = rs > D ? (rs - D) * (rs - D) * (1.0 - 1e-12) : 0.0;
   const double fv = (1 + o.e0) * (1 + o.e1);
   const double up = (1 + hi / a.sst) * (1 + __builtin_sqrt(hi / (double)a.n)) / fv - 1.0;
   const double dn = 1.0 - (1 + lo / a.sst) * (1 + __builtin_sqrt(lo / (double)a.n)) / fv;
   o.err = (up > dn ? up : dn) + 1e-12;
   o.flags = (r2 != r2 || o.e1 != o.e1 || !(o.err < 1.0)) ? RR_F_NEEDS64 : 0;
-  if (o.flags) o.err = __builtin_huge_val();
+  if (o.flags) { o.err = __builtin_huge_val(); o.rank = -NR_BOUND; }
   a.fit[cand] = o;
 }
 
+// FP32 prediction with per-sample bounds: out_e = +inf where FP32 cannot
+// vouch for the sample, out_v = NaN and out_e = 0 where the FP64 value is NaN.
+__global__ __launch_bounds__(256) void k_predict_fp32(const uint32_t* code, int len, const double* K, const float* X,
+                                                      int64_t n, int64_t ldx, float* out_v, float* out_e) {
+  const int64_t base = (int64_t)blockIdx.x * 256 + (threadIdx.x & ~63u);
+  const int64_t row0 = base + (threadIdx.x & 63u);
+  VE f[1];
+  uint32_t st[1];
+  eval_f32<1>(code, len, K, X, ldx, row0, f, st);
+  if (row0 < n) {
+    out_v[row0] = st[0] == S_NAN ? __builtin_nanf("") : f[0].v;
+    out_e[row0] = st[0] == S_NAN ? 0.f : (st[0] == S_UNC ? FINF : f[0].e);
+  }
+}
+
+// FP32 copy of a column block.  Entries FP32 cannot carry with relative
+// error 2^-24 (non-finite, beyond the FP32 range, nonzero below 2^-100) are
+// stored as NaN, which the evaluator reports as uncertain.
 __global__ void k_to_fp32(const double* src, float* dst, int64_t count) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < count) dst[i] = (float)src[i];
+  if (i < count) {
+    const double x = src[i];
+    const double ax = __builtin_fabs(x);
+    const bool ok = ax == 0.0 || (ax >= 0x1p-100 && ax <= 0x1p127);
+    dst[i] = ok ? (float)x : __builtin_nanf("");
+  }
 }
 
 __global__ void k_needs64(const int32_t* order, int count, rr_fit* fit) {
@@ -668,25 +667,46 @@ hipError_t launch_to_fp32(const double* src, float* dst, int64_t count, hipStrea
   return hipGetLastError();
 }
 
-hipError_t launch_fp32(const Fp32Launch& L, const int32_t* order, const int* bucket_count, hipStream_t s) {
+hipError_t launch_predict_fp32(const uint32_t* code, int len, const double* K, const float* X, int64_t n, int64_t ldx,
+                               float* out_v, float* out_e, hipStream_t s) {
+  // rows up to the padded ldx are readable, so whole blocks of 256 are safe
+  // as long as ldx is a multiple of 256
+  const int64_t blocks = (n + 255) / 256;
+  hipLaunchKernelGGL(k_predict_fp32, dim3((unsigned)blocks), dim3(256), 0, s, code, len, K, X, n, ldx, out_v, out_e);
+  return hipGetLastError();
+}
+
+hipError_t launch_fp32(const Fp32Launch& L, const int32_t* order, const int* bucket_count, const hipStream_t* streams) {
   Fp32Args a;
   a.cand_start = L.cand_start; a.prog_start = L.prog_start; a.code = L.code; a.K = L.K; a.X32 = L.X32;
-  a.y = L.y; a.n = L.n; a.ldx = L.ldx; a.sst = L.sst; a.ysum = L.ysum; a.coefs = L.coefs; a.fit = L.fit;
-  // order lists candidates grouped by bucket: k<=2, k<=5, k<=9, larger (no FP32 form)
-  int off = 0;
+  a.y = L.y; a.n = L.n; a.ldx = L.ldx; a.sst = L.sst; a.ysum = L.ysum; a.yy = L.yy; a.coefs = L.coefs;
+  a.fit = L.fit;
+  const bool r2 = L.rows == 2;
   if (L.raw) {
     a.order = order;
-    if (L.n_cands > 0) hipLaunchKernelGGL(k_raw_fp32<4>, dim3(L.n_cands), dim3(64), 0, s, a);
+    if (L.n_cands > 0) {
+      if (r2) hipLaunchKernelGGL(k_raw_fp32<2>, dim3(L.n_cands), dim3(64), 0, streams[0], a);
+      else hipLaunchKernelGGL(k_raw_fp32<1>, dim3(L.n_cands), dim3(64), 0, streams[0], a);
+    }
     return hipGetLastError();
   }
-  if (bucket_count[0] > 0) { a.order = order + off; hipLaunchKernelGGL((k_ols_fp32<2, 4>), dim3(bucket_count[0]), dim3(64), 0, s, a); }
-  off += bucket_count[0];
-  if (bucket_count[1] > 0) { a.order = order + off; hipLaunchKernelGGL((k_ols_fp32<5, 2>), dim3(bucket_count[1]), dim3(64), 0, s, a); }
-  off += bucket_count[1];
-  if (bucket_count[2] > 0) { a.order = order + off; hipLaunchKernelGGL((k_ols_fp32<9, 1>), dim3(bucket_count[2]), dim3(64), 0, s, a); }
-  off += bucket_count[2];
-  if (bucket_count[3] > 0)
-    hipLaunchKernelGGL(k_needs64, dim3((bucket_count[3] + 255) / 256), dim3(256), 0, s, order + off, bucket_count[3], L.fit);
+  int off[FP32_BUCKETS];
+  for (int b = 0, acc = 0; b < FP32_BUCKETS; ++b) { off[b] = acc; acc += bucket_count[b]; }
+#define RR_OLS32(B, NF)                                                                                 \
+  if (bucket_count[B] > 0) {                                                                            \
+    a.order = order + off[B];                                                                           \
+    if (r2) hipLaunchKernelGGL((k_ols_fp32<NF, 2>), dim3(bucket_count[B]), dim3(64), 0, streams[B], a); \
+    else hipLaunchKernelGGL((k_ols_fp32<NF, 1>), dim3(bucket_count[B]), dim3(64), 0, streams[B], a);   \
+  }
+  // the wide buckets first: few candidates, long per-candidate runs
+  RR_OLS32(3, 8)
+  RR_OLS32(2, 4)
+  RR_OLS32(1, 2)
+  RR_OLS32(0, 1)
+#undef RR_OLS32
+  if (bucket_count[4] > 0)
+    hipLaunchKernelGGL(k_needs64, dim3((bucket_count[4] + 255) / 256), dim3(256), 0, streams[4], order + off[4],
+                       bucket_count[4], L.fit);
   return hipGetLastError();
 }
 

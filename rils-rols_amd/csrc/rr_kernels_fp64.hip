@@ -470,16 +470,19 @@ __global__ __launch_bounds__(256) void k_predict_fp64(const uint32_t* code, int 
 }
 
 // mean(y) and sum((y - mean)^2), canonical order (R2, rils_rols_cpp.cpp:40-45)
-__global__ __launch_bounds__(64) void k_stats_fp64(const double* y, int64_t n, double* out3) {
+// out4: mean of y, sum (y - mean)^2, sum y (canonical sums), sum y^2 (any
+// order: used by the FP32 form only)
+__global__ __launch_bounds__(64) void k_stats_fp64(const double* y, int64_t n, double* out4) {
   const int lane = (int)threadIdx.x;
-  double s = 0.0;
-  for (int64_t i = lane; i < n; i += WAVE) s = s + y[i];
+  double s = 0.0, yy = 0.0;
+  for (int64_t i = lane; i < n; i += WAVE) { s = s + y[i]; yy = __builtin_fma(y[i], y[i], yy); }
   s = wave_sum(s);
+  yy = wave_sum(yy);
   const double ybar = s / (double)n;
   double q = 0.0;
   for (int64_t i = lane; i < n; i += WAVE) q = q + (y[i] - ybar) * (y[i] - ybar);
   q = wave_sum(q);
-  if (lane == 0) { out3[0] = ybar; out3[1] = q; out3[2] = s; }
+  if (lane == 0) { out4[0] = ybar; out4[1] = q; out4[2] = s; out4[3] = yy; }
 }
 
 }  // namespace
@@ -520,8 +523,8 @@ hipError_t launch_predict_fp64(const uint32_t* code, int len, const double* K, c
   return hipGetLastError();
 }
 
-hipError_t launch_stats_fp64(const double* y, int64_t n, double* out3, hipStream_t s) {
-  hipLaunchKernelGGL(k_stats_fp64, dim3(1), dim3(64), 0, s, y, n, out3);
+hipError_t launch_stats_fp64(const double* y, int64_t n, double* out4, hipStream_t s) {
+  hipLaunchKernelGGL(k_stats_fp64, dim3(1), dim3(64), 0, s, y, n, out4);
   return hipGetLastError();
 }
 

@@ -96,14 +96,21 @@ class GpuSession {
   // factors[c] are the OLS factors of candidate c; coefficients of c start at coefs[off[c]]
   void ols(const std::vector<std::vector<Flat>>& factors, size_t count, int precision, std::vector<double>& coefs,
            std::vector<int64_t>& off, std::vector<rr_fit>& out, Stats& st) {
+    ols_subset(factors, nullptr, count, precision, coefs, off, out, st);
+  }
+  // the same for the candidates idx[0..count) (all of 0..count when idx is null);
+  // results are indexed by position in idx
+  void ols_subset(const std::vector<std::vector<Flat>>& factors, const int32_t* idx, size_t count, int precision,
+                  std::vector<double>& coefs, std::vector<int64_t>& off, std::vector<rr_fit>& out, Stats& st) {
     Builder B;
     off.resize(count);
     int64_t total = 0;
-    for (size_t c = 0; c < count; ++c) {
-      for (const Flat& f : factors[c]) B.add(f);
+    for (size_t q = 0; q < count; ++q) {
+      const std::vector<Flat>& fc = factors[idx ? (size_t)idx[q] : q];
+      for (const Flat& f : fc) B.add(f);
       B.close_candidate();
-      off[c] = total;
-      total += (int64_t)factors[c].size() + 1;
+      off[q] = total;
+      total += (int64_t)fc.size() + 1;
     }
     coefs.resize(std::max<int64_t>(total, 1));
     out.resize(count);
@@ -436,6 +443,69 @@ Flat RilsRols::tune_and_fit(const Flat& cand, Fit* fit) {   // tune_constants (4
   return tree;
 }
 
+// FP32 sweeps.  A certified FP32 fitness f32 with relative bound err puts the
+// FP64 fitness value in [f32 (1 - err), f32 (1 + err)]; the OLS-tree size is
+// exact (the kernel certifies the coefficient threshold decisions).  A
+// candidate is rejected on FP32 evidence only when compare_fitness(tf, curr)
+// < 0 is impossible for every fitness in that interval; every other candidate
+// (possible acceptance, uncertified, or near the incumbent) takes its FP64
+// result, so acceptance decisions, the accepted trees and their coefficients
+// are those of the FP64 form.
+bool RilsRols::fp32_rejects(const rr_fit& r, const std::vector<int32_t>& fsz, const double* c,
+                            const Fit& curr) const {
+  if (r.flags & RR_F_NEEDS64) return false;
+  const int size = (r.flags & RR_F_SENTINEL) ? 1000 : ols_tree_size(fsz, c);
+  if ((size > max_complexity_ || curr.size > max_complexity_) && size != curr.size) return size - curr.size > 0;
+  const Fit tf = to_fit(r, size);
+  const double lo = fitness_value(tf) * (1.0 - r.err);
+  return lo >= fitness_value(curr);
+}
+
+// One FP64 batch, before the replay, for the candidates the replay is likely
+// to need: the uncertified ones and those that could beat the incumbent the
+// sweep starts from.  Its results replace the FP32 ones in place.
+void RilsRols::rescore_speculative(const std::vector<std::vector<Flat>>& facs, size_t m, const Fit& curr,
+                                   std::vector<double>& coefs, const std::vector<int64_t>& off,
+                                   std::vector<rr_fit>& res, std::vector<uint8_t>& is64) {
+  is64.assign(m, 0);
+  std::vector<int32_t> idx;
+  std::vector<int32_t> fsz;
+  for (size_t j = 0; j < m; ++j) {
+    fsz.clear();
+    for (const Flat& f : facs[j]) fsz.push_back((int32_t)f.size());
+    if (!fp32_rejects(res[j], fsz, coefs.data() + off[j], curr)) idx.push_back((int32_t)j);
+  }
+  if (idx.empty()) return;
+  std::vector<double> c64;
+  std::vector<int64_t> o64;
+  std::vector<rr_fit> r64;
+  gpu_->ols_subset(facs, idx.data(), idx.size(), RR_FP64, c64, o64, r64, stats_);
+  for (size_t q = 0; q < idx.size(); ++q) {
+    const size_t j = (size_t)idx[q];
+    std::copy(c64.begin() + o64[q], c64.begin() + o64[q] + (int64_t)facs[j].size() + 1, coefs.begin() + off[j]);
+    res[j] = r64[q];
+    is64[j] = 1;
+  }
+  stats_.rescored_fp64 += (int64_t)idx.size();
+}
+
+// Make res[j] and the coefficients at coefs[off[j]] the FP64 results of
+// candidate j, scoring it alone unless the speculative batch already did.
+void RilsRols::ensure_fp64(const std::vector<std::vector<Flat>>& facs, size_t j, std::vector<double>& coefs,
+                           const std::vector<int64_t>& off, std::vector<rr_fit>& res, std::vector<uint8_t>& is64) {
+  if (is64[j]) return;
+  std::vector<double> c64;
+  std::vector<int64_t> o64;
+  std::vector<rr_fit> r64;
+  const int32_t one = (int32_t)j;
+  gpu_->ols_subset(facs, &one, 1, RR_FP64, c64, o64, r64, stats_);
+  std::copy(c64.begin(), c64.begin() + (int64_t)facs[j].size() + 1, coefs.begin() + off[j]);
+  res[j] = r64[0];
+  is64[j] = 1;
+  stats_.rescored_fp64++;
+  stats_.single_calls++;
+}
+
 Flat RilsRols::local_search(const Flat& passed, Fit* out_fit) {   // 601-643
   std::vector<Fit> pareto;
   ls_calls_++;
@@ -447,6 +517,7 @@ Flat RilsRols::local_search(const Flat& passed, Fit* out_fit) {   // 601-643
   std::vector<int64_t> off;
   std::vector<rr_fit> res;
   std::vector<int32_t> fsz;
+  std::vector<uint8_t> is64;
   while (improved && !finished()) {
     improved = false;
     auto t0 = std::chrono::steady_clock::now();
@@ -460,9 +531,11 @@ Flat RilsRols::local_search(const Flat& passed, Fit* out_fit) {   // 601-643
     if (facs.size() < m) facs.resize(m);
     for (size_t j = 0; j < m; ++j) ols_factors(perts[j], facs[j], pool_);
     stats_.host_prep_s += secs_since(t0);
-    gpu_->ols(facs, m, precision_ == RR_FP32_REFINED ? RR_FP32_REFINED : RR_FP64, coefs, off, res, stats_);
+    const bool fp32 = precision_ == RR_FP32_REFINED;
+    gpu_->ols(facs, m, fp32 ? RR_FP32_REFINED : RR_FP64, coefs, off, res, stats_);
     stats_.sweeps++;
     stats_.batched_cands += (int64_t)m;
+    if (fp32) rescore_speculative(facs, m, curr_fit, coefs, off, res, is64);
 
     t0 = std::chrono::steady_clock::now();
     for (size_t j = 0; j < perts.size(); ++j) {
@@ -472,7 +545,13 @@ Flat RilsRols::local_search(const Flat& passed, Fit* out_fit) {   // 601-643
       fsz.clear();
       for (const Flat& f : facs[j]) fsz.push_back((int32_t)f.size());
       fit_calls_++;
-      Fit tf = to_fit(res[j], ols_tree_size(fsz, cj));
+      Fit tf;
+      if (fp32 && !is64[j]) {
+        // an FP32 result only ever rejects; anything that might be accepted is decided in FP64
+        if (fp32_rejects(res[j], fsz, cj, curr_fit)) continue;
+        ensure_fp64(facs, j, coefs, off, res, is64);
+      }
+      tf = to_fit(res[j], ols_tree_size(fsz, cj));
       if (verbose_ && fit_calls_ % 10000 == 0) print_state(curr_fit);
       if (!is_dominated(pareto, tf) && compare_fitness(tf, curr_fit) < 0) {
         improved = true;

@@ -111,6 +111,13 @@ class RilsRols {
   int compare_fitness(const Fit& a, const Fit& b) const;
   static bool dominates(const Fit& p, const Fit& f);
   static bool is_dominated(const std::vector<Fit>& pareto, const Fit& f);
+  // FP32 sweeps (search.cpp)
+  bool fp32_rejects(const rr_fit& r, const std::vector<int32_t>& fsz, const double* c, const Fit& curr) const;
+  void rescore_speculative(const std::vector<std::vector<Flat>>& facs, size_t m, const Fit& curr,
+                           std::vector<double>& coefs, const std::vector<int64_t>& off, std::vector<rr_fit>& res,
+                           std::vector<uint8_t>& is64);
+  void ensure_fp64(const std::vector<std::vector<Flat>>& facs, size_t j, std::vector<double>& coefs,
+                   const std::vector<int64_t>& off, std::vector<rr_fit>& res, std::vector<uint8_t>& is64);
   static void add_to_pareto(std::vector<Fit>& pareto, const Fit& f);
   void print_state(const Fit& curr) const;
   void log_move(int kind, const Flat& t, const Fit& f);

@@ -64,6 +64,9 @@ def lib():
                                        vp]
         L.rr_predict.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_uint32), ctypes.c_int32, dp,
                                  ctypes.c_int32, dp]
+        fp = ctypes.POINTER(ctypes.c_float)
+        L.rr_predict_fp32.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_uint32), ctypes.c_int32, dp,
+                                      ctypes.c_int32, fp, fp]
         L.rr_set_option.argtypes = [vp, ctypes.c_char_p, ctypes.c_int64]
         L.rr_get_option.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64)]
         L.rr_last_kernel_ms.argtypes = [vp, dp]
@@ -272,3 +275,14 @@ class Context:
         check(lib().rr_predict(self.p, h, code.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), len(code),
                                _dp(arrs["consts"]), arrs["n_consts"], _dp(out)))
         return out
+
+    def predict_fp32(self, h, prefix, n):
+        """FP32 values and their per-sample bounds (inf: uncertain)."""
+        arrs = build_batch([[prefix]])
+        v = np.empty(n, dtype=np.float32)
+        e = np.empty(n, dtype=np.float32)
+        code = arrs["code"]
+        fp = ctypes.POINTER(ctypes.c_float)
+        check(lib().rr_predict_fp32(self.p, h, code.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), len(code),
+                                    _dp(arrs["consts"]), arrs["n_consts"], v.ctypes.data_as(fp), e.ctypes.data_as(fp)))
+        return v, e

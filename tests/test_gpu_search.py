@@ -111,3 +111,25 @@ def test_python_api_drop_in():
         clf.fit(Xc, y)   # non-binary targets
     with pytest.raises(Exception):
         RILSROLSRegressor().predict(X)   # predict before fit
+
+
+@pytest.mark.parametrize("case", ["config1", "random"])
+def test_fp32_sweeps_accept_the_same_moves(case):
+    """Sweeps scored by the certified FP32 form (FP64 only where a decision
+    needs it) must give the run of the FP64 form: same moves, fitness bits,
+    fit calls and model."""
+    print("case: fp32 sweeps %s" % case, flush=True)
+    if case == "config1":
+        X, y = load_config1()
+        kw = dict(max_fit_calls=30000, random_state=12345)
+    else:
+        rng = np.random.default_rng(5)
+        X = rng.uniform(-3, 3, (20000, 4))
+        y = np.exp(0.3 * X[:, 0]) * X[:, 1] - np.sqrt(np.abs(X[:, 2])) + rng.normal(0, 0.01, 20000)
+        kw = dict(max_fit_calls=6000, random_state=5)
+    want = O.fit(X, y, max_time=BIG_TIME, **kw)
+    rr = run_product(X, y, precision="fp32", **kw)
+    assert_same_run(rr, want)
+    st = rr.get_stats()
+    print("fp32 sweeps: batched %d, re-scored in FP64 %d" % (st["batched_cands"], st["rescored_fp64"]), flush=True)
+    assert st["batched_cands"] > 0
