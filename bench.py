@@ -10,8 +10,20 @@ tree plus N(0, 0.01) noise.  Seeds: trees 12345, data 23654.
 One step = one rr_fitness_batch call over all 4096 candidates (program upload,
 kernels, result download), with the dataset resident in HBM.  Both forms run:
 the plain FP64 form (bit-for-bit the canonical algorithm) and the FP32
-precision form (FP32 evaluation, compensated Gram sums, FP64 solve + one
-refinement step).  `value` is the FP32 form's evals/s.
+precision form (single pass: FP32 evaluation with per-value error bounds, Gram
+sums in FP64, FP64 solve + one refinement step, certified fitness bound); an
+FP32 step also re-scores in FP64 every candidate the FP32 kernels leave
+uncertified, so each step yields a trusted fitness for every candidate.
+`value` is the FP32 form's evals/s.
+
+Roofline: algorithmic flops per candidate n (E + k^2 + 5k + 3) (E operator
+nodes, k columns with the free term) over the launch time of the dominant
+kernel, measured with HIP events around its launch: for the FP32 form the
+one-factor bucket kernel (run alone on the one-factor candidates), for the
+FP64 form the single OLS launch.  `traffic` comes from the rocprofv3 PMC
+summary under profiles/ when one is there for the kernel (FETCH_SIZE and
+WRITE_SIZE of separate runs, scaled by the same counters on a plain copy of
+known size: `python bench.py --copy-probe`).
 
 Prints one JSON line.  --gpus N with N > 1 exits: this build measures one GPU.
 """
@@ -147,17 +159,62 @@ def timed_steps(runner, rescorer, precision, warmup, steps):
     return total, np.array(lat), np.array(kms), coefs, fit, n64
 
 
-def cpu_baseline(X, y, trees, seconds_target=15.0):
-    """The oracle (CPU restatement, g++ -O3) on a bounded sample of the same
-    workload, single thread: tune_constants + fitness per candidate."""
+def cpu_baseline(X, y, trees, seconds_target=15.0, threads=1):
+    """The oracle (CPU restatement of the reference path, g++ -O3) on a bounded
+    prefix of the same candidates: tune_constants + fitness per candidate,
+    `threads` host threads over candidates."""
     import oracle_lib as O
     od = O.Dataset(X, y)
-    probe = max(4, 8)
-    secs, _ = od.batch_tune(trees[:probe], threads=1)
+    probe = 8 * threads
+    secs, _ = od.batch_tune(trees[:probe], threads=threads)
     per = secs / probe
     count = int(min(len(trees), max(probe, seconds_target / max(per, 1e-6))))
-    secs, fit = od.batch_tune(trees[:count], threads=1)
+    secs, fit = od.batch_tune(trees[:count], threads=threads)
     return count / secs, count, secs
+
+
+def host_threads():
+    """Host threads this job may use: the CPU share of the box (at most 16)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def kernel_probe(runner, idx, precision, reps=5):
+    """Average HIP-event time of one launch over the candidates idx alone."""
+    runner.select(idx)
+    runner.run_selected(precision)
+    ms = []
+    for _ in range(reps):
+        runner.run_selected(precision)
+        ms.append(runner.last_kernel_ms())
+    return float(np.mean(ms))
+
+
+def pmc_traffic(kernel_tag):
+    """HBM bytes per launch of a kernel from a committed PMC summary
+    (profiles/pmc_<tag>.json: fetch_kib and write_kib per dispatch, and the
+    copy-kernel scale factors), or None."""
+    path = os.path.join(ROOT, "profiles", "pmc_%s.json" % kernel_tag)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    return 1024.0 * (d["fetch_kib"] * d["fetch_scale"] + d["write_kib"] * d["write_scale"])
+
+
+def copy_probe():
+    """A plain device copy of known size, for scaling the PMC byte counters."""
+    import torch
+    nbytes = 1 << 30
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": nbytes, "copy_probe_bytes_written": nbytes, "launches": 5}))
 
 
 def main():
@@ -170,7 +227,11 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--fp64-rows", type=int, default=None, help="samples per lane per tile, FP64 kernels (1, 2, 4)")
     ap.add_argument("--fp32-rows", type=int, default=None, help="samples per lane per tile, FP32 kernels (1, 2)")
+    ap.add_argument("--copy-probe", action="store_true", help="run only a plain 1 GiB device copy (PMC scaling)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: collectives and multi-GPU sharding are not part of this build "
               "(see DESIGN.md for how the path would shard)", file=sys.stderr)
@@ -190,7 +251,8 @@ def main():
     t_prep = time.perf_counter() - t0
     ks = np.array(nf) + 1
     E = np.array([sum(ops_in(f) for f in fs) for fs in factors])
-    flops = float(np.sum(N_ROWS * (E + ks * ks + 5 * ks + 3)))
+    flops_c = N_ROWS * (E + ks * ks + 5 * ks + 3).astype(np.float64)
+    flops = float(np.sum(flops_c))
 
     res = {}
     for prec in (["fp64", "fp32"] if args.precision == "both" else [args.precision]):
@@ -206,31 +268,43 @@ def main():
         steps = flags >> 16
         byts = float(sum(fp64_algorithmic_bytes(int(k), int(st), bool(fl & 1), N_ROWS)
                          for k, st, fl in zip(ks, steps, flags)))
-        kernel_s = r["kms"].mean() / 1e3
+        kernel_s = r["kms"].mean() / 1e3   # the FP64 form is one launch of k_ols_fp64
+        achieved = flops / kernel_s / 1e12
         out["fp64"] = {
             "evals_per_s": N_TREES * args.steps / r["total"],
             "ms_per_step": 1e3 * r["total"] / args.steps,
             "kernel_ms": float(r["kms"].mean()),
             "p50_step_ms": float(np.percentile(r["lat"], 50) * 1e3),
             "p99_step_ms": float(np.percentile(r["lat"], 99) * 1e3),
-            "roofline": {"bound": "hbm", "achieved": byts / kernel_s / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                         "frac": byts / kernel_s / 1e9 / PEAK_HBM_GBS, "traffic": None,
-                         "algorithmic_bytes": byts},
-            "valu_tflops": flops / kernel_s / 1e12,
+            "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64_VALU_TFLOPS, "unit": "TFLOP/s",
+                         "frac": achieved / PEAK_FP64_VALU_TFLOPS, "traffic": pmc_traffic("k_ols_fp64"),
+                         "kernel": "k_ols_fp64<16,%d>" % (args.fp64_rows or 2), "launch_ms": float(r["kms"].mean()),
+                         "algorithmic_flops": flops},
+            "hbm_algorithmic_GBps": byts / kernel_s / 1e9,
+            "hbm_algorithmic_bytes": byts,
             "sentinel_frac": float(np.mean(r["fit"][:, 4].astype(int) & 1)),
         }
     if "fp32" in res:
         r = res["fp32"]
-        kernel_s = r["kms"].mean() / 1e3
+        fl32 = r["fit"][:, 4].astype(int)
+        one = np.nonzero(np.array(nf) <= 1)[0]
+        t1 = kernel_probe(runner, one, "fp32")
+        f1 = float(np.sum(flops_c[one]))
+        achieved = f1 / (t1 / 1e3) / 1e12
+        reasons = np.bincount(np.where((fl32 & 2) > 0, -r["fit"][:, 3].astype(int), 0), minlength=8).tolist()
         out["fp32"] = {
             "evals_per_s": N_TREES * args.steps / r["total"],
             "ms_per_step": 1e3 * r["total"] / args.steps,
             "kernel_ms": float(r["kms"].mean()),
             "p50_step_ms": float(np.percentile(r["lat"], 50) * 1e3),
             "p99_step_ms": float(np.percentile(r["lat"], 99) * 1e3),
-            "roofline": {"bound": "valu", "achieved": flops / kernel_s / 1e12, "peak": PEAK_FP32_TFLOPS,
-                         "unit": "TFLOP/s", "frac": flops / kernel_s / 1e12 / PEAK_FP32_TFLOPS, "traffic": None},
-            "needs_fp64_frac": float(np.mean((r["fit"][:, 4].astype(int) & 2) > 0)),
+            "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                         "frac": achieved / PEAK_FP32_TFLOPS, "traffic": pmc_traffic("k_ols_fp32_nf1"),
+                         "kernel": "k_ols_fp32<1,%d>" % (args.fp32_rows or 1), "launch_ms": t1,
+                         "candidates": int(len(one)), "algorithmic_flops": f1},
+            "batch_valu_tflops": flops / (r["kms"].mean() / 1e3) / 1e12,
+            "needs_fp64_frac": float(np.mean((fl32 & 2) > 0)),
+            "needs_fp64_reasons": reasons,
         }
         if "fp64" in res:
             a, b = res["fp64"]["fit"], r["fit"]
@@ -240,11 +314,15 @@ def main():
             out["fp32"]["r2_abs_err_median"] = float(np.median(np.abs(r2a - r2b))) if ok.any() else None
 
     cpu = None
+    cpu1 = None
     if not args.no_cpu:
-        v, count, secs = cpu_baseline(X, y, trees, args.cpu_seconds)
-        cpu = {"value": v, "unit": "fitness evals/s", "cores": 1, "kind": "port",
+        nt = host_threads()
+        v, count, secs = cpu_baseline(X, y, trees, args.cpu_seconds, threads=nt)
+        cpu = {"value": v, "unit": "fitness evals/s", "cores": nt, "kind": "port",
                "sample": "first %d of the 4096 candidates at full n=1e5, oracle tune_constants+fitness, "
-                         "single thread, %.1f s" % (count, secs)}
+                         "%d threads over candidates, %.1f s" % (count, nt, secs)}
+        v1, count1, secs1 = cpu_baseline(X, y, trees, args.cpu_seconds / 2, threads=1)
+        cpu1 = {"value": v1, "cores": 1, "sample": "first %d candidates, single thread, %.1f s" % (count1, secs1)}
 
     head = "fp32" if "fp32" in out else "fp64"
     h = out[head]
@@ -268,6 +346,13 @@ def main():
         "p99_step_ms": h["p99_step_ms"],
         "roofline": h["roofline"],
         "cpu_baseline": cpu,
+        "cpu_single_thread": cpu1,
+        "ratios": {
+            "fp32_over_fp64": (out["fp32"]["evals_per_s"] / out["fp64"]["evals_per_s"])
+            if "fp32" in out and "fp64" in out else None,
+            "over_cpu_all_cores": h["evals_per_s"] / cpu["value"] if cpu else None,
+            "over_cpu_one_core": h["evals_per_s"] / cpu1["value"] if cpu1 else None,
+        },
         "forms": out,
         "host": {"workload_gen_s": t_gen, "host_prepare_s": t_prep, "mean_k": float(ks.mean()),
                  "mean_ops": float(E.mean())},

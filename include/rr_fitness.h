@@ -108,7 +108,8 @@ typedef struct rr_batch {
 
 /* flags of rr_fit */
 #define RR_F_SENTINEL 1  /* R2 or RMSE was NaN: fitness is (1000, 1000, 1000) */
-#define RR_F_NEEDS64 2   /* FP32 result not trustworthy to its stated bound */
+#define RR_F_NEEDS64 2   /* FP32 form could not certify this candidate: fitness
+                            and coefficients (NaN) must come from RR_FP64 */
 /* FP64 OLS form: bits 16..31 of flags hold the number of Householder steps
  * that ran passes over the samples (for traffic accounting) */
 #define RR_F_STEPS(flags) (((uint32_t)(flags)) >> 16)
@@ -155,6 +156,15 @@ int rr_dataset_stats(rr_ctx* ctx, int32_t handle, double* ybar, double* sst);
  * component of the fitness tuple is the caller's).  RAW mode: coefs_out may
  * be NULL. */
 int rr_fitness_batch(rr_ctx* ctx, int32_t handle, const rr_batch* batch, int32_t mode, int32_t precision,
+                     double* coefs_out, rr_fit* fit_out);
+
+/* The same for candidates of several runs at once (continuous batching of
+ * independent searches): candidate c is scored against dataset handles[c].
+ * Results per candidate are bit-for-bit those of rr_fitness_batch on its own
+ * dataset (every candidate's sums run on one wave in a fixed order).  No
+ * counterpart in the reference, which scores one candidate at a time; it
+ * serves the request loop that keeps many regression runs resident. */
+int rr_fitness_multi(rr_ctx* ctx, const int32_t* handles, const rr_batch* batch, int32_t mode, int32_t precision,
                      double* coefs_out, rr_fit* fit_out);
 
 /* Evaluate one program over all samples of the dataset (node::evaluate_all,

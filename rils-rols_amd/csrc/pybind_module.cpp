@@ -266,7 +266,9 @@ PYBIND11_MODULE(rils_rols_cpp, m) {
              else throw std::invalid_argument("precision must be 'fp64' or 'fp32'");
            })
       .def("set_device", &RilsRols::set_device)
-      .def("set_slab_budget_mb", &RilsRols::set_slab_budget_mb);
+      .def("set_slab_budget_mb", &RilsRols::set_slab_budget_mb)
+      .def("set_host_threads", &RilsRols::set_host_threads,
+           "host threads for the per-candidate rewrites of a sweep (0: min(hardware, 8))");
 
   py::class_<BatchRunner>(m, "BatchRunner")
       .def(py::init<py::array_t<double, py::array::c_style | py::array::forcecast>,

@@ -88,7 +88,7 @@ struct rr_ctx {
   hipStream_t side[rr::FP32_BUCKETS - 1] = {};
   hipEvent_t fork = nullptr, join[rr::FP32_BUCKETS - 1] = {};
   std::vector<Dataset> ds;
-  DevBuf d_cand, d_prog, d_code, d_K, d_slab, d_slab_off, d_coefs, d_fit, d_misc, d_pred_e;
+  DevBuf d_cand, d_prog, d_code, d_K, d_slab, d_slab_off, d_coefs, d_fit, d_misc, d_pred_e, d_dstab, d_cand_ds;
   HostBuf h_stage, h_out;
   int64_t slab_budget_mb = 32768;
   int64_t fp64_rows = 2;
@@ -222,7 +222,7 @@ int rr_destroy(rr_ctx* ctx) {
     (void)hipFree(d.X); (void)hipFree(d.y); (void)hipFree(d.X32);
   }
   DevBuf* bufs[] = {&ctx->d_cand, &ctx->d_prog, &ctx->d_code, &ctx->d_K, &ctx->d_slab, &ctx->d_slab_off,
-                    &ctx->d_coefs, &ctx->d_fit, &ctx->d_misc, &ctx->d_pred_e};
+                    &ctx->d_coefs, &ctx->d_fit, &ctx->d_misc, &ctx->d_pred_e, &ctx->d_dstab, &ctx->d_cand_ds};
   for (DevBuf* b : bufs) b->release();
   ctx->h_stage.release();
   ctx->h_out.release();
@@ -296,29 +296,77 @@ int rr_dataset_stats(rr_ctx* ctx, int32_t handle, double* ybar, double* sst) {
   return RR_OK;
 }
 
-int rr_fitness_batch(rr_ctx* ctx, int32_t handle, const rr_batch* b, int32_t mode, int32_t precision,
-                     double* coefs_out, rr_fit* fit_out) {
+}  // extern "C"
+
+namespace {
+
+// One batched fitness call.  handles == nullptr: every candidate against
+// dataset `handle`; otherwise candidate c against dataset handles[c].
+int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_batch* b, int32_t mode,
+                 int32_t precision, double* coefs_out, rr_fit* fit_out) {
   int rc = set_device(ctx);
   if (rc) return rc;
-  Dataset* D = get_ds(ctx, handle);
-  if (!D) return fail(RR_E_ARG, "bad dataset handle");
   if (mode != RR_MODE_OLS && mode != RR_MODE_RAW) return fail(RR_E_ARG, "bad mode");
   if (precision != RR_FP64 && precision != RR_FP32_REFINED) return fail(RR_E_ARG, "bad precision");
-  rc = check_batch(b, mode, D->d);
-  if (rc) return rc;
+  if (!b || b->n_cands < 0) return fail(RR_E_ARG, "bad batch");
+  const int32_t nc = b->n_cands;
+
+  // the datasets of this call: table entries and per-candidate indices
+  std::vector<int32_t> uniq;          // dataset handles in table order
+  std::vector<int32_t> cand_ds;       // per candidate, index into uniq
+  int32_t min_d = 0;
+  if (!handles) {
+    Dataset* D = get_ds(ctx, handle);
+    if (!D) return fail(RR_E_ARG, "bad dataset handle");
+    uniq.push_back(handle);
+    min_d = D->d;
+  } else {
+    cand_ds.resize(nc);
+    min_d = INT32_MAX;
+    for (int32_t c = 0; c < nc; ++c) {
+      Dataset* D = get_ds(ctx, handles[c]);
+      if (!D) return fail(RR_E_ARG, "bad dataset handle for candidate " + std::to_string(c));
+      size_t u = 0;
+      while (u < uniq.size() && uniq[u] != handles[c]) ++u;   // few datasets per call
+      if (u == uniq.size()) uniq.push_back(handles[c]);
+      cand_ds[c] = (int32_t)u;
+      min_d = std::min(min_d, D->d);
+    }
+  }
+  if (handles) {
+    // structure, constants and stack discipline first, then every program
+    // against the feature count of its own candidate's dataset
+    if ((rc = check_batch(b, mode, INT32_MAX))) return rc;
+    for (int32_t c = 0; c < nc; ++c) {
+      const int32_t d = get_ds(ctx, handles[c])->d;
+      for (int32_t p = b->cand_start[c]; p < b->cand_start[c + 1]; ++p)
+        if ((rc = check_program(b->code + b->prog_start[p], b->prog_start[p + 1] - b->prog_start[p], d, b->n_consts,
+                                nullptr)))
+          return rc;
+    }
+  } else if ((rc = check_batch(b, mode, min_d))) {
+    return rc;
+  }
   if (!fit_out) return fail(RR_E_ARG, "null fit_out");
   if (mode == RR_MODE_OLS && !coefs_out) return fail(RR_E_ARG, "null coefs_out");
-  const int32_t nc = b->n_cands;
   if (nc == 0) { ctx->last_ms = 0.0; return RR_OK; }
   const int32_t np = b->cand_start[nc];
   const int32_t ncode = b->prog_start[np];
   const int64_t ncoef = (int64_t)np + nc;
 
+  std::vector<rr::DsDesc> table(uniq.size());
+  for (size_t u = 0; u < uniq.size(); ++u) {
+    const Dataset& D = ctx->ds[uniq[u]];
+    table[u] = rr::DsDesc{D.X, D.X32, D.y, D.n, D.ldx, D.sst, D.ysum, D.yy};
+  }
+  auto ds_of = [&](int32_t c) -> const Dataset& { return ctx->ds[uniq[handles ? cand_ds[c] : 0]]; };
+
   // stage inputs
   const size_t s_cand = align_up((size_t)(nc + 1) * 4), s_prog = align_up((size_t)(np + 1) * 4);
   const size_t s_code = align_up((size_t)ncode * 4), s_K = align_up((size_t)std::max(b->n_consts, 1) * 8);
   const size_t s_off = align_up((size_t)nc * 8);
-  if (ctx->h_stage.ensure(s_cand + s_prog + s_code + s_K + s_off) != RR_OK) return RR_E_NOMEM;
+  const size_t s_tab = align_up(table.size() * sizeof(rr::DsDesc)), s_cds = align_up((size_t)nc * 4);
+  if (ctx->h_stage.ensure(s_cand + s_prog + s_code + s_K + s_off + s_tab + s_cds) != RR_OK) return RR_E_NOMEM;
   size_t so = 0;
   if ((rc = upload(ctx, ctx->d_cand, b->cand_start, (size_t)(nc + 1) * 4, so))) return rc;
   so += s_cand;
@@ -330,6 +378,14 @@ int rr_fitness_batch(rr_ctx* ctx, int32_t handle, const rr_batch* b, int32_t mod
   if (b->n_consts > 0) { if ((rc = upload(ctx, ctx->d_K, b->consts, (size_t)b->n_consts * 8, so))) return rc; }
   else if (ctx->d_K.ensure(8) != RR_OK) return RR_E_NOMEM;
   so += s_K;
+  if ((rc = upload(ctx, ctx->d_dstab, table.data(), table.size() * sizeof(rr::DsDesc), so))) return rc;
+  so += s_tab;
+  const int32_t* dcds = nullptr;
+  if (handles) {
+    if ((rc = upload(ctx, ctx->d_cand_ds, cand_ds.data(), (size_t)nc * 4, so))) return rc;
+    dcds = (const int32_t*)ctx->d_cand_ds.p;
+  }
+  so += s_cds;
   if (ctx->d_fit.ensure((size_t)nc * sizeof(rr_fit)) != RR_OK) return RR_E_NOMEM;
   if (ctx->d_coefs.ensure((size_t)std::max<int64_t>(ncoef, 1) * 8) != RR_OK) return RR_E_NOMEM;
 
@@ -337,22 +393,22 @@ int rr_fitness_batch(rr_ctx* ctx, int32_t handle, const rr_batch* b, int32_t mod
   const int32_t* dp = (const int32_t*)ctx->d_prog.p;
   const uint32_t* dcode = (const uint32_t*)ctx->d_code.p;
   const double* dK = (const double*)ctx->d_K.p;
+  const rr::DsDesc* dtab = (const rr::DsDesc*)ctx->d_dstab.p;
   rr_fit* dfit = (rr_fit*)ctx->d_fit.p;
   double* dcoef = (double*)ctx->d_coefs.p;
 
   if (mode == RR_MODE_RAW) {
     RR_HIP(hipEventRecord(ctx->ev0, ctx->stream));
     if (precision == RR_FP64) {
-      rr::RawLaunch L{dc, dp, dcode, dK, D->X, D->y, D->n, D->ldx, D->sst, dfit, nc, (int)ctx->fp64_rows};
+      rr::RawLaunch L{dc, dp, dcode, dK, dtab, dcds, dfit, nc, (int)ctx->fp64_rows};
       RR_HIP(rr::launch_raw_fp64(L, ctx->stream));
     } else {
       std::vector<int32_t> order(nc);
       for (int32_t c = 0; c < nc; ++c) order[c] = c;
       if ((rc = upload(ctx, ctx->d_slab_off, order.data(), (size_t)nc * 4, so))) return rc;
       rr::Fp32Launch L{};
-      L.cand_start = dc; L.prog_start = dp; L.code = dcode; L.K = dK; L.X32 = D->X32; L.y = D->y; L.n = D->n;
-      L.ldx = D->ldx; L.sst = D->sst; L.ysum = D->ysum; L.yy = D->yy; L.coefs = dcoef; L.fit = dfit;
-      L.n_cands = nc; L.raw = 1; L.rows = (int)ctx->fp32_rows;
+      L.cand_start = dc; L.prog_start = dp; L.code = dcode; L.K = dK; L.ds = dtab; L.cand_ds = dcds;
+      L.coefs = dcoef; L.fit = dfit; L.n_cands = nc; L.raw = 1; L.rows = (int)ctx->fp32_rows;
       const int counts[rr::FP32_BUCKETS] = {nc, 0, 0, 0, 0};
       const hipStream_t st[rr::FP32_BUCKETS] = {ctx->stream, ctx->stream, ctx->stream, ctx->stream, ctx->stream};
       RR_HIP(rr::launch_fp32(L, (const int32_t*)ctx->d_slab_off.p, counts, st));
@@ -370,7 +426,7 @@ int rr_fitness_batch(rr_ctx* ctx, int32_t handle, const rr_batch* b, int32_t mod
       int c1 = c, maxk = 0;
       while (c1 < nc) {
         const int k = b->cand_start[c1 + 1] - b->cand_start[c1] + 1;
-        const int64_t need = (int64_t)(2 * k) * D->ldx;   // A, Q^T y, kept factor values
+        const int64_t need = (int64_t)(2 * k) * ds_of(c1).ldx;   // A, Q^T y, kept factor values
         if (c1 > c && (used + need) * 8 > budget) break;
         off[c1] = used;
         used += need;
@@ -386,15 +442,15 @@ int rr_fitness_batch(rr_ctx* ctx, int32_t handle, const rr_batch* b, int32_t mod
     RR_HIP(hipEventRecord(ctx->ev0, ctx->stream));
     for (const Chunk& ch : chunks) {
       rr::OlsLaunch L;
-      L.cand_start = dc; L.prog_start = dp; L.code = dcode; L.K = dK; L.X = D->X; L.y = D->y;
-      L.n = D->n; L.ldx = D->ldx; L.sst = D->sst; L.slab = (double*)ctx->d_slab.p;
+      L.cand_start = dc; L.prog_start = dp; L.code = dcode; L.K = dK; L.ds = dtab; L.cand_ds = dcds;
+      L.slab = (double*)ctx->d_slab.p;
       L.slab_off = (const int64_t*)ctx->d_slab_off.p + ch.c0; L.coefs = dcoef; L.fit = dfit;
       L.cand_base = ch.c0; L.n_cands = ch.c1 - ch.c0; L.max_cols = ch.maxk; L.rows = (int)ctx->fp64_rows;
       RR_HIP(rr::launch_ols_fp64(L, ctx->stream));
     }
     RR_HIP(hipEventRecord(ctx->ev1, ctx->stream));
   } else {
-    // group candidates by column count so each kernel instance keeps only
+    // group candidates by factor count so each kernel instance keeps only
     // the accumulators its bucket needs
     std::vector<int32_t> order(nc);
     int counts[rr::FP32_BUCKETS] = {0, 0, 0, 0, 0};
@@ -411,9 +467,8 @@ int rr_fitness_batch(rr_ctx* ctx, int32_t handle, const rr_batch* b, int32_t mod
     for (int32_t c = 0; c < nc; ++c) order[start[bucket[c]]++] = c;
     if ((rc = upload(ctx, ctx->d_slab_off, order.data(), (size_t)nc * 4, so))) return rc;
     rr::Fp32Launch L{};
-    L.cand_start = dc; L.prog_start = dp; L.code = dcode; L.K = dK; L.X32 = D->X32; L.y = D->y; L.n = D->n;
-    L.ldx = D->ldx; L.sst = D->sst; L.ysum = D->ysum; L.yy = D->yy; L.coefs = dcoef; L.fit = dfit;
-    L.n_cands = nc; L.raw = 0; L.rows = (int)ctx->fp32_rows;
+    L.cand_start = dc; L.prog_start = dp; L.code = dcode; L.K = dK; L.ds = dtab; L.cand_ds = dcds;
+    L.coefs = dcoef; L.fit = dfit; L.n_cands = nc; L.raw = 0; L.rows = (int)ctx->fp32_rows;
     // bucket 0 (the bulk) on the main stream, the others on side streams
     hipStream_t st[rr::FP32_BUCKETS];
     st[0] = ctx->stream;
@@ -437,6 +492,27 @@ int rr_fitness_batch(rr_ctx* ctx, int32_t handle, const rr_batch* b, int32_t mod
   RR_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
   ctx->last_ms = ms;
   return RR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rr_fitness_batch(rr_ctx* ctx, int32_t handle, const rr_batch* b, int32_t mode, int32_t precision,
+                     double* coefs_out, rr_fit* fit_out) {
+  return fitness_impl(ctx, handle, nullptr, b, mode, precision, coefs_out, fit_out);
+}
+
+int rr_fitness_multi(rr_ctx* ctx, const int32_t* handles, const rr_batch* b, int32_t mode, int32_t precision,
+                     double* coefs_out, rr_fit* fit_out) {
+  if (b && b->n_cands > 0 && !handles) return fail(RR_E_ARG, "null dataset handles");
+  if (!b || b->n_cands == 0) {
+    if (!ctx) return fail(RR_E_ARG, "null context");
+    if (!b) return fail(RR_E_ARG, "bad batch");
+    ctx->last_ms = 0.0;
+    return RR_OK;
+  }
+  return fitness_impl(ctx, -1, handles, b, mode, precision, coefs_out, fit_out);
 }
 
 int rr_predict(rr_ctx* ctx, int32_t handle, const uint32_t* code, int32_t len, const double* consts, int32_t n_consts,

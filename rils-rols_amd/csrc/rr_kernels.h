@@ -8,15 +8,25 @@
 
 namespace rr {
 
+// One dataset as the kernels see it.  A launch carries a table of these and,
+// per candidate, the index of the one it is scored against (cand_ds; null
+// means entry 0 for every candidate), so candidates of independent runs over
+// different datasets can share one launch.
+struct DsDesc {
+  const double* X;    // feature-major FP64, column stride ldx
+  const float* X32;   // FP32 copy, same layout
+  const double* y;
+  int64_t n, ldx;
+  double sst, ysum, yy;   // sum (y - mean)^2 (canonical order), sum y, sum y^2
+};
+
 struct OlsLaunch {
   const int32_t* cand_start;  // device
   const int32_t* prog_start;
   const uint32_t* code;
   const double* K;
-  const double* X;            // feature-major, column stride ldx
-  const double* y;
-  int64_t n, ldx;
-  double sst;
+  const DsDesc* ds;           // device table of datasets
+  const int32_t* cand_ds;     // per candidate (batch index), or null
   double* slab;               // work columns
   const int64_t* slab_off;    // per launched candidate, in doubles
   double* coefs;
@@ -32,10 +42,8 @@ struct RawLaunch {
   const int32_t* prog_start;
   const uint32_t* code;
   const double* K;
-  const double* X;
-  const double* y;
-  int64_t n, ldx;
-  double sst;
+  const DsDesc* ds;
+  const int32_t* cand_ds;
   rr_fit* fit;
   int n_cands;
   int rows;

@@ -4,6 +4,7 @@
 #include <stdint.h>
 
 #include "../../include/rr_fitness.h"
+#include "rr_kernels.h"
 
 namespace rr {
 
@@ -12,10 +13,8 @@ struct Fp32Launch {
   const int32_t* prog_start;
   const uint32_t* code;
   const double* K;
-  const float* X32;   // FP32 copy of X, same layout
-  const double* y;
-  int64_t n, ldx;
-  double sst, ysum, yy;   // sum (y - ybar)^2, sum y, sum y^2
+  const DsDesc* ds;         // device table of datasets
+  const int32_t* cand_ds;   // per candidate (batch index), or null
   double* coefs;
   rr_fit* fit;
   int n_cands;

@@ -290,6 +290,9 @@ struct Fp32Args {
   const int32_t* prog_start;
   const uint32_t* code;
   const double* K;
+  const DsDesc* ds;
+  const int32_t* cand_ds;
+  // the candidate's dataset, bound at kernel start
   const float* X32;
   const double* y;
   int64_t n, ldx;
@@ -299,6 +302,11 @@ struct Fp32Args {
   const int32_t* order;   // candidates of this launch (indices into the batch)
 };
 
+__device__ inline void bind_dataset(Fp32Args& a, int cand) {
+  const DsDesc& D = a.ds[a.cand_ds ? a.cand_ds[cand] : 0];
+  a.X32 = D.X32; a.y = D.y; a.n = D.n; a.ldx = D.ldx; a.sst = D.sst; a.ysum = D.ysum; a.yy = D.yy;
+}
+
 // reasons a candidate is left to the FP64 form (in rr_fit.rank, negated)
 enum { NR_UNCERTAIN = 1, NR_SHORT = 2, NR_NAN_COND = 3, NR_COND = 4, NR_THRESHOLD = 5, NR_BOUND = 6, NR_WIDE = 7 };
 
@@ -306,6 +314,12 @@ __device__ inline void write_needs64(rr_fit* f, int reason) {
   rr_fit o;
   o.e0 = 0; o.e1 = 0; o.err = __builtin_huge_val(); o.rank = -reason; o.flags = RR_F_NEEDS64;
   *f = o;
+}
+
+// the same with the candidate's k coefficients set to NaN (one lane)
+__device__ inline void write_needs64(rr_fit* f, double* co, int k, int reason) {
+  for (int i = 0; i < k; ++i) co[i] = rrm::qnan();
+  write_needs64(f, reason);
 }
 
 __device__ inline void write_sentinel(rr_fit* f, int rank) {
@@ -381,13 +395,14 @@ __global__ __launch_bounds__(64) void k_ols_fp32(Fp32Args a) {
   constexpr int NG = NF * (NF + 1) / 2;
   __shared__ Solve<KB> S;
   const int cand = a.order[blockIdx.x];
+  bind_dataset(a, cand);
   const int lane = (int)threadIdx.x;
   const int pbeg = a.cand_start[cand];
   const int nf = a.cand_start[cand + 1] - pbeg;
   const int k = nf + 1;
   const int64_t n = a.n, ldx = a.ldx;
   double* const co = a.coefs + (int64_t)pbeg + cand;
-  if (n < k) { if (lane == 0) write_needs64(a.fit + cand, NR_SHORT); return; }
+  if (n < k) { if (lane == 0) write_needs64(a.fit + cand, co, k, NR_SHORT); return; }
 
   double G[NG], S1[NF], SY[NF], BD[NF];
 #pragma unroll
@@ -449,7 +464,7 @@ __global__ __launch_bounds__(64) void k_ols_fp32(Fp32Args a) {
   }
 #pragma unroll
   for (int s = 32; s >= 1; s >>= 1) { unc |= __shfl_xor(unc, s); nanc |= __shfl_xor(nanc, s); }
-  if (unc) { if (lane == 0) write_needs64(a.fit + cand, NR_UNCERTAIN); return; }
+  if (unc) { if (lane == 0) write_needs64(a.fit + cand, co, k, NR_UNCERTAIN); return; }
 #pragma unroll
   for (int j = 0; j < NG; ++j) G[j] = wave_sum(G[j]);
 #pragma unroll
@@ -485,12 +500,12 @@ __global__ __launch_bounds__(64) void k_ols_fp32(Fp32Args a) {
       for (int i = 0; i < m; ++i) for (int j = 0; j < m; ++j) T[i][j] = S.M[idx[i]][idx[j]];
       certain = chol<KB>(S, T, m);
     }
-    if (!certain) { write_needs64(a.fit + cand, NR_NAN_COND); return; }
+    if (!certain) { write_needs64(a.fit + cand, co, k, NR_NAN_COND); return; }
     for (int i = 0; i < k; ++i) co[i] = rrm::qnan();
     write_sentinel(a.fit + cand, k);
     return;
   }
-  if (!chol<KB>(S, S.M, k)) { write_needs64(a.fit + cand, NR_COND); return; }
+  if (!chol<KB>(S, S.M, k)) { write_needs64(a.fit + cand, co, k, NR_COND); return; }
 
   // solve, one refinement step on the normal equations
   double c[KB], r1[KB], dcr[KB];
@@ -549,7 +564,6 @@ __global__ __launch_bounds__(64) void k_ols_fp32(Fp32Args a) {
     if (i != nf && !(a1 - dc >= EPS12 || a1 + dc < EPS12)) ambiguous = true;
   }
   D *= D_ROUND;
-  for (int i = 0; i < k; ++i) co[i] = c[i];
 
   const double rhi = __builtin_sqrt(ssr + acc_err) + D;
   const double rlo = __builtin_sqrt(ssr > acc_err ? ssr - acc_err : 0.0) - D;
@@ -566,7 +580,8 @@ __global__ __launch_bounds__(64) void k_ols_fp32(Fp32Args a) {
   o.flags = 0;
   int reason = ambiguous ? NR_THRESHOLD : 0;
   if (r2 != r2 || o.e1 != o.e1 || !(o.err < 1.0)) reason = NR_BOUND;
-  if (reason) { o.flags = RR_F_NEEDS64; o.err = __builtin_huge_val(); o.rank = -reason; }
+  if (reason) { write_needs64(a.fit + cand, co, k, reason); return; }
+  for (int i = 0; i < k; ++i) co[i] = c[i];
   a.fit[cand] = o;
 }
 
@@ -574,6 +589,7 @@ __global__ __launch_bounds__(64) void k_ols_fp32(Fp32Args a) {
 template <int R>
 __global__ __launch_bounds__(64) void k_raw_fp32(Fp32Args a) {
   const int cand = a.order[blockIdx.x];
+  bind_dataset(a, cand);
   const int lane = (int)threadIdx.x;
   const int p = a.cand_start[cand];
   const uint32_t* code = a.code + a.prog_start[p];
@@ -654,9 +670,13 @@ __global__ void k_to_fp32(const double* src, float* dst, int64_t count) {
   }
 }
 
-__global__ void k_needs64(const int32_t* order, int count, rr_fit* fit) {
+__global__ void k_needs64(const int32_t* order, int count, const int32_t* cand_start, double* coefs, rr_fit* fit) {
   const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  if (i < count) write_needs64(fit + order[i], NR_WIDE);
+  if (i < count) {
+    const int c = order[i];
+    const int p = cand_start[c];
+    write_needs64(fit + c, coefs + p + c, cand_start[c + 1] - p + 1, NR_WIDE);
+  }
 }
 
 }  // namespace
@@ -678,9 +698,8 @@ hipError_t launch_predict_fp32(const uint32_t* code, int len, const double* K, c
 
 hipError_t launch_fp32(const Fp32Launch& L, const int32_t* order, const int* bucket_count, const hipStream_t* streams) {
   Fp32Args a;
-  a.cand_start = L.cand_start; a.prog_start = L.prog_start; a.code = L.code; a.K = L.K; a.X32 = L.X32;
-  a.y = L.y; a.n = L.n; a.ldx = L.ldx; a.sst = L.sst; a.ysum = L.ysum; a.yy = L.yy; a.coefs = L.coefs;
-  a.fit = L.fit;
+  a.cand_start = L.cand_start; a.prog_start = L.prog_start; a.code = L.code; a.K = L.K;
+  a.ds = L.ds; a.cand_ds = L.cand_ds; a.coefs = L.coefs; a.fit = L.fit;
   const bool r2 = L.rows == 2;
   if (L.raw) {
     a.order = order;
@@ -706,7 +725,7 @@ hipError_t launch_fp32(const Fp32Launch& L, const int32_t* order, const int* buc
 #undef RR_OLS32
   if (bucket_count[4] > 0)
     hipLaunchKernelGGL(k_needs64, dim3((bucket_count[4] + 255) / 256), dim3(256), 0, streams[4], order + off[4],
-                       bucket_count[4], L.fit);
+                       bucket_count[4], L.cand_start, L.coefs, L.fit);
   return hipGetLastError();
 }
 

@@ -45,6 +45,9 @@ struct Fp64Args {
   const int32_t* prog_start;
   const uint32_t* code;
   const double* K;
+  const DsDesc* ds;
+  const int32_t* cand_ds;
+  // the candidate's dataset, bound at kernel start
   const double* X;
   const double* y;
   int64_t n, ldx;
@@ -71,6 +74,13 @@ __device__ inline void write_fit(rr_fit* f, double ssr, double sst, int64_t n, i
   *f = o;
 }
 
+// Bind the dataset candidate `cand` is scored against (uniform: scalar loads).
+template <class A>
+__device__ inline void bind_dataset(A& a, int cand) {
+  const DsDesc& D = a.ds[a.cand_ds ? a.cand_ds[cand] : 0];
+  a.X = D.X; a.y = D.y; a.n = D.n; a.ldx = D.ldx; a.sst = D.sst;
+}
+
 __device__ inline int64_t tile_start(int64_t lo, int R) { return (lo / (64 * R)) * (64 * R); }
 
 template <int MAXK, int R>
@@ -80,6 +90,7 @@ __global__ __launch_bounds__(64) void k_ols_fp64(Fp64Args a) {
   __shared__ int phys[MAXK];
 
   const int cand = a.cand_base + (int)blockIdx.x;
+  bind_dataset(a, cand);
   const int lane = (int)threadIdx.x;
   const int pbeg = a.cand_start[cand];
   const int nf = a.cand_start[cand + 1] - pbeg;
@@ -426,6 +437,8 @@ struct RawArgs {
   const int32_t* prog_start;
   const uint32_t* code;
   const double* K;
+  const DsDesc* ds;
+  const int32_t* cand_ds;
   const double* X;
   const double* y;
   int64_t n, ldx;
@@ -438,6 +451,7 @@ struct RawArgs {
 template <int R>
 __global__ __launch_bounds__(64) void k_raw_fp64(RawArgs a) {
   const int cand = (int)blockIdx.x;
+  bind_dataset(a, cand);
   const int lane = (int)threadIdx.x;
   const int p = a.cand_start[cand];
   const uint32_t* code = a.code + a.prog_start[p];
@@ -490,7 +504,7 @@ __global__ __launch_bounds__(64) void k_stats_fp64(const double* y, int64_t n, d
 hipError_t launch_ols_fp64(const OlsLaunch& L, hipStream_t s) {
   Fp64Args a;
   a.cand_start = L.cand_start; a.prog_start = L.prog_start; a.code = L.code; a.K = L.K;
-  a.X = L.X; a.y = L.y; a.n = L.n; a.ldx = L.ldx; a.sst = L.sst;
+  a.ds = L.ds; a.cand_ds = L.cand_ds;
   a.slab = L.slab; a.slab_off = L.slab_off; a.coefs = L.coefs; a.fit = L.fit; a.cand_base = L.cand_base;
   const dim3 grid(L.n_cands), block(64);
   const int R = L.rows;
@@ -509,7 +523,7 @@ hipError_t launch_ols_fp64(const OlsLaunch& L, hipStream_t s) {
 hipError_t launch_raw_fp64(const RawLaunch& L, hipStream_t s) {
   RawArgs a;
   a.cand_start = L.cand_start; a.prog_start = L.prog_start; a.code = L.code; a.K = L.K;
-  a.X = L.X; a.y = L.y; a.n = L.n; a.ldx = L.ldx; a.sst = L.sst; a.fit = L.fit;
+  a.ds = L.ds; a.cand_ds = L.cand_ds; a.fit = L.fit;
   if (L.rows == 1) hipLaunchKernelGGL(k_raw_fp64<1>, dim3(L.n_cands), dim3(64), 0, s, a);
   else if (L.rows == 2) hipLaunchKernelGGL(k_raw_fp64<2>, dim3(L.n_cands), dim3(64), 0, s, a);
   else hipLaunchKernelGGL(k_raw_fp64<4>, dim3(L.n_cands), dim3(64), 0, s, a);

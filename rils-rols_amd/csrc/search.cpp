@@ -30,30 +30,56 @@ static void rr_check(int rc, const char* what) {
   if (rc != RR_OK) throw std::runtime_error(std::string(what) + ": " + rr_last_error());
 }
 
-class GpuSession {
+class DirectBackend : public FitnessBackend {
  public:
   rr_ctx* ctx = nullptr;
+  DirectBackend(int device, int64_t slab_mb) {
+    rr_check(rr_create(device, &ctx), "rr_create (the MI355X fitness library needs a HIP device)");
+    if (slab_mb > 0) rr_check(rr_set_option(ctx, "slab_budget_mb", slab_mb), "rr_set_option");
+  }
+  ~DirectBackend() override { if (ctx) rr_destroy(ctx); }
+  int32_t upload(const double* X_fm, const double* y, int64_t n, int32_t d) override {
+    int32_t h = -1;
+    rr_check(rr_dataset_upload(ctx, X_fm, y, n, d, &h), "rr_dataset_upload");
+    return h;
+  }
+  void release(int32_t h) override { rr_dataset_free(ctx, h); }
+  void fitness(int32_t h, const rr_batch& b, int32_t mode, int32_t precision, double* coefs, rr_fit* fit,
+               double* kernel_ms) override {
+    rr_check(rr_fitness_batch(ctx, h, &b, mode, precision, coefs, fit), "rr_fitness_batch");
+    rr_last_kernel_ms(ctx, kernel_ms);
+  }
+  rr_ctx* context() override { return ctx; }
+};
+
+class GpuSession {
+ public:
+  FitnessBackend* be = nullptr;
+  bool own = false;
   int32_t ds = -1;
   int64_t n = 0;
   int d = 0;
 
-  GpuSession(int device, int64_t slab_mb) {
-    rr_check(rr_create(device, &ctx), "rr_create (the MI355X fitness library needs a HIP device)");
-    if (slab_mb > 0) rr_check(rr_set_option(ctx, "slab_budget_mb", slab_mb), "rr_set_option");
+  GpuSession(FitnessBackend* backend, int device, int64_t slab_mb) {
+    if (backend) {
+      be = backend;
+    } else {
+      be = new DirectBackend(device, slab_mb);
+      own = true;
+    }
   }
   ~GpuSession() {
-    if (ctx) {
-      if (ds >= 0) rr_dataset_free(ctx, ds);
-      rr_destroy(ctx);
-    }
+    if (ds >= 0) be->release(ds);
+    if (own) delete be;
   }
   void upload(const std::vector<std::vector<double>>& X, const std::vector<double>& y) {
     n = (int64_t)y.size();
     d = (int)X.size();
     std::vector<double> fm((size_t)n * std::max(d, 1));
     for (int j = 0; j < d; ++j) std::copy(X[j].begin(), X[j].end(), fm.begin() + (size_t)j * n);
-    if (ds >= 0) rr_dataset_free(ctx, ds);
-    rr_check(rr_dataset_upload(ctx, fm.data(), y.data(), n, d, &ds), "rr_dataset_upload");
+    if (ds >= 0) be->release(ds);
+    ds = -1;
+    ds = be->upload(fm.data(), y.data(), n, d);
   }
 
   struct Builder {
@@ -84,10 +110,9 @@ class GpuSession {
     if (trees.empty()) return;
     const rr_batch b = B.view();
     const auto t0 = std::chrono::steady_clock::now();
-    rr_check(rr_fitness_batch(ctx, ds, &b, RR_MODE_RAW, RR_FP64, nullptr, out.data()), "rr_fitness_batch(raw)");
-    st.gpu_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     double ms = 0;
-    rr_last_kernel_ms(ctx, &ms);
+    be->fitness(ds, b, RR_MODE_RAW, RR_FP64, nullptr, out.data(), &ms);
+    st.gpu_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     st.kernel_ms += ms;
     st.raw_batches++;
     st.raw_cands += (int64_t)trees.size();
@@ -117,10 +142,9 @@ class GpuSession {
     if (count == 0) return;
     const rr_batch b = B.view();
     const auto t0 = std::chrono::steady_clock::now();
-    rr_check(rr_fitness_batch(ctx, ds, &b, RR_MODE_OLS, precision, coefs.data(), out.data()), "rr_fitness_batch(ols)");
-    st.gpu_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     double ms = 0;
-    rr_last_kernel_ms(ctx, &ms);
+    be->fitness(ds, b, RR_MODE_OLS, precision, coefs.data(), out.data(), &ms);
+    st.gpu_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     st.kernel_ms += ms;
   }
 };
@@ -138,7 +162,29 @@ RilsRols::RilsRols(bool classification, int max_fit_calls, int max_seconds, doub
   reset();
 }
 
-RilsRols::~RilsRols() { delete gpu_; }
+RilsRols::~RilsRols() {
+  delete gpu_;
+  delete workers_;
+}
+
+void RilsRols::set_host_threads(int t) {
+  delete workers_;
+  workers_ = nullptr;
+  host_threads_ = t < 0 ? 0 : t;
+}
+
+Workers& RilsRols::workers() {
+  if (!workers_) {
+    int t = host_threads_;
+    if (t == 0) {
+      const unsigned hw = std::thread::hardware_concurrency();
+      t = (int)std::min<unsigned>(hw ? hw : 1, 8);
+    }
+    workers_ = new Workers(t);
+    pools_.assign((size_t)workers_->size(), Pool());
+  }
+  return *workers_;
+}
 
 void RilsRols::reset() {   // rils_rols_cpp.cpp:106-117
   main_it_ = 0;
@@ -402,13 +448,17 @@ std::vector<Flat> RilsRols::all_candidates(const Flat& sol, bool local_search) {
     if (a >= 1) { site(p + 1); q.push_back(p + 1); }
     if (a >= 2) { const int r = p + 1 + sz[p + 1]; site(r); q.push_back(r); }
   }
-  for (Flat& f : all) rewrite_rounds(f, true, pool_);
+  std::vector<std::string> strs(all.size());
+  workers().parallel_for(all.size(), [&](size_t i, int tid) {
+    rewrite_rounds(all[i], true, pools_[(size_t)tid]);
+    strs[i] = to_string(all[i]);
+  });
   std::unordered_set<std::string> seen2;
   seen2.reserve(all.size() * 2);
   std::vector<Flat> out;
   out.reserve(all.size());
-  for (Flat& f : all) {
-    if (seen2.insert(to_string(f)).second) out.push_back(std::move(f));
+  for (size_t i = 0; i < all.size(); ++i) {
+    if (seen2.insert(std::move(strs[i])).second) out.push_back(std::move(all[i]));
   }
   return out;
 }
@@ -520,7 +570,8 @@ Flat RilsRols::local_search(const Flat& passed, Fit* out_fit) {   // 601-643
   std::vector<uint8_t> is64;
   while (improved && !finished()) {
     improved = false;
-    auto t0 = std::chrono::steady_clock::now();
+    const auto sweep0 = std::chrono::steady_clock::now();
+    auto t0 = sweep0;
     std::vector<Flat> perts = all_candidates(curr, true);
     stats_.host_gen_s += secs_since(t0);
 
@@ -529,7 +580,7 @@ Flat RilsRols::local_search(const Flat& passed, Fit* out_fit) {   // 601-643
     const size_t m = std::min<size_t>(perts.size(), (size_t)room);
     t0 = std::chrono::steady_clock::now();
     if (facs.size() < m) facs.resize(m);
-    for (size_t j = 0; j < m; ++j) ols_factors(perts[j], facs[j], pool_);
+    workers().parallel_for(m, [&](size_t j, int tid) { ols_factors(perts[j], facs[j], pools_[(size_t)tid]); });
     stats_.host_prep_s += secs_since(t0);
     const bool fp32 = precision_ == RR_FP32_REFINED;
     gpu_->ols(facs, m, fp32 ? RR_FP32_REFINED : RR_FP64, coefs, off, res, stats_);
@@ -566,6 +617,7 @@ Flat RilsRols::local_search(const Flat& passed, Fit* out_fit) {   // 601-643
       }
     }
     stats_.replay_s += secs_since(t0);
+    sweep_ms_.push_back((float)(1e3 * secs_since(sweep0)));
   }
   *out_fit = curr_fit;
   return curr;
@@ -614,6 +666,7 @@ void RilsRols::fit_inner(const std::vector<std::vector<double>>& X_all, const st
   reset();
   moves_.clear();
   stats_ = Stats();
+  sweep_ms_.clear();
   const int64_t N = (int64_t)y_all.size();
   const int sample_cnt = int(sample_size_ * (double)N);
   std::vector<int> selected(N);
@@ -629,7 +682,7 @@ void RilsRols::fit_inner(const std::vector<std::vector<double>>& X_all, const st
   if (sample_cnt <= 0) throw std::invalid_argument("sample_size * n must be at least one sample");
   std::vector<int> rel = relevant_features(X, y);
   setup_nodes(rel);
-  if (!gpu_) gpu_ = new GpuSession(device_, slab_budget_mb_);
+  if (!gpu_) gpu_ = new GpuSession(backend_, device_, slab_budget_mb_);
   gpu_->upload(X, y);
 
   final_solution_ = {tok_const(0.0)};
@@ -712,19 +765,21 @@ std::vector<double> RilsRols::predict(const double* Xp, int64_t x_size, int data
                                 " is not the same as the product of data count and feature count " +
                                 std::to_string((int64_t)data_cnt * feat_cnt));
   if (final_solution_.empty()) throw std::runtime_error("predict called before fit");
-  if (!gpu_) gpu_ = new GpuSession(device_, slab_budget_mb_);
+  if (!gpu_) gpu_ = new GpuSession(backend_, device_, slab_budget_mb_);
+  rr_ctx* ctx = gpu_->be->context();
+  if (!ctx) throw std::runtime_error("predict needs a fitness backend with a device context");
   std::vector<double> fm((size_t)data_cnt * std::max(feat_cnt, 1));
   for (int i = 0; i < data_cnt; ++i)
     for (int j = 0; j < feat_cnt; ++j) fm[(size_t)j * data_cnt + i] = Xp[(int64_t)i * feat_cnt + j];
   std::vector<double> zeros(data_cnt, 0.0), out(data_cnt);
   int32_t h = -1;
-  rr_check(rr_dataset_upload(gpu_->ctx, fm.data(), zeros.data(), data_cnt, feat_cnt, &h), "rr_dataset_upload");
+  rr_check(rr_dataset_upload(ctx, fm.data(), zeros.data(), data_cnt, feat_cnt, &h), "rr_dataset_upload");
   std::vector<uint32_t> code;
   std::vector<double> consts;
   compile(final_solution_, code, consts);
-  const int rc = rr_predict(gpu_->ctx, h, code.data(), (int32_t)code.size(), consts.data(), (int32_t)consts.size(),
+  const int rc = rr_predict(ctx, h, code.data(), (int32_t)code.size(), consts.data(), (int32_t)consts.size(),
                             out.data());
-  rr_dataset_free(gpu_->ctx, h);
+  rr_dataset_free(ctx, h);
   rr_check(rc, "rr_predict");
   if (classification_)
     for (double& v : out) v = v >= 0.5 ? 1.0 : 0.0;

@@ -18,6 +18,7 @@
 
 #include "../../include/rr_fitness.h"
 #include "tree.h"
+#include "workers.h"
 
 namespace rrh {
 
@@ -51,6 +52,24 @@ class CRand {
   struct random_data data_ {};
 };
 
+// Where a search's fitness calls go.  The default is a context of its own
+// (DirectBackend in search.cpp); the request loop of serve.h passes a shared
+// dispatcher that merges the calls of many concurrent searches into one
+// launch.  Implementations are thread-safe where they are shared.
+class FitnessBackend {
+ public:
+  virtual ~FitnessBackend() = default;
+  // upload a dataset (feature-major X, n x d); returns its handle
+  virtual int32_t upload(const double* X_fm, const double* y, int64_t n, int32_t d) = 0;
+  virtual void release(int32_t handle) = 0;
+  // one batched fitness call (rr_fitness_batch semantics); kernel_ms gets the
+  // kernel time of the launch that carried it
+  virtual void fitness(int32_t handle, const rr_batch& b, int32_t mode, int32_t precision, double* coefs,
+                       rr_fit* fit, double* kernel_ms) = 0;
+  // a context for predict, or null when the backend has none to lend
+  virtual rr_ctx* context() = 0;
+};
+
 class GpuSession;
 
 class RilsRols {
@@ -76,6 +95,11 @@ class RilsRols {
   int precision() const { return precision_; }
   void set_device(int d) { device_ = d; }
   void set_slab_budget_mb(int64_t mb) { slab_budget_mb_ = mb; }
+  void set_host_threads(int t);
+  // route fitness calls through `be` (not owned; must outlive the fit)
+  void set_backend(FitnessBackend* be) { backend_ = be; }
+  // per local-search sweep: wall time of batch + replay, ms
+  const std::vector<float>& sweep_ms() const { return sweep_ms_; }
   std::string final_prefix() const { return final_solution_.empty() ? "" : to_prefix(final_solution_); }
 
   // generators (rils_rols_cpp.cpp:121-443), public for tests
@@ -102,7 +126,13 @@ class RilsRols {
   int device_ = 0;
   int64_t slab_budget_mb_ = 0;
   GpuSession* gpu_ = nullptr;
+  FitnessBackend* backend_ = nullptr;
+  std::vector<float> sweep_ms_;
   Pool pool_;
+  int host_threads_ = 0;                  // 0: min(hardware threads, 8)
+  Workers* workers_ = nullptr;
+  std::vector<Pool> pools_;               // one per worker thread
+  Workers& workers();
 
   void reset();
   bool finished() const;

@@ -62,6 +62,8 @@ def lib():
         L.rr_dataset_stats.argtypes = [vp, ctypes.c_int32, dp, dp]
         L.rr_fitness_batch.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(Batch), ctypes.c_int32, ctypes.c_int32, dp,
                                        vp]
+        L.rr_fitness_multi.argtypes = [vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(Batch), ctypes.c_int32,
+                                       ctypes.c_int32, dp, vp]
         L.rr_predict.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_uint32), ctypes.c_int32, dp,
                                  ctypes.c_int32, dp]
         fp = ctypes.POINTER(ctypes.c_float)
@@ -252,7 +254,9 @@ class Context:
         check(lib().rr_last_kernel_ms(self.p, ctypes.byref(v)))
         return v.value
 
-    def fitness(self, h, arrs, mode="ols", precision="fp64"):
+    def fitness(self, h, arrs, mode="ols", precision="fp64", handles=None):
+        """Score a batch against dataset h, or candidate c against handles[c]
+        (rr_fitness_multi) when handles is given."""
         m = D["RR_MODE_OLS"] if mode == "ols" else D["RR_MODE_RAW"]
         pr = D["RR_FP64"] if precision == "fp64" else D["RR_FP32_REFINED"]
         nc = arrs["n_cands"]
@@ -260,7 +264,13 @@ class Context:
         coefs = np.zeros(max(ncoef, 1))
         fit = np.zeros(max(nc, 1), dtype=FIT_DTYPE)
         b = _as_batch(arrs)
-        check(lib().rr_fitness_batch(self.p, h, ctypes.byref(b), m, pr, _dp(coefs), fit.ctypes.data_as(ctypes.c_void_p)))
+        if handles is None:
+            check(lib().rr_fitness_batch(self.p, h, ctypes.byref(b), m, pr, _dp(coefs),
+                                         fit.ctypes.data_as(ctypes.c_void_p)))
+        else:
+            hs = np.ascontiguousarray(handles, dtype=np.int32)
+            check(lib().rr_fitness_multi(self.p, hs.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), ctypes.byref(b), m,
+                                         pr, _dp(coefs), fit.ctypes.data_as(ctypes.c_void_p)))
         per = []
         cs = arrs["cand_start"]
         for c in range(nc):

@@ -123,3 +123,38 @@ def test_bad_program_rejected_before_launch(ctx):
     with pytest.raises(RuntimeError):
         ctx.fitness(h, arrs, mode="raw")
     ctx.free(h)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_multi_dataset_batch_same_bits_as_solo(ctx, precision):
+    """Candidates of three datasets (different n, d, y) in one rr_fitness_multi
+    launch, interleaved, give the same bits as each dataset's own batch."""
+    print("case: multi-dataset batch %s" % precision, flush=True)
+    shapes = [(1500, 3, 31), (7000, 2, 32), (333, 4, 33)]
+    hs, per_ds = [], []
+    for n, d, seed in shapes:
+        X, y = make_data(seed, n, d)
+        od = O.Dataset(X, y)
+        trees = treegen.random_trees(seed, 40, 3, 30, d)
+        per_ds.append([od.tune(t)["factors"] for t in trees])
+        hs.append(ctx.upload(X, y))
+    for mode in ["ols", "raw"]:
+        lists = per_ds if mode == "ols" else [[[f[0]] for f in fl if f] for fl in per_ds]
+        solo = [ctx.fitness(h, capi.build_batch(fl), mode=mode, precision=precision) for h, fl in zip(hs, lists)]
+        mixed, owner = [], []
+        for i in range(max(len(fl) for fl in lists)):
+            for s, fl in enumerate(lists):
+                if i < len(fl):
+                    mixed.append(fl[i])
+                    owner.append((s, i))
+        cm, fm = ctx.fitness(None, capi.build_batch(mixed), mode=mode, precision=precision,
+                             handles=[hs[s] for s, _ in owner])
+        for q, (s, i) in enumerate(owner):
+            cs, fs = solo[s]
+            assert same_bits([fm[q]["e0"], fm[q]["e1"], fm[q]["err"]], [fs[i]["e0"], fs[i]["e1"], fs[i]["err"]]), \
+                (mode, s, i)
+            assert fm[q]["flags"] == fs[i]["flags"] and fm[q]["rank"] == fs[i]["rank"]
+            if mode == "ols":
+                assert same_bits(cm[q], cs[i]), (s, i)
+    for h in hs:
+        ctx.free(h)
