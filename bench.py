@@ -133,17 +133,21 @@ def timed_steps(runner, rescorer, precision, warmup, steps):
     """K steps of one batched call.  In the FP32 form a step also re-scores in
     FP64 every candidate the FP32 kernels could not certify (RR_F_NEEDS64),
     so each step delivers a trusted fitness for all 4096 candidates."""
+    fp32_fit = {}
+
     def one():
         coefs, fit = runner.run(precision)
         ms = runner.last_kernel_ms()
         n64 = 0
         if precision == "fp32":
+            fp32_fit["fit"] = fit.copy()          # the FP32 form's own flags and bounds
             need = np.nonzero(fit[:, 4].astype(np.int64) & 2)[0]
             n64 = len(need)
             if n64:
                 rescorer.select(need)
                 c2, f2 = rescorer.run_selected("fp64")
                 ms += rescorer.last_kernel_ms()
+                fp32_fit["rescore_ms"] = rescorer.last_kernel_ms()
                 fit[need] = f2
         return coefs, fit, ms, n64
     for _ in range(warmup):
@@ -156,7 +160,7 @@ def timed_steps(runner, rescorer, precision, warmup, steps):
         lat.append(time.perf_counter() - s)
         kms.append(ms)
     total = time.perf_counter() - t0
-    return total, np.array(lat), np.array(kms), coefs, fit, n64
+    return total, np.array(lat), np.array(kms), coefs, fit, n64, fp32_fit
 
 
 def cpu_baseline(X, y, trees, seconds_target=15.0, threads=1):
@@ -227,6 +231,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--fp64-rows", type=int, default=None, help="samples per lane per tile, FP64 kernels (1, 2, 4)")
     ap.add_argument("--fp32-rows", type=int, default=None, help="samples per lane per tile, FP32 kernels (1, 2)")
+    ap.add_argument("--fp32-chunk", type=int, default=None, help="samples per FP32 workgroup (multiple of 512)")
+    ap.add_argument("--fp32-streams", type=int, default=None, help="1: FP32 buckets on concurrent streams")
     ap.add_argument("--copy-probe", action="store_true", help="run only a plain 1 GiB device copy (PMC scaling)")
     args = ap.parse_args()
     if args.copy_probe:
@@ -246,6 +252,10 @@ def main():
         runner.set_option("fp64_rows", args.fp64_rows)
     if args.fp32_rows:
         runner.set_option("fp32_rows", args.fp32_rows)
+    if args.fp32_chunk:
+        runner.set_option("fp32_chunk_rows", args.fp32_chunk)
+    if args.fp32_streams is not None:
+        runner.set_option("fp32_streams", args.fp32_streams)
     t0 = time.perf_counter()
     nf = runner.prepare_ols(trees)
     t_prep = time.perf_counter() - t0
@@ -256,8 +266,8 @@ def main():
 
     res = {}
     for prec in (["fp64", "fp32"] if args.precision == "both" else [args.precision]):
-        total, lat, kms, coefs, fit, n64 = timed_steps(runner, runner, prec, args.warmup, args.steps)
-        res[prec] = dict(total=total, lat=lat, kms=kms, coefs=coefs, fit=fit, n64=n64)
+        total, lat, kms, coefs, fit, n64, f32 = timed_steps(runner, runner, prec, args.warmup, args.steps)
+        res[prec] = dict(total=total, lat=lat, kms=kms, coefs=coefs, fit=fit, n64=n64, f32=f32)
         print("[bench] %s: %.3f ms/step wall, %.3f ms kernel, %.0f evals/s" %
               (prec, 1e3 * total / args.steps, kms.mean(), N_TREES * args.steps / total), file=sys.stderr, flush=True)
 
@@ -286,12 +296,19 @@ def main():
         }
     if "fp32" in res:
         r = res["fp32"]
-        fl32 = r["fit"][:, 4].astype(int)
+        own = r["f32"]["fit"]                     # before the FP64 re-scores
+        fl32 = own[:, 4].astype(int)
         one = np.nonzero(np.array(nf) <= 1)[0]
         t1 = kernel_probe(runner, one, "fp32")
         f1 = float(np.sum(flops_c[one]))
         achieved = f1 / (t1 / 1e3) / 1e12
-        reasons = np.bincount(np.where((fl32 & 2) > 0, -r["fit"][:, 3].astype(int), 0), minlength=8).tolist()
+        reasons = np.bincount(np.where((fl32 & 2) > 0, -own[:, 3].astype(int), 0), minlength=8).tolist()
+        nfa = np.array(nf)
+        bucket_ms = {}
+        for lo, hi in [(2, 2), (3, 4), (5, 8)]:
+            idx = np.nonzero((nfa >= lo) & (nfa <= hi))[0]
+            if len(idx):
+                bucket_ms["nf%d-%d" % (lo, hi)] = {"candidates": int(len(idx)), "ms": kernel_probe(runner, idx, "fp32", 2)}
         out["fp32"] = {
             "evals_per_s": N_TREES * args.steps / r["total"],
             "ms_per_step": 1e3 * r["total"] / args.steps,
@@ -300,18 +317,29 @@ def main():
             "p99_step_ms": float(np.percentile(r["lat"], 99) * 1e3),
             "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                          "frac": achieved / PEAK_FP32_TFLOPS, "traffic": pmc_traffic("k_ols_fp32_nf1"),
-                         "kernel": "k_ols_fp32<1,%d>" % (args.fp32_rows or 1), "launch_ms": t1,
+                         "kernel": "k_acc_fp32<1,%d> + k_fin_fp32<1>" % (args.fp32_rows or 1), "launch_ms": t1,
                          "candidates": int(len(one)), "algorithmic_flops": f1},
             "batch_valu_tflops": flops / (r["kms"].mean() / 1e3) / 1e12,
             "needs_fp64_frac": float(np.mean((fl32 & 2) > 0)),
             "needs_fp64_reasons": reasons,
+            "rescore_fp64_ms": r["f32"].get("rescore_ms", 0.0),
+            "bucket_launch_ms": bucket_ms,
         }
         if "fp64" in res:
-            a, b = res["fp64"]["fit"], r["fit"]
+            # certified FP32 results (not sentinels, not re-scored) against the FP64 form
+            a, b = res["fp64"]["fit"], own
             ok = ((a[:, 4].astype(int) & 1) == 0) & ((b[:, 4].astype(int) & 3) == 0)
             r2a, r2b = 1 - a[ok, 0], 1 - b[ok, 0]
+            fa = (1 + a[ok, 0]) * (1 + a[ok, 1])
+            fb = (1 + b[ok, 0]) * (1 + b[ok, 1])
+            out["fp32"]["certified"] = int(ok.sum())
             out["fp32"]["r2_abs_err_max"] = float(np.max(np.abs(r2a - r2b))) if ok.any() else None
             out["fp32"]["r2_abs_err_median"] = float(np.median(np.abs(r2a - r2b))) if ok.any() else None
+            out["fp32"]["fitness_rel_err_max"] = float(np.max(np.abs(fa - fb) / fa)) if ok.any() else None
+            out["fp32"]["bound_over_err_min"] = float(np.min(b[ok, 2] / np.maximum(np.abs(fa - fb) / fa, 1e-300))) \
+                if ok.any() else None
+            out["fp32"]["sentinels_agree"] = bool(np.all(((a[:, 4].astype(int) & 1) > 0) >=
+                                                         ((b[:, 4].astype(int) & 1) > 0)))
 
     cpu = None
     cpu1 = None

@@ -29,9 +29,10 @@
 // Arithmetic conventions (the reference's are partly unreadable: Eigen's
 // reduction kernels and vectorised transcendentals live in Eigen/Core, which
 // the vendored copy lacks):
-//   * every sum over samples uses one canonical order: 64 partial sums, row i
-//     added to partial (i mod 64) in increasing i, then an xor-butterfly
-//     (pairs at distance 32, 16, 8, 4, 2, 1); canon_sum() below;
+//   * every sum over samples uses one canonical order: 1024 partial sums,
+//     row i added to partial (i mod 1024) in increasing i, each group of 64
+//     partials combined by an xor-butterfly (pairs at distance 32, 16, 8, 4,
+//     2, 1), the 16 group sums added in order; rrm::canon_sum (rr_math.h);
 //   * maxCoeff() is a left-to-right scan with strict '>' (first index wins);
 //   * sin/cos/exp/log come from rr_math.h; + - * / sqrt are IEEE;
 //   * no floating-point contraction (built with -ffp-contract=off);
@@ -398,17 +399,10 @@ struct Data {
   double ybar = 0, sst = 0;
 };
 
+// the canonical order of sums over samples (rr_math.h)
 template <class F>
 static double canon_sum(int64_t lo, int64_t hi, F f) {
-  double acc[64];
-  for (int l = 0; l < 64; ++l) acc[l] = 0.0;
-  for (int64_t i = lo; i < hi; ++i) acc[i & 63] = acc[i & 63] + f(i);
-  for (int s = 32; s >= 1; s >>= 1) {
-    double nx[64];
-    for (int l = 0; l < 64; ++l) nx[l] = acc[l] + acc[l ^ s];
-    for (int l = 0; l < 64; ++l) acc[l] = nx[l];
-  }
-  return acc[0];
+  return rrm::canon_sum(lo, hi, f);
 }
 
 static void finish_data(Data& D) {

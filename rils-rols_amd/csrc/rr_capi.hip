@@ -89,10 +89,13 @@ struct rr_ctx {
   hipEvent_t fork = nullptr, join[rr::FP32_BUCKETS - 1] = {};
   std::vector<Dataset> ds;
   DevBuf d_cand, d_prog, d_code, d_K, d_slab, d_slab_off, d_coefs, d_fit, d_misc, d_pred_e, d_dstab, d_cand_ds;
+  DevBuf d_blk_cand, d_blk_chunk, d_chunk_base, d_work, d_work_flags;   // FP32 chunk map and partials
   HostBuf h_stage, h_out;
   int64_t slab_budget_mb = 32768;
   int64_t fp64_rows = 2;
   int64_t fp32_rows = 1;
+  int64_t fp32_chunk_rows = 8192;
+  int64_t fp32_streams = 0;   // 1: bucket kernels on concurrent side streams
   double last_ms = 0.0;
 };
 
@@ -222,7 +225,8 @@ int rr_destroy(rr_ctx* ctx) {
     (void)hipFree(d.X); (void)hipFree(d.y); (void)hipFree(d.X32);
   }
   DevBuf* bufs[] = {&ctx->d_cand, &ctx->d_prog, &ctx->d_code, &ctx->d_K, &ctx->d_slab, &ctx->d_slab_off,
-                    &ctx->d_coefs, &ctx->d_fit, &ctx->d_misc, &ctx->d_pred_e, &ctx->d_dstab, &ctx->d_cand_ds};
+                    &ctx->d_coefs, &ctx->d_fit, &ctx->d_misc, &ctx->d_pred_e, &ctx->d_dstab, &ctx->d_cand_ds,
+                    &ctx->d_blk_cand, &ctx->d_blk_chunk, &ctx->d_chunk_base, &ctx->d_work, &ctx->d_work_flags};
   for (DevBuf* b : bufs) b->release();
   ctx->h_stage.release();
   ctx->h_out.release();
@@ -361,12 +365,52 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
   }
   auto ds_of = [&](int32_t c) -> const Dataset& { return ctx->ds[uniq[handles ? cand_ds[c] : 0]]; };
 
+  // FP32 form: candidates in factor-count buckets (OLS) or as given (RAW),
+  // and the chunk workgroups of the accumulation pass in that order
+  std::vector<int32_t> order, blk_cand, blk_chunk, chunk_base;
+  int counts[rr::FP32_BUCKETS] = {0, 0, 0, 0, 0}, blocks[rr::FP32_BUCKETS] = {0, 0, 0, 0, 0};
+  int64_t total_chunks = 0;
+  if (precision == RR_FP32_REFINED) {
+    order.resize(nc);
+    if (mode == RR_MODE_RAW) {
+      for (int32_t c = 0; c < nc; ++c) order[c] = c;
+      counts[0] = nc;
+    } else {
+      std::vector<int8_t> bucket(nc);
+      for (int32_t c = 0; c < nc; ++c) {
+        const int nfc = b->cand_start[c + 1] - b->cand_start[c];
+        int kb = 0;
+        while (kb < rr::FP32_BUCKETS - 1 && nfc > rr::FP32_BUCKET_FACTORS[kb]) ++kb;
+        bucket[c] = (int8_t)kb;
+        ++counts[kb];
+      }
+      int start[rr::FP32_BUCKETS];
+      for (int kb = 0, acc = 0; kb < rr::FP32_BUCKETS; ++kb) { start[kb] = acc; acc += counts[kb]; }
+      for (int32_t c = 0; c < nc; ++c) order[start[bucket[c]]++] = c;
+    }
+    chunk_base.assign(nc, 0);
+    for (int kb = 0, q = 0; kb < rr::FP32_BUCKETS; ++kb) {
+      for (int i = 0; i < counts[kb]; ++i, ++q) {
+        const int32_t c = order[q];
+        if (kb == rr::FP32_BUCKETS - 1 && mode == RR_MODE_OLS) continue;   // flagged without evaluation
+        const int64_t nch = (ds_of(c).n + ctx->fp32_chunk_rows - 1) / ctx->fp32_chunk_rows;
+        if (total_chunks + nch > INT32_MAX) return fail(RR_E_LIMIT, "too many FP32 chunks in one call");
+        chunk_base[c] = (int32_t)total_chunks;
+        for (int64_t ch = 0; ch < nch; ++ch) { blk_cand.push_back(c); blk_chunk.push_back((int32_t)ch); }
+        total_chunks += nch;
+        blocks[kb] += (int)nch;
+      }
+    }
+  }
+
   // stage inputs
   const size_t s_cand = align_up((size_t)(nc + 1) * 4), s_prog = align_up((size_t)(np + 1) * 4);
   const size_t s_code = align_up((size_t)ncode * 4), s_K = align_up((size_t)std::max(b->n_consts, 1) * 8);
   const size_t s_off = align_up((size_t)nc * 8);
   const size_t s_tab = align_up(table.size() * sizeof(rr::DsDesc)), s_cds = align_up((size_t)nc * 4);
-  if (ctx->h_stage.ensure(s_cand + s_prog + s_code + s_K + s_off + s_tab + s_cds) != RR_OK) return RR_E_NOMEM;
+  const size_t s_chunks = 2 * align_up((size_t)total_chunks * 4) + align_up(chunk_base.size() * 4);
+  if (ctx->h_stage.ensure(s_cand + s_prog + s_code + s_K + s_off + s_tab + s_cds + s_chunks) != RR_OK)
+    return RR_E_NOMEM;
   size_t so = 0;
   if ((rc = upload(ctx, ctx->d_cand, b->cand_start, (size_t)(nc + 1) * 4, so))) return rc;
   so += s_cand;
@@ -397,21 +441,36 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
   rr_fit* dfit = (rr_fit*)ctx->d_fit.p;
   double* dcoef = (double*)ctx->d_coefs.p;
 
+  // FP32: order, chunk map and workspace
+  rr::Fp32Launch L32{};
+  if (precision == RR_FP32_REFINED) {
+    if ((rc = upload(ctx, ctx->d_slab_off, order.data(), (size_t)nc * 4, so))) return rc;
+    so += s_off;
+    if ((rc = upload(ctx, ctx->d_blk_cand, blk_cand.data(), (size_t)total_chunks * 4, so))) return rc;
+    so += align_up((size_t)total_chunks * 4);
+    if ((rc = upload(ctx, ctx->d_blk_chunk, blk_chunk.data(), (size_t)total_chunks * 4, so))) return rc;
+    so += align_up((size_t)total_chunks * 4);
+    if ((rc = upload(ctx, ctx->d_chunk_base, chunk_base.data(), chunk_base.size() * 4, so))) return rc;
+    so += align_up(chunk_base.size() * 4);
+    if (ctx->d_work.ensure((size_t)std::max<int64_t>(total_chunks, 1) * rr::FP32_ACC_STRIDE * 8) != RR_OK ||
+        ctx->d_work_flags.ensure((size_t)std::max<int64_t>(total_chunks, 1) * 8) != RR_OK)
+      return RR_E_NOMEM;
+    L32.cand_start = dc; L32.prog_start = dp; L32.code = dcode; L32.K = dK; L32.ds = dtab; L32.cand_ds = dcds;
+    L32.coefs = dcoef; L32.fit = dfit; L32.n_cands = nc; L32.raw = mode == RR_MODE_RAW ? 1 : 0;
+    L32.rows = (int)ctx->fp32_rows;
+    L32.blk_cand = (const int32_t*)ctx->d_blk_cand.p; L32.blk_chunk = (const int32_t*)ctx->d_blk_chunk.p;
+    L32.chunk_base = (const int32_t*)ctx->d_chunk_base.p; L32.chunk_rows = ctx->fp32_chunk_rows;
+    L32.part = (double*)ctx->d_work.p; L32.flags = (uint32_t*)ctx->d_work_flags.p;
+  }
+
   if (mode == RR_MODE_RAW) {
     RR_HIP(hipEventRecord(ctx->ev0, ctx->stream));
     if (precision == RR_FP64) {
       rr::RawLaunch L{dc, dp, dcode, dK, dtab, dcds, dfit, nc, (int)ctx->fp64_rows};
       RR_HIP(rr::launch_raw_fp64(L, ctx->stream));
     } else {
-      std::vector<int32_t> order(nc);
-      for (int32_t c = 0; c < nc; ++c) order[c] = c;
-      if ((rc = upload(ctx, ctx->d_slab_off, order.data(), (size_t)nc * 4, so))) return rc;
-      rr::Fp32Launch L{};
-      L.cand_start = dc; L.prog_start = dp; L.code = dcode; L.K = dK; L.ds = dtab; L.cand_ds = dcds;
-      L.coefs = dcoef; L.fit = dfit; L.n_cands = nc; L.raw = 1; L.rows = (int)ctx->fp32_rows;
-      const int counts[rr::FP32_BUCKETS] = {nc, 0, 0, 0, 0};
       const hipStream_t st[rr::FP32_BUCKETS] = {ctx->stream, ctx->stream, ctx->stream, ctx->stream, ctx->stream};
-      RR_HIP(rr::launch_fp32(L, (const int32_t*)ctx->d_slab_off.p, counts, st));
+      RR_HIP(rr::launch_fp32(L32, (const int32_t*)ctx->d_slab_off.p, counts, blocks, st));
     }
     RR_HIP(hipEventRecord(ctx->ev1, ctx->stream));
   } else if (precision == RR_FP64) {
@@ -450,36 +509,24 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
     }
     RR_HIP(hipEventRecord(ctx->ev1, ctx->stream));
   } else {
-    // group candidates by factor count so each kernel instance keeps only
-    // the accumulators its bucket needs
-    std::vector<int32_t> order(nc);
-    int counts[rr::FP32_BUCKETS] = {0, 0, 0, 0, 0};
-    std::vector<int8_t> bucket(nc);
-    for (int32_t c = 0; c < nc; ++c) {
-      const int nf = b->cand_start[c + 1] - b->cand_start[c];
-      int kb = 0;
-      while (kb < rr::FP32_BUCKETS - 1 && nf > rr::FP32_BUCKET_FACTORS[kb]) ++kb;
-      bucket[c] = (int8_t)kb;
-      ++counts[kb];
-    }
-    int start[rr::FP32_BUCKETS];
-    for (int kb = 0, acc = 0; kb < rr::FP32_BUCKETS; ++kb) { start[kb] = acc; acc += counts[kb]; }
-    for (int32_t c = 0; c < nc; ++c) order[start[bucket[c]]++] = c;
-    if ((rc = upload(ctx, ctx->d_slab_off, order.data(), (size_t)nc * 4, so))) return rc;
-    rr::Fp32Launch L{};
-    L.cand_start = dc; L.prog_start = dp; L.code = dcode; L.K = dK; L.ds = dtab; L.cand_ds = dcds;
-    L.coefs = dcoef; L.fit = dfit; L.n_cands = nc; L.raw = 0; L.rows = (int)ctx->fp32_rows;
-    // bucket 0 (the bulk) on the main stream, the others on side streams
+    // factor-count buckets, each kernel instance keeping only the
+    // accumulators its bucket needs; bucket kernels back to back on the context's stream, or (option
+    // fp32_streams) bucket 0 there and the others on side streams
+    const bool side = ctx->fp32_streams != 0;
     hipStream_t st[rr::FP32_BUCKETS];
     st[0] = ctx->stream;
-    for (int i = 1; i < rr::FP32_BUCKETS; ++i) st[i] = ctx->side[i - 1];
+    for (int i = 1; i < rr::FP32_BUCKETS; ++i) st[i] = side ? ctx->side[i - 1] : ctx->stream;
     RR_HIP(hipEventRecord(ctx->ev0, ctx->stream));
-    RR_HIP(hipEventRecord(ctx->fork, ctx->stream));
-    for (int i = 0; i < rr::FP32_BUCKETS - 1; ++i) RR_HIP(hipStreamWaitEvent(ctx->side[i], ctx->fork, 0));
-    RR_HIP(rr::launch_fp32(L, (const int32_t*)ctx->d_slab_off.p, counts, st));
-    for (int i = 0; i < rr::FP32_BUCKETS - 1; ++i) {
-      RR_HIP(hipEventRecord(ctx->join[i], ctx->side[i]));
-      RR_HIP(hipStreamWaitEvent(ctx->stream, ctx->join[i], 0));
+    if (side) {
+      RR_HIP(hipEventRecord(ctx->fork, ctx->stream));
+      for (int i = 0; i < rr::FP32_BUCKETS - 1; ++i) RR_HIP(hipStreamWaitEvent(ctx->side[i], ctx->fork, 0));
+    }
+    RR_HIP(rr::launch_fp32(L32, (const int32_t*)ctx->d_slab_off.p, counts, blocks, st));
+    if (side) {
+      for (int i = 0; i < rr::FP32_BUCKETS - 1; ++i) {
+        RR_HIP(hipEventRecord(ctx->join[i], ctx->side[i]));
+        RR_HIP(hipStreamWaitEvent(ctx->stream, ctx->join[i], 0));
+      }
     }
     RR_HIP(hipEventRecord(ctx->ev1, ctx->stream));
   }
@@ -578,6 +625,12 @@ int rr_set_option(rr_ctx* ctx, const char* name, int64_t value) {
   } else if (s == "fp32_rows") {
     if (value != 1 && value != 2) return fail(RR_E_ARG, "fp32_rows must be 1 or 2");
     ctx->fp32_rows = value;
+  } else if (s == "fp32_streams") {
+    if (value != 0 && value != 1) return fail(RR_E_ARG, "fp32_streams must be 0 or 1");
+    ctx->fp32_streams = value;
+  } else if (s == "fp32_chunk_rows") {
+    if (value < 512 || value > (int64_t)1 << 30 || value % 512) return fail(RR_E_ARG, "fp32_chunk_rows must be a multiple of 512 in [512, 2^30]");
+    ctx->fp32_chunk_rows = value;
   } else return fail(RR_E_ARG, "unknown option " + s);
   return RR_OK;
 }
@@ -588,6 +641,8 @@ int rr_get_option(rr_ctx* ctx, const char* name, int64_t* value) {
   if (s == "slab_budget_mb") *value = ctx->slab_budget_mb;
   else if (s == "fp64_rows") *value = ctx->fp64_rows;
   else if (s == "fp32_rows") *value = ctx->fp32_rows;
+  else if (s == "fp32_chunk_rows") *value = ctx->fp32_chunk_rows;
+  else if (s == "fp32_streams") *value = ctx->fp32_streams;
   else return fail(RR_E_ARG, "unknown option " + s);
   return RR_OK;
 }

@@ -13,19 +13,47 @@ namespace rr {
 
 constexpr int WAVE = 64;
 
-// Canonical sum over rows [lo, n): row i goes to lane (i mod 64), each lane
-// adds its rows in increasing order starting from +0.0, then the 64 partials
-// are combined by an xor butterfly at distances 32, 16, 8, 4, 2, 1.  Every
-// lane ends with the same bits (a + b == b + a in IEEE arithmetic).
-__device__ inline int64_t first_row(int64_t lo, int lane) {
-  const int64_t i = (lo & ~int64_t(63)) + lane;
-  return i < lo ? i + 64 : i;
+// Canonical sums over samples (the order rrm::canon_sum defines): a
+// workgroup of CW = 16 waves, thread t = 64 w + l, adds rows i with
+// i mod 1024 == t in increasing order starting from +0.0; each wave combines
+// its 64 partials by an xor butterfly at distances 32, 16, 8, 4, 2, 1 (every
+// lane ends with the same bits: a + b == b + a), and the 16 wave sums are
+// added in wave order.
+constexpr int CW = rrm::CANON_GROUPS;
+constexpr int CP = rrm::CANON_PARTIALS;
+
+// first row >= lo that thread t (of CP) owns
+__device__ inline int64_t first_row(int64_t lo, int t) {
+  const int64_t i = (lo & ~int64_t(CP - 1)) + t;
+  return i < lo ? i + CP : i;
 }
 
 __device__ inline double wave_sum(double v) {
 #pragma unroll
   for (int s = 32; s >= 1; s >>= 1) v = v + __shfl_xor(v, s);
   return v;
+}
+
+// Canonical block sums of NV values at once; every thread gets the results.
+// red: LDS scratch of CW x NV doubles.  Contains two barriers.
+template <int NV>
+__device__ inline void block_sums(double (&v)[NV], double (*red)[NV]) {
+  const int lane = (int)(threadIdx.x & 63u), wave = (int)(threadIdx.x >> 6);
+#pragma unroll
+  for (int t = 0; t < NV; ++t) v[t] = wave_sum(v[t]);
+  if (lane == 0) {
+#pragma unroll
+    for (int t = 0; t < NV; ++t) red[wave][t] = v[t];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int t = 0; t < NV; ++t) {
+    double s = red[0][t];
+#pragma nounroll
+    for (int w = 1; w < CW; ++w) s = s + red[w][t];
+    v[t] = s;
+  }
+  __syncthreads();
 }
 
 __device__ inline double bin_f64(uint32_t bop, double a, double b) {
@@ -51,17 +79,16 @@ __device__ __noinline__ double call_cos(double x) { return rrm::cos(x); }
 __device__ __noinline__ double call_log(double x) { return rrm::log(x); }
 __device__ __noinline__ double call_exp(double x) { return rrm::exp(x); }
 
-// Rows are padded so that a tile of 64*RR_TILE_ROWS rows never reads past
-// the end of a column: the row stride of every dataset and work column is a
-// multiple of RR_ROW_PAD.
-constexpr int RR_TILE_ROWS = 8;
-constexpr int64_t RR_ROW_PAD = 64 * RR_TILE_ROWS;
+// Rows are padded so that a tile (CP threads x up to 4 rows, or 64 lanes x up
+// to 8 rows) never reads past the end of a column: the row stride of every
+// dataset and work column is a multiple of RR_ROW_PAD.
+constexpr int64_t RR_ROW_PAD = 4 * CP;
 
-// Evaluate program code[0..len) at R samples: row0, row0+64, ...,
-// row0+64(R-1).  X is feature-major with column stride ldx.  Per-sample
+// Evaluate program code[0..len) at R samples: row0, row0+ST, ...,
+// row0+ST(R-1).  X is feature-major with column stride ldx.  Per-sample
 // semantics: node::evaluate_inner (rils_rols_cpp/node.cpp:23-95) with
 // rr_math functions.  The opcode is decoded once per R samples.
-template <int R>
+template <int R, int ST = CP>
 __device__ inline void eval_f64(const uint32_t* __restrict__ code, int len, const double* __restrict__ K,
                                 const double* __restrict__ X, int64_t ldx, int64_t row0, double (&acc)[R]) {
   // stack slots 0-1 in registers; deeper slots (rare with Sethi-Ullman
@@ -99,11 +126,11 @@ __device__ inline void eval_f64(const uint32_t* __restrict__ code, int len, cons
         }
         case RR_M_V:
 #pragma unroll
-          for (int r = 0; r < R; ++r) { l[r] = acc[r]; rr[r] = X[(int64_t)arg * ldx + row0 + 64 * r]; }
+          for (int r = 0; r < R; ++r) { l[r] = acc[r]; rr[r] = X[(int64_t)arg * ldx + row0 + (int64_t)ST * r]; }
           break;
         case RR_M_VR:
 #pragma unroll
-          for (int r = 0; r < R; ++r) { l[r] = X[(int64_t)arg * ldx + row0 + 64 * r]; rr[r] = acc[r]; }
+          for (int r = 0; r < R; ++r) { l[r] = X[(int64_t)arg * ldx + row0 + (int64_t)ST * r]; rr[r] = acc[r]; }
           break;
         case RR_M_C: {
           const double k = K[arg];
@@ -129,7 +156,7 @@ __device__ inline void eval_f64(const uint32_t* __restrict__ code, int len, cons
       switch (op) {
         case RR_OP_LDV:
 #pragma unroll
-          for (int r = 0; r < R; ++r) acc[r] = X[(int64_t)arg * ldx + row0 + 64 * r];
+          for (int r = 0; r < R; ++r) acc[r] = X[(int64_t)arg * ldx + row0 + (int64_t)ST * r];
           break;
         case RR_OP_LDC: {
           const double k = K[arg];

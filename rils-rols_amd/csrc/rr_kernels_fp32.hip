@@ -389,20 +389,48 @@ __device__ inline void chol_solve(const Solve<KB>& S, int k, const double* b, do
 }
 
 // Single-pass OLS fitness of the candidates with at most NF factors.
+// Chunked single pass.  The samples of a candidate are cut into chunks of
+// chunk_rows rows; k_acc_fp32 gives every (candidate, chunk) pair a workgroup
+// of ACC_WAVES waves that evaluates the factors over the chunk and writes
+// partial sums (Gram, column sums, A^T y, sum e^2) and flags to a workspace;
+// k_fin_fp32 adds a candidate's chunk partials in chunk order and solves.
+// Wave w of a chunk takes its tiles of 64 R rows numbered w, w + ACC_WAVES,
+// ..., and partial sums are combined in wave order, so every result is fixed
+// by the inputs and chunk_rows alone.
+constexpr int ACC_WAVES = 4;
+constexpr int ACC_STRIDE = FP32_ACC_STRIDE;   // doubles per chunk partial (>= NG + 3 NF for NF <= 8)
+constexpr uint32_t CH_NAN0 = 1u << 31;   // flag word 1: factor 0 certainly NaN at a row >= 1
+
+struct ChunkArgs {
+  const int32_t* blk_cand;    // per workgroup of k_acc: candidate (batch index)
+  const int32_t* blk_chunk;   // per workgroup of k_acc: chunk of that candidate
+  const int32_t* chunk_base;  // per candidate: index of its first chunk in the workspace
+  int64_t chunk_rows;
+  double* part;               // ACC_STRIDE doubles per chunk
+  uint32_t* flags;            // 2 words per chunk: uncertain columns, NaN columns (| CH_NAN0)
+};
+
 template <int NF, int R>
-__global__ __launch_bounds__(64) void k_ols_fp32(Fp32Args a) {
-  constexpr int KB = NF + 1;
+__global__ __launch_bounds__(64 * ACC_WAVES) void k_acc_fp32(Fp32Args a, ChunkArgs w, int blk0) {
   constexpr int NG = NF * (NF + 1) / 2;
-  __shared__ Solve<KB> S;
-  const int cand = a.order[blockIdx.x];
+  constexpr int NACC = NG + 3 * NF;
+  static_assert(NACC <= ACC_STRIDE, "chunk partial stride");
+  __shared__ double part[ACC_WAVES][NACC];
+  __shared__ uint32_t part_flags[ACC_WAVES][2];
+  __shared__ int abort_flag;
+  const int q = blk0 + (int)blockIdx.x;
+  const int cand = w.blk_cand[q];
+  const int64_t chunk = w.blk_chunk[q];
   bind_dataset(a, cand);
-  const int lane = (int)threadIdx.x;
+  const int lane = (int)(threadIdx.x & 63u);
+  const int wave = (int)(threadIdx.x >> 6);
   const int pbeg = a.cand_start[cand];
   const int nf = a.cand_start[cand + 1] - pbeg;
-  const int k = nf + 1;
   const int64_t n = a.n, ldx = a.ldx;
-  double* const co = a.coefs + (int64_t)pbeg + cand;
-  if (n < k) { if (lane == 0) write_needs64(a.fit + cand, co, k, NR_SHORT); return; }
+  const int64_t r0 = chunk * w.chunk_rows;
+  const int64_t r1 = (r0 + w.chunk_rows < n) ? r0 + w.chunk_rows : n;
+  if (threadIdx.x == 0) abort_flag = 0;
+  __syncthreads();
 
   double G[NG], S1[NF], SY[NF], BD[NF];
 #pragma unroll
@@ -410,7 +438,9 @@ __global__ __launch_bounds__(64) void k_ols_fp32(Fp32Args a) {
 #pragma unroll
   for (int j = 0; j < NF; ++j) { S1[j] = 0.0; SY[j] = 0.0; BD[j] = 0.0; }
   uint32_t unc = 0, nanc = 0;
-  for (int64_t base = 0; base < n; base += 64 * R) {
+  const volatile int* abort_seen = &abort_flag;
+  for (int64_t base = r0 + (int64_t)wave * 64 * R; base < r1; base += (int64_t)ACC_WAVES * 64 * R) {
+    if (*abort_seen) break;
     const int64_t row0 = base + lane;
     double v[NF][R];
     float e[NF][R];
@@ -430,7 +460,7 @@ __global__ __launch_bounds__(64) void k_ols_fp32(Fp32Args a) {
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const int64_t i = row0 + 64 * r;
-        const bool in = i < n;
+        const bool in = i < r1;
         if (in && st[r] == S_UNC) unc |= 1u << j;
         if (in && st[r] == S_NAN) { nanc |= 1u << j; if (j == 0 && i >= 1) nan0_tail = true; }
         const double fv = in ? (double)f[r].v : 0.0;
@@ -439,17 +469,15 @@ __global__ __launch_bounds__(64) void k_ols_fp32(Fp32Args a) {
         for (int jj = 0; jj < NF; ++jj) if (jj == j) { v[jj][r] = fv; e[jj][r] = fe; }
       }
     }
-    // Factor 0 certainly NaN at a row >= 1 decides the FP64 outcome: its
-    // column norm is NaN, so the rank threshold is NaN (no cut), factor 0
-    // pivots first and its reflector is NaN -- every coefficient is NaN.
+    // Factor 0 certainly NaN at a row >= 1 decides the candidate (see
+    // k_fin_fp32); the chunk stops there.
     if (__any((int)nan0_tail)) {
-      for (int i = lane; i < k; i += WAVE) co[i] = rrm::qnan();
-      if (lane == 0) write_sentinel(a.fit + cand, k);
-      return;
+      if (lane == 0) abort_flag = 1;
+      break;
     }
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      const double yi = a.y[row0 + 64 * r] ;   // zero past n (padded column)
+      const double yi = a.y[row0 + 64 * r];   // zero past n (padded column)
       int g = 0;
 #pragma unroll
       for (int j = 0; j < NF; ++j) {
@@ -464,12 +492,80 @@ __global__ __launch_bounds__(64) void k_ols_fp32(Fp32Args a) {
   }
 #pragma unroll
   for (int s = 32; s >= 1; s >>= 1) { unc |= __shfl_xor(unc, s); nanc |= __shfl_xor(nanc, s); }
-  if (unc) { if (lane == 0) write_needs64(a.fit + cand, co, k, NR_UNCERTAIN); return; }
 #pragma unroll
   for (int j = 0; j < NG; ++j) G[j] = wave_sum(G[j]);
 #pragma unroll
   for (int j = 0; j < NF; ++j) { S1[j] = wave_sum(S1[j]); SY[j] = wave_sum(SY[j]); BD[j] = wave_sum(BD[j]); }
+  if (lane == 0) {
+#pragma unroll
+    for (int j = 0; j < NG; ++j) part[wave][j] = G[j];
+#pragma unroll
+    for (int j = 0; j < NF; ++j) { part[wave][NG + j] = S1[j]; part[wave][NG + NF + j] = SY[j]; part[wave][NG + 2 * NF + j] = BD[j]; }
+    part_flags[wave][0] = unc;
+    part_flags[wave][1] = nanc;
+  }
+  __syncthreads();
+  const int64_t cid = (int64_t)w.chunk_base[cand] + chunk;
+  if ((int)threadIdx.x < NACC) {
+    const int j = (int)threadIdx.x;
+    double t = part[0][j];
+    for (int ww = 1; ww < ACC_WAVES; ++ww) t = t + part[ww][j];
+    w.part[cid * ACC_STRIDE + j] = t;
+  }
+  if (threadIdx.x == 0) {
+    uint32_t fu = 0, fn = 0;
+    for (int ww = 0; ww < ACC_WAVES; ++ww) { fu |= part_flags[ww][0]; fn |= part_flags[ww][1]; }
+    w.flags[2 * cid] = fu;
+    w.flags[2 * cid + 1] = fn | (abort_flag ? CH_NAN0 : 0u);
+  }
+}
+
+// One wave per candidate: chunk partials in chunk order, then the solve and
+// the certificate on one lane.
+template <int NF>
+__global__ __launch_bounds__(64) void k_fin_fp32(Fp32Args a, ChunkArgs w) {
+  constexpr int KB = NF + 1;
+  constexpr int NG = NF * (NF + 1) / 2;
+  constexpr int NACC = NG + 3 * NF;
+  __shared__ Solve<KB> S;
+  __shared__ double accs[NACC];
+  const int cand = a.order[blockIdx.x];
+  bind_dataset(a, cand);
+  const int lane = (int)threadIdx.x;
+  const int pbeg = a.cand_start[cand];
+  const int nf = a.cand_start[cand + 1] - pbeg;
+  const int k = nf + 1;
+  const int64_t n = a.n;
+  double* const co = a.coefs + (int64_t)pbeg + cand;
+  const int64_t c0 = w.chunk_base[cand];
+  const int64_t nch = (n + w.chunk_rows - 1) / w.chunk_rows;
+  uint32_t unc = 0, nanc = 0;
+  for (int64_t c = lane; c < nch; c += WAVE) { unc |= w.flags[2 * (c0 + c)]; nanc |= w.flags[2 * (c0 + c) + 1]; }
+#pragma unroll
+  for (int s = 32; s >= 1; s >>= 1) { unc |= __shfl_xor(unc, s); nanc |= __shfl_xor(nanc, s); }
+  if (lane < NACC) {
+    double t = w.part[c0 * ACC_STRIDE + lane];
+    for (int64_t c = 1; c < nch; ++c) t = t + w.part[(c0 + c) * ACC_STRIDE + lane];
+    accs[lane] = t;
+  }
+  __syncthreads();
   if (lane != 0) return;
+  if (n < k) { write_needs64(a.fit + cand, co, k, NR_SHORT); return; }
+  // Factor 0 certainly NaN at a row >= 1 decides the FP64 outcome: its
+  // column norm is NaN, so the rank threshold is NaN (no cut), factor 0
+  // pivots first and its reflector is NaN -- every coefficient is NaN.
+  if (nanc & CH_NAN0) {
+    for (int i = 0; i < k; ++i) co[i] = rrm::qnan();
+    write_sentinel(a.fit + cand, k);
+    return;
+  }
+  nanc &= ~CH_NAN0;
+  if (unc) { write_needs64(a.fit + cand, co, k, NR_UNCERTAIN); return; }
+  double G[NG], S1[NF], SY[NF], BD[NF];
+#pragma unroll
+  for (int j = 0; j < NG; ++j) G[j] = accs[j];
+#pragma unroll
+  for (int j = 0; j < NF; ++j) { S1[j] = accs[NG + j]; SY[j] = accs[NG + NF + j]; BD[j] = accs[NG + 2 * NF + j]; }
 
   // ---- the rest runs on one lane
   {
@@ -585,18 +681,31 @@ __global__ __launch_bounds__(64) void k_ols_fp32(Fp32Args a) {
   a.fit[cand] = o;
 }
 
-// raw scoring of one tree in FP32 with the same certification
+// Raw scoring (a tree as given), chunked like the OLS form: partial SSR and
+// sum e^2 per chunk, then the bound per tree.
 template <int R>
-__global__ __launch_bounds__(64) void k_raw_fp32(Fp32Args a) {
-  const int cand = a.order[blockIdx.x];
+__global__ __launch_bounds__(64 * ACC_WAVES) void k_rawacc_fp32(Fp32Args a, ChunkArgs w, int blk0) {
+  __shared__ double part[ACC_WAVES][2];
+  __shared__ uint32_t part_flags[ACC_WAVES][2];
+  __shared__ int abort_flag;
+  const int q = blk0 + (int)blockIdx.x;
+  const int cand = w.blk_cand[q];
+  const int64_t chunk = w.blk_chunk[q];
   bind_dataset(a, cand);
-  const int lane = (int)threadIdx.x;
+  const int lane = (int)(threadIdx.x & 63u);
+  const int wave = (int)(threadIdx.x >> 6);
   const int p = a.cand_start[cand];
   const uint32_t* code = a.code + a.prog_start[p];
   const int len = a.prog_start[p + 1] - a.prog_start[p];
+  const int64_t r0 = chunk * w.chunk_rows;
+  const int64_t r1 = (r0 + w.chunk_rows < a.n) ? r0 + w.chunk_rows : a.n;
+  if (threadIdx.x == 0) abort_flag = 0;
+  __syncthreads();
   double ssr = 0.0, d2 = 0.0;
   uint32_t unc = 0, nanc = 0;
-  for (int64_t base = 0; base < a.n; base += 64 * R) {
+  const volatile int* abort_seen = &abort_flag;
+  for (int64_t base = r0 + (int64_t)wave * 64 * R; base < r1; base += (int64_t)ACC_WAVES * 64 * R) {
+    if (*abort_seen) break;
     const int64_t row0 = base + lane;
     VE f[R];
     uint32_t st[R];
@@ -604,7 +713,7 @@ __global__ __launch_bounds__(64) void k_raw_fp32(Fp32Args a) {
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int64_t i = row0 + 64 * r;
-      if (i < a.n) {
+      if (i < r1) {
         if (st[r] == S_NAN) nanc = 1;
         else if (st[r] == S_UNC) unc = 1;
         else {
@@ -614,15 +723,43 @@ __global__ __launch_bounds__(64) void k_raw_fp32(Fp32Args a) {
         }
       }
     }
-    if (__any((int)nanc)) break;   // a certain NaN prediction: the fitness is the sentinel
+    if (__any((int)nanc)) {   // a certain NaN prediction: the fitness is the sentinel
+      if (lane == 0) abort_flag = 1;
+      break;
+    }
   }
 #pragma unroll
   for (int s = 32; s >= 1; s >>= 1) { unc |= __shfl_xor(unc, s); nanc |= __shfl_xor(nanc, s); }
   ssr = wave_sum(ssr);
   d2 = wave_sum(d2);
+  if (lane == 0) { part[wave][0] = ssr; part[wave][1] = d2; part_flags[wave][0] = unc; part_flags[wave][1] = nanc; }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  const int64_t cid = (int64_t)w.chunk_base[cand] + chunk;
+  double ts = part[0][0], td = part[0][1];
+  uint32_t fu = part_flags[0][0], fn = part_flags[0][1];
+  for (int ww = 1; ww < ACC_WAVES; ++ww) { ts = ts + part[ww][0]; td = td + part[ww][1]; fu |= part_flags[ww][0]; fn |= part_flags[ww][1]; }
+  w.part[cid * ACC_STRIDE] = ts;
+  w.part[cid * ACC_STRIDE + 1] = td;
+  w.flags[2 * cid] = fu;
+  w.flags[2 * cid + 1] = fn | (abort_flag ? CH_NAN0 : 0u);
+}
+
+__global__ __launch_bounds__(64) void k_rawfin_fp32(Fp32Args a, ChunkArgs w) {
+  const int cand = a.order[blockIdx.x];
+  bind_dataset(a, cand);
+  const int lane = (int)threadIdx.x;
+  const int64_t c0 = w.chunk_base[cand];
+  const int64_t nch = (a.n + w.chunk_rows - 1) / w.chunk_rows;
+  uint32_t unc = 0, nanc = 0;
+  for (int64_t c = lane; c < nch; c += WAVE) { unc |= w.flags[2 * (c0 + c)]; nanc |= w.flags[2 * (c0 + c) + 1]; }
+#pragma unroll
+  for (int s = 32; s >= 1; s >>= 1) { unc |= __shfl_xor(unc, s); nanc |= __shfl_xor(nanc, s); }
   if (lane != 0) return;
   if (nanc) { write_sentinel(a.fit + cand, 0); return; }
   if (unc) { write_needs64(a.fit + cand, NR_UNCERTAIN); return; }
+  double ssr = w.part[c0 * ACC_STRIDE], d2 = w.part[c0 * ACC_STRIDE + 1];
+  for (int64_t c = 1; c < nch; ++c) { ssr = ssr + w.part[(c0 + c) * ACC_STRIDE]; d2 = d2 + w.part[(c0 + c) * ACC_STRIDE + 1]; }
   rr_fit o;
   o.rank = 0;
   const double r2 = 1 - ssr / a.sst;
@@ -696,26 +833,36 @@ hipError_t launch_predict_fp32(const uint32_t* code, int len, const double* K, c
   return hipGetLastError();
 }
 
-hipError_t launch_fp32(const Fp32Launch& L, const int32_t* order, const int* bucket_count, const hipStream_t* streams) {
+hipError_t launch_fp32(const Fp32Launch& L, const int32_t* order, const int* bucket_count, const int* bucket_blocks,
+                       const hipStream_t* streams) {
   Fp32Args a;
   a.cand_start = L.cand_start; a.prog_start = L.prog_start; a.code = L.code; a.K = L.K;
   a.ds = L.ds; a.cand_ds = L.cand_ds; a.coefs = L.coefs; a.fit = L.fit;
+  ChunkArgs w;
+  w.blk_cand = L.blk_cand; w.blk_chunk = L.blk_chunk; w.chunk_base = L.chunk_base; w.chunk_rows = L.chunk_rows;
+  w.part = L.part; w.flags = L.flags;
   const bool r2 = L.rows == 2;
+  const dim3 acc_blk(64 * ACC_WAVES);
   if (L.raw) {
     a.order = order;
     if (L.n_cands > 0) {
-      if (r2) hipLaunchKernelGGL(k_raw_fp32<2>, dim3(L.n_cands), dim3(64), 0, streams[0], a);
-      else hipLaunchKernelGGL(k_raw_fp32<1>, dim3(L.n_cands), dim3(64), 0, streams[0], a);
+      if (r2) hipLaunchKernelGGL(k_rawacc_fp32<2>, dim3(bucket_blocks[0]), acc_blk, 0, streams[0], a, w, 0);
+      else hipLaunchKernelGGL(k_rawacc_fp32<1>, dim3(bucket_blocks[0]), acc_blk, 0, streams[0], a, w, 0);
+      hipLaunchKernelGGL(k_rawfin_fp32, dim3(L.n_cands), dim3(64), 0, streams[0], a, w);
     }
     return hipGetLastError();
   }
-  int off[FP32_BUCKETS];
-  for (int b = 0, acc = 0; b < FP32_BUCKETS; ++b) { off[b] = acc; acc += bucket_count[b]; }
-#define RR_OLS32(B, NF)                                                                                 \
-  if (bucket_count[B] > 0) {                                                                            \
-    a.order = order + off[B];                                                                           \
-    if (r2) hipLaunchKernelGGL((k_ols_fp32<NF, 2>), dim3(bucket_count[B]), dim3(64), 0, streams[B], a); \
-    else hipLaunchKernelGGL((k_ols_fp32<NF, 1>), dim3(bucket_count[B]), dim3(64), 0, streams[B], a);   \
+  int off[FP32_BUCKETS], boff[FP32_BUCKETS];
+  for (int b = 0, acc = 0, bacc = 0; b < FP32_BUCKETS; ++b) {
+    off[b] = acc; acc += bucket_count[b];
+    boff[b] = bacc; bacc += bucket_blocks[b];
+  }
+#define RR_OLS32(B, NF)                                                                                    \
+  if (bucket_count[B] > 0) {                                                                               \
+    a.order = order + off[B];                                                                              \
+    if (r2) hipLaunchKernelGGL((k_acc_fp32<NF, 2>), dim3(bucket_blocks[B]), acc_blk, 0, streams[B], a, w, boff[B]); \
+    else hipLaunchKernelGGL((k_acc_fp32<NF, 1>), dim3(bucket_blocks[B]), acc_blk, 0, streams[B], a, w, boff[B]);   \
+    hipLaunchKernelGGL((k_fin_fp32<NF>), dim3(bucket_count[B]), dim3(64), 0, streams[B], a, w);              \
   }
   // the wide buckets first: few candidates, long per-candidate runs
   RR_OLS32(3, 8)

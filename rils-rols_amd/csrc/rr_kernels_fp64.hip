@@ -1,9 +1,10 @@
 // rr_kernels_fp64.hip -- the plain FP64 form of the candidate-fitness path.
 //
-// One wavefront owns one candidate and walks all n samples in tiles of 64*R
-// rows: lane l handles rows l, l+64, l+128, ..., R of them per tile, in
-// increasing order.  Every sum over samples therefore follows the canonical
-// order of rr_device.h whatever R is, and the column-pivoted Householder QR
+// One workgroup of 16 wavefronts owns one candidate and walks all n samples
+// in tiles of 1024*R rows: thread t handles rows t, t+1024, t+2048, ..., R of
+// them per tile, in increasing order.  Every sum over samples therefore
+// follows the canonical order of rr_math.h / rr_device.h whatever R is, and
+// the column-pivoted Householder QR
 // follows Eigen's ColPivHouseholderQR (eigen/Eigen/src/QR/
 // ColPivHouseholderQR.h:482-607, Householder.h:67-135) operation for
 // operation, so the results are the same bits as the CPU oracle's.
@@ -81,17 +82,22 @@ __device__ inline void bind_dataset(A& a, int cand) {
   a.X = D.X; a.y = D.y; a.n = D.n; a.ldx = D.ldx; a.sst = D.sst;
 }
 
-__device__ inline int64_t tile_start(int64_t lo, int R) { return (lo / (64 * R)) * (64 * R); }
+__device__ inline int64_t tile_start(int64_t lo, int R) { return (lo / (CP * R)) * (CP * R); }
 
+// One workgroup of CW = 16 waves per candidate; thread t takes the rows
+// i = t (mod 1024), R of them per tile of 1024 R rows, so every sum follows
+// the canonical order whatever R is.
 template <int MAXK, int R>
-__global__ __launch_bounds__(64) void k_ols_fp64(Fp64Args a) {
+__global__ __launch_bounds__(CP) void k_ols_fp64(Fp64Args a) {
   constexpr int G = (R == 1) ? 8 : (R == 2) ? 4 : 2;   // columns per QR pass
   __shared__ double normU[MAXK], normD[MAXK], tsq[MAXK], cc[MAXK];
   __shared__ int phys[MAXK];
+  __shared__ double red1[CW][1], red2[CW][2], redG[CW][G], redS[CW][2 * G];
+  __shared__ int abort_flag;
 
   const int cand = a.cand_base + (int)blockIdx.x;
   bind_dataset(a, cand);
-  const int lane = (int)threadIdx.x;
+  const int tid = (int)threadIdx.x;
   const int pbeg = a.cand_start[cand];
   const int nf = a.cand_start[cand + 1] - pbeg;
   const int k = nf + 1;
@@ -101,6 +107,8 @@ __global__ __launch_bounds__(64) void k_ols_fp64(Fp64Args a) {
   double* const S = a.slab + a.slab_off[blockIdx.x];
   const double* const X = a.X;
   double* const co = a.coefs + (int64_t)pbeg + cand;
+  if (tid == 0) abort_flag = 0;
+  __syncthreads();
 
   // ---- 1. factor columns, free term, copy of y; norms
   // Factor 0 NaN at a row >= 1 fixes the outcome: its norm is NaN, so the
@@ -108,49 +116,54 @@ __global__ __launch_bounds__(64) void k_ols_fp64(Fp64Args a) {
   // greater than NaN), its tail norm and hence its reflector are NaN, and
   // with the threshold NaN no later column stops the factorisation: every
   // coefficient is NaN, rank = min(n, k).  Evaluation stops at the first tile
-  // that shows it.
+  // that shows it (any wave; the others stop at their next tile).
+  const volatile int* abort_seen = &abort_flag;
   for (int j = 0; j < nf; ++j) {
     const uint32_t* code = a.code + a.prog_start[pbeg + j];
     const int len = a.prog_start[pbeg + j + 1] - a.prog_start[pbeg + j];
     double* col = S + (int64_t)j * ldx;
     double* keep = S + (int64_t)(k + 1 + j) * ldx;
-    double s_all = 0.0, s_1 = 0.0;
-    for (int64_t base = 0; base < n; base += 64 * R) {
-      const int64_t row0 = base + lane;
+    double sums[2] = {0.0, 0.0};   // rows >= 0, rows >= 1
+    for (int64_t base = 0; base < n; base += CP * R) {
+      if (j == 0 && *abort_seen) break;
+      const int64_t row0 = base + tid;
       double v[R];
       eval_f64<R>(code, len, a.K, X, ldx, row0, v);
       bool nan_tail = false;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        const int64_t i = row0 + 64 * r;
+        const int64_t i = row0 + (int64_t)CP * r;
         col[i] = v[r];
         keep[i] = v[r];
         if (i < n) {
-          s_all = s_all + v[r] * v[r];
-          if (i >= 1) s_1 = s_1 + v[r] * v[r];
+          sums[0] = sums[0] + v[r] * v[r];
+          if (i >= 1) sums[1] = sums[1] + v[r] * v[r];
           if (i >= 1 && v[r] != v[r]) nan_tail = true;
         }
       }
       if (j == 0 && __any((int)nan_tail)) {
-        const int size0 = (int)((n < (int64_t)k) ? n : (int64_t)k);
-        for (int i = lane; i < k; i += WAVE) co[i] = (i < size0) ? rrm::qnan() : 0.0;
-        if (lane == 0) write_fit(a.fit + cand, rrm::qnan(), a.sst, n, size0, 0);
-        return;
+        if ((tid & 63) == 0) abort_flag = 1;
+        break;
       }
     }
-    s_all = wave_sum(s_all);
-    s_1 = wave_sum(s_1);
-    if (lane == 0) { normD[j] = __builtin_sqrt(s_all); tsq[j] = s_1; }
+    block_sums<2>(sums, red2);   // its barriers also publish abort_flag
+    if (j == 0 && abort_flag) {
+      const int size0 = (int)((n < (int64_t)k) ? n : (int64_t)k);
+      for (int i = tid; i < k; i += CP) co[i] = (i < size0) ? rrm::qnan() : 0.0;
+      if (tid == 0) write_fit(a.fit + cand, rrm::qnan(), a.sst, n, size0, 0);
+      return;
+    }
+    if (tid == 0) { normD[j] = __builtin_sqrt(sums[0]); tsq[j] = sums[1]; }
   }
   double* const cy = S + (int64_t)k * ldx;
   {
     double* col = S + (int64_t)nf * ldx;
-    for (int64_t i = lane; i < n; i += WAVE) { col[i] = 1.0; cy[i] = a.y[i]; }
-    if (lane == 0) { normD[nf] = __builtin_sqrt((double)n); tsq[nf] = (double)(n - 1); }
+    for (int64_t i = tid; i < n; i += CP) { col[i] = 1.0; cy[i] = a.y[i]; }
+    if (tid == 0) { normD[nf] = __builtin_sqrt((double)n); tsq[nf] = (double)(n - 1); }
   }
-  for (int j = lane; j < k; j += WAVE) phys[j] = j;
+  for (int j = tid; j < k; j += CP) phys[j] = j;
   __syncthreads();
-  for (int j = lane; j < k; j += WAVE) normU[j] = normD[j];
+  for (int j = tid; j < k; j += CP) normU[j] = normD[j];
   __syncthreads();
 
   // ---- 2. column-pivoted Householder QR
@@ -169,7 +182,7 @@ __global__ __launch_bounds__(64) void k_ols_fp64(Fp64Args a) {
     for (int j = p + 1; j < k; ++j) { const double v = normU[j]; if (v > best) { best = v; bi = j; } }
     if (best * best < thr * (double)(n - p)) { nonzero = p; break; }
     __syncthreads();
-    if (lane == 0 && bi != p) {
+    if (tid == 0 && bi != p) {
       int ti = phys[p]; phys[p] = phys[bi]; phys[bi] = ti;
       double t;
       t = normU[p]; normU[p] = normU[bi]; normU[bi] = t;
@@ -186,7 +199,7 @@ __global__ __launch_bounds__(64) void k_ols_fp64(Fp64Args a) {
     if (zero_tail) {
       tau = 0.0;
       beta = c0;
-      for (int64_t i = first_row(p + 1, lane); i < n; i += WAVE) v[i] = 0.0;
+      for (int64_t i = first_row(p + 1, tid); i < n; i += CP) v[i] = 0.0;
     } else {
       beta = __builtin_sqrt(c0 * c0 + tailSq);
       if (c0 >= 0.0) beta = -beta;
@@ -207,20 +220,20 @@ __global__ __launch_bounds__(64) void k_ols_fp64(Fp64Args a) {
     }
     const double den = c0 - beta;
     const bool apply = tau != 0.0;         // applyHouseholderOnTheLeft does nothing for tau == 0
-    __syncthreads();   // every lane has read c0 before row p is overwritten
-    if (lane == 0) v[p] = beta;
+    __syncthreads();   // every thread has read c0 before row p is overwritten
+    if (tid == 0) v[p] = beta;
 
     const int nrem = k - 1 - p;            // remaining A columns: positions p+1..k-1
     const int ntarget = nrem + 1;          // plus the y column (last target)
     ++steps;
     if (n - p == 1) {
       // applyHouseholderOnTheLeft with a single row: scale by (1 - tau)
-      if (lane == 0) {
+      if (tid == 0) {
         for (int t = 0; t < nrem; ++t) { double* col = S + (int64_t)phys[p + 1 + t] * ldx; col[p] *= (1.0 - tau); }
         cy[p] *= (1.0 - tau);
       }
       __syncthreads();
-      for (int t = 0; t < nrem; ++t) if (lane == 0) tsq[p + 1 + t] = 0.0;
+      for (int t = 0; t < nrem; ++t) if (tid == 0) tsq[p + 1 + t] = 0.0;
       // the LAWN-176 recomputation below needs sums over rows p+1.. : empty
       for (int t = 0; t < nrem; ++t) {
         const int j = p + 1 + t;
@@ -232,7 +245,7 @@ __global__ __launch_bounds__(64) void k_ols_fp64(Fp64Args a) {
           const double ratio = normU[j] / normD[j];
           const double temp2 = temp * (ratio * ratio);
           __syncthreads();
-          if (lane == 0) {
+          if (tid == 0) {
             if (temp2 <= dd_thr) { normD[j] = 0.0; normU[j] = 0.0; }
             else normU[j] *= __builtin_sqrt(temp);
           }
@@ -259,12 +272,12 @@ __global__ __launch_bounds__(64) void k_ols_fp64(Fp64Args a) {
         double acc[G];
 #pragma unroll
         for (int t = 0; t < G; ++t) acc[t] = 0.0;
-        for (int64_t base = t0; base < n; base += 64 * R) {
+        for (int64_t base = t0; base < n; base += CP * R) {
           double e[R];
           bool ok[R];
 #pragma unroll
           for (int r = 0; r < R; ++r) {
-            const int64_t i = base + lane + 64 * r;
+            const int64_t i = base + tid + (int64_t)CP * r;
             ok[r] = i > p && i < n;
             if (g0 == 0) { e[r] = v[i] / den; if (ok[r]) v[i] = e[r]; } else { e[r] = v[i]; }
           }
@@ -273,37 +286,36 @@ __global__ __launch_bounds__(64) void k_ols_fp64(Fp64Args a) {
             if (t < gn) {
 #pragma unroll
               for (int r = 0; r < R; ++r) {
-                const double x = cols[t][base + lane + 64 * r];
+                const double x = cols[t][base + tid + (int64_t)CP * r];
                 if (ok[r]) acc[t] = acc[t] + e[r] * x;
               }
             }
           }
         }
+        block_sums<G>(acc, redG);
 #pragma unroll
         for (int t = 0; t < G; ++t) {
           if (t < gn) {
-            double s = wave_sum(acc[t]);
             const double ap = cols[t][p];
-            s = s + ap;
-            tdot[t] = s;
+            tdot[t] = acc[t] + ap;
           }
         }
         __syncthreads();
-        if (lane == 0) {
+        if (tid == 0) {
 #pragma unroll
           for (int t = 0; t < G; ++t) if (t < gn) cols[t][p] = cols[t][p] - tau * tdot[t];
         }
       }
       // apply the reflector and accumulate tail norms (rows >= p+1 and >= p+2)
-      double s1[G], s2[G];
+      double s12[2 * G];
 #pragma unroll
-      for (int t = 0; t < G; ++t) { s1[t] = 0.0; s2[t] = 0.0; }
-      for (int64_t base = t0; base < n; base += 64 * R) {
+      for (int t = 0; t < 2 * G; ++t) s12[t] = 0.0;
+      for (int64_t base = t0; base < n; base += CP * R) {
         double te[R];
         bool ok[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-          const int64_t i = base + lane + 64 * r;
+          const int64_t i = base + tid + (int64_t)CP * r;
           ok[r] = i > p && i < n;
           te[r] = apply ? tau * v[i] : 0.0;
         }
@@ -312,24 +324,23 @@ __global__ __launch_bounds__(64) void k_ols_fp64(Fp64Args a) {
           if (t < gn) {
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-              const int64_t i = base + lane + 64 * r;
+              const int64_t i = base + tid + (int64_t)CP * r;
               double x = cols[t][i];
               if (ok[r]) {
                 if (apply) { x = x - te[r] * tdot[t]; cols[t][i] = x; }
-                s1[t] = s1[t] + x * x;
-                if (i >= p + 2) s2[t] = s2[t] + x * x;
+                s12[t] = s12[t] + x * x;
+                if (i >= p + 2) s12[G + t] = s12[G + t] + x * x;
               }
             }
           }
         }
       }
+      block_sums<2 * G>(s12, redS);
+      if (tid == 0) {
 #pragma unroll
-      for (int t = 0; t < G; ++t) {
-        if (t < gn) {
-          const double a1 = wave_sum(s1[t]);
-          const double a2 = wave_sum(s2[t]);
+        for (int t = 0; t < G; ++t) {
           const int tg = g0 + t;
-          if (tg < nrem && lane == 0) { cc[p + 1 + tg] = a1; tsq[p + 1 + tg] = a2; }
+          if (t < gn && tg < nrem) { cc[p + 1 + tg] = s12[t]; tsq[p + 1 + tg] = s12[G + t]; }
         }
       }
       __syncthreads();
@@ -347,7 +358,7 @@ __global__ __launch_bounds__(64) void k_ols_fp64(Fp64Args a) {
         const double temp2 = temp * (ratio * ratio);
         const double rec = __builtin_sqrt(cc[j]);
         __syncthreads();
-        if (lane == 0) {
+        if (tid == 0) {
           if (temp2 <= dd_thr) { normD[j] = rec; normU[j] = rec; }
           else normU[j] = nu * __builtin_sqrt(temp);
         }
@@ -358,20 +369,20 @@ __global__ __launch_bounds__(64) void k_ols_fp64(Fp64Args a) {
 
   const int r = nonzero;
   if (nan_exit >= 0) {
-    for (int i = lane; i < k; i += WAVE) co[phys[i]] = (i < r) ? rrm::qnan() : 0.0;
-    if (lane == 0) write_fit(a.fit + cand, rrm::qnan(), a.sst, n, r, steps);
+    for (int i = tid; i < k; i += CP) co[phys[i]] = (i < r) ? rrm::qnan() : 0.0;
+    if (tid == 0) write_fit(a.fit + cand, rrm::qnan(), a.sst, n, r, steps);
     return;
   }
 
   // ---- 3. back substitution on the leading nonzero x nonzero block
-  for (int i = lane; i < r; i += WAVE) cc[i] = cy[i];
+  for (int i = tid; i < r; i += CP) cc[i] = cy[i];
   __syncthreads();
   for (int i = r - 1; i >= 0; --i) {
     const double* coli = S + (int64_t)phys[i] * ldx;
     const double ci = cc[i] / coli[i];
     __syncthreads();
-    for (int l = lane; l < i; l += WAVE) cc[l] = cc[l] - ci * coli[l];
-    if (lane == 0) cc[i] = ci;
+    for (int l = tid; l < i; l += CP) cc[l] = cc[l] - ci * coli[l];
+    if (tid == 0) cc[i] = ci;
     __syncthreads();
   }
   bool any_nan = false;
@@ -379,22 +390,22 @@ __global__ __launch_bounds__(64) void k_ols_fp64(Fp64Args a) {
     const double cv = (i < r) ? cc[i] : 0.0;
     if (cv != cv) any_nan = true;
   }
-  for (int i = lane; i < k; i += WAVE) {
+  for (int i = tid; i < k; i += CP) {
     double cv = (i < r) ? cc[i] : 0.0;
     if (cv != cv) cv = rrm::qnan();
     co[phys[i]] = cv;
   }
   __syncthreads();
   if (any_nan) {   // a kept NaN coefficient: every prediction is NaN
-    if (lane == 0) write_fit(a.fit + cand, rrm::qnan(), a.sst, n, r, steps);
+    if (tid == 0) write_fit(a.fit + cand, rrm::qnan(), a.sst, n, r, steps);
     return;
   }
 
   // ---- 4. prediction of the OLS tree and fitness
   const double cf_free = co[nf];
-  double ssr = 0.0;
-  for (int64_t base = 0; base < n; base += 64 * R) {
-    const int64_t row0 = base + lane;
+  double ssr[1] = {0.0};
+  for (int64_t base = 0; base < n; base += CP * R) {
+    const int64_t row0 = base + tid;
     double yp[R];
     bool any = false;
 #pragma unroll
@@ -405,7 +416,7 @@ __global__ __launch_bounds__(64) void k_ols_fp64(Fp64Args a) {
       const double* keep = S + (int64_t)(k + 1 + j) * ldx;
       double f[R];
 #pragma unroll
-      for (int q = 0; q < R; ++q) f[q] = keep[row0 + 64 * q];
+      for (int q = 0; q < R; ++q) f[q] = keep[row0 + (int64_t)CP * q];
       const bool bare = __builtin_fabs(cj - 1) < EPS12;
 #pragma unroll
       for (int q = 0; q < R; ++q) {
@@ -421,15 +432,15 @@ __global__ __launch_bounds__(64) void k_ols_fp64(Fp64Args a) {
     }
 #pragma unroll
     for (int q = 0; q < R; ++q) {
-      const int64_t i = row0 + 64 * q;
+      const int64_t i = row0 + (int64_t)CP * q;
       if (i < n) {
         const double d = a.y[i] - yp[q];
-        ssr = ssr + d * d;
+        ssr[0] = ssr[0] + d * d;
       }
     }
   }
-  ssr = wave_sum(ssr);
-  if (lane == 0) write_fit(a.fit + cand, ssr, a.sst, n, r, steps);
+  block_sums<1>(ssr, red1);
+  if (tid == 0) write_fit(a.fit + cand, ssr[0], a.sst, n, r, steps);
 }
 
 struct RawArgs {
@@ -449,29 +460,30 @@ struct RawArgs {
 // fitness() of a tree as given (no OLS): main-loop perturbations and the
 // re-scores after acceptance (rils_rols_cpp.cpp:631, 800, 828, 842).
 template <int R>
-__global__ __launch_bounds__(64) void k_raw_fp64(RawArgs a) {
+__global__ __launch_bounds__(CP) void k_raw_fp64(RawArgs a) {
+  __shared__ double red1[CW][1];
   const int cand = (int)blockIdx.x;
   bind_dataset(a, cand);
-  const int lane = (int)threadIdx.x;
+  const int tid = (int)threadIdx.x;
   const int p = a.cand_start[cand];
   const uint32_t* code = a.code + a.prog_start[p];
   const int len = a.prog_start[p + 1] - a.prog_start[p];
-  double ssr = 0.0;
-  for (int64_t base = 0; base < a.n; base += 64 * R) {
-    const int64_t row0 = base + lane;
+  double ssr[1] = {0.0};
+  for (int64_t base = 0; base < a.n; base += CP * R) {
+    const int64_t row0 = base + tid;
     double yp[R];
     eval_f64<R>(code, len, a.K, a.X, a.ldx, row0, yp);
 #pragma unroll
     for (int q = 0; q < R; ++q) {
-      const int64_t i = row0 + 64 * q;
+      const int64_t i = row0 + (int64_t)CP * q;
       if (i < a.n) {
         const double d = a.y[i] - yp[q];
-        ssr = ssr + d * d;
+        ssr[0] = ssr[0] + d * d;
       }
     }
   }
-  ssr = wave_sum(ssr);
-  if (lane == 0) write_fit(a.fit + cand, ssr, a.sst, a.n, 0);
+  block_sums<1>(ssr, red1);
+  if (tid == 0) write_fit(a.fit + cand, ssr[0], a.sst, a.n, 0);
 }
 
 // node::evaluate_all over all samples; many waves, any order (no reduction)
@@ -483,20 +495,20 @@ __global__ __launch_bounds__(256) void k_predict_fp64(const uint32_t* code, int 
   if (i < n) out[i] = v[0];
 }
 
-// mean(y) and sum((y - mean)^2), canonical order (R2, rils_rols_cpp.cpp:40-45)
 // out4: mean of y, sum (y - mean)^2, sum y (canonical sums), sum y^2 (any
 // order: used by the FP32 form only)
-__global__ __launch_bounds__(64) void k_stats_fp64(const double* y, int64_t n, double* out4) {
-  const int lane = (int)threadIdx.x;
-  double s = 0.0, yy = 0.0;
-  for (int64_t i = lane; i < n; i += WAVE) { s = s + y[i]; yy = __builtin_fma(y[i], y[i], yy); }
-  s = wave_sum(s);
-  yy = wave_sum(yy);
-  const double ybar = s / (double)n;
-  double q = 0.0;
-  for (int64_t i = lane; i < n; i += WAVE) q = q + (y[i] - ybar) * (y[i] - ybar);
-  q = wave_sum(q);
-  if (lane == 0) { out4[0] = ybar; out4[1] = q; out4[2] = s; out4[3] = yy; }
+__global__ __launch_bounds__(CP) void k_stats_fp64(const double* y, int64_t n, double* out4) {
+  __shared__ double red2[CW][2];
+  __shared__ double red1[CW][1];
+  const int tid = (int)threadIdx.x;
+  double s[2] = {0.0, 0.0};
+  for (int64_t i = tid; i < n; i += CP) { s[0] = s[0] + y[i]; s[1] = __builtin_fma(y[i], y[i], s[1]); }
+  block_sums<2>(s, red2);
+  const double ybar = s[0] / (double)n;
+  double q[1] = {0.0};
+  for (int64_t i = tid; i < n; i += CP) q[0] = q[0] + (y[i] - ybar) * (y[i] - ybar);
+  block_sums<1>(q, red1);
+  if (tid == 0) { out4[0] = ybar; out4[1] = q[0]; out4[2] = s[0]; out4[3] = s[1]; }
 }
 
 }  // namespace
@@ -506,7 +518,7 @@ hipError_t launch_ols_fp64(const OlsLaunch& L, hipStream_t s) {
   a.cand_start = L.cand_start; a.prog_start = L.prog_start; a.code = L.code; a.K = L.K;
   a.ds = L.ds; a.cand_ds = L.cand_ds;
   a.slab = L.slab; a.slab_off = L.slab_off; a.coefs = L.coefs; a.fit = L.fit; a.cand_base = L.cand_base;
-  const dim3 grid(L.n_cands), block(64);
+  const dim3 grid(L.n_cands), block(CP);
   const int R = L.rows;
   if (L.max_cols <= 16) {
     if (R == 1) hipLaunchKernelGGL((k_ols_fp64<16, 1>), grid, block, 0, s, a);
@@ -524,9 +536,9 @@ hipError_t launch_raw_fp64(const RawLaunch& L, hipStream_t s) {
   RawArgs a;
   a.cand_start = L.cand_start; a.prog_start = L.prog_start; a.code = L.code; a.K = L.K;
   a.ds = L.ds; a.cand_ds = L.cand_ds; a.fit = L.fit;
-  if (L.rows == 1) hipLaunchKernelGGL(k_raw_fp64<1>, dim3(L.n_cands), dim3(64), 0, s, a);
-  else if (L.rows == 2) hipLaunchKernelGGL(k_raw_fp64<2>, dim3(L.n_cands), dim3(64), 0, s, a);
-  else hipLaunchKernelGGL(k_raw_fp64<4>, dim3(L.n_cands), dim3(64), 0, s, a);
+  if (L.rows == 1) hipLaunchKernelGGL(k_raw_fp64<1>, dim3(L.n_cands), dim3(CP), 0, s, a);
+  else if (L.rows == 2) hipLaunchKernelGGL(k_raw_fp64<2>, dim3(L.n_cands), dim3(CP), 0, s, a);
+  else hipLaunchKernelGGL(k_raw_fp64<4>, dim3(L.n_cands), dim3(CP), 0, s, a);
   return hipGetLastError();
 }
 
@@ -538,7 +550,7 @@ hipError_t launch_predict_fp64(const uint32_t* code, int len, const double* K, c
 }
 
 hipError_t launch_stats_fp64(const double* y, int64_t n, double* out4, hipStream_t s) {
-  hipLaunchKernelGGL(k_stats_fp64, dim3(1), dim3(64), 0, s, y, n, out4);
+  hipLaunchKernelGGL(k_stats_fp64, dim3(1), dim3(CP), 0, s, y, n, out4);
   return hipGetLastError();
 }
 

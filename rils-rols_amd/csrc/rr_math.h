@@ -19,7 +19,7 @@
 //          fma-exact double-double subtraction for |x| < 2^28; a Payne-Hanek
 //          reduction against a 1408-bit table of 2/pi for larger |x|; Taylor
 //          polynomials of degree 17 (sin) / 18 (cos) on |r| <= pi/4.
-// Accuracy against mpmath is checked by tests/test_rr_math.py.
+// Accuracy against mpmath is checked by tests/test_oracle.py.
 //
 // The includer defines RR_HD as the function qualifiers for its compiler
 // (empty for g++, "__host__ __device__" for hipcc).
@@ -41,6 +41,33 @@ RR_HD inline uint64_t d2bits(double d) { return __builtin_bit_cast(uint64_t, d);
 // compilers return the same pattern.
 RR_HD inline double qnan() { return bits2d(0xfff8000000000000ULL); }
 RR_HD inline double pinf() { return bits2d(0x7ff0000000000000ULL); }
+
+// The canonical order of every sum over samples in the FP64 form: 1024
+// partial sums, row i added to partial (i mod 1024) in increasing i; the
+// partials form 16 groups of 64 (one wavefront each on the GPU), each group
+// is combined by an xor butterfly at distances 32, 16, 8, 4, 2, 1, and the 16
+// group sums are added in group order.  canon_sum below is the host form; the
+// kernels realise the same order with 16-wave workgroups (rr_device.h).
+constexpr int CANON_GROUPS = 16;
+constexpr int CANON_PARTIALS = 64 * CANON_GROUPS;
+
+template <class F>
+inline double canon_sum(int64_t lo, int64_t hi, F f) {
+  double acc[CANON_PARTIALS];
+  for (int l = 0; l < CANON_PARTIALS; ++l) acc[l] = 0.0;
+  for (int64_t i = lo; i < hi; ++i) acc[i & (CANON_PARTIALS - 1)] = acc[i & (CANON_PARTIALS - 1)] + f(i);
+  double total = 0.0;
+  for (int g = 0; g < CANON_GROUPS; ++g) {
+    double* a = acc + 64 * g;
+    for (int s = 32; s >= 1; s >>= 1) {
+      double nx[64];
+      for (int l = 0; l < 64; ++l) nx[l] = a[l] + a[l ^ s];
+      for (int l = 0; l < 64; ++l) a[l] = nx[l];
+    }
+    total = (g == 0) ? a[0] : total + a[0];
+  }
+  return total;
+}
 
 // 2^k for -1022 <= k <= 1023 (exact).
 RR_HD inline double pow2i(int k) { return bits2d((uint64_t)(k + 1023) << 52); }

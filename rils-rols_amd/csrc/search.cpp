@@ -1,6 +1,9 @@
 // search.cpp -- RILS-ROLS search driver with batched GPU fitness (see search.h).
 #include "search.h"
 
+#define RR_HD
+#include "rr_math.h"
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -623,20 +626,11 @@ Flat RilsRols::local_search(const Flat& passed, Fit* out_fit) {   // 601-643
   return curr;
 }
 
-// canonical-order sum (64 partials, row i to partial i mod 64, xor butterfly)
+// canonical-order sum of the FP64 form (rrm::canon_sum)
 static double canon_sum_host(const double* v, int64_t n, const double* w = nullptr, double wshift = 0.0) {
-  double acc[64];
-  for (int l = 0; l < 64; ++l) acc[l] = 0.0;
-  for (int64_t i = 0; i < n; ++i) {
-    const double t = w ? (v[i] - w[i]) * (v[i] - w[i]) : (v[i] - wshift) * (v[i] - wshift);
-    acc[i & 63] = acc[i & 63] + t;
-  }
-  for (int s = 32; s >= 1; s >>= 1) {
-    double nx[64];
-    for (int l = 0; l < 64; ++l) nx[l] = acc[l] + acc[l ^ s];
-    for (int l = 0; l < 64; ++l) acc[l] = nx[l];
-  }
-  return acc[0];
+  return rrm::canon_sum(0, n, [&](int64_t i) {
+    return w ? (v[i] - w[i]) * (v[i] - w[i]) : (v[i] - wshift) * (v[i] - wshift);
+  });
 }
 
 // 753-770: with more than 200 features keep the 200 with the largest R2(X[i], y)
@@ -648,11 +642,7 @@ std::vector<int> RilsRols::relevant_features(const std::vector<std::vector<doubl
   std::vector<std::tuple<double, int>> by;
   for (int i = 0; i < d; ++i) {
     const double* x = X[i].data();
-    double acc[64];
-    for (int l = 0; l < 64; ++l) acc[l] = 0.0;
-    for (int64_t r = 0; r < n; ++r) acc[r & 63] = acc[r & 63] + x[r];
-    for (int s = 32; s >= 1; s >>= 1) { double nx[64]; for (int l = 0; l < 64; ++l) nx[l] = acc[l] + acc[l ^ s]; for (int l = 0; l < 64; ++l) acc[l] = nx[l]; }
-    const double xbar = acc[0] / (double)n;
+    const double xbar = rrm::canon_sum(0, n, [&](int64_t r) { return x[r]; }) / (double)n;
     const double ssr = canon_sum_host(x, n, y.data());
     const double sst = canon_sum_host(x, n, nullptr, xbar);
     by.push_back(std::tuple<double, int>{1 - ssr / sst, i});
