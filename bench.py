@@ -120,6 +120,8 @@ def fp64_algorithmic_bytes(k, steps, sentinel, n):
     the score pass (skipped for sentinels) reads y and the kept factor
     columns.  Features are read from
     cache-resident X and not counted."""
+    if sentinel and steps == 0:
+        return 0.0   # stopped in the evaluation pass at factor 0's first NaN tile
     words = (k + 1) + (k - 1) + 1
     for p in range(steps):
         t = k - p
@@ -317,7 +319,7 @@ def main():
             "p99_step_ms": float(np.percentile(r["lat"], 99) * 1e3),
             "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                          "frac": achieved / PEAK_FP32_TFLOPS, "traffic": pmc_traffic("k_ols_fp32_nf1"),
-                         "kernel": "k_acc_fp32<1,%d> + k_fin_fp32<1>" % (args.fp32_rows or 1), "launch_ms": t1,
+                         "kernel": "k_acc_fp32<1,%d> + k_fin_fp32<1>" % (args.fp32_rows or 2), "launch_ms": t1,
                          "candidates": int(len(one)), "algorithmic_flops": f1},
             "batch_valu_tflops": flops / (r["kms"].mean() / 1e3) / 1e12,
             "needs_fp64_frac": float(np.mean((fl32 & 2) > 0)),

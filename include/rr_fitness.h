@@ -185,9 +185,9 @@ int rr_predict_fp32(rr_ctx* ctx, int32_t handle, const uint32_t* code, int32_t l
  * Named integer options: "slab_budget_mb" (device memory the FP64 OLS path
  * may use for its per-candidate work columns, default 32768), "fp64_rows"
  * (samples per lane per tile of the FP64 kernels: 1, 2 or 4; default 2),
- * "fp32_rows" (the same for the FP32 kernels: 1 or 2; default 1),
+ * "fp32_rows" (the same for the FP32 kernels: 1 or 2; default 2),
  * "fp32_chunk_rows" (samples per workgroup of the FP32 pass, a multiple of
- * 512; default 8192: a candidate's samples are split into chunks that run as
+ * 512; default 4096: a candidate's samples are split into chunks that run as
  * independent workgroups, and the chunk sums are added in chunk order),
  * "fp32_streams" (1: the FP32 factor-count buckets run on concurrent streams;
  * default 0, back to back on the context's stream).

@@ -93,8 +93,8 @@ struct rr_ctx {
   HostBuf h_stage, h_out;
   int64_t slab_budget_mb = 32768;
   int64_t fp64_rows = 2;
-  int64_t fp32_rows = 1;
-  int64_t fp32_chunk_rows = 8192;
+  int64_t fp32_rows = 2;
+  int64_t fp32_chunk_rows = 4096;
   int64_t fp32_streams = 0;   // 1: bucket kernels on concurrent side streams
   double last_ms = 0.0;
 };
