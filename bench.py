@@ -223,6 +223,82 @@ def copy_probe():
     print(json.dumps({"copy_probe_bytes_read": nbytes, "copy_probe_bytes_written": nbytes, "launches": 5}))
 
 
+def serving_workload(n_runs=64, rate=4.0, seed=31337):
+    """The serving mix: n_runs independent regression runs arriving as a Poisson
+    stream (rate runs/s); per run n log-uniform in [1e3, 1e6], features uniform
+    in [2, 20], target a random 5-20-node tree plus 1% noise (sigma = 0.01 RMS(y),
+    the reference's noisefy convention, rils_rols/utils.py:13-24); a quarter of
+    the runs are binary classification (target thresholded at its median)."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    runs = []
+    cls_idx = set(rng.choice(n_runs, n_runs // 4, replace=False).tolist())
+    for i in range(n_runs):
+        t += rng.exponential(1.0 / rate)
+        n = int(round(10 ** rng.uniform(3, 6)))
+        d = int(rng.integers(2, 21))
+        X = rng.uniform(-3, 3, (n, d))
+        while True:
+            tgt = gen_tree(rng, int(rng.integers(5, 21)), d)
+            yt = eval_prefix(tgt, X)
+            if np.all(np.isfinite(yt)) and np.std(yt) > 1e-6 and np.max(np.abs(yt)) < 1e8:
+                break
+        y = yt + rng.normal(0, 0.01 * np.sqrt(np.mean(yt ** 2)), n)
+        if i in cls_idx:
+            y = (y > np.median(y)).astype(np.float64)
+        runs.append(dict(arrival=t, X=X, y=y, classification=i in cls_idx, seed=1000 + i, target=" ".join(tgt)))
+    return runs
+
+
+def serving_main(args):
+    import rils_rols_cpp
+    t0 = time.perf_counter()
+    runs = serving_workload(args.runs)
+    t_gen = time.perf_counter() - t0
+    srv = rils_rols_cpp.Server(0, 0, args.max_active, 1, 0)
+    start = time.perf_counter()
+    ids = []
+    for r in runs:
+        wait = r["arrival"] - (time.perf_counter() - start)
+        if wait > 0:
+            time.sleep(wait)
+        ids.append(srv.submit(r["X"], r["y"], classification=r["classification"], max_fit_calls=args.run_fit_calls,
+                              max_seconds=10 ** 6, random_state=r["seed"], precision=args.serving_precision))
+        print("[serving] submitted %d n=%d d=%d at %.2fs" % (len(ids) - 1, r["X"].shape[0], r["X"].shape[1],
+                                                            time.perf_counter() - start), file=sys.stderr, flush=True)
+    res = [srv.wait(i) for i in ids]
+    wall = time.perf_counter() - start
+    bad = [r["error"] for r in res if not r["ok"]]
+    if bad:
+        raise RuntimeError("serving runs failed: %s" % bad[:3])
+    sweeps = np.concatenate([np.asarray(r["sweep_ms"], dtype=np.float64) for r in res])
+    fit_calls = sum(r["fit_calls"] for r in res)
+    st = srv.stats()
+    lat = [r["finished_s"] - r["submitted_s"] for r in res]
+    line = {
+        "metric": "serving mix: aggregate fitness evals/s; p50/p99 step latency",
+        "value": fit_calls / wall,
+        "unit": "fitness evals/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "f64" if args.serving_precision == "fp64" else "f32+f64refine",
+        "data": "synthetic",
+        "config": {"workload": "serving mix: %d runs, Poisson arrivals 4/s, n log-uniform [1e3,1e6], d in [2,20], "
+                               "5-20-node targets + 1%% noise, 1/4 classification, %d fit calls per run"
+                               % (args.runs, args.run_fit_calls), "seed": 31337, "max_active": args.max_active},
+        "p50_step_ms": float(np.percentile(sweeps, 50)),
+        "p99_step_ms": float(np.percentile(sweeps, 99)),
+        "steps": int(len(sweeps)),
+        "wall_s": wall,
+        "fit_calls": fit_calls,
+        "run_latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+        "dispatcher": st,
+        "mean_runs_per_launch": st["requests"] / max(st["launches"], 1),
+        "host": {"workload_gen_s": t_gen},
+    }
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -236,6 +312,12 @@ def main():
     ap.add_argument("--fp32-chunk", type=int, default=None, help="samples per FP32 workgroup (multiple of 512)")
     ap.add_argument("--fp32-streams", type=int, default=None, help="1: FP32 buckets on concurrent streams")
     ap.add_argument("--copy-probe", action="store_true", help="run only a plain 1 GiB device copy (PMC scaling)")
+    ap.add_argument("--workload", default="batched", choices=["batched", "serving"],
+                    help="batched: the headline batched candidate eval; serving: the serving mix")
+    ap.add_argument("--runs", type=int, default=64, help="serving: number of runs")
+    ap.add_argument("--run-fit-calls", type=int, default=10000, help="serving: fit-call budget per run")
+    ap.add_argument("--max-active", type=int, default=64, help="serving: resident runs at most")
+    ap.add_argument("--serving-precision", default="fp64", choices=["fp64", "fp32"])
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
@@ -245,6 +327,9 @@ def main():
               "(see DESIGN.md for how the path would shard)", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "serving":
+        serving_main(args)
+        return
     import rils_rols_cpp
     t0 = time.perf_counter()
     X, y, target, trees, factors = workload()

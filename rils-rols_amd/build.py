@@ -18,7 +18,7 @@ CSRC = os.path.join(HERE, "csrc")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 HIP_SOURCES = ["rr_capi.hip", "rr_kernels_fp64.hip", "rr_kernels_fp32.hip"]
-HOST_SOURCES = ["tree.cpp", "search.cpp", "pybind_module.cpp"]
+HOST_SOURCES = ["tree.cpp", "search.cpp", "serve.cpp", "pybind_module.cpp"]
 
 # -ffp-contract=off: no fused multiply-add unless written as fma(); the FP64
 # path must produce the same bits as host code compiled the same way.

@@ -12,6 +12,7 @@
 #include <pybind11/stl.h>
 
 #include "search.h"
+#include "serve.h"
 #include "tree.h"
 
 namespace py = pybind11;
@@ -19,6 +20,44 @@ using rrh::Flat;
 using rrh::RilsRols;
 
 namespace {
+
+py::dict stats_dict(const rrh::Stats& s) {
+  py::dict d;
+  d["sweeps"] = s.sweeps; d["batched_cands"] = s.batched_cands; d["raw_batches"] = s.raw_batches;
+  d["raw_cands"] = s.raw_cands; d["single_calls"] = s.single_calls; d["rescored_fp64"] = s.rescored_fp64;
+  d["host_gen_s"] = s.host_gen_s; d["host_prep_s"] = s.host_prep_s; d["gpu_s"] = s.gpu_s;
+  d["kernel_ms"] = s.kernel_ms; d["replay_s"] = s.replay_s;
+  return d;
+}
+
+py::list moves_list(const std::vector<rrh::Move>& mv) {
+  py::list out;
+  for (const rrh::Move& m : mv) out.append(py::make_tuple(m.kind, m.fit_calls, m.e0, m.e1, m.size, m.model));
+  return out;
+}
+
+// merge hook for tests: batches given as (cand_start, prog_start, code, consts, dataset) tuples
+py::tuple merge_hook(const std::vector<std::tuple<std::vector<int32_t>, std::vector<int32_t>, std::vector<uint32_t>,
+                                                  std::vector<double>, int32_t>>& parts) {
+  std::vector<rr_batch> bs(parts.size());
+  std::vector<rrh::BatchView> views;
+  for (size_t i = 0; i < parts.size(); ++i) {
+    const auto& t = parts[i];
+    rr_batch& b = bs[i];
+    b.n_cands = (int32_t)std::get<0>(t).size() - 1;
+    b.cand_start = std::get<0>(t).data();
+    b.prog_start = std::get<1>(t).data();
+    b.code = std::get<2>(t).data();
+    b.consts = std::get<3>(t).data();
+    b.n_consts = (int32_t)std::get<3>(t).size();
+    views.push_back({&b, std::get<4>(t)});
+  }
+  std::vector<int32_t> cs, ps, hs;
+  std::vector<uint32_t> code;
+  std::vector<double> consts;
+  rrh::merge_batches(views, cs, ps, code, consts, hs);
+  return py::make_tuple(cs, ps, code, consts, hs);
+}
 
 py::array_t<double> as_f64(py::array_t<double, py::array::c_style | py::array::forcecast> a) { return a; }
 
@@ -241,23 +280,9 @@ PYBIND11_MODULE(rils_rols_cpp, m) {
       .def("get_fit_calls", &RilsRols::get_fit_calls)
       .def("get_total_time", &RilsRols::get_total_time)
       // beyond the reference
-      .def("get_moves",
-           [](const RilsRols& self) {
-             py::list out;
-             for (const rrh::Move& mv : self.moves())
-               out.append(py::make_tuple(mv.kind, mv.fit_calls, mv.e0, mv.e1, mv.size, mv.model));
-             return out;
-           })
-      .def("get_stats",
-           [](const RilsRols& self) {
-             const rrh::Stats& s = self.stats();
-             py::dict d;
-             d["sweeps"] = s.sweeps; d["batched_cands"] = s.batched_cands; d["raw_batches"] = s.raw_batches;
-             d["raw_cands"] = s.raw_cands; d["single_calls"] = s.single_calls; d["rescored_fp64"] = s.rescored_fp64;
-             d["host_gen_s"] = s.host_gen_s; d["host_prep_s"] = s.host_prep_s; d["gpu_s"] = s.gpu_s;
-             d["kernel_ms"] = s.kernel_ms; d["replay_s"] = s.replay_s;
-             return d;
-           })
+      .def("get_moves", [](const RilsRols& self) { return moves_list(self.moves()); })
+      .def("get_sweep_ms", &RilsRols::sweep_ms)
+      .def("get_stats", [](const RilsRols& self) { return stats_dict(self.stats()); })
       .def("get_model_prefix", &RilsRols::final_prefix)
       .def("set_precision",
            [](RilsRols& self, const std::string& p) {
@@ -287,6 +312,64 @@ PYBIND11_MODULE(rils_rols_cpp, m) {
       .def("code_words", &BatchRunner::code_words);
 
   // test hooks (host engine only; no GPU needed)
+  py::class_<rrh::Server>(m, "Server",
+                           "Request loop: many resident RILS-ROLS runs on one GPU, their fitness work merged into "
+                           "shared launches as it arrives, runs admitted by HBM footprint")
+      .def(py::init<int, int64_t, int, int, int64_t>(), py::arg("device") = 0, py::arg("hbm_budget_mb") = 0,
+           py::arg("max_active") = 64, py::arg("host_threads_per_run") = 1, py::arg("slab_budget_mb") = 0)
+      .def(
+          "submit",
+          [](rrh::Server& self, py::array_t<double, py::array::c_style | py::array::forcecast> X,
+             py::array_t<double, py::array::c_style | py::array::forcecast> y, bool classification, int max_fit_calls,
+             int max_seconds, double complexity_penalty, int max_complexity, double sample_size, int random_state,
+             const std::string& precision) {
+            py::buffer_info bx = X.request(), by = y.request();
+            if (bx.ndim != 2) throw std::invalid_argument("X must be a 2-d array");
+            rrh::RunSpec r;
+            r.n = (int)bx.shape[0];
+            r.d = (int)bx.shape[1];
+            r.X.assign((const double*)bx.ptr, (const double*)bx.ptr + bx.size);
+            r.y.assign((const double*)by.ptr, (const double*)by.ptr + by.size);
+            r.classification = classification;
+            r.max_fit_calls = max_fit_calls;
+            r.max_seconds = max_seconds;
+            r.complexity_penalty = complexity_penalty;
+            r.max_complexity = max_complexity;
+            r.sample_size = sample_size;
+            r.random_state = random_state;
+            if (precision == "fp64") r.precision = RR_FP64;
+            else if (precision == "fp32") r.precision = RR_FP32_REFINED;
+            else throw std::invalid_argument("precision must be 'fp64' or 'fp32'");
+            return self.submit(std::move(r));
+          },
+          py::arg("X"), py::arg("y"), py::arg("classification") = false, py::arg("max_fit_calls") = 100000,
+          py::arg("max_seconds") = 100, py::arg("complexity_penalty") = 0.001, py::arg("max_complexity") = 50,
+          py::arg("sample_size") = 1.0, py::arg("random_state") = 0, py::arg("precision") = "fp64")
+      .def("done", &rrh::Server::done)
+      .def("wait",
+           [](rrh::Server& self, int id) {
+             rrh::RunResult r;
+             {
+               py::gil_scoped_release nogil;
+               r = self.wait(id);
+             }
+             py::dict d;
+             d["id"] = r.id; d["ok"] = r.ok; d["error"] = r.error; d["model"] = r.model; d["prefix"] = r.prefix;
+             d["fit_calls"] = r.fit_calls; d["moves"] = moves_list(r.moves); d["sweep_ms"] = r.sweep_ms;
+             d["stats"] = stats_dict(r.stats); d["submitted_s"] = r.submitted_s; d["admitted_s"] = r.admitted_s;
+             d["finished_s"] = r.finished_s; d["footprint"] = r.footprint;
+             return d;
+           })
+      .def("resident_bytes", &rrh::Server::resident_bytes)
+      .def("stats", [](rrh::Server& self) {
+        const rrh::Dispatcher::Counters c = self.dispatcher_counters();
+        py::dict d;
+        d["launches"] = c.launches; d["requests"] = c.requests; d["candidates"] = c.candidates;
+        d["max_requests_per_launch"] = c.max_requests_per_launch; d["kernel_ms"] = c.kernel_ms; d["busy_s"] = c.busy_s;
+        return d;
+      })
+      .def_static("footprint", &rrh::Server::footprint);
+  m.def("_merge_batches", &merge_hook);
   m.def("_all_candidates", &candidates_hook, py::arg("prefix"), py::arg("nvars"), py::arg("classification") = false,
         py::arg("local_search") = true);
   m.def("_rewrite", &rewrite_hook);
