@@ -299,6 +299,111 @@ def serving_main(args):
     print(json.dumps(line))
 
 
+def precision_main(args):
+    """16384 trees of 5-50 nodes over n = 1e6 x 20 (seeds: trees 4242, data
+    2424): FP32 form timed, FP64 form once for the error report."""
+    import rils_rols_cpp
+    n, d, ntrees = args.c3_rows, 20, args.c3_trees
+    rng = np.random.default_rng(2424)
+    X = rng.uniform(-5, 5, (n, d))
+    while True:
+        tgt = gen_tree(rng, 15, d)
+        yt = eval_prefix(tgt, X)
+        if np.all(np.isfinite(yt)) and np.std(yt) > 1e-3 and np.max(np.abs(yt)) < 1e6:
+            break
+    y = yt + rng.normal(0, 0.01, n)
+    trng = np.random.default_rng(4242)
+    trees, factors = [], []
+    while len(trees) < ntrees:
+        t = " ".join(gen_tree(trng, int(trng.integers(5, 51)), d))
+        fs = rils_rols_cpp._ols_factors(t)
+        if len(fs) <= 8:
+            trees.append(t)
+            factors.append(fs)
+    runner = rils_rols_cpp.BatchRunner(np.ascontiguousarray(X.T), y, 0)
+    nf = np.array(runner.prepare_ols(trees))
+    total, lat, kms, coefs32, fit32, n64, f32 = timed_steps(runner, runner, "fp32", args.warmup, args.steps)
+    own = f32["fit"]
+    t64 = time.perf_counter()
+    coefs64, fit64 = runner.run("fp64")
+    t64 = time.perf_counter() - t64
+    ms64 = runner.last_kernel_ms()
+    fl32, fl64 = own[:, 4].astype(int), fit64[:, 4].astype(int)
+    ok = ((fl64 & 1) == 0) & ((fl32 & 3) == 0)
+    cs = runner.cand_start()
+    cerr = []
+    for c in np.nonzero(ok)[0]:
+        o = cs[c] + c
+        a, b = coefs64[o:o + nf[c] + 1], coefs32[o:o + nf[c] + 1]
+        cerr.append(float(np.max(np.abs(a - b) / np.maximum(np.abs(a), 1e-300))))
+    r2a, r2b = 1 - fit64[ok, 0], 1 - own[ok, 0]
+    rm = np.abs(fit64[ok, 1] - own[ok, 1]) / np.maximum(fit64[ok, 1], 1e-300)
+
+    def stats(v):
+        v = np.asarray(v)
+        return {"max": float(np.max(v)), "median": float(np.median(v))} if len(v) else None
+    line = {
+        "metric": "precision path: FP32+refine fitness evals/s at n=1e6, errors vs the FP64 form",
+        "value": ntrees * args.steps / total, "unit": "fitness evals/s", "n_gpus": 1, "higher_is_better": True,
+        "dtype": "f32+f64refine", "data": "synthetic",
+        "config": {"workload": "precision path: %d trees of 5-50 nodes, n=%d x %d, FP32 evaluation, FP64 solve + "
+                               "refinement" % (ntrees, n, d), "seeds": {"trees": 4242, "data": 2424}},
+        "ms_per_step": 1e3 * total / args.steps, "steps": args.steps,
+        "fp32_kernel_ms": float(np.mean(kms)), "fp64_launch_ms": ms64, "fp64_evals_per_s": ntrees / t64,
+        "needs_fp64_frac": float(np.mean((fl32 & 2) > 0)), "sentinel_frac": float(np.mean(fl64 & 1)),
+        "certified": int(ok.sum()),
+        "errors_vs_fp64": {"coef_rel": stats(cerr), "rmse_rel": stats(rm), "r2_abs": stats(np.abs(r2a - r2b))},
+    }
+    print(json.dumps(line))
+
+
+def search_main(args):
+    """Full RILS-ROLS search at scale: n = 1e6 x 5 uniform in [-3, 3], target a
+    seeded 15-node tree + 1% noise, max_complexity 30; the CPU oracle runs the
+    same search for a short prefix of fit calls and its rate is extrapolated."""
+    import rils_rols_cpp
+    n, d = args.c4_rows, 5
+    rng = np.random.default_rng(777)
+    X = rng.uniform(-3, 3, (n, d))
+    while True:
+        tgt = gen_tree(rng, 15, d)
+        yt = eval_prefix(tgt, X)
+        if np.all(np.isfinite(yt)) and np.std(yt) > 1e-3 and np.max(np.abs(yt)) < 1e6:
+            break
+    y = yt + rng.normal(0, 0.01 * np.sqrt(np.mean(yt ** 2)), n)
+    # verbose: the search prints its state every 10000 fit calls (rils_rols_cpp.cpp:563-579) on stdout;
+    # the JSON line stays the last line
+    rr = rils_rols_cpp.rils_rols(False, args.c4_fit_calls, 10 ** 7, 0.001, 30, 1.0, True, 12345)
+    rr.set_precision(args.search_precision)
+    t = time.perf_counter()
+    rr.fit(np.ascontiguousarray(X).reshape(-1), y, n, d)
+    wall = time.perf_counter() - t
+    pred = rr.predict(np.ascontiguousarray(X).reshape(-1), n, d)
+    r2 = 1 - np.sum((y - pred) ** 2) / np.sum((y - y.mean()) ** 2)
+    st = rr.get_stats()
+    cpu = None
+    if not args.no_cpu:
+        import oracle_lib as O
+        t = time.perf_counter()
+        want = O.fit(X, y, max_fit_calls=args.c4_cpu_calls, max_time=10 ** 7, random_state=12345, max_complexity=30)
+        cpu_s = time.perf_counter() - t
+        cpu = {"fit_calls": want["fit_calls"], "seconds": cpu_s, "evals_per_s": want["fit_calls"] / cpu_s, "cores": 1,
+               "extrapolated_full_run_s": args.c4_fit_calls / (want["fit_calls"] / cpu_s)}
+    line = {
+        "metric": "full search at scale: wall time and final R2 (n=1e6 x 5)",
+        "value": rr.get_fit_calls() / wall, "unit": "fitness evals/s", "n_gpus": 1, "higher_is_better": True,
+        "dtype": "f64" if args.search_precision == "fp64" else "f32+f64refine", "data": "synthetic",
+        "config": {"workload": "full search: n=%d x %d uniform [-3,3], 15-node target + 1%% noise, max_fit_calls=%d, "
+                               "max_complexity=30" % (n, d, args.c4_fit_calls), "seeds": {"data": 777, "search": 12345},
+                   "target": " ".join(tgt)},
+        "wall_s": wall, "fit_calls": rr.get_fit_calls(), "final_r2": float(r2), "model": rr.get_model_string(),
+        "p50_step_ms": float(np.percentile(rr.get_sweep_ms(), 50)) if len(rr.get_sweep_ms()) else None,
+        "p99_step_ms": float(np.percentile(rr.get_sweep_ms(), 99)) if len(rr.get_sweep_ms()) else None,
+        "stats": st, "cpu_prefix": cpu,
+    }
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -311,9 +416,18 @@ def main():
     ap.add_argument("--fp32-rows", type=int, default=None, help="samples per lane per tile, FP32 kernels (1, 2)")
     ap.add_argument("--fp32-chunk", type=int, default=None, help="samples per FP32 workgroup (multiple of 512)")
     ap.add_argument("--fp32-streams", type=int, default=None, help="1: FP32 buckets on concurrent streams")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default kernel schedule instead of the autotuner's pick")
     ap.add_argument("--copy-probe", action="store_true", help="run only a plain 1 GiB device copy (PMC scaling)")
-    ap.add_argument("--workload", default="batched", choices=["batched", "serving"],
-                    help="batched: the headline batched candidate eval; serving: the serving mix")
+    ap.add_argument("--workload", default="batched", choices=["batched", "serving", "precision", "search"],
+                    help="batched: the headline batched candidate eval; serving: the serving mix; precision: 16384 "
+                         "trees at n=1e6 x 20 in FP32; search: a full search at n=1e6 x 5")
+    ap.add_argument("--c3-rows", type=int, default=1_000_000)
+    ap.add_argument("--c3-trees", type=int, default=16384)
+    ap.add_argument("--c4-rows", type=int, default=1_000_000)
+    ap.add_argument("--c4-fit-calls", type=int, default=1_000_000)
+    ap.add_argument("--c4-cpu-calls", type=int, default=300)
+    ap.add_argument("--search-precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--runs", type=int, default=64, help="serving: number of runs")
     ap.add_argument("--run-fit-calls", type=int, default=10000, help="serving: fit-call budget per run")
     ap.add_argument("--max-active", type=int, default=64, help="serving: resident runs at most")
@@ -329,6 +443,12 @@ def main():
 
     if args.workload == "serving":
         serving_main(args)
+        return
+    if args.workload == "precision":
+        precision_main(args)
+        return
+    if args.workload == "search":
+        search_main(args)
         return
     import rils_rols_cpp
     t0 = time.perf_counter()
@@ -352,7 +472,13 @@ def main():
     flops = float(np.sum(flops_c))
 
     res = {}
+    tuned = {}
     for prec in (["fp64", "fp32"] if args.precision == "both" else [args.precision]):
+        if not args.no_autotune:
+            # the schedule (rows per thread, FP32 chunk size, bucket streams) is picked by the
+            # library's autotuner on this batch, outside the timed region
+            tuned[prec] = runner.autotune(prec, 2)
+            print("[bench] autotune %s: %s" % (prec, tuned[prec]), file=sys.stderr, flush=True)
         total, lat, kms, coefs, fit, n64, f32 = timed_steps(runner, runner, prec, args.warmup, args.steps)
         res[prec] = dict(total=total, lat=lat, kms=kms, coefs=coefs, fit=fit, n64=n64, f32=f32)
         print("[bench] %s: %.3f ms/step wall, %.3f ms kernel, %.0f evals/s" %
@@ -375,7 +501,8 @@ def main():
             "p99_step_ms": float(np.percentile(r["lat"], 99) * 1e3),
             "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64_VALU_TFLOPS, "unit": "TFLOP/s",
                          "frac": achieved / PEAK_FP64_VALU_TFLOPS, "traffic": pmc_traffic("k_ols_fp64"),
-                         "kernel": "k_ols_fp64<16,%d>" % (args.fp64_rows or 2), "launch_ms": float(r["kms"].mean()),
+                         "kernel": "k_ols_fp64<16,%d>" % runner.get_option("fp64_rows"),
+                         "launch_ms": float(r["kms"].mean()),
                          "algorithmic_flops": flops},
             "hbm_algorithmic_GBps": byts / kernel_s / 1e9,
             "hbm_algorithmic_bytes": byts,
@@ -404,7 +531,7 @@ def main():
             "p99_step_ms": float(np.percentile(r["lat"], 99) * 1e3),
             "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                          "frac": achieved / PEAK_FP32_TFLOPS, "traffic": pmc_traffic("k_ols_fp32_nf1"),
-                         "kernel": "k_acc_fp32<1,%d> + k_fin_fp32<1>" % (args.fp32_rows or 2), "launch_ms": t1,
+                         "kernel": "k_acc_fp32<1,%d> + k_fin_fp32<1>" % runner.get_option("fp32_rows"), "launch_ms": t1,
                          "candidates": int(len(one)), "algorithmic_flops": f1},
             "batch_valu_tflops": flops / (r["kms"].mean() / 1e3) / 1e12,
             "needs_fp64_frac": float(np.mean((fl32 & 2) > 0)),
@@ -469,6 +596,10 @@ def main():
             "over_cpu_one_core": h["evals_per_s"] / cpu1["value"] if cpu1 else None,
         },
         "forms": out,
+        "schedule": {"autotuned": tuned, "fp64_rows": runner.get_option("fp64_rows"),
+                     "fp32_rows": runner.get_option("fp32_rows"),
+                     "fp32_chunk_rows": runner.get_option("fp32_chunk_rows"),
+                     "fp32_streams": runner.get_option("fp32_streams")},
         "host": {"workload_gen_s": t_gen, "host_prepare_s": t_prep, "mean_k": float(ks.mean()),
                  "mean_ops": float(E.mean())},
     }

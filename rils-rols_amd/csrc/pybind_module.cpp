@@ -229,6 +229,20 @@ class BatchRunner {
     return py::make_tuple(coefs, fit);
   }
   double last_kernel_ms() { double ms = 0; rr_last_kernel_ms(ctx_, &ms); return ms; }
+  // the autotuner on the prepared batch; returns the best kernel time and the chosen options
+  py::dict autotune(const std::string& precision, int reps) {
+    const int pr = precision == "fp32" ? RR_FP32_REFINED : RR_FP64;
+    rr_batch b{(int32_t)cand_.size() - 1, cand_.data(), prog_.data(), code_.data(), consts_.data(), (int32_t)consts_.size()};
+    double ms = 0;
+    {
+      py::gil_scoped_release nogil;
+      check(rr_autotune(ctx_, h_, &b, mode_, pr, reps, &ms), "rr_autotune");
+    }
+    py::dict d;
+    d["best_ms"] = ms;
+    for (const char* k : {"fp64_rows", "fp32_rows", "fp32_chunk_rows", "fp32_streams"}) d[k] = get_option(k);
+    return d;
+  }
   void set_option(const std::string& k, int64_t v) { check(rr_set_option(ctx_, k.c_str(), v), "rr_set_option"); }
   int64_t get_option(const std::string& k) { int64_t v = 0; check(rr_get_option(ctx_, k.c_str(), &v), "rr_get_option"); return v; }
   std::vector<int> cand_start() const { return std::vector<int>(cand_.begin(), cand_.end()); }
@@ -292,6 +306,7 @@ PYBIND11_MODULE(rils_rols_cpp, m) {
            })
       .def("set_device", &RilsRols::set_device)
       .def("set_slab_budget_mb", &RilsRols::set_slab_budget_mb)
+      .def("set_autotune", &RilsRols::set_autotune)
       .def("set_host_threads", &RilsRols::set_host_threads,
            "host threads for the per-candidate rewrites of a sweep (0: min(hardware, 8))");
 
@@ -307,6 +322,7 @@ PYBIND11_MODULE(rils_rols_cpp, m) {
       .def("run_selected", &BatchRunner::run_selected, py::arg("precision") = "fp64")
       .def("last_kernel_ms", &BatchRunner::last_kernel_ms)
       .def("set_option", &BatchRunner::set_option)
+      .def("autotune", &BatchRunner::autotune, py::arg("precision"), py::arg("reps") = 3)
       .def("get_option", &BatchRunner::get_option)
       .def("cand_start", &BatchRunner::cand_start)
       .def("code_words", &BatchRunner::code_words);

@@ -92,7 +92,7 @@ struct rr_ctx {
   DevBuf d_blk_cand, d_blk_chunk, d_chunk_base, d_work, d_work_flags;   // FP32 chunk map and partials
   HostBuf h_stage, h_out;
   int64_t slab_budget_mb = 32768;
-  int64_t fp64_rows = 2;
+  int64_t fp64_rows = 4;
   int64_t fp32_rows = 2;
   int64_t fp32_chunk_rows = 4096;
   int64_t fp32_streams = 0;   // 1: bucket kernels on concurrent side streams
@@ -612,6 +612,47 @@ int rr_predict_fp32(rr_ctx* ctx, int32_t handle, const uint32_t* code, int32_t l
   float ms = 0.f;
   RR_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
   ctx->last_ms = ms;
+  return RR_OK;
+}
+
+int rr_autotune(rr_ctx* ctx, int32_t handle, const rr_batch* b, int32_t mode, int32_t precision, int32_t reps,
+                double* best_ms) {
+  if (!ctx || !b) return fail(RR_E_ARG, "null argument");
+  if (reps < 1) reps = 1;
+  if (b->n_cands <= 0) return fail(RR_E_ARG, "autotune needs a non-empty batch");
+  const int32_t np = b->cand_start[b->n_cands];
+  std::vector<double> coefs((size_t)np + (size_t)b->n_cands + 1);
+  std::vector<rr_fit> fit((size_t)b->n_cands);
+  struct Cfg { int64_t a, b2, c; };
+  std::vector<Cfg> cfgs;
+  if (precision == RR_FP64) {
+    for (int64_t r : {1, 2, 4}) cfgs.push_back({r, 0, 0});
+  } else {
+    for (int64_t r : {1, 2})
+      for (int64_t ch : {2048, 4096, 8192, 16384})
+        for (int64_t st : {0, 1}) cfgs.push_back({r, ch, st});
+  }
+  const Cfg keep = precision == RR_FP64 ? Cfg{ctx->fp64_rows, 0, 0}
+                                         : Cfg{ctx->fp32_rows, ctx->fp32_chunk_rows, ctx->fp32_streams};
+  auto apply = [&](const Cfg& c) {
+    if (precision == RR_FP64) ctx->fp64_rows = c.a;
+    else { ctx->fp32_rows = c.a; ctx->fp32_chunk_rows = c.b2; ctx->fp32_streams = c.c; }
+  };
+  double best = 1e300;
+  Cfg win = keep;
+  for (const Cfg& c : cfgs) {
+    apply(c);
+    int rc = rr_fitness_batch(ctx, handle, b, mode, precision, coefs.data(), fit.data());   // warm-up
+    if (rc) { apply(keep); return rc; }
+    double t = 1e300;
+    for (int32_t i = 0; i < reps; ++i) {
+      if ((rc = rr_fitness_batch(ctx, handle, b, mode, precision, coefs.data(), fit.data()))) { apply(keep); return rc; }
+      t = std::min(t, ctx->last_ms);
+    }
+    if (t < best) { best = t; win = c; }
+  }
+  apply(win);
+  if (best_ms) *best_ms = best;
   return RR_OK;
 }
 

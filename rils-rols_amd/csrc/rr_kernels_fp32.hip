@@ -28,7 +28,7 @@
 // Candidates that cannot be certified are flagged RR_F_NEEDS64 (reason in
 // rr_fit.rank, negated) and re-scored by the FP64 form: a value whose
 // finiteness FP32 and FP64 can disagree on (overflow, a domain boundary
-// inside the bound); a rank decision (Schur complement below TAU_C of its
+// inside the bound); a rank decision (Schur complement below tau_c of its
 // column); a coefficient whose |c| < 1e-12 or |c - 1| < 1e-12 decision lies
 // inside its bound; more than 8 factors; n < k.
 //
@@ -70,7 +70,12 @@ constexpr float H_EXP = 0x1p-20f;     // relative
 constexpr float H_LOG = 0x1p-20f;     // absolute, scaled by (1 + |v|)
 constexpr float H_TRIG = 0x1p-19f;    // absolute
 constexpr float TRIG_RANGE = 1600.f;  // hardware sin/cos take |x / 2pi| <= 256
-constexpr double TAU_C = 1e-4;        // smallest accepted Schur complement / column norm^2
+// smallest accepted Schur complement / column norm^2: TAU_C_REL times the
+// largest relative RMS error of a column (so that the FP32 perturbation of
+// every pivot stays a few percent of it and the first-order bound applies),
+// and never below TAU_C_MIN, far above the FP64 QR's own rank cut (~1e-31)
+constexpr double TAU_C_REL = 64.0;
+constexpr double TAU_C_MIN = 1e-9;
 constexpr double TAU_S = 1e-20;       // smallest accepted Schur complement / (max norm)^2
 constexpr double EPS12 = 1e-12;
 constexpr double D_ROUND = 1.0 + 0x1p-8;
@@ -342,7 +347,7 @@ struct Solve {
 // first index on ties) into S.L / S.perm; false when a pivot fails the
 // conditioning checks.
 template <int KB>
-__device__ inline bool chol(Solve<KB>& S, const double (&T)[KB][KB], int m) {
+__device__ inline bool chol(Solve<KB>& S, const double (&T)[KB][KB], int m, double tau_c) {
   double diag0[KB], mx2 = 0.0;
   for (int j = 0; j < m; ++j) {
     S.perm[j] = j;
@@ -356,7 +361,7 @@ __device__ inline bool chol(Solve<KB>& S, const double (&T)[KB][KB], int m) {
     for (int j = p + 1; j < m; ++j) { const double v = S.L[S.perm[j]][S.perm[j]]; if (v > best) { best = v; bi = j; } }
     const int t = S.perm[p]; S.perm[p] = S.perm[bi]; S.perm[bi] = t;
     const int q = S.perm[p];
-    if (!(best > TAU_C * diag0[q]) || !(best > TAU_S * mx2)) return false;
+    if (!(best > tau_c * diag0[q]) || !(best > TAU_S * mx2)) return false;
     const double l = __builtin_sqrt(best);
     S.L[q][q] = l;
     for (int j = p + 1; j < m; ++j) { const int a = S.perm[j]; S.L[a][q] = S.L[a][q] / l; }
@@ -591,17 +596,29 @@ __global__ __launch_bounds__(64) void k_fin_fp32(Fp32Args a, ChunkArgs w) {
     if (!certain) {
       int idx[KB];
       int m = 0;
-      for (int j = 0; j < k; ++j) if (j == nf || !((nanc >> j) & 1u)) idx[m++] = j;
+      double rho = 0.0;
+      for (int j = 0; j < k; ++j) {
+        if (j == nf || !((nanc >> j) & 1u)) {
+          idx[m++] = j;
+          if (j < nf) { const double r = __builtin_sqrt(S.bd[j] / S.M[j][j]); rho = (r > rho) ? r : rho; }
+        }
+      }
       double T[KB][KB];
       for (int i = 0; i < m; ++i) for (int j = 0; j < m; ++j) T[i][j] = S.M[idx[i]][idx[j]];
-      certain = chol<KB>(S, T, m);
+      const double tau_c = (TAU_C_REL * rho > TAU_C_MIN) ? TAU_C_REL * rho : TAU_C_MIN;
+      certain = chol<KB>(S, T, m, tau_c);
     }
     if (!certain) { write_needs64(a.fit + cand, co, k, NR_NAN_COND); return; }
     for (int i = 0; i < k; ++i) co[i] = rrm::qnan();
     write_sentinel(a.fit + cand, k);
     return;
   }
-  if (!chol<KB>(S, S.M, k)) { write_needs64(a.fit + cand, co, k, NR_COND); return; }
+  {
+    double rho = 0.0;
+    for (int j = 0; j < nf; ++j) { const double r = __builtin_sqrt(S.bd[j] / S.M[j][j]); rho = (r > rho) ? r : rho; }
+    const double tau_c = (TAU_C_REL * rho > TAU_C_MIN) ? TAU_C_REL * rho : TAU_C_MIN;
+    if (!chol<KB>(S, S.M, k, tau_c)) { write_needs64(a.fit + cand, co, k, NR_COND); return; }
+  }
 
   // solve, one refinement step on the normal equations
   double c[KB], r1[KB], dcr[KB];

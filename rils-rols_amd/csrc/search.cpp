@@ -53,6 +53,10 @@ class DirectBackend : public FitnessBackend {
     rr_last_kernel_ms(ctx, kernel_ms);
   }
   rr_ctx* context() override { return ctx; }
+  void autotune(int32_t h, const rr_batch& b, int32_t mode, int32_t precision) override {
+    double ms = 0;
+    rr_check(rr_autotune(ctx, h, &b, mode, precision, 2, &ms), "rr_autotune");
+  }
 };
 
 class GpuSession {
@@ -128,6 +132,7 @@ class GpuSession {
   }
   // the same for the candidates idx[0..count) (all of 0..count when idx is null);
   // results are indexed by position in idx
+  bool tune_next = false;   // run the autotuner on the next batch before scoring it
   void ols_subset(const std::vector<std::vector<Flat>>& factors, const int32_t* idx, size_t count, int precision,
                   std::vector<double>& coefs, std::vector<int64_t>& off, std::vector<rr_fit>& out, Stats& st) {
     Builder B;
@@ -144,6 +149,10 @@ class GpuSession {
     out.resize(count);
     if (count == 0) return;
     const rr_batch b = B.view();
+    if (tune_next) {
+      tune_next = false;
+      be->autotune(ds, b, RR_MODE_OLS, precision);
+    }
     const auto t0 = std::chrono::steady_clock::now();
     double ms = 0;
     be->fitness(ds, b, RR_MODE_OLS, precision, coefs.data(), out.data(), &ms);
@@ -586,6 +595,10 @@ Flat RilsRols::local_search(const Flat& passed, Fit* out_fit) {   // 601-643
     workers().parallel_for(m, [&](size_t j, int tid) { ols_factors(perts[j], facs[j], pools_[(size_t)tid]); });
     stats_.host_prep_s += secs_since(t0);
     const bool fp32 = precision_ == RR_FP32_REFINED;
+    if (autotune_ && !tuned_ && m >= 256) {
+      gpu_->tune_next = true;
+      tuned_ = true;
+    }
     gpu_->ols(facs, m, fp32 ? RR_FP32_REFINED : RR_FP64, coefs, off, res, stats_);
     stats_.sweeps++;
     stats_.batched_cands += (int64_t)m;
@@ -657,6 +670,7 @@ void RilsRols::fit_inner(const std::vector<std::vector<double>>& X_all, const st
   moves_.clear();
   stats_ = Stats();
   sweep_ms_.clear();
+  tuned_ = false;
   const int64_t N = (int64_t)y_all.size();
   const int sample_cnt = int(sample_size_ * (double)N);
   std::vector<int> selected(N);

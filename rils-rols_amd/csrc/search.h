@@ -68,6 +68,10 @@ class FitnessBackend {
                        rr_fit* fit, double* kernel_ms) = 0;
   // a context for predict, or null when the backend has none to lend
   virtual rr_ctx* context() = 0;
+  // pick the kernel schedule on a representative batch (no-op by default)
+  virtual void autotune(int32_t handle, const rr_batch& b, int32_t mode, int32_t precision) {
+    (void)handle; (void)b; (void)mode; (void)precision;
+  }
 };
 
 class GpuSession;
@@ -98,6 +102,9 @@ class RilsRols {
   void set_host_threads(int t);
   // route fitness calls through `be` (not owned; must outlive the fit)
   void set_backend(FitnessBackend* be) { backend_ = be; }
+  // time the first sweep of at least 256 candidates under every kernel
+  // schedule and keep the fastest (rr_autotune); off by default
+  void set_autotune(bool on) { autotune_ = on; }
   // per local-search sweep: wall time of batch + replay, ms
   const std::vector<float>& sweep_ms() const { return sweep_ms_; }
   std::string final_prefix() const { return final_solution_.empty() ? "" : to_prefix(final_solution_); }
@@ -127,6 +134,7 @@ class RilsRols {
   int64_t slab_budget_mb_ = 0;
   GpuSession* gpu_ = nullptr;
   FitnessBackend* backend_ = nullptr;
+  bool autotune_ = false, tuned_ = false;
   std::vector<float> sweep_ms_;
   Pool pool_;
   int host_threads_ = 0;                  // 0: min(hardware threads, 8)

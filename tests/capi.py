@@ -69,6 +69,8 @@ def lib():
         fp = ctypes.POINTER(ctypes.c_float)
         L.rr_predict_fp32.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_uint32), ctypes.c_int32, dp,
                                       ctypes.c_int32, fp, fp]
+        L.rr_autotune.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(Batch), ctypes.c_int32, ctypes.c_int32,
+                                  ctypes.c_int32, dp]
         L.rr_set_option.argtypes = [vp, ctypes.c_char_p, ctypes.c_int64]
         L.rr_get_option.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64)]
         L.rr_last_kernel_ms.argtypes = [vp, dp]
@@ -296,3 +298,17 @@ class Context:
         check(lib().rr_predict_fp32(self.p, h, code.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), len(code),
                                     _dp(arrs["consts"]), arrs["n_consts"], v.ctypes.data_as(fp), e.ctypes.data_as(fp)))
         return v, e
+
+    def autotune(self, h, arrs, mode="ols", precision="fp64", reps=2):
+        m = D["RR_MODE_OLS"] if mode == "ols" else D["RR_MODE_RAW"]
+        pr = D["RR_FP64"] if precision == "fp64" else D["RR_FP32_REFINED"]
+        b = _as_batch(arrs)
+        best = np.zeros(1)
+        check(lib().rr_autotune(self.p, h, ctypes.byref(b), m, pr, reps, _dp(best)))
+        return float(best[0])
+
+    def get_option(self, name):
+        v = ctypes.c_int64()
+        check(lib().rr_get_option(self.p, name.encode(), ctypes.byref(v)))
+        return v.value
+
