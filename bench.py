@@ -404,6 +404,37 @@ def search_main(args):
     print(json.dumps(line))
 
 
+def small_main(args):
+    """The reference's own small case (test_example recipe, committed as
+    tests/golden/config1_train.csv): 150 rows x 2 features, max_fit_calls
+    100000, random_state 12345.  The GPU search and the CPU oracle's search
+    run to the end; their move sequences must be identical."""
+    import rils_rols_cpp
+    import oracle_lib as O
+    d = np.loadtxt(os.path.join(ROOT, "tests", "golden", "config1_train.csv"), delimiter=",", skiprows=1)
+    X, y = d[:, :2], d[:, 2]
+    n, f = X.shape
+    rr = rils_rols_cpp.rils_rols(False, 100000, 10 ** 7, 0.001, 50, 1.0, False, 12345)
+    t = time.perf_counter()
+    rr.fit(np.ascontiguousarray(X).reshape(-1), y, n, f)
+    gpu_s = time.perf_counter() - t
+    line = {
+        "metric": "plain method, small: full search wall time (150 x 2, 100000 fit calls)",
+        "value": rr.get_fit_calls() / gpu_s, "unit": "fitness evals/s", "n_gpus": 1, "higher_is_better": True,
+        "dtype": "f64", "data": "reference test_example recipe (committed fixture)",
+        "config": {"workload": "plain method, small: 150 x 2, max_fit_calls=100000, random_state=12345"},
+        "gpu_wall_s": gpu_s, "fit_calls": rr.get_fit_calls(), "model": rr.get_model_string(), "stats": rr.get_stats(),
+    }
+    if not args.no_cpu:
+        t = time.perf_counter()
+        want = O.fit(X, y, max_fit_calls=100000, max_time=10 ** 7, random_state=12345)
+        cpu_s = time.perf_counter() - t
+        line["cpu_oracle"] = {"wall_s": cpu_s, "cores": 1, "evals_per_s": want["fit_calls"] / cpu_s}
+        line["moves_identical"] = [tuple(m) for m in rr.get_moves()] == want["moves"]
+        line["speedup_vs_cpu"] = cpu_s / gpu_s
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -419,7 +450,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default kernel schedule instead of the autotuner's pick")
     ap.add_argument("--copy-probe", action="store_true", help="run only a plain 1 GiB device copy (PMC scaling)")
-    ap.add_argument("--workload", default="batched", choices=["batched", "serving", "precision", "search"],
+    ap.add_argument("--workload", default="batched", choices=["batched", "serving", "precision", "search", "small"],
                     help="batched: the headline batched candidate eval; serving: the serving mix; precision: 16384 "
                          "trees at n=1e6 x 20 in FP32; search: a full search at n=1e6 x 5")
     ap.add_argument("--c3-rows", type=int, default=1_000_000)
@@ -449,6 +480,9 @@ def main():
         return
     if args.workload == "search":
         search_main(args)
+        return
+    if args.workload == "small":
+        small_main(args)
         return
     import rils_rols_cpp
     t0 = time.perf_counter()
