@@ -25,7 +25,7 @@ summary under profiles/ when one is there for the kernel (FETCH_SIZE and
 WRITE_SIZE of separate runs, scaled by the same counters on a plain copy of
 known size: `python bench.py --copy-probe`).
 
-Prints one JSON line.  --gpus N with N > 1 exits: this build measures one GPU.
+Prints one JSON line.  --gpus N with N > 1 exits: the program measures one GPU.
 """
 import argparse
 import json
@@ -303,7 +303,7 @@ def precision_main(args):
     """16384 trees of 5-50 nodes over n = 1e6 x 20 (seeds: trees 4242, data
     2424): FP32 form timed, FP64 form once for the error report."""
     import rils_rols_cpp
-    n, d, ntrees = args.c3_rows, 20, args.c3_trees
+    n, d, ntrees = args.precision_rows, 20, args.precision_trees
     rng = np.random.default_rng(2424)
     X = rng.uniform(-5, 5, (n, d))
     while True:
@@ -362,7 +362,7 @@ def search_main(args):
     seeded 15-node tree + 1% noise, max_complexity 30; the CPU oracle runs the
     same search for a short prefix of fit calls and its rate is extrapolated."""
     import rils_rols_cpp
-    n, d = args.c4_rows, 5
+    n, d = args.search_rows, 5
     rng = np.random.default_rng(777)
     X = rng.uniform(-3, 3, (n, d))
     while True:
@@ -373,7 +373,7 @@ def search_main(args):
     y = yt + rng.normal(0, 0.01 * np.sqrt(np.mean(yt ** 2)), n)
     # verbose: the search prints its state every 10000 fit calls (rils_rols_cpp.cpp:563-579) on stdout;
     # the JSON line stays the last line
-    rr = rils_rols_cpp.rils_rols(False, args.c4_fit_calls, 10 ** 7, 0.001, 30, 1.0, True, 12345)
+    rr = rils_rols_cpp.rils_rols(False, args.search_fit_calls, 10 ** 7, 0.001, 30, 1.0, True, 12345)
     rr.set_precision(args.search_precision)
     t = time.perf_counter()
     rr.fit(np.ascontiguousarray(X).reshape(-1), y, n, d)
@@ -385,16 +385,16 @@ def search_main(args):
     if not args.no_cpu:
         import oracle_lib as O
         t = time.perf_counter()
-        want = O.fit(X, y, max_fit_calls=args.c4_cpu_calls, max_time=10 ** 7, random_state=12345, max_complexity=30)
+        want = O.fit(X, y, max_fit_calls=args.search_cpu_calls, max_time=10 ** 7, random_state=12345, max_complexity=30)
         cpu_s = time.perf_counter() - t
         cpu = {"fit_calls": want["fit_calls"], "seconds": cpu_s, "evals_per_s": want["fit_calls"] / cpu_s, "cores": 1,
-               "extrapolated_full_run_s": args.c4_fit_calls / (want["fit_calls"] / cpu_s)}
+               "extrapolated_full_run_s": args.search_fit_calls / (want["fit_calls"] / cpu_s)}
     line = {
         "metric": "full search at scale: wall time and final R2 (n=1e6 x 5)",
         "value": rr.get_fit_calls() / wall, "unit": "fitness evals/s", "n_gpus": 1, "higher_is_better": True,
         "dtype": "f64" if args.search_precision == "fp64" else "f32+f64refine", "data": "synthetic",
         "config": {"workload": "full search: n=%d x %d uniform [-3,3], 15-node target + 1%% noise, max_fit_calls=%d, "
-                               "max_complexity=30" % (n, d, args.c4_fit_calls), "seeds": {"data": 777, "search": 12345},
+                               "max_complexity=30" % (n, d, args.search_fit_calls), "seeds": {"data": 777, "search": 12345},
                    "target": " ".join(tgt)},
         "wall_s": wall, "fit_calls": rr.get_fit_calls(), "final_r2": float(r2), "model": rr.get_model_string(),
         "p50_step_ms": float(np.percentile(rr.get_sweep_ms(), 50)) if len(rr.get_sweep_ms()) else None,
@@ -453,11 +453,11 @@ def main():
     ap.add_argument("--workload", default="batched", choices=["batched", "serving", "precision", "search", "small"],
                     help="batched: the headline batched candidate eval; serving: the serving mix; precision: 16384 "
                          "trees at n=1e6 x 20 in FP32; search: a full search at n=1e6 x 5")
-    ap.add_argument("--c3-rows", type=int, default=1_000_000)
-    ap.add_argument("--c3-trees", type=int, default=16384)
-    ap.add_argument("--c4-rows", type=int, default=1_000_000)
-    ap.add_argument("--c4-fit-calls", type=int, default=1_000_000)
-    ap.add_argument("--c4-cpu-calls", type=int, default=300)
+    ap.add_argument("--precision-rows", type=int, default=1_000_000)
+    ap.add_argument("--precision-trees", type=int, default=16384)
+    ap.add_argument("--search-rows", type=int, default=1_000_000)
+    ap.add_argument("--search-fit-calls", type=int, default=1_000_000)
+    ap.add_argument("--search-cpu-calls", type=int, default=300)
     ap.add_argument("--search-precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--runs", type=int, default=64, help="serving: number of runs")
     ap.add_argument("--run-fit-calls", type=int, default=10000, help="serving: fit-call budget per run")
@@ -468,8 +468,8 @@ def main():
         copy_probe()
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU only: collectives and multi-GPU sharding are not part of this build "
-              "(see DESIGN.md for how the path would shard)", file=sys.stderr)
+        print("bench.py runs on one GPU: the fitness path would shard across GPUs by slices of each candidate "
+              "batch, which this program does not do", file=sys.stderr)
         sys.exit(2)
 
     if args.workload == "serving":
