@@ -38,6 +38,32 @@
 //   * no floating-point contraction (built with -ffp-contract=off);
 //   * a NaN coefficient leaving the solver is replaced by the x86 default
 //     NaN, so model strings print it as "-nan" whatever produced it.
+//
+// What pins each part (the reference holds no golden vectors for this path
+// and cannot be built or imported here; tests/test_oracle.py unless noted):
+//   sin/cos/exp/log ......... < 1 ulp of mpmath over wide ranges; special
+//                             values (test_math_*)
+//   canonical sums .......... exact on integer data, within the fsum error
+//                             bound otherwise; the ORDER itself is this
+//                             project's convention (Eigen's is unreadable):
+//                             last-bit agreement with the reference is
+//                             parity unpinned
+//   ColPivHouseholderQR ..... numpy lstsq on full-rank designs, exact integer
+//                             coefficients, duplicated / constant / zero
+//                             columns, n < k, pivot order of LAPACK geqp3
+//                             (test_qr_*)
+//   score / R2 / RMSE ....... closed forms: exact fits, formulas, NaN sentinel,
+//                             Inf passthrough
+//   rewrites, to_string ..... known answers derived by hand from node.cpp,
+//                             faithful bugs included
+//   all_candidates .......... known answer for the perturbations of x0; the
+//                             local-search lists beyond that: parity unpinned
+//                             (checked for agreement with the product only)
+//   acceptance primitives ... logic tables (test_acceptance_primitives_*)
+//   glibc rand sequence ..... the same library call (test_crand_is_glibc_rand
+//                             in tests/test_host_engine.py)
+//   local_search / fit_inner  determinism and fit-call budget accounting; the
+//   move sequence ........... end-to-end sequence is parity unpinned
 #define RR_HD
 #include "../rils-rols_amd/csrc/rr_math.h"
 
@@ -1186,6 +1212,20 @@ double orc_batch_tune(int h, const char* pres, int threads, double* fit3) {
     for (auto& x : th) x.join();
   }
   return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// Acceptance primitives of the search (rils_rols_cpp.cpp:543-599) on two
+// fitness tuples a = (e0, e1, size), b: out[0] = fitness_value(a),
+// out[1] = compare_fitness(a, b), out[2] = dominates(a, b).
+int orc_accept(const double* a, const double* b, double complexity_penalty, int max_complexity, double* out) {
+  orc::RilsRols rr;
+  rr.complexity_penalty = complexity_penalty;
+  rr.max_complexity = max_complexity;
+  const orc::Fit fa{a[0], a[1], (int)a[2]}, fb{b[0], b[1], (int)b[2]};
+  out[0] = rr.fitness_value(fa);
+  out[1] = (double)rr.compare_fitness(fa, fb);
+  out[2] = orc::RilsRols::dominates(fa, fb) ? 1.0 : 0.0;
+  return 0;
 }
 
 }  // extern "C"

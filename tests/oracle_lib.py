@@ -44,6 +44,7 @@ def lib():
                               ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_int]
         L.orc_fit.restype = ctypes.c_void_p
         L.orc_batch_tune.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.c_int, dp]
+        L.orc_accept.argtypes = [dp, dp, ctypes.c_double, ctypes.c_int, dp]
         L.orc_batch_tune.restype = ctypes.c_double
         _lib = L
     return _lib
@@ -192,3 +193,14 @@ def parse_fit_report(rep):
             out["moves"].append((int(parts[1]), int(parts[2]), float.fromhex(parts[3]), float.fromhex(parts[4]),
                                  int(parts[5]), parts[6]))
     return out
+
+
+def accept(a, b, complexity_penalty=0.001, max_complexity=50):
+    """(fitness_value(a), compare_fitness(a, b), dominates(a, b)) of the
+    oracle's search for fitness tuples a, b = (1-R2, RMSE, size)."""
+    av = np.asarray(a, dtype=np.float64)
+    bv = np.asarray(b, dtype=np.float64)
+    out = np.empty(3)
+    assert lib().orc_accept(_dp(av), _dp(bv), complexity_penalty, max_complexity, _dp(out)) == 0
+    return float(out[0]), int(out[1]), bool(out[2])
+

@@ -242,3 +242,26 @@ def test_search_budget_and_determinism():
     assert a["fit_calls"] in (3000, 3001)            # finished() is checked before each call
     assert all(m[1] <= a["fit_calls"] for m in a["moves"])
     assert a["final"][0] < 0.01                      # R2 > 0.99 on this easy problem
+
+
+# Acceptance primitives, rils_rols_cpp.cpp:543-599, as logic tables:
+# fitness_value = (1+e0)(1+e1)(1+size*penalty); compare_fitness returns the
+# size difference when sizes differ and either exceeds max_complexity, else
+# the sign of the fitness-value comparison; dominates is <= on all three.
+@pytest.mark.parametrize("a,b,pen,maxc,want", [
+    ((0.1, 0.2, 10), (0.1, 0.2, 10), 0.001, 50, ((1 + 0.1) * (1 + 0.2) * (1 + 10 * 0.001), 0, True)),
+    ((0.0, 0.0, 60), (0.5, 0.5, 10), 0.001, 50, ((1.0) * (1.0) * (1 + 60 * 0.001), 50, False)),   # size rule wins
+    ((0.5, 0.5, 10), (0.0, 0.0, 60), 0.001, 50, (1.5 * 1.5 * (1 + 10 * 0.001), -50, False)),
+    ((0.1, 0.1, 60), (0.2, 0.2, 60), 0.001, 50, (1.1 * 1.1 * (1 + 60 * 0.001), -1, True)),      # equal sizes: fitness
+    ((0.2, 0.1, 5), (0.1, 0.2, 5), 0.001, 50, (1.2 * 1.1 * (1 + 5 * 0.001), 0, False)),       # equal values, no dominance
+    ((0.1, 0.1, 5), (0.1, 0.1, 7), 0.001, 50, (1.1 * 1.1 * (1 + 5 * 0.001), -1, True)),
+    ((0.1, 0.1, 7), (0.1, 0.1, 5), 0.0, 50, (1.1 * 1.1 * 1.0, 0, False)),                     # no penalty: tie
+    ((1000, 1000, 1000), (0.3, 0.3, 3), 0.001, 50, (1001 * 1001 * (1 + 1000 * 0.001), 997, False)),  # sentinel
+])
+def test_acceptance_primitives_logic_table(a, b, pen, maxc, want):
+    fv, cmp, dom = O.accept(a, b, pen, maxc)
+    assert fv == want[0]
+    assert (cmp > 0) - (cmp < 0) == (want[1] > 0) - (want[1] < 0)
+    if abs(want[1]) > 1:
+        assert cmp == want[1]        # the size difference itself
+    assert dom == want[2]
