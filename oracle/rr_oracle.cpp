@@ -62,8 +62,11 @@
 //   acceptance primitives ... logic tables (test_acceptance_primitives_*)
 //   glibc rand sequence ..... the same library call (test_crand_is_glibc_rand
 //                             in tests/test_host_engine.py)
-//   local_search / fit_inner  determinism and fit-call budget accounting; the
-//   move sequence ........... end-to-end sequence is parity unpinned
+//   local_search / fit_inner  determinism and fit-call budget accounting;
+//                             exact recovery of the generating formula on seven
+//                             of the reference's own noise-free datasets, where
+//                             the reference reports R2 = 1 (tests/test_recovery.py);
+//   move sequence ........... the end-to-end sequence itself is parity unpinned
 #define RR_HD
 #include "../rils-rols_amd/csrc/rr_math.h"
 
