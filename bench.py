@@ -132,15 +132,19 @@ def fp64_algorithmic_bytes(k, steps, sentinel, n):
 
 
 def timed_steps(runner, rescorer, precision, warmup, steps):
-    """K steps of one batched call.  In the FP32 form a step also re-scores in
-    FP64 every candidate the FP32 kernels could not certify (RR_F_NEEDS64),
-    so each step delivers a trusted fitness for all 4096 candidates."""
+    """K steps of one batched call.  In the FP32 form ("fp32") a step also
+    re-scores in FP64, with a second call, every candidate the FP32 kernels
+    could not certify (RR_F_NEEDS64); the checked form ("fp32c") does both in
+    one call.  Either way each step delivers a trusted fitness for every
+    candidate."""
     fp32_fit = {}
 
     def one():
         coefs, fit = runner.run(precision)
         ms = runner.last_kernel_ms()
         n64 = 0
+        if precision == "fp32c":
+            n64 = int(np.count_nonzero(fit[:, 4].astype(np.int64) & 4))
         if precision == "fp32":
             fp32_fit["fit"] = fit.copy()          # the FP32 form's own flags and bounds
             need = np.nonzero(fit[:, 4].astype(np.int64) & 2)[0]
@@ -442,6 +446,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--precision", default="both", choices=["both", "fp64", "fp32"])
+    ap.add_argument("--fp32-form", default="checked", choices=["checked", "split"],
+                    help="checked: one RR_FP32_CHECKED call per step (FP64 for the candidates it needs, in the "
+                         "same call); split: an RR_FP32_REFINED call, then an RR_FP64 call for the flagged ones")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--fp64-rows", type=int, default=None, help="samples per lane per tile, FP64 kernels (1, 2, 4)")
     ap.add_argument("--fp32-rows", type=int, default=None, help="samples per lane per tile, FP32 kernels (1, 2)")
@@ -508,12 +515,16 @@ def main():
     res = {}
     tuned = {}
     for prec in (["fp64", "fp32"] if args.precision == "both" else [args.precision]):
+        form = "fp32c" if prec == "fp32" and args.fp32_form == "checked" else prec
         if not args.no_autotune:
             # the schedule (rows per thread, FP32 chunk size, bucket streams) is picked by the
             # library's autotuner on this batch, outside the timed region
-            tuned[prec] = runner.autotune(prec, 2)
-            print("[bench] autotune %s: %s" % (prec, tuned[prec]), file=sys.stderr, flush=True)
-        total, lat, kms, coefs, fit, n64, f32 = timed_steps(runner, runner, prec, args.warmup, args.steps)
+            tuned[prec] = runner.autotune(form, 2)
+            print("[bench] autotune %s: %s" % (form, tuned[prec]), file=sys.stderr, flush=True)
+        total, lat, kms, coefs, fit, n64, f32 = timed_steps(runner, runner, form, args.warmup, args.steps)
+        if form == "fp32c":
+            # the FP32 kernels' own flags and bounds, from one untimed call of the unchecked form
+            f32["fit"] = runner.run("fp32")[1]
         res[prec] = dict(total=total, lat=lat, kms=kms, coefs=coefs, fit=fit, n64=n64, f32=f32)
         print("[bench] %s: %.3f ms/step wall, %.3f ms kernel, %.0f evals/s" %
               (prec, 1e3 * total / args.steps, kms.mean(), N_TREES * args.steps / total), file=sys.stderr, flush=True)
@@ -570,7 +581,9 @@ def main():
             "batch_valu_tflops": flops / (r["kms"].mean() / 1e3) / 1e12,
             "needs_fp64_frac": float(np.mean((fl32 & 2) > 0)),
             "needs_fp64_reasons": reasons,
-            "rescore_fp64_ms": r["f32"].get("rescore_ms", 0.0),
+            "form": args.fp32_form,
+            "fp64_results_per_step": int(r["n64"]),
+            "rescore_fp64_ms": r["f32"].get("rescore_ms"),
             "bucket_launch_ms": bucket_ms,
         }
         if "fp64" in res:

@@ -105,11 +105,20 @@ typedef struct rr_batch {
                              a certified bound rr_fit.err; candidates flagged
                              RR_F_NEEDS64 must be re-scored in FP64 before they
                              decide anything */
+#define RR_FP32_CHECKED 2 /* OLS mode only: RR_FP32_REFINED with the FP64 form
+                             in the same call.  Candidates with a factor that
+                             reads no variable or with more than 8 factors go
+                             to RR_FP64 directly (concurrently with the FP32
+                             kernels), and those the FP32 form flags
+                             RR_F_NEEDS64 are re-scored in RR_FP64; their
+                             results carry RR_F_FP64 and are bit-for-bit those
+                             of RR_FP64.  No result is left RR_F_NEEDS64. */
 
 /* flags of rr_fit */
 #define RR_F_SENTINEL 1  /* R2 or RMSE was NaN: fitness is (1000, 1000, 1000) */
 #define RR_F_NEEDS64 2   /* FP32 form could not certify this candidate: fitness
                             and coefficients (NaN) must come from RR_FP64 */
+#define RR_F_FP64 4      /* RR_FP32_CHECKED: this result came from the FP64 form */
 /* FP64 OLS form: bits 16..31 of flags hold the number of Householder steps
  * that ran passes over the samples (for traffic accounting) */
 #define RR_F_STEPS(flags) (((uint32_t)(flags)) >> 16)

@@ -127,6 +127,16 @@ std::vector<int> rand_hook(unsigned seed, int count) {
 // and the serving loop drive.  prepare() runs the host half of
 // tune_constants (expand, simplify, factor split; rils_rols_cpp.cpp:448-475)
 // and compiles the factors; run() is one rr_fitness_batch call.
+// precision names of the batched runners: "fp64" (RR_FP64), "fp32"
+// (RR_FP32_REFINED, uncertified candidates left flagged), "fp32c"
+// (RR_FP32_CHECKED, every result final)
+int batch_precision(const std::string& p) {
+  if (p == "fp64") return RR_FP64;
+  if (p == "fp32") return RR_FP32_REFINED;
+  if (p == "fp32c") return RR_FP32_CHECKED;
+  throw std::invalid_argument("precision must be 'fp64', 'fp32' or 'fp32c'");
+}
+
 class BatchRunner {
  public:
   BatchRunner(py::array_t<double, py::array::c_style | py::array::forcecast> X_fm,
@@ -174,7 +184,7 @@ class BatchRunner {
     }
   }
   py::tuple run(const std::string& precision) {
-    const int pr = precision == "fp32" ? RR_FP32_REFINED : RR_FP64;
+    const int pr = batch_precision(precision);
     const int nc = (int)cand_.size() - 1;
     const int64_t ncoef = (int64_t)cand_.back() + nc;
     py::array_t<double> coefs(std::max<int64_t>(ncoef, 1));
@@ -209,7 +219,7 @@ class BatchRunner {
     }
   }
   py::tuple run_selected(const std::string& precision) {
-    const int pr = precision == "fp32" ? RR_FP32_REFINED : RR_FP64;
+    const int pr = batch_precision(precision);
     const int nc = (int)scand_.size() - 1;
     const int64_t ncoef = (int64_t)scand_.back() + nc;
     py::array_t<double> coefs(std::max<int64_t>(ncoef, 1));
@@ -231,7 +241,7 @@ class BatchRunner {
   double last_kernel_ms() { double ms = 0; rr_last_kernel_ms(ctx_, &ms); return ms; }
   // the autotuner on the prepared batch; returns the best kernel time and the chosen options
   py::dict autotune(const std::string& precision, int reps) {
-    const int pr = precision == "fp32" ? RR_FP32_REFINED : RR_FP64;
+    const int pr = batch_precision(precision);
     rr_batch b{(int32_t)cand_.size() - 1, cand_.data(), prog_.data(), code_.data(), consts_.data(), (int32_t)consts_.size()};
     double ms = 0;
     {

@@ -88,7 +88,12 @@ struct rr_ctx {
   hipStream_t side[rr::FP32_BUCKETS - 1] = {};
   hipEvent_t fork = nullptr, join[rr::FP32_BUCKETS - 1] = {};
   std::vector<Dataset> ds;
-  DevBuf d_cand, d_prog, d_code, d_K, d_slab, d_slab_off, d_coefs, d_fit, d_misc, d_pred_e, d_dstab, d_cand_ds;
+  // FP64 form over a subset (RR_FP32_CHECKED): a stream of its own, forked
+  // from and joined back into `stream`
+  hipStream_t side64 = nullptr;
+  hipEvent_t fork64 = nullptr, join64 = nullptr;
+  DevBuf d_cand, d_prog, d_code, d_K, d_slab, d_order32, d_coefs, d_fit, d_misc, d_pred_e, d_dstab, d_cand_ds;
+  DevBuf d_f64_off, d_f64_ord;   // FP64 slab offsets and launched position -> candidate
   DevBuf d_blk_cand, d_blk_chunk, d_chunk_base, d_work, d_work_flags;   // FP32 chunk map and partials
   HostBuf h_stage, h_out;
   int64_t slab_budget_mb = 32768;
@@ -148,6 +153,20 @@ int check_program(const uint32_t* code, int32_t len, int32_t d, int32_t n_consts
   return RR_OK;
 }
 
+// Does a (checked) program read a sample variable?  One that does not is a
+// constant column.
+bool reads_variable(const uint32_t* code, int32_t len) {
+  for (int32_t pc = 0; pc < len; ++pc) {
+    const uint32_t op = code[pc] & 0xffu;
+    if (op == RR_OP_LDV) return true;
+    if (op >= RR_OP_BIN) {
+      const uint32_t mode = (op - RR_OP_BIN) & 7u;
+      if (mode == RR_M_V || mode == RR_M_VR) return true;
+    }
+  }
+  return false;
+}
+
 int check_batch(const rr_batch* b, int32_t mode, int32_t d) {
   if (!b || b->n_cands < 0) return fail(RR_E_ARG, "bad batch");
   if (b->n_cands == 0) return RR_OK;
@@ -204,7 +223,10 @@ int rr_create(int device, rr_ctx** out) {
   c->device = device;
   bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess &&
             hipEventCreate(&c->ev0) == hipSuccess && hipEventCreate(&c->ev1) == hipSuccess &&
-            hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) == hipSuccess;
+            hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) == hipSuccess &&
+            hipStreamCreateWithFlags(&c->side64, hipStreamNonBlocking) == hipSuccess &&
+            hipEventCreateWithFlags(&c->fork64, hipEventDisableTiming) == hipSuccess &&
+            hipEventCreateWithFlags(&c->join64, hipEventDisableTiming) == hipSuccess;
   for (int i = 0; ok && i < rr::FP32_BUCKETS - 1; ++i)
     ok = hipStreamCreateWithFlags(&c->side[i], hipStreamNonBlocking) == hipSuccess &&
          hipEventCreateWithFlags(&c->join[i], hipEventDisableTiming) == hipSuccess;
@@ -224,15 +246,20 @@ int rr_destroy(rr_ctx* ctx) {
     if (!d.live) continue;
     (void)hipFree(d.X); (void)hipFree(d.y); (void)hipFree(d.X32);
   }
-  DevBuf* bufs[] = {&ctx->d_cand, &ctx->d_prog, &ctx->d_code, &ctx->d_K, &ctx->d_slab, &ctx->d_slab_off,
+  DevBuf* bufs[] = {&ctx->d_cand, &ctx->d_prog, &ctx->d_code, &ctx->d_K, &ctx->d_slab, &ctx->d_order32,
                     &ctx->d_coefs, &ctx->d_fit, &ctx->d_misc, &ctx->d_pred_e, &ctx->d_dstab, &ctx->d_cand_ds,
-                    &ctx->d_blk_cand, &ctx->d_blk_chunk, &ctx->d_chunk_base, &ctx->d_work, &ctx->d_work_flags};
+                    &ctx->d_blk_cand, &ctx->d_blk_chunk, &ctx->d_chunk_base, &ctx->d_work, &ctx->d_work_flags,
+                    &ctx->d_f64_off, &ctx->d_f64_ord};
   for (DevBuf* b : bufs) b->release();
   ctx->h_stage.release();
   ctx->h_out.release();
   (void)hipEventDestroy(ctx->ev0);
   (void)hipEventDestroy(ctx->ev1);
   (void)hipEventDestroy(ctx->fork);
+  (void)hipStreamSynchronize(ctx->side64);
+  (void)hipStreamDestroy(ctx->side64);
+  (void)hipEventDestroy(ctx->fork64);
+  (void)hipEventDestroy(ctx->join64);
   for (int i = 0; i < rr::FP32_BUCKETS - 1; ++i) {
     (void)hipStreamSynchronize(ctx->side[i]);
     (void)hipStreamDestroy(ctx->side[i]);
@@ -311,7 +338,9 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
   int rc = set_device(ctx);
   if (rc) return rc;
   if (mode != RR_MODE_OLS && mode != RR_MODE_RAW) return fail(RR_E_ARG, "bad mode");
-  if (precision != RR_FP64 && precision != RR_FP32_REFINED) return fail(RR_E_ARG, "bad precision");
+  if (precision != RR_FP64 && precision != RR_FP32_REFINED && precision != RR_FP32_CHECKED)
+    return fail(RR_E_ARG, "bad precision");
+  if (precision == RR_FP32_CHECKED && mode != RR_MODE_OLS) return fail(RR_E_ARG, "RR_FP32_CHECKED is an OLS-mode form");
   if (!b || b->n_cands < 0) return fail(RR_E_ARG, "bad batch");
   const int32_t nc = b->n_cands;
 
@@ -365,28 +394,50 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
   }
   auto ds_of = [&](int32_t c) -> const Dataset& { return ctx->ds[uniq[handles ? cand_ds[c] : 0]]; };
 
-  // FP32 form: candidates in factor-count buckets (OLS) or as given (RAW),
-  // and the chunk workgroups of the accumulation pass in that order
+  // Which candidates go to which form.  RR_FP32_CHECKED: candidates with a
+  // factor that reads no variable (its column is exactly collinear with the
+  // free term, so FP64's rank decision depends on rounding) or with more
+  // factors than the FP32 kernels take go straight to the FP64 form, on a
+  // stream of their own, while the FP32 form scores the rest.
+  const bool checked = precision == RR_FP32_CHECKED;
+  std::vector<int32_t> list32, list64;
+  if (precision == RR_FP32_REFINED) {
+    list32.resize(nc);
+    for (int32_t c = 0; c < nc; ++c) list32[c] = c;
+  } else if (checked) {
+    for (int32_t c = 0; c < nc; ++c) {
+      const int32_t p0 = b->cand_start[c], p1 = b->cand_start[c + 1];
+      bool to64 = p1 - p0 > rr::FP32_BUCKET_FACTORS[rr::FP32_BUCKETS - 2];
+      for (int32_t p = p0; p < p1 && !to64; ++p)
+        to64 = !reads_variable(b->code + b->prog_start[p], b->prog_start[p + 1] - b->prog_start[p]);
+      (to64 ? list64 : list32).push_back(c);
+    }
+  }
+
+  // FP32 plan over list32: candidates in factor-count buckets (OLS) or as
+  // given (RAW), and the chunk workgroups of the accumulation pass in that order
   std::vector<int32_t> order, blk_cand, blk_chunk, chunk_base;
   int counts[rr::FP32_BUCKETS] = {0, 0, 0, 0, 0}, blocks[rr::FP32_BUCKETS] = {0, 0, 0, 0, 0};
   int64_t total_chunks = 0;
-  if (precision == RR_FP32_REFINED) {
-    order.resize(nc);
+  const int32_t n32 = (int32_t)list32.size();
+  if (n32 > 0) {
+    order.resize(n32);
     if (mode == RR_MODE_RAW) {
-      for (int32_t c = 0; c < nc; ++c) order[c] = c;
-      counts[0] = nc;
+      order = list32;
+      counts[0] = n32;
     } else {
-      std::vector<int8_t> bucket(nc);
-      for (int32_t c = 0; c < nc; ++c) {
+      std::vector<int8_t> bucket(n32);
+      for (int32_t q = 0; q < n32; ++q) {
+        const int32_t c = list32[q];
         const int nfc = b->cand_start[c + 1] - b->cand_start[c];
         int kb = 0;
         while (kb < rr::FP32_BUCKETS - 1 && nfc > rr::FP32_BUCKET_FACTORS[kb]) ++kb;
-        bucket[c] = (int8_t)kb;
+        bucket[q] = (int8_t)kb;
         ++counts[kb];
       }
       int start[rr::FP32_BUCKETS];
       for (int kb = 0, acc = 0; kb < rr::FP32_BUCKETS; ++kb) { start[kb] = acc; acc += counts[kb]; }
-      for (int32_t c = 0; c < nc; ++c) order[start[bucket[c]]++] = c;
+      for (int32_t q = 0; q < n32; ++q) order[start[bucket[q]]++] = list32[q];
     }
     chunk_base.assign(nc, 0);
     for (int kb = 0, q = 0; kb < rr::FP32_BUCKETS; ++kb) {
@@ -409,7 +460,8 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
   const size_t s_off = align_up((size_t)nc * 8);
   const size_t s_tab = align_up(table.size() * sizeof(rr::DsDesc)), s_cds = align_up((size_t)nc * 4);
   const size_t s_chunks = 2 * align_up((size_t)total_chunks * 4) + align_up(chunk_base.size() * 4);
-  if (ctx->h_stage.ensure(s_cand + s_prog + s_code + s_K + s_off + s_tab + s_cds + s_chunks) != RR_OK)
+  const size_t s_f64 = align_up((size_t)nc * 8) + align_up((size_t)nc * 4);   // FP64 slab offsets + order
+  if (ctx->h_stage.ensure(s_cand + s_prog + s_code + s_K + s_off + s_tab + s_cds + s_chunks + 2 * s_f64) != RR_OK)
     return RR_E_NOMEM;
   size_t so = 0;
   if ((rc = upload(ctx, ctx->d_cand, b->cand_start, (size_t)(nc + 1) * 4, so))) return rc;
@@ -443,8 +495,8 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
 
   // FP32: order, chunk map and workspace
   rr::Fp32Launch L32{};
-  if (precision == RR_FP32_REFINED) {
-    if ((rc = upload(ctx, ctx->d_slab_off, order.data(), (size_t)nc * 4, so))) return rc;
+  if (n32 > 0) {
+    if ((rc = upload(ctx, ctx->d_order32, order.data(), (size_t)n32 * 4, so))) return rc;
     so += s_off;
     if ((rc = upload(ctx, ctx->d_blk_cand, blk_cand.data(), (size_t)total_chunks * 4, so))) return rc;
     so += align_up((size_t)total_chunks * 4);
@@ -456,85 +508,125 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
         ctx->d_work_flags.ensure((size_t)std::max<int64_t>(total_chunks, 1) * 8) != RR_OK)
       return RR_E_NOMEM;
     L32.cand_start = dc; L32.prog_start = dp; L32.code = dcode; L32.K = dK; L32.ds = dtab; L32.cand_ds = dcds;
-    L32.coefs = dcoef; L32.fit = dfit; L32.n_cands = nc; L32.raw = mode == RR_MODE_RAW ? 1 : 0;
+    L32.coefs = dcoef; L32.fit = dfit; L32.n_cands = n32; L32.raw = mode == RR_MODE_RAW ? 1 : 0;
     L32.rows = (int)ctx->fp32_rows;
     L32.blk_cand = (const int32_t*)ctx->d_blk_cand.p; L32.blk_chunk = (const int32_t*)ctx->d_blk_chunk.p;
     L32.chunk_base = (const int32_t*)ctx->d_chunk_base.p; L32.chunk_rows = ctx->fp32_chunk_rows;
     L32.part = (double*)ctx->d_work.p; L32.flags = (uint32_t*)ctx->d_work_flags.p;
+  } else {
+    so += s_off + s_chunks;
   }
 
-  if (mode == RR_MODE_RAW) {
-    RR_HIP(hipEventRecord(ctx->ev0, ctx->stream));
-    if (precision == RR_FP64) {
-      rr::RawLaunch L{dc, dp, dcode, dK, dtab, dcds, dfit, nc, (int)ctx->fp64_rows};
-      RR_HIP(rr::launch_raw_fp64(L, ctx->stream));
-    } else {
-      const hipStream_t st[rr::FP32_BUCKETS] = {ctx->stream, ctx->stream, ctx->stream, ctx->stream, ctx->stream};
-      RR_HIP(rr::launch_fp32(L32, (const int32_t*)ctx->d_slab_off.p, counts, blocks, st));
-    }
-    RR_HIP(hipEventRecord(ctx->ev1, ctx->stream));
-  } else if (precision == RR_FP64) {
-    // chunk so that each launch's slab fits the budget
-    std::vector<int64_t> off(nc);
+  // FP64 OLS launches over a candidate list (null: every candidate), chunked
+  // so that each launch's slab fits the budget.  The device buffers are
+  // reused by a second call only after the host has synchronized with the
+  // first; its staging goes to slot 0 or 1 of the pinned buffer.
+  const size_t so64 = so;
+  auto fp64_ols = [&](const std::vector<int32_t>* list, hipStream_t stream, int slot) -> int {
+    const int32_t m = list ? (int32_t)list->size() : nc;
+    if (m == 0) return RR_OK;
+    auto cand_at = [&](int32_t q) { return list ? (*list)[q] : q; };
+    DevBuf& offbuf = ctx->d_f64_off;
+    DevBuf& ordbuf = ctx->d_f64_ord;
+    std::vector<int64_t> off(m);
     const int64_t budget = std::max<int64_t>(ctx->slab_budget_mb, 1) << 20;
-    struct Chunk { int c0, c1, maxk; int64_t words; };
+    struct Chunk { int c0, c1, maxk; };
     std::vector<Chunk> chunks;
     int64_t max_words = 0;
-    for (int c = 0; c < nc;) {
+    for (int q = 0; q < m;) {
       int64_t used = 0;
-      int c1 = c, maxk = 0;
-      while (c1 < nc) {
-        const int k = b->cand_start[c1 + 1] - b->cand_start[c1] + 1;
-        const int64_t need = (int64_t)(2 * k) * ds_of(c1).ldx;   // A, Q^T y, kept factor values
-        if (c1 > c && (used + need) * 8 > budget) break;
-        off[c1] = used;
+      int q1 = q, maxk = 0;
+      while (q1 < m) {
+        const int32_t c = cand_at(q1);
+        const int k = b->cand_start[c + 1] - b->cand_start[c] + 1;
+        const int64_t need = (int64_t)(2 * k) * ds_of(c).ldx;   // A, Q^T y, kept factor values
+        if (q1 > q && (used + need) * 8 > budget) break;
+        off[q1] = used;
         used += need;
         maxk = std::max(maxk, k);
-        ++c1;
+        ++q1;
       }
-      chunks.push_back({c, c1, maxk, used});
+      chunks.push_back({q, q1, maxk});
       max_words = std::max(max_words, used);
-      c = c1;
+      q = q1;
     }
-    if (ctx->d_slab.ensure((size_t)max_words * 8) != RR_OK) return RR_E_NOMEM;
-    if ((rc = upload(ctx, ctx->d_slab_off, off.data(), (size_t)nc * 8, so))) return rc;
-    RR_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    DevBuf& slab = ctx->d_slab;
+    if (slab.ensure((size_t)max_words * 8) != RR_OK) return RR_E_NOMEM;
+    const size_t base = so64 + (size_t)slot * s_f64;
+    if ((rc = upload(ctx, offbuf, off.data(), (size_t)m * 8, base))) return rc;
+    if (list) { if ((rc = upload(ctx, ordbuf, list->data(), (size_t)m * 4, base + align_up((size_t)nc * 8)))) return rc; }
+    if (stream != ctx->stream) {   // the uploads ran on the context's stream
+      RR_HIP(hipEventRecord(ctx->fork64, ctx->stream));
+      RR_HIP(hipStreamWaitEvent(stream, ctx->fork64, 0));
+    }
     for (const Chunk& ch : chunks) {
       rr::OlsLaunch L;
       L.cand_start = dc; L.prog_start = dp; L.code = dcode; L.K = dK; L.ds = dtab; L.cand_ds = dcds;
-      L.slab = (double*)ctx->d_slab.p;
-      L.slab_off = (const int64_t*)ctx->d_slab_off.p + ch.c0; L.coefs = dcoef; L.fit = dfit;
-      L.cand_base = ch.c0; L.n_cands = ch.c1 - ch.c0; L.max_cols = ch.maxk; L.rows = (int)ctx->fp64_rows;
-      RR_HIP(rr::launch_ols_fp64(L, ctx->stream));
+      L.slab = (double*)slab.p;
+      L.slab_off = (const int64_t*)offbuf.p + ch.c0; L.coefs = dcoef; L.fit = dfit;
+      L.cand_base = ch.c0; L.order = list ? (const int32_t*)ordbuf.p : nullptr;
+      L.n_cands = ch.c1 - ch.c0; L.max_cols = ch.maxk; L.rows = (int)ctx->fp64_rows;
+      RR_HIP(rr::launch_ols_fp64(L, stream));
     }
-    RR_HIP(hipEventRecord(ctx->ev1, ctx->stream));
-  } else {
-    // factor-count buckets, each kernel instance keeping only the
-    // accumulators its bucket needs; bucket kernels back to back on the context's stream, or (option
-    // fp32_streams) bucket 0 there and the others on side streams
-    const bool side = ctx->fp32_streams != 0;
+    return RR_OK;
+  };
+
+  // FP32 launches on the context's stream (bucket kernels back to back, or
+  // with option fp32_streams bucket 0 there and the others on side streams)
+  auto fp32_run = [&]() -> int {
+    if (n32 == 0) return RR_OK;
+    const bool side = ctx->fp32_streams != 0 && mode == RR_MODE_OLS;
     hipStream_t st[rr::FP32_BUCKETS];
     st[0] = ctx->stream;
     for (int i = 1; i < rr::FP32_BUCKETS; ++i) st[i] = side ? ctx->side[i - 1] : ctx->stream;
-    RR_HIP(hipEventRecord(ctx->ev0, ctx->stream));
     if (side) {
       RR_HIP(hipEventRecord(ctx->fork, ctx->stream));
       for (int i = 0; i < rr::FP32_BUCKETS - 1; ++i) RR_HIP(hipStreamWaitEvent(ctx->side[i], ctx->fork, 0));
     }
-    RR_HIP(rr::launch_fp32(L32, (const int32_t*)ctx->d_slab_off.p, counts, blocks, st));
+    RR_HIP(rr::launch_fp32(L32, (const int32_t*)ctx->d_order32.p, counts, blocks, st));
     if (side) {
       for (int i = 0; i < rr::FP32_BUCKETS - 1; ++i) {
         RR_HIP(hipEventRecord(ctx->join[i], ctx->side[i]));
         RR_HIP(hipStreamWaitEvent(ctx->stream, ctx->join[i], 0));
       }
     }
-    RR_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    return RR_OK;
+  };
+
+  std::vector<int32_t> rescored;   // candidates whose result came from the FP64 form (checked mode)
+  RR_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+  if (mode == RR_MODE_RAW && precision == RR_FP64) {
+    rr::RawLaunch L{dc, dp, dcode, dK, dtab, dcds, dfit, nc, (int)ctx->fp64_rows};
+    RR_HIP(rr::launch_raw_fp64(L, ctx->stream));
+  } else if (precision == RR_FP64) {
+    if ((rc = fp64_ols(nullptr, ctx->stream, 0))) return rc;
+  } else if (!checked) {
+    if ((rc = fp32_run())) return rc;
+  } else {
+    // FP64 for the predicted candidates beside the FP32 form, then FP64 for
+    // the ones the FP32 form could not certify
+    if (!list64.empty()) { if ((rc = fp64_ols(&list64, ctx->side64, 0))) return rc; }
+    if ((rc = fp32_run())) return rc;
+    if (!list64.empty()) {
+      RR_HIP(hipEventRecord(ctx->join64, ctx->side64));
+      RR_HIP(hipStreamWaitEvent(ctx->stream, ctx->join64, 0));
+    }
+    std::vector<rr_fit> f32(nc);
+    RR_HIP(hipMemcpyAsync(f32.data(), dfit, (size_t)nc * sizeof(rr_fit), hipMemcpyDeviceToHost, ctx->stream));
+    RR_HIP(hipStreamSynchronize(ctx->stream));
+    std::vector<int32_t> again;
+    for (int32_t c : list32) if (f32[c].flags & RR_F_NEEDS64) again.push_back(c);
+    if (!again.empty()) { if ((rc = fp64_ols(&again, ctx->stream, 1))) return rc; }
+    rescored = list64;
+    rescored.insert(rescored.end(), again.begin(), again.end());
   }
+  RR_HIP(hipEventRecord(ctx->ev1, ctx->stream));
 
   RR_HIP(hipMemcpyAsync(fit_out, dfit, (size_t)nc * sizeof(rr_fit), hipMemcpyDeviceToHost, ctx->stream));
   if (mode == RR_MODE_OLS)
     RR_HIP(hipMemcpyAsync(coefs_out, dcoef, (size_t)ncoef * 8, hipMemcpyDeviceToHost, ctx->stream));
   RR_HIP(hipStreamSynchronize(ctx->stream));
+  for (int32_t c : rescored) fit_out[c].flags |= RR_F_FP64;
   float ms = 0.f;
   RR_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
   ctx->last_ms = ms;

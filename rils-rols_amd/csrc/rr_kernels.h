@@ -31,7 +31,8 @@ struct OlsLaunch {
   const int64_t* slab_off;    // per launched candidate, in doubles
   double* coefs;
   rr_fit* fit;
-  int cand_base;              // first candidate of this launch
+  int cand_base;              // first launched position of this launch
+  const int32_t* order;       // launched position -> candidate (null: identity)
   int n_cands;                // candidates in this launch
   int max_cols;               // largest k among them
   int rows;                   // rows per lane per tile (1, 2 or 4)

@@ -58,6 +58,7 @@ struct Fp64Args {
   double* coefs;
   rr_fit* fit;
   int cand_base;
+  const int32_t* order;   // launched position -> candidate, or null (identity)
 };
 
 __device__ inline void write_fit(rr_fit* f, double ssr, double sst, int64_t n, int rank, int steps = 0) {
@@ -95,7 +96,7 @@ __global__ __launch_bounds__(CP) void k_ols_fp64(Fp64Args a) {
   __shared__ double red1[CW][1], red2[CW][2], redG[CW][G], redS[CW][2 * G];
   __shared__ int abort_flag;
 
-  const int cand = a.cand_base + (int)blockIdx.x;
+  const int cand = a.order ? a.order[a.cand_base + (int)blockIdx.x] : a.cand_base + (int)blockIdx.x;
   bind_dataset(a, cand);
   const int tid = (int)threadIdx.x;
   const int pbeg = a.cand_start[cand];
@@ -518,6 +519,7 @@ hipError_t launch_ols_fp64(const OlsLaunch& L, hipStream_t s) {
   a.cand_start = L.cand_start; a.prog_start = L.prog_start; a.code = L.code; a.K = L.K;
   a.ds = L.ds; a.cand_ds = L.cand_ds;
   a.slab = L.slab; a.slab_off = L.slab_off; a.coefs = L.coefs; a.fit = L.fit; a.cand_base = L.cand_base;
+  a.order = L.order;
   const dim3 grid(L.n_cands), block(CP);
   const int R = L.rows;
   if (L.max_cols <= 16) {

@@ -524,8 +524,10 @@ bool RilsRols::fp32_rejects(const rr_fit& r, const std::vector<int32_t>& fsz, co
 }
 
 // One FP64 batch, before the replay, for the candidates the replay is likely
-// to need: the uncertified ones and those that could beat the incumbent the
-// sweep starts from.  Its results replace the FP32 ones in place.
+// to need: those that could beat the incumbent the sweep starts from (the
+// sweep's RR_FP32_CHECKED call already returned FP64 results, RR_F_FP64, for
+// every candidate the FP32 form could not certify).  Its results replace the
+// FP32 ones in place.
 void RilsRols::rescore_speculative(const std::vector<std::vector<Flat>>& facs, size_t m, const Fit& curr,
                                    std::vector<double>& coefs, const std::vector<int64_t>& off,
                                    std::vector<rr_fit>& res, std::vector<uint8_t>& is64) {
@@ -533,6 +535,12 @@ void RilsRols::rescore_speculative(const std::vector<std::vector<Flat>>& facs, s
   std::vector<int32_t> idx;
   std::vector<int32_t> fsz;
   for (size_t j = 0; j < m; ++j) {
+    if (res[j].flags & RR_F_FP64) {
+      res[j].flags &= ~RR_F_FP64;
+      is64[j] = 1;
+      stats_.rescored_fp64++;
+      continue;
+    }
     fsz.clear();
     for (const Flat& f : facs[j]) fsz.push_back((int32_t)f.size());
     if (!fp32_rejects(res[j], fsz, coefs.data() + off[j], curr)) idx.push_back((int32_t)j);
@@ -599,7 +607,7 @@ Flat RilsRols::local_search(const Flat& passed, Fit* out_fit) {   // 601-643
       gpu_->tune_next = true;
       tuned_ = true;
     }
-    gpu_->ols(facs, m, fp32 ? RR_FP32_REFINED : RR_FP64, coefs, off, res, stats_);
+    gpu_->ols(facs, m, fp32 ? RR_FP32_CHECKED : RR_FP64, coefs, off, res, stats_);
     stats_.sweeps++;
     stats_.batched_cands += (int64_t)m;
     if (fp32) rescore_speculative(facs, m, curr_fit, coefs, off, res, is64);

@@ -215,6 +215,11 @@ def _as_batch(arrs):
     return b
 
 
+def precision_code(name):
+    """RR_FP64 for "fp64", RR_FP32_REFINED for "fp32", RR_FP32_CHECKED for "fp32c"."""
+    return {"fp64": D["RR_FP64"], "fp32": D["RR_FP32_REFINED"], "fp32c": D["RR_FP32_CHECKED"]}[name]
+
+
 def _dp(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
@@ -260,7 +265,7 @@ class Context:
         """Score a batch against dataset h, or candidate c against handles[c]
         (rr_fitness_multi) when handles is given."""
         m = D["RR_MODE_OLS"] if mode == "ols" else D["RR_MODE_RAW"]
-        pr = D["RR_FP64"] if precision == "fp64" else D["RR_FP32_REFINED"]
+        pr = precision_code(precision)
         nc = arrs["n_cands"]
         ncoef = int(arrs["cand_start"][-1]) + nc
         coefs = np.zeros(max(ncoef, 1))
@@ -301,7 +306,7 @@ class Context:
 
     def autotune(self, h, arrs, mode="ols", precision="fp64", reps=2):
         m = D["RR_MODE_OLS"] if mode == "ols" else D["RR_MODE_RAW"]
-        pr = D["RR_FP64"] if precision == "fp64" else D["RR_FP32_REFINED"]
+        pr = precision_code(precision)
         b = _as_batch(arrs)
         best = np.zeros(1)
         check(lib().rr_autotune(self.p, h, ctypes.byref(b), m, pr, reps, _dp(best)))

@@ -67,6 +67,47 @@ def test_fp32_certificates_hold_on_random_trees(ctx, n, d, seed):
     assert certified > len(facs) // 2
 
 
+@pytest.mark.parametrize("n,d,seed", [(3000, 4, 4), (100000, 10, 5)])
+def test_fp32_checked_form_is_final(ctx, n, d, seed):
+    """RR_FP32_CHECKED: no result is left uncertified; the results it takes
+    from the FP64 form (RR_F_FP64) are bit-for-bit those of RR_FP64, and the
+    others are exactly the RR_FP32_REFINED results, certified."""
+    print("case: fp32 checked form n=%d d=%d seed=%d" % (n, d, seed), flush=True)
+    rng = np.random.default_rng(seed)
+    X = rng.uniform(-3, 3, (n, d))
+    y = X[:, 0] * X[:, 1] + np.cos(X[:, 2]) + rng.normal(0, 0.05, n)
+    import rils_rols_cpp
+    trees = treegen.random_trees(seed, 500, 3, 50, d)
+    facs = [rils_rols_cpp._ols_factors(t) for t in trees]
+    facs = [f for f in facs if 1 <= len(f) <= 12]
+    facs += [["c0x1p+1"], ["x0", "c0x1.8p+1"], ["x0", "sin c0x1p+0", "x1"], ["- x0 x0", "x1"],
+             ["x0", "x1", "x2", "x3", "* x0 x1", "* x1 x2", "* x2 x3", "* x0 x3", "sqr x0"]]
+    h = ctx.upload(X, y)
+    arrs = capi.build_batch(facs)
+    c64, f64 = ctx.fitness(h, arrs, mode="ols", precision="fp64")
+    c32, f32 = ctx.fitness(h, arrs, mode="ols", precision="fp32")
+    cc, fc = ctx.fitness(h, arrs, mode="ols", precision="fp32c")
+    ctx.free(h)
+    n_fp64 = 0
+    for i in range(len(facs)):
+        a, b, c = f64[i], f32[i], fc[i]
+        assert not (c["flags"] & 2), (i, facs[i])
+        if c["flags"] & 4:
+            n_fp64 += 1
+            assert c["e0"].tobytes() == a["e0"].tobytes() and c["e1"].tobytes() == a["e1"].tobytes(), (i, facs[i])
+            assert c["rank"] == a["rank"] and (c["flags"] & ~4) == a["flags"], (i, facs[i])
+            assert cc[i].tobytes() == c64[i].tobytes(), (i, facs[i])
+        else:
+            assert not (b["flags"] & 2), (i, facs[i])
+            assert c.tobytes() == b.tobytes(), (i, facs[i])
+            assert cc[i].tobytes() == c32[i].tobytes(), (i, facs[i])
+    # the constant-factor and 9-factor candidates went to the FP64 form
+    for i in range(len(facs) - 5, len(facs)):
+        if i != len(facs) - 2:
+            assert fc[i]["flags"] & 4, (i, facs[i])
+    print("fp64 results %d of %d" % (n_fp64, len(facs)), flush=True)
+
+
 def _check_samples(v64, v32, e32, what):
     """Per-sample contract of the FP32 evaluator: a certain NaN is NaN in
     FP64; a finite bound covers the distance to the FP64 value."""
