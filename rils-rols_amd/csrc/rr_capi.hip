@@ -167,6 +167,26 @@ bool reads_variable(const uint32_t* code, int32_t len) {
   return false;
 }
 
+// Per-sample work of one candidate in the FP64 OLS form, in rough FP64
+// instruction units: its programs (a transcendental costs tens of
+// instructions) and the k^2 column passes of the QR.  Used only to launch the
+// long candidates first, so that one of them does not start last and end the
+// launch alone; it changes no result.
+int64_t fp64_cost(const rr_batch* b, int32_t c) {
+  int64_t w = 0;
+  for (int32_t p = b->cand_start[c]; p < b->cand_start[c + 1]; ++p) {
+    for (int32_t pc = b->prog_start[p]; pc < b->prog_start[p + 1]; ++pc) {
+      const uint32_t op = b->code[pc] & 0xffu;
+      if (op >= RR_OP_BIN) w += ((op - RR_OP_BIN) >> 3) == RR_BOP_DIV ? 8 : 1;
+      else if (op >= RR_OP_SIN && op <= RR_OP_EXP) w += 40;
+      else if (op == RR_OP_SQRT) w += 10;
+      else w += 1;
+    }
+  }
+  const int64_t k = b->cand_start[c + 1] - b->cand_start[c] + 1;
+  return w + 24 * k * k;
+}
+
 int check_batch(const rr_batch* b, int32_t mode, int32_t d) {
   if (!b || b->n_cands < 0) return fail(RR_E_ARG, "bad batch");
   if (b->n_cands == 0) return RR_OK;
@@ -522,10 +542,22 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
   // reused by a second call only after the host has synchronized with the
   // first; its staging goes to slot 0 or 1 of the pinned buffer.
   const size_t so64 = so;
-  auto fp64_ols = [&](const std::vector<int32_t>* list, hipStream_t stream, int slot) -> int {
-    const int32_t m = list ? (int32_t)list->size() : nc;
+  auto fp64_ols = [&](const std::vector<int32_t>* list_in, hipStream_t stream, int slot) -> int {
+    std::vector<int32_t> all;
+    if (!list_in) {
+      all.resize(nc);
+      for (int32_t c = 0; c < nc; ++c) all[c] = c;
+    }
+    std::vector<int32_t> lpt = list_in ? *list_in : all;   // longest first
+    {
+      std::vector<int64_t> cost(nc);
+      for (int32_t c : lpt) cost[c] = fp64_cost(b, c);
+      std::stable_sort(lpt.begin(), lpt.end(), [&](int32_t u, int32_t v) { return cost[u] > cost[v]; });
+    }
+    const std::vector<int32_t>* list = &lpt;
+    const int32_t m = (int32_t)list->size();
     if (m == 0) return RR_OK;
-    auto cand_at = [&](int32_t q) { return list ? (*list)[q] : q; };
+    auto cand_at = [&](int32_t q) { return (*list)[q]; };
     DevBuf& offbuf = ctx->d_f64_off;
     DevBuf& ordbuf = ctx->d_f64_ord;
     std::vector<int64_t> off(m);
@@ -554,7 +586,7 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
     if (slab.ensure((size_t)max_words * 8) != RR_OK) return RR_E_NOMEM;
     const size_t base = so64 + (size_t)slot * s_f64;
     if ((rc = upload(ctx, offbuf, off.data(), (size_t)m * 8, base))) return rc;
-    if (list) { if ((rc = upload(ctx, ordbuf, list->data(), (size_t)m * 4, base + align_up((size_t)nc * 8)))) return rc; }
+    if ((rc = upload(ctx, ordbuf, list->data(), (size_t)m * 4, base + align_up((size_t)nc * 8)))) return rc;
     if (stream != ctx->stream) {   // the uploads ran on the context's stream
       RR_HIP(hipEventRecord(ctx->fork64, ctx->stream));
       RR_HIP(hipStreamWaitEvent(stream, ctx->fork64, 0));
@@ -564,7 +596,7 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
       L.cand_start = dc; L.prog_start = dp; L.code = dcode; L.K = dK; L.ds = dtab; L.cand_ds = dcds;
       L.slab = (double*)slab.p;
       L.slab_off = (const int64_t*)offbuf.p + ch.c0; L.coefs = dcoef; L.fit = dfit;
-      L.cand_base = ch.c0; L.order = list ? (const int32_t*)ordbuf.p : nullptr;
+      L.cand_base = ch.c0; L.order = (const int32_t*)ordbuf.p;
       L.n_cands = ch.c1 - ch.c0; L.max_cols = ch.maxk; L.rows = (int)ctx->fp64_rows;
       RR_HIP(rr::launch_ols_fp64(L, stream));
     }
