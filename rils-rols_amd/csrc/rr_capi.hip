@@ -74,6 +74,7 @@ struct Dataset {
   double* X = nullptr;    // d columns, stride ldx
   double* y = nullptr;
   float* X32 = nullptr;   // FP32 copy for the precision form
+  double* one = nullptr;  // ldx ones (the free term's column of the FP64 form)
   double ybar = 0, sst = 0, ysum = 0, yy = 0;
 };
 
@@ -94,6 +95,7 @@ struct rr_ctx {
   hipEvent_t fork64 = nullptr, join64 = nullptr;
   DevBuf d_cand, d_prog, d_code, d_K, d_slab, d_order32, d_coefs, d_fit, d_misc, d_pred_e, d_dstab, d_cand_ds;
   DevBuf d_f64_off, d_f64_ord;   // FP64 slab offsets and launched position -> candidate
+  DevBuf d_dead;                 // per candidate: decided by the factor-0 pre-pass (OLS mode)
   DevBuf d_blk_cand, d_blk_chunk, d_chunk_base, d_work, d_work_flags;   // FP32 chunk map and partials
   HostBuf h_stage, h_out;
   int64_t slab_budget_mb = 32768;
@@ -264,12 +266,12 @@ int rr_destroy(rr_ctx* ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   for (Dataset& d : ctx->ds) {
     if (!d.live) continue;
-    (void)hipFree(d.X); (void)hipFree(d.y); (void)hipFree(d.X32);
+    (void)hipFree(d.X); (void)hipFree(d.y); (void)hipFree(d.X32); (void)hipFree(d.one);
   }
   DevBuf* bufs[] = {&ctx->d_cand, &ctx->d_prog, &ctx->d_code, &ctx->d_K, &ctx->d_slab, &ctx->d_order32,
                     &ctx->d_coefs, &ctx->d_fit, &ctx->d_misc, &ctx->d_pred_e, &ctx->d_dstab, &ctx->d_cand_ds,
                     &ctx->d_blk_cand, &ctx->d_blk_chunk, &ctx->d_chunk_base, &ctx->d_work, &ctx->d_work_flags,
-                    &ctx->d_f64_off, &ctx->d_f64_ord};
+                    &ctx->d_f64_off, &ctx->d_f64_ord, &ctx->d_dead};
   for (DevBuf* b : bufs) b->release();
   ctx->h_stage.release();
   ctx->h_out.release();
@@ -305,6 +307,11 @@ int rr_dataset_upload(rr_ctx* ctx, const double* X_fm, const double* y, int64_t 
     (void)hipFree(D.X); (void)hipFree(D.y);
     return fail(RR_E_NOMEM, "hipMalloc for the FP32 copy failed");
   }
+  if (hipMalloc(&D.one, (size_t)D.ldx * sizeof(double)) != hipSuccess) {
+    (void)hipFree(D.X); (void)hipFree(D.y); (void)hipFree(D.X32);
+    return fail(RR_E_NOMEM, "hipMalloc for the column of ones failed");
+  }
+  RR_HIP(rr::launch_fill_f64(D.one, D.ldx, 1.0, ctx->stream));
   RR_HIP(hipMemsetAsync(D.X, 0, xb, ctx->stream));
   RR_HIP(hipMemsetAsync(D.y, 0, (size_t)D.ldx * sizeof(double), ctx->stream));
   if (d > 0)
@@ -333,7 +340,7 @@ int rr_dataset_free(rr_ctx* ctx, int32_t handle) {
   Dataset* D = get_ds(ctx, handle);
   if (!D) return fail(RR_E_ARG, "bad dataset handle");
   RR_HIP(hipStreamSynchronize(ctx->stream));
-  (void)hipFree(D->X); (void)hipFree(D->y); (void)hipFree(D->X32);
+  (void)hipFree(D->X); (void)hipFree(D->y); (void)hipFree(D->X32); (void)hipFree(D->one);
   *D = Dataset();
   return RR_OK;
 }
@@ -410,7 +417,7 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
   std::vector<rr::DsDesc> table(uniq.size());
   for (size_t u = 0; u < uniq.size(); ++u) {
     const Dataset& D = ctx->ds[uniq[u]];
-    table[u] = rr::DsDesc{D.X, D.X32, D.y, D.n, D.ldx, D.sst, D.ysum, D.yy};
+    table[u] = rr::DsDesc{D.X, D.X32, D.y, D.n, D.ldx, D.sst, D.ysum, D.yy, D.one};
   }
   auto ds_of = [&](int32_t c) -> const Dataset& { return ctx->ds[uniq[handles ? cand_ds[c] : 0]]; };
 
@@ -504,6 +511,11 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
   so += s_cds;
   if (ctx->d_fit.ensure((size_t)nc * sizeof(rr_fit)) != RR_OK) return RR_E_NOMEM;
   if (ctx->d_coefs.ensure((size_t)std::max<int64_t>(ncoef, 1) * 8) != RR_OK) return RR_E_NOMEM;
+  const int32_t* ddead = nullptr;   // OLS mode: written by the pre-pass for every candidate
+  if (mode == RR_MODE_OLS) {
+    if (ctx->d_dead.ensure((size_t)std::max(nc, 1) * 4) != RR_OK) return RR_E_NOMEM;
+    ddead = (const int32_t*)ctx->d_dead.p;
+  }
 
   const int32_t* dc = (const int32_t*)ctx->d_cand.p;
   const int32_t* dp = (const int32_t*)ctx->d_prog.p;
@@ -528,7 +540,7 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
         ctx->d_work_flags.ensure((size_t)std::max<int64_t>(total_chunks, 1) * 8) != RR_OK)
       return RR_E_NOMEM;
     L32.cand_start = dc; L32.prog_start = dp; L32.code = dcode; L32.K = dK; L32.ds = dtab; L32.cand_ds = dcds;
-    L32.coefs = dcoef; L32.fit = dfit; L32.n_cands = n32; L32.raw = mode == RR_MODE_RAW ? 1 : 0;
+    L32.coefs = dcoef; L32.fit = dfit; L32.dead = ddead; L32.n_cands = n32; L32.raw = mode == RR_MODE_RAW ? 1 : 0;
     L32.rows = (int)ctx->fp32_rows;
     L32.blk_cand = (const int32_t*)ctx->d_blk_cand.p; L32.blk_chunk = (const int32_t*)ctx->d_blk_chunk.p;
     L32.chunk_base = (const int32_t*)ctx->d_chunk_base.p; L32.chunk_rows = ctx->fp32_chunk_rows;
@@ -596,7 +608,7 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
       L.cand_start = dc; L.prog_start = dp; L.code = dcode; L.K = dK; L.ds = dtab; L.cand_ds = dcds;
       L.slab = (double*)slab.p;
       L.slab_off = (const int64_t*)offbuf.p + ch.c0; L.coefs = dcoef; L.fit = dfit;
-      L.cand_base = ch.c0; L.order = (const int32_t*)ordbuf.p;
+      L.cand_base = ch.c0; L.order = (const int32_t*)ordbuf.p; L.dead = ddead;
       L.n_cands = ch.c1 - ch.c0; L.max_cols = ch.maxk; L.rows = (int)ctx->fp64_rows;
       RR_HIP(rr::launch_ols_fp64(L, stream));
     }
@@ -627,6 +639,11 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
 
   std::vector<int32_t> rescored;   // candidates whose result came from the FP64 form (checked mode)
   RR_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+  if (mode == RR_MODE_OLS && nc > 0) {
+    // candidates whose factor 0 is NaN at an early row >= 1: outcome written here, skipped below
+    rr::Nan0Launch L0{dc, dp, dcode, dK, dtab, dcds, dcoef, dfit, (int32_t*)ctx->d_dead.p, nc};
+    RR_HIP(rr::launch_nan0_fp64(L0, ctx->stream));
+  }
   if (mode == RR_MODE_RAW && precision == RR_FP64) {
     rr::RawLaunch L{dc, dp, dcode, dK, dtab, dcds, dfit, nc, (int)ctx->fp64_rows};
     RR_HIP(rr::launch_raw_fp64(L, ctx->stream));

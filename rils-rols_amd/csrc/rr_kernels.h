@@ -18,6 +18,7 @@ struct DsDesc {
   const double* y;
   int64_t n, ldx;
   double sst, ysum, yy;   // sum (y - mean)^2 (canonical order), sum y, sum y^2
+  const double* one;      // ldx ones: the column of the OLS free term
 };
 
 struct OlsLaunch {
@@ -33,6 +34,7 @@ struct OlsLaunch {
   rr_fit* fit;
   int cand_base;              // first launched position of this launch
   const int32_t* order;       // launched position -> candidate (null: identity)
+  const int32_t* dead;        // per candidate: outcome already written by the factor-0 pre-pass (or null)
   int n_cands;                // candidates in this launch
   int max_cols;               // largest k among them
   int rows;                   // rows per lane per tile (1, 2 or 4)
@@ -50,10 +52,30 @@ struct RawLaunch {
   int rows;
 };
 
+// Factor-0 pre-pass of the OLS form: one workgroup per candidate evaluates
+// factor 0 in FP64 on rows 0..NAN0_ROWS-1.  A NaN at a row >= 1 fixes the outcome (every
+// coefficient NaN, the sentinel fitness; the FP64 kernel reaches the same
+// result when it sees it), which is written here; dead[c] = 1 then, else 0.
+struct Nan0Launch {
+  const int32_t* cand_start;
+  const int32_t* prog_start;
+  const uint32_t* code;
+  const double* K;
+  const DsDesc* ds;
+  const int32_t* cand_ds;
+  double* coefs;
+  rr_fit* fit;
+  int32_t* dead;
+  int n_cands;
+};
+constexpr int NAN0_ROWS = 256;
+
 hipError_t launch_ols_fp64(const OlsLaunch& L, hipStream_t s);
+hipError_t launch_nan0_fp64(const Nan0Launch& L, hipStream_t s);
 hipError_t launch_raw_fp64(const RawLaunch& L, hipStream_t s);
 hipError_t launch_predict_fp64(const uint32_t* code, int len, const double* K, const double* X, int64_t n,
                                int64_t ldx, double* out, hipStream_t s);
+hipError_t launch_fill_f64(double* p, int64_t count, double value, hipStream_t s);
 // out4 = {mean(y), sum((y - mean)^2), sum(y), sum(y^2)}
 hipError_t launch_stats_fp64(const double* y, int64_t n, double* out4, hipStream_t s);
 

@@ -17,6 +17,7 @@ struct Fp32Launch {
   const int32_t* cand_ds;   // per candidate (batch index), or null
   double* coefs;
   rr_fit* fit;
+  const int32_t* dead;   // per candidate: outcome written by the factor-0 pre-pass (OLS), or null
   int n_cands;
   int raw;            // 1: score the tree as given, 0: OLS
   int rows;           // samples per lane per tile: 1 or 2

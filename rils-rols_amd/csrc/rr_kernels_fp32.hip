@@ -305,6 +305,7 @@ struct Fp32Args {
   double* coefs;
   rr_fit* fit;
   const int32_t* order;   // candidates of this launch (indices into the batch)
+  const int32_t* dead;    // per candidate: outcome written by the factor-0 pre-pass, or null
 };
 
 __device__ inline void bind_dataset(Fp32Args& a, int cand) {
@@ -425,6 +426,7 @@ __global__ __launch_bounds__(64 * ACC_WAVES) void k_acc_fp32(Fp32Args a, ChunkAr
   __shared__ int abort_flag;
   const int q = blk0 + (int)blockIdx.x;
   const int cand = w.blk_cand[q];
+  if (a.dead && a.dead[cand]) return;   // uniform; k_fin_fp32 skips it too
   const int64_t chunk = w.blk_chunk[q];
   bind_dataset(a, cand);
   const int lane = (int)(threadIdx.x & 63u);
@@ -535,6 +537,7 @@ __global__ __launch_bounds__(64) void k_fin_fp32(Fp32Args a, ChunkArgs w) {
   __shared__ Solve<KB> S;
   __shared__ double accs[NACC];
   const int cand = a.order[blockIdx.x];
+  if (a.dead && a.dead[cand]) return;   // uniform (one wave)
   bind_dataset(a, cand);
   const int lane = (int)threadIdx.x;
   const int pbeg = a.cand_start[cand];
@@ -824,10 +827,12 @@ __global__ void k_to_fp32(const double* src, float* dst, int64_t count) {
   }
 }
 
-__global__ void k_needs64(const int32_t* order, int count, const int32_t* cand_start, double* coefs, rr_fit* fit) {
+__global__ void k_needs64(const int32_t* order, int count, const int32_t* cand_start, const int32_t* dead,
+                          double* coefs, rr_fit* fit) {
   const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   if (i < count) {
     const int c = order[i];
+    if (dead && dead[c]) return;
     const int p = cand_start[c];
     write_needs64(fit + c, coefs + p + c, cand_start[c + 1] - p + 1, NR_WIDE);
   }
@@ -855,6 +860,7 @@ hipError_t launch_fp32(const Fp32Launch& L, const int32_t* order, const int* buc
   Fp32Args a;
   a.cand_start = L.cand_start; a.prog_start = L.prog_start; a.code = L.code; a.K = L.K;
   a.ds = L.ds; a.cand_ds = L.cand_ds; a.coefs = L.coefs; a.fit = L.fit;
+  a.dead = L.raw ? nullptr : L.dead;
   ChunkArgs w;
   w.blk_cand = L.blk_cand; w.blk_chunk = L.blk_chunk; w.chunk_base = L.chunk_base; w.chunk_rows = L.chunk_rows;
   w.part = L.part; w.flags = L.flags;
@@ -889,7 +895,7 @@ hipError_t launch_fp32(const Fp32Launch& L, const int32_t* order, const int* buc
 #undef RR_OLS32
   if (bucket_count[4] > 0)
     hipLaunchKernelGGL(k_needs64, dim3((bucket_count[4] + 255) / 256), dim3(256), 0, streams[4], order + off[4],
-                       bucket_count[4], L.cand_start, L.coefs, L.fit);
+                       bucket_count[4], L.cand_start, a.dead, L.coefs, L.fit);
   return hipGetLastError();
 }
 

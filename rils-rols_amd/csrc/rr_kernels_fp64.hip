@@ -10,21 +10,28 @@
 // operation, so the results are the same bits as the CPU oracle's.
 //
 // Per candidate the kernel
-//   1. evaluates each factor program and writes its column to a work slab in
-//      HBM (tune_constants, rils_rols_cpp.cpp:477-482), accumulating the
-//      column norms on the way;
+//   1. evaluates each factor program (tune_constants, rils_rols_cpp.cpp:
+//      477-482) into a kept column of a work slab in HBM, accumulating the
+//      column norms on the way.  A factor that is a bare variable keeps no
+//      copy: its column is the dataset's;
 //   2. runs the pivoted QR on the slab: per step one pass computing the
 //      Householder dot products and one pass applying the reflector, which
 //      also accumulates the tail norms the next step and the LAWN-176 norm
-//      recomputation need; the reflectors are applied to a copy of y as they
-//      are made (Q^T b of _solve_impl).  The factorisation stops at the first
-//      step under Eigen's rank threshold, after which nothing changes the
-//      solution, and at the first reflector whose beta is NaN, after which
-//      the outcome is fixed (see nan_exit below);
+//      recomputation need; the reflectors are applied to y as they are made
+//      (Q^T b of _solve_impl).  The first step reads every column from where
+//      its values already are (kept column, dataset column, the dataset's
+//      column of ones for the free term, y) and its update pass writes the
+//      working copies; the essential part of a reflector is recomputed
+//      (v / (c0 - beta)) where it is used rather than stored, since nothing
+//      reads it after its step; the last step, with only y left, needs no
+//      update pass.  The factorisation stops at the first step under Eigen's
+//      rank threshold, after which nothing changes the solution, and at the
+//      first reflector whose beta is NaN, after which the outcome is fixed
+//      (see nan_exit below);
 //   3. back-substitutes on the leading r x r block and scatters the
 //      coefficients through the column permutation (zeros for the rest);
-//   4. reads the factor values kept from step 1 (a copy the QR does not
-//      touch) to form the prediction of the OLS tree that tune_constants
+//   4. reads the kept factor values (or the dataset's columns, which the QR
+//      does not touch) to form the prediction of the OLS tree that tune_constants
 //      builds (rils_rols_cpp.cpp:488-517): left-deep sum of
 //      c*f (or f when |c-1| < 1e-12), skipping |c| < 1e-12, free term last,
 //      and reduces SSR into 1-R2 and RMSE (fitness, 520-541).  A kept NaN
@@ -51,6 +58,7 @@ struct Fp64Args {
   // the candidate's dataset, bound at kernel start
   const double* X;
   const double* y;
+  const double* one;   // ldx ones: the free term's column
   int64_t n, ldx;
   double sst;
   double* slab;
@@ -59,6 +67,7 @@ struct Fp64Args {
   rr_fit* fit;
   int cand_base;
   const int32_t* order;   // launched position -> candidate, or null (identity)
+  const int32_t* dead;    // per candidate: outcome written by the factor-0 pre-pass, or null
 };
 
 __device__ inline void write_fit(rr_fit* f, double ssr, double sst, int64_t n, int rank, int steps = 0) {
@@ -80,7 +89,12 @@ __device__ inline void write_fit(rr_fit* f, double ssr, double sst, int64_t n, i
 template <class A>
 __device__ inline void bind_dataset(A& a, int cand) {
   const DsDesc& D = a.ds[a.cand_ds ? a.cand_ds[cand] : 0];
-  a.X = D.X; a.y = D.y; a.n = D.n; a.ldx = D.ldx; a.sst = D.sst;
+  a.X = D.X; a.y = D.y; a.one = D.one; a.n = D.n; a.ldx = D.ldx; a.sst = D.sst;
+}
+
+// Variable index of a program that is one LDV, else -1.
+__device__ inline int bare_variable(const uint32_t* code, int len) {
+  return (len == 1 && (code[0] & 0xffu) == RR_OP_LDV) ? (int)(code[0] >> 8) : -1;
 }
 
 __device__ inline int64_t tile_start(int64_t lo, int R) { return (lo / (CP * R)) * (CP * R); }
@@ -93,10 +107,12 @@ __global__ __launch_bounds__(CP) void k_ols_fp64(Fp64Args a) {
   constexpr int G = (R == 1) ? 8 : (R == 2) ? 4 : 2;   // columns per QR pass
   __shared__ double normU[MAXK], normD[MAXK], tsq[MAXK], cc[MAXK];
   __shared__ int phys[MAXK];
+  __shared__ int ovar[MAXK];   // per factor: its bare variable, or -1 (kept column)
   __shared__ double red1[CW][1], red2[CW][2], redG[CW][G], redS[CW][2 * G];
   __shared__ int abort_flag;
 
   const int cand = a.order ? a.order[a.cand_base + (int)blockIdx.x] : a.cand_base + (int)blockIdx.x;
+  if (a.dead && a.dead[cand]) return;   // uniform: the whole workgroup leaves
   bind_dataset(a, cand);
   const int tid = (int)threadIdx.x;
   const int pbeg = a.cand_start[cand];
@@ -104,7 +120,8 @@ __global__ __launch_bounds__(CP) void k_ols_fp64(Fp64Args a) {
   const int k = nf + 1;
   const int64_t n = a.n, ldx = a.ldx;
   // slab columns 0..k-1: A (factors, free term), k: Q^T y, k+1..k+nf: the
-  // factor values as evaluated (kept for the prediction of step 4)
+  // factor values as evaluated (kept for the first QR step and the
+  // prediction of step 4)
   double* const S = a.slab + a.slab_off[blockIdx.x];
   const double* const X = a.X;
   double* const co = a.coefs + (int64_t)pbeg + cand;
@@ -122,20 +139,26 @@ __global__ __launch_bounds__(CP) void k_ols_fp64(Fp64Args a) {
   for (int j = 0; j < nf; ++j) {
     const uint32_t* code = a.code + a.prog_start[pbeg + j];
     const int len = a.prog_start[pbeg + j + 1] - a.prog_start[pbeg + j];
-    double* col = S + (int64_t)j * ldx;
+    const int var = bare_variable(code, len);
+    if (tid == 0) ovar[j] = var;
     double* keep = S + (int64_t)(k + 1 + j) * ldx;
+    const double* xv = X + (int64_t)(var < 0 ? 0 : var) * ldx;
     double sums[2] = {0.0, 0.0};   // rows >= 0, rows >= 1
     for (int64_t base = 0; base < n; base += CP * R) {
       if (j == 0 && *abort_seen) break;
       const int64_t row0 = base + tid;
       double v[R];
-      eval_f64<R>(code, len, a.K, X, ldx, row0, v);
+      if (var >= 0) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = xv[row0 + (int64_t)CP * r];
+      } else {
+        eval_f64<R>(code, len, a.K, X, ldx, row0, v);
+      }
       bool nan_tail = false;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const int64_t i = row0 + (int64_t)CP * r;
-        col[i] = v[r];
-        keep[i] = v[r];
+        if (var < 0) keep[i] = v[r];
         if (i < n) {
           sums[0] = sums[0] + v[r] * v[r];
           if (i >= 1) sums[1] = sums[1] + v[r] * v[r];
@@ -157,10 +180,19 @@ __global__ __launch_bounds__(CP) void k_ols_fp64(Fp64Args a) {
     if (tid == 0) { normD[j] = __builtin_sqrt(sums[0]); tsq[j] = sums[1]; }
   }
   double* const cy = S + (int64_t)k * ldx;
-  {
-    double* col = S + (int64_t)nf * ldx;
-    for (int64_t i = tid; i < n; i += CP) { col[i] = 1.0; cy[i] = a.y[i]; }
-    if (tid == 0) { normD[nf] = __builtin_sqrt((double)n); tsq[nf] = (double)(n - 1); }
+  // where the values of physical column c are before the first QR step
+  auto origin = [&](int c) -> const double* {
+    if (c == nf) return a.one;
+    const int v = ovar[c];
+    return v >= 0 ? X + (int64_t)v * ldx : S + (int64_t)(k + 1 + c) * ldx;
+  };
+  // row 0 of every working column; rows >= 1 are written by the first
+  // update pass (the single-row and top-row updates work on row p in place)
+  if (tid == 0) {
+    for (int c = 0; c < k; ++c) S[(int64_t)c * ldx] = origin(c)[0];
+    cy[0] = a.y[0];
+    normD[nf] = __builtin_sqrt((double)n);
+    tsq[nf] = (double)(n - 1);
   }
   for (int j = tid; j < k; j += CP) phys[j] = j;
   __syncthreads();
@@ -193,6 +225,7 @@ __global__ __launch_bounds__(CP) void k_ols_fp64(Fp64Args a) {
     __syncthreads();
 
     double* const v = S + (int64_t)phys[p] * ldx;
+    const double* const vr = (p == 0) ? origin(phys[p]) : v;   // rows > p of the pivot column
     const double c0 = v[p];
     const double tailSq = (n - p == 1) ? 0.0 : tsq[p];
     double tau, beta;
@@ -200,7 +233,6 @@ __global__ __launch_bounds__(CP) void k_ols_fp64(Fp64Args a) {
     if (zero_tail) {
       tau = 0.0;
       beta = c0;
-      for (int64_t i = first_row(p + 1, tid); i < n; i += CP) v[i] = 0.0;
     } else {
       beta = __builtin_sqrt(c0 * c0 + tailSq);
       if (c0 >= 0.0) beta = -beta;
@@ -259,11 +291,14 @@ __global__ __launch_bounds__(CP) void k_ols_fp64(Fp64Args a) {
     const int64_t t0 = tile_start(p + 1, R);
     for (int g0 = 0; g0 < ntarget; g0 += G) {
       const int gn = (ntarget - g0 < G) ? (ntarget - g0) : G;
-      double* cols[G];
+      double* cols[G];          // working columns (written)
+      const double* rcols[G];   // where rows > p are read from
 #pragma unroll
       for (int t = 0; t < G; ++t) {
         const int tg = g0 + t;
-        cols[t] = (t < gn) ? ((tg == nrem) ? cy : S + (int64_t)phys[p + 1 + tg] * ldx) : cy;
+        const int c = (t < gn && tg < nrem) ? phys[p + 1 + tg] : -1;
+        cols[t] = c >= 0 ? S + (int64_t)c * ldx : cy;
+        rcols[t] = p > 0 ? cols[t] : (c >= 0 ? origin(c) : a.y);
       }
       double tdot[G];
 #pragma unroll
@@ -280,14 +315,14 @@ __global__ __launch_bounds__(CP) void k_ols_fp64(Fp64Args a) {
           for (int r = 0; r < R; ++r) {
             const int64_t i = base + tid + (int64_t)CP * r;
             ok[r] = i > p && i < n;
-            if (g0 == 0) { e[r] = v[i] / den; if (ok[r]) v[i] = e[r]; } else { e[r] = v[i]; }
+            e[r] = vr[i] / den;   // the essential part of the reflector
           }
 #pragma unroll
           for (int t = 0; t < G; ++t) {
             if (t < gn) {
 #pragma unroll
               for (int r = 0; r < R; ++r) {
-                const double x = cols[t][base + tid + (int64_t)CP * r];
+                const double x = rcols[t][base + tid + (int64_t)CP * r];
                 if (ok[r]) acc[t] = acc[t] + e[r] * x;
               }
             }
@@ -307,7 +342,9 @@ __global__ __launch_bounds__(CP) void k_ols_fp64(Fp64Args a) {
           for (int t = 0; t < G; ++t) if (t < gn) cols[t][p] = cols[t][p] - tau * tdot[t];
         }
       }
-      // apply the reflector and accumulate tail norms (rows >= p+1 and >= p+2)
+      // apply the reflector and accumulate tail norms (rows >= p+1 and >= p+2);
+      // with only y left nothing reads its rows > p again
+      if (nrem == 0) { __syncthreads(); continue; }
       double s12[2 * G];
 #pragma unroll
       for (int t = 0; t < 2 * G; ++t) s12[t] = 0.0;
@@ -318,7 +355,7 @@ __global__ __launch_bounds__(CP) void k_ols_fp64(Fp64Args a) {
         for (int r = 0; r < R; ++r) {
           const int64_t i = base + tid + (int64_t)CP * r;
           ok[r] = i > p && i < n;
-          te[r] = apply ? tau * v[i] : 0.0;
+          te[r] = apply ? tau * (vr[i] / den) : 0.0;
         }
 #pragma unroll
         for (int t = 0; t < G; ++t) {
@@ -326,9 +363,10 @@ __global__ __launch_bounds__(CP) void k_ols_fp64(Fp64Args a) {
 #pragma unroll
             for (int r = 0; r < R; ++r) {
               const int64_t i = base + tid + (int64_t)CP * r;
-              double x = cols[t][i];
+              double x = rcols[t][i];
               if (ok[r]) {
-                if (apply) { x = x - te[r] * tdot[t]; cols[t][i] = x; }
+                if (apply) x = x - te[r] * tdot[t];
+                if (apply || p == 0) cols[t][i] = x;
                 s12[t] = s12[t] + x * x;
                 if (i >= p + 2) s12[G + t] = s12[G + t] + x * x;
               }
@@ -414,7 +452,7 @@ __global__ __launch_bounds__(CP) void k_ols_fp64(Fp64Args a) {
     for (int j = 0; j < nf; ++j) {
       const double cj = co[j];
       if (__builtin_fabs(cj) < EPS12) continue;
-      const double* keep = S + (int64_t)(k + 1 + j) * ldx;
+      const double* keep = origin(j);
       double f[R];
 #pragma unroll
       for (int q = 0; q < R; ++q) f[q] = keep[row0 + (int64_t)CP * q];
@@ -444,6 +482,51 @@ __global__ __launch_bounds__(CP) void k_ols_fp64(Fp64Args a) {
   if (tid == 0) write_fit(a.fit + cand, ssr[0], a.sst, n, r, steps);
 }
 
+// The factor-0 pre-pass: one workgroup of NAN0_ROWS threads per candidate,
+// one row each, so it costs one evaluation of latency.
+struct Nan0Args {
+  const int32_t* cand_start;
+  const int32_t* prog_start;
+  const uint32_t* code;
+  const double* K;
+  const DsDesc* ds;
+  const int32_t* cand_ds;
+  double* coefs;
+  rr_fit* fit;
+  int32_t* dead;
+};
+
+__global__ __launch_bounds__(NAN0_ROWS) void k_nan0_fp64(Nan0Args a) {
+  __shared__ int found;
+  const int cand = (int)blockIdx.x;
+  const int tid = (int)threadIdx.x;
+  const DsDesc& D = a.ds[a.cand_ds ? a.cand_ds[cand] : 0];
+  const int64_t n = D.n;
+  const int pbeg = a.cand_start[cand];
+  const int nf = a.cand_start[cand + 1] - pbeg;
+  if (tid == 0) found = 0;
+  __syncthreads();
+  if (nf > 0) {   // row tid < NAN0_ROWS <= ldx (a multiple of 4096)
+    const uint32_t* code = a.code + a.prog_start[pbeg];
+    const int len = a.prog_start[pbeg + 1] - a.prog_start[pbeg];
+    double v[1];
+    eval_f64<1>(code, len, a.K, D.X, D.ldx, (int64_t)tid, v);
+    if (__any((int)(tid >= 1 && (int64_t)tid < n && v[0] != v[0])) && (tid & 63) == 0) found = 1;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    a.dead[cand] = found;
+    if (found) {
+      // the outcome k_ols_fp64 writes when it sees the same NaN
+      const int k = nf + 1;
+      const int size0 = (int)((n < (int64_t)k) ? n : (int64_t)k);
+      double* co = a.coefs + (int64_t)pbeg + cand;
+      for (int i = 0; i < k; ++i) co[i] = (i < size0) ? rrm::qnan() : 0.0;
+      write_fit(a.fit + cand, rrm::qnan(), D.sst, n, size0, 0);
+    }
+  }
+}
+
 struct RawArgs {
   const int32_t* cand_start;
   const int32_t* prog_start;
@@ -453,6 +536,7 @@ struct RawArgs {
   const int32_t* cand_ds;
   const double* X;
   const double* y;
+  const double* one;
   int64_t n, ldx;
   double sst;
   rr_fit* fit;
@@ -520,6 +604,7 @@ hipError_t launch_ols_fp64(const OlsLaunch& L, hipStream_t s) {
   a.ds = L.ds; a.cand_ds = L.cand_ds;
   a.slab = L.slab; a.slab_off = L.slab_off; a.coefs = L.coefs; a.fit = L.fit; a.cand_base = L.cand_base;
   a.order = L.order;
+  a.dead = L.dead;
   const dim3 grid(L.n_cands), block(CP);
   const int R = L.rows;
   if (L.max_cols <= 16) {
@@ -531,6 +616,13 @@ hipError_t launch_ols_fp64(const OlsLaunch& L, hipStream_t s) {
   } else {
     hipLaunchKernelGGL((k_ols_fp64<RR_MAX_COLS, 2>), grid, block, 0, s, a);
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_nan0_fp64(const Nan0Launch& L, hipStream_t s) {
+  if (L.n_cands <= 0) return hipSuccess;
+  Nan0Args a{L.cand_start, L.prog_start, L.code, L.K, L.ds, L.cand_ds, L.coefs, L.fit, L.dead};
+  hipLaunchKernelGGL(k_nan0_fp64, dim3((unsigned)L.n_cands), dim3(NAN0_ROWS), 0, s, a);
   return hipGetLastError();
 }
 
@@ -548,6 +640,18 @@ hipError_t launch_predict_fp64(const uint32_t* code, int len, const double* K, c
                                int64_t ldx, double* out, hipStream_t s) {
   const int64_t blocks = (n + 255) / 256;
   hipLaunchKernelGGL(k_predict_fp64, dim3((unsigned)blocks), dim3(256), 0, s, code, len, K, X, n, ldx, out);
+  return hipGetLastError();
+}
+
+__global__ void k_fill_f64(double* p, int64_t count, double value) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = value;
+}
+
+hipError_t launch_fill_f64(double* p, int64_t count, double value, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  const int64_t blocks = (count + 255) / 256;
+  hipLaunchKernelGGL(k_fill_f64, dim3((unsigned)(blocks < 4096 ? blocks : 4096)), dim3(256), 0, s, p, count, value);
   return hipGetLastError();
 }
 

@@ -261,7 +261,7 @@ double Server::now() const { return std::chrono::duration<double>(std::chrono::s
 
 int64_t Server::footprint(int64_t n_samples, int d) {
   const int64_t ldx = (n_samples + 4095) / 4096 * 4096;
-  return ldx * std::max(d, 1) * (8 + 4) + ldx * 8;
+  return ldx * std::max(d, 1) * (8 + 4) + ldx * 16;   // X (FP64 + FP32), y, the column of ones
 }
 
 int64_t Server::resident_bytes() {

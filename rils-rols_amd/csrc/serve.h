@@ -12,8 +12,8 @@
 // produces the same moves and model alone and in the mix.
 //
 // Admission: a run is admitted when fewer than max_active runs are resident
-// and its HBM footprint (its dataset: FP64 X and y plus the FP32 copy of X, at
-// the padded row stride) fits in the budget beside the resident ones;
+// and its HBM footprint (its dataset: FP64 X and y, the FP32 copy of X and a
+// column of ones, at the padded row stride) fits in the budget beside the resident ones;
 // otherwise it waits in FIFO order.  The work slab of the FP64 kernels is
 // shared and bounded separately (rr_set_option "slab_budget_mb").
 #pragma once

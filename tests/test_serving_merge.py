@@ -45,5 +45,5 @@ def test_merge_keeps_every_program():
 
 def test_footprint_is_padded_dataset_size():
     import rils_rols_cpp
-    assert rils_rols_cpp.Server.footprint(1, 1) == 4096 * 12 + 4096 * 8
-    assert rils_rols_cpp.Server.footprint(100000, 10) == 102400 * 10 * 12 + 102400 * 8
+    assert rils_rols_cpp.Server.footprint(1, 1) == 4096 * 12 + 4096 * 16
+    assert rils_rols_cpp.Server.footprint(100000, 10) == 102400 * 10 * 12 + 102400 * 16
