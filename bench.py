@@ -30,6 +30,7 @@ Prints one JSON line.  --gpus N with N > 1 exits: the program measures one GPU.
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -41,6 +42,7 @@ for p in (os.path.join(ROOT, "rils-rols_amd"), os.path.join(ROOT, "tests")):
         sys.path.insert(0, p)
 
 N_ROWS, N_FEAT, N_TREES = 100_000, 10, 4096
+BARE = re.compile(r"^x\d+$")   # a factor that is one variable
 PEAK_HBM_GBS = 8000.0
 PEAK_FP64_VALU_TFLOPS = 78.6
 PEAK_FP32_TFLOPS = 157.3
@@ -109,25 +111,36 @@ def ops_in(prefix):
     return sum(1 for t in prefix.split() if not (t.startswith("x") or (t.startswith("c") and t != "cos")))
 
 
-def fp64_algorithmic_bytes(k, steps, sentinel, n):
+def fp64_algorithmic_bytes(k, steps, sentinel, n, bare):
     """HBM bytes the FP64 OLS kernel must move for one candidate with k columns
-    (factors + free term) whose pivoted QR ran `steps` Householder steps: the
-    evaluation pass writes k+1 columns (A and the copy of y) plus a kept copy
-    of the k-1 factor columns and reads y;
-    QR step p reads the pivot column and the k-p remaining targets (k-p-1
-    columns of A plus y) in its dot pass and writes the Householder vector,
-    then reads the vector and the k-p targets and writes the targets back;
-    the score pass (skipped for sentinels) reads y and the kept factor
-    columns.  Features are read from
-    cache-resident X and not counted."""
+    (k-1 factors, `bare` of them bare variables) whose pivoted QR ran `steps`
+    Householder steps.  The dataset's columns (X, y and the column of ones) are
+    shared by every candidate and cache-resident, so they are not counted; the
+    kept columns of the other k-1-bare factors are.
+      evaluation   writes the kept columns;
+      QR step 0    dot pass reads the kept columns; unless y is the only
+                   target, the update pass reads them again and writes the k
+                   working columns (k-1 targets and y);
+      QR step p>0  dot pass reads the pivot and the k-p targets; unless y is
+                   the only target, the update pass reads them again and
+                   writes the targets;
+      score        (not for sentinels) reads the kept columns.
+    A sentinel decided by factor 0's NaN (steps == 0) moves nothing."""
     if sentinel and steps == 0:
-        return 0.0   # stopped in the evaluation pass at factor 0's first NaN tile
-    words = (k + 1) + (k - 1) + 1
-    for p in range(steps):
+        return 0.0
+    kept = k - 1 - bare
+    words = kept
+    if steps >= 1:
+        words += kept
+        if k >= 2:
+            words += kept + k
+    for p in range(1, steps):
         t = k - p
-        words += (1 + t + 1) + (1 + 2 * t)
+        words += 1 + t
+        if p < k - 1:
+            words += (1 + t) + t
     if not sentinel:
-        words += 1 + (k - 1)
+        words += kept
     return 8.0 * n * words
 
 
@@ -534,23 +547,24 @@ def main():
         r = res["fp64"]
         flags = r["fit"][:, 4].astype(np.int64)
         steps = flags >> 16
-        byts = float(sum(fp64_algorithmic_bytes(int(k), int(st), bool(fl & 1), N_ROWS)
-                         for k, st, fl in zip(ks, steps, flags)))
-        kernel_s = r["kms"].mean() / 1e3   # the FP64 form is one launch of k_ols_fp64
-        achieved = flops / kernel_s / 1e12
+        bare = [sum(1 for x in fs if BARE.match(x)) for fs in factors]
+        byts = float(sum(fp64_algorithmic_bytes(int(k), int(st), bool(fl & 1), N_ROWS, b)
+                         for k, st, fl, b in zip(ks, steps, flags, bare)))
+        # HIP events on the context's stream bracket the factor-0 pre-pass and the k_ols_fp64 launch
+        kernel_s = r["kms"].mean() / 1e3
+        achieved = byts / kernel_s / 1e9
         out["fp64"] = {
             "evals_per_s": N_TREES * args.steps / r["total"],
             "ms_per_step": 1e3 * r["total"] / args.steps,
             "kernel_ms": float(r["kms"].mean()),
             "p50_step_ms": float(np.percentile(r["lat"], 50) * 1e3),
             "p99_step_ms": float(np.percentile(r["lat"], 99) * 1e3),
-            "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64_VALU_TFLOPS, "unit": "TFLOP/s",
-                         "frac": achieved / PEAK_FP64_VALU_TFLOPS, "traffic": pmc_traffic("k_ols_fp64"),
-                         "kernel": "k_ols_fp64<16,%d>" % runner.get_option("fp64_rows"),
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                         "frac": achieved / PEAK_HBM_GBS, "traffic": pmc_traffic("k_ols_fp64"),
+                         "kernel": "k_nan0_fp64 + k_ols_fp64<16,%d>" % runner.get_option("fp64_rows"),
                          "launch_ms": float(r["kms"].mean()),
-                         "algorithmic_flops": flops},
-            "hbm_algorithmic_GBps": byts / kernel_s / 1e9,
-            "hbm_algorithmic_bytes": byts,
+                         "algorithmic_bytes": byts},
+            "valu_tflops_by_formula": flops / kernel_s / 1e12,
             "sentinel_frac": float(np.mean(r["fit"][:, 4].astype(int) & 1)),
         }
     if "fp32" in res:

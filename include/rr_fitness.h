@@ -167,6 +167,16 @@ int rr_dataset_stats(rr_ctx* ctx, int32_t handle, double* ybar, double* sst);
 int rr_fitness_batch(rr_ctx* ctx, int32_t handle, const rr_batch* batch, int32_t mode, int32_t precision,
                      double* coefs_out, rr_fit* fit_out);
 
+/* Validate a batch as rr_fitness_batch does before it launches anything:
+ * structure (monotone offsets, one program per candidate in RAW mode, at most
+ * RR_MAX_COLS OLS columns), and every program against the bytecode rules
+ * above (known opcodes, operands inside [0, n_features) and [0, n_consts),
+ * an accumulator where an operator needs one, stack depth within
+ * RR_MAX_STACK, and a balanced stack with a value at the end).  Needs no
+ * context or device.  No counterpart in the reference, whose trees are
+ * objects rather than code. */
+int rr_check_batch(const rr_batch* batch, int32_t mode, int32_t n_features);
+
 /* The same for candidates of several runs at once (continuous batching of
  * independent searches): candidate c is scored against dataset handles[c].
  * Results per candidate are bit-for-bit those of rr_fitness_batch on its own

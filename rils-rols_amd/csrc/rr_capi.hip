@@ -169,27 +169,65 @@ bool reads_variable(const uint32_t* code, int32_t len) {
   return false;
 }
 
-// Per-sample work of one candidate in the FP64 OLS form, in rough FP64
-// instruction units: its programs (a transcendental costs tens of
-// instructions) and the k^2 column passes of the QR.  Used only to launch the
-// long candidates first, so that one of them does not start last and end the
-// launch alone; it changes no result.
-int64_t fp64_cost(const rr_batch* b, int32_t c) {
-  int64_t w = 0;
-  for (int32_t p = b->cand_start[c]; p < b->cand_start[c + 1]; ++p) {
-    for (int32_t pc = b->prog_start[p]; pc < b->prog_start[p + 1]; ++pc) {
-      const uint32_t op = b->code[pc] & 0xffu;
-      if (op >= RR_OP_BIN) w += ((op - RR_OP_BIN) >> 3) == RR_BOP_DIV ? 8 : 1;
-      else if (op >= RR_OP_SIN && op <= RR_OP_EXP) w += 40;
-      else if (op == RR_OP_SQRT) w += 10;
-      else w += 1;
+// Per-opcode tables for the batch check: the class of every opcode value
+// (0: invalid) and its rough per-sample cost in FP64 instruction units (a
+// transcendental costs tens of instructions), so that the check of a whole
+// batch runs without a branch per instruction.
+enum : uint8_t { OC_BAD = 0, OC_LDV, OC_LDC, OC_PUSH, OC_UN, OC_BIN_STACK, OC_BIN_VAR, OC_BIN_CONST };
+struct OpTables {
+  uint8_t cls[256];
+  uint16_t cost[256];
+  OpTables() {
+    for (int op = 0; op < 256; ++op) {
+      cls[op] = OC_BAD;
+      cost[op] = 1;
+      if (op == (int)RR_OP_LDV) cls[op] = OC_LDV;
+      else if (op == (int)RR_OP_LDC) cls[op] = OC_LDC;
+      else if (op == (int)RR_OP_PUSH) cls[op] = OC_PUSH;
+      else if (op >= (int)RR_OP_SIN && op <= (int)RR_OP_SQR) {
+        cls[op] = OC_UN;
+        cost[op] = op <= (int)RR_OP_EXP ? 40 : op == (int)RR_OP_SQRT ? 10 : 1;
+      } else if (op >= (int)RR_OP_BIN) {
+        const int bop = (op - (int)RR_OP_BIN) >> 3, m = (op - (int)RR_OP_BIN) & 7;
+        if (bop <= (int)RR_BOP_MAX && m <= (int)RR_M_CR)
+          cls[op] = (m == (int)RR_M_S || m == (int)RR_M_SR) ? OC_BIN_STACK
+                    : (m == (int)RR_M_V || m == (int)RR_M_VR) ? OC_BIN_VAR : OC_BIN_CONST;
+        if (bop == (int)RR_BOP_DIV) cost[op] = 8;
+      }
     }
   }
-  const int64_t k = b->cand_start[c + 1] - b->cand_start[c] + 1;
-  return w + 24 * k * k;
+};
+const OpTables g_ops;
+
+// check_program's accept/reject decision without its messages; adds the
+// program's cost to *w.
+bool program_ok(const uint32_t* code, int32_t len, int32_t d, int32_t n_consts, int64_t* w) {
+  if (len <= 0) return false;
+  int sp = 0, acc = 0, bad = 0;
+  int64_t cw = 0;
+  for (int32_t pc = 0; pc < len; ++pc) {
+    const uint32_t ins = code[pc], op = ins & 0xffu, arg = ins >> 8;
+    const int k = g_ops.cls[op];
+    cw += g_ops.cost[op];
+    bad |= k == OC_BAD;
+    bad |= (k >= OC_PUSH) & !acc;   // push, unary and binary operators need the accumulator
+    sp += (k == OC_PUSH) - (k == OC_BIN_STACK);
+    bad |= (sp < 0) | (sp > RR_MAX_STACK);
+    bad |= ((k == OC_LDV) | (k == OC_BIN_VAR)) & ((int64_t)arg >= d);
+    bad |= ((k == OC_LDC) | (k == OC_BIN_CONST)) & ((int64_t)arg >= n_consts);
+    acc = (k != OC_PUSH) & (acc | (k == OC_LDV) | (k == OC_LDC));
+  }
+  *w += cw;
+  return !bad && sp == 0 && acc;
 }
 
-int check_batch(const rr_batch* b, int32_t mode, int32_t d) {
+// Structure of a batch and every program, each against the feature count of
+// its candidate's dataset (cand_d[c], or d for all when cand_d is null).
+// cost[c], when asked for, gets the candidate's per-sample work in the FP64
+// OLS form (programs plus 24 k^2 for the QR passes): used only to launch the
+// long candidates first, so that one of them does not start last and end the
+// launch alone; it changes no result.
+int check_batch(const rr_batch* b, int32_t mode, int32_t d, const int32_t* cand_d, int64_t* cost) {
   if (!b || b->n_cands < 0) return fail(RR_E_ARG, "bad batch");
   if (b->n_cands == 0) return RR_OK;
   if (!b->cand_start || !b->prog_start || (b->n_consts > 0 && !b->consts)) return fail(RR_E_ARG, "null batch array");
@@ -203,14 +241,24 @@ int check_batch(const rr_batch* b, int32_t mode, int32_t d) {
   const int32_t n_progs = b->cand_start[b->n_cands];
   if (b->prog_start[0] != 0) return fail(RR_E_ARG, "prog_start[0] must be 0");
   if (b->prog_start[n_progs] > 0 && !b->code) return fail(RR_E_ARG, "null code array");
-  for (int32_t p = 0; p < n_progs; ++p) {
-    const int32_t len = b->prog_start[p + 1] - b->prog_start[p];
-    int rc = check_program(b->code + b->prog_start[p], len, d, b->n_consts, nullptr);
-    if (rc != RR_OK) {
+  for (int32_t p = 0; p < n_progs; ++p)
+    if (b->prog_start[p + 1] < b->prog_start[p]) return fail(RR_E_ARG, "prog_start not monotone");
+  for (int32_t c = 0; c < b->n_cands; ++c) {
+    const int32_t dc = cand_d ? cand_d[c] : d;
+    int64_t w = 0;
+    for (int32_t p = b->cand_start[c]; p < b->cand_start[c + 1]; ++p) {
+      const int32_t len = b->prog_start[p + 1] - b->prog_start[p];
+      if (program_ok(b->code + b->prog_start[p], len, dc, b->n_consts, &w)) continue;
+      int rc = check_program(b->code + b->prog_start[p], len, dc, b->n_consts, nullptr);   // for the message
+      if (rc == RR_OK) rc = fail(RR_E_ARG, "invalid program");
       char m[64];
       snprintf(m, sizeof m, " (program %d)", p);
       g_err += m;
       return rc;
+    }
+    if (cost) {
+      const int64_t k = b->cand_start[c + 1] - b->cand_start[c] + 1;
+      cost[c] = w + 24 * k * k;
     }
   }
   return RR_OK;
@@ -232,6 +280,12 @@ size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
 extern "C" {
 
 const char* rr_last_error(void) { return g_err.c_str(); }
+
+int rr_check_batch(const rr_batch* b, int32_t mode, int32_t n_features) {
+  if (mode != RR_MODE_OLS && mode != RR_MODE_RAW) return fail(RR_E_ARG, "bad mode");
+  if (n_features < 0) return fail(RR_E_ARG, "bad feature count");
+  return check_batch(b, mode, n_features, nullptr, nullptr);
+}
 
 int rr_create(int device, rr_ctx** out) {
   if (!out) return fail(RR_E_ARG, "null output pointer");
@@ -393,19 +447,14 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
       min_d = std::min(min_d, D->d);
     }
   }
-  if (handles) {
-    // structure, constants and stack discipline first, then every program
-    // against the feature count of its own candidate's dataset
-    if ((rc = check_batch(b, mode, INT32_MAX))) return rc;
-    for (int32_t c = 0; c < nc; ++c) {
-      const int32_t d = get_ds(ctx, handles[c])->d;
-      for (int32_t p = b->cand_start[c]; p < b->cand_start[c + 1]; ++p)
-        if ((rc = check_program(b->code + b->prog_start[p], b->prog_start[p + 1] - b->prog_start[p], d, b->n_consts,
-                                nullptr)))
-          return rc;
+  std::vector<int64_t> cost(std::max(nc, 0));
+  {
+    std::vector<int32_t> cand_d;
+    if (handles) {
+      cand_d.resize(nc);
+      for (int32_t c = 0; c < nc; ++c) cand_d[c] = ctx->ds[uniq[cand_ds[c]]].d;
     }
-  } else if ((rc = check_batch(b, mode, min_d))) {
-    return rc;
+    if ((rc = check_batch(b, mode, min_d, handles ? cand_d.data() : nullptr, cost.data()))) return rc;
   }
   if (!fit_out) return fail(RR_E_ARG, "null fit_out");
   if (mode == RR_MODE_OLS && !coefs_out) return fail(RR_E_ARG, "null coefs_out");
@@ -554,34 +603,35 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
   // reused by a second call only after the host has synchronized with the
   // first; its staging goes to slot 0 or 1 of the pinned buffer.
   const size_t so64 = so;
-  auto fp64_ols = [&](const std::vector<int32_t>* list_in, hipStream_t stream, int slot) -> int {
-    std::vector<int32_t> all;
-    if (!list_in) {
-      all.resize(nc);
-      for (int32_t c = 0; c < nc; ++c) all[c] = c;
-    }
-    std::vector<int32_t> lpt = list_in ? *list_in : all;   // longest first
+  // FP64 OLS launches over a candidate list (null: every candidate), longest
+  // first, chunked so that each launch's slab fits the budget.  plan64 does
+  // the host work and the uploads, launch64 only launches.
+  struct Chunk64 { int c0, c1, maxk; };
+  auto plan64 = [&](const std::vector<int32_t>* list_in, int slot, std::vector<Chunk64>& chunks) -> int {
+    chunks.clear();
+    const int32_t m = list_in ? (int32_t)list_in->size() : nc;
+    std::vector<int32_t> lpt(m);
     {
-      std::vector<int64_t> cost(nc);
-      for (int32_t c : lpt) cost[c] = fp64_cost(b, c);
-      std::stable_sort(lpt.begin(), lpt.end(), [&](int32_t u, int32_t v) { return cost[u] > cost[v]; });
+      // counting sort by cost, longest first (ties in list order)
+      constexpr int64_t NB = 8192;
+      auto key = [&](int32_t c) { return (int)(NB - 1 - std::min<int64_t>(cost[c], NB - 1)); };
+      std::vector<int32_t> cnt(NB + 1, 0);
+      for (int32_t q = 0; q < m; ++q) ++cnt[key(list_in ? (*list_in)[q] : q)];
+      for (int64_t i = 0, a = 0; i <= NB; ++i) { const int32_t t = cnt[i]; cnt[i] = (int32_t)a; a += t; }
+      for (int32_t q = 0; q < m; ++q) {
+        const int32_t c = list_in ? (*list_in)[q] : q;
+        lpt[cnt[key(c)]++] = c;
+      }
     }
-    const std::vector<int32_t>* list = &lpt;
-    const int32_t m = (int32_t)list->size();
     if (m == 0) return RR_OK;
-    auto cand_at = [&](int32_t q) { return (*list)[q]; };
-    DevBuf& offbuf = ctx->d_f64_off;
-    DevBuf& ordbuf = ctx->d_f64_ord;
     std::vector<int64_t> off(m);
     const int64_t budget = std::max<int64_t>(ctx->slab_budget_mb, 1) << 20;
-    struct Chunk { int c0, c1, maxk; };
-    std::vector<Chunk> chunks;
     int64_t max_words = 0;
     for (int q = 0; q < m;) {
       int64_t used = 0;
       int q1 = q, maxk = 0;
       while (q1 < m) {
-        const int32_t c = cand_at(q1);
+        const int32_t c = lpt[q1];
         const int k = b->cand_start[c + 1] - b->cand_start[c] + 1;
         const int64_t need = (int64_t)(2 * k) * ds_of(c).ldx;   // A, Q^T y, kept factor values
         if (q1 > q && (used + need) * 8 > budget) break;
@@ -594,21 +644,24 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
       max_words = std::max(max_words, used);
       q = q1;
     }
-    DevBuf& slab = ctx->d_slab;
-    if (slab.ensure((size_t)max_words * 8) != RR_OK) return RR_E_NOMEM;
+    if (ctx->d_slab.ensure((size_t)max_words * 8) != RR_OK) return RR_E_NOMEM;
     const size_t base = so64 + (size_t)slot * s_f64;
-    if ((rc = upload(ctx, offbuf, off.data(), (size_t)m * 8, base))) return rc;
-    if ((rc = upload(ctx, ordbuf, list->data(), (size_t)m * 4, base + align_up((size_t)nc * 8)))) return rc;
-    if (stream != ctx->stream) {   // the uploads ran on the context's stream
+    if ((rc = upload(ctx, ctx->d_f64_off, off.data(), (size_t)m * 8, base))) return rc;
+    if ((rc = upload(ctx, ctx->d_f64_ord, lpt.data(), (size_t)m * 4, base + align_up((size_t)nc * 8)))) return rc;
+    return RR_OK;
+  };
+  auto launch64 = [&](const std::vector<Chunk64>& chunks, hipStream_t stream) -> int {
+    if (chunks.empty()) return RR_OK;
+    if (stream != ctx->stream) {   // after the work already queued on the context's stream
       RR_HIP(hipEventRecord(ctx->fork64, ctx->stream));
       RR_HIP(hipStreamWaitEvent(stream, ctx->fork64, 0));
     }
-    for (const Chunk& ch : chunks) {
+    for (const Chunk64& ch : chunks) {
       rr::OlsLaunch L;
       L.cand_start = dc; L.prog_start = dp; L.code = dcode; L.K = dK; L.ds = dtab; L.cand_ds = dcds;
-      L.slab = (double*)slab.p;
-      L.slab_off = (const int64_t*)offbuf.p + ch.c0; L.coefs = dcoef; L.fit = dfit;
-      L.cand_base = ch.c0; L.order = (const int32_t*)ordbuf.p; L.dead = ddead;
+      L.slab = (double*)ctx->d_slab.p;
+      L.slab_off = (const int64_t*)ctx->d_f64_off.p + ch.c0; L.coefs = dcoef; L.fit = dfit;
+      L.cand_base = ch.c0; L.order = (const int32_t*)ctx->d_f64_ord.p; L.dead = ddead;
       L.n_cands = ch.c1 - ch.c0; L.max_cols = ch.maxk; L.rows = (int)ctx->fp64_rows;
       RR_HIP(rr::launch_ols_fp64(L, stream));
     }
@@ -638,6 +691,10 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
   };
 
   std::vector<int32_t> rescored;   // candidates whose result came from the FP64 form (checked mode)
+  std::vector<Chunk64> plan;
+  if (mode == RR_MODE_OLS && precision == RR_FP64) { if ((rc = plan64(nullptr, 0, plan))) return rc; }
+  if (checked && !list64.empty()) { if ((rc = plan64(&list64, 0, plan))) return rc; }
+  // the timed region starts with every input uploaded
   RR_HIP(hipEventRecord(ctx->ev0, ctx->stream));
   if (mode == RR_MODE_OLS && nc > 0) {
     // candidates whose factor 0 is NaN at an early row >= 1: outcome written here, skipped below
@@ -648,13 +705,13 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
     rr::RawLaunch L{dc, dp, dcode, dK, dtab, dcds, dfit, nc, (int)ctx->fp64_rows};
     RR_HIP(rr::launch_raw_fp64(L, ctx->stream));
   } else if (precision == RR_FP64) {
-    if ((rc = fp64_ols(nullptr, ctx->stream, 0))) return rc;
+    if ((rc = launch64(plan, ctx->stream))) return rc;
   } else if (!checked) {
     if ((rc = fp32_run())) return rc;
   } else {
     // FP64 for the predicted candidates beside the FP32 form, then FP64 for
     // the ones the FP32 form could not certify
-    if (!list64.empty()) { if ((rc = fp64_ols(&list64, ctx->side64, 0))) return rc; }
+    if ((rc = launch64(plan, ctx->side64))) return rc;
     if ((rc = fp32_run())) return rc;
     if (!list64.empty()) {
       RR_HIP(hipEventRecord(ctx->join64, ctx->side64));
@@ -665,7 +722,10 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
     RR_HIP(hipStreamSynchronize(ctx->stream));
     std::vector<int32_t> again;
     for (int32_t c : list32) if (f32[c].flags & RR_F_NEEDS64) again.push_back(c);
-    if (!again.empty()) { if ((rc = fp64_ols(&again, ctx->stream, 1))) return rc; }
+    if (!again.empty()) {
+      if ((rc = plan64(&again, 1, plan))) return rc;
+      if ((rc = launch64(plan, ctx->stream))) return rc;
+    }
     rescored = list64;
     rescored.insert(rescored.end(), again.begin(), again.end());
   }

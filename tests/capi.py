@@ -74,6 +74,7 @@ def lib():
         L.rr_set_option.argtypes = [vp, ctypes.c_char_p, ctypes.c_int64]
         L.rr_get_option.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64)]
         L.rr_last_kernel_ms.argtypes = [vp, dp]
+        L.rr_check_batch.argtypes = [ctypes.POINTER(Batch), ctypes.c_int32, ctypes.c_int32]
         _lib = L
     return _lib
 
@@ -213,6 +214,13 @@ def _as_batch(arrs):
     b.consts = arrs["consts"].ctypes.data_as(ctypes.POINTER(ctypes.c_double))
     b.n_consts = arrs["n_consts"]
     return b
+
+
+def check_batch(arrs, mode="ols", n_features=0):
+    """rr_check_batch: (return code, message); needs no device."""
+    m = D["RR_MODE_OLS"] if mode == "ols" else D["RR_MODE_RAW"]
+    rc = lib().rr_check_batch(ctypes.byref(_as_batch(arrs)), m, n_features)
+    return rc, (lib().rr_last_error().decode() if rc else "")
 
 
 def precision_code(name):
