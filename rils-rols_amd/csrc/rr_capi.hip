@@ -95,6 +95,7 @@ struct rr_ctx {
   hipEvent_t fork64 = nullptr, join64 = nullptr;
   DevBuf d_cand, d_prog, d_code, d_K, d_slab, d_order32, d_coefs, d_fit, d_misc, d_pred_e, d_dstab, d_cand_ds;
   DevBuf d_f64_off, d_f64_ord;   // FP64 slab offsets and launched position -> candidate
+  DevBuf d_pair_pos, d_pair_fac, d_pair_item;   // FP64 factor-evaluation work list
   DevBuf d_dead;                 // per candidate: decided by the factor-0 pre-pass (OLS mode)
   DevBuf d_blk_cand, d_blk_chunk, d_chunk_base, d_work, d_work_flags;   // FP32 chunk map and partials
   HostBuf h_stage, h_out;
@@ -325,7 +326,8 @@ int rr_destroy(rr_ctx* ctx) {
   DevBuf* bufs[] = {&ctx->d_cand, &ctx->d_prog, &ctx->d_code, &ctx->d_K, &ctx->d_slab, &ctx->d_order32,
                     &ctx->d_coefs, &ctx->d_fit, &ctx->d_misc, &ctx->d_pred_e, &ctx->d_dstab, &ctx->d_cand_ds,
                     &ctx->d_blk_cand, &ctx->d_blk_chunk, &ctx->d_chunk_base, &ctx->d_work, &ctx->d_work_flags,
-                    &ctx->d_f64_off, &ctx->d_f64_ord, &ctx->d_dead};
+                    &ctx->d_f64_off, &ctx->d_f64_ord, &ctx->d_dead, &ctx->d_pair_pos, &ctx->d_pair_fac,
+                    &ctx->d_pair_item};
   for (DevBuf* b : bufs) b->release();
   ctx->h_stage.release();
   ctx->h_out.release();
@@ -536,7 +538,8 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
   const size_t s_off = align_up((size_t)nc * 8);
   const size_t s_tab = align_up(table.size() * sizeof(rr::DsDesc)), s_cds = align_up((size_t)nc * 4);
   const size_t s_chunks = 2 * align_up((size_t)total_chunks * 4) + align_up(chunk_base.size() * 4);
-  const size_t s_f64 = align_up((size_t)nc * 8) + align_up((size_t)nc * 4);   // FP64 slab offsets + order
+  // FP64 slab offsets, order and the three arrays of the evaluation work list
+  const size_t s_f64 = align_up((size_t)nc * 8) + align_up((size_t)nc * 4) + 3 * align_up((size_t)(np + 1) * 4);
   if (ctx->h_stage.ensure(s_cand + s_prog + s_code + s_K + s_off + s_tab + s_cds + s_chunks + 2 * s_f64) != RR_OK)
     return RR_E_NOMEM;
   size_t so = 0;
@@ -606,7 +609,9 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
   // FP64 OLS launches over a candidate list (null: every candidate), longest
   // first, chunked so that each launch's slab fits the budget.  plan64 does
   // the host work and the uploads, launch64 only launches.
-  struct Chunk64 { int c0, c1, maxk; };
+  // positions c0..c1-1 share one slab; their factor-evaluation pairs are
+  // p0..p0+npair-1 of the work list, with nitem work items
+  struct Chunk64 { int c0, c1, maxk, p0, npair, nitem; };
   auto plan64 = [&](const std::vector<int32_t>* list_in, int slot, std::vector<Chunk64>& chunks) -> int {
     chunks.clear();
     const int32_t m = list_in ? (int32_t)list_in->size() : nc;
@@ -640,14 +645,45 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
         maxk = std::max(maxk, k);
         ++q1;
       }
-      chunks.push_back({q, q1, maxk});
+      chunks.push_back({q, q1, maxk, 0, 0, 0});
       max_words = std::max(max_words, used);
       q = q1;
     }
     if (ctx->d_slab.ensure((size_t)max_words * 8) != RR_OK) return RR_E_NOMEM;
+    // factor-evaluation work list: per chunk, every (position, factor that is
+    // not a bare variable), each with ceil(n / EVAL_CHUNK_ROWS) work items
+    std::vector<int32_t> ppos, pfac, pitem;
+    for (Chunk64& ch : chunks) {
+      ch.p0 = (int)ppos.size();
+      int64_t items = 0;
+      for (int q = ch.c0; q < ch.c1; ++q) {
+        const int32_t c = lpt[q];
+        const int64_t nch = (ds_of(c).n + rr::EVAL_CHUNK_ROWS - 1) / rr::EVAL_CHUNK_ROWS;
+        for (int32_t p = b->cand_start[c]; p < b->cand_start[c + 1]; ++p) {
+          const int32_t len = b->prog_start[p + 1] - b->prog_start[p];
+          if (len == 1 && (b->code[b->prog_start[p]] & 0xffu) == RR_OP_LDV) continue;   // read from X
+          ppos.push_back(q);
+          pfac.push_back(p - b->cand_start[c]);
+          pitem.push_back((int32_t)items);
+          items += nch;
+        }
+      }
+      if (items > INT32_MAX) return fail(RR_E_LIMIT, "too many factor-evaluation work items in one launch");
+      ch.npair = (int)ppos.size() - ch.p0;
+      ch.nitem = (int)items;
+    }
     const size_t base = so64 + (size_t)slot * s_f64;
-    if ((rc = upload(ctx, ctx->d_f64_off, off.data(), (size_t)m * 8, base))) return rc;
-    if ((rc = upload(ctx, ctx->d_f64_ord, lpt.data(), (size_t)m * 4, base + align_up((size_t)nc * 8)))) return rc;
+    const size_t sp = align_up((size_t)(np + 1) * 4);
+    size_t o = base;
+    if ((rc = upload(ctx, ctx->d_f64_off, off.data(), (size_t)m * 8, o))) return rc;
+    o += align_up((size_t)nc * 8);
+    if ((rc = upload(ctx, ctx->d_f64_ord, lpt.data(), (size_t)m * 4, o))) return rc;
+    o += align_up((size_t)nc * 4);
+    if (!ppos.empty()) {
+      if ((rc = upload(ctx, ctx->d_pair_pos, ppos.data(), ppos.size() * 4, o))) return rc;
+      if ((rc = upload(ctx, ctx->d_pair_fac, pfac.data(), pfac.size() * 4, o + sp))) return rc;
+      if ((rc = upload(ctx, ctx->d_pair_item, pitem.data(), pitem.size() * 4, o + 2 * sp))) return rc;
+    }
     return RR_OK;
   };
   auto launch64 = [&](const std::vector<Chunk64>& chunks, hipStream_t stream) -> int {
@@ -657,6 +693,17 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
       RR_HIP(hipStreamWaitEvent(stream, ctx->fork64, 0));
     }
     for (const Chunk64& ch : chunks) {
+      if (ch.npair > 0) {
+        rr::EvalLaunch E;
+        E.cand_start = dc; E.prog_start = dp; E.code = dcode; E.K = dK; E.ds = dtab; E.cand_ds = dcds;
+        E.slab = (double*)ctx->d_slab.p; E.slab_off = (const int64_t*)ctx->d_f64_off.p;
+        E.order = (const int32_t*)ctx->d_f64_ord.p; E.dead = ddead;
+        E.pair_pos = (const int32_t*)ctx->d_pair_pos.p + ch.p0;
+        E.pair_fac = (const int32_t*)ctx->d_pair_fac.p + ch.p0;
+        E.pair_item = (const int32_t*)ctx->d_pair_item.p + ch.p0;
+        E.n_pairs = ch.npair; E.n_items = ch.nitem; E.chunk_rows = rr::EVAL_CHUNK_ROWS;
+        RR_HIP(rr::launch_eval_fp64(E, stream));
+      }
       rr::OlsLaunch L;
       L.cand_start = dc; L.prog_start = dp; L.code = dcode; L.K = dK; L.ds = dtab; L.cand_ds = dcds;
       L.slab = (double*)ctx->d_slab.p;

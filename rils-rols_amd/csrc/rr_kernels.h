@@ -70,7 +70,33 @@ struct Nan0Launch {
 };
 constexpr int NAN0_ROWS = 256;
 
+// Factor evaluation of the FP64 OLS form into the kept slab columns: one
+// workgroup per (launched position, non-bare factor, chunk of chunk_rows
+// rows); pair p = (pair_pos[p], pair_fac[p]) owns work items pair_item[p] ..
+// pair_item[p+1]-1, n_items in all.  Must run before the k_ols_fp64 launch
+// over the same positions.
+struct EvalLaunch {
+  const int32_t* cand_start;
+  const int32_t* prog_start;
+  const uint32_t* code;
+  const double* K;
+  const DsDesc* ds;
+  const int32_t* cand_ds;
+  double* slab;
+  const int64_t* slab_off;   // per launched position
+  const int32_t* order;      // launched position -> candidate
+  const int32_t* dead;
+  const int32_t* pair_pos;
+  const int32_t* pair_fac;
+  const int32_t* pair_item;
+  int n_pairs;
+  int n_items;
+  int64_t chunk_rows;        // a multiple of 512
+};
+constexpr int64_t EVAL_CHUNK_ROWS = 4096;
+
 hipError_t launch_ols_fp64(const OlsLaunch& L, hipStream_t s);
+hipError_t launch_eval_fp64(const EvalLaunch& L, hipStream_t s);
 hipError_t launch_nan0_fp64(const Nan0Launch& L, hipStream_t s);
 hipError_t launch_raw_fp64(const RawLaunch& L, hipStream_t s);
 hipError_t launch_predict_fp64(const uint32_t* code, int len, const double* K, const double* X, int64_t n,

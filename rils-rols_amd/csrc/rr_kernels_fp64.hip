@@ -9,11 +9,13 @@
 // ColPivHouseholderQR.h:482-607, Householder.h:67-135) operation for
 // operation, so the results are the same bits as the CPU oracle's.
 //
-// Per candidate the kernel
-//   1. evaluates each factor program (tune_constants, rils_rols_cpp.cpp:
-//      477-482) into a kept column of a work slab in HBM, accumulating the
-//      column norms on the way.  A factor that is a bare variable keeps no
-//      copy: its column is the dataset's;
+// Per candidate, k_eval_fp64 first evaluates each factor program
+// (tune_constants, rils_rols_cpp.cpp:477-482) into a kept column of a work
+// slab in HBM, spread over many small workgroups (row chunks), so that a
+// long program does not hold one compute unit for the whole column.  A
+// factor that is a bare variable keeps no copy: its column is the dataset's.
+// Then k_ols_fp64
+//   1. reads the factor columns once for their norms;
 //   2. runs the pivoted QR on the slab: per step one pass computing the
 //      Householder dot products and one pass applying the reflector, which
 //      also accumulates the tail norms the next step and the LAWN-176 norm
@@ -128,12 +130,12 @@ __global__ __launch_bounds__(CP) void k_ols_fp64(Fp64Args a) {
   if (tid == 0) abort_flag = 0;
   __syncthreads();
 
-  // ---- 1. factor columns, free term, copy of y; norms
+  // ---- 1. norms of the factor columns (written by k_eval_fp64)
   // Factor 0 NaN at a row >= 1 fixes the outcome: its norm is NaN, so the
   // rank threshold is NaN (no cut), factor 0 pivots first (nothing compares
   // greater than NaN), its tail norm and hence its reflector are NaN, and
   // with the threshold NaN no later column stops the factorisation: every
-  // coefficient is NaN, rank = min(n, k).  Evaluation stops at the first tile
+  // coefficient is NaN, rank = min(n, k).  The pass stops at the first tile
   // that shows it (any wave; the others stop at their next tile).
   const volatile int* abort_seen = &abort_flag;
   for (int j = 0; j < nf; ++j) {
@@ -141,24 +143,18 @@ __global__ __launch_bounds__(CP) void k_ols_fp64(Fp64Args a) {
     const int len = a.prog_start[pbeg + j + 1] - a.prog_start[pbeg + j];
     const int var = bare_variable(code, len);
     if (tid == 0) ovar[j] = var;
-    double* keep = S + (int64_t)(k + 1 + j) * ldx;
-    const double* xv = X + (int64_t)(var < 0 ? 0 : var) * ldx;
+    const double* src = var >= 0 ? X + (int64_t)var * ldx : S + (int64_t)(k + 1 + j) * ldx;
     double sums[2] = {0.0, 0.0};   // rows >= 0, rows >= 1
     for (int64_t base = 0; base < n; base += CP * R) {
       if (j == 0 && *abort_seen) break;
       const int64_t row0 = base + tid;
       double v[R];
-      if (var >= 0) {
 #pragma unroll
-        for (int r = 0; r < R; ++r) v[r] = xv[row0 + (int64_t)CP * r];
-      } else {
-        eval_f64<R>(code, len, a.K, X, ldx, row0, v);
-      }
+      for (int r = 0; r < R; ++r) v[r] = src[row0 + (int64_t)CP * r];   // rows >= n: not used
       bool nan_tail = false;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const int64_t i = row0 + (int64_t)CP * r;
-        if (var < 0) keep[i] = v[r];
         if (i < n) {
           sums[0] = sums[0] + v[r] * v[r];
           if (i >= 1) sums[1] = sums[1] + v[r] * v[r];
@@ -482,6 +478,58 @@ __global__ __launch_bounds__(CP) void k_ols_fp64(Fp64Args a) {
   if (tid == 0) write_fit(a.fit + cand, ssr[0], a.sst, n, r, steps);
 }
 
+// Factor evaluation for k_ols_fp64: one workgroup of EV_THREADS threads per
+// (candidate, non-bare factor, chunk of chunk_rows rows).  Work item b is
+// the b-th chunk in pair order; pair_item[p] is the first item of pair p.
+struct EvalArgs {
+  const int32_t* cand_start;
+  const int32_t* prog_start;
+  const uint32_t* code;
+  const double* K;
+  const DsDesc* ds;
+  const int32_t* cand_ds;
+  double* slab;
+  const int64_t* slab_off;   // per launched position
+  const int32_t* order;      // launched position -> candidate
+  const int32_t* dead;
+  const int32_t* pair_pos;   // per pair: launched position
+  const int32_t* pair_fac;   // per pair: factor index
+  const int32_t* pair_item;  // per pair: first work item
+  int n_pairs;
+  int64_t chunk_rows;        // a multiple of EV_THREADS * R
+};
+constexpr int EV_THREADS = 256;
+
+template <int R>
+__global__ __launch_bounds__(EV_THREADS) void k_eval_fp64(EvalArgs a) {
+  const int item = (int)blockIdx.x;
+  int lo = 0, hi = a.n_pairs - 1;   // last pair whose first item <= item (uniform)
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (a.pair_item[mid] <= item) lo = mid; else hi = mid - 1;
+  }
+  const int pos = a.pair_pos[lo], j = a.pair_fac[lo];
+  const int cand = a.order[pos];
+  if (a.dead && a.dead[cand]) return;
+  const DsDesc& D = a.ds[a.cand_ds ? a.cand_ds[cand] : 0];
+  const int pbeg = a.cand_start[cand];
+  const int k = a.cand_start[cand + 1] - pbeg + 1;
+  const uint32_t* code = a.code + a.prog_start[pbeg + j];
+  const int len = a.prog_start[pbeg + j + 1] - a.prog_start[pbeg + j];
+  double* const keep = a.slab + a.slab_off[pos] + (int64_t)(k + 1 + j) * D.ldx;
+  const int64_t r0 = (int64_t)(item - a.pair_item[lo]) * a.chunk_rows;
+  const int64_t r1 = (r0 + a.chunk_rows < D.n) ? r0 + a.chunk_rows : D.n;
+  // base is a multiple of EV_THREADS * R (which divides 4096) and below n,
+  // so every row touched is below ldx
+  for (int64_t base = r0; base < r1; base += (int64_t)EV_THREADS * R) {
+    const int64_t row0 = base + (int64_t)threadIdx.x;
+    double v[R];
+    eval_f64<R, EV_THREADS>(code, len, a.K, D.X, D.ldx, row0, v);
+#pragma unroll
+    for (int r = 0; r < R; ++r) keep[row0 + (int64_t)EV_THREADS * r] = v[r];
+  }
+}
+
 // The factor-0 pre-pass: one workgroup of NAN0_ROWS threads per candidate,
 // one row each, so it costs one evaluation of latency.
 struct Nan0Args {
@@ -616,6 +664,14 @@ hipError_t launch_ols_fp64(const OlsLaunch& L, hipStream_t s) {
   } else {
     hipLaunchKernelGGL((k_ols_fp64<RR_MAX_COLS, 2>), grid, block, 0, s, a);
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_eval_fp64(const EvalLaunch& L, hipStream_t s) {
+  if (L.n_items <= 0) return hipSuccess;
+  EvalArgs a{L.cand_start, L.prog_start, L.code, L.K, L.ds, L.cand_ds, L.slab, L.slab_off, L.order, L.dead,
+             L.pair_pos, L.pair_fac, L.pair_item, L.n_pairs, L.chunk_rows};
+  hipLaunchKernelGGL(k_eval_fp64<2>, dim3((unsigned)L.n_items), dim3(EV_THREADS), 0, s, a);
   return hipGetLastError();
 }
 
