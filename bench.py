@@ -658,6 +658,7 @@ def main():
         },
         "forms": out,
         "schedule": {"autotuned": tuned, "fp64_rows": runner.get_option("fp64_rows"),
+                     "fp64_eval_rows": runner.get_option("fp64_eval_rows"),
                      "fp32_rows": runner.get_option("fp32_rows"),
                      "fp32_chunk_rows": runner.get_option("fp32_chunk_rows"),
                      "fp32_streams": runner.get_option("fp32_streams")},

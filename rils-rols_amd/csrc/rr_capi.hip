@@ -101,6 +101,7 @@ struct rr_ctx {
   HostBuf h_stage, h_out;
   int64_t slab_budget_mb = 32768;
   int64_t fp64_rows = 4;
+  int64_t fp64_eval_rows = 2;
   int64_t fp32_rows = 2;
   int64_t fp32_chunk_rows = 4096;
   int64_t fp32_streams = 0;   // 1: bucket kernels on concurrent side streams
@@ -611,7 +612,7 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
   // the host work and the uploads, launch64 only launches.
   // positions c0..c1-1 share one slab; their factor-evaluation pairs are
   // p0..p0+npair-1 of the work list, with nitem work items
-  struct Chunk64 { int c0, c1, maxk, p0, npair, nitem; };
+  struct Chunk64 { int c0, c1, maxk, p0, npair, nitem, per; };
   auto plan64 = [&](const std::vector<int32_t>* list_in, int slot, std::vector<Chunk64>& chunks) -> int {
     chunks.clear();
     const int32_t m = list_in ? (int32_t)list_in->size() : nc;
@@ -645,7 +646,7 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
         maxk = std::max(maxk, k);
         ++q1;
       }
-      chunks.push_back({q, q1, maxk, 0, 0, 0});
+      chunks.push_back({q, q1, maxk, 0, 0, 0, 0});
       max_words = std::max(max_words, used);
       q = q1;
     }
@@ -655,7 +656,7 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
     std::vector<int32_t> ppos, pfac, pitem;
     for (Chunk64& ch : chunks) {
       ch.p0 = (int)ppos.size();
-      int64_t items = 0;
+      int64_t items = 0, per = -1;   // per: the common item count of the pairs, or 0
       for (int q = ch.c0; q < ch.c1; ++q) {
         const int32_t c = lpt[q];
         const int64_t nch = (ds_of(c).n + rr::EVAL_CHUNK_ROWS - 1) / rr::EVAL_CHUNK_ROWS;
@@ -666,11 +667,13 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
           pfac.push_back(p - b->cand_start[c]);
           pitem.push_back((int32_t)items);
           items += nch;
+          per = (per < 0 || per == nch) ? nch : 0;
         }
       }
       if (items > INT32_MAX) return fail(RR_E_LIMIT, "too many factor-evaluation work items in one launch");
       ch.npair = (int)ppos.size() - ch.p0;
       ch.nitem = (int)items;
+      ch.per = per > 0 ? (int)per : 0;
     }
     const size_t base = so64 + (size_t)slot * s_f64;
     const size_t sp = align_up((size_t)(np + 1) * 4);
@@ -701,7 +704,8 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
         E.pair_pos = (const int32_t*)ctx->d_pair_pos.p + ch.p0;
         E.pair_fac = (const int32_t*)ctx->d_pair_fac.p + ch.p0;
         E.pair_item = (const int32_t*)ctx->d_pair_item.p + ch.p0;
-        E.n_pairs = ch.npair; E.n_items = ch.nitem; E.chunk_rows = rr::EVAL_CHUNK_ROWS;
+        E.n_pairs = ch.npair; E.n_items = ch.nitem; E.items_per_pair = ch.per; E.chunk_rows = rr::EVAL_CHUNK_ROWS;
+        E.rows = (int)ctx->fp64_eval_rows;
         RR_HIP(rr::launch_eval_fp64(E, stream));
       }
       rr::OlsLaunch L;
@@ -874,16 +878,17 @@ int rr_autotune(rr_ctx* ctx, int32_t handle, const rr_batch* b, int32_t mode, in
   struct Cfg { int64_t a, b2, c; };
   std::vector<Cfg> cfgs;
   if (precision == RR_FP64) {
-    for (int64_t r : {1, 2, 4}) cfgs.push_back({r, 0, 0});
+    for (int64_t r : {1, 2, 4})
+      for (int64_t e : {1, 2, 4}) cfgs.push_back({r, e, 0});
   } else {
     for (int64_t r : {1, 2})
       for (int64_t ch : {2048, 4096, 8192, 16384})
         for (int64_t st : {0, 1}) cfgs.push_back({r, ch, st});
   }
-  const Cfg keep = precision == RR_FP64 ? Cfg{ctx->fp64_rows, 0, 0}
+  const Cfg keep = precision == RR_FP64 ? Cfg{ctx->fp64_rows, ctx->fp64_eval_rows, 0}
                                          : Cfg{ctx->fp32_rows, ctx->fp32_chunk_rows, ctx->fp32_streams};
   auto apply = [&](const Cfg& c) {
-    if (precision == RR_FP64) ctx->fp64_rows = c.a;
+    if (precision == RR_FP64) { ctx->fp64_rows = c.a; ctx->fp64_eval_rows = c.b2; }
     else { ctx->fp32_rows = c.a; ctx->fp32_chunk_rows = c.b2; ctx->fp32_streams = c.c; }
   };
   double best = 1e300;
@@ -911,6 +916,9 @@ int rr_set_option(rr_ctx* ctx, const char* name, int64_t value) {
   else if (s == "fp64_rows") {
     if (value != 1 && value != 2 && value != 4) return fail(RR_E_ARG, "fp64_rows must be 1, 2 or 4");
     ctx->fp64_rows = value;
+  } else if (s == "fp64_eval_rows") {
+    if (value != 1 && value != 2 && value != 4) return fail(RR_E_ARG, "fp64_eval_rows must be 1, 2 or 4");
+    ctx->fp64_eval_rows = value;
   } else if (s == "fp32_rows") {
     if (value != 1 && value != 2) return fail(RR_E_ARG, "fp32_rows must be 1 or 2");
     ctx->fp32_rows = value;
@@ -929,6 +937,7 @@ int rr_get_option(rr_ctx* ctx, const char* name, int64_t* value) {
   const std::string s(name);
   if (s == "slab_budget_mb") *value = ctx->slab_budget_mb;
   else if (s == "fp64_rows") *value = ctx->fp64_rows;
+  else if (s == "fp64_eval_rows") *value = ctx->fp64_eval_rows;
   else if (s == "fp32_rows") *value = ctx->fp32_rows;
   else if (s == "fp32_chunk_rows") *value = ctx->fp32_chunk_rows;
   else if (s == "fp32_streams") *value = ctx->fp32_streams;

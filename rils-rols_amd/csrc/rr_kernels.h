@@ -91,7 +91,9 @@ struct EvalLaunch {
   const int32_t* pair_item;
   int n_pairs;
   int n_items;
-  int64_t chunk_rows;        // a multiple of 512
+  int items_per_pair;        // > 0 when every pair has that many items (pair = item / items_per_pair)
+  int64_t chunk_rows;        // a multiple of 1024
+  int rows;                  // rows per thread per pass: 1, 2 or 4
 };
 constexpr int64_t EVAL_CHUNK_ROWS = 4096;
 

@@ -179,7 +179,7 @@ __global__ __launch_bounds__(CP) void k_ols_fp64(Fp64Args a) {
   // where the values of physical column c are before the first QR step
   auto origin = [&](int c) -> const double* {
     if (c == nf) return a.one;
-    const int v = ovar[c];
+    const int v = __builtin_amdgcn_readfirstlane(ovar[c]);   // uniform: keep it scalar
     return v >= 0 ? X + (int64_t)v * ldx : S + (int64_t)(k + 1 + c) * ldx;
   };
   // row 0 of every working column; rows >= 1 are written by the first
@@ -220,8 +220,9 @@ __global__ __launch_bounds__(CP) void k_ols_fp64(Fp64Args a) {
     }
     __syncthreads();
 
-    double* const v = S + (int64_t)phys[p] * ldx;
-    const double* const vr = (p == 0) ? origin(phys[p]) : v;   // rows > p of the pivot column
+    const int pc = __builtin_amdgcn_readfirstlane(phys[p]);   // uniform: keep it scalar
+    double* const v = S + (int64_t)pc * ldx;
+    const double* const vr = (p == 0) ? origin(pc) : v;   // rows > p of the pivot column
     const double c0 = v[p];
     const double tailSq = (n - p == 1) ? 0.0 : tsq[p];
     double tau, beta;
@@ -292,7 +293,7 @@ __global__ __launch_bounds__(CP) void k_ols_fp64(Fp64Args a) {
 #pragma unroll
       for (int t = 0; t < G; ++t) {
         const int tg = g0 + t;
-        const int c = (t < gn && tg < nrem) ? phys[p + 1 + tg] : -1;
+        const int c = (t < gn && tg < nrem) ? __builtin_amdgcn_readfirstlane(phys[p + 1 + tg]) : -1;
         cols[t] = c >= 0 ? S + (int64_t)c * ldx : cy;
         rcols[t] = p > 0 ? cols[t] : (c >= 0 ? origin(c) : a.y);
       }
@@ -496,6 +497,7 @@ struct EvalArgs {
   const int32_t* pair_fac;   // per pair: factor index
   const int32_t* pair_item;  // per pair: first work item
   int n_pairs;
+  int items_per_pair;        // > 0: every pair has this many work items (one dataset)
   int64_t chunk_rows;        // a multiple of EV_THREADS * R
 };
 constexpr int EV_THREADS = 256;
@@ -503,10 +505,16 @@ constexpr int EV_THREADS = 256;
 template <int R>
 __global__ __launch_bounds__(EV_THREADS) void k_eval_fp64(EvalArgs a) {
   const int item = (int)blockIdx.x;
-  int lo = 0, hi = a.n_pairs - 1;   // last pair whose first item <= item (uniform)
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (a.pair_item[mid] <= item) lo = mid; else hi = mid - 1;
+  int lo;
+  if (a.items_per_pair > 0) {
+    lo = item / a.items_per_pair;
+  } else {   // last pair whose first item <= item (uniform)
+    lo = 0;
+    int hi = a.n_pairs - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (a.pair_item[mid] <= item) lo = mid; else hi = mid - 1;
+    }
   }
   const int pos = a.pair_pos[lo], j = a.pair_fac[lo];
   const int cand = a.order[pos];
@@ -670,8 +678,11 @@ hipError_t launch_ols_fp64(const OlsLaunch& L, hipStream_t s) {
 hipError_t launch_eval_fp64(const EvalLaunch& L, hipStream_t s) {
   if (L.n_items <= 0) return hipSuccess;
   EvalArgs a{L.cand_start, L.prog_start, L.code, L.K, L.ds, L.cand_ds, L.slab, L.slab_off, L.order, L.dead,
-             L.pair_pos, L.pair_fac, L.pair_item, L.n_pairs, L.chunk_rows};
-  hipLaunchKernelGGL(k_eval_fp64<2>, dim3((unsigned)L.n_items), dim3(EV_THREADS), 0, s, a);
+             L.pair_pos, L.pair_fac, L.pair_item, L.n_pairs, L.items_per_pair, L.chunk_rows};
+  const dim3 grid((unsigned)L.n_items), block(EV_THREADS);
+  if (L.rows == 1) hipLaunchKernelGGL(k_eval_fp64<1>, grid, block, 0, s, a);
+  else if (L.rows == 4) hipLaunchKernelGGL(k_eval_fp64<4>, grid, block, 0, s, a);
+  else hipLaunchKernelGGL(k_eval_fp64<2>, grid, block, 0, s, a);
   return hipGetLastError();
 }
 
