@@ -32,7 +32,10 @@
 //   * every sum over samples uses one canonical order: 1024 partial sums,
 //     row i added to partial (i mod 1024) in increasing i, each group of 64
 //     partials combined by an xor-butterfly (pairs at distance 32, 16, 8, 4,
-//     2, 1), the 16 group sums added in order; rrm::canon_sum (rr_math.h);
+//     2, 1), the 16 group sums added in order; rrm::canon_sum (rr_math.h).
+//     The QR's initial column norms and its first step's tail sum use the
+//     chunked order of rrm::chunk_sum instead (chunks of 4096 rows, 256
+//     partials each, the chunk sums added in order);
 //   * maxCoeff() is a left-to-right scan with strict '>' (first index wins);
 //   * sin/cos/exp/log come from rr_math.h; + - * / sqrt are IEEE;
 //   * no floating-point contraction (built with -ffp-contract=off);
@@ -493,10 +496,14 @@ static std::vector<double> colpiv_qr_solve(std::vector<std::vector<double>> A, c
   const double eps = 2.220446049250313e-16;    // NumTraits<double>::epsilon()
   std::vector<double> hc(size), normU(cols), normD(cols);
   std::vector<int> transp(size);
+  // initial norms and the first step's tail sums in the chunked order
+  // (rrm::chunk_sum), every later sum in the canonical one
+  std::vector<double> tail1(cols);
   for (int j = 0; j < cols; ++j) {
     const std::vector<double>& a = A[j];
-    normD[j] = std::sqrt(canon_sum(0, rows, [&](int64_t i) { return a[i] * a[i]; }));
+    normD[j] = std::sqrt(rrm::chunk_sum(0, rows, [&](int64_t i) { return a[i] * a[i]; }));
     normU[j] = normD[j];
+    tail1[j] = rrm::chunk_sum(1, rows, [&](int64_t i) { return a[i] * a[i]; });
   }
   double mx = normU[0];
   for (int j = 1; j < cols; ++j) mx = (mx < normU[j]) ? normU[j] : mx;
@@ -515,11 +522,14 @@ static std::vector<double> colpiv_qr_solve(std::vector<std::vector<double>> A, c
       std::swap(A[k], A[bi]);
       std::swap(normU[k], normU[bi]);
       std::swap(normD[k], normD[bi]);
+      std::swap(tail1[k], tail1[bi]);
     }
     // makeHouseholderInPlace on A[k][k..rows)
     std::vector<double>& v = A[k];
     const double c0 = v[k];
-    const double tailSq = (rows - k == 1) ? 0.0 : canon_sum(k + 1, rows, [&](int64_t i) { return v[i] * v[i]; });
+    const double tailSq = (rows - k == 1) ? 0.0
+                          : (k == 0) ? tail1[0]
+                                     : canon_sum(k + 1, rows, [&](int64_t i) { return v[i] * v[i]; });
     double tau, beta;
     if (tailSq <= 2.2250738585072014e-308) {
       tau = 0.0; beta = c0;
@@ -1020,6 +1030,10 @@ int orc_math(int op, const double* in, double* out, int64_t n) {
     }
   }
   return 0;
+}
+
+double orc_chunk_sum(const double* v, int64_t lo, int64_t hi) {
+  return rrm::chunk_sum(lo, hi, [&](int64_t i) { return v[i]; });
 }
 
 double orc_canon_sum(const double* v, int64_t lo, int64_t hi) {

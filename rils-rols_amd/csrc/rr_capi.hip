@@ -75,6 +75,7 @@ struct Dataset {
   double* y = nullptr;
   float* X32 = nullptr;   // FP32 copy for the precision form
   double* one = nullptr;  // ldx ones (the free term's column of the FP64 form)
+  double* xnorm = nullptr;   // per feature: sums of squares over rows >= 0, >= 1 (chunked order)
   double ybar = 0, sst = 0, ysum = 0, yy = 0;
 };
 
@@ -96,6 +97,7 @@ struct rr_ctx {
   DevBuf d_cand, d_prog, d_code, d_K, d_slab, d_order32, d_coefs, d_fit, d_misc, d_pred_e, d_dstab, d_cand_ds;
   DevBuf d_f64_off, d_f64_ord;   // FP64 slab offsets and launched position -> candidate
   DevBuf d_pair_pos, d_pair_fac, d_pair_item;   // FP64 factor-evaluation work list
+  DevBuf d_prog_item, d_norm_part;              // per program: first work item; per work item: chunk sums
   DevBuf d_dead;                 // per candidate: decided by the factor-0 pre-pass (OLS mode)
   DevBuf d_blk_cand, d_blk_chunk, d_chunk_base, d_work, d_work_flags;   // FP32 chunk map and partials
   HostBuf h_stage, h_out;
@@ -322,13 +324,13 @@ int rr_destroy(rr_ctx* ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   for (Dataset& d : ctx->ds) {
     if (!d.live) continue;
-    (void)hipFree(d.X); (void)hipFree(d.y); (void)hipFree(d.X32); (void)hipFree(d.one);
+    (void)hipFree(d.X); (void)hipFree(d.y); (void)hipFree(d.X32); (void)hipFree(d.one); (void)hipFree(d.xnorm);
   }
   DevBuf* bufs[] = {&ctx->d_cand, &ctx->d_prog, &ctx->d_code, &ctx->d_K, &ctx->d_slab, &ctx->d_order32,
                     &ctx->d_coefs, &ctx->d_fit, &ctx->d_misc, &ctx->d_pred_e, &ctx->d_dstab, &ctx->d_cand_ds,
                     &ctx->d_blk_cand, &ctx->d_blk_chunk, &ctx->d_chunk_base, &ctx->d_work, &ctx->d_work_flags,
                     &ctx->d_f64_off, &ctx->d_f64_ord, &ctx->d_dead, &ctx->d_pair_pos, &ctx->d_pair_fac,
-                    &ctx->d_pair_item};
+                    &ctx->d_pair_item, &ctx->d_prog_item, &ctx->d_norm_part};
   for (DevBuf* b : bufs) b->release();
   ctx->h_stage.release();
   ctx->h_out.release();
@@ -369,6 +371,10 @@ int rr_dataset_upload(rr_ctx* ctx, const double* X_fm, const double* y, int64_t 
     return fail(RR_E_NOMEM, "hipMalloc for the column of ones failed");
   }
   RR_HIP(rr::launch_fill_f64(D.one, D.ldx, 1.0, ctx->stream));
+  if (hipMalloc(&D.xnorm, (size_t)std::max(d, 1) * 2 * sizeof(double)) != hipSuccess) {
+    (void)hipFree(D.X); (void)hipFree(D.y); (void)hipFree(D.X32); (void)hipFree(D.one);
+    return fail(RR_E_NOMEM, "hipMalloc for the feature norms failed");
+  }
   RR_HIP(hipMemsetAsync(D.X, 0, xb, ctx->stream));
   RR_HIP(hipMemsetAsync(D.y, 0, (size_t)D.ldx * sizeof(double), ctx->stream));
   if (d > 0)
@@ -376,6 +382,7 @@ int rr_dataset_upload(rr_ctx* ctx, const double* X_fm, const double* y, int64_t 
                             (size_t)n * sizeof(double), (size_t)d, hipMemcpyHostToDevice, ctx->stream));
   RR_HIP(hipMemcpyAsync(D.y, y, (size_t)n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
   RR_HIP(rr::launch_to_fp32(D.X, D.X32, (int64_t)D.ldx * std::max(d, 1), ctx->stream));
+  RR_HIP(rr::launch_xnorm_fp64(D.X, n, D.ldx, d, D.xnorm, ctx->stream));
   if (ctx->d_misc.ensure(4 * sizeof(double)) != RR_OK) return RR_E_NOMEM;
   RR_HIP(rr::launch_stats_fp64(D.y, n, (double*)ctx->d_misc.p, ctx->stream));
   double st[4];
@@ -397,7 +404,7 @@ int rr_dataset_free(rr_ctx* ctx, int32_t handle) {
   Dataset* D = get_ds(ctx, handle);
   if (!D) return fail(RR_E_ARG, "bad dataset handle");
   RR_HIP(hipStreamSynchronize(ctx->stream));
-  (void)hipFree(D->X); (void)hipFree(D->y); (void)hipFree(D->X32); (void)hipFree(D->one);
+  (void)hipFree(D->X); (void)hipFree(D->y); (void)hipFree(D->X32); (void)hipFree(D->one); (void)hipFree(D->xnorm);
   *D = Dataset();
   return RR_OK;
 }
@@ -469,7 +476,7 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
   std::vector<rr::DsDesc> table(uniq.size());
   for (size_t u = 0; u < uniq.size(); ++u) {
     const Dataset& D = ctx->ds[uniq[u]];
-    table[u] = rr::DsDesc{D.X, D.X32, D.y, D.n, D.ldx, D.sst, D.ysum, D.yy, D.one};
+    table[u] = rr::DsDesc{D.X, D.X32, D.y, D.n, D.ldx, D.sst, D.ysum, D.yy, D.one, D.xnorm};
   }
   auto ds_of = [&](int32_t c) -> const Dataset& { return ctx->ds[uniq[handles ? cand_ds[c] : 0]]; };
 
@@ -540,7 +547,7 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
   const size_t s_tab = align_up(table.size() * sizeof(rr::DsDesc)), s_cds = align_up((size_t)nc * 4);
   const size_t s_chunks = 2 * align_up((size_t)total_chunks * 4) + align_up(chunk_base.size() * 4);
   // FP64 slab offsets, order and the three arrays of the evaluation work list
-  const size_t s_f64 = align_up((size_t)nc * 8) + align_up((size_t)nc * 4) + 3 * align_up((size_t)(np + 1) * 4);
+  const size_t s_f64 = align_up((size_t)nc * 8) + align_up((size_t)nc * 4) + 4 * align_up((size_t)(np + 1) * 4);
   if (ctx->h_stage.ensure(s_cand + s_prog + s_code + s_K + s_off + s_tab + s_cds + s_chunks + 2 * s_f64) != RR_OK)
     return RR_E_NOMEM;
   size_t so = 0;
@@ -653,7 +660,8 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
     if (ctx->d_slab.ensure((size_t)max_words * 8) != RR_OK) return RR_E_NOMEM;
     // factor-evaluation work list: per chunk, every (position, factor that is
     // not a bare variable), each with ceil(n / EVAL_CHUNK_ROWS) work items
-    std::vector<int32_t> ppos, pfac, pitem;
+    std::vector<int32_t> ppos, pfac, pitem, prog_item(std::max(np, 1), 0);
+    int64_t max_items = 0;
     for (Chunk64& ch : chunks) {
       ch.p0 = (int)ppos.size();
       int64_t items = 0, per = -1;   // per: the common item count of the pairs, or 0
@@ -666,6 +674,7 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
           ppos.push_back(q);
           pfac.push_back(p - b->cand_start[c]);
           pitem.push_back((int32_t)items);
+          prog_item[p] = (int32_t)items;
           items += nch;
           per = (per < 0 || per == nch) ? nch : 0;
         }
@@ -673,6 +682,7 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
       if (items > INT32_MAX) return fail(RR_E_LIMIT, "too many factor-evaluation work items in one launch");
       ch.npair = (int)ppos.size() - ch.p0;
       ch.nitem = (int)items;
+      max_items = std::max(max_items, items);
       ch.per = per > 0 ? (int)per : 0;
     }
     const size_t base = so64 + (size_t)slot * s_f64;
@@ -687,6 +697,8 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
       if ((rc = upload(ctx, ctx->d_pair_fac, pfac.data(), pfac.size() * 4, o + sp))) return rc;
       if ((rc = upload(ctx, ctx->d_pair_item, pitem.data(), pitem.size() * 4, o + 2 * sp))) return rc;
     }
+    if ((rc = upload(ctx, ctx->d_prog_item, prog_item.data(), prog_item.size() * 4, o + 3 * sp))) return rc;
+    if (ctx->d_norm_part.ensure((size_t)std::max<int64_t>(max_items, 1) * 16) != RR_OK) return RR_E_NOMEM;
     return RR_OK;
   };
   auto launch64 = [&](const std::vector<Chunk64>& chunks, hipStream_t stream) -> int {
@@ -704,8 +716,8 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
         E.pair_pos = (const int32_t*)ctx->d_pair_pos.p + ch.p0;
         E.pair_fac = (const int32_t*)ctx->d_pair_fac.p + ch.p0;
         E.pair_item = (const int32_t*)ctx->d_pair_item.p + ch.p0;
-        E.n_pairs = ch.npair; E.n_items = ch.nitem; E.items_per_pair = ch.per; E.chunk_rows = rr::EVAL_CHUNK_ROWS;
-        E.rows = (int)ctx->fp64_eval_rows;
+        E.n_pairs = ch.npair; E.n_items = ch.nitem; E.items_per_pair = ch.per;
+        E.norm_part = (double*)ctx->d_norm_part.p; E.rows = (int)ctx->fp64_eval_rows;
         RR_HIP(rr::launch_eval_fp64(E, stream));
       }
       rr::OlsLaunch L;
@@ -713,6 +725,7 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
       L.slab = (double*)ctx->d_slab.p;
       L.slab_off = (const int64_t*)ctx->d_f64_off.p + ch.c0; L.coefs = dcoef; L.fit = dfit;
       L.cand_base = ch.c0; L.order = (const int32_t*)ctx->d_f64_ord.p; L.dead = ddead;
+      L.norm_part = (const double*)ctx->d_norm_part.p; L.prog_item = (const int32_t*)ctx->d_prog_item.p;
       L.n_cands = ch.c1 - ch.c0; L.max_cols = ch.maxk; L.rows = (int)ctx->fp64_rows;
       RR_HIP(rr::launch_ols_fp64(L, stream));
     }

@@ -19,6 +19,7 @@ struct DsDesc {
   int64_t n, ldx;
   double sst, ysum, yy;   // sum (y - mean)^2 (canonical order), sum y, sum y^2
   const double* one;      // ldx ones: the column of the OLS free term
+  const double* xnorm;    // per feature: sums of squares over rows >= 0 and >= 1 (chunked order)
 };
 
 struct OlsLaunch {
@@ -35,6 +36,8 @@ struct OlsLaunch {
   int cand_base;              // first launched position of this launch
   const int32_t* order;       // launched position -> candidate (null: identity)
   const int32_t* dead;        // per candidate: outcome already written by the factor-0 pre-pass (or null)
+  const double* norm_part;    // the evaluation's chunk sums (EvalLaunch::norm_part)
+  const int32_t* prog_item;   // per program: its first evaluation work item in this launch
   int n_cands;                // candidates in this launch
   int max_cols;               // largest k among them
   int rows;                   // rows per lane per tile (1, 2 or 4)
@@ -71,10 +74,11 @@ struct Nan0Launch {
 constexpr int NAN0_ROWS = 256;
 
 // Factor evaluation of the FP64 OLS form into the kept slab columns: one
-// workgroup per (launched position, non-bare factor, chunk of chunk_rows
-// rows); pair p = (pair_pos[p], pair_fac[p]) owns work items pair_item[p] ..
-// pair_item[p+1]-1, n_items in all.  Must run before the k_ols_fp64 launch
-// over the same positions.
+// workgroup per (launched position, non-bare factor, chunk of
+// EVAL_CHUNK_ROWS rows); pair p = (pair_pos[p], pair_fac[p]) owns work items
+// pair_item[p] .. pair_item[p+1]-1, n_items in all.  Work item b also leaves
+// the sums of squares of its chunk in norm_part[2b], norm_part[2b+1].  Must
+// run before the k_ols_fp64 launch over the same positions.
 struct EvalLaunch {
   const int32_t* cand_start;
   const int32_t* prog_start;
@@ -92,13 +96,16 @@ struct EvalLaunch {
   int n_pairs;
   int n_items;
   int items_per_pair;        // > 0 when every pair has that many items (pair = item / items_per_pair)
-  int64_t chunk_rows;        // a multiple of 1024
+  double* norm_part;         // 2 doubles per work item
   int rows;                  // rows per thread per pass: 1, 2 or 4
 };
-constexpr int64_t EVAL_CHUNK_ROWS = 4096;
+constexpr int64_t EVAL_CHUNK_ROWS = 4096;   // the chunk of the norms' order (rrm::NORM_CHUNK)
 
 hipError_t launch_ols_fp64(const OlsLaunch& L, hipStream_t s);
 hipError_t launch_eval_fp64(const EvalLaunch& L, hipStream_t s);
+// out[2f], out[2f+1]: sums of squares of feature f's column over rows >= 0
+// and rows >= 1 in the chunked order (rrm::chunk_sum)
+hipError_t launch_xnorm_fp64(const double* X, int64_t n, int64_t ldx, int32_t d, double* out, hipStream_t s);
 hipError_t launch_nan0_fp64(const Nan0Launch& L, hipStream_t s);
 hipError_t launch_raw_fp64(const RawLaunch& L, hipStream_t s);
 hipError_t launch_predict_fp64(const uint32_t* code, int len, const double* K, const double* X, int64_t n,

@@ -70,6 +70,9 @@ struct Fp64Args {
   int cand_base;
   const int32_t* order;   // launched position -> candidate, or null (identity)
   const int32_t* dead;    // per candidate: outcome written by the factor-0 pre-pass, or null
+  const double* norm_part;   // per evaluation work item: its chunk's sums of squares (rows >= 0, >= 1)
+  const int32_t* prog_item;  // per program: its first evaluation work item (bare variables: unused)
+  const double* xnorm;       // the candidate's dataset: per feature, the same two sums over its column
 };
 
 __device__ inline void write_fit(rr_fit* f, double ssr, double sst, int64_t n, int rank, int steps = 0) {
@@ -93,6 +96,7 @@ __device__ inline void bind_dataset(A& a, int cand) {
   const DsDesc& D = a.ds[a.cand_ds ? a.cand_ds[cand] : 0];
   a.X = D.X; a.y = D.y; a.one = D.one; a.n = D.n; a.ldx = D.ldx; a.sst = D.sst;
 }
+__device__ inline void bind_xnorm(Fp64Args& a, int cand) { a.xnorm = a.ds[a.cand_ds ? a.cand_ds[cand] : 0].xnorm; }
 
 // Variable index of a program that is one LDV, else -1.
 __device__ inline int bare_variable(const uint32_t* code, int len) {
@@ -110,12 +114,13 @@ __global__ __launch_bounds__(CP) void k_ols_fp64(Fp64Args a) {
   __shared__ double normU[MAXK], normD[MAXK], tsq[MAXK], cc[MAXK];
   __shared__ int phys[MAXK];
   __shared__ int ovar[MAXK];   // per factor: its bare variable, or -1 (kept column)
-  __shared__ double red1[CW][1], red2[CW][2], redG[CW][G], redS[CW][2 * G];
+  __shared__ double red1[CW][1], redG[CW][G], redS[CW][2 * G];
   __shared__ int abort_flag;
 
   const int cand = a.order ? a.order[a.cand_base + (int)blockIdx.x] : a.cand_base + (int)blockIdx.x;
   if (a.dead && a.dead[cand]) return;   // uniform: the whole workgroup leaves
   bind_dataset(a, cand);
+  bind_xnorm(a, cand);
   const int tid = (int)threadIdx.x;
   const int pbeg = a.cand_start[cand];
   const int nf = a.cand_start[cand + 1] - pbeg;
@@ -130,50 +135,43 @@ __global__ __launch_bounds__(CP) void k_ols_fp64(Fp64Args a) {
   if (tid == 0) abort_flag = 0;
   __syncthreads();
 
-  // ---- 1. norms of the factor columns (written by k_eval_fp64)
-  // Factor 0 NaN at a row >= 1 fixes the outcome: its norm is NaN, so the
-  // rank threshold is NaN (no cut), factor 0 pivots first (nothing compares
-  // greater than NaN), its tail norm and hence its reflector are NaN, and
-  // with the threshold NaN no later column stops the factorisation: every
-  // coefficient is NaN, rank = min(n, k).  The pass stops at the first tile
-  // that shows it (any wave; the others stop at their next tile).
-  const volatile int* abort_seen = &abort_flag;
-  for (int j = 0; j < nf; ++j) {
-    const uint32_t* code = a.code + a.prog_start[pbeg + j];
-    const int len = a.prog_start[pbeg + j + 1] - a.prog_start[pbeg + j];
-    const int var = bare_variable(code, len);
-    if (tid == 0) ovar[j] = var;
-    const double* src = var >= 0 ? X + (int64_t)var * ldx : S + (int64_t)(k + 1 + j) * ldx;
-    double sums[2] = {0.0, 0.0};   // rows >= 0, rows >= 1
-    for (int64_t base = 0; base < n; base += CP * R) {
-      if (j == 0 && *abort_seen) break;
-      const int64_t row0 = base + tid;
-      double v[R];
-#pragma unroll
-      for (int r = 0; r < R; ++r) v[r] = src[row0 + (int64_t)CP * r];   // rows >= n: not used
-      bool nan_tail = false;
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const int64_t i = row0 + (int64_t)CP * r;
-        if (i < n) {
-          sums[0] = sums[0] + v[r] * v[r];
-          if (i >= 1) sums[1] = sums[1] + v[r] * v[r];
-          if (i >= 1 && v[r] != v[r]) nan_tail = true;
-        }
+  // ---- 1. norms of the factor columns and their sums over rows >= 1, in the
+  // chunked order (rrm::chunk_sum): k_eval_fp64 left one pair of sums per
+  // 4096-row chunk, added here in chunk order; a bare variable's come with
+  // the dataset.
+  // Factor 0 NaN at a row >= 1 (its rows >= 1 sum is NaN exactly then) fixes
+  // the outcome: its norm is NaN, so the rank threshold is NaN (no cut),
+  // factor 0 pivots first (nothing compares greater than NaN), its tail norm
+  // and hence its reflector are NaN, and with the threshold NaN no later
+  // column stops the factorisation: every coefficient is NaN, rank = min(n, k).
+  if (tid == 0) {
+    const int64_t nch = (n + rrm::NORM_CHUNK - 1) / rrm::NORM_CHUNK;
+    for (int j = 0; j < nf; ++j) {
+      const uint32_t* code = a.code + a.prog_start[pbeg + j];
+      const int len = a.prog_start[pbeg + j + 1] - a.prog_start[pbeg + j];
+      const int var = bare_variable(code, len);
+      ovar[j] = var;
+      double s0, s1;
+      if (var >= 0) {
+        s0 = a.xnorm[2 * var];
+        s1 = a.xnorm[2 * var + 1];
+      } else {
+        const double* pp = a.norm_part + 2 * (int64_t)a.prog_item[pbeg + j];
+        s0 = pp[0];
+        s1 = pp[1];
+        for (int64_t c = 1; c < nch; ++c) { s0 = s0 + pp[2 * c]; s1 = s1 + pp[2 * c + 1]; }
       }
-      if (j == 0 && __any((int)nan_tail)) {
-        if ((tid & 63) == 0) abort_flag = 1;
-        break;
-      }
+      normD[j] = __builtin_sqrt(s0);
+      tsq[j] = s1;
+      if (j == 0 && s1 != s1) abort_flag = 1;
     }
-    block_sums<2>(sums, red2);   // its barriers also publish abort_flag
-    if (j == 0 && abort_flag) {
-      const int size0 = (int)((n < (int64_t)k) ? n : (int64_t)k);
-      for (int i = tid; i < k; i += CP) co[i] = (i < size0) ? rrm::qnan() : 0.0;
-      if (tid == 0) write_fit(a.fit + cand, rrm::qnan(), a.sst, n, size0, 0);
-      return;
-    }
-    if (tid == 0) { normD[j] = __builtin_sqrt(sums[0]); tsq[j] = sums[1]; }
+  }
+  __syncthreads();
+  if (abort_flag) {
+    const int size0 = (int)((n < (int64_t)k) ? n : (int64_t)k);
+    for (int i = tid; i < k; i += CP) co[i] = (i < size0) ? rrm::qnan() : 0.0;
+    if (tid == 0) write_fit(a.fit + cand, rrm::qnan(), a.sst, n, size0, 0);
+    return;
   }
   double* const cy = S + (int64_t)k * ldx;
   // where the values of physical column c are before the first QR step
@@ -480,8 +478,10 @@ __global__ __launch_bounds__(CP) void k_ols_fp64(Fp64Args a) {
 }
 
 // Factor evaluation for k_ols_fp64: one workgroup of EV_THREADS threads per
-// (candidate, non-bare factor, chunk of chunk_rows rows).  Work item b is
-// the b-th chunk in pair order; pair_item[p] is the first item of pair p.
+// (candidate, non-bare factor, chunk of rrm::NORM_CHUNK rows).  Work item b
+// is the b-th chunk in pair order; pair_item[p] is the first item of pair p.
+// Besides the values, each work item leaves its chunk's two sums of squares
+// (rows >= 0 and rows >= 1) in the chunked order for the QR's norms.
 struct EvalArgs {
   const int32_t* cand_start;
   const int32_t* prog_start;
@@ -498,12 +498,15 @@ struct EvalArgs {
   const int32_t* pair_item;  // per pair: first work item
   int n_pairs;
   int items_per_pair;        // > 0: every pair has this many work items (one dataset)
-  int64_t chunk_rows;        // a multiple of EV_THREADS * R
+  double* norm_part;         // per work item: its chunk's sums of squares (rows >= 0, rows >= 1)
 };
 constexpr int EV_THREADS = 256;
+static_assert(EVAL_CHUNK_ROWS == rrm::NORM_CHUNK, "evaluation work items are the chunks of the norm order");
 
 template <int R>
 __global__ __launch_bounds__(EV_THREADS) void k_eval_fp64(EvalArgs a) {
+  static_assert(EV_THREADS == rrm::NORM_PARTIALS, "one thread per partial of the chunked order");
+  __shared__ double red[EV_THREADS / WAVE][2];
   const int item = (int)blockIdx.x;
   int lo;
   if (a.items_per_pair > 0) {
@@ -525,17 +528,69 @@ __global__ __launch_bounds__(EV_THREADS) void k_eval_fp64(EvalArgs a) {
   const uint32_t* code = a.code + a.prog_start[pbeg + j];
   const int len = a.prog_start[pbeg + j + 1] - a.prog_start[pbeg + j];
   double* const keep = a.slab + a.slab_off[pos] + (int64_t)(k + 1 + j) * D.ldx;
-  const int64_t r0 = (int64_t)(item - a.pair_item[lo]) * a.chunk_rows;
-  const int64_t r1 = (r0 + a.chunk_rows < D.n) ? r0 + a.chunk_rows : D.n;
-  // base is a multiple of EV_THREADS * R (which divides 4096) and below n,
-  // so every row touched is below ldx
+  const int64_t r0 = (int64_t)(item - a.pair_item[lo]) * rrm::NORM_CHUNK;
+  const int64_t r1 = (r0 + rrm::NORM_CHUNK < D.n) ? r0 + rrm::NORM_CHUNK : D.n;
+  // thread t adds the squares of rows r0 + t (mod 256) in increasing order:
+  // the chunk's partials of rrm::chunk_sum.  base is a multiple of
+  // EV_THREADS * R (which divides 4096) and below n, so every row touched is
+  // below ldx.
+  double s[2] = {0.0, 0.0};   // rows >= 0, rows >= 1
   for (int64_t base = r0; base < r1; base += (int64_t)EV_THREADS * R) {
     const int64_t row0 = base + (int64_t)threadIdx.x;
     double v[R];
     eval_f64<R, EV_THREADS>(code, len, a.K, D.X, D.ldx, row0, v);
 #pragma unroll
-    for (int r = 0; r < R; ++r) keep[row0 + (int64_t)EV_THREADS * r] = v[r];
+    for (int r = 0; r < R; ++r) {
+      const int64_t i = row0 + (int64_t)EV_THREADS * r;
+      keep[i] = v[r];
+      if (i < r1) {
+        s[0] = s[0] + v[r] * v[r];
+        if (i >= 1) s[1] = s[1] + v[r] * v[r];
+      }
+    }
   }
+  s[0] = wave_sum(s[0]);
+  s[1] = wave_sum(s[1]);
+  const int wave = (int)(threadIdx.x / WAVE);
+  if ((threadIdx.x & (WAVE - 1)) == 0) { red[wave][0] = s[0]; red[wave][1] = s[1]; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t0 = red[0][0], t1 = red[0][1];
+    for (int w = 1; w < EV_THREADS / WAVE; ++w) { t0 = t0 + red[w][0]; t1 = t1 + red[w][1]; }
+    a.norm_part[2 * (int64_t)item] = t0;
+    a.norm_part[2 * (int64_t)item + 1] = t1;
+  }
+}
+
+// Per feature of a dataset, the sums of squares of its column over rows >= 0
+// and rows >= 1 in the chunked order (one workgroup per feature, the chunks
+// in order): the norms the FP64 OLS form uses for a bare-variable factor.
+__global__ __launch_bounds__(EV_THREADS) void k_xnorm_fp64(const double* X, int64_t n, int64_t ldx, double* out) {
+  __shared__ double red[EV_THREADS / WAVE][2];
+  const double* x = X + (int64_t)blockIdx.x * ldx;
+  const int wave = (int)(threadIdx.x / WAVE);
+  double tot0 = 0.0, tot1 = 0.0;
+  for (int64_t r0 = 0; r0 < n; r0 += rrm::NORM_CHUNK) {
+    const int64_t r1 = (r0 + rrm::NORM_CHUNK < n) ? r0 + rrm::NORM_CHUNK : n;
+    double s[2] = {0.0, 0.0};
+    for (int64_t i = r0 + (int64_t)threadIdx.x; i < r1; i += EV_THREADS) {
+      const double v = x[i];
+      s[0] = s[0] + v * v;
+      if (i >= 1) s[1] = s[1] + v * v;
+    }
+    s[0] = wave_sum(s[0]);
+    s[1] = wave_sum(s[1]);
+    if ((threadIdx.x & (WAVE - 1)) == 0) { red[wave][0] = s[0]; red[wave][1] = s[1]; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t0 = red[0][0], t1 = red[0][1];
+      for (int w = 1; w < EV_THREADS / WAVE; ++w) { t0 = t0 + red[w][0]; t1 = t1 + red[w][1]; }
+      tot0 = (r0 == 0) ? t0 : tot0 + t0;
+      tot1 = (r0 == 0) ? t1 : tot1 + t1;
+    }
+    __syncthreads();   // red is reused by the next chunk
+  }
+  if (threadIdx.x == 0) { out[2 * blockIdx.x] = tot0; out[2 * blockIdx.x + 1] = tot1; }
 }
 
 // The factor-0 pre-pass: one workgroup of NAN0_ROWS threads per candidate,
@@ -661,6 +716,9 @@ hipError_t launch_ols_fp64(const OlsLaunch& L, hipStream_t s) {
   a.slab = L.slab; a.slab_off = L.slab_off; a.coefs = L.coefs; a.fit = L.fit; a.cand_base = L.cand_base;
   a.order = L.order;
   a.dead = L.dead;
+  a.norm_part = L.norm_part;
+  a.prog_item = L.prog_item;
+  a.xnorm = nullptr;   // bound per candidate
   const dim3 grid(L.n_cands), block(CP);
   const int R = L.rows;
   if (L.max_cols <= 16) {
@@ -678,11 +736,17 @@ hipError_t launch_ols_fp64(const OlsLaunch& L, hipStream_t s) {
 hipError_t launch_eval_fp64(const EvalLaunch& L, hipStream_t s) {
   if (L.n_items <= 0) return hipSuccess;
   EvalArgs a{L.cand_start, L.prog_start, L.code, L.K, L.ds, L.cand_ds, L.slab, L.slab_off, L.order, L.dead,
-             L.pair_pos, L.pair_fac, L.pair_item, L.n_pairs, L.items_per_pair, L.chunk_rows};
+             L.pair_pos, L.pair_fac, L.pair_item, L.n_pairs, L.items_per_pair, L.norm_part};
   const dim3 grid((unsigned)L.n_items), block(EV_THREADS);
   if (L.rows == 1) hipLaunchKernelGGL(k_eval_fp64<1>, grid, block, 0, s, a);
   else if (L.rows == 4) hipLaunchKernelGGL(k_eval_fp64<4>, grid, block, 0, s, a);
   else hipLaunchKernelGGL(k_eval_fp64<2>, grid, block, 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_xnorm_fp64(const double* X, int64_t n, int64_t ldx, int32_t d, double* out, hipStream_t s) {
+  if (d <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_xnorm_fp64, dim3((unsigned)d), dim3(EV_THREADS), 0, s, X, n, ldx, out);
   return hipGetLastError();
 }
 

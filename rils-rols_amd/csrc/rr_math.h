@@ -69,6 +69,40 @@ inline double canon_sum(int64_t lo, int64_t hi, F f) {
   return total;
 }
 
+// The order of the initial column norms of the FP64 OLS form and of its
+// first Householder step's tail sum: the rows fall into chunks of NORM_CHUNK;
+// within a chunk, 256 partial sums (row i added to partial i mod 256 in
+// increasing i), each group of 64 combined by the xor butterfly above, the 4
+// group sums added in order; the chunk sums are added in chunk order.  The
+// factor-evaluation kernel computes one chunk per workgroup while it writes
+// the column, so the QR kernel needs no pass over the column for its norms.
+constexpr int64_t NORM_CHUNK = 4096;
+constexpr int NORM_GROUPS = 4;
+constexpr int NORM_PARTIALS = 64 * NORM_GROUPS;
+
+template <class F>
+inline double chunk_sum(int64_t lo, int64_t hi, F f) {
+  double total = 0.0;
+  for (int64_t c0 = 0; c0 < hi; c0 += NORM_CHUNK) {
+    double acc[NORM_PARTIALS];
+    for (int l = 0; l < NORM_PARTIALS; ++l) acc[l] = 0.0;
+    const int64_t a0 = lo > c0 ? lo : c0, a1 = hi < c0 + NORM_CHUNK ? hi : c0 + NORM_CHUNK;
+    for (int64_t i = a0; i < a1; ++i) acc[i & (NORM_PARTIALS - 1)] = acc[i & (NORM_PARTIALS - 1)] + f(i);
+    double ct = 0.0;
+    for (int g = 0; g < NORM_GROUPS; ++g) {
+      double* a = acc + 64 * g;
+      for (int s = 32; s >= 1; s >>= 1) {
+        double nx[64];
+        for (int l = 0; l < 64; ++l) nx[l] = a[l] + a[l ^ s];
+        for (int l = 0; l < 64; ++l) a[l] = nx[l];
+      }
+      ct = (g == 0) ? a[0] : ct + a[0];
+    }
+    total = (c0 == 0) ? ct : total + ct;
+  }
+  return total;
+}
+
 // 2^k for -1022 <= k <= 1023 (exact).
 RR_HD inline double pow2i(int k) { return bits2d((uint64_t)(k + 1023) << 52); }
 

@@ -28,6 +28,8 @@ def lib():
         L.orc_math.argtypes = [ctypes.c_int, dp, dp, ctypes.c_int64]
         L.orc_canon_sum.argtypes = [dp, ctypes.c_int64, ctypes.c_int64]
         L.orc_canon_sum.restype = ctypes.c_double
+        L.orc_chunk_sum.argtypes = [dp, ctypes.c_int64, ctypes.c_int64]
+        L.orc_chunk_sum.restype = ctypes.c_double
         L.orc_dataset.argtypes = [dp, dp, ctypes.c_int64, ctypes.c_int]
         L.orc_dataset_free.argtypes = [ctypes.c_int]
         L.orc_dataset_stats.argtypes = [ctypes.c_int, dp, dp]
@@ -76,6 +78,11 @@ def math(op, x):
 def canon_sum(v, lo=0, hi=None):
     v = np.ascontiguousarray(v, dtype=np.float64)
     return lib().orc_canon_sum(_dp(v), lo, v.size if hi is None else hi)
+
+
+def chunk_sum(v, lo=0, hi=None):
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    return lib().orc_chunk_sum(_dp(v), lo, v.size if hi is None else hi)
 
 
 class Dataset:

@@ -76,6 +76,30 @@ def test_canonical_sum_exact_on_integers_and_close_to_fsum():
     assert O.canon_sum(one) == 3.25 and O.canon_sum(one, 138) == 0.0
 
 
+def test_chunked_sum_exact_on_integers_and_close_to_fsum():
+    for n in (1, 255, 4096, 4097, 12289, 100003):
+        v = np.arange(1, n + 1, dtype=np.float64)
+        assert O.chunk_sum(v) == n * (n + 1) / 2
+        assert O.chunk_sum(v, 1) == n * (n + 1) / 2 - 1
+    r = np.random.default_rng(5).normal(size=100003)
+    assert abs(O.chunk_sum(r) - math.fsum(r)) <= 1e-12 * math.fsum(abs(r))
+    assert O.chunk_sum(np.array([])) == 0.0
+
+
+def test_chunked_sum_order_known_answer():
+    """Rows 0 and 1 hold 1, row 4096 holds 2^53.  Chunked order: chunk 0 sums
+    to 2, chunk 1 to 2^53, total 2^53 + 2 (exact).  Canonical order: row 4096
+    joins row 0 in partial 0 first, 1 + 2^53 rounds to 2^53 (ties to even),
+    then partial 1 adds 1, which rounds away again: 2^53.  From row 1 on, the
+    chunked sum is 1 + 2^53, a tie that rounds to 2^53."""
+    v = np.zeros(8192)
+    v[0] = v[1] = 1.0
+    v[4096] = 2.0 ** 53
+    assert O.chunk_sum(v) == 2.0 ** 53 + 2
+    assert O.canon_sum(v) == 2.0 ** 53
+    assert O.chunk_sum(v, 1) == 2.0 ** 53
+
+
 # ---------------------------------------------------------------- QR solve
 def test_qr_full_rank_matches_lstsq():
     rng = np.random.default_rng(2)
