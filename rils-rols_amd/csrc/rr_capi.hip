@@ -227,11 +227,11 @@ bool program_ok(const uint32_t* code, int32_t len, int32_t d, int32_t n_consts, 
 
 // Structure of a batch and every program, each against the feature count of
 // its candidate's dataset (cand_d[c], or d for all when cand_d is null).
-// cost[c], when asked for, gets the candidate's per-sample work in the FP64
-// OLS form (programs plus 24 k^2 for the QR passes): used only to launch the
-// long candidates first, so that one of them does not start last and end the
-// launch alone; it changes no result.
-int check_batch(const rr_batch* b, int32_t mode, int32_t d, const int32_t* cand_d, int64_t* cost) {
+// prog_cost[p], when asked for, gets program p's per-sample evaluation work
+// (rough FP64 instruction units): used only to launch the long evaluations
+// first, so that one of them does not start last and end the launch alone;
+// it changes no result.
+int check_batch(const rr_batch* b, int32_t mode, int32_t d, const int32_t* cand_d, std::vector<int32_t>* prog_cost) {
   if (!b || b->n_cands < 0) return fail(RR_E_ARG, "bad batch");
   if (b->n_cands == 0) return RR_OK;
   if (!b->cand_start || !b->prog_start || (b->n_consts > 0 && !b->consts)) return fail(RR_E_ARG, "null batch array");
@@ -247,22 +247,22 @@ int check_batch(const rr_batch* b, int32_t mode, int32_t d, const int32_t* cand_
   if (b->prog_start[n_progs] > 0 && !b->code) return fail(RR_E_ARG, "null code array");
   for (int32_t p = 0; p < n_progs; ++p)
     if (b->prog_start[p + 1] < b->prog_start[p]) return fail(RR_E_ARG, "prog_start not monotone");
+  if (prog_cost) prog_cost->assign(std::max(n_progs, 1), 0);
   for (int32_t c = 0; c < b->n_cands; ++c) {
     const int32_t dc = cand_d ? cand_d[c] : d;
-    int64_t w = 0;
     for (int32_t p = b->cand_start[c]; p < b->cand_start[c + 1]; ++p) {
       const int32_t len = b->prog_start[p + 1] - b->prog_start[p];
-      if (program_ok(b->code + b->prog_start[p], len, dc, b->n_consts, &w)) continue;
+      int64_t w = 0;
+      if (program_ok(b->code + b->prog_start[p], len, dc, b->n_consts, &w)) {
+        if (prog_cost) (*prog_cost)[p] = (int32_t)std::min<int64_t>(w, INT32_MAX);
+        continue;
+      }
       int rc = check_program(b->code + b->prog_start[p], len, dc, b->n_consts, nullptr);   // for the message
       if (rc == RR_OK) rc = fail(RR_E_ARG, "invalid program");
       char m[64];
       snprintf(m, sizeof m, " (program %d)", p);
       g_err += m;
       return rc;
-    }
-    if (cost) {
-      const int64_t k = b->cand_start[c + 1] - b->cand_start[c] + 1;
-      cost[c] = w + 24 * k * k;
     }
   }
   return RR_OK;
@@ -457,14 +457,14 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
       min_d = std::min(min_d, D->d);
     }
   }
-  std::vector<int64_t> cost(std::max(nc, 0));
+  std::vector<int32_t> pcost;   // per program: evaluation cost (filled by the check)
   {
     std::vector<int32_t> cand_d;
     if (handles) {
       cand_d.resize(nc);
       for (int32_t c = 0; c < nc; ++c) cand_d[c] = ctx->ds[uniq[cand_ds[c]]].d;
     }
-    if ((rc = check_batch(b, mode, min_d, handles ? cand_d.data() : nullptr, cost.data()))) return rc;
+    if ((rc = check_batch(b, mode, min_d, handles ? cand_d.data() : nullptr, &pcost))) return rc;
   }
   if (!fit_out) return fail(RR_E_ARG, "null fit_out");
   if (mode == RR_MODE_OLS && !coefs_out) return fail(RR_E_ARG, "null coefs_out");
@@ -625,9 +625,16 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
     const int32_t m = list_in ? (int32_t)list_in->size() : nc;
     std::vector<int32_t> lpt(m);
     {
-      // counting sort by cost, longest first (ties in list order)
+      // counting sort by cost, longest first (ties in list order): the
+      // candidate's programs plus 24 k^2 for the QR passes
       constexpr int64_t NB = 8192;
-      auto key = [&](int32_t c) { return (int)(NB - 1 - std::min<int64_t>(cost[c], NB - 1)); };
+      auto cost = [&](int32_t c) {
+        const int64_t k = b->cand_start[c + 1] - b->cand_start[c] + 1;
+        int64_t w = 24 * k * k;
+        for (int32_t p = b->cand_start[c]; p < b->cand_start[c + 1]; ++p) w += pcost[p];
+        return w;
+      };
+      auto key = [&](int32_t c) { return (int)(NB - 1 - std::min<int64_t>(cost(c), NB - 1)); };
       std::vector<int32_t> cnt(NB + 1, 0);
       for (int32_t q = 0; q < m; ++q) ++cnt[key(list_in ? (*list_in)[q] : q)];
       for (int64_t i = 0, a = 0; i <= NB; ++i) { const int32_t t = cnt[i]; cnt[i] = (int32_t)a; a += t; }
@@ -664,20 +671,34 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
     int64_t max_items = 0;
     for (Chunk64& ch : chunks) {
       ch.p0 = (int)ppos.size();
-      int64_t items = 0, per = -1;   // per: the common item count of the pairs, or 0
+      // the chunk's programs that are not bare variables, longest first
+      // (counting sort on the evaluation cost, ties in position order)
+      constexpr int NB = 4096;
+      std::vector<int32_t> cnt(NB + 1, 0), progs, qs;
+      auto key = [&](int32_t p) { return NB - 1 - std::min(pcost[p], NB - 1); };
       for (int q = ch.c0; q < ch.c1; ++q) {
         const int32_t c = lpt[q];
-        const int64_t nch = (ds_of(c).n + rr::EVAL_CHUNK_ROWS - 1) / rr::EVAL_CHUNK_ROWS;
         for (int32_t p = b->cand_start[c]; p < b->cand_start[c + 1]; ++p) {
           const int32_t len = b->prog_start[p + 1] - b->prog_start[p];
           if (len == 1 && (b->code[b->prog_start[p]] & 0xffu) == RR_OP_LDV) continue;   // read from X
-          ppos.push_back(q);
-          pfac.push_back(p - b->cand_start[c]);
-          pitem.push_back((int32_t)items);
-          prog_item[p] = (int32_t)items;
-          items += nch;
-          per = (per < 0 || per == nch) ? nch : 0;
+          progs.push_back(p);
+          qs.push_back(q);
+          ++cnt[key(p)];
         }
+      }
+      for (int i = 0, a = 0; i <= NB; ++i) { const int32_t t = cnt[i]; cnt[i] = a; a += t; }
+      std::vector<int32_t> sorted(progs.size());
+      for (size_t i = 0; i < progs.size(); ++i) sorted[cnt[key(progs[i])]++] = (int32_t)i;
+      int64_t items = 0, per = -1;   // per: the common item count of the pairs, or 0
+      for (int32_t i : sorted) {
+        const int32_t p = progs[i], q = qs[i], c = lpt[q];
+        const int64_t nch = (ds_of(c).n + rr::EVAL_CHUNK_ROWS - 1) / rr::EVAL_CHUNK_ROWS;
+        ppos.push_back(q);
+        pfac.push_back(p - b->cand_start[c]);
+        pitem.push_back((int32_t)items);
+        prog_item[p] = (int32_t)items;
+        items += nch;
+        per = (per < 0 || per == nch) ? nch : 0;
       }
       if (items > INT32_MAX) return fail(RR_E_LIMIT, "too many factor-evaluation work items in one launch");
       ch.npair = (int)ppos.size() - ch.p0;
