@@ -550,7 +550,8 @@ def main():
         bare = [sum(1 for x in fs if BARE.match(x)) for fs in factors]
         byts = float(sum(fp64_algorithmic_bytes(int(k), int(st), bool(fl & 1), N_ROWS, b)
                          for k, st, fl, b in zip(ks, steps, flags, bare)))
-        # HIP events on the context's stream bracket the factor-0 pre-pass and the k_ols_fp64 launch
+        # HIP events on the context's stream bracket the three kernels of the step: the factor-0
+        # pre-pass, the factor evaluation and the QR (uploads happen before the first event)
         kernel_s = r["kms"].mean() / 1e3
         achieved = byts / kernel_s / 1e9
         out["fp64"] = {
@@ -561,7 +562,8 @@ def main():
             "p99_step_ms": float(np.percentile(r["lat"], 99) * 1e3),
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                          "frac": achieved / PEAK_HBM_GBS, "traffic": pmc_traffic("k_ols_fp64"),
-                         "kernel": "k_nan0_fp64 + k_ols_fp64<16,%d>" % runner.get_option("fp64_rows"),
+                         "kernel": "k_nan0_fp64 + k_eval_fp64<%d> + k_ols_fp64<16,%d>" % (
+                             runner.get_option("fp64_eval_rows"), runner.get_option("fp64_rows")),
                          "launch_ms": float(r["kms"].mean()),
                          "algorithmic_bytes": byts},
             "valu_tflops_by_formula": flops / kernel_s / 1e12,
