@@ -33,9 +33,10 @@
 //     row i added to partial (i mod 1024) in increasing i, each group of 64
 //     partials combined by an xor-butterfly (pairs at distance 32, 16, 8, 4,
 //     2, 1), the 16 group sums added in order; rrm::canon_sum (rr_math.h).
-//     The QR's initial column norms and its first step's tail sum use the
-//     chunked order of rrm::chunk_sum instead (chunks of 4096 rows, 256
-//     partials each, the chunk sums added in order);
+//     The QR's initial column norms, its first step's tail sum and the sum
+//     of squared residuals of the fitness use the chunked order of
+//     rrm::chunk_sum instead (chunks of 4096 rows, 256 partials each, the
+//     chunk sums added in order);
 //   * maxCoeff() is a left-to-right scan with strict '>' (first index wins);
 //   * sin/cos/exp/log come from rr_math.h; + - * / sqrt are IEEE;
 //   * no floating-point contraction (built with -ffp-contract=off);
@@ -611,7 +612,9 @@ static bool fit_eq(const Fit& a, const Fit& b) { return a.e0 == b.e0 && a.e1 == 
 static Fit score(const Node& tree, const Data& D) {
   const std::vector<double> yp = evaluate(tree, D);
   const std::vector<double>& y = D.y;
-  const double ssr = canon_sum(0, D.n, [&](int64_t i) { return (y[i] - yp[i]) * (y[i] - yp[i]); });
+  // the chunked order (rrm::chunk_sum): the GPU scores a candidate's rows in
+  // 4096-row chunks on separate workgroups
+  const double ssr = rrm::chunk_sum(0, D.n, [&](int64_t i) { return (y[i] - yp[i]) * (y[i] - yp[i]); });
   const double r2 = 1 - ssr / D.sst;
   const double rmse = std::sqrt(ssr / (double)D.n);
   if (r2 != r2 || rmse != rmse) return Fit{1000, 1000, 1000};

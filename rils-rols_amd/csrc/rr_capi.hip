@@ -98,6 +98,7 @@ struct rr_ctx {
   DevBuf d_f64_off, d_f64_ord;   // FP64 slab offsets and launched position -> candidate
   DevBuf d_pair_pos, d_pair_fac, d_pair_item;   // FP64 factor-evaluation work list
   DevBuf d_prog_item, d_norm_part;              // per program: first work item; per work item: chunk sums
+  DevBuf d_info, d_pos_item, d_score_part;      // FP64 scoring: QR outcome per candidate, work list, chunk sums
   DevBuf d_dead;                 // per candidate: decided by the factor-0 pre-pass (OLS mode)
   DevBuf d_blk_cand, d_blk_chunk, d_chunk_base, d_work, d_work_flags;   // FP32 chunk map and partials
   HostBuf h_stage, h_out;
@@ -330,7 +331,8 @@ int rr_destroy(rr_ctx* ctx) {
                     &ctx->d_coefs, &ctx->d_fit, &ctx->d_misc, &ctx->d_pred_e, &ctx->d_dstab, &ctx->d_cand_ds,
                     &ctx->d_blk_cand, &ctx->d_blk_chunk, &ctx->d_chunk_base, &ctx->d_work, &ctx->d_work_flags,
                     &ctx->d_f64_off, &ctx->d_f64_ord, &ctx->d_dead, &ctx->d_pair_pos, &ctx->d_pair_fac,
-                    &ctx->d_pair_item, &ctx->d_prog_item, &ctx->d_norm_part};
+                    &ctx->d_pair_item, &ctx->d_prog_item, &ctx->d_norm_part, &ctx->d_info, &ctx->d_pos_item,
+                    &ctx->d_score_part};
   for (DevBuf* b : bufs) b->release();
   ctx->h_stage.release();
   ctx->h_out.release();
@@ -547,7 +549,8 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
   const size_t s_tab = align_up(table.size() * sizeof(rr::DsDesc)), s_cds = align_up((size_t)nc * 4);
   const size_t s_chunks = 2 * align_up((size_t)total_chunks * 4) + align_up(chunk_base.size() * 4);
   // FP64 slab offsets, order and the three arrays of the evaluation work list
-  const size_t s_f64 = align_up((size_t)nc * 8) + align_up((size_t)nc * 4) + 4 * align_up((size_t)(np + 1) * 4);
+  const size_t s_f64 = align_up((size_t)nc * 8) + align_up((size_t)nc * 4) + 4 * align_up((size_t)(np + 1) * 4) +
+                       align_up((size_t)(nc + 1) * 4);   // + the scoring work list
   if (ctx->h_stage.ensure(s_cand + s_prog + s_code + s_K + s_off + s_tab + s_cds + s_chunks + 2 * s_f64) != RR_OK)
     return RR_E_NOMEM;
   size_t so = 0;
@@ -574,6 +577,7 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
   const int32_t* ddead = nullptr;   // OLS mode: written by the pre-pass for every candidate
   if (mode == RR_MODE_OLS) {
     if (ctx->d_dead.ensure((size_t)std::max(nc, 1) * 4) != RR_OK) return RR_E_NOMEM;
+    if (ctx->d_info.ensure((size_t)std::max(nc, 1) * 4) != RR_OK) return RR_E_NOMEM;
     ddead = (const int32_t*)ctx->d_dead.p;
   }
 
@@ -619,7 +623,7 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
   // the host work and the uploads, launch64 only launches.
   // positions c0..c1-1 share one slab; their factor-evaluation pairs are
   // p0..p0+npair-1 of the work list, with nitem work items
-  struct Chunk64 { int c0, c1, maxk, p0, npair, nitem, per; };
+  struct Chunk64 { int c0, c1, maxk, p0, npair, nitem, per, sitem, sper; };
   auto plan64 = [&](const std::vector<int32_t>* list_in, int slot, std::vector<Chunk64>& chunks) -> int {
     chunks.clear();
     const int32_t m = list_in ? (int32_t)list_in->size() : nc;
@@ -660,7 +664,7 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
         maxk = std::max(maxk, k);
         ++q1;
       }
-      chunks.push_back({q, q1, maxk, 0, 0, 0, 0});
+      chunks.push_back({q, q1, maxk, 0, 0, 0, 0, 0, 0});
       max_words = std::max(max_words, used);
       q = q1;
     }
@@ -720,6 +724,25 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
     }
     if ((rc = upload(ctx, ctx->d_prog_item, prog_item.data(), prog_item.size() * 4, o + 3 * sp))) return rc;
     if (ctx->d_norm_part.ensure((size_t)std::max<int64_t>(max_items, 1) * 16) != RR_OK) return RR_E_NOMEM;
+    // scoring work list: per position, ceil(n / EVAL_CHUNK_ROWS) chunks, numbered per slab chunk
+    std::vector<int32_t> pos_item(m + 1);
+    int64_t max_sitems = 0;
+    for (Chunk64& ch : chunks) {
+      int64_t it = 0, per = -1;
+      for (int q = ch.c0; q < ch.c1; ++q) {
+        const int64_t nch = (ds_of(lpt[q]).n + rr::EVAL_CHUNK_ROWS - 1) / rr::EVAL_CHUNK_ROWS;
+        pos_item[q] = (int32_t)it;
+        it += nch;
+        per = (per < 0 || per == nch) ? nch : 0;
+      }
+      if (it > INT32_MAX) return fail(RR_E_LIMIT, "too many scoring work items in one launch");
+      ch.sitem = (int)it;
+      ch.sper = per > 0 ? (int)per : 0;
+      max_sitems = std::max(max_sitems, it);
+    }
+    pos_item[m] = 0;
+    if ((rc = upload(ctx, ctx->d_pos_item, pos_item.data(), pos_item.size() * 4, o + 4 * sp))) return rc;
+    if (ctx->d_score_part.ensure((size_t)std::max<int64_t>(max_sitems, 1) * 8) != RR_OK) return RR_E_NOMEM;
     return RR_OK;
   };
   auto launch64 = [&](const std::vector<Chunk64>& chunks, hipStream_t stream) -> int {
@@ -747,8 +770,18 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
       L.slab_off = (const int64_t*)ctx->d_f64_off.p + ch.c0; L.coefs = dcoef; L.fit = dfit;
       L.cand_base = ch.c0; L.order = (const int32_t*)ctx->d_f64_ord.p; L.dead = ddead;
       L.norm_part = (const double*)ctx->d_norm_part.p; L.prog_item = (const int32_t*)ctx->d_prog_item.p;
+      L.info = (int32_t*)ctx->d_info.p;
       L.n_cands = ch.c1 - ch.c0; L.max_cols = ch.maxk; L.rows = (int)ctx->fp64_rows;
       RR_HIP(rr::launch_ols_fp64(L, stream));
+      rr::ScoreLaunch Sc{};
+      Sc.cand_start = dc; Sc.prog_start = dp; Sc.code = dcode; Sc.K = dK; Sc.ds = dtab; Sc.cand_ds = dcds;
+      Sc.coefs = dcoef; Sc.slab = (const double*)ctx->d_slab.p;
+      Sc.slab_off = (const int64_t*)ctx->d_f64_off.p + ch.c0;
+      Sc.order = (const int32_t*)ctx->d_f64_ord.p + ch.c0; Sc.dead = ddead; Sc.info = (const int32_t*)ctx->d_info.p;
+      Sc.pos_item = (const int32_t*)ctx->d_pos_item.p + ch.c0; Sc.n_pos = ch.c1 - ch.c0; Sc.n_items = ch.sitem;
+      Sc.items_per_pos = ch.sper; Sc.raw = 0; Sc.rows = (int)ctx->fp64_eval_rows;
+      Sc.part = (double*)ctx->d_score_part.p; Sc.fit = dfit;
+      RR_HIP(rr::launch_score_fp64(Sc, stream));
     }
     return RR_OK;
   };
@@ -778,6 +811,26 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
   std::vector<int32_t> rescored;   // candidates whose result came from the FP64 form (checked mode)
   std::vector<Chunk64> plan;
   if (mode == RR_MODE_OLS && precision == RR_FP64) { if ((rc = plan64(nullptr, 0, plan))) return rc; }
+  rr::ScoreLaunch raw64{};   // RAW mode, plain FP64: every candidate scored in chunks
+  if (mode == RR_MODE_RAW && precision == RR_FP64) {
+    std::vector<int32_t> pos_item(nc + 1);
+    int64_t it = 0, per = -1;
+    for (int32_t c = 0; c < nc; ++c) {
+      const int64_t nch = (ds_of(c).n + rr::EVAL_CHUNK_ROWS - 1) / rr::EVAL_CHUNK_ROWS;
+      pos_item[c] = (int32_t)it;
+      it += nch;
+      per = (per < 0 || per == nch) ? nch : 0;
+    }
+    if (it > INT32_MAX) return fail(RR_E_LIMIT, "too many scoring work items in one launch");
+    pos_item[nc] = (int32_t)it;
+    if ((rc = upload(ctx, ctx->d_pos_item, pos_item.data(), pos_item.size() * 4, so64))) return rc;
+    if (ctx->d_score_part.ensure((size_t)std::max<int64_t>(it, 1) * 8) != RR_OK) return RR_E_NOMEM;
+    raw64.cand_start = dc; raw64.prog_start = dp; raw64.code = dcode; raw64.K = dK; raw64.ds = dtab;
+    raw64.cand_ds = dcds; raw64.order = nullptr; raw64.dead = nullptr; raw64.info = nullptr;
+    raw64.pos_item = (const int32_t*)ctx->d_pos_item.p; raw64.n_pos = nc; raw64.n_items = (int)it;
+    raw64.items_per_pos = per > 0 ? (int)per : 0; raw64.raw = 1; raw64.rows = (int)ctx->fp64_eval_rows;
+    raw64.part = (double*)ctx->d_score_part.p; raw64.fit = dfit;
+  }
   if (checked && !list64.empty()) { if ((rc = plan64(&list64, 0, plan))) return rc; }
   // the timed region starts with every input uploaded
   RR_HIP(hipEventRecord(ctx->ev0, ctx->stream));
@@ -787,8 +840,7 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
     RR_HIP(rr::launch_nan0_fp64(L0, ctx->stream));
   }
   if (mode == RR_MODE_RAW && precision == RR_FP64) {
-    rr::RawLaunch L{dc, dp, dcode, dK, dtab, dcds, dfit, nc, (int)ctx->fp64_rows};
-    RR_HIP(rr::launch_raw_fp64(L, ctx->stream));
+    RR_HIP(rr::launch_score_fp64(raw64, ctx->stream));
   } else if (precision == RR_FP64) {
     if ((rc = launch64(plan, ctx->stream))) return rc;
   } else if (!checked) {

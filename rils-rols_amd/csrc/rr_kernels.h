@@ -38,21 +38,40 @@ struct OlsLaunch {
   const int32_t* dead;        // per candidate: outcome already written by the factor-0 pre-pass (or null)
   const double* norm_part;    // the evaluation's chunk sums (EvalLaunch::norm_part)
   const int32_t* prog_item;   // per program: its first evaluation work item in this launch
+  int32_t* info;              // per candidate: -1 (fitness written) or rank | steps << 16 (to be scored)
   int n_cands;                // candidates in this launch
   int max_cols;               // largest k among them
   int rows;                   // rows per lane per tile (1, 2 or 4)
 };
 
-struct RawLaunch {
+// Scoring of the FP64 form: one workgroup per (launched position, chunk of
+// EVAL_CHUNK_ROWS rows) computes the chunk's sum of squared residuals --
+// RAW: of the candidate's one program; OLS: of the OLS tree from the
+// coefficients and the factor columns (kept slab columns or X), for the
+// candidates whose info is >= 0 -- and then one thread per position adds the
+// chunk sums in order and writes the fitness.  Position p owns work items
+// pos_item[p] .. pos_item[p+1]-1 (items_per_pos > 0: p * items_per_pos ..).
+struct ScoreLaunch {
   const int32_t* cand_start;
   const int32_t* prog_start;
   const uint32_t* code;
   const double* K;
   const DsDesc* ds;
   const int32_t* cand_ds;
+  const double* coefs;       // OLS
+  const double* slab;        // OLS
+  const int64_t* slab_off;   // OLS, per launched position
+  const int32_t* order;      // launched position -> candidate (null: identity)
+  const int32_t* dead;       // or null
+  const int32_t* info;       // OLS: written by k_ols_fp64
+  const int32_t* pos_item;
+  int n_pos;
+  int n_items;
+  int items_per_pos;
+  int raw;
+  int rows;                  // RAW: rows per thread per pass of the evaluation (1, 2, 4)
+  double* part;              // per work item
   rr_fit* fit;
-  int n_cands;
-  int rows;
 };
 
 // Factor-0 pre-pass of the OLS form: one workgroup per candidate evaluates
@@ -107,7 +126,7 @@ hipError_t launch_eval_fp64(const EvalLaunch& L, hipStream_t s);
 // and rows >= 1 in the chunked order (rrm::chunk_sum)
 hipError_t launch_xnorm_fp64(const double* X, int64_t n, int64_t ldx, int32_t d, double* out, hipStream_t s);
 hipError_t launch_nan0_fp64(const Nan0Launch& L, hipStream_t s);
-hipError_t launch_raw_fp64(const RawLaunch& L, hipStream_t s);
+hipError_t launch_score_fp64(const ScoreLaunch& L, hipStream_t s);
 hipError_t launch_predict_fp64(const uint32_t* code, int len, const double* K, const double* X, int64_t n,
                                int64_t ldx, double* out, hipStream_t s);
 hipError_t launch_fill_f64(double* p, int64_t count, double value, hipStream_t s);

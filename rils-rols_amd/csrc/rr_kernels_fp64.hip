@@ -32,13 +32,18 @@
 //      (see nan_exit below);
 //   3. back-substitutes on the leading r x r block and scatters the
 //      coefficients through the column permutation (zeros for the rest);
+// Then k_score_fp64, one workgroup per (candidate, 4096-row chunk),
 //   4. reads the kept factor values (or the dataset's columns, which the QR
 //      does not touch) to form the prediction of the OLS tree that tune_constants
 //      builds (rils_rols_cpp.cpp:488-517): left-deep sum of
 //      c*f (or f when |c-1| < 1e-12), skipping |c| < 1e-12, free term last,
-//      and reduces SSR into 1-R2 and RMSE (fitness, 520-541).  A kept NaN
-//      coefficient makes every prediction NaN, so that case is written as
-//      the sentinel without the pass.
+//      and forms the chunk's sum of squared residuals in the chunked order
+//      (rrm::chunk_sum); k_fit_fp64 adds a candidate's chunk sums in order
+//      into 1-R2 and RMSE (fitness, 520-541).  A kept NaN coefficient makes
+//      every prediction NaN, so k_ols_fp64 writes that case as the sentinel
+//      and it is not scored.
+// A tree scored as given (RAW mode) takes the same two kernels, evaluating
+// the tree in place of the OLS prediction.
 #include "rr_device.h"
 #include "rr_kernels.h"
 
@@ -73,6 +78,7 @@ struct Fp64Args {
   const double* norm_part;   // per evaluation work item: its chunk's sums of squares (rows >= 0, >= 1)
   const int32_t* prog_item;  // per program: its first evaluation work item (bare variables: unused)
   const double* xnorm;       // the candidate's dataset: per feature, the same two sums over its column
+  int32_t* info;             // per candidate: -1 (fitness written here) or rank | steps << 16 (to be scored)
 };
 
 __device__ inline void write_fit(rr_fit* f, double ssr, double sst, int64_t n, int rank, int steps = 0) {
@@ -114,7 +120,7 @@ __global__ __launch_bounds__(CP) void k_ols_fp64(Fp64Args a) {
   __shared__ double normU[MAXK], normD[MAXK], tsq[MAXK], cc[MAXK];
   __shared__ int phys[MAXK];
   __shared__ int ovar[MAXK];   // per factor: its bare variable, or -1 (kept column)
-  __shared__ double red1[CW][1], redG[CW][G], redS[CW][2 * G];
+  __shared__ double redG[CW][G], redS[CW][2 * G];
   __shared__ int abort_flag;
 
   const int cand = a.order ? a.order[a.cand_base + (int)blockIdx.x] : a.cand_base + (int)blockIdx.x;
@@ -170,7 +176,7 @@ __global__ __launch_bounds__(CP) void k_ols_fp64(Fp64Args a) {
   if (abort_flag) {
     const int size0 = (int)((n < (int64_t)k) ? n : (int64_t)k);
     for (int i = tid; i < k; i += CP) co[i] = (i < size0) ? rrm::qnan() : 0.0;
-    if (tid == 0) write_fit(a.fit + cand, rrm::qnan(), a.sst, n, size0, 0);
+    if (tid == 0) { write_fit(a.fit + cand, rrm::qnan(), a.sst, n, size0, 0); a.info[cand] = -1; }
     return;
   }
   double* const cy = S + (int64_t)k * ldx;
@@ -404,7 +410,7 @@ __global__ __launch_bounds__(CP) void k_ols_fp64(Fp64Args a) {
   const int r = nonzero;
   if (nan_exit >= 0) {
     for (int i = tid; i < k; i += CP) co[phys[i]] = (i < r) ? rrm::qnan() : 0.0;
-    if (tid == 0) write_fit(a.fit + cand, rrm::qnan(), a.sst, n, r, steps);
+    if (tid == 0) { write_fit(a.fit + cand, rrm::qnan(), a.sst, n, r, steps); a.info[cand] = -1; }
     return;
   }
 
@@ -430,51 +436,14 @@ __global__ __launch_bounds__(CP) void k_ols_fp64(Fp64Args a) {
     co[phys[i]] = cv;
   }
   __syncthreads();
-  if (any_nan) {   // a kept NaN coefficient: every prediction is NaN
-    if (tid == 0) write_fit(a.fit + cand, rrm::qnan(), a.sst, n, r, steps);
-    return;
-  }
-
-  // ---- 4. prediction of the OLS tree and fitness
-  const double cf_free = co[nf];
-  double ssr[1] = {0.0};
-  for (int64_t base = 0; base < n; base += CP * R) {
-    const int64_t row0 = base + tid;
-    double yp[R];
-    bool any = false;
-#pragma unroll
-    for (int q = 0; q < R; ++q) yp[q] = 0.0;
-    for (int j = 0; j < nf; ++j) {
-      const double cj = co[j];
-      if (__builtin_fabs(cj) < EPS12) continue;
-      const double* keep = origin(j);
-      double f[R];
-#pragma unroll
-      for (int q = 0; q < R; ++q) f[q] = keep[row0 + (int64_t)CP * q];
-      const bool bare = __builtin_fabs(cj - 1) < EPS12;
-#pragma unroll
-      for (int q = 0; q < R; ++q) {
-        const double term = bare ? f[q] : cj * f[q];
-        yp[q] = any ? yp[q] + term : term;
-      }
-      any = true;
-    }
-    if (!(__builtin_fabs(cf_free) < EPS12)) {
-      const double term = cf_free * 1.0;
-#pragma unroll
-      for (int q = 0; q < R; ++q) yp[q] = any ? yp[q] + term : term;
-    }
-#pragma unroll
-    for (int q = 0; q < R; ++q) {
-      const int64_t i = row0 + (int64_t)CP * q;
-      if (i < n) {
-        const double d = a.y[i] - yp[q];
-        ssr[0] = ssr[0] + d * d;
-      }
+  if (tid == 0) {
+    if (any_nan) {   // a kept NaN coefficient: every prediction is NaN
+      write_fit(a.fit + cand, rrm::qnan(), a.sst, n, r, steps);
+      a.info[cand] = -1;
+    } else {
+      a.info[cand] = r | (steps << 16);   // scored by k_score_fp64 / k_fit_fp64
     }
   }
-  block_sums<1>(ssr, red1);
-  if (tid == 0) write_fit(a.fit + cand, ssr[0], a.sst, n, r, steps);
 }
 
 // Factor evaluation for k_ols_fp64: one workgroup of EV_THREADS threads per
@@ -638,48 +607,128 @@ __global__ __launch_bounds__(NAN0_ROWS) void k_nan0_fp64(Nan0Args a) {
   }
 }
 
-struct RawArgs {
+// Scoring in the chunked order: work item b is chunk (b - pos_item[p]) of
+// launched position p (pos_item[p] <= b < pos_item[p+1]); items_per_pos > 0
+// when every position has that many (one dataset).
+struct ScoreArgs {
   const int32_t* cand_start;
   const int32_t* prog_start;
   const uint32_t* code;
   const double* K;
   const DsDesc* ds;
   const int32_t* cand_ds;
-  const double* X;
-  const double* y;
-  const double* one;
-  int64_t n, ldx;
-  double sst;
+  const double* coefs;
+  const double* slab;
+  const int64_t* slab_off;   // per launched position
+  const int32_t* order;      // launched position -> candidate, or null (identity)
+  const int32_t* dead;
+  const int32_t* info;
+  const int32_t* pos_item;
+  int n_pos;
+  int items_per_pos;
+  double* part;
   rr_fit* fit;
 };
 
-// fitness() of a tree as given (no OLS): main-loop perturbations and the
-// re-scores after acceptance (rils_rols_cpp.cpp:631, 800, 828, 842).
-template <int R>
-__global__ __launch_bounds__(CP) void k_raw_fp64(RawArgs a) {
-  __shared__ double red1[CW][1];
-  const int cand = (int)blockIdx.x;
-  bind_dataset(a, cand);
-  const int tid = (int)threadIdx.x;
-  const int p = a.cand_start[cand];
-  const uint32_t* code = a.code + a.prog_start[p];
-  const int len = a.prog_start[p + 1] - a.prog_start[p];
-  double ssr[1] = {0.0};
-  for (int64_t base = 0; base < a.n; base += CP * R) {
-    const int64_t row0 = base + tid;
-    double yp[R];
-    eval_f64<R>(code, len, a.K, a.X, a.ldx, row0, yp);
+__device__ inline int score_pos(const ScoreArgs& a, int item) {
+  if (a.items_per_pos > 0) return item / a.items_per_pos;
+  int lo = 0, hi = a.n_pos - 1;   // last position whose first item <= item (uniform)
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (a.pos_item[mid] <= item) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// RAW: the tree (the candidate's one program) over the chunk; OLS: the OLS
+// tree's prediction from the coefficients and the factor columns, with the
+// arithmetic of tune_constants' tree (rils_rols_cpp.cpp:488-517).
+template <bool RAW, int R>
+__global__ __launch_bounds__(EV_THREADS) void k_score_fp64(ScoreArgs a) {
+  __shared__ double red[EV_THREADS / WAVE];
+  const int item = (int)blockIdx.x;
+  const int pos = score_pos(a, item);
+  const int cand = a.order ? a.order[pos] : pos;
+  if (a.dead && a.dead[cand]) return;   // uniform
+  if (!RAW && a.info[cand] < 0) return;
+  const DsDesc& D = a.ds[a.cand_ds ? a.cand_ds[cand] : 0];
+  const int64_t n = D.n, ldx = D.ldx;
+  const int64_t first = a.items_per_pos > 0 ? (int64_t)pos * a.items_per_pos : a.pos_item[pos];
+  const int64_t r0 = (item - first) * rrm::NORM_CHUNK;
+  const int64_t r1 = (r0 + rrm::NORM_CHUNK < n) ? r0 + rrm::NORM_CHUNK : n;
+  const int pbeg = a.cand_start[cand];
+  double s = 0.0;   // thread t: rows r0 + t (mod 256) in increasing order
+  if (RAW) {
+    const uint32_t* code = a.code + a.prog_start[pbeg];
+    const int len = a.prog_start[pbeg + 1] - a.prog_start[pbeg];
+    for (int64_t base = r0; base < r1; base += (int64_t)EV_THREADS * R) {   // rows < ldx
+      const int64_t row0 = base + (int64_t)threadIdx.x;
+      double v[R];
+      eval_f64<R, EV_THREADS>(code, len, a.K, D.X, ldx, row0, v);
 #pragma unroll
-    for (int q = 0; q < R; ++q) {
-      const int64_t i = row0 + (int64_t)CP * q;
-      if (i < a.n) {
-        const double d = a.y[i] - yp[q];
-        ssr[0] = ssr[0] + d * d;
+      for (int r = 0; r < R; ++r) {
+        const int64_t i = row0 + (int64_t)EV_THREADS * r;
+        if (i < r1) { const double d = D.y[i] - v[r]; s = s + d * d; }
       }
     }
+  } else {
+    const int nf = a.cand_start[cand + 1] - pbeg;
+    const int k = nf + 1;
+    const double* co = a.coefs + (int64_t)pbeg + cand;
+    const double* S = a.slab + a.slab_off[pos];
+    const double cf_free = co[nf];
+    for (int64_t i = r0 + (int64_t)threadIdx.x; i < r1; i += EV_THREADS) {
+      double yp = 0.0;
+      bool any = false;
+      for (int j = 0; j < nf; ++j) {
+        const double cj = co[j];
+        if (__builtin_fabs(cj) < EPS12) continue;
+        const uint32_t* code = a.code + a.prog_start[pbeg + j];
+        const int var = bare_variable(code, a.prog_start[pbeg + j + 1] - a.prog_start[pbeg + j]);
+        const double f = var >= 0 ? D.X[(int64_t)var * ldx + i] : S[(int64_t)(k + 1 + j) * ldx + i];
+        const double term = (__builtin_fabs(cj - 1) < EPS12) ? f : cj * f;
+        yp = any ? yp + term : term;
+        any = true;
+      }
+      if (!(__builtin_fabs(cf_free) < EPS12)) {
+        const double term = cf_free * 1.0;
+        yp = any ? yp + term : term;
+      }
+      const double d = D.y[i] - yp;
+      s = s + d * d;
+    }
   }
-  block_sums<1>(ssr, red1);
-  if (tid == 0) write_fit(a.fit + cand, ssr[0], a.sst, a.n, 0);
+  s = wave_sum(s);
+  if ((threadIdx.x & (WAVE - 1)) == 0) red[threadIdx.x / WAVE] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = red[0];
+    for (int w = 1; w < EV_THREADS / WAVE; ++w) t = t + red[w];
+    a.part[item] = t;
+  }
+}
+
+// One thread per launched position: the chunk sums in chunk order, then the
+// fitness (rils_rols_cpp.cpp:520-541).
+template <bool RAW>
+__global__ __launch_bounds__(256) void k_fit_fp64(ScoreArgs a) {
+  const int pos = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (pos >= a.n_pos) return;
+  const int cand = a.order ? a.order[pos] : pos;
+  if (a.dead && a.dead[cand]) return;
+  int rank = 0, steps = 0;
+  if (!RAW) {
+    const int32_t info = a.info[cand];
+    if (info < 0) return;
+    rank = info & 0xffff;
+    steps = info >> 16;
+  }
+  const DsDesc& D = a.ds[a.cand_ds ? a.cand_ds[cand] : 0];
+  const int64_t first = a.items_per_pos > 0 ? (int64_t)pos * a.items_per_pos : a.pos_item[pos];
+  const int64_t nch = (D.n + rrm::NORM_CHUNK - 1) / rrm::NORM_CHUNK;
+  double ssr = a.part[first];
+  for (int64_t c = 1; c < nch; ++c) ssr = ssr + a.part[first + c];
+  write_fit(a.fit + cand, ssr, D.sst, D.n, rank, steps);
 }
 
 // node::evaluate_all over all samples; many waves, any order (no reduction)
@@ -718,6 +767,7 @@ hipError_t launch_ols_fp64(const OlsLaunch& L, hipStream_t s) {
   a.dead = L.dead;
   a.norm_part = L.norm_part;
   a.prog_item = L.prog_item;
+  a.info = L.info;
   a.xnorm = nullptr;   // bound per candidate
   const dim3 grid(L.n_cands), block(CP);
   const int R = L.rows;
@@ -757,13 +807,20 @@ hipError_t launch_nan0_fp64(const Nan0Launch& L, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_raw_fp64(const RawLaunch& L, hipStream_t s) {
-  RawArgs a;
-  a.cand_start = L.cand_start; a.prog_start = L.prog_start; a.code = L.code; a.K = L.K;
-  a.ds = L.ds; a.cand_ds = L.cand_ds; a.fit = L.fit;
-  if (L.rows == 1) hipLaunchKernelGGL(k_raw_fp64<1>, dim3(L.n_cands), dim3(CP), 0, s, a);
-  else if (L.rows == 2) hipLaunchKernelGGL(k_raw_fp64<2>, dim3(L.n_cands), dim3(CP), 0, s, a);
-  else hipLaunchKernelGGL(k_raw_fp64<4>, dim3(L.n_cands), dim3(CP), 0, s, a);
+hipError_t launch_score_fp64(const ScoreLaunch& L, hipStream_t s) {
+  if (L.n_pos <= 0) return hipSuccess;
+  ScoreArgs a{L.cand_start, L.prog_start, L.code, L.K, L.ds, L.cand_ds, L.coefs, L.slab, L.slab_off, L.order,
+              L.dead, L.info, L.pos_item, L.n_pos, L.items_per_pos, L.part, L.fit};
+  const dim3 grid((unsigned)L.n_items), block(EV_THREADS), fgrid((unsigned)((L.n_pos + 255) / 256));
+  if (L.raw) {
+    if (L.rows == 1) hipLaunchKernelGGL((k_score_fp64<true, 1>), grid, block, 0, s, a);
+    else if (L.rows == 4) hipLaunchKernelGGL((k_score_fp64<true, 4>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((k_score_fp64<true, 2>), grid, block, 0, s, a);
+    hipLaunchKernelGGL(k_fit_fp64<true>, fgrid, dim3(256), 0, s, a);
+  } else {
+    hipLaunchKernelGGL((k_score_fp64<false, 1>), grid, block, 0, s, a);
+    hipLaunchKernelGGL(k_fit_fp64<false>, fgrid, dim3(256), 0, s, a);
+  }
   return hipGetLastError();
 }
 
