@@ -33,10 +33,11 @@
 //     row i added to partial (i mod 1024) in increasing i, each group of 64
 //     partials combined by an xor-butterfly (pairs at distance 32, 16, 8, 4,
 //     2, 1), the 16 group sums added in order; rrm::canon_sum (rr_math.h).
-//     The QR's initial column norms, its first step's tail sum and the sum
-//     of squared residuals of the fitness use the chunked order of
-//     rrm::chunk_sum instead (chunks of 4096 rows, 256 partials each, the
-//     chunk sums added in order);
+//     Every sum of the column-pivoted QR (norms, tail sums, Householder dot
+//     products) and the fitness's sum of squared residuals use the chunked
+//     order of rrm::chunk_sum instead (chunks of 4096 rows, 256 partials
+//     each, the chunk sums added in order): the GPU does each pass over a
+//     candidate's rows as independent chunks;
 //   * maxCoeff() is a left-to-right scan with strict '>' (first index wins);
 //   * sin/cos/exp/log come from rr_math.h; + - * / sqrt are IEEE;
 //   * no floating-point contraction (built with -ffp-contract=off);
@@ -497,14 +498,11 @@ static std::vector<double> colpiv_qr_solve(std::vector<std::vector<double>> A, c
   const double eps = 2.220446049250313e-16;    // NumTraits<double>::epsilon()
   std::vector<double> hc(size), normU(cols), normD(cols);
   std::vector<int> transp(size);
-  // initial norms and the first step's tail sums in the chunked order
-  // (rrm::chunk_sum), every later sum in the canonical one
-  std::vector<double> tail1(cols);
+  // every sum over rows in the chunked order (rrm::chunk_sum)
   for (int j = 0; j < cols; ++j) {
     const std::vector<double>& a = A[j];
     normD[j] = std::sqrt(rrm::chunk_sum(0, rows, [&](int64_t i) { return a[i] * a[i]; }));
     normU[j] = normD[j];
-    tail1[j] = rrm::chunk_sum(1, rows, [&](int64_t i) { return a[i] * a[i]; });
   }
   double mx = normU[0];
   for (int j = 1; j < cols; ++j) mx = (mx < normU[j]) ? normU[j] : mx;
@@ -523,14 +521,11 @@ static std::vector<double> colpiv_qr_solve(std::vector<std::vector<double>> A, c
       std::swap(A[k], A[bi]);
       std::swap(normU[k], normU[bi]);
       std::swap(normD[k], normD[bi]);
-      std::swap(tail1[k], tail1[bi]);
     }
     // makeHouseholderInPlace on A[k][k..rows)
     std::vector<double>& v = A[k];
     const double c0 = v[k];
-    const double tailSq = (rows - k == 1) ? 0.0
-                          : (k == 0) ? tail1[0]
-                                     : canon_sum(k + 1, rows, [&](int64_t i) { return v[i] * v[i]; });
+    const double tailSq = (rows - k == 1) ? 0.0 : rrm::chunk_sum(k + 1, rows, [&](int64_t i) { return v[i] * v[i]; });
     double tau, beta;
     if (tailSq <= 2.2250738585072014e-308) {
       tau = 0.0; beta = c0;
@@ -550,7 +545,7 @@ static std::vector<double> colpiv_qr_solve(std::vector<std::vector<double>> A, c
     } else if (tau != 0.0) {
       for (int j = k + 1; j < cols; ++j) {
         std::vector<double>& a = A[j];
-        double t = canon_sum(k + 1, rows, [&](int64_t i) { return v[i] * a[i]; });
+        double t = rrm::chunk_sum(k + 1, rows, [&](int64_t i) { return v[i] * a[i]; });
         t = t + a[k];
         a[k] = a[k] - tau * t;
         for (int64_t i = k + 1; i < rows; ++i) a[i] = a[i] - (tau * v[i]) * t;
@@ -566,7 +561,7 @@ static std::vector<double> colpiv_qr_solve(std::vector<std::vector<double>> A, c
         const double temp2 = temp * (ratio * ratio);
         if (temp2 <= norm_downdate_threshold) {
           const std::vector<double>& a = A[j];
-          normD[j] = std::sqrt(canon_sum(k + 1, rows, [&](int64_t i) { return a[i] * a[i]; }));
+          normD[j] = std::sqrt(rrm::chunk_sum(k + 1, rows, [&](int64_t i) { return a[i] * a[i]; }));
           normU[j] = normD[j];
         } else {
           normU[j] *= std::sqrt(temp);
@@ -588,7 +583,7 @@ static std::vector<double> colpiv_qr_solve(std::vector<std::vector<double>> A, c
     if (rows - h == 1) {
       c[h] *= (1.0 - tau);
     } else if (tau != 0.0) {
-      double t = canon_sum(h + 1, rows, [&](int64_t i) { return v[i] * c[i]; });
+      double t = rrm::chunk_sum(h + 1, rows, [&](int64_t i) { return v[i] * c[i]; });
       t = t + c[h];
       c[h] = c[h] - tau * t;
       for (int64_t i = h + 1; i < rows; ++i) c[i] = c[i] - (tau * v[i]) * t;

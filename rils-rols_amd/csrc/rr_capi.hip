@@ -99,6 +99,7 @@ struct rr_ctx {
   DevBuf d_pair_pos, d_pair_fac, d_pair_item;   // FP64 factor-evaluation work list
   DevBuf d_prog_item, d_norm_part;              // per program: first work item; per work item: chunk sums
   DevBuf d_info, d_pos_item, d_score_part;      // FP64 scoring: QR outcome per candidate, work list, chunk sums
+  DevBuf d_qst, d_qphys, d_qcold, d_qpart;      // FP64 QR state between its kernels
   DevBuf d_dead;                 // per candidate: decided by the factor-0 pre-pass (OLS mode)
   DevBuf d_blk_cand, d_blk_chunk, d_chunk_base, d_work, d_work_flags;   // FP32 chunk map and partials
   HostBuf h_stage, h_out;
@@ -332,7 +333,7 @@ int rr_destroy(rr_ctx* ctx) {
                     &ctx->d_blk_cand, &ctx->d_blk_chunk, &ctx->d_chunk_base, &ctx->d_work, &ctx->d_work_flags,
                     &ctx->d_f64_off, &ctx->d_f64_ord, &ctx->d_dead, &ctx->d_pair_pos, &ctx->d_pair_fac,
                     &ctx->d_pair_item, &ctx->d_prog_item, &ctx->d_norm_part, &ctx->d_info, &ctx->d_pos_item,
-                    &ctx->d_score_part};
+                    &ctx->d_score_part, &ctx->d_qst, &ctx->d_qphys, &ctx->d_qcold, &ctx->d_qpart};
   for (DevBuf* b : bufs) b->release();
   ctx->h_stage.release();
   ctx->h_out.release();
@@ -623,7 +624,13 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
   // the host work and the uploads, launch64 only launches.
   // positions c0..c1-1 share one slab; their factor-evaluation pairs are
   // p0..p0+npair-1 of the work list, with nitem work items
-  struct Chunk64 { int c0, c1, maxk, p0, npair, nitem, per, sitem, sper; };
+  // sitem chunk work items of the QR and scoring (sper each, or 0), and per QR
+  // step s the positions taking part (step_pos[s]: those with more than s
+  // columns, a prefix) and their chunk work items (step_items[s])
+  struct Chunk64 {
+    int c0, c1, maxk, p0, npair, nitem, per, sitem, sper;
+    std::vector<int32_t> step_pos, step_items;
+  };
   auto plan64 = [&](const std::vector<int32_t>* list_in, int slot, std::vector<Chunk64>& chunks) -> int {
     chunks.clear();
     const int32_t m = list_in ? (int32_t)list_in->size() : nc;
@@ -647,6 +654,16 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
         lpt[cnt[key(c)]++] = c;
       }
     }
+    {
+      // then by column count, most first, keeping that order within a count:
+      // step p of the QR takes the positions with more than p columns, a prefix
+      std::vector<int32_t> cnt(RR_MAX_COLS + 2, 0), tmp(m);
+      auto kk = [&](int32_t c) { return RR_MAX_COLS + 1 - (b->cand_start[c + 1] - b->cand_start[c] + 1); };
+      for (int32_t q = 0; q < m; ++q) ++cnt[kk(lpt[q])];
+      for (int i = 0, acc = 0; i < RR_MAX_COLS + 2; ++i) { const int32_t t = cnt[i]; cnt[i] = acc; acc += t; }
+      for (int32_t q = 0; q < m; ++q) tmp[cnt[kk(lpt[q])]++] = lpt[q];
+      lpt.swap(tmp);
+    }
     if (m == 0) return RR_OK;
     std::vector<int64_t> off(m);
     const int64_t budget = std::max<int64_t>(ctx->slab_budget_mb, 1) << 20;
@@ -664,7 +681,7 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
         maxk = std::max(maxk, k);
         ++q1;
       }
-      chunks.push_back({q, q1, maxk, 0, 0, 0, 0, 0, 0});
+      chunks.push_back({q, q1, maxk, 0, 0, 0, 0, 0, 0, {}, {}});
       max_words = std::max(max_words, used);
       q = q1;
     }
@@ -726,7 +743,7 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
     if (ctx->d_norm_part.ensure((size_t)std::max<int64_t>(max_items, 1) * 16) != RR_OK) return RR_E_NOMEM;
     // scoring work list: per position, ceil(n / EVAL_CHUNK_ROWS) chunks, numbered per slab chunk
     std::vector<int32_t> pos_item(m + 1);
-    int64_t max_sitems = 0;
+    int64_t max_sitems = 0, max_qpos = 0, max_qcol = 0, max_qpart = 0;
     for (Chunk64& ch : chunks) {
       int64_t it = 0, per = -1;
       for (int q = ch.c0; q < ch.c1; ++q) {
@@ -739,10 +756,26 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
       ch.sitem = (int)it;
       ch.sper = per > 0 ? (int)per : 0;
       max_sitems = std::max(max_sitems, it);
+      for (int st = 0; st < ch.maxk; ++st) {
+        int na = 0;
+        while (ch.c0 + na < ch.c1 && b->cand_start[lpt[ch.c0 + na] + 1] - b->cand_start[lpt[ch.c0 + na]] + 1 > st) ++na;
+        if (na == 0) break;
+        ch.step_pos.push_back(na);
+        ch.step_items.push_back(ch.c0 + na < ch.c1 ? pos_item[ch.c0 + na] : ch.sitem);
+      }
+      const int mc = ch.c1 - ch.c0;
+      max_qpos = std::max<int64_t>(max_qpos, mc);
+      max_qcol = std::max<int64_t>(max_qcol, (int64_t)mc * ch.maxk);
+      max_qpart = std::max<int64_t>(max_qpart, it * 2 * ch.maxk);
     }
     pos_item[m] = 0;
     if ((rc = upload(ctx, ctx->d_pos_item, pos_item.data(), pos_item.size() * 4, o + 4 * sp))) return rc;
     if (ctx->d_score_part.ensure((size_t)std::max<int64_t>(max_sitems, 1) * 8) != RR_OK) return RR_E_NOMEM;
+    if (ctx->d_qst.ensure((size_t)std::max<int64_t>(max_qpos, 1) * sizeof(rr::QrState)) != RR_OK ||
+        ctx->d_qphys.ensure((size_t)std::max<int64_t>(max_qcol, 1) * 4) != RR_OK ||
+        ctx->d_qcold.ensure((size_t)std::max<int64_t>(max_qcol, 1) * 5 * 8) != RR_OK ||
+        ctx->d_qpart.ensure((size_t)std::max<int64_t>(max_qpart, 1) * 8) != RR_OK)
+      return RR_E_NOMEM;
     return RR_OK;
   };
   auto launch64 = [&](const std::vector<Chunk64>& chunks, hipStream_t stream) -> int {
@@ -764,15 +797,18 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
         E.norm_part = (double*)ctx->d_norm_part.p; E.rows = (int)ctx->fp64_eval_rows;
         RR_HIP(rr::launch_eval_fp64(E, stream));
       }
-      rr::OlsLaunch L;
-      L.cand_start = dc; L.prog_start = dp; L.code = dcode; L.K = dK; L.ds = dtab; L.cand_ds = dcds;
-      L.slab = (double*)ctx->d_slab.p;
-      L.slab_off = (const int64_t*)ctx->d_f64_off.p + ch.c0; L.coefs = dcoef; L.fit = dfit;
-      L.cand_base = ch.c0; L.order = (const int32_t*)ctx->d_f64_ord.p; L.dead = ddead;
-      L.norm_part = (const double*)ctx->d_norm_part.p; L.prog_item = (const int32_t*)ctx->d_prog_item.p;
-      L.info = (int32_t*)ctx->d_info.p;
-      L.n_cands = ch.c1 - ch.c0; L.max_cols = ch.maxk; L.rows = (int)ctx->fp64_rows;
-      RR_HIP(rr::launch_ols_fp64(L, stream));
+      rr::QrLaunch Q{};
+      Q.cand_start = dc; Q.prog_start = dp; Q.code = dcode; Q.ds = dtab; Q.cand_ds = dcds;
+      Q.coefs = dcoef; Q.fit = dfit; Q.dead = ddead; Q.info = (int32_t*)ctx->d_info.p;
+      Q.order = (const int32_t*)ctx->d_f64_ord.p + ch.c0; Q.slab = (double*)ctx->d_slab.p;
+      Q.slab_off = (const int64_t*)ctx->d_f64_off.p + ch.c0;
+      Q.pos_item = (const int32_t*)ctx->d_pos_item.p + ch.c0; Q.items_per_pos = ch.sper; Q.n_pos = ch.c1 - ch.c0;
+      Q.norm_part = (const double*)ctx->d_norm_part.p; Q.prog_item = (const int32_t*)ctx->d_prog_item.p;
+      Q.st = (rr::QrState*)ctx->d_qst.p; Q.phys = (int32_t*)ctx->d_qphys.p; Q.cold = (double*)ctx->d_qcold.p;
+      Q.ks = ch.maxk; Q.part = (double*)ctx->d_qpart.p;
+      RR_HIP(rr::launch_qr_init(Q, stream));
+      for (size_t st = 0; st < ch.step_pos.size(); ++st) RR_HIP(rr::launch_qr_step(Q, ch.step_pos[st], ch.step_items[st], stream));
+      RR_HIP(rr::launch_qr_solve(Q, stream));
       rr::ScoreLaunch Sc{};
       Sc.cand_start = dc; Sc.prog_start = dp; Sc.code = dcode; Sc.K = dK; Sc.ds = dtab; Sc.cand_ds = dcds;
       Sc.coefs = dcoef; Sc.slab = (const double*)ctx->d_slab.p;

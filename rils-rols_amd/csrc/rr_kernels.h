@@ -22,26 +22,40 @@ struct DsDesc {
   const double* xnorm;    // per feature: sums of squares over rows >= 0 and >= 1 (chunked order)
 };
 
-struct OlsLaunch {
-  const int32_t* cand_start;  // device
+// Step-synchronous column-pivoted QR over the positions 0..n_pos-1 of a
+// launch (sorted by column count, most first; pointers already offset to the
+// launch's first position): launch_qr_init, then per step p launch_qr_step
+// over the first n_act positions (those with more than p columns) and their
+// chunk work items, then launch_qr_solve.  ks: the most columns of any
+// position; phys: n_pos x ks ints; cold: 5 x n_pos x ks doubles; part: 2 ks
+// doubles per chunk work item.
+struct QrState {
+  int32_t k, size, p, nonzero, nan_exit, steps, flags, pad;
+  double thr, tau, den, pad2;
+};
+struct QrLaunch {
+  const int32_t* cand_start;
   const int32_t* prog_start;
   const uint32_t* code;
-  const double* K;
-  const DsDesc* ds;           // device table of datasets
-  const int32_t* cand_ds;     // per candidate (batch index), or null
-  double* slab;               // work columns
-  const int64_t* slab_off;    // per launched candidate, in doubles
+  const DsDesc* ds;
+  const int32_t* cand_ds;
   double* coefs;
   rr_fit* fit;
-  int cand_base;              // first launched position of this launch
-  const int32_t* order;       // launched position -> candidate (null: identity)
-  const int32_t* dead;        // per candidate: outcome already written by the factor-0 pre-pass (or null)
-  const double* norm_part;    // the evaluation's chunk sums (EvalLaunch::norm_part)
-  const int32_t* prog_item;   // per program: its first evaluation work item in this launch
+  const int32_t* dead;
   int32_t* info;              // per candidate: -1 (fitness written) or rank | steps << 16 (to be scored)
-  int n_cands;                // candidates in this launch
-  int max_cols;               // largest k among them
-  int rows;                   // rows per lane per tile (1, 2 or 4)
+  const int32_t* order;       // launched position -> candidate
+  double* slab;
+  const int64_t* slab_off;    // per launched position
+  const int32_t* pos_item;    // per launched position: first chunk work item
+  int items_per_pos;          // > 0 when every position has that many chunk items
+  int n_pos;
+  const double* norm_part;    // the evaluation's chunk sums (EvalLaunch::norm_part)
+  const int32_t* prog_item;   // per program: its first evaluation work item
+  QrState* st;
+  int32_t* phys;
+  double* cold;
+  int ks;
+  double* part;
 };
 
 // Scoring of the FP64 form: one workgroup per (launched position, chunk of
@@ -120,7 +134,9 @@ struct EvalLaunch {
 };
 constexpr int64_t EVAL_CHUNK_ROWS = 4096;   // the chunk of the norms' order (rrm::NORM_CHUNK)
 
-hipError_t launch_ols_fp64(const OlsLaunch& L, hipStream_t s);
+hipError_t launch_qr_init(const QrLaunch& L, hipStream_t s);
+hipError_t launch_qr_step(const QrLaunch& L, int n_act, int n_items, hipStream_t s);
+hipError_t launch_qr_solve(const QrLaunch& L, hipStream_t s);
 hipError_t launch_eval_fp64(const EvalLaunch& L, hipStream_t s);
 // out[2f], out[2f+1]: sums of squares of feature f's column over rows >= 0
 // and rows >= 1 in the chunked order (rrm::chunk_sum)

@@ -14,7 +14,7 @@
 // slab in HBM, spread over many small workgroups (row chunks), so that a
 // long program does not hold one compute unit for the whole column.  A
 // factor that is a bare variable keeps no copy: its column is the dataset's.
-// Then k_ols_fp64
+// Then the QR kernels (k_qr_*)
 //   1. reads the factor columns once for their norms;
 //   2. runs the pivoted QR on the slab: per step one pass computing the
 //      Householder dot products and one pass applying the reflector, which
@@ -40,7 +40,7 @@
 //      and forms the chunk's sum of squared residuals in the chunked order
 //      (rrm::chunk_sum); k_fit_fp64 adds a candidate's chunk sums in order
 //      into 1-R2 and RMSE (fitness, 520-541).  A kept NaN coefficient makes
-//      every prediction NaN, so k_ols_fp64 writes that case as the sentinel
+//      every prediction NaN, so k_qr_solve writes that case as the sentinel
 //      and it is not scored.
 // A tree scored as given (RAW mode) takes the same two kernels, evaluating
 // the tree in place of the OLS prediction.
@@ -54,32 +54,6 @@ namespace {
 constexpr double EPS12 = 1e-12;                 // node.h:13-14
 constexpr double DBL_EPS = 2.220446049250313e-16;
 constexpr double DBL_MIN_ = 2.2250738585072014e-308;
-
-struct Fp64Args {
-  const int32_t* cand_start;
-  const int32_t* prog_start;
-  const uint32_t* code;
-  const double* K;
-  const DsDesc* ds;
-  const int32_t* cand_ds;
-  // the candidate's dataset, bound at kernel start
-  const double* X;
-  const double* y;
-  const double* one;   // ldx ones: the free term's column
-  int64_t n, ldx;
-  double sst;
-  double* slab;
-  const int64_t* slab_off;
-  double* coefs;
-  rr_fit* fit;
-  int cand_base;
-  const int32_t* order;   // launched position -> candidate, or null (identity)
-  const int32_t* dead;    // per candidate: outcome written by the factor-0 pre-pass, or null
-  const double* norm_part;   // per evaluation work item: its chunk's sums of squares (rows >= 0, >= 1)
-  const int32_t* prog_item;  // per program: its first evaluation work item (bare variables: unused)
-  const double* xnorm;       // the candidate's dataset: per feature, the same two sums over its column
-  int32_t* info;             // per candidate: -1 (fitness written here) or rank | steps << 16 (to be scored)
-};
 
 __device__ inline void write_fit(rr_fit* f, double ssr, double sst, int64_t n, int rank, int steps = 0) {
   const double r2 = 1 - ssr / sst;
@@ -96,357 +70,475 @@ __device__ inline void write_fit(rr_fit* f, double ssr, double sst, int64_t n, i
   *f = o;
 }
 
-// Bind the dataset candidate `cand` is scored against (uniform: scalar loads).
-template <class A>
-__device__ inline void bind_dataset(A& a, int cand) {
-  const DsDesc& D = a.ds[a.cand_ds ? a.cand_ds[cand] : 0];
-  a.X = D.X; a.y = D.y; a.one = D.one; a.n = D.n; a.ldx = D.ldx; a.sst = D.sst;
-}
-__device__ inline void bind_xnorm(Fp64Args& a, int cand) { a.xnorm = a.ds[a.cand_ds ? a.cand_ds[cand] : 0].xnorm; }
-
 // Variable index of a program that is one LDV, else -1.
 __device__ inline int bare_variable(const uint32_t* code, int len) {
   return (len == 1 && (code[0] & 0xffu) == RR_OP_LDV) ? (int)(code[0] >> 8) : -1;
 }
 
-__device__ inline int64_t tile_start(int64_t lo, int R) { return (lo / (CP * R)) * (CP * R); }
+// ---------------------------------------------------------------- the QR
+// The batch's candidates (launched positions 0..n_pos-1, sorted by column
+// count, most first) factor in lockstep: step p of every candidate with more
+// than p columns runs as
+//   k_qr_begin   one thread per candidate: pivot choice, rank threshold,
+//                reflector (beta, tau), R(p,p);
+//   k_qr_dot     one workgroup per (candidate, 4096-row chunk): the chunk's
+//                Householder dot products (v / (c0 - beta)) . x for every
+//                target column and y;
+//   k_qr_mid     one thread per candidate: dot products over the chunks in
+//                order, top-row update of the targets;
+//   k_qr_update  one workgroup per (candidate, chunk): x -= tau v t for rows
+//                > p, and the chunk's tail sums of squares of the targets;
+//   k_qr_end     one thread per candidate: tail sums over the chunks in
+//                order, LAWN-176 norm downdate.
+// Every sum is in the chunked order (rrm::chunk_sum), so a pass spreads over
+// as many workgroups as the candidate has chunks.  State between kernels
+// lives in global memory (QrState and the per-column arrays).
+constexpr int QR_THREADS = 256;
+static_assert(QR_THREADS == rrm::NORM_PARTIALS, "one thread per partial of the chunked order");
+constexpr int QR_G = 4;   // target columns per pass over a chunk's rows
 
-// One workgroup of CW = 16 waves per candidate; thread t takes the rows
-// i = t (mod 1024), R of them per tile of 1024 R rows, so every sum follows
-// the canonical order whatever R is.
-template <int MAXK, int R>
-__global__ __launch_bounds__(CP) void k_ols_fp64(Fp64Args a) {
-  constexpr int G = (R == 1) ? 8 : (R == 2) ? 4 : 2;   // columns per QR pass
-  __shared__ double normU[MAXK], normD[MAXK], tsq[MAXK], cc[MAXK];
-  __shared__ int phys[MAXK];
-  __shared__ int ovar[MAXK];   // per factor: its bare variable, or -1 (kept column)
-  __shared__ double redG[CW][G], redS[CW][2 * G];
-  __shared__ int abort_flag;
+enum : int32_t { QF_DONE = 1, QF_PASS = 2, QF_APPLY = 4, QF_NOUPD = 8 };
 
-  const int cand = a.order ? a.order[a.cand_base + (int)blockIdx.x] : a.cand_base + (int)blockIdx.x;
-  if (a.dead && a.dead[cand]) return;   // uniform: the whole workgroup leaves
-  bind_dataset(a, cand);
-  bind_xnorm(a, cand);
-  const int tid = (int)threadIdx.x;
-  const int pbeg = a.cand_start[cand];
-  const int nf = a.cand_start[cand + 1] - pbeg;
-  const int k = nf + 1;
-  const int64_t n = a.n, ldx = a.ldx;
-  // slab columns 0..k-1: A (factors, free term), k: Q^T y, k+1..k+nf: the
-  // factor values as evaluated (kept for the first QR step and the
-  // prediction of step 4)
-  double* const S = a.slab + a.slab_off[blockIdx.x];
-  const double* const X = a.X;
-  double* const co = a.coefs + (int64_t)pbeg + cand;
-  if (tid == 0) abort_flag = 0;
+struct QrArgs {
+  const int32_t* cand_start;
+  const int32_t* prog_start;
+  const uint32_t* code;
+  const DsDesc* ds;
+  const int32_t* cand_ds;
+  double* coefs;
+  rr_fit* fit;
+  const int32_t* dead;
+  int32_t* info;
+  const int32_t* order;      // launched position -> candidate
+  double* slab;
+  const int64_t* slab_off;   // per launched position
+  const int32_t* pos_item;   // per launched position: first chunk work item
+  int items_per_pos;         // > 0: every position has that many (one dataset)
+  int n_pos;                 // positions taking part in this launch
+  const double* norm_part;   // k_eval_fp64's chunk sums of squares
+  const int32_t* prog_item;  // per program: its first evaluation work item
+  QrState* st;
+  int32_t* phys;             // per position: ks entries
+  double* cold;              // normU, normD, tsq, cc, tdot: 5 blocks of (positions x ks)
+  int ks;
+  int64_t col_block;         // positions x ks of the whole launch chunk
+  double* part;              // per chunk work item: 2 ks doubles
+};
+
+struct Cand {   // a candidate's data as one thread or workgroup sees it (uniform)
+  int cand, pbeg, nf, k;
+  int64_t n, ldx;
+  const DsDesc* D;
+  double* S;
+};
+
+__device__ inline Cand cand_of(const QrArgs& a, int pos) {
+  Cand c;
+  c.cand = a.order[pos];
+  c.D = a.ds + (a.cand_ds ? a.cand_ds[c.cand] : 0);
+  c.pbeg = a.cand_start[c.cand];
+  c.nf = a.cand_start[c.cand + 1] - c.pbeg;
+  c.k = c.nf + 1;
+  c.n = c.D->n;
+  c.ldx = c.D->ldx;
+  c.S = a.slab + a.slab_off[pos];
+  return c;
+}
+
+// where the values of physical column col are before the first reflector:
+// the free term's ones, a bare variable's X column, or the kept column
+__device__ inline const double* origin_col(const QrArgs& a, const Cand& c, int col) {
+  if (col == c.nf) return c.D->one;
+  const int var = bare_variable(a.code + a.prog_start[c.pbeg + col],
+                                a.prog_start[c.pbeg + col + 1] - a.prog_start[c.pbeg + col]);
+  return var >= 0 ? c.D->X + (int64_t)var * c.ldx : c.S + (int64_t)(c.k + 1 + col) * c.ldx;
+}
+
+__device__ inline double* cold(const QrArgs& a, int which, int pos, int j) {
+  return a.cold + which * a.col_block + (int64_t)pos * a.ks + j;
+}
+#define NU(j) (*cold(a, 0, pos, (j)))
+#define ND(j) (*cold(a, 1, pos, (j)))
+#define TS(j) (*cold(a, 2, pos, (j)))
+#define CCS(j) (*cold(a, 3, pos, (j)))
+#define TD(j) (*cold(a, 4, pos, (j)))
+#define PHYS(j) (a.phys[(int64_t)pos * a.ks + (j)])
+
+__device__ inline int chunk_pos(const QrArgs& a, int item, int n_pos, int64_t* first) {
+  int pos;
+  if (a.items_per_pos > 0) {
+    pos = item / a.items_per_pos;
+    *first = (int64_t)pos * a.items_per_pos;
+  } else {
+    int lo = 0, hi = n_pos - 1;   // last position whose first item <= item (uniform)
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (a.pos_item[mid] <= item) lo = mid; else hi = mid - 1;
+    }
+    pos = lo;
+    *first = a.pos_item[pos];
+  }
+  return pos;
+}
+
+__device__ inline int64_t first_item(const QrArgs& a, int pos) {
+  return a.items_per_pos > 0 ? (int64_t)pos * a.items_per_pos : a.pos_item[pos];
+}
+
+// chunk sum of NV per-thread partials, in the chunked order; thread 0 gets them
+template <int NV>
+__device__ inline void chunk_sums(double (&v)[NV], double (*red)[NV]) {
+  const int wave = (int)(threadIdx.x / WAVE);
+#pragma unroll
+  for (int t = 0; t < NV; ++t) v[t] = wave_sum(v[t]);
+  if ((threadIdx.x & (WAVE - 1)) == 0) {
+#pragma unroll
+    for (int t = 0; t < NV; ++t) red[wave][t] = v[t];
+  }
   __syncthreads();
-
-  // ---- 1. norms of the factor columns and their sums over rows >= 1, in the
-  // chunked order (rrm::chunk_sum): k_eval_fp64 left one pair of sums per
-  // 4096-row chunk, added here in chunk order; a bare variable's come with
-  // the dataset.
-  // Factor 0 NaN at a row >= 1 (its rows >= 1 sum is NaN exactly then) fixes
-  // the outcome: its norm is NaN, so the rank threshold is NaN (no cut),
-  // factor 0 pivots first (nothing compares greater than NaN), its tail norm
-  // and hence its reflector are NaN, and with the threshold NaN no later
-  // column stops the factorisation: every coefficient is NaN, rank = min(n, k).
-  if (tid == 0) {
-    const int64_t nch = (n + rrm::NORM_CHUNK - 1) / rrm::NORM_CHUNK;
-    for (int j = 0; j < nf; ++j) {
-      const uint32_t* code = a.code + a.prog_start[pbeg + j];
-      const int len = a.prog_start[pbeg + j + 1] - a.prog_start[pbeg + j];
-      const int var = bare_variable(code, len);
-      ovar[j] = var;
-      double s0, s1;
-      if (var >= 0) {
-        s0 = a.xnorm[2 * var];
-        s1 = a.xnorm[2 * var + 1];
-      } else {
-        const double* pp = a.norm_part + 2 * (int64_t)a.prog_item[pbeg + j];
-        s0 = pp[0];
-        s1 = pp[1];
-        for (int64_t c = 1; c < nch; ++c) { s0 = s0 + pp[2 * c]; s1 = s1 + pp[2 * c + 1]; }
-      }
-      normD[j] = __builtin_sqrt(s0);
-      tsq[j] = s1;
-      if (j == 0 && s1 != s1) abort_flag = 1;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int t = 0; t < NV; ++t) {
+      double s = red[0][t];
+      for (int w = 1; w < QR_THREADS / WAVE; ++w) s = s + red[w][t];
+      v[t] = s;
     }
   }
-  __syncthreads();
-  if (abort_flag) {
-    const int size0 = (int)((n < (int64_t)k) ? n : (int64_t)k);
-    for (int i = tid; i < k; i += CP) co[i] = (i < size0) ? rrm::qnan() : 0.0;
-    if (tid == 0) { write_fit(a.fit + cand, rrm::qnan(), a.sst, n, size0, 0); a.info[cand] = -1; }
-    return;
+  __syncthreads();   // red is reused
+}
+
+// ---- initial state: norms from the evaluation's chunk sums, row 0 of the
+// working columns.  Factor 0 NaN at a row >= 1 (its rows >= 1 sum is NaN
+// exactly then) fixes the outcome: its norm is NaN, so the rank threshold is
+// NaN (no cut), factor 0 pivots first (nothing compares greater than NaN),
+// its tail norm and hence its reflector are NaN, and with the threshold NaN
+// no later column stops the factorisation: every coefficient is NaN, rank =
+// min(n, k).
+__global__ __launch_bounds__(256) void k_qr_init(QrArgs a) {
+  const int pos = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (pos >= a.n_pos) return;
+  QrState& s = a.st[pos];
+  s.flags = QF_DONE;
+  s.nan_exit = -2;   // nothing for k_qr_solve
+  const int cand0 = a.order[pos];
+  if (a.dead && a.dead[cand0]) return;
+  const Cand c = cand_of(a, pos);
+  const int64_t nch = (c.n + rrm::NORM_CHUNK - 1) / rrm::NORM_CHUNK;
+  double* const co = a.coefs + (int64_t)c.pbeg + c.cand;
+  for (int j = 0; j < c.nf; ++j) {
+    const int var = bare_variable(a.code + a.prog_start[c.pbeg + j],
+                                  a.prog_start[c.pbeg + j + 1] - a.prog_start[c.pbeg + j]);
+    double s0, s1;
+    if (var >= 0) {
+      s0 = c.D->xnorm[2 * var];
+      s1 = c.D->xnorm[2 * var + 1];
+    } else {
+      const double* pp = a.norm_part + 2 * (int64_t)a.prog_item[c.pbeg + j];
+      s0 = pp[0];
+      s1 = pp[1];
+      for (int64_t q = 1; q < nch; ++q) { s0 = s0 + pp[2 * q]; s1 = s1 + pp[2 * q + 1]; }
+    }
+    if (j == 0 && s1 != s1) {
+      const int size0 = (int)((c.n < (int64_t)c.k) ? c.n : (int64_t)c.k);
+      for (int i = 0; i < c.k; ++i) co[i] = (i < size0) ? rrm::qnan() : 0.0;
+      write_fit(a.fit + c.cand, rrm::qnan(), c.D->sst, c.n, size0, 0);
+      a.info[c.cand] = -1;
+      return;
+    }
+    ND(j) = __builtin_sqrt(s0);
+    TS(j) = s1;
   }
-  double* const cy = S + (int64_t)k * ldx;
-  // where the values of physical column c are before the first QR step
-  auto origin = [&](int c) -> const double* {
-    if (c == nf) return a.one;
-    const int v = __builtin_amdgcn_readfirstlane(ovar[c]);   // uniform: keep it scalar
-    return v >= 0 ? X + (int64_t)v * ldx : S + (int64_t)(k + 1 + c) * ldx;
-  };
+  ND(c.nf) = __builtin_sqrt((double)c.n);
+  TS(c.nf) = (double)(c.n - 1);
+  for (int j = 0; j < c.k; ++j) { PHYS(j) = j; NU(j) = ND(j); }
   // row 0 of every working column; rows >= 1 are written by the first
   // update pass (the single-row and top-row updates work on row p in place)
-  if (tid == 0) {
-    for (int c = 0; c < k; ++c) S[(int64_t)c * ldx] = origin(c)[0];
-    cy[0] = a.y[0];
-    normD[nf] = __builtin_sqrt((double)n);
-    tsq[nf] = (double)(n - 1);
-  }
-  for (int j = tid; j < k; j += CP) phys[j] = j;
-  __syncthreads();
-  for (int j = tid; j < k; j += CP) normU[j] = normD[j];
-  __syncthreads();
+  for (int col = 0; col < c.k; ++col) c.S[(int64_t)col * c.ldx] = origin_col(a, c, col)[0];
+  c.S[(int64_t)c.k * c.ldx] = c.D->y[0];
+  double mx = NU(0);
+  for (int j = 1; j < c.k; ++j) { const double v = NU(j); mx = (mx < v) ? v : mx; }
+  s.thr = ((mx * DBL_EPS) * (mx * DBL_EPS)) / (double)c.n;
+  s.k = c.k;
+  s.size = (int)((c.n < (int64_t)c.k) ? c.n : (int64_t)c.k);
+  s.p = 0;
+  s.nonzero = s.size;
+  s.nan_exit = -1;
+  s.steps = 0;
+  s.flags = 0;
+}
 
-  // ---- 2. column-pivoted Householder QR
-  const int size = (int)((n < (int64_t)k) ? n : (int64_t)k);
-  double mx = normU[0];
-  for (int j = 1; j < k; ++j) { const double v = normU[j]; mx = (mx < v) ? v : mx; }
-  const double thr = ((mx * DBL_EPS) * (mx * DBL_EPS)) / (double)n;
+__device__ inline void lawn_downdate(const QrArgs& a, int pos, const Cand& c, int p, bool recompute_empty) {
   const double dd_thr = __builtin_sqrt(DBL_EPS);
-  int nonzero = size;
-  int nan_exit = -1;   // step whose reflector had a NaN beta
-  int steps = 0;       // steps that made passes over the samples
-
-  for (int p = 0; p < size; ++p) {
-    int bi = p;
-    double best = normU[p];
-    for (int j = p + 1; j < k; ++j) { const double v = normU[j]; if (v > best) { best = v; bi = j; } }
-    if (best * best < thr * (double)(n - p)) { nonzero = p; break; }
-    __syncthreads();
-    if (tid == 0 && bi != p) {
-      int ti = phys[p]; phys[p] = phys[bi]; phys[bi] = ti;
-      double t;
-      t = normU[p]; normU[p] = normU[bi]; normU[bi] = t;
-      t = normD[p]; normD[p] = normD[bi]; normD[bi] = t;
-      t = tsq[p]; tsq[p] = tsq[bi]; tsq[bi] = t;
-    }
-    __syncthreads();
-
-    const int pc = __builtin_amdgcn_readfirstlane(phys[p]);   // uniform: keep it scalar
-    double* const v = S + (int64_t)pc * ldx;
-    const double* const vr = (p == 0) ? origin(pc) : v;   // rows > p of the pivot column
-    const double c0 = v[p];
-    const double tailSq = (n - p == 1) ? 0.0 : tsq[p];
-    double tau, beta;
-    const bool zero_tail = tailSq <= DBL_MIN_;
-    if (zero_tail) {
-      tau = 0.0;
-      beta = c0;
-    } else {
-      beta = __builtin_sqrt(c0 * c0 + tailSq);
-      if (c0 >= 0.0) beta = -beta;
-      tau = (beta - c0) / beta;
-    }
-    if (!zero_tail && beta != beta && n - p > 1) {
-      // A NaN reflector turns every remaining column and Q^T y into NaN, so
-      // R(p,p) = NaN and every coefficient up to the rank is NaN.  The norm
-      // downdate leaves columns with norm 0 at 0 and makes all others NaN;
-      // later steps pivot in place, and the first column at 0 stops the
-      // factorisation if the threshold is positive.
-      nan_exit = p;
-      int r = size;
-      if (thr > 0.0)
-        for (int j = p + 1; j < size; ++j) if (normU[j] == 0.0) { r = j; break; }
-      nonzero = r;
-      break;
-    }
-    const double den = c0 - beta;
-    const bool apply = tau != 0.0;         // applyHouseholderOnTheLeft does nothing for tau == 0
-    __syncthreads();   // every thread has read c0 before row p is overwritten
-    if (tid == 0) v[p] = beta;
-
-    const int nrem = k - 1 - p;            // remaining A columns: positions p+1..k-1
-    const int ntarget = nrem + 1;          // plus the y column (last target)
-    ++steps;
-    if (n - p == 1) {
-      // applyHouseholderOnTheLeft with a single row: scale by (1 - tau)
-      if (tid == 0) {
-        for (int t = 0; t < nrem; ++t) { double* col = S + (int64_t)phys[p + 1 + t] * ldx; col[p] *= (1.0 - tau); }
-        cy[p] *= (1.0 - tau);
-      }
-      __syncthreads();
-      for (int t = 0; t < nrem; ++t) if (tid == 0) tsq[p + 1 + t] = 0.0;
-      // the LAWN-176 recomputation below needs sums over rows p+1.. : empty
-      for (int t = 0; t < nrem; ++t) {
-        const int j = p + 1 + t;
-        if (normU[j] != 0.0) {
-          const double apj = S[(int64_t)phys[j] * ldx + p];
-          double temp = __builtin_fabs(apj) / normU[j];
-          temp = (1.0 + temp) * (1.0 - temp);
-          temp = temp < 0.0 ? 0.0 : temp;
-          const double ratio = normU[j] / normD[j];
-          const double temp2 = temp * (ratio * ratio);
-          __syncthreads();
-          if (tid == 0) {
-            if (temp2 <= dd_thr) { normD[j] = 0.0; normU[j] = 0.0; }
-            else normU[j] *= __builtin_sqrt(temp);
-          }
-          __syncthreads();
-        }
-      }
-      continue;
-    }
-
-    const int64_t t0 = tile_start(p + 1, R);
-    for (int g0 = 0; g0 < ntarget; g0 += G) {
-      const int gn = (ntarget - g0 < G) ? (ntarget - g0) : G;
-      double* cols[G];          // working columns (written)
-      const double* rcols[G];   // where rows > p are read from
-#pragma unroll
-      for (int t = 0; t < G; ++t) {
-        const int tg = g0 + t;
-        const int c = (t < gn && tg < nrem) ? __builtin_amdgcn_readfirstlane(phys[p + 1 + tg]) : -1;
-        cols[t] = c >= 0 ? S + (int64_t)c * ldx : cy;
-        rcols[t] = p > 0 ? cols[t] : (c >= 0 ? origin(c) : a.y);
-      }
-      double tdot[G];
-#pragma unroll
-      for (int t = 0; t < G; ++t) tdot[t] = 0.0;
-      if (apply) {
-        // Householder dot products  t = ess . a[p+1..] (+ a[p])
-        double acc[G];
-#pragma unroll
-        for (int t = 0; t < G; ++t) acc[t] = 0.0;
-        for (int64_t base = t0; base < n; base += CP * R) {
-          double e[R];
-          bool ok[R];
-#pragma unroll
-          for (int r = 0; r < R; ++r) {
-            const int64_t i = base + tid + (int64_t)CP * r;
-            ok[r] = i > p && i < n;
-            e[r] = vr[i] / den;   // the essential part of the reflector
-          }
-#pragma unroll
-          for (int t = 0; t < G; ++t) {
-            if (t < gn) {
-#pragma unroll
-              for (int r = 0; r < R; ++r) {
-                const double x = rcols[t][base + tid + (int64_t)CP * r];
-                if (ok[r]) acc[t] = acc[t] + e[r] * x;
-              }
-            }
-          }
-        }
-        block_sums<G>(acc, redG);
-#pragma unroll
-        for (int t = 0; t < G; ++t) {
-          if (t < gn) {
-            const double ap = cols[t][p];
-            tdot[t] = acc[t] + ap;
-          }
-        }
-        __syncthreads();
-        if (tid == 0) {
-#pragma unroll
-          for (int t = 0; t < G; ++t) if (t < gn) cols[t][p] = cols[t][p] - tau * tdot[t];
-        }
-      }
-      // apply the reflector and accumulate tail norms (rows >= p+1 and >= p+2);
-      // with only y left nothing reads its rows > p again
-      if (nrem == 0) { __syncthreads(); continue; }
-      double s12[2 * G];
-#pragma unroll
-      for (int t = 0; t < 2 * G; ++t) s12[t] = 0.0;
-      for (int64_t base = t0; base < n; base += CP * R) {
-        double te[R];
-        bool ok[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-          const int64_t i = base + tid + (int64_t)CP * r;
-          ok[r] = i > p && i < n;
-          te[r] = apply ? tau * (vr[i] / den) : 0.0;
-        }
-#pragma unroll
-        for (int t = 0; t < G; ++t) {
-          if (t < gn) {
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-              const int64_t i = base + tid + (int64_t)CP * r;
-              double x = rcols[t][i];
-              if (ok[r]) {
-                if (apply) x = x - te[r] * tdot[t];
-                if (apply || p == 0) cols[t][i] = x;
-                s12[t] = s12[t] + x * x;
-                if (i >= p + 2) s12[G + t] = s12[G + t] + x * x;
-              }
-            }
-          }
-        }
-      }
-      block_sums<2 * G>(s12, redS);
-      if (tid == 0) {
-#pragma unroll
-        for (int t = 0; t < G; ++t) {
-          const int tg = g0 + t;
-          if (t < gn && tg < nrem) { cc[p + 1 + tg] = s12[t]; tsq[p + 1 + tg] = s12[G + t]; }
-        }
-      }
-      __syncthreads();
-    }
-
-    // LAWN-176 norm downdate (cc[] holds the recomputed tail norms^2)
-    for (int j = p + 1; j < k; ++j) {
-      const double nu = normU[j];
-      if (nu != 0.0) {
-        const double apj = S[(int64_t)phys[j] * ldx + p];
-        double temp = __builtin_fabs(apj) / nu;
-        temp = (1.0 + temp) * (1.0 - temp);
-        temp = temp < 0.0 ? 0.0 : temp;
-        const double ratio = nu / normD[j];
-        const double temp2 = temp * (ratio * ratio);
-        const double rec = __builtin_sqrt(cc[j]);
-        __syncthreads();
-        if (tid == 0) {
-          if (temp2 <= dd_thr) { normD[j] = rec; normU[j] = rec; }
-          else normU[j] = nu * __builtin_sqrt(temp);
-        }
-      }
-    }
-    __syncthreads();
-  }
-
-  const int r = nonzero;
-  if (nan_exit >= 0) {
-    for (int i = tid; i < k; i += CP) co[phys[i]] = (i < r) ? rrm::qnan() : 0.0;
-    if (tid == 0) { write_fit(a.fit + cand, rrm::qnan(), a.sst, n, r, steps); a.info[cand] = -1; }
-    return;
-  }
-
-  // ---- 3. back substitution on the leading nonzero x nonzero block
-  for (int i = tid; i < r; i += CP) cc[i] = cy[i];
-  __syncthreads();
-  for (int i = r - 1; i >= 0; --i) {
-    const double* coli = S + (int64_t)phys[i] * ldx;
-    const double ci = cc[i] / coli[i];
-    __syncthreads();
-    for (int l = tid; l < i; l += CP) cc[l] = cc[l] - ci * coli[l];
-    if (tid == 0) cc[i] = ci;
-    __syncthreads();
-  }
-  bool any_nan = false;
-  for (int i = 0; i < k; ++i) {
-    const double cv = (i < r) ? cc[i] : 0.0;
-    if (cv != cv) any_nan = true;
-  }
-  for (int i = tid; i < k; i += CP) {
-    double cv = (i < r) ? cc[i] : 0.0;
-    if (cv != cv) cv = rrm::qnan();
-    co[phys[i]] = cv;
-  }
-  __syncthreads();
-  if (tid == 0) {
-    if (any_nan) {   // a kept NaN coefficient: every prediction is NaN
-      write_fit(a.fit + cand, rrm::qnan(), a.sst, n, r, steps);
-      a.info[cand] = -1;
-    } else {
-      a.info[cand] = r | (steps << 16);   // scored by k_score_fp64 / k_fit_fp64
+  for (int j = p + 1; j < c.k; ++j) {
+    const double nu = NU(j);
+    if (nu != 0.0) {
+      const double apj = c.S[(int64_t)PHYS(j) * c.ldx + p];
+      double temp = __builtin_fabs(apj) / nu;
+      temp = (1.0 + temp) * (1.0 - temp);
+      temp = temp < 0.0 ? 0.0 : temp;
+      const double ratio = nu / ND(j);
+      const double temp2 = temp * (ratio * ratio);
+      const double rec = recompute_empty ? 0.0 : __builtin_sqrt(CCS(j));
+      if (temp2 <= dd_thr) { ND(j) = rec; NU(j) = rec; }
+      else NU(j) = nu * __builtin_sqrt(temp);
     }
   }
 }
 
-// Factor evaluation for k_ols_fp64: one workgroup of EV_THREADS threads per
+__global__ __launch_bounds__(256) void k_qr_begin(QrArgs a) {
+  const int pos = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (pos >= a.n_pos) return;
+  QrState& s = a.st[pos];
+  if (s.flags & QF_DONE) return;
+  const int p = s.p;
+  if (p >= s.size) { s.flags = QF_DONE; return; }
+  const Cand c = cand_of(a, pos);
+  int bi = p;
+  double best = NU(p);
+  for (int j = p + 1; j < c.k; ++j) { const double v = NU(j); if (v > best) { best = v; bi = j; } }
+  if (best * best < s.thr * (double)(c.n - p)) { s.nonzero = p; s.flags = QF_DONE; return; }
+  if (bi != p) {
+    const int ti = PHYS(p); PHYS(p) = PHYS(bi); PHYS(bi) = ti;
+    double t;
+    t = NU(p); NU(p) = NU(bi); NU(bi) = t;
+    t = ND(p); ND(p) = ND(bi); ND(bi) = t;
+    t = TS(p); TS(p) = TS(bi); TS(bi) = t;
+  }
+  double* const v = c.S + (int64_t)PHYS(p) * c.ldx;
+  const double c0 = v[p];
+  const double tailSq = (c.n - p == 1) ? 0.0 : TS(p);
+  double tau, beta;
+  const bool zero_tail = tailSq <= DBL_MIN_;
+  if (zero_tail) {
+    tau = 0.0;
+    beta = c0;
+  } else {
+    beta = __builtin_sqrt(c0 * c0 + tailSq);
+    if (c0 >= 0.0) beta = -beta;
+    tau = (beta - c0) / beta;
+  }
+  if (!zero_tail && beta != beta && c.n - p > 1) {
+    // A NaN reflector turns every remaining column and Q^T y into NaN, so
+    // R(p,p) = NaN and every coefficient up to the rank is NaN.  The norm
+    // downdate leaves columns with norm 0 at 0 and makes all others NaN;
+    // later steps pivot in place, and the first column at 0 stops the
+    // factorisation if the threshold is positive.
+    s.nan_exit = p;
+    int r = s.size;
+    if (s.thr > 0.0)
+      for (int j = p + 1; j < s.size; ++j) if (NU(j) == 0.0) { r = j; break; }
+    s.nonzero = r;
+    s.flags = QF_DONE;
+    return;
+  }
+  const double den = c0 - beta;
+  v[p] = beta;
+  const int nrem = c.k - 1 - p;   // remaining A columns: positions p+1..k-1
+  s.steps = s.steps + 1;
+  if (c.n - p == 1) {
+    // applyHouseholderOnTheLeft with a single row: scale by (1 - tau)
+    for (int t = 0; t < nrem; ++t) c.S[(int64_t)PHYS(p + 1 + t) * c.ldx + p] *= (1.0 - tau);
+    c.S[(int64_t)c.k * c.ldx + p] *= (1.0 - tau);
+    for (int t = 0; t < nrem; ++t) TS(p + 1 + t) = 0.0;
+    lawn_downdate(a, pos, c, p, true);   // the recomputation's sums over rows p+1.. are empty
+    s.p = p + 1;
+    s.flags = 0;
+    return;
+  }
+  s.tau = tau;
+  s.den = den;
+  s.flags = QF_PASS | (tau != 0.0 ? QF_APPLY : 0) | (nrem == 0 ? QF_NOUPD : 0);
+}
+
+// target t (0..ntarget-1) of step p: physical column PHYS(p+1+t), or y last
+__device__ inline double* target_col(const QrArgs& a, int pos, const Cand& c, int p, int t, int nrem) {
+  return t < nrem ? c.S + (int64_t)PHYS(p + 1 + t) * c.ldx : c.S + (int64_t)c.k * c.ldx;
+}
+__device__ inline const double* target_src(const QrArgs& a, int pos, const Cand& c, int p, int t, int nrem) {
+  if (p > 0) return target_col(a, pos, c, p, t, nrem);
+  return t < nrem ? origin_col(a, c, PHYS(1 + t)) : c.D->y;
+}
+
+__global__ __launch_bounds__(QR_THREADS) void k_qr_dot(QrArgs a) {
+  __shared__ double red[QR_THREADS / WAVE][QR_G];
+  const int item = (int)blockIdx.x;
+  int64_t first;
+  const int pos = chunk_pos(a, item, a.n_pos, &first);
+  const QrState& s = a.st[pos];
+  if ((s.flags & (QF_PASS | QF_APPLY)) != (QF_PASS | QF_APPLY)) return;   // uniform
+  const Cand c = cand_of(a, pos);
+  const int p = s.p, nrem = c.k - 1 - p, ntarget = nrem + 1;
+  const double den = s.den;
+  const double* const vr = (p == 0) ? origin_col(a, c, PHYS(0)) : c.S + (int64_t)PHYS(p) * c.ldx;
+  const int64_t r0 = (item - first) * rrm::NORM_CHUNK;
+  const int64_t r1 = (r0 + rrm::NORM_CHUNK < c.n) ? r0 + rrm::NORM_CHUNK : c.n;
+  double* const out = a.part + (int64_t)item * 2 * a.ks;
+  for (int g0 = 0; g0 < ntarget; g0 += QR_G) {
+    const int gn = (ntarget - g0 < QR_G) ? ntarget - g0 : QR_G;
+    const double* src[QR_G];
+#pragma unroll
+    for (int t = 0; t < QR_G; ++t) src[t] = target_src(a, pos, c, p, g0 + (t < gn ? t : 0), nrem);
+    double acc[QR_G];
+#pragma unroll
+    for (int t = 0; t < QR_G; ++t) acc[t] = 0.0;
+    for (int64_t i = r0 + (int64_t)threadIdx.x; i < r1; i += QR_THREADS) {
+      if (i > p) {
+        const double e = vr[i] / den;   // the essential part of the reflector
+#pragma unroll
+        for (int t = 0; t < QR_G; ++t) if (t < gn) acc[t] = acc[t] + e * src[t][i];
+      }
+    }
+    chunk_sums<QR_G>(acc, red);
+    if (threadIdx.x == 0)
+      for (int t = 0; t < gn; ++t) out[g0 + t] = acc[t];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_qr_mid(QrArgs a) {
+  const int pos = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (pos >= a.n_pos) return;
+  QrState& s = a.st[pos];
+  if (!(s.flags & QF_PASS)) return;
+  const Cand c = cand_of(a, pos);
+  const int p = s.p, nrem = c.k - 1 - p, ntarget = nrem + 1;
+  if (s.flags & QF_APPLY) {
+    const int64_t first = first_item(a, pos), nch = (c.n + rrm::NORM_CHUNK - 1) / rrm::NORM_CHUNK;
+    for (int t = 0; t < ntarget; ++t) {
+      double sum = a.part[first * 2 * a.ks + t];
+      for (int64_t q = 1; q < nch; ++q) sum = sum + a.part[(first + q) * 2 * a.ks + t];
+      double* const col = target_col(a, pos, c, p, t, nrem);
+      const double td = sum + col[p];
+      TD(t) = td;
+      col[p] = col[p] - s.tau * td;
+    }
+  } else {
+    for (int t = 0; t < ntarget; ++t) TD(t) = 0.0;
+  }
+  if (s.flags & QF_NOUPD) {   // only y was left: rows below p of Q^T y are never read
+    s.p = p + 1;
+    s.flags = 0;
+  }
+}
+
+__global__ __launch_bounds__(QR_THREADS) void k_qr_update(QrArgs a) {
+  __shared__ double red[QR_THREADS / WAVE][2 * QR_G];
+  const int item = (int)blockIdx.x;
+  int64_t first;
+  const int pos = chunk_pos(a, item, a.n_pos, &first);
+  const QrState& s = a.st[pos];
+  if (!(s.flags & QF_PASS) || (s.flags & QF_NOUPD)) return;   // uniform
+  const Cand c = cand_of(a, pos);
+  const int p = s.p, nrem = c.k - 1 - p, ntarget = nrem + 1;
+  const bool apply = (s.flags & QF_APPLY) != 0;
+  const double den = s.den, tau = s.tau;
+  const double* const vr = (p == 0) ? origin_col(a, c, PHYS(0)) : c.S + (int64_t)PHYS(p) * c.ldx;
+  const int64_t r0 = (item - first) * rrm::NORM_CHUNK;
+  const int64_t r1 = (r0 + rrm::NORM_CHUNK < c.n) ? r0 + rrm::NORM_CHUNK : c.n;
+  double* const out = a.part + (int64_t)item * 2 * a.ks;
+  for (int g0 = 0; g0 < ntarget; g0 += QR_G) {
+    const int gn = (ntarget - g0 < QR_G) ? ntarget - g0 : QR_G;
+    const double* src[QR_G];
+    double* dst[QR_G];
+    double td[QR_G];
+#pragma unroll
+    for (int t = 0; t < QR_G; ++t) {
+      const int tg = g0 + (t < gn ? t : 0);
+      src[t] = target_src(a, pos, c, p, tg, nrem);
+      dst[t] = target_col(a, pos, c, p, tg, nrem);
+      td[t] = TD(tg);
+    }
+    double s12[2 * QR_G];   // rows >= p+1, rows >= p+2
+#pragma unroll
+    for (int t = 0; t < 2 * QR_G; ++t) s12[t] = 0.0;
+    for (int64_t i = r0 + (int64_t)threadIdx.x; i < r1; i += QR_THREADS) {
+      if (i > p) {
+        const double te = apply ? tau * (vr[i] / den) : 0.0;
+#pragma unroll
+        for (int t = 0; t < QR_G; ++t) {
+          if (t < gn) {
+            double x = src[t][i];
+            if (apply) x = x - te * td[t];
+            if (apply || p == 0) dst[t][i] = x;
+            s12[t] = s12[t] + x * x;
+            if (i >= p + 2) s12[QR_G + t] = s12[QR_G + t] + x * x;
+          }
+        }
+      }
+    }
+    chunk_sums<2 * QR_G>(s12, red);
+    if (threadIdx.x == 0)
+      for (int t = 0; t < gn; ++t)
+        if (g0 + t < nrem) { out[2 * (g0 + t)] = s12[t]; out[2 * (g0 + t) + 1] = s12[QR_G + t]; }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_qr_end(QrArgs a) {
+  const int pos = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (pos >= a.n_pos) return;
+  QrState& s = a.st[pos];
+  if (!(s.flags & QF_PASS)) return;
+  const Cand c = cand_of(a, pos);
+  const int p = s.p, nrem = c.k - 1 - p;
+  const int64_t first = first_item(a, pos), nch = (c.n + rrm::NORM_CHUNK - 1) / rrm::NORM_CHUNK;
+  for (int t = 0; t < nrem; ++t) {
+    double s1 = a.part[first * 2 * a.ks + 2 * t], s2 = a.part[first * 2 * a.ks + 2 * t + 1];
+    for (int64_t q = 1; q < nch; ++q) {
+      s1 = s1 + a.part[(first + q) * 2 * a.ks + 2 * t];
+      s2 = s2 + a.part[(first + q) * 2 * a.ks + 2 * t + 1];
+    }
+    CCS(p + 1 + t) = s1;
+    TS(p + 1 + t) = s2;
+  }
+  lawn_downdate(a, pos, c, p, false);
+  s.p = p + 1;
+  s.flags = 0;
+}
+
+// back substitution on the leading r x r block, coefficients through the
+// column permutation (zeros for the rest); the outcome for the scoring
+__global__ __launch_bounds__(256) void k_qr_solve(QrArgs a) {
+  const int pos = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (pos >= a.n_pos) return;
+  const QrState& s = a.st[pos];
+  if (s.nan_exit == -2) return;   // skipped, or decided by factor 0
+  const Cand c = cand_of(a, pos);
+  double* const co = a.coefs + (int64_t)c.pbeg + c.cand;
+  const int r = s.nonzero;
+  if (s.nan_exit >= 0) {
+    for (int i = 0; i < c.k; ++i) co[PHYS(i)] = (i < r) ? rrm::qnan() : 0.0;
+    write_fit(a.fit + c.cand, rrm::qnan(), c.D->sst, c.n, r, s.steps);
+    a.info[c.cand] = -1;
+    return;
+  }
+  const double* const cy = c.S + (int64_t)c.k * c.ldx;
+  for (int i = 0; i < r; ++i) TD(i) = cy[i];
+  for (int i = r - 1; i >= 0; --i) {
+    const double* coli = c.S + (int64_t)PHYS(i) * c.ldx;
+    const double ci = TD(i) / coli[i];
+    for (int l = 0; l < i; ++l) TD(l) = TD(l) - ci * coli[l];
+    TD(i) = ci;
+  }
+  bool any_nan = false;
+  for (int i = 0; i < c.k; ++i) {
+    double cv = (i < r) ? TD(i) : 0.0;
+    if (cv != cv) { any_nan = true; cv = rrm::qnan(); }
+    co[PHYS(i)] = cv;
+  }
+  if (any_nan) {   // a kept NaN coefficient: every prediction is NaN
+    write_fit(a.fit + c.cand, rrm::qnan(), c.D->sst, c.n, r, s.steps);
+    a.info[c.cand] = -1;
+  } else {
+    a.info[c.cand] = r | (s.steps << 16);   // scored by k_score_fp64 / k_fit_fp64
+  }
+}
+#undef NU
+#undef ND
+#undef TS
+#undef CCS
+#undef TD
+#undef PHYS
+
+// Factor evaluation for the QR: one workgroup of EV_THREADS threads per
 // (candidate, non-bare factor, chunk of rrm::NORM_CHUNK rows).  Work item b
 // is the b-th chunk in pair order; pair_item[p] is the first item of pair p.
 // Besides the values, each work item leaves its chunk's two sums of squares
@@ -597,7 +689,7 @@ __global__ __launch_bounds__(NAN0_ROWS) void k_nan0_fp64(Nan0Args a) {
   if (tid == 0) {
     a.dead[cand] = found;
     if (found) {
-      // the outcome k_ols_fp64 writes when it sees the same NaN
+      // the outcome k_qr_init writes when it sees the same NaN
       const int k = nf + 1;
       const int size0 = (int)((n < (int64_t)k) ? n : (int64_t)k);
       double* co = a.coefs + (int64_t)pbeg + cand;
@@ -758,28 +850,40 @@ __global__ __launch_bounds__(CP) void k_stats_fp64(const double* y, int64_t n, d
 
 }  // namespace
 
-hipError_t launch_ols_fp64(const OlsLaunch& L, hipStream_t s) {
-  Fp64Args a;
-  a.cand_start = L.cand_start; a.prog_start = L.prog_start; a.code = L.code; a.K = L.K;
-  a.ds = L.ds; a.cand_ds = L.cand_ds;
-  a.slab = L.slab; a.slab_off = L.slab_off; a.coefs = L.coefs; a.fit = L.fit; a.cand_base = L.cand_base;
-  a.order = L.order;
-  a.dead = L.dead;
-  a.norm_part = L.norm_part;
-  a.prog_item = L.prog_item;
-  a.info = L.info;
-  a.xnorm = nullptr;   // bound per candidate
-  const dim3 grid(L.n_cands), block(CP);
-  const int R = L.rows;
-  if (L.max_cols <= 16) {
-    if (R == 1) hipLaunchKernelGGL((k_ols_fp64<16, 1>), grid, block, 0, s, a);
-    else if (R == 2) hipLaunchKernelGGL((k_ols_fp64<16, 2>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((k_ols_fp64<16, 4>), grid, block, 0, s, a);
-  } else if (L.max_cols <= 64) {
-    hipLaunchKernelGGL((k_ols_fp64<64, 2>), grid, block, 0, s, a);
-  } else {
-    hipLaunchKernelGGL((k_ols_fp64<RR_MAX_COLS, 2>), grid, block, 0, s, a);
-  }
+static QrArgs qr_args(const QrLaunch& L, int n_pos) {
+  QrArgs a;
+  a.cand_start = L.cand_start; a.prog_start = L.prog_start; a.code = L.code; a.ds = L.ds; a.cand_ds = L.cand_ds;
+  a.coefs = L.coefs; a.fit = L.fit; a.dead = L.dead; a.info = L.info;
+  a.order = L.order; a.slab = L.slab; a.slab_off = L.slab_off;
+  a.pos_item = L.pos_item; a.items_per_pos = L.items_per_pos; a.n_pos = n_pos;
+  a.norm_part = L.norm_part; a.prog_item = L.prog_item;
+  a.st = L.st; a.phys = L.phys; a.cold = L.cold; a.ks = L.ks;
+  a.col_block = (int64_t)L.n_pos * L.ks;
+  a.part = L.part;
+  return a;
+}
+
+hipError_t launch_qr_init(const QrLaunch& L, hipStream_t s) {
+  if (L.n_pos <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_qr_init, dim3((unsigned)((L.n_pos + 255) / 256)), dim3(256), 0, s, qr_args(L, L.n_pos));
+  return hipGetLastError();
+}
+
+hipError_t launch_qr_step(const QrLaunch& L, int n_act, int n_items, hipStream_t s) {
+  if (n_act <= 0) return hipSuccess;
+  const QrArgs a = qr_args(L, n_act);
+  const dim3 pgrid((unsigned)((n_act + 255) / 256)), pblock(256), cgrid((unsigned)n_items), cblock(QR_THREADS);
+  hipLaunchKernelGGL(k_qr_begin, pgrid, pblock, 0, s, a);
+  hipLaunchKernelGGL(k_qr_dot, cgrid, cblock, 0, s, a);
+  hipLaunchKernelGGL(k_qr_mid, pgrid, pblock, 0, s, a);
+  hipLaunchKernelGGL(k_qr_update, cgrid, cblock, 0, s, a);
+  hipLaunchKernelGGL(k_qr_end, pgrid, pblock, 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_qr_solve(const QrLaunch& L, hipStream_t s) {
+  if (L.n_pos <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_qr_solve, dim3((unsigned)((L.n_pos + 255) / 256)), dim3(256), 0, s, qr_args(L, L.n_pos));
   return hipGetLastError();
 }
 
