@@ -463,7 +463,7 @@ def main():
                     help="checked: one RR_FP32_CHECKED call per step (FP64 for the candidates it needs, in the "
                          "same call); split: an RR_FP32_REFINED call, then an RR_FP64 call for the flagged ones")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--fp64-rows", type=int, default=None, help="samples per lane per tile, FP64 kernels (1, 2, 4)")
+    ap.add_argument("--fp64-eval-rows", type=int, default=None, help="samples per thread of the FP64 evaluation (1, 2, 4)")
     ap.add_argument("--fp32-rows", type=int, default=None, help="samples per lane per tile, FP32 kernels (1, 2)")
     ap.add_argument("--fp32-chunk", type=int, default=None, help="samples per FP32 workgroup (multiple of 512)")
     ap.add_argument("--fp32-streams", type=int, default=None, help="1: FP32 buckets on concurrent streams")
@@ -509,8 +509,8 @@ def main():
     X, y, target, trees, factors = workload()
     t_gen = time.perf_counter() - t0
     runner = rils_rols_cpp.BatchRunner(np.ascontiguousarray(X.T), y, 0)
-    if args.fp64_rows:
-        runner.set_option("fp64_rows", args.fp64_rows)
+    if args.fp64_eval_rows:
+        runner.set_option("fp64_eval_rows", args.fp64_eval_rows)
     if args.fp32_rows:
         runner.set_option("fp32_rows", args.fp32_rows)
     if args.fp32_chunk:
@@ -561,9 +561,9 @@ def main():
             "p50_step_ms": float(np.percentile(r["lat"], 50) * 1e3),
             "p99_step_ms": float(np.percentile(r["lat"], 99) * 1e3),
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                         "frac": achieved / PEAK_HBM_GBS, "traffic": pmc_traffic("k_ols_fp64"),
-                         "kernel": "k_nan0_fp64 + k_eval_fp64<%d> + k_ols_fp64<16,%d>" % (
-                             runner.get_option("fp64_eval_rows"), runner.get_option("fp64_rows")),
+                         "frac": achieved / PEAK_HBM_GBS, "traffic": pmc_traffic("fp64_step"),
+                         "kernel": "k_nan0_fp64 + k_eval_fp64<%d> + k_qr_* + k_score_fp64 + k_fit_fp64" %
+                                   runner.get_option("fp64_eval_rows"),
                          "launch_ms": float(r["kms"].mean()),
                          "algorithmic_bytes": byts},
             "valu_tflops_by_formula": flops / kernel_s / 1e12,
@@ -591,7 +591,7 @@ def main():
             "p50_step_ms": float(np.percentile(r["lat"], 50) * 1e3),
             "p99_step_ms": float(np.percentile(r["lat"], 99) * 1e3),
             "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                         "frac": achieved / PEAK_FP32_TFLOPS, "traffic": pmc_traffic("k_ols_fp32_nf1"),
+                         "frac": achieved / PEAK_FP32_TFLOPS, "traffic": pmc_traffic("fp32_nf1"),
                          "kernel": "k_acc_fp32<1,%d> + k_fin_fp32<1>" % runner.get_option("fp32_rows"), "launch_ms": t1,
                          "candidates": int(len(one)), "algorithmic_flops": f1},
             "batch_valu_tflops": flops / (r["kms"].mean() / 1e3) / 1e12,
@@ -659,7 +659,7 @@ def main():
             "over_cpu_one_core": h["evals_per_s"] / cpu1["value"] if cpu1 else None,
         },
         "forms": out,
-        "schedule": {"autotuned": tuned, "fp64_rows": runner.get_option("fp64_rows"),
+        "schedule": {"autotuned": tuned,
                      "fp64_eval_rows": runner.get_option("fp64_eval_rows"),
                      "fp32_rows": runner.get_option("fp32_rows"),
                      "fp32_chunk_rows": runner.get_option("fp32_chunk_rows"),

@@ -202,10 +202,9 @@ int rr_predict_fp32(rr_ctx* ctx, int32_t handle, const uint32_t* code, int32_t l
 
 /* ---- tuning ---------------------------------------------------------------
  * Named integer options: "slab_budget_mb" (device memory the FP64 OLS path
- * may use for its per-candidate work columns, default 32768), "fp64_rows"
- * (samples per thread per tile of the FP64 kernels: 1, 2 or 4; default 4),
- * "fp64_eval_rows" (samples per thread per pass of the FP64 factor
- * evaluation: 1, 2 or 4; default 2), "fp32_rows" (the same for the FP32 kernels: 1 or 2; default 2),
+ * may use for its per-candidate work columns, default 32768),
+ * "fp64_eval_rows" (samples per thread per pass of the FP64 evaluation of
+ * factors and trees: 1, 2 or 4; default 2), "fp32_rows" (the same for the FP32 kernels: 1 or 2; default 2),
  * "fp32_chunk_rows" (samples per workgroup of the FP32 pass, a multiple of
  * 512; default 4096: a candidate's samples are split into chunks that run as
  * independent workgroups, and the chunk sums are added in chunk order),
@@ -215,8 +214,8 @@ int rr_predict_fp32(rr_ctx* ctx, int32_t handle, const uint32_t* code, int32_t l
 int rr_set_option(rr_ctx* ctx, const char* name, int64_t value);
 
 /* Autotuner: time `batch` (mode, precision) under every schedule of that
- * precision -- FP64: samples per thread per tile 1, 2, 4 x samples per
- * thread of the factor evaluation 1, 2, 4; FP32: samples per
+ * precision -- FP64: samples per thread of the evaluation 1, 2, 4; FP32:
+ * samples per
  * lane 1, 2 x chunk rows 2048..16384 x bucket streams off/on -- taking the
  * best of `reps` runs each, and keep the fastest as the context's options
  * (read them back with rr_get_option).  FP64 results are the same bits under

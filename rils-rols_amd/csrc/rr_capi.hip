@@ -104,7 +104,6 @@ struct rr_ctx {
   DevBuf d_blk_cand, d_blk_chunk, d_chunk_base, d_work, d_work_flags;   // FP32 chunk map and partials
   HostBuf h_stage, h_out;
   int64_t slab_budget_mb = 32768;
-  int64_t fp64_rows = 4;
   int64_t fp64_eval_rows = 2;
   int64_t fp32_rows = 2;
   int64_t fp32_chunk_rows = 4096;
@@ -1000,17 +999,16 @@ int rr_autotune(rr_ctx* ctx, int32_t handle, const rr_batch* b, int32_t mode, in
   struct Cfg { int64_t a, b2, c; };
   std::vector<Cfg> cfgs;
   if (precision == RR_FP64) {
-    for (int64_t r : {1, 2, 4})
-      for (int64_t e : {1, 2, 4}) cfgs.push_back({r, e, 0});
+    for (int64_t e : {1, 2, 4}) cfgs.push_back({e, 0, 0});
   } else {
     for (int64_t r : {1, 2})
       for (int64_t ch : {2048, 4096, 8192, 16384})
         for (int64_t st : {0, 1}) cfgs.push_back({r, ch, st});
   }
-  const Cfg keep = precision == RR_FP64 ? Cfg{ctx->fp64_rows, ctx->fp64_eval_rows, 0}
+  const Cfg keep = precision == RR_FP64 ? Cfg{ctx->fp64_eval_rows, 0, 0}
                                          : Cfg{ctx->fp32_rows, ctx->fp32_chunk_rows, ctx->fp32_streams};
   auto apply = [&](const Cfg& c) {
-    if (precision == RR_FP64) { ctx->fp64_rows = c.a; ctx->fp64_eval_rows = c.b2; }
+    if (precision == RR_FP64) ctx->fp64_eval_rows = c.a;
     else { ctx->fp32_rows = c.a; ctx->fp32_chunk_rows = c.b2; ctx->fp32_streams = c.c; }
   };
   double best = 1e300;
@@ -1035,10 +1033,7 @@ int rr_set_option(rr_ctx* ctx, const char* name, int64_t value) {
   if (!ctx || !name) return fail(RR_E_ARG, "null argument");
   const std::string s(name);
   if (s == "slab_budget_mb") { if (value < 1) return fail(RR_E_ARG, "slab_budget_mb must be >= 1"); ctx->slab_budget_mb = value; }
-  else if (s == "fp64_rows") {
-    if (value != 1 && value != 2 && value != 4) return fail(RR_E_ARG, "fp64_rows must be 1, 2 or 4");
-    ctx->fp64_rows = value;
-  } else if (s == "fp64_eval_rows") {
+  else if (s == "fp64_eval_rows") {
     if (value != 1 && value != 2 && value != 4) return fail(RR_E_ARG, "fp64_eval_rows must be 1, 2 or 4");
     ctx->fp64_eval_rows = value;
   } else if (s == "fp32_rows") {
@@ -1058,7 +1053,6 @@ int rr_get_option(rr_ctx* ctx, const char* name, int64_t* value) {
   if (!ctx || !name || !value) return fail(RR_E_ARG, "null argument");
   const std::string s(name);
   if (s == "slab_budget_mb") *value = ctx->slab_budget_mb;
-  else if (s == "fp64_rows") *value = ctx->fp64_rows;
   else if (s == "fp64_eval_rows") *value = ctx->fp64_eval_rows;
   else if (s == "fp32_rows") *value = ctx->fp32_rows;
   else if (s == "fp32_chunk_rows") *value = ctx->fp32_chunk_rows;

@@ -77,7 +77,7 @@ struct ScoreLaunch {
   const int64_t* slab_off;   // OLS, per launched position
   const int32_t* order;      // launched position -> candidate (null: identity)
   const int32_t* dead;       // or null
-  const int32_t* info;       // OLS: written by k_ols_fp64
+  const int32_t* info;       // OLS: written by the QR (k_qr_init / k_qr_solve)
   const int32_t* pos_item;
   int n_pos;
   int n_items;
@@ -111,7 +111,7 @@ constexpr int NAN0_ROWS = 256;
 // EVAL_CHUNK_ROWS rows); pair p = (pair_pos[p], pair_fac[p]) owns work items
 // pair_item[p] .. pair_item[p+1]-1, n_items in all.  Work item b also leaves
 // the sums of squares of its chunk in norm_part[2b], norm_part[2b+1].  Must
-// run before the k_ols_fp64 launch over the same positions.
+// run before the QR launches over the same positions.
 struct EvalLaunch {
   const int32_t* cand_start;
   const int32_t* prog_start;

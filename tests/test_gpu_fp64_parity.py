@@ -162,23 +162,23 @@ def test_multi_dataset_batch_same_bits_as_solo(ctx, precision):
 
 def test_autotune_keeps_fp64_bits(ctx):
     """Whatever schedule the autotuner picks, FP64 results are the same bits
-    (the canonical order does not depend on rows per thread)."""
+    (the summation order does not depend on rows per thread)."""
     print("case: autotune fp64", flush=True)
     X, y = make_data(41, 20000, 4)
     trees = treegen.random_trees(41, 150, 3, 30, 4)
     od = O.Dataset(X, y)
     arrs = capi.build_batch([od.tune(t)["factors"] for t in trees])
     h = ctx.upload(X, y)
-    ctx.set_option("fp64_rows", 1)
+    ctx.set_option("fp64_eval_rows", 1)
     c1, f1 = ctx.fitness(h, arrs)
     ms = ctx.autotune(h, arrs, precision="fp64")
-    assert ms > 0 and ctx.get_option("fp64_rows") in (1, 2, 4)
+    assert ms > 0 and ctx.get_option("fp64_eval_rows") in (1, 2, 4)
     c2, f2 = ctx.fitness(h, arrs)
     assert all(same_bits(a, b) for a, b in zip(c1, c2))
     assert same_bits(f1["e0"], f2["e0"]) and same_bits(f1["e1"], f2["e1"])
     ms32 = ctx.autotune(h, arrs, precision="fp32")
     assert ms32 > 0 and ctx.get_option("fp32_chunk_rows") in (2048, 4096, 8192, 16384)
-    ctx.set_option("fp64_rows", 4)
+    ctx.set_option("fp64_eval_rows", 2)
     ctx.set_option("fp32_rows", 2)
     ctx.set_option("fp32_chunk_rows", 4096)
     ctx.set_option("fp32_streams", 0)
