@@ -209,7 +209,8 @@ int rr_predict_fp32(rr_ctx* ctx, int32_t handle, const uint32_t* code, int32_t l
  * 512; default 4096: a candidate's samples are split into chunks that run as
  * independent workgroups, and the chunk sums are added in chunk order),
  * "fp32_streams" (1: the FP32 factor-count buckets run on concurrent streams;
- * default 0, back to back on the context's stream).
+ * default 0, back to back on the context's stream), "host_threads" (host
+ * threads a call may use to check a large batch, 1..64; default 4).
  * Unknown names return RR_E_ARG. */
 int rr_set_option(rr_ctx* ctx, const char* name, int64_t value);
 

@@ -12,6 +12,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/rr_fitness.h"
@@ -81,6 +82,13 @@ struct Dataset {
 
 }  // namespace
 
+// An entry of the FP64 planner's table of unique factors (by program hash);
+// it belongs to the slab chunk being planned when gen matches.
+struct FacSlot {
+  uint64_t h;
+  int32_t u, gen;
+};
+
 struct rr_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -96,10 +104,11 @@ struct rr_ctx {
   hipEvent_t fork64 = nullptr, join64 = nullptr;
   DevBuf d_cand, d_prog, d_code, d_K, d_slab, d_order32, d_coefs, d_fit, d_misc, d_pred_e, d_dstab, d_cand_ds;
   DevBuf d_f64_off, d_f64_ord;   // FP64 slab offsets and launched position -> candidate
-  DevBuf d_pair_pos, d_pair_fac, d_pair_item;   // FP64 factor-evaluation work list
+  DevBuf d_fac_prog, d_fac_cand, d_fac_item;   // FP64 factor-evaluation work list
   DevBuf d_prog_item, d_norm_part;              // per program: first work item; per work item: chunk sums
   DevBuf d_info, d_pos_item, d_score_part;      // FP64 scoring: QR outcome per candidate, work list, chunk sums
   DevBuf d_qst, d_qphys, d_qcold, d_qpart;      // FP64 QR state between its kernels
+  DevBuf d_prog_kept, d_fac_kept, d_ustart, d_users, d_alive;   // shared kept columns of equal factors
   DevBuf d_dead;                 // per candidate: decided by the factor-0 pre-pass (OLS mode)
   DevBuf d_blk_cand, d_blk_chunk, d_chunk_base, d_work, d_work_flags;   // FP32 chunk map and partials
   HostBuf h_stage, h_out;
@@ -108,7 +117,10 @@ struct rr_ctx {
   int64_t fp32_rows = 2;
   int64_t fp32_chunk_rows = 4096;
   int64_t fp32_streams = 0;   // 1: bucket kernels on concurrent side streams
+  int64_t host_threads = 4;   // host threads for checking a large batch
   double last_ms = 0.0;
+  std::vector<FacSlot> fac_table;   // kept across calls: never cleared, only re-stamped
+  int32_t fac_gen = 0;
 };
 
 namespace {
@@ -206,10 +218,15 @@ const OpTables g_ops;
 
 // check_program's accept/reject decision without its messages; adds the
 // program's cost to *w.
-bool program_ok(const uint32_t* code, int32_t len, int32_t d, int32_t n_consts, int64_t* w) {
+// HASH: *h gets a hash of the program in which a constant operand counts by
+// its value (bits), not its pool index, so equal programs hash equal.
+template <bool HASH>
+bool program_ok(const uint32_t* code, int32_t len, int32_t d, int32_t n_consts, const double* K, int64_t* w,
+                uint64_t* h) {
   if (len <= 0) return false;
   int sp = 0, acc = 0, bad = 0;
   int64_t cw = 0;
+  uint64_t hh = 0x243F6A8885A308D3ULL;
   for (int32_t pc = 0; pc < len; ++pc) {
     const uint32_t ins = code[pc], op = ins & 0xffu, arg = ins >> 8;
     const int k = g_ops.cls[op];
@@ -219,10 +236,22 @@ bool program_ok(const uint32_t* code, int32_t len, int32_t d, int32_t n_consts, 
     sp += (k == OC_PUSH) - (k == OC_BIN_STACK);
     bad |= (sp < 0) | (sp > RR_MAX_STACK);
     bad |= ((k == OC_LDV) | (k == OC_BIN_VAR)) & ((int64_t)arg >= d);
-    bad |= ((k == OC_LDC) | (k == OC_BIN_CONST)) & ((int64_t)arg >= n_consts);
+    const bool is_const = (k == OC_LDC) | (k == OC_BIN_CONST);
+    bad |= is_const & ((int64_t)arg >= n_consts);
     acc = (k != OC_PUSH) & (acc | (k == OC_LDV) | (k == OC_LDC));
+    if (HASH) {   // no branch on the operand kind: a rotate-xor chain
+      uint64_t bits;
+      memcpy(&bits, K + (is_const && (int64_t)arg < n_consts ? arg : 0), 8);
+      const uint64_t v = is_const ? (op ^ (bits * 0x9E3779B97F4A7C15ULL)) : ins;
+      hh = ((hh << 23) | (hh >> 41)) ^ v;
+    }
   }
   *w += cw;
+  if (HASH) {   // splitmix64 finalizer: every output bit depends on every input bit
+    hh ^= hh >> 30; hh *= 0xbf58476d1ce4e5b9ULL;
+    hh ^= hh >> 27; hh *= 0x94d049bb133111ebULL;
+    *h = hh ^ (hh >> 31);
+  }
   return !bad && sp == 0 && acc;
 }
 
@@ -231,8 +260,11 @@ bool program_ok(const uint32_t* code, int32_t len, int32_t d, int32_t n_consts, 
 // prog_cost[p], when asked for, gets program p's per-sample evaluation work
 // (rough FP64 instruction units): used only to launch the long evaluations
 // first, so that one of them does not start last and end the launch alone;
-// it changes no result.
-int check_batch(const rr_batch* b, int32_t mode, int32_t d, const int32_t* cand_d, std::vector<int32_t>* prog_cost) {
+// it changes no result.  prog_hash[p], when asked for, gets the hash of
+// program_ok<true> (equal programs: equal hashes).  threads: host threads it
+// may use for a large batch.
+int check_batch(const rr_batch* b, int32_t mode, int32_t d, const int32_t* cand_d, std::vector<int32_t>* prog_cost,
+                std::vector<uint64_t>* prog_hash = nullptr, int threads = 1) {
   if (!b || b->n_cands < 0) return fail(RR_E_ARG, "bad batch");
   if (b->n_cands == 0) return RR_OK;
   if (!b->cand_start || !b->prog_start || (b->n_consts > 0 && !b->consts)) return fail(RR_E_ARG, "null batch array");
@@ -249,22 +281,66 @@ int check_batch(const rr_batch* b, int32_t mode, int32_t d, const int32_t* cand_
   for (int32_t p = 0; p < n_progs; ++p)
     if (b->prog_start[p + 1] < b->prog_start[p]) return fail(RR_E_ARG, "prog_start not monotone");
   if (prog_cost) prog_cost->assign(std::max(n_progs, 1), 0);
-  for (int32_t c = 0; c < b->n_cands; ++c) {
-    const int32_t dc = cand_d ? cand_d[c] : d;
-    for (int32_t p = b->cand_start[c]; p < b->cand_start[c + 1]; ++p) {
-      const int32_t len = b->prog_start[p + 1] - b->prog_start[p];
-      int64_t w = 0;
-      if (program_ok(b->code + b->prog_start[p], len, dc, b->n_consts, &w)) {
+  if (prog_hash) prog_hash->resize(std::max(n_progs, 1));
+  static const double zero = 0.0;   // constant reads of a batch without constants
+  const double* K = b->n_consts > 0 ? b->consts : &zero;
+  // candidates [c0, c1): the first invalid program, or -1
+  auto check_range = [&](int32_t c0, int32_t c1, int32_t* bad_prog, int32_t* bad_cand) {
+    for (int32_t c = c0; c < c1; ++c) {
+      const int32_t dc = cand_d ? cand_d[c] : d;
+      for (int32_t p = b->cand_start[c]; p < b->cand_start[c + 1]; ++p) {
+        const int32_t len = b->prog_start[p + 1] - b->prog_start[p];
+        int64_t w = 0;
+        const bool ok = prog_hash ? program_ok<true>(b->code + b->prog_start[p], len, dc, b->n_consts, K, &w,
+                                                     &(*prog_hash)[p])
+                                  : program_ok<false>(b->code + b->prog_start[p], len, dc, b->n_consts, K, &w, nullptr);
+        if (!ok) { *bad_prog = p; *bad_cand = c; return; }
         if (prog_cost) (*prog_cost)[p] = (int32_t)std::min<int64_t>(w, INT32_MAX);
-        continue;
       }
-      int rc = check_program(b->code + b->prog_start[p], len, dc, b->n_consts, nullptr);   // for the message
-      if (rc == RR_OK) rc = fail(RR_E_ARG, "invalid program");
-      char m[64];
-      snprintf(m, sizeof m, " (program %d)", p);
-      g_err += m;
-      return rc;
     }
+  };
+  // large batches: contiguous candidate ranges of about equal instruction
+  // count on host threads; the first invalid program (in batch order) is the
+  // one reported, whatever the split
+  const int64_t n_ins = b->prog_start[n_progs];
+  const int nt = (int)std::max<int64_t>(1, std::min<int64_t>(threads, n_ins / 16384));
+  std::vector<int32_t> bad_prog(nt, -1), bad_cand(nt, -1);
+  if (nt == 1) {
+    check_range(0, b->n_cands, &bad_prog[0], &bad_cand[0]);
+  } else {
+    std::vector<int32_t> cut(nt + 1, b->n_cands);
+    cut[0] = 0;
+    for (int t = 1; t < nt; ++t) {   // first candidate whose programs start at or after t/nt of the code
+      const int64_t target = n_ins * t / nt;
+      int32_t lo = cut[t - 1], hi = b->n_cands;
+      while (lo < hi) {
+        const int32_t mid = lo + (hi - lo) / 2;
+        if (b->prog_start[b->cand_start[mid]] < target) lo = mid + 1; else hi = mid;
+      }
+      cut[t] = lo;
+    }
+    std::vector<std::thread> th;
+    th.reserve(nt - 1);
+    for (int t = 1; t < nt; ++t) {
+      try {
+        th.emplace_back(check_range, cut[t], cut[t + 1], &bad_prog[t], &bad_cand[t]);
+      } catch (...) {   // no thread to be had: this range here
+        check_range(cut[t], cut[t + 1], &bad_prog[t], &bad_cand[t]);
+      }
+    }
+    check_range(cut[0], cut[1], &bad_prog[0], &bad_cand[0]);
+    for (auto& x : th) x.join();
+  }
+  for (int t = 0; t < nt; ++t) {
+    if (bad_prog[t] < 0) continue;
+    const int32_t p = bad_prog[t], c = bad_cand[t];
+    const int32_t len = b->prog_start[p + 1] - b->prog_start[p];
+    int rc = check_program(b->code + b->prog_start[p], len, cand_d ? cand_d[c] : d, b->n_consts, nullptr);   // the message
+    if (rc == RR_OK) rc = fail(RR_E_ARG, "invalid program");
+    char m[64];
+    snprintf(m, sizeof m, " (program %d)", p);
+    g_err += m;
+    return rc;
   }
   return RR_OK;
 }
@@ -289,7 +365,7 @@ const char* rr_last_error(void) { return g_err.c_str(); }
 int rr_check_batch(const rr_batch* b, int32_t mode, int32_t n_features) {
   if (mode != RR_MODE_OLS && mode != RR_MODE_RAW) return fail(RR_E_ARG, "bad mode");
   if (n_features < 0) return fail(RR_E_ARG, "bad feature count");
-  return check_batch(b, mode, n_features, nullptr, nullptr);
+  return check_batch(b, mode, n_features, nullptr, nullptr, nullptr, 4);
 }
 
 int rr_create(int device, rr_ctx** out) {
@@ -330,9 +406,10 @@ int rr_destroy(rr_ctx* ctx) {
   DevBuf* bufs[] = {&ctx->d_cand, &ctx->d_prog, &ctx->d_code, &ctx->d_K, &ctx->d_slab, &ctx->d_order32,
                     &ctx->d_coefs, &ctx->d_fit, &ctx->d_misc, &ctx->d_pred_e, &ctx->d_dstab, &ctx->d_cand_ds,
                     &ctx->d_blk_cand, &ctx->d_blk_chunk, &ctx->d_chunk_base, &ctx->d_work, &ctx->d_work_flags,
-                    &ctx->d_f64_off, &ctx->d_f64_ord, &ctx->d_dead, &ctx->d_pair_pos, &ctx->d_pair_fac,
-                    &ctx->d_pair_item, &ctx->d_prog_item, &ctx->d_norm_part, &ctx->d_info, &ctx->d_pos_item,
-                    &ctx->d_score_part, &ctx->d_qst, &ctx->d_qphys, &ctx->d_qcold, &ctx->d_qpart};
+                    &ctx->d_f64_off, &ctx->d_f64_ord, &ctx->d_dead, &ctx->d_fac_prog, &ctx->d_fac_cand,
+                    &ctx->d_fac_item, &ctx->d_prog_item, &ctx->d_norm_part, &ctx->d_info, &ctx->d_pos_item,
+                    &ctx->d_score_part, &ctx->d_qst, &ctx->d_qphys, &ctx->d_qcold, &ctx->d_qpart,
+                    &ctx->d_prog_kept, &ctx->d_fac_kept, &ctx->d_ustart, &ctx->d_users, &ctx->d_alive};
   for (DevBuf* b : bufs) b->release();
   ctx->h_stage.release();
   ctx->h_out.release();
@@ -460,13 +537,16 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
     }
   }
   std::vector<int32_t> pcost;   // per program: evaluation cost (filled by the check)
+  std::vector<uint64_t> phash;  // per program: hash by value (filled by the check, OLS mode)
   {
     std::vector<int32_t> cand_d;
     if (handles) {
       cand_d.resize(nc);
       for (int32_t c = 0; c < nc; ++c) cand_d[c] = ctx->ds[uniq[cand_ds[c]]].d;
     }
-    if ((rc = check_batch(b, mode, min_d, handles ? cand_d.data() : nullptr, &pcost))) return rc;
+    if ((rc = check_batch(b, mode, min_d, handles ? cand_d.data() : nullptr, &pcost,
+                          mode == RR_MODE_OLS ? &phash : nullptr, (int)ctx->host_threads)))
+      return rc;
   }
   if (!fit_out) return fail(RR_E_ARG, "null fit_out");
   if (mode == RR_MODE_OLS && !coefs_out) return fail(RR_E_ARG, "null coefs_out");
@@ -550,7 +630,9 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
   const size_t s_chunks = 2 * align_up((size_t)total_chunks * 4) + align_up(chunk_base.size() * 4);
   // FP64 slab offsets, order and the three arrays of the evaluation work list
   const size_t s_f64 = align_up((size_t)nc * 8) + align_up((size_t)nc * 4) + 4 * align_up((size_t)(np + 1) * 4) +
-                       align_up((size_t)(nc + 1) * 4);   // + the scoring work list
+                       align_up((size_t)(nc + 1) * 4) +       // the scoring work list
+                       2 * align_up((size_t)(np + 1) * 8) +   // kept-column offsets per program and per unique factor
+                       2 * align_up((size_t)(np + 1) * 4);    // the users of each unique factor
   if (ctx->h_stage.ensure(s_cand + s_prog + s_code + s_K + s_off + s_tab + s_cds + s_chunks + 2 * s_f64) != RR_OK)
     return RR_E_NOMEM;
   size_t so = 0;
@@ -621,8 +703,8 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
   // FP64 OLS launches over a candidate list (null: every candidate), longest
   // first, chunked so that each launch's slab fits the budget.  plan64 does
   // the host work and the uploads, launch64 only launches.
-  // positions c0..c1-1 share one slab; their factor-evaluation pairs are
-  // p0..p0+npair-1 of the work list, with nitem work items
+  // positions c0..c1-1 share one slab; their unique factors are
+  // p0..p0+npair-1 of the evaluation work list, with nitem work items
   // sitem chunk work items of the QR and scoring (sper each, or 0), and per QR
   // step s the positions taking part (step_pos[s]: those with more than s
   // columns, a prefix) and their chunk work items (step_items[s])
@@ -664,81 +746,194 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
       lpt.swap(tmp);
     }
     if (m == 0) return RR_OK;
+    // Slab: per position its k+1 working columns (A, Q^T y), then the kept
+    // columns of the factors it is the first in its chunk to use.  Factors
+    // that are the same program over the same dataset (same opcodes and
+    // operands, constants compared by value) share one kept column and its
+    // norms: one evaluation.
     std::vector<int64_t> off(m);
+    std::vector<int64_t> prog_kept(std::max(np, 1), 0);
+    std::vector<int32_t> ufac(std::max(np, 1), -1);   // per non-bare program: its unique factor
+    std::vector<int32_t> u_prog, u_cand;              // per unique factor: a program and candidate using it
+    std::vector<int64_t> u_kept;                      // per unique factor: its kept column
+    std::vector<int32_t> use_prog, use_cand;          // every non-bare program in the list, with its candidate
+    auto is_const_op = [](uint32_t op) {
+      return op == RR_OP_LDC ||
+             (op >= RR_OP_BIN && (((op - RR_OP_BIN) & 7u) == RR_M_C || ((op - RR_OP_BIN) & 7u) == RR_M_CR));
+    };
+    auto dsi_of = [&](int32_t c) { return handles ? cand_ds[c] : 0; };
+    auto same_prog = [&](int32_t p1, int32_t p2) {
+      const int32_t l1 = b->prog_start[p1 + 1] - b->prog_start[p1], l2 = b->prog_start[p2 + 1] - b->prog_start[p2];
+      if (l1 != l2) return false;
+      for (int32_t i = 0; i < l1; ++i) {
+        const uint32_t a1 = b->code[b->prog_start[p1] + i], a2 = b->code[b->prog_start[p2] + i];
+        if ((a1 & 0xffu) != (a2 & 0xffu)) return false;
+        if (is_const_op(a1 & 0xffu)) {
+          if (memcmp(&b->consts[a1 >> 8], &b->consts[a2 >> 8], 8) != 0) return false;
+        } else if (a1 != a2) {
+          return false;
+        }
+      }
+      return true;
+    };
+    auto is_bare = [&](int32_t p) {
+      return b->prog_start[p + 1] - b->prog_start[p] == 1 && (b->code[b->prog_start[p]] & 0xffu) == RR_OP_LDV;
+    };
     const int64_t budget = std::max<int64_t>(ctx->slab_budget_mb, 1) << 20;
     int64_t max_words = 0;
+    // per chunk, the unique factors by hash: open addressing, linear probing;
+    // an entry belongs to the current chunk when its gen matches, so the
+    // table is not cleared between chunks or calls, and it holds at least
+    // twice the programs of the list
+    int64_t nprog = 0;
+    for (int32_t q = 0; q < m; ++q) nprog += b->cand_start[lpt[q] + 1] - b->cand_start[lpt[q]];
+    size_t tsize = 64;
+    while (tsize < (size_t)(2 * nprog)) tsize <<= 1;
+    std::vector<FacSlot>& table = ctx->fac_table;
+    if (table.size() < tsize || ctx->fac_gen > INT32_MAX - m - 1) {
+      table.assign(std::max(tsize, table.size()), FacSlot{0, -1, -1});
+      ctx->fac_gen = 0;
+    }
+    tsize = table.size();
+    std::vector<uint64_t> ph(std::max(np, 1));
+    std::vector<int32_t> pfound(std::max(np, 1));   // phase 1: unique factor found in the chunk, or -1
+    std::vector<int32_t> pearlier(std::max(np, 1)); // phase 1: same factor earlier in the candidate, or -1
+    int32_t gen = ctx->fac_gen;
+    auto lookup = [&](int32_t p, int32_t dsi) {
+      for (size_t t = ph[p] & (tsize - 1);; t = (t + 1) & (tsize - 1)) {
+        const FacSlot& e = table[t];
+        if (e.gen != gen) return -1;
+        if (e.h == ph[p] && dsi_of(u_cand[e.u]) == dsi && same_prog(u_prog[e.u], p)) return e.u;
+      }
+    };
+    auto insert = [&](int32_t p, int32_t u) {
+      size_t t = ph[p] & (tsize - 1);
+      while (table[t].gen == gen) t = (t + 1) & (tsize - 1);
+      table[t] = FacSlot{ph[p], u, gen};
+    };
     for (int q = 0; q < m;) {
       int64_t used = 0;
       int q1 = q, maxk = 0;
+      const int u0 = (int)u_prog.size();
+      ++gen;
       while (q1 < m) {
         const int32_t c = lpt[q1];
-        const int k = b->cand_start[c + 1] - b->cand_start[c] + 1;
-        const int64_t need = (int64_t)(2 * k) * ds_of(c).ldx;   // A, Q^T y, kept factor values
+        const int32_t pb = b->cand_start[c], pe = b->cand_start[c + 1];
+        const int k = pe - pb + 1;
+        const int64_t ldx = ds_of(c).ldx;
+        const int32_t dsi = dsi_of(c);
+        // the kept columns it adds: factors new to the chunk, once each
+        int64_t fresh = 0;
+        for (int32_t p = pb; p < pe; ++p) {
+          if (is_bare(p)) continue;
+          ph[p] = phash[p] ^ ((uint64_t)(uint32_t)dsi * 0x9E3779B97F4A7C15ULL);
+          pfound[p] = lookup(p, dsi);
+          pearlier[p] = -1;
+          if (pfound[p] >= 0) continue;
+          for (int32_t p2 = pb; p2 < p && pearlier[p] < 0; ++p2)
+            if (!is_bare(p2) && pfound[p2] < 0 && pearlier[p2] < 0 && ph[p2] == ph[p] && same_prog(p2, p)) pearlier[p] = p2;
+          if (pearlier[p] < 0) ++fresh;
+        }
+        const int64_t need = ((int64_t)(k + 1) + fresh) * ldx;
         if (q1 > q && (used + need) * 8 > budget) break;
         off[q1] = used;
-        used += need;
+        used += (int64_t)(k + 1) * ldx;
+        for (int32_t p = pb; p < pe; ++p) {
+          if (is_bare(p)) continue;
+          int32_t u = pfound[p] >= 0 ? pfound[p] : pearlier[p] >= 0 ? ufac[pearlier[p]] : -1;
+          if (u < 0) {
+            u = (int32_t)u_prog.size();
+            u_prog.push_back(p);
+            u_cand.push_back(c);
+            u_kept.push_back(used);
+            used += ldx;
+            insert(p, u);
+          }
+          ufac[p] = u;
+          prog_kept[p] = u_kept[u];
+          use_prog.push_back(p);
+          use_cand.push_back(c);
+        }
         maxk = std::max(maxk, k);
         ++q1;
       }
-      chunks.push_back({q, q1, maxk, 0, 0, 0, 0, 0, 0, {}, {}});
+      Chunk64 ch{};
+      ch.c0 = q; ch.c1 = q1; ch.maxk = maxk; ch.p0 = u0; ch.npair = (int)u_prog.size() - u0;
+      chunks.push_back(ch);
       max_words = std::max(max_words, used);
       q = q1;
     }
+    ctx->fac_gen = gen;
     if (ctx->d_slab.ensure((size_t)max_words * 8) != RR_OK) return RR_E_NOMEM;
-    // factor-evaluation work list: per chunk, every (position, factor that is
-    // not a bare variable), each with ceil(n / EVAL_CHUNK_ROWS) work items
-    std::vector<int32_t> ppos, pfac, pitem, prog_item(std::max(np, 1), 0);
+    // factor-evaluation work list: per chunk its unique factors, longest
+    // first (counting sort on the evaluation cost), each with
+    // ceil(n / EVAL_CHUNK_ROWS) work items
+    const int32_t nu = (int32_t)u_prog.size();
+    std::vector<int32_t> pprog(std::max(nu, 1)), pcand(std::max(nu, 1)), pitem(std::max(nu, 1)),
+        prog_item(std::max(np, 1), 0), unew(std::max(nu, 1));
+    std::vector<int64_t> pkept(std::max(nu, 1));
     int64_t max_items = 0;
     for (Chunk64& ch : chunks) {
-      ch.p0 = (int)ppos.size();
-      // the chunk's programs that are not bare variables, longest first
-      // (counting sort on the evaluation cost, ties in position order)
       constexpr int NB = 4096;
-      std::vector<int32_t> cnt(NB + 1, 0), progs, qs;
-      auto key = [&](int32_t p) { return NB - 1 - std::min(pcost[p], NB - 1); };
-      for (int q = ch.c0; q < ch.c1; ++q) {
-        const int32_t c = lpt[q];
-        for (int32_t p = b->cand_start[c]; p < b->cand_start[c + 1]; ++p) {
-          const int32_t len = b->prog_start[p + 1] - b->prog_start[p];
-          if (len == 1 && (b->code[b->prog_start[p]] & 0xffu) == RR_OP_LDV) continue;   // read from X
-          progs.push_back(p);
-          qs.push_back(q);
-          ++cnt[key(p)];
-        }
-      }
+      std::vector<int32_t> cnt(NB + 1, 0);
+      auto key = [&](int32_t u) { return NB - 1 - std::min(pcost[u_prog[u]], NB - 1); };
+      for (int u = ch.p0; u < ch.p0 + ch.npair; ++u) ++cnt[key(u)];
       for (int i = 0, a = 0; i <= NB; ++i) { const int32_t t = cnt[i]; cnt[i] = a; a += t; }
-      std::vector<int32_t> sorted(progs.size());
-      for (size_t i = 0; i < progs.size(); ++i) sorted[cnt[key(progs[i])]++] = (int32_t)i;
-      int64_t items = 0, per = -1;   // per: the common item count of the pairs, or 0
-      for (int32_t i : sorted) {
-        const int32_t p = progs[i], q = qs[i], c = lpt[q];
-        const int64_t nch = (ds_of(c).n + rr::EVAL_CHUNK_ROWS - 1) / rr::EVAL_CHUNK_ROWS;
-        ppos.push_back(q);
-        pfac.push_back(p - b->cand_start[c]);
-        pitem.push_back((int32_t)items);
-        prog_item[p] = (int32_t)items;
+      std::vector<int32_t> sorted(ch.npair);
+      for (int u = ch.p0; u < ch.p0 + ch.npair; ++u) sorted[cnt[key(u)]++] = u;
+      int64_t items = 0, per = -1;   // per: the common item count of the factors, or 0
+      for (int i = 0; i < ch.npair; ++i) {
+        const int32_t u = sorted[i], w = ch.p0 + i;
+        const int64_t nch = (ds_of(u_cand[u]).n + rr::EVAL_CHUNK_ROWS - 1) / rr::EVAL_CHUNK_ROWS;
+        pprog[w] = u_prog[u];
+        pcand[w] = u_cand[u];
+        pkept[w] = u_kept[u];
+        pitem[w] = (int32_t)items;
+        unew[u] = w;
         items += nch;
         per = (per < 0 || per == nch) ? nch : 0;
       }
       if (items > INT32_MAX) return fail(RR_E_LIMIT, "too many factor-evaluation work items in one launch");
-      ch.npair = (int)ppos.size() - ch.p0;
       ch.nitem = (int)items;
       max_items = std::max(max_items, items);
       ch.per = per > 0 ? (int)per : 0;
     }
+    for (int32_t p = 0; p < np; ++p)
+      if (ufac[p] >= 0) prog_item[p] = pitem[unew[ufac[p]]];
+    // per unique factor (work-list order) the candidates using it, for the
+    // liveness check after the factor-0 pre-pass
+    const int32_t nuse = (int32_t)use_prog.size();
+    std::vector<int32_t> ustart(nu + 1, 0), users(std::max(nuse, 1));
+    for (int32_t i = 0; i < nuse; ++i) ++ustart[unew[ufac[use_prog[i]]] + 1];
+    for (int32_t u = 0; u < nu; ++u) ustart[u + 1] += ustart[u];
+    {
+      std::vector<int32_t> fill(ustart.begin(), ustart.end() - 1);
+      for (int32_t i = 0; i < nuse; ++i) users[fill[unew[ufac[use_prog[i]]]]++] = use_cand[i];
+    }
     const size_t base = so64 + (size_t)slot * s_f64;
-    const size_t sp = align_up((size_t)(np + 1) * 4);
+    const size_t sp = align_up((size_t)(np + 1) * 4), sp8 = align_up((size_t)(np + 1) * 8);
     size_t o = base;
     if ((rc = upload(ctx, ctx->d_f64_off, off.data(), (size_t)m * 8, o))) return rc;
     o += align_up((size_t)nc * 8);
     if ((rc = upload(ctx, ctx->d_f64_ord, lpt.data(), (size_t)m * 4, o))) return rc;
     o += align_up((size_t)nc * 4);
-    if (!ppos.empty()) {
-      if ((rc = upload(ctx, ctx->d_pair_pos, ppos.data(), ppos.size() * 4, o))) return rc;
-      if ((rc = upload(ctx, ctx->d_pair_fac, pfac.data(), pfac.size() * 4, o + sp))) return rc;
-      if ((rc = upload(ctx, ctx->d_pair_item, pitem.data(), pitem.size() * 4, o + 2 * sp))) return rc;
+    if (nu > 0) {
+      if ((rc = upload(ctx, ctx->d_fac_prog, pprog.data(), (size_t)nu * 4, o))) return rc;
+      if ((rc = upload(ctx, ctx->d_fac_cand, pcand.data(), (size_t)nu * 4, o + sp))) return rc;
+      if ((rc = upload(ctx, ctx->d_fac_item, pitem.data(), (size_t)nu * 4, o + 2 * sp))) return rc;
     }
     if ((rc = upload(ctx, ctx->d_prog_item, prog_item.data(), prog_item.size() * 4, o + 3 * sp))) return rc;
+    {
+      size_t o2 = o + 4 * sp + align_up((size_t)(nc + 1) * 4);   // after pos_item
+      if ((rc = upload(ctx, ctx->d_prog_kept, prog_kept.data(), prog_kept.size() * 8, o2))) return rc;
+      o2 += sp8;
+      if ((rc = upload(ctx, ctx->d_fac_kept, pkept.data(), pkept.size() * 8, o2))) return rc;
+      o2 += sp8;
+      if ((rc = upload(ctx, ctx->d_ustart, ustart.data(), ustart.size() * 4, o2))) return rc;
+      o2 += sp;
+      if ((rc = upload(ctx, ctx->d_users, users.data(), users.size() * 4, o2))) return rc;
+      if (ctx->d_alive.ensure((size_t)std::max(nu, 1) * 4) != RR_OK) return RR_E_NOMEM;
+    }
     if (ctx->d_norm_part.ensure((size_t)std::max<int64_t>(max_items, 1) * 16) != RR_OK) return RR_E_NOMEM;
     // scoring work list: per position, ceil(n / EVAL_CHUNK_ROWS) chunks, numbered per slab chunk
     std::vector<int32_t> pos_item(m + 1);
@@ -783,16 +978,21 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
       RR_HIP(hipEventRecord(ctx->fork64, ctx->stream));
       RR_HIP(hipStreamWaitEvent(stream, ctx->fork64, 0));
     }
+    // which unique factors a live candidate uses (after the factor-0 pre-pass)
+    const int32_t nu = chunks.back().p0 + chunks.back().npair;
+    RR_HIP(rr::launch_alive_fp64((const int32_t*)ctx->d_ustart.p, (const int32_t*)ctx->d_users.p, ddead, nu,
+                                 (int32_t*)ctx->d_alive.p, stream));
     for (const Chunk64& ch : chunks) {
       if (ch.npair > 0) {
         rr::EvalLaunch E;
-        E.cand_start = dc; E.prog_start = dp; E.code = dcode; E.K = dK; E.ds = dtab; E.cand_ds = dcds;
-        E.slab = (double*)ctx->d_slab.p; E.slab_off = (const int64_t*)ctx->d_f64_off.p;
-        E.order = (const int32_t*)ctx->d_f64_ord.p; E.dead = ddead;
-        E.pair_pos = (const int32_t*)ctx->d_pair_pos.p + ch.p0;
-        E.pair_fac = (const int32_t*)ctx->d_pair_fac.p + ch.p0;
-        E.pair_item = (const int32_t*)ctx->d_pair_item.p + ch.p0;
-        E.n_pairs = ch.npair; E.n_items = ch.nitem; E.items_per_pair = ch.per;
+        E.prog_start = dp; E.code = dcode; E.K = dK; E.ds = dtab; E.cand_ds = dcds;
+        E.slab = (double*)ctx->d_slab.p;
+        E.fac_prog = (const int32_t*)ctx->d_fac_prog.p + ch.p0;
+        E.fac_cand = (const int32_t*)ctx->d_fac_cand.p + ch.p0;
+        E.fac_item = (const int32_t*)ctx->d_fac_item.p + ch.p0;
+        E.fac_kept = (const int64_t*)ctx->d_fac_kept.p + ch.p0;
+        E.alive = (const int32_t*)ctx->d_alive.p + ch.p0;
+        E.n_facs = ch.npair; E.n_items = ch.nitem; E.items_per_fac = ch.per;
         E.norm_part = (double*)ctx->d_norm_part.p; E.rows = (int)ctx->fp64_eval_rows;
         RR_HIP(rr::launch_eval_fp64(E, stream));
       }
@@ -803,6 +1003,7 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
       Q.slab_off = (const int64_t*)ctx->d_f64_off.p + ch.c0;
       Q.pos_item = (const int32_t*)ctx->d_pos_item.p + ch.c0; Q.items_per_pos = ch.sper; Q.n_pos = ch.c1 - ch.c0;
       Q.norm_part = (const double*)ctx->d_norm_part.p; Q.prog_item = (const int32_t*)ctx->d_prog_item.p;
+      Q.prog_kept = (const int64_t*)ctx->d_prog_kept.p;
       Q.st = (rr::QrState*)ctx->d_qst.p; Q.phys = (int32_t*)ctx->d_qphys.p; Q.cold = (double*)ctx->d_qcold.p;
       Q.ks = ch.maxk; Q.part = (double*)ctx->d_qpart.p;
       RR_HIP(rr::launch_qr_init(Q, stream));
@@ -810,8 +1011,7 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
       RR_HIP(rr::launch_qr_solve(Q, stream));
       rr::ScoreLaunch Sc{};
       Sc.cand_start = dc; Sc.prog_start = dp; Sc.code = dcode; Sc.K = dK; Sc.ds = dtab; Sc.cand_ds = dcds;
-      Sc.coefs = dcoef; Sc.slab = (const double*)ctx->d_slab.p;
-      Sc.slab_off = (const int64_t*)ctx->d_f64_off.p + ch.c0;
+      Sc.coefs = dcoef; Sc.slab = (const double*)ctx->d_slab.p; Sc.prog_kept = (const int64_t*)ctx->d_prog_kept.p;
       Sc.order = (const int32_t*)ctx->d_f64_ord.p + ch.c0; Sc.dead = ddead; Sc.info = (const int32_t*)ctx->d_info.p;
       Sc.pos_item = (const int32_t*)ctx->d_pos_item.p + ch.c0; Sc.n_pos = ch.c1 - ch.c0; Sc.n_items = ch.sitem;
       Sc.items_per_pos = ch.sper; Sc.raw = 0; Sc.rows = (int)ctx->fp64_eval_rows;
@@ -861,7 +1061,7 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
     if ((rc = upload(ctx, ctx->d_pos_item, pos_item.data(), pos_item.size() * 4, so64))) return rc;
     if (ctx->d_score_part.ensure((size_t)std::max<int64_t>(it, 1) * 8) != RR_OK) return RR_E_NOMEM;
     raw64.cand_start = dc; raw64.prog_start = dp; raw64.code = dcode; raw64.K = dK; raw64.ds = dtab;
-    raw64.cand_ds = dcds; raw64.order = nullptr; raw64.dead = nullptr; raw64.info = nullptr;
+    raw64.cand_ds = dcds; raw64.prog_kept = nullptr; raw64.order = nullptr; raw64.dead = nullptr; raw64.info = nullptr;
     raw64.pos_item = (const int32_t*)ctx->d_pos_item.p; raw64.n_pos = nc; raw64.n_items = (int)it;
     raw64.items_per_pos = per > 0 ? (int)per : 0; raw64.raw = 1; raw64.rows = (int)ctx->fp64_eval_rows;
     raw64.part = (double*)ctx->d_score_part.p; raw64.fit = dfit;
@@ -1042,6 +1242,9 @@ int rr_set_option(rr_ctx* ctx, const char* name, int64_t value) {
   } else if (s == "fp32_streams") {
     if (value != 0 && value != 1) return fail(RR_E_ARG, "fp32_streams must be 0 or 1");
     ctx->fp32_streams = value;
+  } else if (s == "host_threads") {
+    if (value < 1 || value > 64) return fail(RR_E_ARG, "host_threads must be in [1, 64]");
+    ctx->host_threads = value;
   } else if (s == "fp32_chunk_rows") {
     if (value < 512 || value > (int64_t)1 << 30 || value % 512) return fail(RR_E_ARG, "fp32_chunk_rows must be a multiple of 512 in [512, 2^30]");
     ctx->fp32_chunk_rows = value;
@@ -1057,6 +1260,7 @@ int rr_get_option(rr_ctx* ctx, const char* name, int64_t* value) {
   else if (s == "fp32_rows") *value = ctx->fp32_rows;
   else if (s == "fp32_chunk_rows") *value = ctx->fp32_chunk_rows;
   else if (s == "fp32_streams") *value = ctx->fp32_streams;
+  else if (s == "host_threads") *value = ctx->host_threads;
   else return fail(RR_E_ARG, "unknown option " + s);
   return RR_OK;
 }

@@ -50,7 +50,8 @@ struct QrLaunch {
   int items_per_pos;          // > 0 when every position has that many chunk items
   int n_pos;
   const double* norm_part;    // the evaluation's chunk sums (EvalLaunch::norm_part)
-  const int32_t* prog_item;   // per program: its first evaluation work item
+  const int32_t* prog_item;   // per non-bare program: the first evaluation work item of its factor
+  const int64_t* prog_kept;   // per non-bare program: slab offset of its factor's kept column
   QrState* st;
   int32_t* phys;
   double* cold;
@@ -61,8 +62,9 @@ struct QrLaunch {
 // Scoring of the FP64 form: one workgroup per (launched position, chunk of
 // EVAL_CHUNK_ROWS rows) computes the chunk's sum of squared residuals --
 // RAW: of the candidate's one program; OLS: of the OLS tree from the
-// coefficients and the factor columns (kept slab columns or X), for the
-// candidates whose info is >= 0 -- and then one thread per position adds the
+// coefficients and the factor columns (kept slab columns, at slab +
+// prog_kept[program], or X), for the candidates whose info is >= 0 -- and
+// then one thread per position adds the
 // chunk sums in order and writes the fitness.  Position p owns work items
 // pos_item[p] .. pos_item[p+1]-1 (items_per_pos > 0: p * items_per_pos ..).
 struct ScoreLaunch {
@@ -74,7 +76,7 @@ struct ScoreLaunch {
   const int32_t* cand_ds;
   const double* coefs;       // OLS
   const double* slab;        // OLS
-  const int64_t* slab_off;   // OLS, per launched position
+  const int64_t* prog_kept;  // OLS, per non-bare program
   const int32_t* order;      // launched position -> candidate (null: identity)
   const int32_t* dead;       // or null
   const int32_t* info;       // OLS: written by the QR (k_qr_init / k_qr_solve)
@@ -107,28 +109,29 @@ struct Nan0Launch {
 constexpr int NAN0_ROWS = 256;
 
 // Factor evaluation of the FP64 OLS form into the kept slab columns: one
-// workgroup per (launched position, non-bare factor, chunk of
-// EVAL_CHUNK_ROWS rows); pair p = (pair_pos[p], pair_fac[p]) owns work items
-// pair_item[p] .. pair_item[p+1]-1, n_items in all.  Work item b also leaves
-// the sums of squares of its chunk in norm_part[2b], norm_part[2b+1].  Must
-// run before the QR launches over the same positions.
+// workgroup per (unique factor, chunk of EVAL_CHUNK_ROWS rows).  A unique
+// factor is a program over a dataset that one or more candidates use;
+// factor f is program fac_prog[f] over the dataset of candidate fac_cand[f],
+// kept at slab + fac_kept[f], and owns work items fac_item[f] ..
+// fac_item[f+1]-1, n_items in all.  Work item b also leaves the sums of
+// squares of its chunk in norm_part[2b], norm_part[2b+1].  Factors with
+// alive[f] == 0 (every user candidate settled by the factor-0 pre-pass) are
+// skipped.  Must run before the QR launches over the same candidates.
 struct EvalLaunch {
-  const int32_t* cand_start;
   const int32_t* prog_start;
   const uint32_t* code;
   const double* K;
   const DsDesc* ds;
   const int32_t* cand_ds;
   double* slab;
-  const int64_t* slab_off;   // per launched position
-  const int32_t* order;      // launched position -> candidate
-  const int32_t* dead;
-  const int32_t* pair_pos;
-  const int32_t* pair_fac;
-  const int32_t* pair_item;
-  int n_pairs;
+  const int32_t* fac_prog;
+  const int32_t* fac_cand;
+  const int32_t* fac_item;
+  const int64_t* fac_kept;
+  const int32_t* alive;
+  int n_facs;
   int n_items;
-  int items_per_pair;        // > 0 when every pair has that many items (pair = item / items_per_pair)
+  int items_per_fac;         // > 0 when every factor has that many items (factor = item / items_per_fac)
   double* norm_part;         // 2 doubles per work item
   int rows;                  // rows per thread per pass: 1, 2 or 4
 };
@@ -138,6 +141,10 @@ hipError_t launch_qr_init(const QrLaunch& L, hipStream_t s);
 hipError_t launch_qr_step(const QrLaunch& L, int n_act, int n_items, hipStream_t s);
 hipError_t launch_qr_solve(const QrLaunch& L, hipStream_t s);
 hipError_t launch_eval_fp64(const EvalLaunch& L, hipStream_t s);
+// alive[u] = 1 if some candidate users[ustart[u] .. ustart[u+1]-1] has
+// dead == 0, else 0, for u < n
+hipError_t launch_alive_fp64(const int32_t* ustart, const int32_t* users, const int32_t* dead, int32_t n,
+                             int32_t* alive, hipStream_t s);
 // out[2f], out[2f+1]: sums of squares of feature f's column over rows >= 0
 // and rows >= 1 in the chunked order (rrm::chunk_sum)
 hipError_t launch_xnorm_fp64(const double* X, int64_t n, int64_t ldx, int32_t d, double* out, hipStream_t s);

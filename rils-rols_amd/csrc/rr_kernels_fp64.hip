@@ -116,7 +116,8 @@ struct QrArgs {
   int items_per_pos;         // > 0: every position has that many (one dataset)
   int n_pos;                 // positions taking part in this launch
   const double* norm_part;   // k_eval_fp64's chunk sums of squares
-  const int32_t* prog_item;  // per program: its first evaluation work item
+  const int32_t* prog_item;  // per non-bare program: its factor's first evaluation work item
+  const int64_t* prog_kept;  // per non-bare program: slab offset of its factor's kept column
   QrState* st;
   int32_t* phys;             // per position: ks entries
   double* cold;              // normU, normD, tsq, cc, tdot: 5 blocks of (positions x ks)
@@ -151,7 +152,7 @@ __device__ inline const double* origin_col(const QrArgs& a, const Cand& c, int c
   if (col == c.nf) return c.D->one;
   const int var = bare_variable(a.code + a.prog_start[c.pbeg + col],
                                 a.prog_start[c.pbeg + col + 1] - a.prog_start[c.pbeg + col]);
-  return var >= 0 ? c.D->X + (int64_t)var * c.ldx : c.S + (int64_t)(c.k + 1 + col) * c.ldx;
+  return var >= 0 ? c.D->X + (int64_t)var * c.ldx : a.slab + a.prog_kept[c.pbeg + col];
 }
 
 __device__ inline double* cold(const QrArgs& a, int which, int pos, int j) {
@@ -359,6 +360,85 @@ __device__ inline const double* target_src(const QrArgs& a, int pos, const Cand&
   return t < nrem ? origin_col(a, c, PHYS(1 + t)) : c.D->y;
 }
 
+// Row passes over one chunk [r0, r1) for GN target columns, QR_R rows per
+// thread per step: every load of a step is issued before its arithmetic.
+// Thread t still takes rows r0 + t (mod QR_THREADS) in increasing order (the
+// partials of rrm::chunk_sum); rows at or beyond r1 load row r0 instead and
+// are not used.  The columns are device memory: global address space.
+constexpr int QR_R = 4;
+typedef const __attribute__((address_space(1))) double* gcptr;
+typedef __attribute__((address_space(1))) double* gptr;
+
+// acc[t] += e_i * src[t][i] over rows i > p, e_i = vr[i] / den
+template <int GN>
+__device__ inline void dot_rows(const double* vr_, const double* const* src_, double den, int p, int64_t r0, int64_t r1,
+                                double* acc) {
+  const gcptr vr = (gcptr)vr_;
+  gcptr src[GN];
+#pragma unroll
+  for (int t = 0; t < GN; ++t) src[t] = (gcptr)src_[t];
+  for (int64_t base = r0; base < r1; base += (int64_t)QR_THREADS * QR_R) {
+    double v[QR_R], x[QR_R][GN];
+#pragma unroll
+    for (int r = 0; r < QR_R; ++r) {
+      const int64_t i = base + (int64_t)threadIdx.x + (int64_t)QR_THREADS * r;
+      const int64_t ii = i < r1 ? i : r0;
+      v[r] = vr[ii];
+#pragma unroll
+      for (int t = 0; t < GN; ++t) x[r][t] = src[t][ii];
+    }
+#pragma unroll
+    for (int r = 0; r < QR_R; ++r) {
+      const int64_t i = base + (int64_t)threadIdx.x + (int64_t)QR_THREADS * r;
+      if (i < r1 && i > p) {
+        const double e = v[r] / den;   // the essential part of the reflector
+#pragma unroll
+        for (int t = 0; t < GN; ++t) acc[t] = acc[t] + e * x[r][t];
+      }
+    }
+  }
+}
+
+// rows i > p: x = src[t][i] (- tau * (vr[i] / den) * td[t] when applying),
+// stored to dst[t][i] when applying or at p == 0; s12[t] += x^2, and
+// s12[QR_G + t] += x^2 for rows >= p + 2
+template <int GN>
+__device__ inline void update_rows(const double* vr_, const double* const* src_, double* const* dst_, const double* td,
+                                   double den, double tau, bool apply, int p, int64_t r0, int64_t r1, double* s12) {
+  const gcptr vr = (gcptr)vr_;
+  gcptr src[GN];
+  gptr dst[GN];
+#pragma unroll
+  for (int t = 0; t < GN; ++t) { src[t] = (gcptr)src_[t]; dst[t] = (gptr)dst_[t]; }
+  const bool store = apply || p == 0;
+  for (int64_t base = r0; base < r1; base += (int64_t)QR_THREADS * QR_R) {
+    double v[QR_R], x[QR_R][GN];
+#pragma unroll
+    for (int r = 0; r < QR_R; ++r) {
+      const int64_t i = base + (int64_t)threadIdx.x + (int64_t)QR_THREADS * r;
+      const int64_t ii = i < r1 ? i : r0;
+      v[r] = apply ? vr[ii] : 0.0;
+#pragma unroll
+      for (int t = 0; t < GN; ++t) x[r][t] = src[t][ii];
+    }
+#pragma unroll
+    for (int r = 0; r < QR_R; ++r) {
+      const int64_t i = base + (int64_t)threadIdx.x + (int64_t)QR_THREADS * r;
+      if (i < r1 && i > p) {
+        const double te = apply ? tau * (v[r] / den) : 0.0;
+#pragma unroll
+        for (int t = 0; t < GN; ++t) {
+          double y = x[r][t];
+          if (apply) y = y - te * td[t];
+          if (store) dst[t][i] = y;
+          s12[t] = s12[t] + y * y;
+          if (i >= p + 2) s12[QR_G + t] = s12[QR_G + t] + y * y;
+        }
+      }
+    }
+  }
+}
+
 __global__ __launch_bounds__(QR_THREADS) void k_qr_dot(QrArgs a) {
   __shared__ double red[QR_THREADS / WAVE][QR_G];
   const int item = (int)blockIdx.x;
@@ -381,12 +461,11 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr_dot(QrArgs a) {
     double acc[QR_G];
 #pragma unroll
     for (int t = 0; t < QR_G; ++t) acc[t] = 0.0;
-    for (int64_t i = r0 + (int64_t)threadIdx.x; i < r1; i += QR_THREADS) {
-      if (i > p) {
-        const double e = vr[i] / den;   // the essential part of the reflector
-#pragma unroll
-        for (int t = 0; t < QR_G; ++t) if (t < gn) acc[t] = acc[t] + e * src[t][i];
-      }
+    switch (gn) {   // uniform
+      case 1: dot_rows<1>(vr, src, den, p, r0, r1, acc); break;
+      case 2: dot_rows<2>(vr, src, den, p, r0, r1, acc); break;
+      case 3: dot_rows<3>(vr, src, den, p, r0, r1, acc); break;
+      default: dot_rows<QR_G>(vr, src, den, p, r0, r1, acc); break;
     }
     chunk_sums<QR_G>(acc, red);
     if (threadIdx.x == 0)
@@ -450,20 +529,11 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr_update(QrArgs a) {
     double s12[2 * QR_G];   // rows >= p+1, rows >= p+2
 #pragma unroll
     for (int t = 0; t < 2 * QR_G; ++t) s12[t] = 0.0;
-    for (int64_t i = r0 + (int64_t)threadIdx.x; i < r1; i += QR_THREADS) {
-      if (i > p) {
-        const double te = apply ? tau * (vr[i] / den) : 0.0;
-#pragma unroll
-        for (int t = 0; t < QR_G; ++t) {
-          if (t < gn) {
-            double x = src[t][i];
-            if (apply) x = x - te * td[t];
-            if (apply || p == 0) dst[t][i] = x;
-            s12[t] = s12[t] + x * x;
-            if (i >= p + 2) s12[QR_G + t] = s12[QR_G + t] + x * x;
-          }
-        }
-      }
+    switch (gn) {   // uniform
+      case 1: update_rows<1>(vr, src, dst, td, den, tau, apply, p, r0, r1, s12); break;
+      case 2: update_rows<2>(vr, src, dst, td, den, tau, apply, p, r0, r1, s12); break;
+      case 3: update_rows<3>(vr, src, dst, td, den, tau, apply, p, r0, r1, s12); break;
+      default: update_rows<QR_G>(vr, src, dst, td, den, tau, apply, p, r0, r1, s12); break;
     }
     chunk_sums<2 * QR_G>(s12, red);
     if (threadIdx.x == 0)
@@ -539,26 +609,26 @@ __global__ __launch_bounds__(256) void k_qr_solve(QrArgs a) {
 #undef PHYS
 
 // Factor evaluation for the QR: one workgroup of EV_THREADS threads per
-// (candidate, non-bare factor, chunk of rrm::NORM_CHUNK rows).  Work item b
-// is the b-th chunk in pair order; pair_item[p] is the first item of pair p.
-// Besides the values, each work item leaves its chunk's two sums of squares
-// (rows >= 0 and rows >= 1) in the chunked order for the QR's norms.
+// (unique factor, chunk of rrm::NORM_CHUNK rows).  A unique factor is one
+// program over one dataset, shared by every candidate of the launch that
+// uses it.  Work item b is the b-th chunk in factor order; fac_item[f] is the
+// first item of factor f.  Besides the values, each work item leaves its
+// chunk's two sums of squares (rows >= 0 and rows >= 1) in the chunked order
+// for the QR's norms.
 struct EvalArgs {
-  const int32_t* cand_start;
   const int32_t* prog_start;
   const uint32_t* code;
   const double* K;
   const DsDesc* ds;
   const int32_t* cand_ds;
   double* slab;
-  const int64_t* slab_off;   // per launched position
-  const int32_t* order;      // launched position -> candidate
-  const int32_t* dead;
-  const int32_t* pair_pos;   // per pair: launched position
-  const int32_t* pair_fac;   // per pair: factor index
-  const int32_t* pair_item;  // per pair: first work item
-  int n_pairs;
-  int items_per_pair;        // > 0: every pair has this many work items (one dataset)
+  const int32_t* fac_prog;   // per factor: its program
+  const int32_t* fac_cand;   // per factor: a candidate using it (for the dataset)
+  const int32_t* fac_item;   // per factor: first work item
+  const int64_t* fac_kept;   // per factor: slab offset of its kept column
+  const int32_t* alive;      // per factor: 0 when no live candidate uses it
+  int n_facs;
+  int items_per_fac;         // > 0: every factor has this many work items (one dataset)
   double* norm_part;         // per work item: its chunk's sums of squares (rows >= 0, rows >= 1)
 };
 constexpr int EV_THREADS = 256;
@@ -570,26 +640,23 @@ __global__ __launch_bounds__(EV_THREADS) void k_eval_fp64(EvalArgs a) {
   __shared__ double red[EV_THREADS / WAVE][2];
   const int item = (int)blockIdx.x;
   int lo;
-  if (a.items_per_pair > 0) {
-    lo = item / a.items_per_pair;
-  } else {   // last pair whose first item <= item (uniform)
+  if (a.items_per_fac > 0) {
+    lo = item / a.items_per_fac;
+  } else {   // last factor whose first item <= item (uniform)
     lo = 0;
-    int hi = a.n_pairs - 1;
+    int hi = a.n_facs - 1;
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
-      if (a.pair_item[mid] <= item) lo = mid; else hi = mid - 1;
+      if (a.fac_item[mid] <= item) lo = mid; else hi = mid - 1;
     }
   }
-  const int pos = a.pair_pos[lo], j = a.pair_fac[lo];
-  const int cand = a.order[pos];
-  if (a.dead && a.dead[cand]) return;
+  if (!a.alive[lo]) return;
+  const int cand = a.fac_cand[lo], prog = a.fac_prog[lo];
   const DsDesc& D = a.ds[a.cand_ds ? a.cand_ds[cand] : 0];
-  const int pbeg = a.cand_start[cand];
-  const int k = a.cand_start[cand + 1] - pbeg + 1;
-  const uint32_t* code = a.code + a.prog_start[pbeg + j];
-  const int len = a.prog_start[pbeg + j + 1] - a.prog_start[pbeg + j];
-  double* const keep = a.slab + a.slab_off[pos] + (int64_t)(k + 1 + j) * D.ldx;
-  const int64_t r0 = (int64_t)(item - a.pair_item[lo]) * rrm::NORM_CHUNK;
+  const uint32_t* code = a.code + a.prog_start[prog];
+  const int len = a.prog_start[prog + 1] - a.prog_start[prog];
+  double* const keep = a.slab + a.fac_kept[lo];
+  const int64_t r0 = (int64_t)(item - a.fac_item[lo]) * rrm::NORM_CHUNK;
   const int64_t r1 = (r0 + rrm::NORM_CHUNK < D.n) ? r0 + rrm::NORM_CHUNK : D.n;
   // thread t adds the squares of rows r0 + t (mod 256) in increasing order:
   // the chunk's partials of rrm::chunk_sum.  base is a multiple of
@@ -621,6 +688,16 @@ __global__ __launch_bounds__(EV_THREADS) void k_eval_fp64(EvalArgs a) {
     a.norm_part[2 * (int64_t)item] = t0;
     a.norm_part[2 * (int64_t)item + 1] = t1;
   }
+}
+
+// One thread per unique factor: whether a candidate still to be scored uses it.
+__global__ __launch_bounds__(256) void k_alive_fp64(const int32_t* ustart, const int32_t* users, const int32_t* dead,
+                                                    int32_t n, int32_t* alive) {
+  const int u = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (u >= n) return;
+  int32_t any = 0;
+  for (int32_t i = ustart[u]; i < ustart[u + 1] && !any; ++i) any = dead[users[i]] == 0;
+  alive[u] = any;
 }
 
 // Per feature of a dataset, the sums of squares of its column over rows >= 0
@@ -711,7 +788,7 @@ struct ScoreArgs {
   const int32_t* cand_ds;
   const double* coefs;
   const double* slab;
-  const int64_t* slab_off;   // per launched position
+  const int64_t* prog_kept;  // per non-bare program: slab offset of its factor's kept column
   const int32_t* order;      // launched position -> candidate, or null (identity)
   const int32_t* dead;
   const int32_t* info;
@@ -738,6 +815,11 @@ __device__ inline int score_pos(const ScoreArgs& a, int item) {
 template <bool RAW, int R>
 __global__ __launch_bounds__(EV_THREADS) void k_score_fp64(ScoreArgs a) {
   __shared__ double red[EV_THREADS / WAVE];
+  // OLS: per factor its column and coefficient; kind 0 skipped (|c| < 1e-12),
+  // 1 the column itself (|c - 1| < 1e-12), 2 c times the column
+  __shared__ const double* fcol[RAW ? 1 : RR_MAX_COLS];
+  __shared__ double fcoef[RAW ? 1 : RR_MAX_COLS];
+  __shared__ int8_t fkind[RAW ? 1 : RR_MAX_COLS];
   const int item = (int)blockIdx.x;
   const int pos = score_pos(a, item);
   const int cand = a.order ? a.order[pos] : pos;
@@ -765,29 +847,59 @@ __global__ __launch_bounds__(EV_THREADS) void k_score_fp64(ScoreArgs a) {
     }
   } else {
     const int nf = a.cand_start[cand + 1] - pbeg;
-    const int k = nf + 1;
     const double* co = a.coefs + (int64_t)pbeg + cand;
-    const double* S = a.slab + a.slab_off[pos];
     const double cf_free = co[nf];
-    for (int64_t i = r0 + (int64_t)threadIdx.x; i < r1; i += EV_THREADS) {
-      double yp = 0.0;
-      bool any = false;
+    for (int j = (int)threadIdx.x; j < nf; j += EV_THREADS) {
+      const double cj = co[j];
+      const uint32_t* code = a.code + a.prog_start[pbeg + j];
+      const int var = bare_variable(code, a.prog_start[pbeg + j + 1] - a.prog_start[pbeg + j]);
+      fcol[j] = var >= 0 ? D.X + (int64_t)var * ldx : a.slab + a.prog_kept[pbeg + j];
+      fcoef[j] = cj;
+      fkind[j] = (__builtin_fabs(cj) < EPS12) ? 0 : (__builtin_fabs(cj - 1) < EPS12) ? 1 : 2;
+    }
+    __syncthreads();
+    const gcptr yv = (gcptr)D.y;
+    // thread t: rows r0 + t (mod 256) in increasing order, QR_R at a time;
+    // rows at or beyond r1 read row r0 and are not added
+    for (int64_t base = r0; base < r1; base += (int64_t)EV_THREADS * QR_R) {
+      int64_t ii[QR_R];
+#pragma unroll
+      for (int r = 0; r < QR_R; ++r) {
+        const int64_t i = base + (int64_t)threadIdx.x + (int64_t)EV_THREADS * r;
+        ii[r] = i < r1 ? i : r0;
+      }
+      double yp[QR_R], f[QR_R];
+      bool any = false;   // the same for every row
       for (int j = 0; j < nf; ++j) {
-        const double cj = co[j];
-        if (__builtin_fabs(cj) < EPS12) continue;
-        const uint32_t* code = a.code + a.prog_start[pbeg + j];
-        const int var = bare_variable(code, a.prog_start[pbeg + j + 1] - a.prog_start[pbeg + j]);
-        const double f = var >= 0 ? D.X[(int64_t)var * ldx + i] : S[(int64_t)(k + 1 + j) * ldx + i];
-        const double term = (__builtin_fabs(cj - 1) < EPS12) ? f : cj * f;
-        yp = any ? yp + term : term;
+        const int kind = fkind[j];
+        if (kind == 0) continue;
+        const gcptr col = (gcptr)fcol[j];
+        const double cj = fcoef[j];
+#pragma unroll
+        for (int r = 0; r < QR_R; ++r) f[r] = col[ii[r]];
+#pragma unroll
+        for (int r = 0; r < QR_R; ++r) {
+          const double term = kind == 1 ? f[r] : cj * f[r];
+          yp[r] = any ? yp[r] + term : term;
+        }
         any = true;
       }
       if (!(__builtin_fabs(cf_free) < EPS12)) {
         const double term = cf_free * 1.0;
-        yp = any ? yp + term : term;
+#pragma unroll
+        for (int r = 0; r < QR_R; ++r) yp[r] = any ? yp[r] + term : term;
+        any = true;
       }
-      const double d = D.y[i] - yp;
-      s = s + d * d;
+      double yy[QR_R];
+#pragma unroll
+      for (int r = 0; r < QR_R; ++r) yy[r] = yv[ii[r]];
+#pragma unroll
+      for (int r = 0; r < QR_R; ++r) {
+        if (base + (int64_t)threadIdx.x + (int64_t)EV_THREADS * r < r1) {
+          const double d = yy[r] - (any ? yp[r] : 0.0);
+          s = s + d * d;
+        }
+      }
     }
   }
   s = wave_sum(s);
@@ -856,7 +968,7 @@ static QrArgs qr_args(const QrLaunch& L, int n_pos) {
   a.coefs = L.coefs; a.fit = L.fit; a.dead = L.dead; a.info = L.info;
   a.order = L.order; a.slab = L.slab; a.slab_off = L.slab_off;
   a.pos_item = L.pos_item; a.items_per_pos = L.items_per_pos; a.n_pos = n_pos;
-  a.norm_part = L.norm_part; a.prog_item = L.prog_item;
+  a.norm_part = L.norm_part; a.prog_item = L.prog_item; a.prog_kept = L.prog_kept;
   a.st = L.st; a.phys = L.phys; a.cold = L.cold; a.ks = L.ks;
   a.col_block = (int64_t)L.n_pos * L.ks;
   a.part = L.part;
@@ -889,12 +1001,19 @@ hipError_t launch_qr_solve(const QrLaunch& L, hipStream_t s) {
 
 hipError_t launch_eval_fp64(const EvalLaunch& L, hipStream_t s) {
   if (L.n_items <= 0) return hipSuccess;
-  EvalArgs a{L.cand_start, L.prog_start, L.code, L.K, L.ds, L.cand_ds, L.slab, L.slab_off, L.order, L.dead,
-             L.pair_pos, L.pair_fac, L.pair_item, L.n_pairs, L.items_per_pair, L.norm_part};
+  EvalArgs a{L.prog_start, L.code,     L.K,        L.ds,       L.cand_ds, L.slab,          L.fac_prog,
+             L.fac_cand,   L.fac_item, L.fac_kept, L.alive, L.n_facs,  L.items_per_fac, L.norm_part};
   const dim3 grid((unsigned)L.n_items), block(EV_THREADS);
   if (L.rows == 1) hipLaunchKernelGGL(k_eval_fp64<1>, grid, block, 0, s, a);
   else if (L.rows == 4) hipLaunchKernelGGL(k_eval_fp64<4>, grid, block, 0, s, a);
   else hipLaunchKernelGGL(k_eval_fp64<2>, grid, block, 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_alive_fp64(const int32_t* ustart, const int32_t* users, const int32_t* dead, int32_t n,
+                             int32_t* alive, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_alive_fp64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, ustart, users, dead, n, alive);
   return hipGetLastError();
 }
 
@@ -913,7 +1032,7 @@ hipError_t launch_nan0_fp64(const Nan0Launch& L, hipStream_t s) {
 
 hipError_t launch_score_fp64(const ScoreLaunch& L, hipStream_t s) {
   if (L.n_pos <= 0) return hipSuccess;
-  ScoreArgs a{L.cand_start, L.prog_start, L.code, L.K, L.ds, L.cand_ds, L.coefs, L.slab, L.slab_off, L.order,
+  ScoreArgs a{L.cand_start, L.prog_start, L.code, L.K, L.ds, L.cand_ds, L.coefs, L.slab, L.prog_kept, L.order,
               L.dead, L.info, L.pos_item, L.n_pos, L.items_per_pos, L.part, L.fit};
   const dim3 grid((unsigned)L.n_items), block(EV_THREADS), fgrid((unsigned)((L.n_pos + 255) / 256));
   if (L.raw) {

@@ -125,3 +125,23 @@ def test_structure_errors():
     assert capi.check_batch(bad, "ols", 2)[0] == D["RR_E_ARG"]
     wide = capi.build_batch([["x0"] * D["RR_MAX_COLS"]])
     assert capi.check_batch(wide, "ols", 1)[0] == D["RR_E_LIMIT"]
+
+
+def test_large_batch_reports_first_invalid_program():
+    """A batch big enough to be checked on several host threads: valid as a
+    whole, and with invalid programs planted late and early, the one reported
+    is the first in batch order."""
+    trees = treegen.random_trees(9, 4000, 10, 60, 4)
+    arrs = capi.build_batch([[t, "x0"] for t in trees])
+    assert arrs["prog_start"][-1] > 4 * 16384        # several ranges of the split
+    assert capi.check_batch(arrs, "ols", 4) == (0, "")
+    code = arrs["code"].copy()
+    n_progs = len(arrs["prog_start"]) - 1
+    late, early = n_progs - 5, 3 * n_progs // 10      # in different ranges
+    for p in (late, early):
+        code[arrs["prog_start"][p]] = D["RR_OP_PUSH"]  # push on an empty accumulator
+    rc, msg = capi.check_batch(dict(arrs, code=code), "ols", 4)
+    assert rc == D["RR_E_ARG"] and msg.endswith("(program %d)" % early), msg
+    code[arrs["prog_start"][early]] = arrs["code"][arrs["prog_start"][early]]
+    rc, msg = capi.check_batch(dict(arrs, code=code), "ols", 4)
+    assert rc == D["RR_E_ARG"] and msg.endswith("(program %d)" % late), msg

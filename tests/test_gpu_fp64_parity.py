@@ -183,3 +183,45 @@ def test_autotune_keeps_fp64_bits(ctx):
     ctx.set_option("fp32_chunk_rows", 4096)
     ctx.set_option("fp32_streams", 0)
     ctx.free(h)
+
+
+def test_shared_factors_same_bits(ctx):
+    """Candidates that repeat the same factors (equal programs, constants at
+    different pool indexes), inside one candidate too, over two datasets in one
+    launch, with candidates settled by the factor-0 pre-pass (ln x0 first over
+    x0 < 0) sharing factors with live ones, and factors only such candidates
+    use: every result is the same bits as the candidate scored alone, and the
+    coefficients of candidates with finite columns are the oracle's QR
+    solution of [factors, 1] against y."""
+    print("case: shared factors, two datasets", flush=True)
+    pool = ["sin x0", "* x0 c0x1p+1", "sqr + x1 c0x1.8p+1", "cos * x0 x1", "exp * x1 c0x1p-2",
+            "* x0 c0x1.0000000000001p+1", "+ x0 c0x0p+0", "+ x0 c-0x0p+0", "/ x1 + sqr x0 c0x1p+0"]
+    only_dead = ["sqrt x1", "sqr sin x1"]
+    rng = np.random.default_rng(51)
+    cands = []
+    for i in range(160):
+        f = [pool[j] for j in rng.choice(len(pool), size=int(rng.integers(1, 5)), replace=True)]
+        if i % 7 == 0:
+            f.append("x%d" % (i % 2))
+        if i % 5 == 0:
+            f = ["ln x0"] + f + [only_dead[i % 2]]
+        cands.append(f)
+    cands.append(["sin x0", "sin x0"])
+    datasets = [make_data(52, 4100, 2), make_data(53, 9000, 2)]
+    hs = [ctx.upload(X, y) for X, y in datasets]
+    owner = [i % 2 for i in range(len(cands))]
+    cm, fm = ctx.fitness(None, capi.build_batch(cands), handles=[hs[s] for s in owner])
+    ods = [O.Dataset(X, y) for X, y in datasets]
+    for q, (f, s) in enumerate(zip(cands, owner)):
+        cs, fs = ctx.fitness(hs[s], capi.build_batch([f]))
+        assert same_bits([fm[q]["e0"], fm[q]["e1"]], [fs[0]["e0"], fs[0]["e1"]]), (q, f)
+        assert fm[q]["rank"] == fs[0]["rank"] and fm[q]["flags"] == fs[0]["flags"], (q, f)
+        assert same_bits(cm[q], cs[0]), (q, f)
+        cols = [ods[s].eval(p) for p in f] + [np.ones(datasets[s][1].size)]
+        A = np.stack(cols, axis=1)
+        if np.all(np.isfinite(A)):
+            c, rank = O.qr_solve(A, datasets[s][1])
+            assert same_bits(cm[q], c) and fm[q]["rank"] == rank, (q, f)
+    assert any(np.isnan(c).all() for c in cm)   # the pre-pass settled some
+    for h in hs:
+        ctx.free(h)
