@@ -215,6 +215,30 @@ __device__ inline void chunk_sums(double (&v)[NV], double (*red)[NV]) {
 // its tail norm and hence its reflector are NaN, and with the threshold NaN
 // no later column stops the factorisation: every coefficient is NaN, rank =
 // min(n, k).
+// Chunk sums added in chunk order, NV interleaved series at once: out[v] =
+// base[v] + base[stride + v] + ... + base[(n-1) stride + v], left to right.
+// The loads of eight chunks are issued before their additions.
+template <int NV>
+__device__ inline void ordered_sums(const double* base, int64_t stride, int64_t n, double* out) {
+#pragma unroll
+  for (int v = 0; v < NV; ++v) out[v] = base[v];
+  int64_t q = 1;
+  for (; q + 8 <= n; q += 8) {
+    double x[8][NV];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+#pragma unroll
+      for (int v = 0; v < NV; ++v) x[u][v] = base[(q + u) * stride + v];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+#pragma unroll
+      for (int v = 0; v < NV; ++v) out[v] = out[v] + x[u][v];
+  }
+  for (; q < n; ++q)
+#pragma unroll
+    for (int v = 0; v < NV; ++v) out[v] = out[v] + base[q * stride + v];
+}
+
 __global__ __launch_bounds__(256) void k_qr_init(QrArgs a) {
   const int pos = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   if (pos >= a.n_pos) return;
@@ -234,10 +258,10 @@ __global__ __launch_bounds__(256) void k_qr_init(QrArgs a) {
       s0 = c.D->xnorm[2 * var];
       s1 = c.D->xnorm[2 * var + 1];
     } else {
-      const double* pp = a.norm_part + 2 * (int64_t)a.prog_item[c.pbeg + j];
-      s0 = pp[0];
-      s1 = pp[1];
-      for (int64_t q = 1; q < nch; ++q) { s0 = s0 + pp[2 * q]; s1 = s1 + pp[2 * q + 1]; }
+      double s01[2];
+      ordered_sums<2>(a.norm_part + 2 * (int64_t)a.prog_item[c.pbeg + j], 2, nch, s01);
+      s0 = s01[0];
+      s1 = s01[1];
     }
     if (j == 0 && s1 != s1) {
       const int size0 = (int)((c.n < (int64_t)c.k) ? c.n : (int64_t)c.k);
@@ -468,8 +492,11 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr_dot(QrArgs a) {
       default: dot_rows<QR_G>(vr, src, den, p, r0, r1, acc); break;
     }
     chunk_sums<QR_G>(acc, red);
-    if (threadIdx.x == 0)
-      for (int t = 0; t < gn; ++t) out[g0 + t] = acc[t];
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int t = 0; t < QR_G; ++t)   // constant indices: acc stays in registers
+        if (t < gn) out[g0 + t] = acc[t];
+    }
   }
 }
 
@@ -483,8 +510,8 @@ __global__ __launch_bounds__(256) void k_qr_mid(QrArgs a) {
   if (s.flags & QF_APPLY) {
     const int64_t first = first_item(a, pos), nch = (c.n + rrm::NORM_CHUNK - 1) / rrm::NORM_CHUNK;
     for (int t = 0; t < ntarget; ++t) {
-      double sum = a.part[first * 2 * a.ks + t];
-      for (int64_t q = 1; q < nch; ++q) sum = sum + a.part[(first + q) * 2 * a.ks + t];
+      double sum;
+      ordered_sums<1>(a.part + first * 2 * a.ks + t, 2 * a.ks, nch, &sum);
       double* const col = target_col(a, pos, c, p, t, nrem);
       const double td = sum + col[p];
       TD(t) = td;
@@ -536,9 +563,11 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr_update(QrArgs a) {
       default: update_rows<QR_G>(vr, src, dst, td, den, tau, apply, p, r0, r1, s12); break;
     }
     chunk_sums<2 * QR_G>(s12, red);
-    if (threadIdx.x == 0)
-      for (int t = 0; t < gn; ++t)
-        if (g0 + t < nrem) { out[2 * (g0 + t)] = s12[t]; out[2 * (g0 + t) + 1] = s12[QR_G + t]; }
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int t = 0; t < QR_G; ++t)   // constant indices: s12 stays in registers
+        if (t < gn && g0 + t < nrem) { out[2 * (g0 + t)] = s12[t]; out[2 * (g0 + t) + 1] = s12[QR_G + t]; }
+    }
   }
 }
 
@@ -551,13 +580,10 @@ __global__ __launch_bounds__(256) void k_qr_end(QrArgs a) {
   const int p = s.p, nrem = c.k - 1 - p;
   const int64_t first = first_item(a, pos), nch = (c.n + rrm::NORM_CHUNK - 1) / rrm::NORM_CHUNK;
   for (int t = 0; t < nrem; ++t) {
-    double s1 = a.part[first * 2 * a.ks + 2 * t], s2 = a.part[first * 2 * a.ks + 2 * t + 1];
-    for (int64_t q = 1; q < nch; ++q) {
-      s1 = s1 + a.part[(first + q) * 2 * a.ks + 2 * t];
-      s2 = s2 + a.part[(first + q) * 2 * a.ks + 2 * t + 1];
-    }
-    CCS(p + 1 + t) = s1;
-    TS(p + 1 + t) = s2;
+    double s12[2];
+    ordered_sums<2>(a.part + first * 2 * a.ks + 2 * t, 2 * a.ks, nch, s12);
+    CCS(p + 1 + t) = s12[0];
+    TS(p + 1 + t) = s12[1];
   }
   lawn_downdate(a, pos, c, p, false);
   s.p = p + 1;
@@ -930,8 +956,8 @@ __global__ __launch_bounds__(256) void k_fit_fp64(ScoreArgs a) {
   const DsDesc& D = a.ds[a.cand_ds ? a.cand_ds[cand] : 0];
   const int64_t first = a.items_per_pos > 0 ? (int64_t)pos * a.items_per_pos : a.pos_item[pos];
   const int64_t nch = (D.n + rrm::NORM_CHUNK - 1) / rrm::NORM_CHUNK;
-  double ssr = a.part[first];
-  for (int64_t c = 1; c < nch; ++c) ssr = ssr + a.part[first + c];
+  double ssr;
+  ordered_sums<1>(a.part + first, 1, nch, &ssr);
   write_fit(a.fit + cand, ssr, D.sst, D.n, rank, steps);
 }
 
@@ -975,16 +1001,22 @@ static QrArgs qr_args(const QrLaunch& L, int n_pos) {
   return a;
 }
 
+// The per-position kernels are latency-bound serial loops: one wave per
+// workgroup spreads a launch over more compute units.
+constexpr int POS_THREADS = 64;
+
 hipError_t launch_qr_init(const QrLaunch& L, hipStream_t s) {
   if (L.n_pos <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_qr_init, dim3((unsigned)((L.n_pos + 255) / 256)), dim3(256), 0, s, qr_args(L, L.n_pos));
+  hipLaunchKernelGGL(k_qr_init, dim3((unsigned)((L.n_pos + POS_THREADS - 1) / POS_THREADS)), dim3(POS_THREADS), 0, s,
+                     qr_args(L, L.n_pos));
   return hipGetLastError();
 }
 
 hipError_t launch_qr_step(const QrLaunch& L, int n_act, int n_items, hipStream_t s) {
   if (n_act <= 0) return hipSuccess;
   const QrArgs a = qr_args(L, n_act);
-  const dim3 pgrid((unsigned)((n_act + 255) / 256)), pblock(256), cgrid((unsigned)n_items), cblock(QR_THREADS);
+  const dim3 pgrid((unsigned)((n_act + POS_THREADS - 1) / POS_THREADS)), pblock(POS_THREADS), cgrid((unsigned)n_items),
+      cblock(QR_THREADS);
   hipLaunchKernelGGL(k_qr_begin, pgrid, pblock, 0, s, a);
   hipLaunchKernelGGL(k_qr_dot, cgrid, cblock, 0, s, a);
   hipLaunchKernelGGL(k_qr_mid, pgrid, pblock, 0, s, a);
@@ -995,7 +1027,8 @@ hipError_t launch_qr_step(const QrLaunch& L, int n_act, int n_items, hipStream_t
 
 hipError_t launch_qr_solve(const QrLaunch& L, hipStream_t s) {
   if (L.n_pos <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_qr_solve, dim3((unsigned)((L.n_pos + 255) / 256)), dim3(256), 0, s, qr_args(L, L.n_pos));
+  hipLaunchKernelGGL(k_qr_solve, dim3((unsigned)((L.n_pos + POS_THREADS - 1) / POS_THREADS)), dim3(POS_THREADS), 0, s,
+                     qr_args(L, L.n_pos));
   return hipGetLastError();
 }
 
@@ -1034,15 +1067,15 @@ hipError_t launch_score_fp64(const ScoreLaunch& L, hipStream_t s) {
   if (L.n_pos <= 0) return hipSuccess;
   ScoreArgs a{L.cand_start, L.prog_start, L.code, L.K, L.ds, L.cand_ds, L.coefs, L.slab, L.prog_kept, L.order,
               L.dead, L.info, L.pos_item, L.n_pos, L.items_per_pos, L.part, L.fit};
-  const dim3 grid((unsigned)L.n_items), block(EV_THREADS), fgrid((unsigned)((L.n_pos + 255) / 256));
+  const dim3 grid((unsigned)L.n_items), block(EV_THREADS), fgrid((unsigned)((L.n_pos + POS_THREADS - 1) / POS_THREADS));
   if (L.raw) {
     if (L.rows == 1) hipLaunchKernelGGL((k_score_fp64<true, 1>), grid, block, 0, s, a);
     else if (L.rows == 4) hipLaunchKernelGGL((k_score_fp64<true, 4>), grid, block, 0, s, a);
     else hipLaunchKernelGGL((k_score_fp64<true, 2>), grid, block, 0, s, a);
-    hipLaunchKernelGGL(k_fit_fp64<true>, fgrid, dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_fit_fp64<true>, fgrid, dim3(POS_THREADS), 0, s, a);
   } else {
     hipLaunchKernelGGL((k_score_fp64<false, 1>), grid, block, 0, s, a);
-    hipLaunchKernelGGL(k_fit_fp64<false>, fgrid, dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_fit_fp64<false>, fgrid, dim3(POS_THREADS), 0, s, a);
   }
   return hipGetLastError();
 }
