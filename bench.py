@@ -467,6 +467,8 @@ def main():
     ap.add_argument("--fp32-rows", type=int, default=None, help="samples per lane per tile, FP32 kernels (1, 2)")
     ap.add_argument("--fp32-chunk", type=int, default=None, help="samples per FP32 workgroup (multiple of 512)")
     ap.add_argument("--fp32-streams", type=int, default=None, help="1: FP32 buckets on concurrent streams")
+    ap.add_argument("--slab-budget-mb", type=int, default=None,
+                    help="device memory of one FP64 launch's work columns (candidates are launched in chunks that fit)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default kernel schedule instead of the autotuner's pick")
     ap.add_argument("--copy-probe", action="store_true", help="run only a plain 1 GiB device copy (PMC scaling)")
@@ -517,6 +519,8 @@ def main():
         runner.set_option("fp32_chunk_rows", args.fp32_chunk)
     if args.fp32_streams is not None:
         runner.set_option("fp32_streams", args.fp32_streams)
+    if args.slab_budget_mb:
+        runner.set_option("slab_budget_mb", args.slab_budget_mb)
     t0 = time.perf_counter()
     nf = runner.prepare_ols(trees)
     t_prep = time.perf_counter() - t0
@@ -562,7 +566,7 @@ def main():
             "p99_step_ms": float(np.percentile(r["lat"], 99) * 1e3),
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                          "frac": achieved / PEAK_HBM_GBS, "traffic": pmc_traffic("fp64_step"),
-                         "kernel": "k_nan0_fp64 + k_eval_fp64<%d> + k_qr_* + k_score_fp64 + k_fit_fp64" %
+                         "kernel": "k_nan0_fp64 + k_alive_fp64 + k_eval_fp64<%d> + k_qr_* + k_score_fp64 + k_fit_fp64" %
                                    runner.get_option("fp64_eval_rows"),
                          "launch_ms": float(r["kms"].mean()),
                          "algorithmic_bytes": byts},

@@ -79,6 +79,25 @@ __device__ __noinline__ double call_cos(double x) { return rrm::cos(x); }
 __device__ __noinline__ double call_log(double x) { return rrm::log(x); }
 __device__ __noinline__ double call_exp(double x) { return rrm::exp(x); }
 
+// The same for the R samples of one interpreter step in one call: the R
+// independent evaluations share the call and overlap in the callee.
+template <int R>
+struct Rows {
+  double v[R];
+};
+#define RR_CALL_ROWS(NAME, FN)                                      \
+  template <int R>                                                  \
+  __device__ __noinline__ Rows<R> NAME(Rows<R> x) {                 \
+    Rows<R> y;                                                      \
+    _Pragma("unroll") for (int r = 0; r < R; ++r) y.v[r] = FN(x.v[r]); \
+    return y;                                                       \
+  }
+RR_CALL_ROWS(call_sin_rows, rrm::sin)
+RR_CALL_ROWS(call_cos_rows, rrm::cos)
+RR_CALL_ROWS(call_log_rows, rrm::log)
+RR_CALL_ROWS(call_exp_rows, rrm::exp)
+#undef RR_CALL_ROWS
+
 // Rows are padded so that a tile (CP threads x up to 4 rows, or 64 lanes x up
 // to 8 rows) never reads past the end of a column: the row stride of every
 // dataset and work column is a multiple of RR_ROW_PAD.
@@ -177,22 +196,42 @@ __device__ inline void eval_f64(const uint32_t* __restrict__ code, int len, cons
           }
           ++sp;
           break;
-        case RR_OP_SIN:
+        case RR_OP_SIN: {
+          Rows<R> t;
 #pragma unroll
-          for (int r = 0; r < R; ++r) acc[r] = call_sin(acc[r]);
-          break;
-        case RR_OP_COS:
+          for (int r = 0; r < R; ++r) t.v[r] = acc[r];
+          t = call_sin_rows<R>(t);
 #pragma unroll
-          for (int r = 0; r < R; ++r) acc[r] = call_cos(acc[r]);
+          for (int r = 0; r < R; ++r) acc[r] = t.v[r];
           break;
-        case RR_OP_LN:
+        }
+        case RR_OP_COS: {
+          Rows<R> t;
 #pragma unroll
-          for (int r = 0; r < R; ++r) acc[r] = call_log(acc[r]);
-          break;
-        case RR_OP_EXP:
+          for (int r = 0; r < R; ++r) t.v[r] = acc[r];
+          t = call_cos_rows<R>(t);
 #pragma unroll
-          for (int r = 0; r < R; ++r) acc[r] = call_exp(acc[r]);
+          for (int r = 0; r < R; ++r) acc[r] = t.v[r];
           break;
+        }
+        case RR_OP_LN: {
+          Rows<R> t;
+#pragma unroll
+          for (int r = 0; r < R; ++r) t.v[r] = acc[r];
+          t = call_log_rows<R>(t);
+#pragma unroll
+          for (int r = 0; r < R; ++r) acc[r] = t.v[r];
+          break;
+        }
+        case RR_OP_EXP: {
+          Rows<R> t;
+#pragma unroll
+          for (int r = 0; r < R; ++r) t.v[r] = acc[r];
+          t = call_exp_rows<R>(t);
+#pragma unroll
+          for (int r = 0; r < R; ++r) acc[r] = t.v[r];
+          break;
+        }
         case RR_OP_SQRT:
 #pragma unroll
           for (int r = 0; r < R; ++r) acc[r] = rrm::sqrt(acc[r]);

@@ -384,12 +384,17 @@ __device__ inline const double* target_src(const QrArgs& a, int pos, const Cand&
   return t < nrem ? origin_col(a, c, PHYS(1 + t)) : c.D->y;
 }
 
-// Row passes over one chunk [r0, r1) for GN target columns, QR_R rows per
-// thread per step: every load of a step is issued before its arithmetic.
+// Row passes over one chunk [r0, r1) for GN target columns, QR_DOT_R or
+// QR_UPD_R rows per thread per step: every load of a step is issued before
+// its arithmetic.
 // Thread t still takes rows r0 + t (mod QR_THREADS) in increasing order (the
 // partials of rrm::chunk_sum); rows at or beyond r1 load row r0 instead and
 // are not used.  The columns are device memory: global address space.
-constexpr int QR_R = 4;
+// (2 and 4 rows per step measure the same: the passes stream at the same
+// rate; 2 needs fewer registers)
+constexpr int QR_R = 4;       // rows per thread per step of the OLS scoring
+constexpr int QR_DOT_R = 2;   // ... of the dot pass
+constexpr int QR_UPD_R = 2;   // ... of the update pass
 typedef const __attribute__((address_space(1))) double* gcptr;
 typedef __attribute__((address_space(1))) double* gptr;
 
@@ -401,10 +406,10 @@ __device__ inline void dot_rows(const double* vr_, const double* const* src_, do
   gcptr src[GN];
 #pragma unroll
   for (int t = 0; t < GN; ++t) src[t] = (gcptr)src_[t];
-  for (int64_t base = r0; base < r1; base += (int64_t)QR_THREADS * QR_R) {
-    double v[QR_R], x[QR_R][GN];
+  for (int64_t base = r0; base < r1; base += (int64_t)QR_THREADS * QR_DOT_R) {
+    double v[QR_DOT_R], x[QR_DOT_R][GN];
 #pragma unroll
-    for (int r = 0; r < QR_R; ++r) {
+    for (int r = 0; r < QR_DOT_R; ++r) {
       const int64_t i = base + (int64_t)threadIdx.x + (int64_t)QR_THREADS * r;
       const int64_t ii = i < r1 ? i : r0;
       v[r] = vr[ii];
@@ -412,7 +417,7 @@ __device__ inline void dot_rows(const double* vr_, const double* const* src_, do
       for (int t = 0; t < GN; ++t) x[r][t] = src[t][ii];
     }
 #pragma unroll
-    for (int r = 0; r < QR_R; ++r) {
+    for (int r = 0; r < QR_DOT_R; ++r) {
       const int64_t i = base + (int64_t)threadIdx.x + (int64_t)QR_THREADS * r;
       if (i < r1 && i > p) {
         const double e = v[r] / den;   // the essential part of the reflector
@@ -435,10 +440,10 @@ __device__ inline void update_rows(const double* vr_, const double* const* src_,
 #pragma unroll
   for (int t = 0; t < GN; ++t) { src[t] = (gcptr)src_[t]; dst[t] = (gptr)dst_[t]; }
   const bool store = apply || p == 0;
-  for (int64_t base = r0; base < r1; base += (int64_t)QR_THREADS * QR_R) {
-    double v[QR_R], x[QR_R][GN];
+  for (int64_t base = r0; base < r1; base += (int64_t)QR_THREADS * QR_UPD_R) {
+    double v[QR_UPD_R], x[QR_UPD_R][GN];
 #pragma unroll
-    for (int r = 0; r < QR_R; ++r) {
+    for (int r = 0; r < QR_UPD_R; ++r) {
       const int64_t i = base + (int64_t)threadIdx.x + (int64_t)QR_THREADS * r;
       const int64_t ii = i < r1 ? i : r0;
       v[r] = apply ? vr[ii] : 0.0;
@@ -446,7 +451,7 @@ __device__ inline void update_rows(const double* vr_, const double* const* src_,
       for (int t = 0; t < GN; ++t) x[r][t] = src[t][ii];
     }
 #pragma unroll
-    for (int r = 0; r < QR_R; ++r) {
+    for (int r = 0; r < QR_UPD_R; ++r) {
       const int64_t i = base + (int64_t)threadIdx.x + (int64_t)QR_THREADS * r;
       if (i < r1 && i > p) {
         const double te = apply ? tau * (v[r] / den) : 0.0;
