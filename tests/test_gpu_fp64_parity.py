@@ -225,3 +225,52 @@ def test_shared_factors_same_bits(ctx):
     assert any(np.isnan(c).all() for c in cm)   # the pre-pass settled some
     for h in hs:
         ctx.free(h)
+
+
+def _shared_factor_batch(seed, count):
+    """Candidates drawn from a small pool of factors (equal programs with
+    constants at different pool indexes), some with a constant factor."""
+    pool = ["sin x0", "* x0 c0x1p+1", "sqr + x1 c0x1.8p+1", "cos * x0 x1", "exp * x1 c0x1p-2", "sqrt x1",
+            "/ x1 + sqr x0 c0x1p+0", "* sin x0 cos x1", "c0x1.8p+1"]
+    rng = np.random.default_rng(seed)
+    return [[pool[j] for j in rng.choice(len(pool), size=int(rng.integers(1, 6)), replace=False)]
+            for _ in range(count)]
+
+
+def test_shared_factors_across_slab_chunks_same_bits(ctx):
+    """Kept columns are shared within a launch chunk only: with a 1 MB slab
+    budget (a few candidates per chunk) the results are the same bits as in
+    one chunk."""
+    print("case: shared factors, many slab chunks", flush=True)
+    X, y = make_data(61, 6000, 2)
+    arrs = capi.build_batch(_shared_factor_batch(61, 300))
+    h = ctx.upload(X, y)
+    c1, f1 = ctx.fitness(h, arrs)
+    ctx.set_option("slab_budget_mb", 1)
+    c2, f2 = ctx.fitness(h, arrs)
+    ctx.set_option("slab_budget_mb", 32768)
+    ctx.free(h)
+    assert all(same_bits(a, b) for a, b in zip(c1, c2))
+    assert f1.tobytes() == f2.tobytes()
+
+
+def test_shared_factors_checked_form_takes_fp64_bits(ctx):
+    """RR_FP32_CHECKED plans FP64 launches over subsets of the batch (the
+    candidates sent to FP64 up front, then the ones the FP32 form could not
+    certify); on a batch that repeats its factors, every result it takes
+    from the FP64 form is the plain FP64 form's, bit for bit."""
+    print("case: shared factors, checked form", flush=True)
+    X, y = make_data(62, 5000, 2)
+    arrs = capi.build_batch(_shared_factor_batch(62, 300))
+    h = ctx.upload(X, y)
+    c64, f64 = ctx.fitness(h, arrs, precision="fp64")
+    cc, fc = ctx.fitness(h, arrs, precision="fp32c")
+    ctx.free(h)
+    n64 = 0
+    for i in range(len(fc)):
+        if fc[i]["flags"] & 4:
+            n64 += 1
+            assert fc[i]["e0"].tobytes() == f64[i]["e0"].tobytes(), i
+            assert fc[i]["e1"].tobytes() == f64[i]["e1"].tobytes(), i
+            assert cc[i].tobytes() == c64[i].tobytes(), i
+    assert n64 > 0   # the constant-factor candidates at least
