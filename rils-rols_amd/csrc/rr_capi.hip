@@ -795,7 +795,19 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
       ctx->fac_gen = 0;
     }
     tsize = table.size();
+    // per program, in program order (one sequential pass over what the check
+    // threads wrote, instead of cache misses in launch order): the hash
+    // mixed with the dataset, and whether it is a bare variable
     std::vector<uint64_t> ph(std::max(np, 1));
+    std::vector<uint8_t> bare(std::max(np, 1), 0);
+    for (int32_t q = 0; q < m; ++q) {
+      const int32_t c = list_in ? (*list_in)[q] : q;
+      const uint64_t mx = (uint64_t)(uint32_t)dsi_of(c) * 0x9E3779B97F4A7C15ULL;
+      for (int32_t p = b->cand_start[c]; p < b->cand_start[c + 1]; ++p) {
+        ph[p] = phash[p] ^ mx;
+        bare[p] = is_bare(p);
+      }
+    }
     std::vector<int32_t> pfound(std::max(np, 1));   // phase 1: unique factor found in the chunk, or -1
     std::vector<int32_t> pearlier(std::max(np, 1)); // phase 1: same factor earlier in the candidate, or -1
     int32_t gen = ctx->fac_gen;
@@ -825,13 +837,12 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
         // the kept columns it adds: factors new to the chunk, once each
         int64_t fresh = 0;
         for (int32_t p = pb; p < pe; ++p) {
-          if (is_bare(p)) continue;
-          ph[p] = phash[p] ^ ((uint64_t)(uint32_t)dsi * 0x9E3779B97F4A7C15ULL);
+          if (bare[p]) continue;
           pfound[p] = lookup(p, dsi);
           pearlier[p] = -1;
           if (pfound[p] >= 0) continue;
           for (int32_t p2 = pb; p2 < p && pearlier[p] < 0; ++p2)
-            if (!is_bare(p2) && pfound[p2] < 0 && pearlier[p2] < 0 && ph[p2] == ph[p] && same_prog(p2, p)) pearlier[p] = p2;
+            if (!bare[p2] && pfound[p2] < 0 && pearlier[p2] < 0 && ph[p2] == ph[p] && same_prog(p2, p)) pearlier[p] = p2;
           if (pearlier[p] < 0) ++fresh;
         }
         const int64_t need = ((int64_t)(k + 1) + fresh) * ldx;
@@ -839,7 +850,7 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
         off[q1] = used;
         used += (int64_t)(k + 1) * ldx;
         for (int32_t p = pb; p < pe; ++p) {
-          if (is_bare(p)) continue;
+          if (bare[p]) continue;
           int32_t u = pfound[p] >= 0 ? pfound[p] : pearlier[p] >= 0 ? ufac[pearlier[p]] : -1;
           if (u < 0) {
             u = (int32_t)u_prog.size();
