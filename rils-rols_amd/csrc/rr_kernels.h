@@ -30,8 +30,10 @@ struct DsDesc {
 // position; phys: n_pos x ks ints; cold: 5 x n_pos x ks doubles; part: 2 ks
 // doubles per chunk work item.
 struct QrState {
-  int32_t k, size, p, nonzero, nan_exit, steps, flags, pad;
-  double thr, tau, den, pad2;
+  int32_t k, size, p, nonzero, nan_exit, steps, flags, papply;
+  double thr, tau, den;
+  double ptau, pden, ptd0, ptd1;   // the second-to-last step's reflector and dot products, kept for the last
+  double pad;
 };
 struct QrLaunch {
   const int32_t* cand_start;

@@ -429,11 +429,12 @@ __device__ inline void dot_rows(const double* vr_, const double* const* src_, do
 }
 
 // rows i > p: x = src[t][i] (- tau * (vr[i] / den) * td[t] when applying),
-// stored to dst[t][i] when applying or at p == 0; s12[t] += x^2, and
-// s12[QR_G + t] += x^2 for rows >= p + 2
+// stored to dst[t][i] (rows up to wlim) when applying or at p == 0;
+// s12[t] += x^2, and s12[QR_G + t] += x^2 for rows >= p + 2
 template <int GN>
 __device__ inline void update_rows(const double* vr_, const double* const* src_, double* const* dst_, const double* td,
-                                   double den, double tau, bool apply, int p, int64_t r0, int64_t r1, double* s12) {
+                                   double den, double tau, bool apply, int p, int64_t wlim, int64_t r0, int64_t r1,
+                                   double* s12) {
   const gcptr vr = (gcptr)vr_;
   gcptr src[GN];
   gptr dst[GN];
@@ -459,7 +460,7 @@ __device__ inline void update_rows(const double* vr_, const double* const* src_,
         for (int t = 0; t < GN; ++t) {
           double y = x[r][t];
           if (apply) y = y - te * td[t];
-          if (store) dst[t][i] = y;
+          if (store && i <= wlim) dst[t][i] = y;
           s12[t] = s12[t] + y * y;
           if (i >= p + 2) s12[QR_G + t] = s12[QR_G + t] + y * y;
         }
@@ -468,8 +469,44 @@ __device__ inline void update_rows(const double* vr_, const double* const* src_,
   }
 }
 
+// The last step's dot product (only y left) when the second-to-last step
+// stored row p of its two targets only: rows i > p of the pivot column and
+// of y are recomputed from that step's sources with its update's operations
+// (x = src - tau' (vr'_i / den') td'), then acc += (x_pivot / den) x_y.
+__device__ inline void dot_last_rows(const double* vrp_, const double* s0_, const double* s1_, const QrState& s,
+                                     int p, int64_t r0, int64_t r1, double* acc) {
+  const gcptr vrp = (gcptr)vrp_, s0 = (gcptr)s0_, s1 = (gcptr)s1_;
+  const bool papply = s.papply != 0;
+  for (int64_t base = r0; base < r1; base += (int64_t)QR_THREADS * QR_DOT_R) {
+    double v[QR_DOT_R], x0[QR_DOT_R], x1[QR_DOT_R];
+#pragma unroll
+    for (int r = 0; r < QR_DOT_R; ++r) {
+      const int64_t i = base + (int64_t)threadIdx.x + (int64_t)QR_THREADS * r;
+      const int64_t ii = i < r1 ? i : r0;
+      v[r] = papply ? vrp[ii] : 0.0;
+      x0[r] = s0[ii];
+      x1[r] = s1[ii];
+    }
+#pragma unroll
+    for (int r = 0; r < QR_DOT_R; ++r) {
+      const int64_t i = base + (int64_t)threadIdx.x + (int64_t)QR_THREADS * r;
+      if (i < r1 && i > p) {
+        double xp = x0[r], xy = x1[r];
+        if (papply) {
+          const double te = s.ptau * (v[r] / s.pden);
+          xp = xp - te * s.ptd0;
+          xy = xy - te * s.ptd1;
+        }
+        const double e = xp / s.den;   // the essential part of the reflector
+        acc[0] = acc[0] + e * xy;
+      }
+    }
+  }
+}
+
 __global__ __launch_bounds__(QR_THREADS) void k_qr_dot(QrArgs a) {
   __shared__ double red[QR_THREADS / WAVE][QR_G];
+  __shared__ double red1[QR_THREADS / WAVE][1];
   const int item = (int)blockIdx.x;
   int64_t first;
   const int pos = chunk_pos(a, item, a.n_pos, &first);
@@ -482,6 +519,14 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr_dot(QrArgs a) {
   const int64_t r0 = (item - first) * rrm::NORM_CHUNK;
   const int64_t r1 = (r0 + rrm::NORM_CHUNK < c.n) ? r0 + rrm::NORM_CHUNK : c.n;
   double* const out = a.part + (int64_t)item * 2 * a.ks;
+  if (nrem == 0 && p >= 1) {   // the last step: from the second-to-last step's sources
+    double acc[1] = {0.0};
+    const double* const vrp = (p == 1) ? origin_col(a, c, PHYS(0)) : c.S + (int64_t)PHYS(p - 1) * c.ldx;
+    dot_last_rows(vrp, target_src(a, pos, c, p - 1, 0, 1), target_src(a, pos, c, p - 1, 1, 1), s, p, r0, r1, acc);
+    chunk_sums<1>(acc, red1);
+    if (threadIdx.x == 0) out[0] = acc[0];
+    return;
+  }
   for (int g0 = 0; g0 < ntarget; g0 += QR_G) {
     const int gn = (ntarget - g0 < QR_G) ? ntarget - g0 : QR_G;
     const double* src[QR_G];
@@ -525,6 +570,13 @@ __global__ __launch_bounds__(256) void k_qr_mid(QrArgs a) {
   } else {
     for (int t = 0; t < ntarget; ++t) TD(t) = 0.0;
   }
+  if (nrem == 1) {   // the second-to-last step: what the last step's dot recomputes from
+    s.papply = (s.flags & QF_APPLY) ? 1 : 0;
+    s.ptau = s.tau;
+    s.pden = s.den;
+    s.ptd0 = TD(0);
+    s.ptd1 = TD(1);
+  }
   if (s.flags & QF_NOUPD) {   // only y was left: rows below p of Q^T y are never read
     s.p = p + 1;
     s.flags = 0;
@@ -542,6 +594,10 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr_update(QrArgs a) {
   const int p = s.p, nrem = c.k - 1 - p, ntarget = nrem + 1;
   const bool apply = (s.flags & QF_APPLY) != 0;
   const double den = s.den, tau = s.tau;
+  // the second-to-last step stores row p + 1 only (the last step's pivot
+  // row and row p + 1 of Q^T y); the last step's dot recomputes the rows
+  // below from this step's sources
+  const int64_t wlim = (nrem == 1) ? (int64_t)p + 1 : INT64_MAX;
   const double* const vr = (p == 0) ? origin_col(a, c, PHYS(0)) : c.S + (int64_t)PHYS(p) * c.ldx;
   const int64_t r0 = (item - first) * rrm::NORM_CHUNK;
   const int64_t r1 = (r0 + rrm::NORM_CHUNK < c.n) ? r0 + rrm::NORM_CHUNK : c.n;
@@ -562,10 +618,10 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr_update(QrArgs a) {
 #pragma unroll
     for (int t = 0; t < 2 * QR_G; ++t) s12[t] = 0.0;
     switch (gn) {   // uniform
-      case 1: update_rows<1>(vr, src, dst, td, den, tau, apply, p, r0, r1, s12); break;
-      case 2: update_rows<2>(vr, src, dst, td, den, tau, apply, p, r0, r1, s12); break;
-      case 3: update_rows<3>(vr, src, dst, td, den, tau, apply, p, r0, r1, s12); break;
-      default: update_rows<QR_G>(vr, src, dst, td, den, tau, apply, p, r0, r1, s12); break;
+      case 1: update_rows<1>(vr, src, dst, td, den, tau, apply, p, wlim, r0, r1, s12); break;
+      case 2: update_rows<2>(vr, src, dst, td, den, tau, apply, p, wlim, r0, r1, s12); break;
+      case 3: update_rows<3>(vr, src, dst, td, den, tau, apply, p, wlim, r0, r1, s12); break;
+      default: update_rows<QR_G>(vr, src, dst, td, den, tau, apply, p, wlim, r0, r1, s12); break;
     }
     chunk_sums<2 * QR_G>(s12, red);
     if (threadIdx.x == 0) {
