@@ -118,27 +118,32 @@ def fp64_algorithmic_bytes(k, steps, sentinel, n, bare):
     shared by every candidate and cache-resident, so they are not counted; the
     kept columns of the other k-1-bare factors are.
       evaluation   writes the kept columns;
-      QR step 0    dot pass reads the kept columns; unless y is the only
-                   target, the update pass reads them again and writes the k
-                   working columns (k-1 targets and y);
-      QR step p>0  dot pass reads the pivot and the k-p targets; unless y is
-                   the only target, the update pass reads them again and
-                   writes the targets;
+      QR step 0    (k >= 2) the dot pass reads the kept columns, the update
+                   pass reads them again and writes the k working columns
+                   (k-1 targets and y) -- rows p+1 only if it is the
+                   second-to-last step (k == 2);
+      QR step p    (0 < p < k-1) the dot pass reads the pivot and the k-p
+                   targets, the update pass reads them again and writes the
+                   targets -- row p+1 only at the second-to-last step;
+      last step    (p = k-1 >= 1, only y left) the dot pass reads the step
+                   before's pivot and two targets and recomputes their rows:
+                   the kept columns among them when that step was step 0,
+                   three working columns otherwise;
       score        (not for sentinels) reads the kept columns.
     A sentinel decided by factor 0's NaN (steps == 0) moves nothing."""
     if sentinel and steps == 0:
         return 0.0
     kept = k - 1 - bare
     words = kept
-    if steps >= 1:
-        words += kept
-        if k >= 2:
-            words += kept + k
-    for p in range(1, steps):
-        t = k - p
-        words += 1 + t
-        if p < k - 1:
-            words += (1 + t) + t
+    for p in range(steps):
+        if p == k - 1:
+            if p >= 1:
+                words += kept if p == 1 else 3
+        elif p == 0:
+            words += 2 * kept + (k if k - 2 != 0 else 0)
+        else:
+            t = k - p
+            words += 2 * (1 + t) + (t if p != k - 2 else 0)
     if not sentinel:
         words += kept
     return 8.0 * n * words
