@@ -191,9 +191,19 @@ bool reads_variable(const uint32_t* code, int32_t len) {
 // transcendental costs tens of instructions), so that the check of a whole
 // batch runs without a branch per instruction.
 enum : uint8_t { OC_BAD = 0, OC_LDV, OC_LDC, OC_PUSH, OC_UN, OC_BIN_STACK, OC_BIN_VAR, OC_BIN_CONST };
+// Per opcode, what program_ok's loop needs, packed in one entry: invalid,
+// needs the accumulator, operand is a variable / a constant, keeps / sets the
+// accumulator, stack change, evaluation cost.
+struct OpInfo {
+  uint8_t bad, needacc, var, cst, keep, set;
+  int8_t dsp;
+  uint8_t pad;
+  int32_t cost;
+};
 struct OpTables {
   uint8_t cls[256];
   uint16_t cost[256];
+  OpInfo info[256];
   OpTables() {
     for (int op = 0; op < 256; ++op) {
       cls[op] = OC_BAD;
@@ -212,6 +222,19 @@ struct OpTables {
         if (bop == (int)RR_BOP_DIV) cost[op] = 8;
       }
     }
+    for (int op = 0; op < 256; ++op) {
+      const int k = cls[op];
+      OpInfo& o = info[op];
+      o.bad = k == OC_BAD;
+      o.needacc = k >= OC_PUSH;   // push, unary and binary operators
+      o.var = k == OC_LDV || k == OC_BIN_VAR;
+      o.cst = k == OC_LDC || k == OC_BIN_CONST;
+      o.keep = k != OC_PUSH;
+      o.set = k == OC_LDV || k == OC_LDC;
+      o.dsp = (int8_t)((k == OC_PUSH) - (k == OC_BIN_STACK));
+      o.pad = 0;
+      o.cost = cost[op];
+    }
   }
 };
 const OpTables g_ops;
@@ -224,25 +247,24 @@ template <bool HASH>
 bool program_ok(const uint32_t* code, int32_t len, int32_t d, int32_t n_consts, const double* K, int64_t* w,
                 uint64_t* h) {
   if (len <= 0) return false;
-  int sp = 0, acc = 0, bad = 0;
+  // branch-free: one table entry per instruction, the state in integers
+  uint32_t bad = 0, acc = 0;
+  int32_t sp = 0;
   int64_t cw = 0;
   uint64_t hh = 0x243F6A8885A308D3ULL;
+  const uint32_t ud = d > 0 ? (uint32_t)d : 0u, uk = n_consts > 0 ? (uint32_t)n_consts : 0u;
   for (int32_t pc = 0; pc < len; ++pc) {
     const uint32_t ins = code[pc], op = ins & 0xffu, arg = ins >> 8;
-    const int k = g_ops.cls[op];
-    cw += g_ops.cost[op];
-    bad |= k == OC_BAD;
-    bad |= (k >= OC_PUSH) & !acc;   // push, unary and binary operators need the accumulator
-    sp += (k == OC_PUSH) - (k == OC_BIN_STACK);
-    bad |= (sp < 0) | (sp > RR_MAX_STACK);
-    bad |= ((k == OC_LDV) | (k == OC_BIN_VAR)) & ((int64_t)arg >= d);
-    const bool is_const = (k == OC_LDC) | (k == OC_BIN_CONST);
-    bad |= is_const & ((int64_t)arg >= n_consts);
-    acc = (k != OC_PUSH) & (acc | (k == OC_LDV) | (k == OC_LDC));
-    if (HASH) {   // no branch on the operand kind: a rotate-xor chain
+    const OpInfo o = g_ops.info[op];
+    cw += o.cost;
+    sp += o.dsp;
+    bad |= o.bad | (o.needacc & (acc ^ 1u)) | (uint32_t)((uint32_t)sp > (uint32_t)RR_MAX_STACK) |
+           (o.var & (uint32_t)(arg >= ud)) | (o.cst & (uint32_t)(arg >= uk));
+    acc = o.keep & (acc | o.set);
+    if (HASH) {   // a rotate-xor chain
       uint64_t bits;
-      memcpy(&bits, K + (is_const && (int64_t)arg < n_consts ? arg : 0), 8);
-      const uint64_t v = is_const ? (op ^ (bits * 0x9E3779B97F4A7C15ULL)) : ins;
+      memcpy(&bits, K + ((o.cst & (uint32_t)(arg < uk)) ? arg : 0u), 8);
+      const uint64_t v = o.cst ? (op ^ (bits * 0x9E3779B97F4A7C15ULL)) : ins;
       hh = ((hh << 23) | (hh >> 41)) ^ v;
     }
   }
