@@ -721,8 +721,10 @@ struct EvalArgs {
 constexpr int EV_THREADS = 256;
 static_assert(EVAL_CHUNK_ROWS == rrm::NORM_CHUNK, "evaluation work items are the chunks of the norm order");
 
+// Eight waves per SIMD: at two rows per thread the interpreter then fits in
+// 64 VGPRs without new spills (68 unbounded, seven waves); 3% faster.
 template <int R>
-__global__ __launch_bounds__(EV_THREADS) void k_eval_fp64(EvalArgs a) {
+__global__ __launch_bounds__(EV_THREADS, 8) void k_eval_fp64(EvalArgs a) {
   static_assert(EV_THREADS == rrm::NORM_PARTIALS, "one thread per partial of the chunked order");
   __shared__ double red[EV_THREADS / WAVE][2];
   const int item = (int)blockIdx.x;
