@@ -902,7 +902,7 @@ __device__ inline int score_pos(const ScoreArgs& a, int item) {
 // tree's prediction from the coefficients and the factor columns, with the
 // arithmetic of tune_constants' tree (rils_rols_cpp.cpp:488-517).
 template <bool RAW, int R>
-__global__ __launch_bounds__(EV_THREADS) void k_score_fp64(ScoreArgs a) {
+__global__ __launch_bounds__(EV_THREADS, 8) void k_score_fp64(ScoreArgs a) {   // as k_eval_fp64
   __shared__ double red[EV_THREADS / WAVE];
   // OLS: per factor its column and coefficient; kind 0 skipped (|c| < 1e-12),
   // 1 the column itself (|c - 1| < 1e-12), 2 c times the column
