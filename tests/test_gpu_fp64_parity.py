@@ -274,3 +274,31 @@ def test_shared_factors_checked_form_takes_fp64_bits(ctx):
             assert fc[i]["e1"].tobytes() == f64[i]["e1"].tobytes(), i
             assert cc[i].tobytes() == c64[i].tobytes(), i
     assert n64 > 0   # the constant-factor candidates at least
+
+
+@pytest.mark.parametrize("n", [2, 3, 4, 5, 9, 300, 4097])
+def test_sparse_columns_and_tiny_n_bitexact(ctx, n):
+    """The QR's last two steps on the edge cases they branch on: columns that
+    are zero below row 0 or 1 (an identity reflector, tau = 0, at the step
+    before the last), a single row left (n - p == 1), and n at or just above
+    the column count."""
+    print("case: sparse columns n=%d" % n, flush=True)
+    rng = np.random.default_rng(70 + n)
+    X = np.zeros((n, 4))
+    X[:, 0] = rng.uniform(-3, 3, n)
+    X[0, 1] = 2.5                      # zero below row 0
+    X[:2, 2] = [1.5, -0.75]            # zero below row 1
+    X[:, 3] = rng.uniform(0.5, 2.0, n)
+    y = 1.5 * X[:, 0] - 0.5 * X[:, 3] + rng.normal(0, 0.01, n)
+    trees = ["x1", "x2", "+ x1 x0", "+ x2 x0", "+ x1 x2", "+ + x0 x1 x2", "+ + x1 x2 x3", "* x1 c0x1p+1",
+             "+ x0 x3", "+ + x0 x3 sqr x1", "+ + + x0 x1 x2 x3", "+ sin x1 x0", "+ sqr x2 x1"]
+    od = O.Dataset(X, y)
+    tuned = [od.tune(t) for t in trees]
+    arrs = capi.build_batch([tu["factors"] for tu in tuned])
+    h = ctx.upload(X, y)
+    coefs, fit = ctx.fitness(h, arrs, mode="ols")
+    ctx.free(h)
+    for t, tu, c, f in zip(trees, tuned, coefs, fit):
+        assert same_bits(c, tu["coefs"]), (t, c, tu["coefs"])
+        assert f["rank"] == tu["rank"], t
+        assert same_bits([f["e0"], f["e1"]], [tu["fit"][0], tu["fit"][1]]), (t, f, tu["fit"])
