@@ -472,6 +472,7 @@ def main():
     ap.add_argument("--fp32-rows", type=int, default=None, help="samples per lane per tile, FP32 kernels (1, 2)")
     ap.add_argument("--fp32-chunk", type=int, default=None, help="samples per FP32 workgroup (multiple of 512)")
     ap.add_argument("--fp32-streams", type=int, default=None, help="1: FP32 buckets on concurrent streams")
+    ap.add_argument("--host-threads", type=int, default=None, help="host threads that check a large batch (1..64)")
     ap.add_argument("--slab-budget-mb", type=int, default=None,
                     help="device memory of one FP64 launch's work columns (candidates are launched in chunks that fit)")
     ap.add_argument("--no-autotune", action="store_true",
@@ -526,6 +527,8 @@ def main():
         runner.set_option("fp32_streams", args.fp32_streams)
     if args.slab_budget_mb:
         runner.set_option("slab_budget_mb", args.slab_budget_mb)
+    if args.host_threads:
+        runner.set_option("host_threads", args.host_threads)
     t0 = time.perf_counter()
     nf = runner.prepare_ols(trees)
     t_prep = time.perf_counter() - t0
