@@ -31,9 +31,10 @@ struct DsDesc {
 // doubles per chunk work item.
 struct QrState {
   int32_t k, size, p, nonzero, nan_exit, steps, flags, papply;
+  int32_t qphys0, qphys1, qapply, pad;   // the third-to-last step: its two A targets, reflector applied
   double thr, tau, den;
-  double ptau, pden, ptd0, ptd1;   // the second-to-last step's reflector and dot products, kept for the last
-  double pad;
+  double ptau, pden, ptd0, ptd1;          // the second-to-last step's reflector and dot products
+  double qtau, qden, qtd0, qtd1, qtd2;    // the third-to-last step's (targets qphys0, qphys1, y)
 };
 struct QrLaunch {
   const int32_t* cand_start;

@@ -504,6 +504,64 @@ __device__ inline void dot_last_rows(const double* vrp_, const double* s0_, cons
   }
 }
 
+// Candidates with three or more columns: the third-to-last step (q) stores
+// row q+1 only of its three targets; the second-to-last step's passes
+// recompute their inputs from step q's sources (x = src - tau_q (vr_q,i /
+// den_q) td_q, the update's own operations) and store rows up to its p+1;
+// the last step recomputes through both reflectors.
+struct StepQ {   // step q's reflector and sources, for one candidate
+  gcptr vr, sP, sT, sy;   // pivot of step q; the sources of physical columns P, T and of y
+  double tau, den, tdP, tdT, tdy;
+  bool apply;
+};
+__device__ inline StepQ step_q(const QrArgs& a, int pos, const Cand& c, const QrState& s, int q, int P, int T) {
+  StepQ r;
+  r.vr = (gcptr)((q == 0) ? origin_col(a, c, PHYS(0)) : c.S + (int64_t)PHYS(q) * c.ldx);
+  r.sP = (gcptr)((q == 0) ? origin_col(a, c, P) : c.S + (int64_t)P * c.ldx);
+  r.sT = (gcptr)((q == 0) ? origin_col(a, c, T) : c.S + (int64_t)T * c.ldx);
+  r.sy = (gcptr)((q == 0) ? c.D->y : c.S + (int64_t)c.k * c.ldx);
+  r.tau = s.qtau;
+  r.den = s.qden;
+  r.apply = s.qapply != 0;
+  r.tdP = (P == s.qphys0) ? s.qtd0 : s.qtd1;
+  r.tdT = (T == s.qphys0) ? s.qtd0 : s.qtd1;
+  r.tdy = s.qtd2;
+  return r;
+}
+
+// rows [r0, r1) beyond row p of x^(q+1) for P, T and y, QR_DOT_R rows per
+// thread per step (thread t: rows r0 + t mod QR_THREADS, increasing), handed
+// to f(i, xP, xT, xy) in that order
+template <class F>
+__device__ inline void rows_q1(const StepQ& Q, int p, int64_t r0, int64_t r1, F&& f) {
+  for (int64_t base = r0; base < r1; base += (int64_t)QR_THREADS * QR_DOT_R) {
+    double v[QR_DOT_R], xp[QR_DOT_R], xt[QR_DOT_R], xy[QR_DOT_R];
+#pragma unroll
+    for (int r = 0; r < QR_DOT_R; ++r) {
+      const int64_t i = base + (int64_t)threadIdx.x + (int64_t)QR_THREADS * r;
+      const int64_t ii = i < r1 ? i : r0;
+      v[r] = Q.apply ? Q.vr[ii] : 0.0;
+      xp[r] = Q.sP[ii];
+      xt[r] = Q.sT[ii];
+      xy[r] = Q.sy[ii];
+    }
+#pragma unroll
+    for (int r = 0; r < QR_DOT_R; ++r) {
+      const int64_t i = base + (int64_t)threadIdx.x + (int64_t)QR_THREADS * r;
+      if (i < r1 && i > p) {
+        double P = xp[r], T = xt[r], Y = xy[r];
+        if (Q.apply) {
+          const double te = Q.tau * (v[r] / Q.den);
+          P = P - te * Q.tdP;
+          T = T - te * Q.tdT;
+          Y = Y - te * Q.tdy;
+        }
+        f(i, P, T, Y);
+      }
+    }
+  }
+}
+
 __global__ __launch_bounds__(QR_THREADS) void k_qr_dot(QrArgs a) {
   __shared__ double red[QR_THREADS / WAVE][QR_G];
   __shared__ double red1[QR_THREADS / WAVE][1];
@@ -519,7 +577,36 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr_dot(QrArgs a) {
   const int64_t r0 = (item - first) * rrm::NORM_CHUNK;
   const int64_t r1 = (r0 + rrm::NORM_CHUNK < c.n) ? r0 + rrm::NORM_CHUNK : c.n;
   double* const out = a.part + (int64_t)item * 2 * a.ks;
-  if (nrem == 0 && p >= 1) {   // the last step: from the second-to-last step's sources
+  if (nrem == 1 && p >= 1) {   // the second-to-last step, from step p-1's sources
+    const StepQ Q = step_q(a, pos, c, s, p - 1, PHYS(p), PHYS(p + 1));
+    double acc[QR_G] = {0.0, 0.0, 0.0, 0.0};
+    rows_q1(Q, p, r0, r1, [&](int64_t, double xP, double xT, double xy) {
+      const double e = xP / den;   // the essential part of the reflector
+      acc[0] = acc[0] + e * xT;
+      acc[1] = acc[1] + e * xy;
+    });
+    chunk_sums<QR_G>(acc, red);
+    if (threadIdx.x == 0) { out[0] = acc[0]; out[1] = acc[1]; }
+    return;
+  }
+  if (nrem == 0 && p >= 2) {   // the last step, through the two reflectors before it
+    const StepQ Q = step_q(a, pos, c, s, p - 2, PHYS(p - 1), PHYS(p));
+    const bool papply = s.papply != 0;
+    double acc[1] = {0.0};
+    rows_q1(Q, p, r0, r1, [&](int64_t, double xP, double xT, double xy) {
+      if (papply) {
+        const double te = s.ptau * (xP / s.pden);
+        xT = xT - te * s.ptd0;
+        xy = xy - te * s.ptd1;
+      }
+      const double e = xT / den;
+      acc[0] = acc[0] + e * xy;
+    });
+    chunk_sums<1>(acc, red1);
+    if (threadIdx.x == 0) out[0] = acc[0];
+    return;
+  }
+  if (nrem == 0 && p >= 1) {   // the last step of a two-column candidate: from step 0's sources
     double acc[1] = {0.0};
     const double* const vrp = (p == 1) ? origin_col(a, c, PHYS(0)) : c.S + (int64_t)PHYS(p - 1) * c.ldx;
     dot_last_rows(vrp, target_src(a, pos, c, p - 1, 0, 1), target_src(a, pos, c, p - 1, 1, 1), s, p, r0, r1, acc);
@@ -570,6 +657,16 @@ __global__ __launch_bounds__(256) void k_qr_mid(QrArgs a) {
   } else {
     for (int t = 0; t < ntarget; ++t) TD(t) = 0.0;
   }
+  if (nrem == 2) {   // the third-to-last step: what the next two steps recompute from
+    s.qapply = (s.flags & QF_APPLY) ? 1 : 0;
+    s.qtau = s.tau;
+    s.qden = s.den;
+    s.qtd0 = TD(0);
+    s.qtd1 = TD(1);
+    s.qtd2 = TD(2);
+    s.qphys0 = PHYS(p + 1);
+    s.qphys1 = PHYS(p + 2);
+  }
   if (nrem == 1) {   // the second-to-last step: what the last step's dot recomputes from
     s.papply = (s.flags & QF_APPLY) ? 1 : 0;
     s.ptau = s.tau;
@@ -594,14 +691,33 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr_update(QrArgs a) {
   const int p = s.p, nrem = c.k - 1 - p, ntarget = nrem + 1;
   const bool apply = (s.flags & QF_APPLY) != 0;
   const double den = s.den, tau = s.tau;
-  // the second-to-last step stores row p + 1 only (the last step's pivot
-  // row and row p + 1 of Q^T y); the last step's dot recomputes the rows
-  // below from this step's sources
-  const int64_t wlim = (nrem == 1) ? (int64_t)p + 1 : INT64_MAX;
+  // the second- and third-to-last steps store row p + 1 only (the next
+  // pivot row and row p + 1 of the other targets); the steps after them
+  // recompute the rows below from the sources they read
+  const int64_t wlim = (nrem == 1 || nrem == 2) ? (int64_t)p + 1 : INT64_MAX;
   const double* const vr = (p == 0) ? origin_col(a, c, PHYS(0)) : c.S + (int64_t)PHYS(p) * c.ldx;
   const int64_t r0 = (item - first) * rrm::NORM_CHUNK;
   const int64_t r1 = (r0 + rrm::NORM_CHUNK < c.n) ? r0 + rrm::NORM_CHUNK : c.n;
   double* const out = a.part + (int64_t)item * 2 * a.ks;
+  if (nrem == 1 && p >= 1) {   // the second-to-last step, from step p-1's sources; rows up to p+1 stored always
+    const StepQ Q = step_q(a, pos, c, s, p - 1, PHYS(p), PHYS(p + 1));
+    const gptr dT = (gptr)(c.S + (int64_t)PHYS(p + 1) * c.ldx), dy = (gptr)(c.S + (int64_t)c.k * c.ldx);
+    const double td0 = TD(0), td1 = TD(1);
+    double s12[2 * QR_G] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    rows_q1(Q, p, r0, r1, [&](int64_t i, double xP, double xT, double xy) {
+      const double te = apply ? tau * (xP / den) : 0.0;
+      if (apply) {
+        xT = xT - te * td0;
+        xy = xy - te * td1;
+      }
+      if (i <= wlim) { dT[i] = xT; dy[i] = xy; }
+      s12[0] = s12[0] + xT * xT;
+      if (i >= p + 2) s12[QR_G] = s12[QR_G] + xT * xT;
+    });
+    chunk_sums<2 * QR_G>(s12, red);
+    if (threadIdx.x == 0) { out[0] = s12[0]; out[1] = s12[QR_G]; }
+    return;
+  }
   for (int g0 = 0; g0 < ntarget; g0 += QR_G) {
     const int gn = (ntarget - g0 < QR_G) ? ntarget - g0 : QR_G;
     const double* src[QR_G];
