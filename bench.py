@@ -118,17 +118,16 @@ def fp64_algorithmic_bytes(k, steps, sentinel, n, bare):
     shared by every candidate and cache-resident, so they are not counted; the
     kept columns of the other k-1-bare factors are.
       evaluation   writes the kept columns;
-      QR step 0    (k >= 2) the dot pass reads the kept columns, the update
-                   pass reads them again and writes the k working columns
-                   (k-1 targets and y) -- rows p+1 only if it is the
-                   second-to-last step (k == 2);
-      QR step p    (0 < p < k-1) the dot pass reads the pivot and the k-p
-                   targets, the update pass reads them again and writes the
-                   targets -- row p+1 only at the second-to-last step;
-      last step    (p = k-1 >= 1, only y left) the dot pass reads the step
-                   before's pivot and two targets and recomputes their rows:
-                   the kept columns among them when that step was step 0,
-                   three working columns otherwise;
+      QR step p    the dot pass reads the pivot and the k-p targets, the update
+                   pass reads them again and writes the targets; at step 0 the
+                   reads are the kept columns (the rest is cache-resident);
+      the third- and second-to-last steps write row p+1 only;
+      second-to-last step (k >= 3): each pass reads the step before's pivot
+                   and three sources instead (the kept columns when that was
+                   step 0, four working columns otherwise);
+      last step    (only y left) the dot pass reads the sources of the step
+                   one (k == 2) or two steps before: the kept columns when
+                   that was step 0, four working columns otherwise;
       score        (not for sentinels) reads the kept columns.
     A sentinel decided by factor 0's NaN (steps == 0) moves nothing."""
     if sentinel and steps == 0:
@@ -137,13 +136,17 @@ def fp64_algorithmic_bytes(k, steps, sentinel, n, bare):
     words = kept
     for p in range(steps):
         if p == k - 1:
-            if p >= 1:
-                words += kept if p == 1 else 3
+            if p == 1:
+                words += kept
+            elif p >= 2:
+                words += kept if p == 2 else 4
+        elif p == k - 2 and p >= 1:
+            words += 2 * (kept if p == 1 else 4)
         elif p == 0:
-            words += 2 * kept + (k if k - 2 != 0 else 0)
+            words += 2 * kept + (k if k >= 4 else 0)
         else:
             t = k - p
-            words += 2 * (1 + t) + (t if p != k - 2 else 0)
+            words += 2 * (1 + t) + (t if p < k - 3 else 0)
     if not sentinel:
         words += kept
     return 8.0 * n * words
