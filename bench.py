@@ -248,12 +248,14 @@ def copy_probe():
     print(json.dumps({"copy_probe_bytes_read": nbytes, "copy_probe_bytes_written": nbytes, "launches": 5}))
 
 
-def serving_workload(n_runs=64, rate=4.0, seed=31337):
+def serving_workload(n_runs=64, rate=4.0, seed=31337, keep=None):
     """The serving mix: n_runs independent regression runs arriving as a Poisson
     stream (rate runs/s); per run n log-uniform in [1e3, 1e6], features uniform
     in [2, 20], target a random 5-20-node tree plus 1% noise (sigma = 0.01 RMS(y),
     the reference's noisefy convention, rils_rols/utils.py:13-24); a quarter of
-    the runs are binary classification (target thresholded at its median)."""
+    the runs are binary classification (target thresholded at its median).
+    keep: indices of the runs to return (all by default); the others are drawn
+    all the same, so a kept run is the same whatever else is kept."""
     rng = np.random.default_rng(seed)
     t = 0.0
     runs = []
@@ -271,8 +273,69 @@ def serving_workload(n_runs=64, rate=4.0, seed=31337):
         y = yt + rng.normal(0, 0.01 * np.sqrt(np.mean(yt ** 2)), n)
         if i in cls_idx:
             y = (y > np.median(y)).astype(np.float64)
-        runs.append(dict(arrival=t, X=X, y=y, classification=i in cls_idx, seed=1000 + i, target=" ".join(tgt)))
+        if keep is None or i in keep:
+            runs.append(dict(index=i, arrival=t, X=X, y=y, classification=i in cls_idx, seed=1000 + i,
+                             target=" ".join(tgt)))
     return runs
+
+
+def serving_plan(n_runs=64, rate=4.0, seed=31337):
+    """(index, arrival, n, d, classification) of every run of serving_workload,
+    without keeping the datasets."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    cls_idx = set(rng.choice(n_runs, n_runs // 4, replace=False).tolist())
+    for i in range(n_runs):
+        t += rng.exponential(1.0 / rate)
+        n = int(round(10 ** rng.uniform(3, 6)))
+        d = int(rng.integers(2, 21))
+        X = rng.uniform(-3, 3, (n, d))
+        while True:
+            tgt = gen_tree(rng, int(rng.integers(5, 21)), d)
+            yt = eval_prefix(tgt, X)
+            if np.all(np.isfinite(yt)) and np.std(yt) > 1e-6 and np.max(np.abs(yt)) < 1e8:
+                break
+        rng.normal(0, 0.01 * np.sqrt(np.mean(yt ** 2)), n)
+        out.append((i, t, n, d, i in cls_idx))
+    return out
+
+
+def precision_workload(n=1_000_000, ntrees=16384, d=20):
+    """The precision-path batch: ntrees trees of 5-50 nodes (seed 4242) with at
+    most 8 additive factors, over n x d uniform in [-5, 5] (seed 2424) with a
+    15-node target plus N(0, 0.01) noise."""
+    import rils_rols_cpp
+    rng = np.random.default_rng(2424)
+    X = rng.uniform(-5, 5, (n, d))
+    while True:
+        tgt = gen_tree(rng, 15, d)
+        yt = eval_prefix(tgt, X)
+        if np.all(np.isfinite(yt)) and np.std(yt) > 1e-3 and np.max(np.abs(yt)) < 1e6:
+            break
+    y = yt + rng.normal(0, 0.01, n)
+    trng = np.random.default_rng(4242)
+    trees = []
+    while len(trees) < ntrees:
+        t = " ".join(gen_tree(trng, int(trng.integers(5, 51)), d))
+        if len(rils_rols_cpp._ols_factors(t)) <= 8:
+            trees.append(t)
+    return X, y, trees
+
+
+def search_workload(n=1_000_000, d=5):
+    """The full-search dataset: n x d uniform in [-3, 3] (seed 777), target a
+    seeded 15-node tree plus noise of SD 0.01 RMS(y) (the reference's noisefy
+    convention, rils_rols/utils.py:13-24)."""
+    rng = np.random.default_rng(777)
+    X = rng.uniform(-3, 3, (n, d))
+    while True:
+        tgt = gen_tree(rng, 15, d)
+        yt = eval_prefix(tgt, X)
+        if np.all(np.isfinite(yt)) and np.std(yt) > 1e-3 and np.max(np.abs(yt)) < 1e6:
+            break
+    y = yt + rng.normal(0, 0.01 * np.sqrt(np.mean(yt ** 2)), n)
+    return X, y, tgt
 
 
 def serving_main(args):
@@ -329,22 +392,7 @@ def precision_main(args):
     2424): FP32 form timed, FP64 form once for the error report."""
     import rils_rols_cpp
     n, d, ntrees = args.precision_rows, 20, args.precision_trees
-    rng = np.random.default_rng(2424)
-    X = rng.uniform(-5, 5, (n, d))
-    while True:
-        tgt = gen_tree(rng, 15, d)
-        yt = eval_prefix(tgt, X)
-        if np.all(np.isfinite(yt)) and np.std(yt) > 1e-3 and np.max(np.abs(yt)) < 1e6:
-            break
-    y = yt + rng.normal(0, 0.01, n)
-    trng = np.random.default_rng(4242)
-    trees, factors = [], []
-    while len(trees) < ntrees:
-        t = " ".join(gen_tree(trng, int(trng.integers(5, 51)), d))
-        fs = rils_rols_cpp._ols_factors(t)
-        if len(fs) <= 8:
-            trees.append(t)
-            factors.append(fs)
+    X, y, trees = precision_workload(n, ntrees)
     runner = rils_rols_cpp.BatchRunner(np.ascontiguousarray(X.T), y, 0)
     nf = np.array(runner.prepare_ols(trees))
     total, lat, kms, coefs32, fit32, n64, f32 = timed_steps(runner, runner, "fp32", args.warmup, args.steps)
@@ -388,14 +436,7 @@ def search_main(args):
     same search for a short prefix of fit calls and its rate is extrapolated."""
     import rils_rols_cpp
     n, d = args.search_rows, 5
-    rng = np.random.default_rng(777)
-    X = rng.uniform(-3, 3, (n, d))
-    while True:
-        tgt = gen_tree(rng, 15, d)
-        yt = eval_prefix(tgt, X)
-        if np.all(np.isfinite(yt)) and np.std(yt) > 1e-3 and np.max(np.abs(yt)) < 1e6:
-            break
-    y = yt + rng.normal(0, 0.01 * np.sqrt(np.mean(yt ** 2)), n)
+    X, y, tgt = search_workload(n)
     # verbose: the search prints its state every 10000 fit calls (rils_rols_cpp.cpp:563-579) on stdout;
     # the JSON line stays the last line
     rr = rils_rols_cpp.rils_rols(False, args.search_fit_calls, 10 ** 7, 0.001, 30, 1.0, True, 12345)
@@ -592,7 +633,7 @@ def main():
         t1 = kernel_probe(runner, one, "fp32")
         f1 = float(np.sum(flops_c[one]))
         achieved = f1 / (t1 / 1e3) / 1e12
-        reasons = np.bincount(np.where((fl32 & 2) > 0, -own[:, 3].astype(int), 0), minlength=8).tolist()
+        reasons = np.bincount(np.where((fl32 & 2) > 0, -own[:, 3].astype(int), 0), minlength=9).tolist()
         nfa = np.array(nf)
         bucket_ms = {}
         for lo, hi in [(2, 2), (3, 4), (5, 8)]:

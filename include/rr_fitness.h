@@ -102,7 +102,9 @@ typedef struct rr_batch {
 #define RR_FP32_REFINED 1 /* single pass: FP32 evaluation with per-value error
                              bounds, Gram sums of the FP32 values in FP64 (exact
                              products), FP64 solve with one refinement step, and
-                             a certified bound rr_fit.err; candidates flagged
+                             a certified bound rr_fit.err; a result is certified
+                             only when its R2 provably lies within 1e-9 of the
+                             FP64 form's; candidates flagged
                              RR_F_NEEDS64 must be re-scored in FP64 before they
                              decide anything */
 #define RR_FP32_CHECKED 2 /* OLS mode only: RR_FP32_REFINED with the FP64 form
@@ -132,7 +134,8 @@ typedef struct rr_fit {
                      value, 2 fewer samples than columns, 3 NaN column with
                      ill-conditioned finite columns, 4 ill-conditioned, 5 a
                      coefficient threshold inside the bound, 6 bound too wide,
-                     7 more than 8 factors) */
+                     7 more than 8 factors, 8 the bound on |R2 - R2_FP64|
+                     above 1e-9) */
   int32_t flags;  /* RR_F_* */
 } rr_fit;
 

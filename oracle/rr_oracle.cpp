@@ -1229,6 +1229,49 @@ double orc_batch_tune(int h, const char* pres, int threads, double* fit3) {
   return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
+// orc_tune over many prefixes ('\n'-separated) on `threads` std::threads:
+// per candidate i its coefficient count k_out[i], coefficients at
+// coefs[cap*i ...], rank rank_out[i], fitness fit3[3i..3i+2]; *factors_out
+// holds the factor programs of every candidate, '\n'-separated within a
+// candidate and candidates separated by '\t'.  Returns 0, or -1 (message in
+// orc_last_error) when a candidate has more than `cap` coefficients.
+int orc_batch_tune_full(int h, const char* pres, int threads, double* coefs, int cap, int* k_out, int* rank_out,
+                        double* fit3, char** factors_out) {
+  std::vector<std::string> items;
+  for (const char* p = pres; *p;) {
+    const char* q = strchr(p, '\n');
+    if (!q) { items.emplace_back(p); break; }
+    items.emplace_back(p, q - p);
+    p = q + 1;
+  }
+  const Data& D = *g_data[h];
+  std::vector<std::string> facs(items.size());
+  std::atomic<int> bad{-1};
+  auto one = [&](size_t i) {
+    P t = parse(items[i]);
+    Tuned tu = tune_constants(*t, D);
+    const int k = (int)tu.coefs.size();
+    k_out[i] = k;
+    rank_out[i] = tu.rank;
+    if (k > cap) { bad = (int)i; return; }
+    for (int j = 0; j < k; ++j) coefs[(size_t)cap * i + j] = tu.coefs[j];
+    Fit f = score(*tu.ols, D);
+    fit3[3 * i] = f.e0; fit3[3 * i + 1] = f.e1; fit3[3 * i + 2] = f.size;
+    for (size_t j = 0; j < tu.factors.size(); ++j) { if (j) facs[i] += '\n'; facs[i] += tu.factors[j]; }
+  };
+  std::atomic<size_t> next{0};
+  auto worker = [&]() { size_t i; while ((i = next.fetch_add(1)) < items.size()) one(i); };
+  std::vector<std::thread> th;
+  for (int t = 1; t < std::max(threads, 1); ++t) th.emplace_back(worker);
+  worker();
+  for (auto& x : th) x.join();
+  if (bad >= 0) { g_err = "candidate " + std::to_string(bad.load()) + " has more coefficients than cap"; return -1; }
+  std::string s;
+  for (size_t i = 0; i < facs.size(); ++i) { if (i) s += '\t'; s += facs[i]; }
+  *factors_out = dup_str(s);
+  return 0;
+}
+
 // Acceptance primitives of the search (rils_rols_cpp.cpp:543-599) on two
 // fitness tuples a = (e0, e1, size), b: out[0] = fitness_value(a),
 // out[1] = compare_fitness(a, b), out[2] = dominates(a, b).

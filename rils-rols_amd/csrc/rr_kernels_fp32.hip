@@ -314,7 +314,10 @@ __device__ inline void bind_dataset(Fp32Args& a, int cand) {
 }
 
 // reasons a candidate is left to the FP64 form (in rr_fit.rank, negated)
-enum { NR_UNCERTAIN = 1, NR_SHORT = 2, NR_NAN_COND = 3, NR_COND = 4, NR_THRESHOLD = 5, NR_BOUND = 6, NR_WIDE = 7 };
+enum { NR_UNCERTAIN = 1, NR_SHORT = 2, NR_NAN_COND = 3, NR_COND = 4, NR_THRESHOLD = 5, NR_BOUND = 6, NR_WIDE = 7,
+       NR_R2 = 8 };
+// the largest certified |R2_32 - R2_64|: a bound above it sends the candidate to FP64
+constexpr double R2_TOL = 1e-9;
 
 __device__ inline void write_needs64(rr_fit* f, int reason) {
   rr_fit o;
@@ -695,6 +698,8 @@ __global__ __launch_bounds__(64) void k_fin_fp32(Fp32Args a, ChunkArgs w) {
   o.rank = k;
   o.flags = 0;
   int reason = ambiguous ? NR_THRESHOLD : 0;
+  // |R2_32 - R2_64| = |SSR32 - SSR64| / SST <= max(hi - ssr, ssr - lo) / SST
+  if (!reason && !((hi - ssr) <= R2_TOL * a.sst && (ssr - lo) <= R2_TOL * a.sst)) reason = NR_R2;
   if (r2 != r2 || o.e1 != o.e1 || !(o.err < 1.0)) reason = NR_BOUND;
   if (reason) { write_needs64(a.fit + cand, co, k, reason); return; }
   for (int i = 0; i < k; ++i) co[i] = c[i];
@@ -794,8 +799,9 @@ __global__ __launch_bounds__(64) void k_rawfin_fp32(Fp32Args a, ChunkArgs w) {
   const double up = (1 + hi / a.sst) * (1 + __builtin_sqrt(hi / (double)a.n)) / fv - 1.0;
   const double dn = 1.0 - (1 + lo / a.sst) * (1 + __builtin_sqrt(lo / (double)a.n)) / fv;
   o.err = (up > dn ? up : dn) + 1e-12;
-  o.flags = (r2 != r2 || o.e1 != o.e1 || !(o.err < 1.0)) ? RR_F_NEEDS64 : 0;
-  if (o.flags) { o.err = __builtin_huge_val(); o.rank = -NR_BOUND; }
+  const bool r2_wide = !((hi - ssr) <= R2_TOL * a.sst && (ssr - lo) <= R2_TOL * a.sst);
+  o.flags = (r2 != r2 || o.e1 != o.e1 || !(o.err < 1.0) || r2_wide) ? RR_F_NEEDS64 : 0;
+  if (o.flags) { o.err = __builtin_huge_val(); o.rank = r2_wide && r2 == r2 && o.e1 == o.e1 ? -NR_R2 : -NR_BOUND; }
   a.fit[cand] = o;
 }
 

@@ -48,6 +48,8 @@ def lib():
         L.orc_batch_tune.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.c_int, dp]
         L.orc_accept.argtypes = [dp, dp, ctypes.c_double, ctypes.c_int, dp]
         L.orc_batch_tune.restype = ctypes.c_double
+        L.orc_batch_tune_full.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.c_int, dp, ctypes.c_int, ip, ip, dp,
+                                          ctypes.POINTER(ctypes.c_void_p)]
         _lib = L
     return _lib
 
@@ -138,6 +140,24 @@ class Dataset:
             "rank": rank.value,
             "fit": (fit[0], fit[1], int(fit[2])),
         }
+
+    def batch_tune_full(self, prefixes, threads=1, cap=16):
+        """tune() for many trees on `threads` host threads: a list of dicts
+        with factors, coefs, rank and fit (no OLS-tree string)."""
+        m = len(prefixes)
+        coefs = np.empty(cap * max(m, 1))
+        k = np.empty(max(m, 1), dtype=np.int32)
+        rank = np.empty(max(m, 1), dtype=np.int32)
+        fit = np.empty(3 * max(m, 1))
+        fac = ctypes.c_void_p()
+        ip = ctypes.POINTER(ctypes.c_int)
+        rc = lib().orc_batch_tune_full(self.h, "\n".join(prefixes).encode(), threads, _dp(coefs), cap,
+                                       k.ctypes.data_as(ip), rank.ctypes.data_as(ip), _dp(fit), ctypes.byref(fac))
+        if rc != 0:
+            raise RuntimeError(lib().orc_last_error().decode())
+        facs = _take_str(fac.value).split("\t") if fac.value else [""] * m
+        return [{"factors": [f for f in facs[i].split("\n") if f], "coefs": coefs[cap * i:cap * i + k[i]].copy(),
+                 "rank": int(rank[i]), "fit": (fit[3 * i], fit[3 * i + 1], int(fit[3 * i + 2]))} for i in range(m)]
 
     def batch_tune(self, prefixes, threads=1):
         fit = np.empty(3 * len(prefixes))
