@@ -1194,6 +1194,66 @@ char* orc_fit(const double* X_rm, const double* y, int n, int d, int classificat
   } catch (std::exception& e) { g_err = e.what(); return nullptr; }
 }
 
+// One local_search (rils_rols_cpp.cpp:601-643) from the tree `pre` on
+// row-major X (every feature allowed), with a fit-call budget.  Report as
+// orc_fit's: model and prefix of the returned solution, fit_calls, the
+// fitness of the returned solution (one more fitness call, not counted),
+// and one "move" line per local-search acceptance.
+char* orc_local_search(const double* X_rm, const double* y, int n, int d, const char* pre, int max_fit_calls,
+                       double complexity_penalty, int max_complexity) {
+  try {
+    RilsRols rr;
+    rr.max_fit_calls = max_fit_calls;
+    rr.max_seconds = INT32_MAX;
+    rr.complexity_penalty = complexity_penalty;
+    rr.max_complexity = max_complexity;
+    rr.reset();
+    Data D;
+    D.n = n;
+    D.d = d;
+    D.X.assign(d, std::vector<double>(n));
+    for (int i = 0; i < n; ++i) for (int j = 0; j < d; ++j) D.X[j][i] = X_rm[(int64_t)i * d + j];
+    D.y.assign(y, y + n);
+    finish_data(D);
+    std::vector<int> rel(d);
+    std::iota(rel.begin(), rel.end(), 0);
+    rr.setup_nodes(rel);
+    P t = parse(pre);
+    P res = rr.local_search(*t, D);
+    const Fit f = score(*res, D);
+    std::string s = "model\t" + to_string(*res) + "\n";
+    s += "prefix\t" + prefix(*res) + "\n";
+    s += "fit_calls\t" + std::to_string(rr.fit_calls) + "\n";
+    char b[256];
+    snprintf(b, sizeof b, "final\t%a\t%a\t%d\n", f.e0, f.e1, f.size);
+    s += b;
+    for (const Move& m : rr.moves) {
+      snprintf(b, sizeof b, "move\t%d\t%d\t%a\t%a\t%d\t", m.kind, m.fit_calls, m.e0, m.e1, m.size);
+      s += b;
+      s += m.model;
+      s += "\n";
+    }
+    return dup_str(s);
+  } catch (std::exception& e) { g_err = e.what(); return nullptr; }
+}
+
+// relevant_features (rils_rols_cpp.cpp:753-770) of row-major X and y: the
+// kept feature indices in order into out (at most max(d, 200) of them);
+// returns their count.
+int orc_relevant_features(const double* X_rm, const double* y, int n, int d, int* out) {
+  RilsRols rr;
+  Data D;
+  D.n = n;
+  D.d = d;
+  D.X.assign(d, std::vector<double>(n));
+  for (int i = 0; i < n; ++i) for (int j = 0; j < d; ++j) D.X[j][i] = X_rm[(int64_t)i * d + j];
+  D.y.assign(y, y + n);
+  finish_data(D);
+  std::vector<int> rel = rr.relevant_features(D);
+  for (size_t i = 0; i < rel.size(); ++i) out[i] = rel[i];
+  return (int)rel.size();
+}
+
 // Timing helper for the CPU baseline: tune_constants + fitness for each
 // prefix string in `pres` ('\n'-separated) with `threads` std::threads over
 // candidates.  Returns elapsed seconds; fitness written to fit3 (3 per cand).

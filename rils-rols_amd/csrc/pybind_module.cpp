@@ -73,6 +73,21 @@ std::vector<std::string> candidates_hook(const std::string& prefix, int nvars, b
   return out;
 }
 
+// relevant_features of a fresh search over X (n x d row-major) and y
+std::vector<int> features_hook(py::array_t<double, py::array::c_style | py::array::forcecast> X,
+                               py::array_t<double, py::array::c_style | py::array::forcecast> y) {
+  py::buffer_info bx = X.request(), by = y.request();
+  if (bx.ndim != 2 || by.ndim != 1 || by.shape[0] != bx.shape[0]) throw std::invalid_argument("X must be n x d, y n");
+  const int64_t n = bx.shape[0], d = bx.shape[1];
+  const double* xp = (const double*)bx.ptr;
+  std::vector<std::vector<double>> Xf(d, std::vector<double>(n));
+  for (int64_t i = 0; i < n; ++i)
+    for (int64_t j = 0; j < d; ++j) Xf[j][i] = xp[i * d + j];
+  std::vector<double> yv((const double*)by.ptr, (const double*)by.ptr + n);
+  RilsRols rr(false, 1, 1, 0.001, 50, 1.0, false, 0);
+  return rr.relevant_features(Xf, yv);
+}
+
 py::object rewrite_hook(const std::string& prefix, const std::string& op) {
   Flat f = rrh::parse_prefix(prefix);
   rrh::Pool pool;
@@ -286,6 +301,18 @@ PYBIND11_MODULE(rils_rols_cpp, m) {
              py::gil_scoped_release nogil;
              self.fit(xp, bx.size, yp, by.size, data_cnt, feat_cnt);
            })
+      .def("local_search",
+           [](RilsRols& self, py::array_t<double, py::array::c_style | py::array::forcecast> X,
+              py::array_t<double, py::array::c_style | py::array::forcecast> y, int data_cnt, int feat_cnt,
+              const std::string& prefix) {
+             py::buffer_info bx = X.request(), by = y.request();
+             const double* xp = (const double*)bx.ptr;
+             const double* yp = (const double*)by.ptr;
+             py::gil_scoped_release nogil;
+             self.local_search_from(xp, bx.size, yp, by.size, data_cnt, feat_cnt, prefix);
+           },
+           "One local search (rils_rols_cpp.cpp:601-643) from the tree `prefix` (exchange format); the result "
+           "becomes the model")
       .def("predict",
            [](RilsRols& self, py::array_t<double, py::array::c_style | py::array::forcecast> X, int data_cnt,
               int feat_cnt) {
@@ -399,6 +426,7 @@ PYBIND11_MODULE(rils_rols_cpp, m) {
   m.def("_all_candidates", &candidates_hook, py::arg("prefix"), py::arg("nvars"), py::arg("classification") = false,
         py::arg("local_search") = true);
   m.def("_rewrite", &rewrite_hook);
+  m.def("_relevant_features", &features_hook);
   m.def("_ols_factors", &factors_hook);
   m.def("_compile", &compile_hook);
   m.def("_ols_tree", &ols_tree_hook);

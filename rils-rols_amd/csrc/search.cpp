@@ -771,6 +771,30 @@ void RilsRols::fit(const double* Xp, int64_t x_size, const double* yp, int64_t y
   fit_inner(Xe, ye);
 }
 
+void RilsRols::local_search_from(const double* Xp, int64_t x_size, const double* yp, int64_t y_size, int data_cnt,
+                                 int feat_cnt, const std::string& prefix) {
+  if (x_size != (int64_t)data_cnt * feat_cnt || y_size != data_cnt || data_cnt <= 0)
+    throw std::invalid_argument("X must be data_cnt x feat_cnt and y of length data_cnt");
+  const Flat start = parse_prefix(prefix);
+  std::vector<std::vector<double>> X(feat_cnt, std::vector<double>(data_cnt));
+  for (int i = 0; i < data_cnt; ++i)
+    for (int j = 0; j < feat_cnt; ++j) X[j][i] = Xp[(int64_t)i * feat_cnt + j];
+  std::vector<double> y(yp, yp + data_cnt);
+  reset();
+  moves_.clear();
+  stats_ = Stats();
+  sweep_ms_.clear();
+  allowed_.clear();
+  std::vector<int> rel(feat_cnt);
+  std::iota(rel.begin(), rel.end(), 0);
+  setup_nodes(rel);
+  if (!gpu_) gpu_ = new GpuSession(backend_, device_, slab_budget_mb_);
+  gpu_->upload(X, y);
+  Fit f;
+  final_solution_ = local_search(start, &f);
+  final_fitness_ = f;
+}
+
 std::vector<double> RilsRols::predict(const double* Xp, int64_t x_size, int data_cnt, int feat_cnt) {
   if (x_size != (int64_t)data_cnt * feat_cnt)
     throw std::invalid_argument("Size of X " + std::to_string(x_size) +

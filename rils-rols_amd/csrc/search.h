@@ -84,6 +84,12 @@ class RilsRols {
 
   // rils_rols_cpp.cpp:717-728 (X row-major, data_cnt x feat_cnt)
   void fit(const double* X, int64_t x_size, const double* y, int64_t y_size, int data_cnt, int feat_cnt);
+  // One local_search (rils_rols_cpp.cpp:601-643) from the tree in `prefix`
+  // (exchange format) on X row-major, every feature allowed, no sampling;
+  // the result becomes the model.  Beyond the reference's API: it lets a
+  // caller (and the tests) drive the batched sweep without the main loop.
+  void local_search_from(const double* X, int64_t x_size, const double* y, int64_t y_size, int data_cnt,
+                         int feat_cnt, const std::string& prefix);
   // rils_rols_cpp.cpp:730-750
   std::vector<double> predict(const double* X, int64_t x_size, int data_cnt, int feat_cnt);
 
@@ -109,8 +115,10 @@ class RilsRols {
   const std::vector<float>& sweep_ms() const { return sweep_ms_; }
   std::string final_prefix() const { return final_solution_.empty() ? "" : to_prefix(final_solution_); }
 
-  // generators (rils_rols_cpp.cpp:121-443), public for tests
+  // generators (rils_rols_cpp.cpp:121-443) and the feature filter
+  // (753-770), public for tests
   void setup_nodes(const std::vector<int>& rel_feat);
+  std::vector<int> relevant_features(const std::vector<std::vector<double>>& X, const std::vector<double>& y);
   std::vector<Flat> all_candidates(const Flat& solution, bool local_search);
 
  private:
@@ -175,7 +183,6 @@ class RilsRols {
   Fit fitness_raw(const Flat& t);                     // one fit call
   Flat tune_and_fit(const Flat& cand, Fit* fit);      // tune_constants + fitness, one fit call
   Flat local_search(const Flat& passed, Fit* out_fit);
-  std::vector<int> relevant_features(const std::vector<std::vector<double>>& X, const std::vector<double>& y);
   void fit_inner(const std::vector<std::vector<double>>& X_all, const std::vector<double>& y_all);
 };
 

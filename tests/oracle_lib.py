@@ -48,6 +48,10 @@ def lib():
         L.orc_batch_tune.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.c_int, dp]
         L.orc_accept.argtypes = [dp, dp, ctypes.c_double, ctypes.c_int, dp]
         L.orc_batch_tune.restype = ctypes.c_double
+        L.orc_local_search.argtypes = [dp, dp, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_int,
+                                       ctypes.c_double, ctypes.c_int]
+        L.orc_local_search.restype = ctypes.c_void_p
+        L.orc_relevant_features.argtypes = [dp, dp, ctypes.c_int, ctypes.c_int, ip]
         L.orc_batch_tune_full.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.c_int, dp, ctypes.c_int, ip, ip, dp,
                                           ctypes.POINTER(ctypes.c_void_p)]
         _lib = L
@@ -202,6 +206,27 @@ def fit(X, y, classification=False, max_fit_calls=100000, max_time=100000, compl
                                   float(complexity_penalty), int(max_complexity), float(sample_size),
                                   int(random_state)))
     return parse_fit_report(rep)
+
+
+def local_search(X, y, prefix, max_fit_calls=100000, complexity_penalty=0.001, max_complexity=50):
+    """One local_search of the oracle's search from `prefix` (every feature
+    allowed): the report of parse_fit_report."""
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    n, d = X.shape
+    rep = _take_str(lib().orc_local_search(_dp(X), _dp(y), n, d, prefix.encode(), int(max_fit_calls),
+                                           float(complexity_penalty), int(max_complexity)))
+    return parse_fit_report(rep)
+
+
+def relevant_features(X, y):
+    """Feature indices the oracle's search keeps (all when d <= 200)."""
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    n, d = X.shape
+    out = np.empty(max(d, 1), dtype=np.int32)
+    k = lib().orc_relevant_features(_dp(X), _dp(y), n, d, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    return out[:k].tolist()
 
 
 def parse_fit_report(rep):

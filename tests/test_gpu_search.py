@@ -133,3 +133,66 @@ def test_fp32_sweeps_accept_the_same_moves(case):
     st = rr.get_stats()
     print("fp32 sweeps: batched %d, re-scored in FP64 %d" % (st["batched_cands"], st["rescored_fp64"]), flush=True)
     assert st["batched_cands"] > 0
+
+
+@pytest.mark.parametrize("budget,calls,nmoves,model", [(10 ** 6, 37, 1, "x0"), (8, 8, 1, "x0"), (3, 4, 1, "x0"),
+                                                       (2, 2, 0, "2.500000")])
+def test_product_local_search_hand_trace(budget, calls, nmoves, model):
+    """The hand-traced local search of tests/test_search_pins.py on the GPU path
+    (x0 = [1, 2, 3, 4], y = x0, from the tree "1")."""
+    import rils_rols_cpp
+    from test_search_pins import MOVE, TRACE_X, TRACE_Y
+    print("case: local search hand trace budget=%d" % budget, flush=True)
+    rr = rils_rols_cpp.rils_rols(False, budget, BIG_TIME, 0.001, 50, 1.0, False, 0)
+    rr.local_search(TRACE_X.reshape(-1), TRACE_Y, 4, 1, "c0x1p+0")
+    assert rr.get_fit_calls() == calls
+    assert [tuple(m) for m in rr.get_moves()] == [MOVE] * nmoves
+    assert rr.get_model_string() == model
+
+
+def test_more_than_200_features_move_sequence_identical():
+    """d = 210 > max_feat: the search keeps the 200 features of highest
+    R2(x_i, y) (relevant_features, rils_rols_cpp.cpp:753-770) and runs the
+    oracle's moves."""
+    from test_search_pins import feature_filter_data
+    print("case: 210 features", flush=True)
+    X, y = feature_filter_data(n=300, noise=True)
+    want = O.fit(X, y, max_fit_calls=3000, max_time=BIG_TIME, random_state=4)
+    rr = run_product(X, y, max_fit_calls=3000, random_state=4)
+    assert_same_run(rr, want)
+
+
+@pytest.mark.parametrize("rows", [20000, 8000])
+def test_python_api_sample_size_auto(rows, capsys):
+    """sample_size=0 (rils_rols.py:50-88): above 10000 rows a 10% split is fit
+    with 1% of the budget and kept when its R2 and the whole set's R2 differ by
+    less than 0.01; the real fit then runs with the rest of the budget.  The
+    oracle, driven through the same decision, must end on the same model."""
+    from sklearn.metrics import r2_score
+    from sklearn.model_selection import train_test_split
+
+    from rils_rols import RILSROLSRegressor
+    print("case: sample_size=0 rows=%d" % rows, flush=True)
+    rng = np.random.default_rng(rows)
+    X = rng.uniform(-3, 3, (rows, 2))
+    y = 2.0 * np.sin(X[:, 0]) + X[:, 1] + rng.normal(0, 0.01, rows)
+    reg = RILSROLSRegressor(max_fit_calls=4000, max_time=10 ** 6, sample_size=0, random_state=9)
+    reg.fit(X, y)
+    out = capsys.readouterr().out
+    assert "Automatically tuning sample size:" in out
+    if rows <= 10000:
+        assert "Training size is smaller than 10000 so setting it to full sample 1 (100%)." in out
+        assert reg.sample_size == 1 and reg.max_fit_calls == 4000
+        want = O.fit(X, y, max_fit_calls=4000, max_time=BIG_TIME, random_state=9)
+    else:
+        Xp, _, yp, _ = train_test_split(X, y, train_size=0.1, random_state=9)
+        trial = O.fit(Xp, yp, max_fit_calls=40, max_time=BIG_TIME, random_state=9)
+        od_p, od_a = O.Dataset(Xp, yp), O.Dataset(X, y)
+        gap = abs(r2_score(y, od_a.eval(trial["prefix"])) - r2_score(yp, od_p.eval(trial["prefix"])))
+        share = 0.1 if gap < 0.01 else 1
+        assert "Setting sample_size=%s" % share in out
+        assert reg.sample_size == share and reg.max_fit_calls == 4000 - 40.0
+        want = O.fit(X, y, max_fit_calls=3960, max_time=BIG_TIME, random_state=9, sample_size=share)
+    assert reg.model == want["model"]
+    assert reg.fit_calls == want["fit_calls"]
+    assert "maxFitCalls=%s" % reg.max_fit_calls in reg.fit_report_string()
