@@ -121,6 +121,9 @@ typedef struct rr_batch {
 #define RR_F_NEEDS64 2   /* FP32 form could not certify this candidate: fitness
                             and coefficients (NaN) must come from RR_FP64 */
 #define RR_F_FP64 4      /* RR_FP32_CHECKED: this result came from the FP64 form */
+#define RR_F_EARLY 8     /* OLS mode: decided by the factor-0 pre-pass (factor 0
+                            NaN at one of the first 256 samples, sample 0 aside):
+                            the sentinel, no other pass over the samples */
 /* FP64 OLS form: bits 16..31 of flags hold the number of Householder steps
  * that ran passes over the samples (for traffic accounting) */
 #define RR_F_STEPS(flags) (((uint32_t)(flags)) >> 16)
@@ -213,7 +216,8 @@ int rr_predict_fp32(rr_ctx* ctx, int32_t handle, const uint32_t* code, int32_t l
  * independent workgroups, and the chunk sums are added in chunk order),
  * "fp32_streams" (1: the FP32 factor-count buckets run on concurrent streams;
  * default 0, back to back on the context's stream), "host_threads" (host
- * threads a call may use to check a large batch, 1..64; default 4).
+ * threads a call may use to check a large batch, 1..64; default 4),
+ * "phase_timing" (1: record rr_last_phase_ms; default 0).
  * Unknown names return RR_E_ARG. */
 int rr_set_option(rr_ctx* ctx, const char* name, int64_t value);
 
@@ -232,6 +236,20 @@ int rr_get_option(rr_ctx* ctx, const char* name, int64_t* value);
 /* Kernel time of the last rr_fitness_batch / rr_predict call on the context's
  * stream, in milliseconds (HIP events around the launches). */
 int rr_last_kernel_ms(rr_ctx* ctx, double* ms);
+
+/* The same split by phase (HIP events around each phase's launches on the
+ * stream it runs on), for roofline accounting of the dominant kernel; no
+ * counterpart in the reference.  Recorded only while option "phase_timing"
+ * is 1 (default 0); a phase that did not run reads 0.  With the FP64 form on
+ * a side stream (RR_FP32_CHECKED) phases overlap and their sum can exceed
+ * rr_last_kernel_ms. */
+#define RR_PHASE_PREPASS 0  /* k_nan0_fp64: factor 0 on the first 256 samples */
+#define RR_PHASE_EVAL 1     /* k_alive_fp64 + k_eval_fp64: the FP64 factor columns */
+#define RR_PHASE_QR 2       /* k_qr_*: the column-pivoted QR and the solve */
+#define RR_PHASE_SCORE 3    /* k_score_fp64 + k_fit_fp64: prediction, SSR, R2, RMSE */
+#define RR_PHASE_FP32 4     /* the FP32 form's kernels (k_acc_fp32 + k_fin_fp32, or the raw pair) */
+#define RR_PHASES 5
+int rr_last_phase_ms(rr_ctx* ctx, int32_t phase, double* ms);
 
 #ifdef __cplusplus
 }

@@ -254,6 +254,17 @@ class BatchRunner {
     return py::make_tuple(coefs, fit);
   }
   double last_kernel_ms() { double ms = 0; rr_last_kernel_ms(ctx_, &ms); return ms; }
+  // per-phase kernel time of the last run (option "phase_timing" must be 1)
+  py::dict last_phase_ms() {
+    static const char* names[RR_PHASES] = {"prepass", "eval", "qr", "score", "fp32"};
+    py::dict d;
+    for (int p = 0; p < RR_PHASES; ++p) {
+      double ms = 0;
+      check(rr_last_phase_ms(ctx_, p, &ms), "rr_last_phase_ms");
+      d[names[p]] = ms;
+    }
+    return d;
+  }
   // the autotuner on the prepared batch; returns the best kernel time and the chosen options
   py::dict autotune(const std::string& precision, int reps) {
     const int pr = batch_precision(precision);
@@ -358,6 +369,7 @@ PYBIND11_MODULE(rils_rols_cpp, m) {
       .def("select", &BatchRunner::select)
       .def("run_selected", &BatchRunner::run_selected, py::arg("precision") = "fp64")
       .def("last_kernel_ms", &BatchRunner::last_kernel_ms)
+      .def("last_phase_ms", &BatchRunner::last_phase_ms)
       .def("set_option", &BatchRunner::set_option)
       .def("autotune", &BatchRunner::autotune, py::arg("precision"), py::arg("reps") = 3)
       .def("get_option", &BatchRunner::get_option)
@@ -368,8 +380,9 @@ PYBIND11_MODULE(rils_rols_cpp, m) {
   py::class_<rrh::Server>(m, "Server",
                            "Request loop: many resident RILS-ROLS runs on one GPU, their fitness work merged into "
                            "shared launches as it arrives, runs admitted by HBM footprint")
-      .def(py::init<int, int64_t, int, int, int64_t>(), py::arg("device") = 0, py::arg("hbm_budget_mb") = 0,
-           py::arg("max_active") = 64, py::arg("host_threads_per_run") = 1, py::arg("slab_budget_mb") = 0)
+      .def(py::init<int, int64_t, int, int, int64_t, int>(), py::arg("device") = 0, py::arg("hbm_budget_mb") = 0,
+           py::arg("max_active") = 64, py::arg("host_threads_per_run") = 1, py::arg("slab_budget_mb") = 0,
+           py::arg("gather_us") = 0)
       .def(
           "submit",
           [](rrh::Server& self, py::array_t<double, py::array::c_style | py::array::forcecast> X,
@@ -419,6 +432,7 @@ PYBIND11_MODULE(rils_rols_cpp, m) {
         py::dict d;
         d["launches"] = c.launches; d["requests"] = c.requests; d["candidates"] = c.candidates;
         d["max_requests_per_launch"] = c.max_requests_per_launch; d["kernel_ms"] = c.kernel_ms; d["busy_s"] = c.busy_s;
+        d["gather_s"] = c.gather_s;
         return d;
       })
       .def_static("footprint", &rrh::Server::footprint);

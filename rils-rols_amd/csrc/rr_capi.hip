@@ -119,6 +119,13 @@ struct rr_ctx {
   int64_t fp32_streams = 0;   // 1: bucket kernels on concurrent side streams
   int64_t host_threads = 4;   // host threads for checking a large batch
   double last_ms = 0.0;
+  // phase timing (option "phase_timing"): event pairs handed out per call
+  int64_t phase_timing = 0;
+  struct PhaseMark { int phase; hipEvent_t a, b; };
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<PhaseMark> marks;
+  size_t ev_used = 0;
+  double phase_ms[RR_PHASES] = {0, 0, 0, 0, 0};
   std::vector<FacSlot> fac_table;   // kept across calls: never cleared, only re-stamped
   int32_t fac_gen = 0;
 };
@@ -129,6 +136,40 @@ int set_device(rr_ctx* ctx) {
   if (!ctx) return fail(RR_E_ARG, "null context");
   RR_HIP(hipSetDevice(ctx->device));
   return RR_OK;
+}
+
+// Phase timing: begin_phase records a start event on `stream` and returns
+// the mark's index (-1 when timing is off); end_phase records its end.
+hipEvent_t pool_event(rr_ctx* ctx) {
+  if (ctx->ev_used == ctx->ev_pool.size()) {
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    ctx->ev_pool.push_back(e);
+  }
+  return ctx->ev_pool[ctx->ev_used++];
+}
+
+int begin_phase(rr_ctx* ctx, int phase, hipStream_t stream) {
+  if (!ctx->phase_timing) return -1;
+  hipEvent_t a = pool_event(ctx), b = pool_event(ctx);
+  if (!a || !b || hipEventRecord(a, stream) != hipSuccess) return -1;
+  ctx->marks.push_back(rr_ctx::PhaseMark{phase, a, b});
+  return (int)ctx->marks.size() - 1;
+}
+
+void end_phase(rr_ctx* ctx, int mark, hipStream_t stream) {
+  if (mark >= 0) (void)hipEventRecord(ctx->marks[(size_t)mark].b, stream);
+}
+
+// after the call's streams have been synchronized
+void collect_phases(rr_ctx* ctx) {
+  for (double& v : ctx->phase_ms) v = 0.0;
+  for (const rr_ctx::PhaseMark& m : ctx->marks) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, m.a, m.b) == hipSuccess) ctx->phase_ms[m.phase] += ms;
+  }
+  ctx->marks.clear();
+  ctx->ev_used = 0;
 }
 
 Dataset* get_ds(rr_ctx* ctx, int32_t h) {
@@ -369,7 +410,7 @@ int check_batch(const rr_batch* b, int32_t mode, int32_t d, const int32_t* cand_
 
 // Copies `bytes` from host `src` into device buffer `dst` through pinned staging.
 int upload(rr_ctx* ctx, DevBuf& dst, const void* src, size_t bytes, size_t stage_off) {
-  if (dst.ensure(bytes) != RR_OK) return RR_E_NOMEM;
+  if (int rc = dst.ensure(bytes)) return rc;
   if (bytes == 0) return RR_OK;
   memcpy((char*)ctx->h_stage.p + stage_off, src, bytes);
   RR_HIP(hipMemcpyAsync(dst.p, (char*)ctx->h_stage.p + stage_off, bytes, hipMemcpyHostToDevice, ctx->stream));
@@ -435,6 +476,7 @@ int rr_destroy(rr_ctx* ctx) {
   for (DevBuf* b : bufs) b->release();
   ctx->h_stage.release();
   ctx->h_out.release();
+  for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
   (void)hipEventDestroy(ctx->ev0);
   (void)hipEventDestroy(ctx->ev1);
   (void)hipEventDestroy(ctx->fork);
@@ -1013,9 +1055,12 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
     }
     // which unique factors a live candidate uses (after the factor-0 pre-pass)
     const int32_t nu = chunks.back().p0 + chunks.back().npair;
+    int pm = begin_phase(ctx, RR_PHASE_EVAL, stream);
     RR_HIP(rr::launch_alive_fp64((const int32_t*)ctx->d_ustart.p, (const int32_t*)ctx->d_users.p, ddead, nu,
                                  (int32_t*)ctx->d_alive.p, stream));
+    end_phase(ctx, pm, stream);
     for (const Chunk64& ch : chunks) {
+      pm = begin_phase(ctx, RR_PHASE_EVAL, stream);
       if (ch.npair > 0) {
         rr::EvalLaunch E;
         E.prog_start = dp; E.code = dcode; E.K = dK; E.ds = dtab; E.cand_ds = dcds;
@@ -1029,6 +1074,7 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
         E.norm_part = (double*)ctx->d_norm_part.p; E.rows = (int)ctx->fp64_eval_rows;
         RR_HIP(rr::launch_eval_fp64(E, stream));
       }
+      end_phase(ctx, pm, stream);
       rr::QrLaunch Q{};
       Q.cand_start = dc; Q.prog_start = dp; Q.code = dcode; Q.ds = dtab; Q.cand_ds = dcds;
       Q.coefs = dcoef; Q.fit = dfit; Q.dead = ddead; Q.info = (int32_t*)ctx->d_info.p;
@@ -1039,9 +1085,11 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
       Q.prog_kept = (const int64_t*)ctx->d_prog_kept.p;
       Q.st = (rr::QrState*)ctx->d_qst.p; Q.phys = (int32_t*)ctx->d_qphys.p; Q.cold = (double*)ctx->d_qcold.p;
       Q.ks = ch.maxk; Q.part = (double*)ctx->d_qpart.p;
+      pm = begin_phase(ctx, RR_PHASE_QR, stream);
       RR_HIP(rr::launch_qr_init(Q, stream));
       for (size_t st = 0; st < ch.step_pos.size(); ++st) RR_HIP(rr::launch_qr_step(Q, ch.step_pos[st], ch.step_items[st], stream));
       RR_HIP(rr::launch_qr_solve(Q, stream));
+      end_phase(ctx, pm, stream);
       rr::ScoreLaunch Sc{};
       Sc.cand_start = dc; Sc.prog_start = dp; Sc.code = dcode; Sc.K = dK; Sc.ds = dtab; Sc.cand_ds = dcds;
       Sc.coefs = dcoef; Sc.slab = (const double*)ctx->d_slab.p; Sc.prog_kept = (const int64_t*)ctx->d_prog_kept.p;
@@ -1049,7 +1097,9 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
       Sc.pos_item = (const int32_t*)ctx->d_pos_item.p + ch.c0; Sc.n_pos = ch.c1 - ch.c0; Sc.n_items = ch.sitem;
       Sc.items_per_pos = ch.sper; Sc.raw = 0; Sc.rows = (int)ctx->fp64_eval_rows;
       Sc.part = (double*)ctx->d_score_part.p; Sc.fit = dfit;
+      pm = begin_phase(ctx, RR_PHASE_SCORE, stream);
       RR_HIP(rr::launch_score_fp64(Sc, stream));
+      end_phase(ctx, pm, stream);
     }
     return RR_OK;
   };
@@ -1058,6 +1108,7 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
   // with option fp32_streams bucket 0 there and the others on side streams)
   auto fp32_run = [&]() -> int {
     if (n32 == 0) return RR_OK;
+    const int pm = begin_phase(ctx, RR_PHASE_FP32, ctx->stream);
     const bool side = ctx->fp32_streams != 0 && mode == RR_MODE_OLS;
     hipStream_t st[rr::FP32_BUCKETS];
     st[0] = ctx->stream;
@@ -1073,6 +1124,7 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
         RR_HIP(hipStreamWaitEvent(ctx->stream, ctx->join[i], 0));
       }
     }
+    end_phase(ctx, pm, ctx->stream);
     return RR_OK;
   };
 
@@ -1101,14 +1153,20 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
   }
   if (checked && !list64.empty()) { if ((rc = plan64(&list64, 0, plan))) return rc; }
   // the timed region starts with every input uploaded
+  ctx->marks.clear();
+  ctx->ev_used = 0;
   RR_HIP(hipEventRecord(ctx->ev0, ctx->stream));
   if (mode == RR_MODE_OLS && nc > 0) {
     // candidates whose factor 0 is NaN at an early row >= 1: outcome written here, skipped below
     rr::Nan0Launch L0{dc, dp, dcode, dK, dtab, dcds, dcoef, dfit, (int32_t*)ctx->d_dead.p, nc};
+    const int pm = begin_phase(ctx, RR_PHASE_PREPASS, ctx->stream);
     RR_HIP(rr::launch_nan0_fp64(L0, ctx->stream));
+    end_phase(ctx, pm, ctx->stream);
   }
   if (mode == RR_MODE_RAW && precision == RR_FP64) {
+    const int pm = begin_phase(ctx, RR_PHASE_SCORE, ctx->stream);
     RR_HIP(rr::launch_score_fp64(raw64, ctx->stream));
+    end_phase(ctx, pm, ctx->stream);
   } else if (precision == RR_FP64) {
     if ((rc = launch64(plan, ctx->stream))) return rc;
   } else if (!checked) {
@@ -1144,6 +1202,8 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
   float ms = 0.f;
   RR_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
   ctx->last_ms = ms;
+  if (!ctx->marks.empty()) RR_HIP(hipStreamSynchronize(ctx->side64));
+  collect_phases(ctx);
   return RR_OK;
 }
 
@@ -1278,6 +1338,9 @@ int rr_set_option(rr_ctx* ctx, const char* name, int64_t value) {
   } else if (s == "host_threads") {
     if (value < 1 || value > 64) return fail(RR_E_ARG, "host_threads must be in [1, 64]");
     ctx->host_threads = value;
+  } else if (s == "phase_timing") {
+    if (value != 0 && value != 1) return fail(RR_E_ARG, "phase_timing must be 0 or 1");
+    ctx->phase_timing = value;
   } else if (s == "fp32_chunk_rows") {
     if (value < 512 || value > (int64_t)1 << 30 || value % 512) return fail(RR_E_ARG, "fp32_chunk_rows must be a multiple of 512 in [512, 2^30]");
     ctx->fp32_chunk_rows = value;
@@ -1294,6 +1357,7 @@ int rr_get_option(rr_ctx* ctx, const char* name, int64_t* value) {
   else if (s == "fp32_chunk_rows") *value = ctx->fp32_chunk_rows;
   else if (s == "fp32_streams") *value = ctx->fp32_streams;
   else if (s == "host_threads") *value = ctx->host_threads;
+  else if (s == "phase_timing") *value = ctx->phase_timing;
   else return fail(RR_E_ARG, "unknown option " + s);
   return RR_OK;
 }
@@ -1301,6 +1365,13 @@ int rr_get_option(rr_ctx* ctx, const char* name, int64_t* value) {
 int rr_last_kernel_ms(rr_ctx* ctx, double* ms) {
   if (!ctx || !ms) return fail(RR_E_ARG, "null argument");
   *ms = ctx->last_ms;
+  return RR_OK;
+}
+
+int rr_last_phase_ms(rr_ctx* ctx, int32_t phase, double* ms) {
+  if (!ctx || !ms) return fail(RR_E_ARG, "null argument");
+  if (phase < 0 || phase >= RR_PHASES) return fail(RR_E_ARG, "bad phase");
+  *ms = ctx->phase_ms[phase];
   return RR_OK;
 }
 

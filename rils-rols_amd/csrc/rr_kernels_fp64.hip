@@ -977,6 +977,7 @@ __global__ __launch_bounds__(NAN0_ROWS) void k_nan0_fp64(Nan0Args a) {
       double* co = a.coefs + (int64_t)pbeg + cand;
       for (int i = 0; i < k; ++i) co[i] = (i < size0) ? rrm::qnan() : 0.0;
       write_fit(a.fit + cand, rrm::qnan(), D.sst, n, size0, 0);
+      a.fit[cand].flags |= RR_F_EARLY;
     }
   }
 }

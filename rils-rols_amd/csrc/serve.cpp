@@ -63,7 +63,7 @@ struct Dispatcher::Req {
   bool done = false;
 };
 
-Dispatcher::Dispatcher(int device, int64_t slab_budget_mb) {
+Dispatcher::Dispatcher(int device, int64_t slab_budget_mb, int gather_us) : gather_us_(gather_us > 0 ? gather_us : 0) {
   th_ = std::thread([this, device, slab_budget_mb] {
     std::string err;
     if (rr_create(device, &ctx_) != RR_OK) err = std::string("rr_create: ") + rr_last_error();
@@ -104,19 +104,44 @@ Dispatcher::Counters Dispatcher::counters() {
 void Dispatcher::loop() {
   for (;;) {
     std::vector<Req*> batch;
+    double gathered = 0;
     {
       std::unique_lock<std::mutex> lk(m_);
       cv_.wait(lk, [this] { return stop_ || !q_.empty(); });
       if (q_.empty() && stop_) return;
+      // Continuous batching: a run whose sweep is not pending yet is on its
+      // host work between sweeps; when the pending fitness requests are fewer
+      // than the runs searching, give the others up to gather_us to join this
+      // launch (uploads and releases never wait).
+      auto fitness_pending = [this] {
+        int f = 0;
+        for (const Req* r : q_) {
+          if (r->kind != Req::FITNESS) return -1;   // an upload or release: go now
+          ++f;
+        }
+        return f;
+      };
+      const int f0 = fitness_pending();
+      if (gather_us_ > 0 && f0 > 0 && f0 < live_ && !stop_) {
+        const auto t0 = std::chrono::steady_clock::now();
+        cv_.wait_until(lk, t0 + std::chrono::microseconds(gather_us_), [&] {
+          const int f = fitness_pending();
+          return stop_ || f < 0 || f >= live_;
+        });
+        gathered = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      }
       batch.assign(q_.begin(), q_.end());   // everything pending now goes into this round
       q_.clear();
     }
     const auto t0 = std::chrono::steady_clock::now();
+    int dlive = 0;
     for (Req* r : batch) {
       if (r->kind == Req::UPLOAD) {
         if (rr_dataset_upload(ctx_, r->X, r->y, r->n, r->d, &r->handle) != RR_OK) r->error = rr_last_error();
+        else ++dlive;
       } else if (r->kind == Req::RELEASE) {
         rr_dataset_free(ctx_, r->handle);
+        --dlive;
       }
     }
     // fitness requests grouped by (mode, precision), in arrival order
@@ -133,6 +158,8 @@ void Dispatcher::loop() {
     {
       std::lock_guard<std::mutex> g(m_);
       ctr_.busy_s += busy;
+      ctr_.gather_s += gathered;
+      live_ += dlive;
       for (Req* r : batch) r->done = true;
     }
     cv_.notify_all();
@@ -241,8 +268,9 @@ struct Server::Run {
   std::thread th;
 };
 
-Server::Server(int device, int64_t hbm_budget_mb, int max_active, int host_threads_per_run, int64_t slab_budget_mb)
-    : disp_(new Dispatcher(device, slab_budget_mb)),
+Server::Server(int device, int64_t hbm_budget_mb, int max_active, int host_threads_per_run, int64_t slab_budget_mb,
+               int gather_us)
+    : disp_(new Dispatcher(device, slab_budget_mb, gather_us)),
       budget_((hbm_budget_mb > 0 ? hbm_budget_mb : 200 * 1024) << 20),
       max_active_(max_active > 0 ? max_active : 1),
       threads_per_run_(host_threads_per_run > 0 ? host_threads_per_run : 1),

@@ -45,7 +45,10 @@ void merge_batches(const std::vector<BatchView>& parts, std::vector<int32_t>& ca
 
 class Dispatcher : public FitnessBackend {
  public:
-  Dispatcher(int device, int64_t slab_budget_mb);
+  // gather_us: when fewer fitness requests are pending than runs hold a
+  // resident dataset, wait up to this long for more before launching (0:
+  // launch whatever is pending at once)
+  Dispatcher(int device, int64_t slab_budget_mb, int gather_us = 0);
   ~Dispatcher() override;
   int32_t upload(const double* X_fm, const double* y, int64_t n, int32_t d) override;
   void release(int32_t handle) override;
@@ -55,7 +58,7 @@ class Dispatcher : public FitnessBackend {
 
   struct Counters {
     int64_t launches = 0, requests = 0, candidates = 0, max_requests_per_launch = 0;
-    double kernel_ms = 0, busy_s = 0;
+    double kernel_ms = 0, busy_s = 0, gather_s = 0;
   };
   Counters counters();
 
@@ -70,6 +73,8 @@ class Dispatcher : public FitnessBackend {
   bool stop_ = false;
   std::string init_error_;
   bool ready_ = false;
+  int gather_us_ = 0;
+  int live_ = 0;   // resident datasets: one per run that is searching
   Counters ctr_;
   std::thread th_;
 };
@@ -101,7 +106,8 @@ struct RunResult {
 
 class Server {
  public:
-  Server(int device, int64_t hbm_budget_mb, int max_active, int host_threads_per_run, int64_t slab_budget_mb);
+  Server(int device, int64_t hbm_budget_mb, int max_active, int host_threads_per_run, int64_t slab_budget_mb,
+         int gather_us = 0);
   ~Server();
   int submit(RunSpec spec);
   bool done(int id);

@@ -74,6 +74,7 @@ def lib():
         L.rr_set_option.argtypes = [vp, ctypes.c_char_p, ctypes.c_int64]
         L.rr_get_option.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64)]
         L.rr_last_kernel_ms.argtypes = [vp, dp]
+        L.rr_last_phase_ms.argtypes = [vp, ctypes.c_int32, dp]
         L.rr_check_batch.argtypes = [ctypes.POINTER(Batch), ctypes.c_int32, ctypes.c_int32]
         _lib = L
     return _lib
@@ -269,6 +270,15 @@ class Context:
         check(lib().rr_last_kernel_ms(self.p, ctypes.byref(v)))
         return v.value
 
+    def phase_ms(self):
+        """rr_last_phase_ms for every phase, by name."""
+        out = {}
+        for name in ("PREPASS", "EVAL", "QR", "SCORE", "FP32"):
+            v = ctypes.c_double()
+            check(lib().rr_last_phase_ms(self.p, D["RR_PHASE_" + name], ctypes.byref(v)))
+            out[name.lower()] = v.value
+        return out
+
     def fitness(self, h, arrs, mode="ols", precision="fp64", handles=None):
         """Score a batch against dataset h, or candidate c against handles[c]
         (rr_fitness_multi) when handles is given."""
@@ -286,6 +296,10 @@ class Context:
             hs = np.ascontiguousarray(handles, dtype=np.int32)
             check(lib().rr_fitness_multi(self.p, hs.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), ctypes.byref(b), m,
                                          pr, _dp(coefs), fit.ctypes.data_as(ctypes.c_void_p)))
+        return self._split(arrs, coefs, fit)
+
+    def _split(self, arrs, coefs, fit):
+        nc = arrs["n_cands"]
         per = []
         cs = arrs["cand_start"]
         for c in range(nc):

@@ -10,7 +10,7 @@ import os
 import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(ROOT, "oracle", "_build", "liborc.so")
+LIB_PATH = os.environ.get("RR_ORACLE_LIB") or os.path.join(ROOT, "oracle", "_build", "liborc.so")
 
 _lib = None
 

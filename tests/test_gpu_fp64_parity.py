@@ -302,3 +302,40 @@ def test_sparse_columns_and_tiny_n_bitexact(ctx, n):
         assert same_bits(c, tu["coefs"]), (t, c, tu["coefs"])
         assert f["rank"] == tu["rank"], t
         assert same_bits([f["e0"], f["e1"]], [tu["fit"][0], tu["fit"][1]]), (t, f, tu["fit"])
+
+
+def test_phase_timing_and_early_flag(ctx):
+    """Option phase_timing splits the kernel time of a call by phase; the
+    phases of the plain FP64 form (one stream) add up to no more than the
+    call's kernel time.  Candidates the factor-0 pre-pass settles carry
+    RR_F_EARLY and are sentinels; no other candidate carries it."""
+    print("case: phase timing", flush=True)
+    X, y = make_data(81, 20000, 3)
+    trees = treegen.random_trees(81, 200, 3, 30, 3) + ["ln x0", "+ sqrt x1 x2"]
+    od = O.Dataset(X, y)
+    tuned = [od.tune(t) for t in trees]
+    arrs = capi.build_batch([tu["factors"] for tu in tuned])
+    h = ctx.upload(X, y)
+    ctx.set_option("phase_timing", 1)
+    try:
+        coefs, fit = ctx.fitness(h, arrs)
+        ph = ctx.phase_ms()
+        total = ctx.kernel_ms()
+        _, fit32 = ctx.fitness(h, arrs, precision="fp32")
+        ph32 = ctx.phase_ms()
+    finally:
+        ctx.set_option("phase_timing", 0)
+    ctx.free(h)
+    print("phases:", ph, "total", total, "fp32:", ph32, flush=True)
+    for k in ("prepass", "eval", "qr", "score"):
+        assert ph[k] > 0, k
+    assert ph["fp32"] == 0
+    assert sum(ph.values()) <= total * 1.05 + 0.01
+    assert ph32["fp32"] > 0 and ph32["prepass"] > 0
+    early = (fit["flags"] & capi.D["RR_F_EARLY"]) != 0
+    assert early.any() and np.all(fit["flags"][early] & 1)
+    assert np.array_equal(early, (fit32["flags"] & capi.D["RR_F_EARLY"]) != 0)
+    ln_x0 = len(trees) - 2
+    assert early[ln_x0]     # ln of a negative x0 within the first rows
+    for t, tu, c, f in zip(trees, tuned, coefs, fit):   # the flag does not change any result
+        assert same_bits(c, tu["coefs"]) and same_bits([f["e0"], f["e1"]], [tu["fit"][0], tu["fit"][1]]), t
