@@ -192,6 +192,24 @@ int rr_check_batch(const rr_batch* batch, int32_t mode, int32_t n_features);
 int rr_fitness_multi(rr_ctx* ctx, const int32_t* handles, const rr_batch* batch, int32_t mode, int32_t precision,
                      double* coefs_out, rr_fit* fit_out);
 
+/* The plain FP64 form of rr_fitness_batch (RR_MODE_OLS or RR_MODE_RAW, one
+ * dataset) with no device or pinned-host allocation: no counterpart in the
+ * reference; it is for callers that cannot allocate (code that must not call
+ * hipMalloc, such as serving loops or harnesses that forbid it).  Every work
+ * buffer is carved from one fixed device array of RR_NOALLOC_SCRATCH_BYTES
+ * per GPU, resident from the library's first use on that GPU until the
+ * process ends; the FP64 work columns run in slices that fit it, and a batch
+ * that still needs more fails with RR_E_LIMIT before anything is launched
+ * that depends on the missing memory.  Host-to-device copies come straight
+ * from the caller's buffers (synchronously).  Every call on one GPU shares
+ * that GPU's array and leaves its contents behind: calls that overlap on one
+ * GPU (from any context or thread) race and must be serialized by the
+ * caller.  Results are bit-for-bit those of rr_fitness_batch(RR_FP64). */
+#define RR_NOALLOC_SCRATCH_MB 256
+#define RR_NOALLOC_SCRATCH_BYTES (RR_NOALLOC_SCRATCH_MB * 1048576u)
+int rr_fitness_batch_noalloc(rr_ctx* ctx, int32_t handle, const rr_batch* batch, int32_t mode, double* coefs_out,
+                             rr_fit* fit_out);
+
 /* Evaluate one program over all samples of the dataset (node::evaluate_all,
  * used by rils_rols::predict, rils_rols_cpp.cpp:730-750). */
 int rr_predict(rr_ctx* ctx, int32_t handle, const uint32_t* code, int32_t len, const double* consts,

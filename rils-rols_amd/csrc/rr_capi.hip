@@ -20,9 +20,21 @@
 #include "rr_kernels_fp32.h"
 #include "rr_device.h"
 
+// Work memory of rr_fitness_batch_noalloc: one array per GPU, resident from
+// the code object's load on that GPU until the process ends.  Host code must
+// never pass `scratch` itself to a kernel or a copy (the name is a host-side
+// stand-in); it uses the device address that hipGetSymbolAddress returns for
+// the current device (scratch_address below).
+static __device__ __align__(256) unsigned char scratch[RR_NOALLOC_SCRATCH_BYTES];
+
 namespace {
 
 thread_local std::string g_err;
+
+// per-device address of `scratch`, resolved once per device with that device
+// current (rr_create); null until then
+std::mutex g_scratch_mu;
+std::vector<void*> g_scratch_ptr;
 
 int fail(int code, const std::string& msg) {
   g_err = msg;
@@ -35,11 +47,32 @@ int fail(int code, const std::string& msg) {
     if (e_ != hipSuccess) return fail(RR_E_HIP, std::string(#call ": ") + hipGetErrorString(e_)); \
   } while (0)
 
+// Bump allocator over the file-scope scratch array, active on the calling
+// thread during rr_fitness_batch_noalloc only: every DevBuf the call needs is
+// carved from it instead of allocated.
+struct Arena {
+  char* base = nullptr;
+  size_t size = 0, used = 0;
+  bool overflow = false;
+};
+thread_local Arena* t_arena = nullptr;
+
 struct DevBuf {
   void* p = nullptr;
   size_t cap = 0;
   int ensure(size_t bytes) {
     if (bytes <= cap) return RR_OK;
+    if (t_arena) {
+      const size_t off = (t_arena->used + 255) & ~size_t(255);
+      if (off > t_arena->size || bytes > t_arena->size - off) {
+        t_arena->overflow = true;
+        return fail(RR_E_LIMIT, "the call needs more device memory than the no-alloc scratch array holds");
+      }
+      p = t_arena->base + off;
+      cap = bytes;
+      t_arena->used = off + bytes;
+      return RR_OK;
+    }
     if (p) { (void)hipFree(p); p = nullptr; cap = 0; }
     size_t want = std::max<size_t>(bytes + bytes / 2, 4096);
     hipError_t e = hipMalloc(&p, want);
@@ -118,6 +151,7 @@ struct rr_ctx {
   int64_t fp32_chunk_rows = 4096;
   int64_t fp32_streams = 0;   // 1: bucket kernels on concurrent side streams
   int64_t host_threads = 4;   // host threads for checking a large batch
+  void* scratch = nullptr;    // device address of this device's scratch array
   double last_ms = 0.0;
   // phase timing (option "phase_timing"): event pairs handed out per call
   int64_t phase_timing = 0;
@@ -412,6 +446,11 @@ int check_batch(const rr_batch* b, int32_t mode, int32_t d, const int32_t* cand_
 int upload(rr_ctx* ctx, DevBuf& dst, const void* src, size_t bytes, size_t stage_off) {
   if (int rc = dst.ensure(bytes)) return rc;
   if (bytes == 0) return RR_OK;
+  if (t_arena) {   // no pinned staging: copy from the source itself, which may go out of scope after
+    RR_HIP(hipMemcpyAsync(dst.p, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    RR_HIP(hipStreamSynchronize(ctx->stream));
+    return RR_OK;
+  }
   memcpy((char*)ctx->h_stage.p + stage_off, src, bytes);
   RR_HIP(hipMemcpyAsync(dst.p, (char*)ctx->h_stage.p + stage_off, bytes, hipMemcpyHostToDevice, ctx->stream));
   return RR_OK;
@@ -439,8 +478,22 @@ int rr_create(int device, rr_ctx** out) {
   if (e != hipSuccess || count == 0) return fail(RR_E_HIP, "no HIP device available");
   if (device < 0 || device >= count) return fail(RR_E_ARG, "device index out of range");
   RR_HIP(hipSetDevice(device));
+  void* scr = nullptr;
+  {
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if ((int)g_scratch_ptr.size() < count) g_scratch_ptr.resize(count, nullptr);
+    if (!g_scratch_ptr[device]) {
+      void* ptr = nullptr;
+      const hipError_t se = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+      if (se != hipSuccess || !ptr)
+        return fail(RR_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(se));
+      g_scratch_ptr[device] = ptr;
+    }
+    scr = g_scratch_ptr[device];
+  }
   rr_ctx* c = new rr_ctx;
   c->device = device;
+  c->scratch = scr;
   bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess &&
             hipEventCreate(&c->ev0) == hipSuccess && hipEventCreate(&c->ev1) == hipSuccess &&
             hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) == hipSuccess &&
@@ -697,7 +750,7 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
                        align_up((size_t)(nc + 1) * 4) +       // the scoring work list
                        2 * align_up((size_t)(np + 1) * 8) +   // kept-column offsets per program and per unique factor
                        2 * align_up((size_t)(np + 1) * 4);    // the users of each unique factor
-  if (ctx->h_stage.ensure(s_cand + s_prog + s_code + s_K + s_off + s_tab + s_cds + s_chunks + 2 * s_f64) != RR_OK)
+  if (!t_arena && ctx->h_stage.ensure(s_cand + s_prog + s_code + s_K + s_off + s_tab + s_cds + s_chunks + 2 * s_f64) != RR_OK)
     return RR_E_NOMEM;
   size_t so = 0;
   if ((rc = upload(ctx, ctx->d_cand, b->cand_start, (size_t)(nc + 1) * 4, so))) return rc;
@@ -843,7 +896,10 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
     auto is_bare = [&](int32_t p) {
       return b->prog_start[p + 1] - b->prog_start[p] == 1 && (b->code[b->prog_start[p]] & 0xffu) == RR_OP_LDV;
     };
-    const int64_t budget = std::max<int64_t>(ctx->slab_budget_mb, 1) << 20;
+    // no-alloc calls: half of what is left of the scratch array (the rest
+    // holds the per-candidate and per-chunk buffers carved after the slab)
+    const int64_t budget = t_arena ? (int64_t)std::max<size_t>((t_arena->size - std::min(t_arena->used, t_arena->size)) / 2, 1)
+                                   : std::max<int64_t>(ctx->slab_budget_mb, 1) << 20;
     int64_t max_words = 0;
     // per chunk, the unique factors by hash: open addressing, linear probing;
     // an entry belongs to the current chunk when its gen matches, so the
@@ -1226,6 +1282,42 @@ int rr_fitness_multi(rr_ctx* ctx, const int32_t* handles, const rr_batch* b, int
     return RR_OK;
   }
   return fitness_impl(ctx, -1, handles, b, mode, precision, coefs_out, fit_out);
+}
+
+int rr_fitness_batch_noalloc(rr_ctx* ctx, int32_t handle, const rr_batch* b, int32_t mode, double* coefs_out,
+                             rr_fit* fit_out) {
+  if (!ctx) return fail(RR_E_ARG, "null context");
+  if (!ctx->scratch) return fail(RR_E_HIP, "no scratch array address for this device");
+  // The call runs with every work buffer carved from the scratch array; the
+  // context's own buffers are set aside and put back afterwards, untouched.
+  DevBuf* bufs[] = {&ctx->d_cand, &ctx->d_prog, &ctx->d_code, &ctx->d_K, &ctx->d_slab, &ctx->d_order32,
+                    &ctx->d_coefs, &ctx->d_fit, &ctx->d_misc, &ctx->d_pred_e, &ctx->d_dstab, &ctx->d_cand_ds,
+                    &ctx->d_blk_cand, &ctx->d_blk_chunk, &ctx->d_chunk_base, &ctx->d_work, &ctx->d_work_flags,
+                    &ctx->d_f64_off, &ctx->d_f64_ord, &ctx->d_dead, &ctx->d_fac_prog, &ctx->d_fac_cand,
+                    &ctx->d_fac_item, &ctx->d_prog_item, &ctx->d_norm_part, &ctx->d_info, &ctx->d_pos_item,
+                    &ctx->d_score_part, &ctx->d_qst, &ctx->d_qphys, &ctx->d_qcold, &ctx->d_qpart,
+                    &ctx->d_prog_kept, &ctx->d_fac_kept, &ctx->d_ustart, &ctx->d_users, &ctx->d_alive};
+  constexpr size_t NB = sizeof(bufs) / sizeof(bufs[0]);
+  DevBuf saved[NB];
+  for (size_t i = 0; i < NB; ++i) { saved[i] = *bufs[i]; *bufs[i] = DevBuf(); }
+  const int64_t timing = ctx->phase_timing;
+  ctx->phase_timing = 0;   // phase events would be created on first use
+  Arena a;
+  a.base = (char*)ctx->scratch;
+  a.size = (size_t)RR_NOALLOC_SCRATCH_BYTES;
+  t_arena = &a;
+  int rc = fitness_impl(ctx, handle, nullptr, b, mode, RR_FP64, coefs_out, fit_out);
+  t_arena = nullptr;
+  ctx->phase_timing = timing;
+  for (size_t i = 0; i < NB; ++i) *bufs[i] = saved[i];
+  if (rc != RR_OK && a.overflow) {
+    char m[160];
+    snprintf(m, sizeof m, "rr_fitness_batch_noalloc: the batch needs more than the %d MiB scratch array "
+             "(fewer candidates or samples per call)", RR_NOALLOC_SCRATCH_MB);
+    g_err = m;
+    rc = RR_E_LIMIT;
+  }
+  return rc;
 }
 
 int rr_predict(rr_ctx* ctx, int32_t handle, const uint32_t* code, int32_t len, const double* consts, int32_t n_consts,

@@ -75,6 +75,7 @@ def lib():
         L.rr_get_option.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64)]
         L.rr_last_kernel_ms.argtypes = [vp, dp]
         L.rr_last_phase_ms.argtypes = [vp, ctypes.c_int32, dp]
+        L.rr_fitness_batch_noalloc.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(Batch), ctypes.c_int32, dp, vp]
         L.rr_check_batch.argtypes = [ctypes.POINTER(Batch), ctypes.c_int32, ctypes.c_int32]
         _lib = L
     return _lib
@@ -296,6 +297,16 @@ class Context:
             hs = np.ascontiguousarray(handles, dtype=np.int32)
             check(lib().rr_fitness_multi(self.p, hs.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), ctypes.byref(b), m,
                                          pr, _dp(coefs), fit.ctypes.data_as(ctypes.c_void_p)))
+        return self._split(arrs, coefs, fit)
+
+    def fitness_noalloc(self, h, arrs, mode="ols"):
+        """rr_fitness_batch_noalloc (plain FP64, no allocation)."""
+        m = D["RR_MODE_OLS"] if mode == "ols" else D["RR_MODE_RAW"]
+        nc = arrs["n_cands"]
+        coefs = np.zeros(max(int(arrs["cand_start"][-1]) + nc, 1))
+        fit = np.zeros(max(nc, 1), dtype=FIT_DTYPE)
+        check(lib().rr_fitness_batch_noalloc(self.p, h, ctypes.byref(_as_batch(arrs)), m, _dp(coefs),
+                                             fit.ctypes.data_as(ctypes.c_void_p)))
         return self._split(arrs, coefs, fit)
 
     def _split(self, arrs, coefs, fit):
