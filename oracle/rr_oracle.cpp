@@ -80,6 +80,7 @@
 #include <thread>
 #include <chrono>
 #include <climits>
+#include <limits>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -487,11 +488,246 @@ static std::vector<double> evaluate(const Node& nd, const Data& D) {
   return out;
 }
 
+// ------------------------------------------------------------ chunked column-pivoted QR
+// The canonical least-squares algorithm for up to TS_MAX_COLS columns: the
+// column-pivoted Householder QR of ColPivHouseholderQR.h:482-607 carried out
+// on the triangular factor of the data instead of on the n x k data itself.
+//   1. Initial norms: ||A_j|| in the chunked order (rrm::chunk_sum), as
+//      below.  A column whose norm is Inf is pivoted at Eigen's step 0 (or a
+//      NaN-norm column at position 0 is), whose reflector is NaN: the outcome
+//      is all-NaN coefficients.  A NaN-norm column never moves (a NaN norm
+//      never compares greater, and a swap moves only the column at the step's
+//      position), so it is pivoted, with the same outcome, exactly when the
+//      factorisation reaches its position before the rank cut; otherwise its
+//      coefficient is zero.  The finite columns take part in steps 2-4.
+//   2. Per TS_ROWS-row chunk of [finite columns | y]: unpivoted Householder
+//      QR (Eigen's makeHouseholder), its triangular factor R_c.  Every sum in
+//      a chunk is ts_dot's order.  A reflector (tau, essential part x / den)
+//      is applied to a column t as w = t_j + (x . t) / den, t_j -= tau w,
+//      t_i -= x_i g with g = (tau w) / den for the rows below: one division
+//      per column, and every product-sum and update a fused multiply-add.
+//   3. The chunk factors combined in chunk order: R = R_0, then R = the
+//      triangular factor of [R; R_c] (Householder on each column of the
+//      stacked pair, sums in row order, the same update).
+//   4. Eigen's pivoted steps on R: pivot = largest current norm (left to
+//      right, strict >), rank cut at biggest^2 < (max initial norm * eps)^2 /
+//      n * (n - p), makeHouseholder and its application to the remaining
+//      columns and to R's y column; current norms recomputed from R's rows
+//      below the step (in row order) instead of downdated.  Then the
+//      triangular solve of the leading r x r block, zero for the rest.
+// In exact arithmetic every pivot, cut and coefficient is the one of the
+// pivoted QR on the data; rounding is this order's.
+constexpr int64_t TS_ROWS = 4096;      // rows per chunk
+constexpr int TS_MAX_COLS = 9;         // columns (free term included) handled this way
+
+// The column bucket of a candidate with k columns: columns with y of its
+// chunk factors (3, 5 or 10), and the threads of its chunk sums: 256 threads
+// of 16 rows for the two small buckets, 1024 of 4 rows for the wide one.
+static int ts_bucket(int k) { return k <= 2 ? 3 : k <= 4 ? 5 : 10; }
+static int ts_threads(int nb) { return nb <= 5 ? 256 : 1024; }
+
+// Sum of x[i] y[i] over the chunk-local rows lo <= i < m (<= TS_ROWS) with T
+// threads: per thread t its rows t + T u (u = 0 .. TS_ROWS/T - 1) in order,
+// each accumulated by a fused multiply-add, each group of 64 threads combined
+// by the xor butterfly of strides 1, 2, 4, ..., 32 (lane l adds lane l ^ s),
+// the T/64 group sums in order.
+static double ts_dot(int T, int64_t m, int64_t lo, const double* x, const double* y) {
+  double total = 0.0;
+  for (int w = 0; w < T / 64; ++w) {
+    double a[64];
+    for (int l = 0; l < 64; ++l) {
+      const int t = 64 * w + l;
+      double p = 0.0;
+      for (int u = 0; u < (int)(TS_ROWS / T); ++u) {
+        const int64_t r = t + (int64_t)T * u;
+        if (r >= lo && r < m) p = std::fma(x[r], y[r], p);
+      }
+      a[l] = p;
+    }
+    for (int s = 1; s <= 32; s <<= 1) {
+      double nx[64];
+      for (int l = 0; l < 64; ++l) nx[l] = a[l] + a[l ^ s];
+      for (int l = 0; l < 64; ++l) a[l] = nx[l];
+    }
+    total = (w == 0) ? a[0] : total + a[0];
+  }
+  return total;
+}
+
+// Eigen's makeHouseholder from the pivot c0 and the tail's sum of squares.
+static void householder(double c0, double tail_sq, double* beta, double* tau, double* den) {
+  if (tail_sq <= 2.2250738585072014e-308) {
+    *tau = 0.0; *beta = c0; *den = 1.0;
+  } else {
+    double b = std::sqrt(c0 * c0 + tail_sq);
+    if (c0 >= 0.0) b = -b;
+    *beta = b;
+    *den = c0 - b;
+    *tau = (b - c0) / b;
+  }
+}
+
+static std::vector<double> ts_qr_solve(const std::vector<std::vector<double>>& A, const std::vector<double>& b,
+                                       int64_t n, int* rank_out) {
+  const int k = (int)A.size();
+  const double eps = 2.220446049250313e-16;
+  const int size = (int)std::min<int64_t>(n, k);
+  std::vector<double> nd(k);
+  for (int j = 0; j < k; ++j) {
+    const std::vector<double>& a = A[j];
+    nd[j] = std::sqrt(rrm::chunk_sum(0, n, [&](int64_t i) { return a[i] * a[i]; }));
+  }
+  std::vector<double> coefs(k, 0.0);
+  const double inf = std::numeric_limits<double>::infinity();
+  std::vector<int> fin;   // finite columns in column order
+  for (int j = 0; j < k; ++j)
+    if (nd[j] == nd[j] && nd[j] != inf) fin.push_back(j);
+  std::vector<int> perm(k);   // position -> column
+  std::iota(perm.begin(), perm.end(), 0);
+  // a reflector from a non-finite column is NaN; nothing is cut after it and
+  // every coefficient of the first `size` positions is NaN
+  auto all_nan = [&]() {
+    std::vector<double> out(k, 0.0);
+    for (int i = 0; i < size; ++i) out[perm[i]] = rrm::qnan();
+    *rank_out = size;
+    return out;
+  };
+  {   // step 0's pivot: non-finite when column 0's norm is NaN or any norm is Inf
+    int bi = 0;
+    double best = nd[0];
+    for (int j = 1; j < k; ++j)
+      if (nd[j] > best) { best = nd[j]; bi = j; }
+    if (!(best == best) || best == inf) {
+      std::swap(perm[0], perm[bi]);
+      return all_nan();
+    }
+  }
+  const int kf = (int)fin.size();
+  const int nc = kf + 1;   // finite columns and y
+  const int nthr = ts_threads(ts_bucket(k));
+  // 2. chunk factors; 3. their combination.  R: kf rows x nc columns, R[i][c].
+  std::vector<double> R((size_t)kf * nc, 0.0), T((size_t)kf * nc);
+  const int64_t nch = (n + TS_ROWS - 1) / TS_ROWS;
+  for (int64_t ch = 0; ch < nch; ++ch) {
+    const int64_t r0 = ch * TS_ROWS, m = std::min<int64_t>(TS_ROWS, n - r0);
+    std::vector<std::vector<double>> P(nc, std::vector<double>((size_t)m));
+    for (int c = 0; c < nc; ++c) {
+      const std::vector<double>& src = c < kf ? A[fin[c]] : b;
+      for (int64_t i = 0; i < m; ++i) P[c][i] = src[r0 + i];
+    }
+    const int steps = (int)std::min<int64_t>(kf, m);
+    for (int j = 0; j < steps; ++j) {
+      const std::vector<double>& x = P[j];
+      const double c0 = x[j];
+      const double sxx = ts_dot(nthr, m, j + 1, x.data(), x.data());
+      std::vector<double> sx(nc, 0.0);
+      for (int c = j + 1; c < nc; ++c) sx[c] = ts_dot(nthr, m, j + 1, x.data(), P[c].data());
+      double beta, tau, den;
+      householder(c0, sxx, &beta, &tau, &den);
+      if (tau != 0.0) {
+        for (int c = j + 1; c < nc; ++c) {
+          std::vector<double>& t = P[c];
+          const double w = t[j] + sx[c] / den;
+          const double g = (tau * w) / den;
+          t[j] = t[j] - tau * w;
+          for (int64_t i = j + 1; i < m; ++i) t[i] = std::fma(-x[i], g, t[i]);
+        }
+      }
+      P[j][j] = beta;
+    }
+    double* dst = ch == 0 ? R.data() : T.data();
+    for (int i = 0; i < kf; ++i)
+      for (int c = 0; c < nc; ++c) dst[(size_t)i * nc + c] = (i < m && c >= i) ? P[c][i] : 0.0;
+    if (ch == 0) continue;
+    for (int j = 0; j < kf; ++j) {   // [R; T] -> R, T's column j has rows 0..j
+      double sxx = 0.0;
+      for (int i = 0; i <= j; ++i) sxx = std::fma(T[(size_t)i * nc + j], T[(size_t)i * nc + j], sxx);
+      double beta, tau, den;
+      householder(R[(size_t)j * nc + j], sxx, &beta, &tau, &den);
+      if (tau != 0.0) {
+        for (int c = j + 1; c < nc; ++c) {
+          double sx = 0.0;
+          for (int i = 0; i <= j; ++i) sx = std::fma(T[(size_t)i * nc + j], T[(size_t)i * nc + c], sx);
+          const double w = R[(size_t)j * nc + c] + sx / den;
+          const double g = (tau * w) / den;
+          R[(size_t)j * nc + c] = R[(size_t)j * nc + c] - tau * w;
+          for (int i = 0; i <= j; ++i) T[(size_t)i * nc + c] = std::fma(-T[(size_t)i * nc + j], g, T[(size_t)i * nc + c]);
+        }
+      }
+      R[(size_t)j * nc + j] = beta;
+    }
+  }
+  // 4. pivoted steps on R
+  std::vector<int> fidx(k, -1);   // column -> its column in R
+  for (int f = 0; f < kf; ++f) fidx[fin[f]] = f;
+  double mx = nd[0];
+  for (int j = 1; j < k; ++j) mx = (mx < nd[j]) ? nd[j] : mx;
+  const double thr = ((mx * eps) * (mx * eps)) / (double)n;
+  std::vector<double> nu(nd);   // current norms by position
+  int nonzero = size;
+  auto at = [&](int i, int c) -> double& { return R[(size_t)i * nc + c]; };
+  for (int p = 0; p < size; ++p) {
+    if (fidx[perm[p]] < 0) return all_nan();   // a NaN-norm column reached before the cut
+    int bi = p;
+    double best = nu[p];
+    for (int j = p + 1; j < k; ++j)
+      if (nu[j] > best) { best = nu[j]; bi = j; }
+    if (best * best < thr * (double)(n - p)) { nonzero = p; break; }
+    std::swap(perm[p], perm[bi]);
+    std::swap(nu[p], nu[bi]);
+    const int f = fidx[perm[p]];
+    double sxx = 0.0;
+    for (int i = p + 1; i < kf; ++i) sxx = std::fma(at(i, f), at(i, f), sxx);
+    double beta, tau, den;
+    householder(at(p, f), sxx, &beta, &tau, &den);
+    // the remaining finite columns (positions > p) and y
+    for (int q = p + 1; q <= k; ++q) {
+      const int g = q < k ? fidx[perm[q]] : kf;
+      if (g < 0) continue;
+      if (tau != 0.0) {
+        double sx = 0.0;
+        for (int i = p + 1; i < kf; ++i) sx = std::fma(at(i, f), at(i, g), sx);
+        const double w = at(p, g) + sx / den;
+        const double gg = (tau * w) / den;
+        at(p, g) = at(p, g) - tau * w;
+        for (int i = p + 1; i < kf; ++i) at(i, g) = std::fma(-at(i, f), gg, at(i, g));
+      }
+      if (q < k) {
+        double s = 0.0;
+        for (int i = p + 1; i < kf; ++i) s = std::fma(at(i, g), at(i, g), s);
+        nu[q] = std::sqrt(s);
+      }
+    }
+    at(p, f) = beta;
+  }
+  *rank_out = nonzero;
+  if (nonzero == 0) return coefs;
+  std::vector<double> c(nonzero);
+  for (int i = 0; i < nonzero; ++i) c[i] = at(i, kf);
+  for (int i = nonzero - 1; i >= 0; --i) {
+    c[i] = c[i] / at(i, fidx[perm[i]]);
+    for (int l = 0; l < i; ++l) c[l] = std::fma(-c[i], at(l, fidx[perm[i]]), c[l]);
+  }
+  for (int i = 0; i < nonzero; ++i) coefs[perm[i]] = c[i];
+  for (int j = 0; j < k; ++j)
+    if (coefs[j] != coefs[j]) coefs[j] = rrm::qnan();
+  return coefs;
+}
+
 // ------------------------------------------------------------ column-pivoted QR solve
 // A: k columns of n rows (consumed), b: n.  Returns coefficients in column
-// order and the number of nonzero pivots.
+// order and the number of nonzero pivots.  Up to TS_MAX_COLS columns: the
+// chunked form above; more: Eigen's algorithm on the data itself, below.
+static std::vector<double> colpiv_qr_data(std::vector<std::vector<double>> A, const std::vector<double>& b,
+                                          int64_t n, int* rank_out);
 static std::vector<double> colpiv_qr_solve(std::vector<std::vector<double>> A, const std::vector<double>& b,
                                            int64_t n, int* rank_out) {
+  if ((int)A.size() <= TS_MAX_COLS) return ts_qr_solve(A, b, n, rank_out);
+  return colpiv_qr_data(std::move(A), b, n, rank_out);
+}
+
+static std::vector<double> colpiv_qr_data(std::vector<std::vector<double>> A, const std::vector<double>& b,
+                                          int64_t n, int* rank_out) {
   const int cols = (int)A.size();
   const int64_t rows = n;
   const int size = (int)std::min<int64_t>(rows, cols);

@@ -141,6 +141,7 @@ struct rr_ctx {
   DevBuf d_prog_item, d_norm_part;              // per program: first work item; per work item: chunk sums
   DevBuf d_info, d_pos_item, d_score_part;      // FP64 scoring: QR outcome per candidate, work list, chunk sums
   DevBuf d_qst, d_qphys, d_qcold, d_qpart;      // FP64 QR state between its kernels
+  DevBuf d_rp_off, d_tsst, d_tslist;            // chunked QR: chunk-factor offsets in the slab, state, work list
   DevBuf d_prog_kept, d_fac_kept, d_ustart, d_users, d_alive;   // shared kept columns of equal factors
   DevBuf d_dead;                 // per candidate: decided by the factor-0 pre-pass (OLS mode)
   DevBuf d_blk_cand, d_blk_chunk, d_chunk_base, d_work, d_work_flags;   // FP32 chunk map and partials
@@ -525,7 +526,8 @@ int rr_destroy(rr_ctx* ctx) {
                     &ctx->d_f64_off, &ctx->d_f64_ord, &ctx->d_dead, &ctx->d_fac_prog, &ctx->d_fac_cand,
                     &ctx->d_fac_item, &ctx->d_prog_item, &ctx->d_norm_part, &ctx->d_info, &ctx->d_pos_item,
                     &ctx->d_score_part, &ctx->d_qst, &ctx->d_qphys, &ctx->d_qcold, &ctx->d_qpart,
-                    &ctx->d_prog_kept, &ctx->d_fac_kept, &ctx->d_ustart, &ctx->d_users, &ctx->d_alive};
+                    &ctx->d_prog_kept, &ctx->d_fac_kept, &ctx->d_ustart, &ctx->d_users, &ctx->d_alive,
+                    &ctx->d_rp_off, &ctx->d_tsst, &ctx->d_tslist};
   for (DevBuf* b : bufs) b->release();
   ctx->h_stage.release();
   ctx->h_out.release();
@@ -749,7 +751,8 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
   const size_t s_f64 = align_up((size_t)nc * 8) + align_up((size_t)nc * 4) + 4 * align_up((size_t)(np + 1) * 4) +
                        align_up((size_t)(nc + 1) * 4) +       // the scoring work list
                        2 * align_up((size_t)(np + 1) * 8) +   // kept-column offsets per program and per unique factor
-                       2 * align_up((size_t)(np + 1) * 4);    // the users of each unique factor
+                       2 * align_up((size_t)(np + 1) * 4) +   // the users of each unique factor
+                       align_up((size_t)nc * 8);              // chunk-factor offsets of the chunked QR
   if (!t_arena && ctx->h_stage.ensure(s_cand + s_prog + s_code + s_K + s_off + s_tab + s_cds + s_chunks + 2 * s_f64) != RR_OK)
     return RR_E_NOMEM;
   size_t so = 0;
@@ -822,13 +825,23 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
   // the host work and the uploads, launch64 only launches.
   // positions c0..c1-1 share one slab; their unique factors are
   // p0..p0+npair-1 of the evaluation work list, with nitem work items
-  // sitem chunk work items of the QR and scoring (sper each, or 0), and per QR
-  // step s the positions taking part (step_pos[s]: those with more than s
-  // columns, a prefix) and their chunk work items (step_items[s])
+  // sitem chunk work items of the QR and scoring (sper each, or 0).  The
+  // first ne positions (more than TS_MAX_COLS columns) take the
+  // step-synchronous QR: per step s the positions taking part (step_pos[s]:
+  // those with more than s columns, a prefix) and their chunk work items
+  // (step_items[s]).  The rest take the chunked QR in three column buckets:
+  // positions ts_pos[i] .. ts_pos[i+1]-1 and chunk work items ts_it[i] ..
+  // ts_it[i+1]-1 have TS_NC[i] columns with y (positions and items relative to
+  // the chunk's first).
   struct Chunk64 {
-    int c0, c1, maxk, p0, npair, nitem, per, sitem, sper;
+    int c0, c1, maxk, p0, npair, nitem, per, sitem, sper, ne;
+    int ts_pos[4], ts_it[4];
+    int64_t ts_lw[4];   // the buckets' work-item descriptors: first word, and the words in all
     std::vector<int32_t> step_pos, step_items;
   };
+  static constexpr int TS_NC[3] = {10, 5, 3};
+  // a chunked-QR position has as many chunk work items as scoring ones
+  static_assert(rr::TS_ROWS == rr::EVAL_CHUNK_ROWS, "chunked QR and scoring share the chunk numbering");
   auto plan64 = [&](const std::vector<int32_t>* list_in, int slot, std::vector<Chunk64>& chunks) -> int {
     chunks.clear();
     const int32_t m = list_in ? (int32_t)list_in->size() : nc;
@@ -868,7 +881,7 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
     // that are the same program over the same dataset (same opcodes and
     // operands, constants compared by value) share one kept column and its
     // norms: one evaluation.
-    std::vector<int64_t> off(m);
+    std::vector<int64_t> off(m), rp_off(m, 0);
     std::vector<int64_t> prog_kept(std::max(np, 1), 0);
     std::vector<int32_t> ufac(std::max(np, 1), -1);   // per non-bare program: its unique factor
     std::vector<int32_t> u_prog, u_cand;              // per unique factor: a program and candidate using it
@@ -944,7 +957,7 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
       table[t] = FacSlot{ph[p], u, gen};
     };
     for (int q = 0; q < m;) {
-      int64_t used = 0;
+      int64_t used = 0, rused = 0;   // slab words; chunk-factor words of the chunked QR
       int q1 = q, maxk = 0;
       const int u0 = (int)u_prog.size();
       ++gen;
@@ -965,10 +978,17 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
             if (!bare[p2] && pfound[p2] < 0 && pearlier[p2] < 0 && ph[p2] == ph[p] && same_prog(p2, p)) pearlier[p] = p2;
           if (pearlier[p] < 0) ++fresh;
         }
-        const int64_t need = ((int64_t)(k + 1) + fresh) * ldx;
-        if (q1 > q && (used + need) * 8 > budget) break;
+        // working columns (A, Q^T y) for the step-synchronous QR, chunk
+        // factors for the chunked one (placed after the chunk's columns)
+        const bool ts = k <= rr::TS_MAX_COLS;
+        const int64_t nb = rr::ts_bucket(k);
+        const int64_t rneed = ts ? (ds_of(c).n + rr::TS_ROWS - 1) / rr::TS_ROWS * nb * (nb - 1) : 0;
+        const int64_t need = ((ts ? 0 : (int64_t)(k + 1)) + fresh) * ldx;
+        if (q1 > q && (used + rused + need + rneed) * 8 > budget) break;
         off[q1] = used;
-        used += (int64_t)(k + 1) * ldx;
+        if (!ts) used += (int64_t)(k + 1) * ldx;
+        rp_off[q1] = rused;
+        rused += rneed;
         for (int32_t p = pb; p < pe; ++p) {
           if (bare[p]) continue;
           int32_t u = pfound[p] >= 0 ? pfound[p] : pearlier[p] >= 0 ? ufac[pearlier[p]] : -1;
@@ -991,11 +1011,12 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
       Chunk64 ch{};
       ch.c0 = q; ch.c1 = q1; ch.maxk = maxk; ch.p0 = u0; ch.npair = (int)u_prog.size() - u0;
       chunks.push_back(ch);
-      max_words = std::max(max_words, used);
+      for (int r = q; r < q1; ++r) rp_off[r] += used;   // the chunk factors follow the chunk's columns
+      max_words = std::max(max_words, used + rused);
       q = q1;
     }
     ctx->fac_gen = gen;
-    if (ctx->d_slab.ensure((size_t)max_words * 8) != RR_OK) return RR_E_NOMEM;
+    if (ctx->d_slab.ensure((size_t)std::max<int64_t>(max_words, 1) * 8) != RR_OK) return RR_E_NOMEM;
     // factor-evaluation work list: per chunk its unique factors, longest
     // first (counting sort on the evaluation cost), each with
     // ceil(n / EVAL_CHUNK_ROWS) work items
@@ -1063,12 +1084,14 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
       if ((rc = upload(ctx, ctx->d_ustart, ustart.data(), ustart.size() * 4, o2))) return rc;
       o2 += sp;
       if ((rc = upload(ctx, ctx->d_users, users.data(), users.size() * 4, o2))) return rc;
+      o2 += sp;
+      if ((rc = upload(ctx, ctx->d_rp_off, rp_off.data(), (size_t)m * 8, o2))) return rc;
       if (ctx->d_alive.ensure((size_t)std::max(nu, 1) * 4) != RR_OK) return RR_E_NOMEM;
     }
     if (ctx->d_norm_part.ensure((size_t)std::max<int64_t>(max_items, 1) * 16) != RR_OK) return RR_E_NOMEM;
     // scoring work list: per position, ceil(n / EVAL_CHUNK_ROWS) chunks, numbered per slab chunk
     std::vector<int32_t> pos_item(m + 1);
-    int64_t max_sitems = 0, max_qpos = 0, max_qcol = 0, max_qpart = 0;
+    int64_t max_sitems = 0, max_qpos = 0, max_qcol = 0, max_qpart = 0, max_lwords = 0;
     for (Chunk64& ch : chunks) {
       int64_t it = 0, per = -1;
       for (int q = ch.c0; q < ch.c1; ++q) {
@@ -1081,22 +1104,41 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
       ch.sitem = (int)it;
       ch.sper = per > 0 ? (int)per : 0;
       max_sitems = std::max(max_sitems, it);
-      for (int st = 0; st < ch.maxk; ++st) {
+      const int mc = ch.c1 - ch.c0;
+      auto kof = [&](int r) { return b->cand_start[lpt[ch.c0 + r] + 1] - b->cand_start[lpt[ch.c0 + r]] + 1; };
+      auto item_at = [&](int r) { return r < mc ? pos_item[ch.c0 + r] : ch.sitem; };
+      ch.ne = 0;
+      while (ch.ne < mc && kof(ch.ne) > rr::TS_MAX_COLS) ++ch.ne;
+      for (int st = 0; st < ch.maxk && ch.ne > 0; ++st) {
         int na = 0;
-        while (ch.c0 + na < ch.c1 && b->cand_start[lpt[ch.c0 + na] + 1] - b->cand_start[lpt[ch.c0 + na]] + 1 > st) ++na;
+        while (na < ch.ne && kof(na) > st) ++na;
         if (na == 0) break;
         ch.step_pos.push_back(na);
-        ch.step_items.push_back(ch.c0 + na < ch.c1 ? pos_item[ch.c0 + na] : ch.sitem);
+        ch.step_items.push_back(item_at(na));
       }
-      const int mc = ch.c1 - ch.c0;
+      // the chunked positions by column bucket (contiguous: sorted by column count, most first)
+      ch.ts_pos[0] = ch.ne;
+      for (int i = 0; i < 3; ++i) {
+        int r = ch.ts_pos[i];
+        while (r < mc && rr::ts_bucket(kof(r)) == TS_NC[i]) ++r;
+        ch.ts_pos[i + 1] = r;
+      }
+      if (ch.ts_pos[3] != mc) return fail(RR_E_ARG, "internal: chunked QR positions out of column order");
+      for (int i = 0; i < 4; ++i) ch.ts_it[i] = item_at(ch.ts_pos[i]);
+      ch.ts_lw[0] = 0;
+      for (int i = 0; i < 3; ++i)
+        ch.ts_lw[i + 1] = ch.ts_lw[i] + (int64_t)(ch.ts_it[i + 1] - ch.ts_it[i]) * rr::ts_desc_words(TS_NC[i]);
+      max_lwords = std::max(max_lwords, ch.ts_lw[3]);
       max_qpos = std::max<int64_t>(max_qpos, mc);
-      max_qcol = std::max<int64_t>(max_qcol, (int64_t)mc * ch.maxk);
-      max_qpart = std::max<int64_t>(max_qpart, it * 2 * ch.maxk);
+      max_qcol = std::max<int64_t>(max_qcol, (int64_t)ch.ne * ch.maxk);
+      max_qpart = std::max<int64_t>(max_qpart, (int64_t)item_at(ch.ne) * 2 * ch.maxk);
     }
     pos_item[m] = 0;
     if ((rc = upload(ctx, ctx->d_pos_item, pos_item.data(), pos_item.size() * 4, o + 4 * sp))) return rc;
     if (ctx->d_score_part.ensure((size_t)std::max<int64_t>(max_sitems, 1) * 8) != RR_OK) return RR_E_NOMEM;
     if (ctx->d_qst.ensure((size_t)std::max<int64_t>(max_qpos, 1) * sizeof(rr::QrState)) != RR_OK ||
+        ctx->d_tsst.ensure((size_t)std::max<int64_t>(max_qpos, 1) * sizeof(rr::TsState)) != RR_OK ||
+        ctx->d_tslist.ensure((size_t)(max_lwords + 2) * 8) != RR_OK ||
         ctx->d_qphys.ensure((size_t)std::max<int64_t>(max_qcol, 1) * 4) != RR_OK ||
         ctx->d_qcold.ensure((size_t)std::max<int64_t>(max_qcol, 1) * 5 * 8) != RR_OK ||
         ctx->d_qpart.ensure((size_t)std::max<int64_t>(max_qpart, 1) * 8) != RR_OK)
@@ -1142,9 +1184,29 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
       Q.st = (rr::QrState*)ctx->d_qst.p; Q.phys = (int32_t*)ctx->d_qphys.p; Q.cold = (double*)ctx->d_qcold.p;
       Q.ks = ch.maxk; Q.part = (double*)ctx->d_qpart.p;
       pm = begin_phase(ctx, RR_PHASE_QR, stream);
-      RR_HIP(rr::launch_qr_init(Q, stream));
-      for (size_t st = 0; st < ch.step_pos.size(); ++st) RR_HIP(rr::launch_qr_step(Q, ch.step_pos[st], ch.step_items[st], stream));
-      RR_HIP(rr::launch_qr_solve(Q, stream));
+      if (ch.ne > 0) {
+        Q.n_pos = ch.ne;
+        RR_HIP(rr::launch_qr_init(Q, stream));
+        for (size_t st = 0; st < ch.step_pos.size(); ++st) RR_HIP(rr::launch_qr_step(Q, ch.step_pos[st], ch.step_items[st], stream));
+        RR_HIP(rr::launch_qr_solve(Q, stream));
+      }
+      const int mc = ch.c1 - ch.c0;
+      if (ch.ne < mc) {
+        rr::TsLaunch T{};
+        T.cand_start = dc; T.prog_start = dp; T.code = dcode; T.ds = dtab; T.cand_ds = dcds;
+        T.coefs = dcoef; T.fit = dfit; T.dead = ddead; T.info = (int32_t*)ctx->d_info.p;
+        T.order = (const int32_t*)ctx->d_f64_ord.p + ch.c0; T.slab = (const double*)ctx->d_slab.p;
+        T.norm_part = (const double*)ctx->d_norm_part.p; T.prog_item = (const int32_t*)ctx->d_prog_item.p;
+        T.prog_kept = (const int64_t*)ctx->d_prog_kept.p;
+        T.rp_off = (const int64_t*)ctx->d_rp_off.p + ch.c0;
+        T.st = (rr::TsState*)ctx->d_tsst.p; T.rpart = (double*)ctx->d_slab.p;
+        // the work list: its three counters, then the descriptors
+        T.cnt = (int32_t*)ctx->d_tslist.p; T.desc = (int64_t*)ctx->d_tslist.p + 2;
+        for (int i = 0; i < 3; ++i) T.list_base[i] = ch.ts_lw[i];
+        RR_HIP(rr::launch_ts_init(T, ch.ne, mc, stream));
+        for (int i = 0; i < 3; ++i) RR_HIP(rr::launch_ts_chunk(T, i, ch.ts_it[i + 1] - ch.ts_it[i], stream));
+        RR_HIP(rr::launch_ts_solve(T, ch.ne, mc, stream));
+      }
       end_phase(ctx, pm, stream);
       rr::ScoreLaunch Sc{};
       Sc.cand_start = dc; Sc.prog_start = dp; Sc.code = dcode; Sc.K = dK; Sc.ds = dtab; Sc.cand_ds = dcds;
@@ -1296,7 +1358,8 @@ int rr_fitness_batch_noalloc(rr_ctx* ctx, int32_t handle, const rr_batch* b, int
                     &ctx->d_f64_off, &ctx->d_f64_ord, &ctx->d_dead, &ctx->d_fac_prog, &ctx->d_fac_cand,
                     &ctx->d_fac_item, &ctx->d_prog_item, &ctx->d_norm_part, &ctx->d_info, &ctx->d_pos_item,
                     &ctx->d_score_part, &ctx->d_qst, &ctx->d_qphys, &ctx->d_qcold, &ctx->d_qpart,
-                    &ctx->d_prog_kept, &ctx->d_fac_kept, &ctx->d_ustart, &ctx->d_users, &ctx->d_alive};
+                    &ctx->d_prog_kept, &ctx->d_fac_kept, &ctx->d_ustart, &ctx->d_users, &ctx->d_alive,
+                    &ctx->d_rp_off, &ctx->d_tsst, &ctx->d_tslist};
   constexpr size_t NB = sizeof(bufs) / sizeof(bufs[0]);
   DevBuf saved[NB];
   for (size_t i = 0; i < NB; ++i) { saved[i] = *bufs[i]; *bufs[i] = DevBuf(); }

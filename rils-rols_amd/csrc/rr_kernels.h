@@ -62,6 +62,73 @@ struct QrLaunch {
   double* part;
 };
 
+// Chunked column-pivoted QR of the FP64 OLS form, for candidates with at most
+// TS_MAX_COLS columns (free term included):
+//   launch_ts_init   one thread per position: the initial norms (from the
+//                    evaluation's chunk sums), the outcome when step 0 would
+//                    pivot a non-finite column, the finite columns, the rank
+//                    threshold; the chunk work items (position, chunk) of the
+//                    live positions appended to their column bucket's list;
+//   launch_ts_chunk  per bucket, a grid of workgroups over the bucket's list:
+//                    unpivoted Householder QR of a chunk of TS_ROWS rows of
+//                    [finite columns | y], its triangular factor to rpart;
+//   launch_ts_solve  one wave per position: the chunk factors combined in
+//                    chunk order, the pivoted steps and the solve on the
+//                    result; coefficients, info (as the step-synchronous QR).
+// Positions pos0..pos1-1 of the arrays (pointers not offset).  Column buckets
+// 0, 1, 2: 10, 5, 3 columns with y (ts_bucket).  Position p's chunk factors
+// sit at rpart + rp_off[p] + chunk * W, W = nc * (nc - 1) words for its
+// bucket's nc.  desc: the chunk work items, ts_desc_words(nc) words each
+// (factor address, rows, finite column count, the nc column addresses at the
+// chunk's first row), bucket b's from word list_base[b] on, room for as many
+// as its positions have chunks; cnt: 3 counters.
+constexpr int TS_MAX_COLS = 9;
+constexpr int64_t TS_ROWS = 4096;
+constexpr int TS_GRID = 2048;   // workgroups of a chunk launch at most
+struct TsState {
+  int32_t k, kf, size, live;            // live: 1 when the chunk and solve kernels have work
+  int32_t fin[TS_MAX_COLS];             // the finite columns, in column order
+  int32_t nch, pad;
+  int64_t n;
+  double thr;
+  double nd[TS_MAX_COLS];               // initial column norms
+  const double* col[TS_MAX_COLS + 1];   // the finite columns' data, then y
+};
+struct TsLaunch {
+  const int32_t* cand_start;
+  const int32_t* prog_start;
+  const uint32_t* code;
+  const DsDesc* ds;
+  const int32_t* cand_ds;
+  double* coefs;
+  rr_fit* fit;
+  const int32_t* dead;
+  int32_t* info;
+  const int32_t* order;       // position -> candidate
+  const double* slab;
+  const double* norm_part;
+  const int32_t* prog_item;
+  const int64_t* prog_kept;
+  const int64_t* rp_off;      // per position: offset of its chunk factors in rpart
+  TsState* st;
+  double* rpart;
+  int32_t* cnt;
+  int64_t* desc;
+  int64_t list_base[3];
+};
+hipError_t launch_ts_init(const TsLaunch& L, int pos0, int pos1, hipStream_t s);
+// bucket 0, 1, 2; max_items: the bucket's chunk work items if every position
+// were live
+hipError_t launch_ts_chunk(const TsLaunch& L, int bucket, int max_items, hipStream_t s);
+hipError_t launch_ts_solve(const TsLaunch& L, int pos0, int pos1, hipStream_t s);
+// the column count with y of the bucket of a candidate with k columns (free term included, k <= TS_MAX_COLS)
+__host__ __device__ inline int ts_bucket(int k) { return k <= 2 ? 3 : k <= 4 ? 5 : 10; }
+// words of a chunk work item's descriptor in the bucket of nc columns with y
+__host__ __device__ constexpr int ts_desc_words(int nc) { return 3 + nc; }
+// threads of the bucket's chunk sums (the order of the oracle's ts_dot): 256
+// threads of 16 rows for 3 and 5 columns, 1024 of 4 rows for 10
+__host__ __device__ constexpr int ts_threads(int nc) { return nc <= 5 ? 256 : 1024; }
+
 // Scoring of the FP64 form: one workgroup per (launched position, chunk of
 // EVAL_CHUNK_ROWS rows) computes the chunk's sum of squared residuals --
 // RAW: of the candidate's one program; OLS: of the OLS tree from the

@@ -1166,6 +1166,434 @@ __global__ __launch_bounds__(CP) void k_stats_fp64(const double* y, int64_t n, d
   if (tid == 0) { out4[0] = ybar; out4[1] = q[0]; out4[2] = s[0]; out4[3] = s[1]; }
 }
 
+
+// ---------------------------------------------------------------- chunked QR
+// Candidates with at most TS_MAX_COLS columns: the column-pivoted QR carried
+// out on the triangular factor of the data (the canonical algorithm and its
+// order are described with the oracle's ts_qr_solve).  State per position in
+// TsState; the chunk factors in rpart.
+struct TsArgs {
+  const int32_t* cand_start;
+  const int32_t* prog_start;
+  const uint32_t* code;
+  const DsDesc* ds;
+  const int32_t* cand_ds;
+  double* coefs;
+  rr_fit* fit;
+  const int32_t* dead;
+  int32_t* info;
+  const int32_t* order;
+  const double* slab;
+  const double* norm_part;
+  const int32_t* prog_item;
+  const int64_t* prog_kept;
+  const int64_t* rp_off;
+  TsState* st;
+  double* rpart;
+  int32_t* ts_cnt;
+  int64_t* ts_desc;
+  int64_t list_base[3];
+  int pos0, pos1, bucket;
+};
+
+// Eigen's makeHouseholder (Householder.h:67-96) from the pivot c0 and the sum
+// of squares of the tail; den = c0 - beta divides the tail into the essential part
+__device__ inline void ts_householder(double c0, double tail_sq, double& beta, double& tau, double& den) {
+  if (tail_sq <= DBL_MIN_) {
+    tau = 0.0; beta = c0; den = 1.0;
+  } else {
+    double b = __builtin_sqrt(c0 * c0 + tail_sq);
+    if (c0 >= 0.0) b = -b;
+    beta = b;
+    den = c0 - b;
+    tau = (b - c0) / b;
+  }
+}
+
+// where physical column col of a candidate lives: the free term's ones, a
+// bare variable's X column, or the kept column of its factor
+__device__ inline const double* ts_col(const TsArgs& a, const DsDesc& D, int pbeg, int nf, int col) {
+  if (col == nf) return D.one;
+  const int var = bare_variable(a.code + a.prog_start[pbeg + col], a.prog_start[pbeg + col + 1] - a.prog_start[pbeg + col]);
+  return var >= 0 ? D.X + (int64_t)var * D.ldx : a.slab + a.prog_kept[pbeg + col];
+}
+
+__global__ __launch_bounds__(64) void k_ts_init(TsArgs a) {
+  const int pos = a.pos0 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (pos >= a.pos1) return;
+  TsState& s = a.st[pos];
+  s.live = 0;
+  const int cand = a.order[pos];
+  if (a.dead && a.dead[cand]) return;
+  const DsDesc& D = a.ds[a.cand_ds ? a.cand_ds[cand] : 0];
+  const int pbeg = a.cand_start[cand];
+  const int nf = a.cand_start[cand + 1] - pbeg;
+  const int k = nf + 1;
+  const int64_t n = D.n;
+  const int64_t nch_eval = (n + rrm::NORM_CHUNK - 1) / rrm::NORM_CHUNK;
+  double nd[TS_MAX_COLS];
+  for (int j = 0; j < nf; ++j) {
+    const int var = bare_variable(a.code + a.prog_start[pbeg + j], a.prog_start[pbeg + j + 1] - a.prog_start[pbeg + j]);
+    double s0;
+    if (var >= 0) {
+      s0 = D.xnorm[2 * var];
+    } else {
+      double s01[2];
+      ordered_sums<2>(a.norm_part + 2 * (int64_t)a.prog_item[pbeg + j], 2, nch_eval, s01);
+      s0 = s01[0];
+    }
+    nd[j] = __builtin_sqrt(s0);
+  }
+  nd[nf] = __builtin_sqrt((double)n);
+  const int size = (int)((n < (int64_t)k) ? n : (int64_t)k);
+  int bi = 0;
+  double best = nd[0];
+  for (int j = 1; j < k; ++j)
+    if (nd[j] > best) { best = nd[j]; bi = j; }
+  if (!(best - best == 0.0)) {
+    // step 0 pivots a non-finite column (swapped to position 0): its reflector
+    // is NaN, nothing is cut after it, every coefficient of the first `size`
+    // positions is NaN
+    double* const co = a.coefs + (int64_t)pbeg + cand;
+    for (int i = 0; i < k; ++i) co[i] = 0.0;
+    for (int i = 0; i < size; ++i) co[i == 0 ? bi : (i == bi ? 0 : i)] = rrm::qnan();
+    write_fit(a.fit + cand, rrm::qnan(), D.sst, n, size, 0);
+    a.info[cand] = -1;
+    return;
+  }
+  int kf = 0;
+  for (int j = 0; j < k; ++j)
+    if (nd[j] - nd[j] == 0.0) s.fin[kf++] = j;
+  double mx = nd[0];
+  for (int j = 1; j < k; ++j) mx = (mx < nd[j]) ? nd[j] : mx;
+  s.thr = ((mx * DBL_EPS) * (mx * DBL_EPS)) / (double)n;
+  for (int j = 0; j < k; ++j) s.nd[j] = nd[j];
+  for (int f = 0; f < kf; ++f) s.col[f] = ts_col(a, D, pbeg, nf, s.fin[f]);
+  s.col[kf] = D.y;
+  s.k = k;
+  s.kf = kf;
+  s.size = size;
+  const int nch = (int)((n + TS_ROWS - 1) / TS_ROWS);
+  s.nch = nch;
+  s.n = n;
+  s.live = 1;
+  // the position's chunk work items, appended to its bucket's list (the
+  // order of the list does not matter: every item writes its own factor) as
+  // self-contained descriptors: where its factor goes, its rows, its finite
+  // column count, its columns from the chunk's first row
+  const int nb = ts_bucket(k);
+  const int b = nb == 10 ? 0 : nb == 5 ? 1 : 2;
+  const int dw = ts_desc_words(nb);
+  const int slot = atomicAdd(a.ts_cnt + b, nch);
+  int64_t* dst = a.ts_desc + a.list_base[b] + (int64_t)slot * dw;
+  double* const out = a.rpart + a.rp_off[pos];
+  for (int c = 0; c < nch; ++c, dst += dw) {
+    const int64_t r0 = (int64_t)c * TS_ROWS;
+    dst[0] = (int64_t)(out + (int64_t)c * nb * (nb - 1));
+    dst[1] = (n - r0 < TS_ROWS) ? n - r0 : TS_ROWS;
+    dst[2] = kf;
+    for (int f = 0; f <= kf; ++f) dst[3 + f] = (int64_t)(s.col[f] + r0);
+  }
+}
+
+// A double moved between lanes by a DPP lane pattern (both halves).
+template <int CTRL>
+__device__ inline double ts_dpp(double v) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ inline double ts_readlane(double v, int lane) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), lane),
+                          __builtin_amdgcn_readlane(__double2loint(v), lane));
+}
+
+// Sum over the 64 lanes of a wave by the xor butterfly of strides 1, 2, 4,
+// ..., 32 (lane l adds lane l ^ s; a + b and b + a are the same bits, so every
+// lane of a combined group holds the same value): strides 1 and 2 by quad
+// permutations; then the groups of 4 and 8 are uniform, so stride 4 is the
+// mirror within 8 lanes and stride 8 the mirror within 16; strides 16 and 32
+// from lanes 0, 16, 32, 48.  Every thread gets the result.  All 64 lanes must
+// be active.
+__device__ inline double ts_wave_sum(double v) {
+  v = v + ts_dpp<0xB1>(v);    // quad_perm [1, 0, 3, 2]: lane ^ 1
+  v = v + ts_dpp<0x4E>(v);    // quad_perm [2, 3, 0, 1]: lane ^ 2
+  v = v + ts_dpp<0x141>(v);   // row_half_mirror: lane 7 - l of its 8
+  v = v + ts_dpp<0x140>(v);   // row_mirror: lane 15 - l of its 16
+  return (ts_readlane(v, 0) + ts_readlane(v, 16)) + (ts_readlane(v, 32) + ts_readlane(v, 48));
+}
+
+// One chunk work item: unpivoted Householder QR of the chunk's [finite
+// columns | y] (m rows, kf finite columns), held in registers: thread t holds
+// rows t + THR u, u = 0 .. RPT-1.  Every sum is in the oracle's order for THR
+// threads: per thread its rows in order, groups of 64 by ts_wave_sum, the
+// group sums in order.  FULL: m == TS_ROWS, so only rows 0..j of step j
+// (thread t <= j, u = 0) need a condition.  One barrier per step: the group
+// sums go to LDS (double-buffered by step parity, buf) and every thread adds
+// them up itself.
+template <int NC, int THR, bool FULL>
+__device__ inline void ts_chunk_item(const int64_t* cur, double (*prow)[NC], double (*red)[THR / WAVE][NC], int& buf) {
+  constexpr int RPT = (int)(TS_ROWS / THR);
+  constexpr int NG = THR / WAVE;
+  const int t = (int)threadIdx.x;
+  const int lane = t & (WAVE - 1), wave = t / WAVE;
+  double* const out = (double*)cur[0];
+  const int64_t m = cur[1];
+  const int kf = (int)cur[2];
+  const int nc = kf + 1;
+  double v[NC][RPT];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    const double* src = c < nc ? (const double*)cur[3 + c] : nullptr;
+#pragma unroll
+    for (int u = 0; u < RPT; ++u) {
+      const int64_t row = t + (int64_t)THR * u;
+      v[c][u] = (src != nullptr && (FULL || row < m)) ? src[row] : 0.0;
+    }
+  }
+  const int steps = (int)((m < (int64_t)kf) ? m : (int64_t)kf);
+#pragma unroll
+  for (int j = 0; j < NC - 1; ++j) {
+    if (j >= steps) continue;   // uniform
+    if (t == j) {
+#pragma unroll
+      for (int c = 0; c < NC; ++c) prow[buf][c] = v[c][0];
+    }
+    double ps[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) ps[c] = 0.0;
+#pragma unroll
+    for (int u = 0; u < RPT; ++u) {
+      const int64_t row = t + (int64_t)THR * u;
+      const bool in = FULL ? (u > 0 || t > j) : (row > j && row < m);
+      if (in) {
+        const double x = v[j][u];
+        ps[j] = __builtin_fma(x, x, ps[j]);
+#pragma unroll
+        for (int c = j + 1; c < NC; ++c)
+          if (c < nc) ps[c] = __builtin_fma(x, v[c][u], ps[c]);
+      }
+    }
+#pragma unroll
+    for (int c = j; c < NC; ++c)
+      if (c < nc) ps[c] = ts_wave_sum(ps[c]);
+    if (lane == 0) {
+#pragma unroll
+      for (int c = j; c < NC; ++c)
+        if (c < nc) red[buf][wave][c] = ps[c];
+    }
+    __syncthreads();
+    double tot[NC];
+#pragma unroll
+    for (int c = j; c < NC; ++c) {
+      if (c >= nc) continue;
+      double acc = red[buf][0][c];
+#pragma unroll
+      for (int g = 1; g < NG; ++g) acc = acc + red[buf][g][c];
+      tot[c] = acc;
+    }
+    double beta, tau, den;
+    ts_householder(prow[buf][j], tot[j], beta, tau, den);
+    if (tau != 0.0) {
+#pragma unroll
+      for (int c = j + 1; c < NC; ++c) {
+        if (c >= nc) continue;
+        const double w = prow[buf][c] + tot[c] / den;
+        const double g = (tau * w) / den;
+#pragma unroll
+        for (int u = 0; u < RPT; ++u) {
+          const int64_t row = t + (int64_t)THR * u;
+          if (u == 0 && t == j) v[c][u] = v[c][u] - tau * w;
+          else if (FULL ? (u > 0 || t > j) : (row > j && row < m)) v[c][u] = __builtin_fma(-v[j][u], g, v[c][u]);
+        }
+      }
+    }
+    if (t == j) v[j][0] = beta;
+    buf ^= 1;   // the next step writes the other half; this one is read until its barrier
+  }
+  if (t < kf) {   // row t of the chunk's factor (rows past m are zero)
+    double* dst = out + (int64_t)t * NC;
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+      if (c < nc) dst[c] = ((int64_t)t < m && c >= t) ? v[c][0] : 0.0;
+  }
+}
+
+// Chunk work items of the live positions, one bucket (positions with NC
+// columns with y) at a time: workgroups of ts_threads(NC) threads take items
+// blockIdx.x, blockIdx.x + gridDim.x, ... of the bucket's list (written by
+// k_ts_init) until the count is reached; the next item's descriptor is loaded
+// while the current one computes (up to 5 columns: the wide bucket has no
+// scalar registers to spare).
+template <int NC>
+__global__ __launch_bounds__(ts_threads(NC)) void k_ts_chunk(TsArgs a) {
+  constexpr int THR = ts_threads(NC);
+  constexpr int DW = ts_desc_words(NC);
+  constexpr bool PREFETCH = NC <= 5;
+  __shared__ double prow[2][NC];
+  __shared__ double red[2][THR / WAVE][NC];
+  const int cnt = a.ts_cnt[a.bucket];   // uniform
+  const int64_t* const desc = a.ts_desc + a.list_base[a.bucket];
+  int buf = 0;   // LDS half of the next step
+  int item = (int)blockIdx.x;
+  if (item >= cnt) return;
+  int64_t cur[DW];
+#pragma unroll
+  for (int w = 0; w < DW; ++w) cur[w] = desc[(int64_t)item * DW + w];
+  for (;;) {
+    const int next = item + (int)gridDim.x;
+    int64_t nxt[DW];
+#pragma unroll
+    for (int w = 0; w < DW; ++w) nxt[w] = (PREFETCH && next < cnt) ? desc[(int64_t)next * DW + w] : 0;
+    if (cur[1] == TS_ROWS) ts_chunk_item<NC, THR, true>(cur, prow, red, buf);
+    else ts_chunk_item<NC, THR, false>(cur, prow, red, buf);
+    if (next >= cnt) break;
+    item = next;
+#pragma unroll
+    for (int w = 0; w < DW; ++w) cur[w] = PREFETCH ? nxt[w] : desc[(int64_t)item * DW + w];
+  }
+}
+
+// One wave per position: the chunk factors combined in chunk order, Eigen's
+// pivoted steps on the result, the solve.  The factor R (kf rows, nc columns
+// with y) lives in LDS; in every step lane c carries column c, its sums in
+// row order as in the oracle.  The next chunk factor is loaded while the
+// current one is folded in.
+constexpr int TS_RW = TS_MAX_COLS * (TS_MAX_COLS + 1);   // words of the largest factor (bucket 10)
+__global__ __launch_bounds__(WAVE) void k_ts_solve(TsArgs a) {
+  const int pos = a.pos0 + (int)blockIdx.x;
+  const TsState& s = a.st[pos];
+  if (!s.live) return;   // uniform
+  __shared__ double R[TS_RW], T[TS_RW];
+  __shared__ int perm[TS_MAX_COLS], fidx[TS_MAX_COLS];
+  __shared__ double nu[TS_MAX_COLS];
+  const int l = (int)threadIdx.x;
+  const int cand = a.order[pos];
+  const DsDesc& D = a.ds[a.cand_ds ? a.cand_ds[cand] : 0];
+  const int pbeg = a.cand_start[cand];
+  const int k = s.k, kf = s.kf, nc = kf + 1, size = s.size;
+  const int NC = ts_bucket(k);
+  const int W = NC * (NC - 1);   // words per chunk factor
+  const int64_t n = D.n;
+  const double* const src = a.rpart + a.rp_off[pos];
+  const int nch = s.nch;
+  double pre0 = 0.0, pre1 = 0.0;   // words l and l + 64 of the next chunk factor
+  if (l < W) R[l] = src[l];
+  if (l + WAVE < W) R[l + WAVE] = src[l + WAVE];
+  if (nch > 1) {
+    if (l < W) pre0 = src[W + l];
+    if (l + WAVE < W) pre1 = src[W + l + WAVE];
+  }
+  for (int ch = 1; ch < nch; ++ch) {
+    if (l < W) T[l] = pre0;
+    if (l + WAVE < W) T[l + WAVE] = pre1;
+    if (ch + 1 < nch) {
+      const double* nx = src + (int64_t)(ch + 1) * W;
+      if (l < W) pre0 = nx[l];
+      if (l + WAVE < W) pre1 = nx[l + WAVE];
+    }
+    __syncthreads();
+    for (int j = 0; j < kf; ++j) {   // [R; T] -> R; T's column j has rows 0..j
+      double sxx = 0.0;
+      for (int i = 0; i <= j; ++i) sxx = __builtin_fma(T[i * NC + j], T[i * NC + j], sxx);
+      double beta, tau, den;
+      ts_householder(R[j * NC + j], sxx, beta, tau, den);
+      if (tau != 0.0 && l > j && l < nc) {
+        const int c = l;
+        double sx = 0.0;
+        for (int i = 0; i <= j; ++i) sx = __builtin_fma(T[i * NC + j], T[i * NC + c], sx);
+        const double w = R[j * NC + c] + sx / den;
+        const double g = (tau * w) / den;
+        R[j * NC + c] = R[j * NC + c] - tau * w;
+        for (int i = 0; i <= j; ++i) T[i * NC + c] = __builtin_fma(-T[i * NC + j], g, T[i * NC + c]);
+      }
+      __syncthreads();
+      if (l == 0) R[j * NC + j] = beta;
+      __syncthreads();
+    }
+  }
+  if (l < k) { perm[l] = l; fidx[l] = -1; nu[l] = s.nd[l]; }
+  __syncthreads();
+  if (l < kf) fidx[s.fin[l]] = l;
+  __syncthreads();
+  int nonzero = size, steps = 0;
+  bool nan_out = false;
+  for (int p = 0; p < size; ++p) {   // every lane follows the same (uniform) decisions
+    if (fidx[perm[p]] < 0) { nan_out = true; break; }   // a NaN-norm column reached before the cut
+    int bi = p;
+    double best = nu[p];
+    for (int j = p + 1; j < k; ++j)
+      if (nu[j] > best) { best = nu[j]; bi = j; }
+    if (best * best < s.thr * (double)(n - p)) { nonzero = p; break; }
+    __syncthreads();
+    if (l == 0) {
+      const int ti = perm[p]; perm[p] = perm[bi]; perm[bi] = ti;
+      const double tv = nu[p]; nu[p] = nu[bi]; nu[bi] = tv;
+    }
+    __syncthreads();
+    const int f = fidx[perm[p]];
+    double sxx = 0.0;
+    for (int i = p + 1; i < kf; ++i) sxx = __builtin_fma(R[i * NC + f], R[i * NC + f], sxx);
+    double beta, tau, den;
+    ts_householder(R[p * NC + f], sxx, beta, tau, den);
+    const int q = p + 1 + l;   // lane l: position q (k: y)
+    if (q <= k) {
+      const int g = q < k ? fidx[perm[q]] : kf;
+      if (g >= 0) {
+        if (tau != 0.0) {
+          double sx = 0.0;
+          for (int i = p + 1; i < kf; ++i) sx = __builtin_fma(R[i * NC + f], R[i * NC + g], sx);
+          const double w = R[p * NC + g] + sx / den;
+          const double gg = (tau * w) / den;
+          R[p * NC + g] = R[p * NC + g] - tau * w;
+          for (int i = p + 1; i < kf; ++i) R[i * NC + g] = __builtin_fma(-R[i * NC + f], gg, R[i * NC + g]);
+        }
+        if (q < k) {
+          double sq = 0.0;
+          for (int i = p + 1; i < kf; ++i) sq = __builtin_fma(R[i * NC + g], R[i * NC + g], sq);
+          nu[q] = __builtin_sqrt(sq);
+        }
+      }
+    }
+    __syncthreads();
+    if (l == 0) R[p * NC + f] = beta;
+    __syncthreads();
+    steps = p + 1;
+  }
+  if (l != 0) return;
+  double* const co = a.coefs + (int64_t)pbeg + cand;
+  if (nan_out) {
+    for (int i = 0; i < k; ++i) co[perm[i]] = (i < size) ? rrm::qnan() : 0.0;
+    write_fit(a.fit + cand, rrm::qnan(), D.sst, n, size, steps);
+    a.info[cand] = -1;
+    return;
+  }
+  const int r = nonzero;
+  double c[TS_MAX_COLS];
+  for (int i = 0; i < r; ++i) c[i] = R[i * NC + kf];
+  for (int i = r - 1; i >= 0; --i) {
+    const int col = fidx[perm[i]];
+    c[i] = c[i] / R[i * NC + col];
+    for (int q = 0; q < i; ++q) c[q] = __builtin_fma(-c[i], R[q * NC + col], c[q]);
+  }
+  bool any_nan = false;
+  for (int i = 0; i < k; ++i) {
+    double cv = (i < r) ? c[i] : 0.0;
+    if (cv != cv) { any_nan = true; cv = rrm::qnan(); }
+    co[perm[i]] = cv;
+  }
+  if (any_nan) {   // a NaN coefficient: every prediction is NaN
+    write_fit(a.fit + cand, rrm::qnan(), D.sst, n, r, steps);
+    a.info[cand] = -1;
+  } else {
+    a.info[cand] = r | (steps << 16);   // scored by k_score_fp64 / k_fit_fp64
+  }
+}
+
+__global__ void k_ts_zero(int32_t* cnt) {
+  if (threadIdx.x < 3) cnt[threadIdx.x] = 0;
+}
+
 }  // namespace
 
 static QrArgs qr_args(const QrLaunch& L, int n_pos) {
@@ -1281,6 +1709,36 @@ hipError_t launch_fill_f64(double* p, int64_t count, double value, hipStream_t s
 
 hipError_t launch_stats_fp64(const double* y, int64_t n, double* out4, hipStream_t s) {
   hipLaunchKernelGGL(k_stats_fp64, dim3(1), dim3(CP), 0, s, y, n, out4);
+  return hipGetLastError();
+}
+
+
+static TsArgs ts_args(const TsLaunch& L, int pos0, int pos1, int bucket) {
+  return TsArgs{L.cand_start, L.prog_start, L.code, L.ds, L.cand_ds, L.coefs, L.fit, L.dead, L.info, L.order,
+                L.slab, L.norm_part, L.prog_item, L.prog_kept, L.rp_off, L.st, L.rpart, L.cnt, L.desc,
+                {L.list_base[0], L.list_base[1], L.list_base[2]}, pos0, pos1, bucket};
+}
+
+hipError_t launch_ts_init(const TsLaunch& L, int pos0, int pos1, hipStream_t s) {
+  if (pos1 <= pos0) return hipSuccess;
+  hipLaunchKernelGGL(k_ts_zero, dim3(1), dim3(WAVE), 0, s, L.cnt);
+  hipLaunchKernelGGL(k_ts_init, dim3((unsigned)((pos1 - pos0 + 63) / 64)), dim3(64), 0, s, ts_args(L, pos0, pos1, 0));
+  return hipGetLastError();
+}
+
+hipError_t launch_ts_chunk(const TsLaunch& L, int bucket, int max_items, hipStream_t s) {
+  if (max_items <= 0) return hipSuccess;
+  const TsArgs a = ts_args(L, 0, 0, bucket);
+  const dim3 grid((unsigned)(max_items < TS_GRID ? max_items : TS_GRID));
+  if (bucket == 2) hipLaunchKernelGGL(k_ts_chunk<3>, grid, dim3(ts_threads(3)), 0, s, a);
+  else if (bucket == 1) hipLaunchKernelGGL(k_ts_chunk<5>, grid, dim3(ts_threads(5)), 0, s, a);
+  else hipLaunchKernelGGL(k_ts_chunk<10>, grid, dim3(ts_threads(10)), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_ts_solve(const TsLaunch& L, int pos0, int pos1, hipStream_t s) {
+  if (pos1 <= pos0) return hipSuccess;
+  hipLaunchKernelGGL(k_ts_solve, dim3((unsigned)(pos1 - pos0)), dim3(WAVE), 0, s, ts_args(L, pos0, pos1, 0));
   return hipGetLastError();
 }
 

@@ -57,18 +57,20 @@ def test_noalloc_matches_fitness_batch_two_calls():
 
 def test_noalloc_slices_columns_to_fit_and_rejects_what_does_not():
     ctx = capi.Context(0)
-    print("case: noalloc n=1e6 d=8, 48 candidates in slices of the scratch array", flush=True)
+    print("case: noalloc n=1e6 d=8, 52 candidates in slices of the scratch array", flush=True)
     X, y = data(92, 1_000_000, 8)
     h = ctx.upload(X, y)
     rng = np.random.default_rng(92)
     pool = ["sin x%d" % i for i in range(8)] + ["* x%d x%d" % (i, (i + 1) % 8) for i in range(8)] + \
            ["exp * x%d c0x1p-2" % i for i in range(8)]
     cands = [[pool[j] for j in rng.choice(len(pool), size=int(rng.integers(1, 6)), replace=False)] for _ in range(48)]
+    # and four of more than 8 factors: the step-synchronous QR with its working columns
+    cands += [[pool[j] for j in rng.choice(len(pool), size=int(rng.integers(9, 12)), replace=False)] for _ in range(4)]
     arrs = capi.build_batch(cands)
     assert_same(ctx.fitness_noalloc(h, arrs), ctx.fitness(h, arrs))
     ctx.free(h)
-    print("case: noalloc n=4e6 d=8, one candidate of 8 factors: more than the array holds", flush=True)
-    X, y = data(93, 4_000_000, 8)
+    print("case: noalloc n=6e6 d=8, one candidate of 8 factors: more than the array holds", flush=True)
+    X, y = data(93, 6_000_000, 8)
     h = ctx.upload(X, y)
     wide = capi.build_batch([["sin x%d" % i for i in range(8)]])
     with pytest.raises(RuntimeError, match="scratch array"):
