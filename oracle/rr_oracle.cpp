@@ -1330,11 +1330,14 @@ int orc_tune(int h, const char* pre, char** ols_out, char** factors_out, double*
 }
 
 // A column-major n x k
-int orc_qr_solve(const double* A, const double* b, int64_t n, int k, double* coefs, int* rank) {
+// direct: 1 to take Eigen's order on the data itself whatever k is (the
+// chunked form's cross-check), 0 for the canonical choice by k
+int orc_qr_solve(const double* A, const double* b, int64_t n, int k, double* coefs, int* rank, int direct) {
   std::vector<std::vector<double>> cols(k, std::vector<double>(n));
   for (int j = 0; j < k; ++j) for (int64_t i = 0; i < n; ++i) cols[j][i] = A[(int64_t)j * n + i];
   std::vector<double> bb(b, b + n);
-  std::vector<double> c = colpiv_qr_solve(std::move(cols), bb, n, rank);
+  std::vector<double> c = direct ? colpiv_qr_data(std::move(cols), bb, n, rank)
+                                 : colpiv_qr_solve(std::move(cols), bb, n, rank);
   for (int j = 0; j < k; ++j) coefs[j] = c[j];
   return 0;
 }

@@ -37,7 +37,7 @@ def lib():
         L.orc_score.argtypes = [ctypes.c_int, ctypes.c_char_p, dp]
         L.orc_tune.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.POINTER(ctypes.c_void_p),
                                ctypes.POINTER(ctypes.c_void_p), dp, ctypes.c_int, ip, ip, dp]
-        L.orc_qr_solve.argtypes = [dp, dp, ctypes.c_int64, ctypes.c_int, dp, ip]
+        L.orc_qr_solve.argtypes = [dp, dp, ctypes.c_int64, ctypes.c_int, dp, ip, ctypes.c_int]
         L.orc_rewrite.argtypes = [ctypes.c_char_p, ctypes.c_int]
         L.orc_rewrite.restype = ctypes.c_void_p
         L.orc_all_candidates.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int]
@@ -169,14 +169,16 @@ class Dataset:
         return secs, fit.reshape(-1, 3)
 
 
-def qr_solve(A, b):
+def qr_solve(A, b, direct=False):
+    """Least squares of the OLS form: the canonical QR (chunked up to 9
+    columns), or with direct=True Eigen's order on the data for any k."""
     A = np.asarray(A, dtype=np.float64)
     n, k = A.shape
     Acm = np.ascontiguousarray(A.T)
     b = np.ascontiguousarray(b, dtype=np.float64)
     coefs = np.empty(k)
     rank = ctypes.c_int()
-    lib().orc_qr_solve(_dp(Acm), _dp(b), n, k, _dp(coefs), ctypes.byref(rank))
+    lib().orc_qr_solve(_dp(Acm), _dp(b), n, k, _dp(coefs), ctypes.byref(rank), int(direct))
     return coefs, rank.value
 
 

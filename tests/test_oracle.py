@@ -163,6 +163,127 @@ def test_qr_pivot_order_agrees_with_lapack_geqp3():
     assert np.allclose(c, ref, rtol=1e-10, atol=1e-12)
 
 
+# The chunked QR (up to 9 columns) at sizes of several 4096-row chunks with a
+# ragged last one, in each column bucket (3, 5, 10 columns with y): against
+# least squares, exact integer fits, and Eigen's order on the data itself
+# (the step-by-step restatement pinned above), including rank cuts and NaN /
+# Inf outcomes whose deciding column lives in a single chunk.
+@pytest.mark.parametrize("n", [4097, 12289])
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 5, 9])
+def test_chunked_qr_matches_lstsq_across_chunks(n, k):
+    rng = np.random.default_rng(100 + k)
+    A = np.column_stack([rng.normal(size=(n, k - 1)) * rng.uniform(0.5, 3, k - 1), np.ones(n)])
+    b = A @ rng.normal(size=k) + rng.normal(size=n)
+    c, rank = O.qr_solve(A, b)
+    ref = np.linalg.lstsq(A, b, rcond=None)[0]
+    assert rank == k
+    assert np.allclose(c, ref, rtol=1e-10, atol=1e-12)
+
+
+@pytest.mark.parametrize("n", [8192, 9000])
+def test_chunked_qr_exact_integer_coefficients(n):
+    rng = np.random.default_rng(11)
+    A = np.column_stack([rng.integers(-9, 10, n).astype(float) for _ in range(6)] + [np.ones(n)])
+    coef = np.array([3.0, -2.0, 0.5, 7.0, -11.0, 0.25, 4.0])
+    c, rank = O.qr_solve(A, A @ coef)
+    assert rank == 7 and np.allclose(c, coef, rtol=0, atol=1e-11)
+
+
+def _designs(n):
+    rng = np.random.default_rng(12)
+    x0, x1, x2 = rng.uniform(1, 2, n), rng.normal(size=n), rng.uniform(-1, 3, n)
+    one = np.ones(n)
+    last = np.zeros(n)
+    last[-5:] = rng.normal(size=5)                  # nonzero only in the ragged last chunk
+    mid = np.zeros(n)
+    mid[4096:4200] = rng.normal(size=104)           # nonzero only in chunk 1
+    yield "well-posed", np.column_stack([x0, x1, x2, one])
+    yield "duplicate", np.column_stack([x0, x1, x0, one])
+    yield "constant", np.column_stack([x1, np.full(n, 0.5), one])
+    yield "zero", np.column_stack([x1, np.zeros(n), x2, one])
+    yield "last-chunk-only", np.column_stack([x1, last, one])
+    yield "middle-chunk-only", np.column_stack([x1, mid, x2, one])
+    yield "collinear-sum", np.column_stack([x0, x1, x0 + 2 * x1, x2, one])
+    yield "wide", np.column_stack([x0, x1, x2, x0 * x1, np.sin(x2), x1 ** 2, np.exp(x2 / 4), np.cos(x1), one])
+
+
+@pytest.mark.parametrize("n", [9001])
+def test_chunked_qr_agrees_with_eigen_order_on_the_data(n):
+    rng = np.random.default_rng(13)
+    for name, A in _designs(n):
+        b = rng.normal(size=n) + A @ rng.normal(size=A.shape[1])
+        c, rank = O.qr_solve(A, b)
+        cd, rankd = O.qr_solve(A, b, direct=True)
+        assert rank == rankd, name
+        assert np.array_equal(c == 0.0, cd == 0.0), name
+        assert np.allclose(c, cd, rtol=1e-9, atol=1e-11), name
+        if rank == A.shape[1]:
+            assert np.allclose(c, np.linalg.lstsq(A, b, rcond=None)[0], rtol=1e-9, atol=1e-11), name
+
+
+def test_chunked_qr_borderline_rank_is_an_order_decision():
+    """x0 - x2 beside x0 and x2 is collinear in exact arithmetic but not after
+    rounding: what remains of it after the other columns is eps-sized noise
+    whose size depends on the order of the operations, and Eigen's cut
+    compares it with eps * (largest column norm).  Which side of the cut it
+    falls on is decided by rounding -- for the chunked order, for Eigen's order
+    on the data, and for the reference's own vectorized sums alike.  Pinned:
+    each order decides rank 8 or 9, and a rank-8 answer is the least-squares
+    fit (its residual sum of squares equals lstsq's to 1e-12)."""
+    n = 9001
+    rng = np.random.default_rng(16)
+    x0, x1, x2 = rng.uniform(1, 2, n), rng.normal(size=n), rng.uniform(-1, 3, n)
+    A = np.column_stack([x0, x1, x2, x0 * x1, np.sin(x2), x1 ** 2, np.exp(x2 / 4), x0 - x2, np.ones(n)])
+    for seed in range(4):
+        b = np.random.default_rng(seed).normal(size=n) + A @ np.random.default_rng(seed + 9).normal(size=9)
+        ssr_ls = math.fsum((b - A @ np.linalg.lstsq(A, b, rcond=None)[0]) ** 2)
+        for direct in (False, True):
+            c, rank = O.qr_solve(A, b, direct=direct)
+            assert rank in (8, 9)
+            if rank == 8:
+                assert abs(math.fsum((b - A @ c) ** 2) - ssr_ls) <= 1e-12 * ssr_ls
+
+
+def test_chunked_qr_nan_and_inf_outcomes():
+    n = 9001
+    rng = np.random.default_rng(14)
+    x, one = rng.uniform(1, 2, n), np.ones(n)
+    b = rng.normal(size=n)
+    inf_col = rng.normal(size=n)
+    inf_col[5000] = np.inf                           # Inf norm: pivoted at step 0, all NaN
+    nan_col = rng.normal(size=n)
+    nan_col[8999] = np.nan                           # NaN norm, in the last chunk
+    cases = [
+        ("inf", np.column_stack([x, inf_col, one]), "all"),
+        ("nan reached", np.column_stack([x, nan_col, one]), "all"),
+        ("nan at position 0", np.column_stack([nan_col, x, one]), "all"),
+        # x = 2 * ones: the rank cut at step 1 comes before the NaN column's position
+        ("nan after the cut", np.column_stack([2 * one, one, nan_col]), "cut"),
+    ]
+    for name, A, want in cases:
+        c, rank = O.qr_solve(A, b)
+        cd, rankd = O.qr_solve(A, b, direct=True)
+        assert rank == rankd, name
+        assert np.array_equal(np.isnan(c), np.isnan(cd)), name
+        if want == "all":
+            assert rank == 3 and np.all(np.isnan(c)), name
+        else:
+            assert rank == 1 and not np.any(np.isnan(c)) and c[2] == 0.0, name
+            assert np.count_nonzero(c) == 1 and abs(A[:, :2] @ c[:2] - b.mean()).max() < 1e-12, name
+
+
+def test_chunked_qr_short_matrix_against_eigen_order():
+    rng = np.random.default_rng(15)
+    for n, k in [(1, 2), (2, 4), (3, 5), (5, 9)]:
+        A = rng.normal(size=(n, k))
+        b = rng.normal(size=n)
+        c, rank = O.qr_solve(A, b)
+        cd, rankd = O.qr_solve(A, b, direct=True)
+        assert rank == rankd == n
+        assert np.array_equal(c == 0.0, cd == 0.0)
+        assert np.allclose(A @ c, b, atol=1e-10) and np.allclose(c, cd, rtol=1e-9, atol=1e-11)
+
+
 # ---------------------------------------------------------------- fitness
 def data(seed=0, n=400):
     rng = np.random.default_rng(seed)
