@@ -478,6 +478,8 @@ def main():
                          "same call); split: an RR_FP32_REFINED call, then an RR_FP64 call for the flagged ones")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--fp64-eval-rows", type=int, default=None, help="samples per thread of the FP64 evaluation (1, 2, 4)")
+    ap.add_argument("--fp64-eval-form", type=int, default=None,
+                    help="FP64 factor interpreter: 0 stack in registers, 1 stack in LDS with inlined math")
     ap.add_argument("--fp32-rows", type=int, default=None, help="samples per lane per tile, FP32 kernels (1, 2)")
     ap.add_argument("--fp32-chunk", type=int, default=None, help="samples per FP32 workgroup (multiple of 512)")
     ap.add_argument("--fp32-streams", type=int, default=None, help="1: FP32 buckets on concurrent streams")
@@ -567,6 +569,8 @@ def batched_main(args):
     runner = rils_rols_cpp.BatchRunner(np.ascontiguousarray(X.T), y, 0)
     if args.fp64_eval_rows:
         runner.set_option("fp64_eval_rows", args.fp64_eval_rows)
+    if args.fp64_eval_form is not None:
+        runner.set_option("fp64_eval_form", args.fp64_eval_form)
     if args.fp32_rows:
         runner.set_option("fp32_rows", args.fp32_rows)
     if args.fp32_chunk:
@@ -627,7 +631,8 @@ def batched_main(args):
             "roofline": {"bound": "valu", "achieved": eval_ach, "peak": PEAK_FP64_VALU_TFLOPS, "unit": "TFLOP/s",
                          "frac": eval_ach / PEAK_FP64_VALU_TFLOPS,
                          "traffic": traffic,
-                         "kernel": "k_alive_fp64 + k_eval_fp64<%d>" % runner.get_option("fp64_eval_rows"),
+                         "kernel": "k_alive_fp64 + k_eval_fp64<%d, %d>" % (runner.get_option("fp64_eval_rows"),
+                                                                          runner.get_option("fp64_eval_form")),
                          "launch_ms": ph["eval"], "algorithmic_flops": eval_flops, "factors_evaluated": n_eval},
             "step_roofline": {"bound": "valu", "achieved": step_flops / kernel_s / 1e12,
                               "peak": PEAK_FP64_VALU_TFLOPS, "unit": "TFLOP/s",
@@ -728,6 +733,7 @@ def batched_main(args):
         "forms": out,
         "schedule": {"autotuned": tuned,
                      "fp64_eval_rows": runner.get_option("fp64_eval_rows"),
+                     "fp64_eval_form": runner.get_option("fp64_eval_form"),
                      "fp32_rows": runner.get_option("fp32_rows"),
                      "fp32_chunk_rows": runner.get_option("fp32_chunk_rows"),
                      "fp32_streams": runner.get_option("fp32_streams")},

@@ -228,7 +228,9 @@ int rr_predict_fp32(rr_ctx* ctx, int32_t handle, const uint32_t* code, int32_t l
  * Named integer options: "slab_budget_mb" (device memory the FP64 OLS path
  * may use for its per-candidate work columns, default 32768),
  * "fp64_eval_rows" (samples per thread per pass of the FP64 evaluation of
- * factors and trees: 1, 2 or 4; default 2), "fp32_rows" (the same for the FP32 kernels: 1 or 2; default 2),
+ * factors and trees: 1, 2 or 4; default 2), "fp64_eval_form" (the FP64
+ * factor interpreter: 0 operand stack in registers and transcendentals
+ * called, 1 operand stack in LDS and transcendentals inlined; default 1), "fp32_rows" (the same for the FP32 kernels: 1 or 2; default 2),
  * "fp32_chunk_rows" (samples per workgroup of the FP32 pass, a multiple of
  * 512; default 4096: a candidate's samples are split into chunks that run as
  * independent workgroups, and the chunk sums are added in chunk order),
@@ -240,7 +242,8 @@ int rr_predict_fp32(rr_ctx* ctx, int32_t handle, const uint32_t* code, int32_t l
 int rr_set_option(rr_ctx* ctx, const char* name, int64_t value);
 
 /* Autotuner: time `batch` (mode, precision) under every schedule of that
- * precision -- FP64: samples per thread of the evaluation 1, 2, 4; FP32:
+ * precision -- FP64: samples per thread of the evaluation 1, 2, 4 x
+ * fp64_eval_form 0, 1; FP32:
  * samples per
  * lane 1, 2 x chunk rows 2048..16384 x bucket streams off/on -- taking the
  * best of `reps` runs each, and keep the fastest as the context's options

@@ -276,7 +276,7 @@ class BatchRunner {
     }
     py::dict d;
     d["best_ms"] = ms;
-    for (const char* k : {"fp64_eval_rows", "fp32_rows", "fp32_chunk_rows", "fp32_streams"}) d[k] = get_option(k);
+    for (const char* k : {"fp64_eval_rows", "fp64_eval_form", "fp32_rows", "fp32_chunk_rows", "fp32_streams"}) d[k] = get_option(k);
     return d;
   }
   void set_option(const std::string& k, int64_t v) { check(rr_set_option(ctx_, k.c_str(), v), "rr_set_option"); }

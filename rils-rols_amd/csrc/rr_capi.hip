@@ -148,6 +148,7 @@ struct rr_ctx {
   HostBuf h_stage, h_out;
   int64_t slab_budget_mb = 32768;
   int64_t fp64_eval_rows = 2;
+  int64_t fp64_eval_form = 1;    // the factor interpreter: 0 stack in registers, 1 stack in LDS
   int64_t fp32_rows = 2;
   int64_t fp32_chunk_rows = 4096;
   int64_t fp32_streams = 0;   // 1: bucket kernels on concurrent side streams
@@ -246,6 +247,17 @@ int check_program(const uint32_t* code, int32_t len, int32_t d, int32_t n_consts
   if (sp != 0 || !acc) return fail(RR_E_ARG, "program leaves the stack unbalanced");
   if (max_depth) *max_depth = std::max(*max_depth, depth);
   return RR_OK;
+}
+
+// The deepest operand stack of a (checked) program.
+int stack_depth(const uint32_t* code, int32_t len) {
+  int sp = 0, depth = 0;
+  for (int32_t pc = 0; pc < len; ++pc) {
+    const uint32_t op = code[pc] & 0xffu;
+    if (op == RR_OP_PUSH) depth = std::max(depth, ++sp);
+    else if (op >= RR_OP_BIN && (((op - RR_OP_BIN) & 7u) == RR_M_S || ((op - RR_OP_BIN) & 7u) == RR_M_SR)) --sp;
+  }
+  return depth;
 }
 
 // Does a (checked) program read a sample variable?  One that does not is a
@@ -834,7 +846,7 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
   // ts_it[i+1]-1 have TS_NC[i] columns with y (positions and items relative to
   // the chunk's first).
   struct Chunk64 {
-    int c0, c1, maxk, p0, npair, nitem, per, sitem, sper, ne;
+    int c0, c1, maxk, p0, npair, nitem, per, sitem, sper, ne, depth;
     int ts_pos[4], ts_it[4];
     int64_t ts_lw[4];   // the buckets' work-item descriptors: first word, and the words in all
     std::vector<int32_t> step_pos, step_items;
@@ -1047,6 +1059,10 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
       }
       if (items > INT32_MAX) return fail(RR_E_LIMIT, "too many factor-evaluation work items in one launch");
       ch.nitem = (int)items;
+      ch.depth = 0;   // the deepest operand stack of the chunk's factor programs
+      for (int u = ch.p0; u < ch.p0 + ch.npair; ++u)
+        ch.depth = std::max(ch.depth, stack_depth(b->code + b->prog_start[u_prog[u]],
+                                                   b->prog_start[u_prog[u] + 1] - b->prog_start[u_prog[u]]));
       max_items = std::max(max_items, items);
       ch.per = per > 0 ? (int)per : 0;
     }
@@ -1169,7 +1185,8 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
         E.fac_kept = (const int64_t*)ctx->d_fac_kept.p + ch.p0;
         E.alive = (const int32_t*)ctx->d_alive.p + ch.p0;
         E.n_facs = ch.npair; E.n_items = ch.nitem; E.items_per_fac = ch.per;
-        E.norm_part = (double*)ctx->d_norm_part.p; E.rows = (int)ctx->fp64_eval_rows;
+        E.norm_part = (double*)ctx->d_norm_part.p; E.rows = (int)ctx->fp64_eval_rows; E.depth = ch.depth;
+        E.form = (int)ctx->fp64_eval_form;
         RR_HIP(rr::launch_eval_fp64(E, stream));
       }
       end_phase(ctx, pm, stream);
@@ -1266,7 +1283,8 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
     raw64.cand_start = dc; raw64.prog_start = dp; raw64.code = dcode; raw64.K = dK; raw64.ds = dtab;
     raw64.cand_ds = dcds; raw64.prog_kept = nullptr; raw64.order = nullptr; raw64.dead = nullptr; raw64.info = nullptr;
     raw64.pos_item = (const int32_t*)ctx->d_pos_item.p; raw64.n_pos = nc; raw64.n_items = (int)it;
-    raw64.items_per_pos = per > 0 ? (int)per : 0; raw64.raw = 1; raw64.rows = (int)ctx->fp64_eval_rows;
+    raw64.items_per_pos = per > 0 ? (int)per : 0; raw64.raw = 1;
+    raw64.rows = (int)ctx->fp64_eval_rows;
     raw64.part = (double*)ctx->d_score_part.p; raw64.fit = dfit;
   }
   if (checked && !list64.empty()) { if ((rc = plan64(&list64, 0, plan))) return rc; }
@@ -1447,16 +1465,17 @@ int rr_autotune(rr_ctx* ctx, int32_t handle, const rr_batch* b, int32_t mode, in
   struct Cfg { int64_t a, b2, c; };
   std::vector<Cfg> cfgs;
   if (precision == RR_FP64) {
-    for (int64_t e : {1, 2, 4}) cfgs.push_back({e, 0, 0});
+    for (int64_t e : {1, 2, 4})
+      for (int64_t f : {0, 1}) cfgs.push_back({e, f, 0});
   } else {
     for (int64_t r : {1, 2})
       for (int64_t ch : {2048, 4096, 8192, 16384})
         for (int64_t st : {0, 1}) cfgs.push_back({r, ch, st});
   }
-  const Cfg keep = precision == RR_FP64 ? Cfg{ctx->fp64_eval_rows, 0, 0}
+  const Cfg keep = precision == RR_FP64 ? Cfg{ctx->fp64_eval_rows, ctx->fp64_eval_form, 0}
                                          : Cfg{ctx->fp32_rows, ctx->fp32_chunk_rows, ctx->fp32_streams};
   auto apply = [&](const Cfg& c) {
-    if (precision == RR_FP64) ctx->fp64_eval_rows = c.a;
+    if (precision == RR_FP64) { ctx->fp64_eval_rows = c.a; ctx->fp64_eval_form = c.b2; }
     else { ctx->fp32_rows = c.a; ctx->fp32_chunk_rows = c.b2; ctx->fp32_streams = c.c; }
   };
   double best = 1e300;
@@ -1484,6 +1503,9 @@ int rr_set_option(rr_ctx* ctx, const char* name, int64_t value) {
   else if (s == "fp64_eval_rows") {
     if (value != 1 && value != 2 && value != 4) return fail(RR_E_ARG, "fp64_eval_rows must be 1, 2 or 4");
     ctx->fp64_eval_rows = value;
+  } else if (s == "fp64_eval_form") {
+    if (value != 0 && value != 1) return fail(RR_E_ARG, "fp64_eval_form must be 0 or 1");
+    ctx->fp64_eval_form = value;
   } else if (s == "fp32_rows") {
     if (value != 1 && value != 2) return fail(RR_E_ARG, "fp32_rows must be 1 or 2");
     ctx->fp32_rows = value;
@@ -1508,6 +1530,7 @@ int rr_get_option(rr_ctx* ctx, const char* name, int64_t* value) {
   const std::string s(name);
   if (s == "slab_budget_mb") *value = ctx->slab_budget_mb;
   else if (s == "fp64_eval_rows") *value = ctx->fp64_eval_rows;
+  else if (s == "fp64_eval_form") *value = ctx->fp64_eval_form;
   else if (s == "fp32_rows") *value = ctx->fp32_rows;
   else if (s == "fp32_chunk_rows") *value = ctx->fp32_chunk_rows;
   else if (s == "fp32_streams") *value = ctx->fp32_streams;

@@ -245,4 +245,109 @@ __device__ inline void eval_f64(const uint32_t* __restrict__ code, int len, cons
   }
 }
 
+// sin / cos with the large-argument reduction as a call (rrm::reduce's policy)
+__device__ __noinline__ void reduce_large_call(double ax, double* rh, double* rl, int* q) {
+  rrm::reduce_large(ax, rh, rl, q);
+}
+struct LargeCall {
+  __device__ static void run(double ax, double* rh, double* rl, int* q) { reduce_large_call(ax, rh, rl, q); }
+};
+
+// The same program semantics as eval_f64 for R samples per thread (row0 +
+// ST r) with the operand stack in LDS -- slot s of sample r of thread t at
+// stk[(s R + r) ST + t], so each thread touches only its own words and no
+// barrier is needed; stk holds the program's stack depth x R x ST doubles.
+// The accumulator stays in registers.  Only it and a few addresses are live
+// across an operation, so the transcendentals are inlined (two samples at a
+// time; the large-argument reduction of sin and cos stays a call) and no
+// registers are saved around calls.
+template <int R, int ST>
+__device__ inline void eval_lds_f64(const uint32_t* __restrict__ code, int len, const double* __restrict__ K,
+                                    const double* __restrict__ X, int64_t ldx, int64_t row0, double (&acc)[R],
+                                    double* __restrict__ stk) {
+  const int t = (int)threadIdx.x;
+#pragma unroll
+  for (int r = 0; r < R; ++r) acc[r] = 0.0;
+  int sp = 0;
+  for (int pc = 0; pc < len; ++pc) {
+    const uint32_t ins = code[pc];
+    const uint32_t op = ins & 0xffu;
+    const uint32_t arg = ins >> 8;
+    if (op >= RR_OP_BIN) {
+      const uint32_t b = op - RR_OP_BIN;
+      const uint32_t bop = b >> 3, mode = b & 7u;
+      double o[R];   // the operand that is not the accumulator
+      if (mode == RR_M_S || mode == RR_M_SR) {
+        --sp;
+#pragma unroll
+        for (int r = 0; r < R; ++r) o[r] = stk[(sp * R + r) * ST + t];
+      } else if (mode == RR_M_V || mode == RR_M_VR) {
+        const double* xc = X + (int64_t)arg * ldx + row0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) o[r] = xc[(int64_t)ST * r];
+      } else {
+        const double k = K[arg];
+#pragma unroll
+        for (int r = 0; r < R; ++r) o[r] = k;
+      }
+      // the accumulator is the left operand for SR, V and C, the right one
+      // for S, VR and CR (one uniform branch each way)
+      if (mode == RR_M_SR || mode == RR_M_V || mode == RR_M_C) {
+        switch (bop) {
+#define RR_BIN(B) case B: _Pragma("unroll") for (int r = 0; r < R; ++r) acc[r] = bin_f64(B, acc[r], o[r]); break;
+          RR_BIN(RR_BOP_ADD) RR_BIN(RR_BOP_SUB) RR_BIN(RR_BOP_MUL) RR_BIN(RR_BOP_DIV) RR_BIN(RR_BOP_LT)
+          RR_BIN(RR_BOP_GT) RR_BIN(RR_BOP_EQ) RR_BIN(RR_BOP_NEQ) RR_BIN(RR_BOP_MIN)
+          default: _Pragma("unroll") for (int r = 0; r < R; ++r) acc[r] = bin_f64(RR_BOP_MAX, acc[r], o[r]); break;
+#undef RR_BIN
+        }
+      } else {
+        switch (bop) {
+#define RR_BIN(B) case B: _Pragma("unroll") for (int r = 0; r < R; ++r) acc[r] = bin_f64(B, o[r], acc[r]); break;
+          RR_BIN(RR_BOP_ADD) RR_BIN(RR_BOP_SUB) RR_BIN(RR_BOP_MUL) RR_BIN(RR_BOP_DIV) RR_BIN(RR_BOP_LT)
+          RR_BIN(RR_BOP_GT) RR_BIN(RR_BOP_EQ) RR_BIN(RR_BOP_NEQ) RR_BIN(RR_BOP_MIN)
+          default: _Pragma("unroll") for (int r = 0; r < R; ++r) acc[r] = bin_f64(RR_BOP_MAX, o[r], acc[r]); break;
+#undef RR_BIN
+        }
+      }
+    } else {
+      switch (op) {
+        case RR_OP_LDV: {
+          const double* xc = X + (int64_t)arg * ldx + row0;
+#pragma unroll
+          for (int r = 0; r < R; ++r) acc[r] = xc[(int64_t)ST * r];
+          break;
+        }
+        case RR_OP_LDC: {
+          const double k = K[arg];
+#pragma unroll
+          for (int r = 0; r < R; ++r) acc[r] = k;
+          break;
+        }
+        case RR_OP_PUSH:
+#pragma unroll
+          for (int r = 0; r < R; ++r) stk[(sp * R + r) * ST + t] = acc[r];
+          ++sp;
+          break;
+#define RR_UN2(OPC, FN)                                          \
+  case OPC:                                                      \
+    _Pragma("unroll") for (int r = 0; r < R; ++r) acc[r] = FN(acc[r]); \
+    break;
+        RR_UN2(RR_OP_SIN, rrm::sin<LargeCall>)
+        RR_UN2(RR_OP_COS, rrm::cos<LargeCall>)
+        RR_UN2(RR_OP_LN, rrm::log)
+        RR_UN2(RR_OP_EXP, rrm::exp)
+#undef RR_UN2
+        case RR_OP_SQRT:
+#pragma unroll
+          for (int r = 0; r < R; ++r) acc[r] = rrm::sqrt(acc[r]);
+          break;
+        default:  // RR_OP_SQR
+#pragma unroll
+          for (int r = 0; r < R; ++r) acc[r] = acc[r] * acc[r];
+          break;
+      }
+    }
+  }
+}
+
 }  // namespace rr

@@ -204,6 +204,8 @@ struct EvalLaunch {
   int items_per_fac;         // > 0 when every factor has that many items (factor = item / items_per_fac)
   double* norm_part;         // 2 doubles per work item
   int rows;                  // rows per thread per pass: 1, 2 or 4
+  int form;                  // 0: operand stack in registers, transcendentals called; 1: stack in LDS, inlined
+  int depth;                 // the deepest operand stack of the launch's factor programs
 };
 constexpr int64_t EVAL_CHUNK_ROWS = 4096;   // the chunk of the norms' order (rrm::NORM_CHUNK)
 

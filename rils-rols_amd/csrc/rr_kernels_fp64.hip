@@ -837,12 +837,16 @@ struct EvalArgs {
 constexpr int EV_THREADS = 256;
 static_assert(EVAL_CHUNK_ROWS == rrm::NORM_CHUNK, "evaluation work items are the chunks of the norm order");
 
-// Eight waves per SIMD: at two rows per thread the interpreter then fits in
-// 64 VGPRs without new spills (68 unbounded, seven waves); 3% faster.
-template <int R>
-__global__ __launch_bounds__(EV_THREADS, 8) void k_eval_fp64(EvalArgs a) {
+// FORM 0: eval_f64 with its stack in registers and the transcendentals
+// called; eight waves per SIMD (at two rows per thread the interpreter then
+// fits in 64 VGPRs without new spills).  FORM 1: eval_lds_f64, the operand
+// stack in dynamic LDS (the launch's deepest program x R x EV_THREADS
+// doubles) and the transcendentals inlined.
+template <int R, int FORM>
+__global__ __launch_bounds__(EV_THREADS, FORM == 0 ? 8 : R <= 2 ? 6 : 4) void k_eval_fp64(EvalArgs a) {
   static_assert(EV_THREADS == rrm::NORM_PARTIALS, "one thread per partial of the chunked order");
   __shared__ double red[EV_THREADS / WAVE][2];
+  extern __shared__ double ev_stk[];
   const int item = (int)blockIdx.x;
   int lo;
   if (a.items_per_fac > 0) {
@@ -871,7 +875,8 @@ __global__ __launch_bounds__(EV_THREADS, 8) void k_eval_fp64(EvalArgs a) {
   for (int64_t base = r0; base < r1; base += (int64_t)EV_THREADS * R) {
     const int64_t row0 = base + (int64_t)threadIdx.x;
     double v[R];
-    eval_f64<R, EV_THREADS>(code, len, a.K, D.X, D.ldx, row0, v);
+    if constexpr (FORM == 1) eval_lds_f64<R, EV_THREADS>(code, len, a.K, D.X, D.ldx, row0, v, ev_stk);
+    else eval_f64<R, EV_THREADS>(code, len, a.K, D.X, D.ldx, row0, v);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int64_t i = row0 + (int64_t)EV_THREADS * r;
@@ -1645,9 +1650,18 @@ hipError_t launch_eval_fp64(const EvalLaunch& L, hipStream_t s) {
   EvalArgs a{L.prog_start, L.code,     L.K,        L.ds,       L.cand_ds, L.slab,          L.fac_prog,
              L.fac_cand,   L.fac_item, L.fac_kept, L.alive, L.n_facs,  L.items_per_fac, L.norm_part};
   const dim3 grid((unsigned)L.n_items), block(EV_THREADS);
-  if (L.rows == 1) hipLaunchKernelGGL(k_eval_fp64<1>, grid, block, 0, s, a);
-  else if (L.rows == 4) hipLaunchKernelGGL(k_eval_fp64<4>, grid, block, 0, s, a);
-  else hipLaunchKernelGGL(k_eval_fp64<2>, grid, block, 0, s, a);
+  // form 1's LDS stack: depth x rows x EV_THREADS doubles, within the 64 KiB
+  // a workgroup may ask for; deeper launches take form 0
+  const size_t stk = (size_t)(L.depth > 0 ? L.depth : 1) * (size_t)L.rows * EV_THREADS * sizeof(double);
+  if (L.form == 1 && stk <= 65536 && (L.rows == 1 || L.rows == 2 || L.rows == 4)) {
+    if (L.rows == 1) hipLaunchKernelGGL((k_eval_fp64<1, 1>), grid, block, stk, s, a);
+    else if (L.rows == 4) hipLaunchKernelGGL((k_eval_fp64<4, 1>), grid, block, stk, s, a);
+    else hipLaunchKernelGGL((k_eval_fp64<2, 1>), grid, block, stk, s, a);
+  } else {
+    if (L.rows == 1) hipLaunchKernelGGL((k_eval_fp64<1, 0>), grid, block, 0, s, a);
+    else if (L.rows == 4) hipLaunchKernelGGL((k_eval_fp64<4, 0>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((k_eval_fp64<2, 0>), grid, block, 0, s, a);
+  }
   return hipGetLastError();
 }
 

@@ -42,6 +42,40 @@ RR_HD inline uint64_t d2bits(double d) { return __builtin_bit_cast(uint64_t, d);
 RR_HD inline double qnan() { return bits2d(0xfff8000000000000ULL); }
 RR_HD inline double pinf() { return bits2d(0x7ff0000000000000ULL); }
 
+// Operations with a constant operand k: a * b + k, a * k + c, a * k.  The same
+// correctly rounded fma / product on every compiler.  In GPU code each is one
+// VOP3 instruction that reads k from a scalar register pair: left to itself
+// the compiler picks v_fmac_f64 (the addend in the destination register) or a
+// vector operand, and copies each 64-bit constant into vector registers with
+// two v_mov_b32 -- vector-unit issue slots the FP64 kernels are short of.
+RR_HD inline double fma_k(double a, double b, double k) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  double d;
+  asm("v_fma_f64 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "s"(k));
+  return d;
+#else
+  return __builtin_fma(a, b, k);
+#endif
+}
+RR_HD inline double fma_kv(double a, double k, double c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  double d;
+  asm("v_fma_f64 %0, %1, %2, %3" : "=v"(d) : "v"(a), "s"(k), "v"(c));
+  return d;
+#else
+  return __builtin_fma(a, k, c);
+#endif
+}
+RR_HD inline double mul_k(double a, double k) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  double d;
+  asm("v_mul_f64 %0, %1, %2" : "=v"(d) : "v"(a), "s"(k));
+  return d;
+#else
+  return a * k;
+#endif
+}
+
 // The canonical order of every sum over samples in the FP64 form: 1024
 // partial sums, row i added to partial (i mod 1024) in increasing i; the
 // partials form 16 groups of 64 (one wavefront each on the GPU), each group
@@ -114,22 +148,21 @@ RR_HD inline double exp(double x) {
   const double INV_LN2 = 0x1.71547652b82fep+0;
   const double LN2_HI = 0x1.62e42fefa39efp-1;
   const double LN2_LO = 0x1.abc9e3b39803fp-56;
-  const double kd = __builtin_rint(x * INV_LN2);
-  double r = __builtin_fma(-kd, LN2_HI, x);
-  r = __builtin_fma(-kd, LN2_LO, r);
+  const double kd = __builtin_rint(mul_k(x, INV_LN2));
+  double r = fma_kv(-kd, LN2_HI, x);
+  r = fma_kv(-kd, LN2_LO, r);
   // P(r) = sum_{j=2}^{13} r^(j-2)/j!
-  double p = 0x1.6124613a86d09p-33;              // 1/13!
-  p = __builtin_fma(p, r, 0x1.1eed8eff8d898p-29); // 1/12!
-  p = __builtin_fma(p, r, 0x1.ae64567f544e4p-26); // 1/11!
-  p = __builtin_fma(p, r, 0x1.27e4fb7789f5cp-22); // 1/10!
-  p = __builtin_fma(p, r, 0x1.71de3a556c734p-19); // 1/9!
-  p = __builtin_fma(p, r, 0x1.a01a01a01a01ap-16); // 1/8!
-  p = __builtin_fma(p, r, 0x1.a01a01a01a01ap-13); // 1/7!
-  p = __builtin_fma(p, r, 0x1.6c16c16c16c17p-10); // 1/6!
-  p = __builtin_fma(p, r, 0x1.1111111111111p-7);  // 1/5!
-  p = __builtin_fma(p, r, 0x1.5555555555555p-5);  // 1/4!
-  p = __builtin_fma(p, r, 0x1.5555555555555p-3);  // 1/3!
-  p = __builtin_fma(p, r, 0x1.0000000000000p-1);  // 1/2!
+  double p = fma_kv(r, 0x1.6124613a86d09p-33, 0x1.1eed8eff8d898p-29);   // 1/13!, 1/12!
+  p = fma_k(p, r, 0x1.ae64567f544e4p-26); // 1/11!
+  p = fma_k(p, r, 0x1.27e4fb7789f5cp-22); // 1/10!
+  p = fma_k(p, r, 0x1.71de3a556c734p-19); // 1/9!
+  p = fma_k(p, r, 0x1.a01a01a01a01ap-16); // 1/8!
+  p = fma_k(p, r, 0x1.a01a01a01a01ap-13); // 1/7!
+  p = fma_k(p, r, 0x1.6c16c16c16c17p-10); // 1/6!
+  p = fma_k(p, r, 0x1.1111111111111p-7);  // 1/5!
+  p = fma_k(p, r, 0x1.5555555555555p-5);  // 1/4!
+  p = fma_k(p, r, 0x1.5555555555555p-3);  // 1/3!
+  p = fma_k(p, r, 0x1.0000000000000p-1);  // 1/2!
   const double em1 = __builtin_fma(r * r, p, r);  // e^r - 1
   const double e = 1.0 + em1;
   const int k = (int)kd;
@@ -154,24 +187,23 @@ RR_HD inline double log(double x) {
   const double s = f / (2.0 + f);
   const double z = s * s;
   // R(z) = sum_{j=1}^{12} 2 z^j/(2j+1)
-  double R = 0x1.47ae147ae147bp-4;               // 2/25
-  R = __builtin_fma(R, z, 0x1.642c8590b2164p-4);  // 2/23
-  R = __builtin_fma(R, z, 0x1.8618618618618p-4);  // 2/21
-  R = __builtin_fma(R, z, 0x1.af286bca1af28p-4);  // 2/19
-  R = __builtin_fma(R, z, 0x1.e1e1e1e1e1e1ep-4);  // 2/17
-  R = __builtin_fma(R, z, 0x1.1111111111111p-3);  // 2/15
-  R = __builtin_fma(R, z, 0x1.3b13b13b13b14p-3);  // 2/13
-  R = __builtin_fma(R, z, 0x1.745d1745d1746p-3);  // 2/11
-  R = __builtin_fma(R, z, 0x1.c71c71c71c71cp-3);  // 2/9
-  R = __builtin_fma(R, z, 0x1.2492492492492p-2);  // 2/7
-  R = __builtin_fma(R, z, 0x1.999999999999ap-2);  // 2/5
-  R = __builtin_fma(R, z, 0x1.5555555555555p-1);  // 2/3
+  double R = fma_kv(z, 0x1.47ae147ae147bp-4, 0x1.642c8590b2164p-4);   // 2/25, 2/23
+  R = fma_k(R, z, 0x1.8618618618618p-4);  // 2/21
+  R = fma_k(R, z, 0x1.af286bca1af28p-4);  // 2/19
+  R = fma_k(R, z, 0x1.e1e1e1e1e1e1ep-4);  // 2/17
+  R = fma_k(R, z, 0x1.1111111111111p-3);  // 2/15
+  R = fma_k(R, z, 0x1.3b13b13b13b14p-3);  // 2/13
+  R = fma_k(R, z, 0x1.745d1745d1746p-3);  // 2/11
+  R = fma_k(R, z, 0x1.c71c71c71c71cp-3);  // 2/9
+  R = fma_k(R, z, 0x1.2492492492492p-2);  // 2/7
+  R = fma_k(R, z, 0x1.999999999999ap-2);  // 2/5
+  R = fma_k(R, z, 0x1.5555555555555p-1);  // 2/3
   R = R * z;
   const double hfsq = 0.5 * f * f;
   const double LN2_HI32 = 0x1.62e42fee00000p-1;  // ln2 to 32 bits: e*LN2_HI32 exact
   const double LN2_LO32 = 0x1.a39ef35793c76p-33;
   const double de = (double)e;
-  return de * LN2_HI32 + (f - (hfsq - (s * (hfsq + R) + de * LN2_LO32)));
+  return mul_k(de, LN2_HI32) + (f - (hfsq - (s * (hfsq + R) + mul_k(de, LN2_LO32))));
 }
 
 // ---------------------------------------------------------------- sin / cos
@@ -258,29 +290,38 @@ RR_HD inline void reduce_large(double ax, double* rh, double* rl, int* q) {
   *q = qq;
 }
 
+// How sin and cos reach reduce_large: inline (the default), or through a
+// policy of the caller's (GPU code that inlines sin and cos keeps the rare
+// large-argument path out of line: its 256-bit arithmetic would otherwise
+// hold registers for the whole kernel).  The arithmetic is the same.
+struct LargeInline {
+  RR_HD static void run(double ax, double* rh, double* rl, int* q) { reduce_large(ax, rh, rl, q); }
+};
+
 // x = q pi/2 + (rh + rl), |rh| <~ pi/4.  x finite.
+template <class L = LargeInline>
 RR_HD inline void reduce(double x, double* rh, double* rl, int* q) {
   const double ax = __builtin_fabs(x);
   if (ax < 0x1p28) {
     const double TWO_OVER_PI = 0x1.45f306dc9c883p-1;
     const double P1 = 0x1.921fb54442d18p+0, P2 = 0x1.1a62633145c07p-54, P3 = -0x1.f1976b7ed8fbcp-110;
-    const double kd = __builtin_rint(x * TWO_OVER_PI);
-    const double p1 = kd * P1, e1 = __builtin_fma(kd, P1, -p1);   // kd*P1 = p1 + e1 exactly
+    const double kd = __builtin_rint(mul_k(x, TWO_OVER_PI));
+    const double p1 = mul_k(kd, P1), e1 = fma_kv(kd, P1, -p1);   // kd*P1 = p1 + e1 exactly
     const double t = x - p1;                                      // exact (Sterbenz) for kd != 0
-    const double p2 = kd * P2, e2 = __builtin_fma(kd, P2, -p2);
+    const double p2 = mul_k(kd, P2), e2 = fma_kv(kd, P2, -p2);
     const double u = t - p2;                                      // two-sum: u + ue = t - p2
     const double bv = u - t;
     double ue = (t - (u - bv)) + (-p2 - bv);
     ue = ue - e1;
     ue = ue - e2;
-    ue = ue - kd * P3;
+    ue = ue - mul_k(kd, P3);
     const double h = u + ue;
     *rl = ue - (h - u);
     *rh = h;
     *q = ((int)kd) & 3;
   } else {
     int qq;
-    reduce_large(ax, rh, rl, &qq);
+    L::run(ax, rh, rl, &qq);
     if (x < 0.0) { *rh = -*rh; *rl = -*rl; qq = (4 - qq) & 3; }
     *q = qq;
   }
@@ -289,14 +330,13 @@ RR_HD inline void reduce(double x, double* rh, double* rl, int* q) {
 // sin(rh + rl) for |rh| <= ~pi/4
 RR_HD inline double ksin(double rh, double rl) {
   const double z = rh * rh;
-  double S = 0x1.952c77030ad4ap-49;               //  1/17!
-  S = __builtin_fma(S, z, -0x1.ae7f3e733b81fp-41); // -1/15!
-  S = __builtin_fma(S, z, 0x1.6124613a86d09p-33);  //  1/13!
-  S = __builtin_fma(S, z, -0x1.ae64567f544e4p-26); // -1/11!
-  S = __builtin_fma(S, z, 0x1.71de3a556c734p-19);  //  1/9!
-  S = __builtin_fma(S, z, -0x1.a01a01a01a01ap-13); // -1/7!
-  S = __builtin_fma(S, z, 0x1.1111111111111p-7);   //  1/5!
-  S = __builtin_fma(S, z, -0x1.5555555555555p-3);  // -1/3!
+  double S = fma_kv(z, 0x1.952c77030ad4ap-49, -0x1.ae7f3e733b81fp-41);   //  1/17!, -1/15!
+  S = fma_k(S, z, 0x1.6124613a86d09p-33);  //  1/13!
+  S = fma_k(S, z, -0x1.ae64567f544e4p-26); // -1/11!
+  S = fma_k(S, z, 0x1.71de3a556c734p-19);  //  1/9!
+  S = fma_k(S, z, -0x1.a01a01a01a01ap-13); // -1/7!
+  S = fma_k(S, z, 0x1.1111111111111p-7);   //  1/5!
+  S = fma_k(S, z, -0x1.5555555555555p-3);  // -1/3!
   const double tail = (z * rh) * S + rl * (1.0 - 0.5 * z);
   return rh + tail;
 }
@@ -305,25 +345,25 @@ RR_HD inline double ksin(double rh, double rl) {
 //   cos r = 1 - z/2 + z^2 C(z),  C(z) = 1/4! - z/6! + z^2/8! - ... - z^7/18!
 RR_HD inline double kcos(double rh, double rl) {
   const double z = rh * rh;
-  double C = -0x1.6827863b97d97p-53;              // -1/18!
-  C = __builtin_fma(C, z, 0x1.ae7f3e733b81fp-45);  //  1/16!
-  C = __builtin_fma(C, z, -0x1.93974a8c07c9dp-37); // -1/14!
-  C = __builtin_fma(C, z, 0x1.1eed8eff8d898p-29);  //  1/12!
-  C = __builtin_fma(C, z, -0x1.27e4fb7789f5cp-22); // -1/10!
-  C = __builtin_fma(C, z, 0x1.a01a01a01a01ap-16);  //  1/8!
-  C = __builtin_fma(C, z, -0x1.6c16c16c16c17p-10); // -1/6!
-  C = __builtin_fma(C, z, 0x1.5555555555555p-5);   //  1/4!
+  double C = fma_kv(z, -0x1.6827863b97d97p-53, 0x1.ae7f3e733b81fp-45);   // -1/18!, 1/16!
+  C = fma_k(C, z, -0x1.93974a8c07c9dp-37); // -1/14!
+  C = fma_k(C, z, 0x1.1eed8eff8d898p-29);  //  1/12!
+  C = fma_k(C, z, -0x1.27e4fb7789f5cp-22); // -1/10!
+  C = fma_k(C, z, 0x1.a01a01a01a01ap-16);  //  1/8!
+  C = fma_k(C, z, -0x1.6c16c16c16c17p-10); // -1/6!
+  C = fma_k(C, z, 0x1.5555555555555p-5);   //  1/4!
   const double hz = 0.5 * z;
   const double w = 1.0 - hz;
   const double err = (1.0 - w) - hz;              // rounding error of w, exact
   return w + (err + ((z * z) * C - rh * rl));
 }
 
+template <class L = LargeInline>
 RR_HD inline double sin(double x) {
   if (!(x - x == 0.0)) return qnan();            // NaN or inf
   if (__builtin_fabs(x) < 0x1p-27) return x;
   double rh, rl; int q;
-  reduce(x, &rh, &rl, &q);
+  reduce<L>(x, &rh, &rl, &q);
   switch (q) {
     case 0: return ksin(rh, rl);
     case 1: return kcos(rh, rl);
@@ -332,11 +372,12 @@ RR_HD inline double sin(double x) {
   }
 }
 
+template <class L = LargeInline>
 RR_HD inline double cos(double x) {
   if (!(x - x == 0.0)) return qnan();
   if (__builtin_fabs(x) < 0x1p-27) return 1.0;
   double rh, rl; int q;
-  reduce(x, &rh, &rl, &q);
+  reduce<L>(x, &rh, &rl, &q);
   switch (q) {
     case 0: return kcos(rh, rl);
     case 1: return -ksin(rh, rl);

@@ -96,6 +96,46 @@ def test_ols_bitexact(ctx, n, d, seed):
     ctx.free(h)
 
 
+def _full_tree(rng, levels, d):
+    """A full binary tree: its program needs levels - 1 stack slots even in
+    the best evaluation order."""
+    if levels == 0:
+        return "x%d" % rng.integers(d)
+    op = rng.choice(["*", "/", "-", "+", "max"]) if levels > 1 else rng.choice(["*", "-", "+"])
+    return "%s %s %s" % (op, _full_tree(rng, levels - 1, d), _full_tree(rng, levels - 1, d))
+
+
+@pytest.mark.parametrize("form,rows", [(1, 1), (1, 4), (0, 1), (0, 2), (0, 4)])
+def test_eval_forms_bitexact(ctx, form, rows):
+    """Both factor interpreters (fp64_eval_form 0: operand stack in registers,
+    transcendentals called; 1: stack in LDS, inlined) at every row count
+    besides the default (form 1, 2 rows, which every other test runs): the
+    same bits as the oracle, over a ragged last chunk, with programs of stack
+    depth up to 4."""
+    print("case: eval form %d rows=%d" % (form, rows), flush=True)
+    X, y = make_data(61, 9001, 4)
+    rng = np.random.default_rng(61)
+    trees = treegen.random_trees(61, 200, 3, 40, 4)
+    shallow = trees[:]
+    deep = ["* sin %s cos %s" % (_full_tree(rng, L, 4), _full_tree(rng, L, 4)) for L in (2, 3, 4, 5) for _ in range(3)]
+    od = O.Dataset(X, y)
+    h = ctx.upload(X, y)
+    ctx.set_option("fp64_eval_rows", rows)
+    ctx.set_option("fp64_eval_form", form)
+    try:
+        for batch in (shallow, shallow + deep):
+            tuned = [od.tune(t) for t in batch]
+            coefs, fit = ctx.fitness(h, capi.build_batch([tu["factors"] for tu in tuned]), mode="ols")
+            for t, tu, c, f in zip(batch, tuned, coefs, fit):
+                assert same_bits(c, tu["coefs"]), (t, c, tu["coefs"])
+                assert f["rank"] == tu["rank"], t
+                assert same_bits([f["e0"], f["e1"]], [tu["fit"][0], tu["fit"][1]]), (t, f, tu["fit"])
+    finally:
+        ctx.set_option("fp64_eval_rows", 2)
+        ctx.set_option("fp64_eval_form", 1)
+    ctx.free(h)
+
+
 def test_ols_chunked_slab_same_bits(ctx):
     """A tiny slab budget forces many launches; results must not change."""
     X, y = make_data(21, 3000, 4)
@@ -172,13 +212,14 @@ def test_autotune_keeps_fp64_bits(ctx):
     ctx.set_option("fp64_eval_rows", 1)
     c1, f1 = ctx.fitness(h, arrs)
     ms = ctx.autotune(h, arrs, precision="fp64")
-    assert ms > 0 and ctx.get_option("fp64_eval_rows") in (1, 2, 4)
+    assert ms > 0 and ctx.get_option("fp64_eval_rows") in (1, 2, 4) and ctx.get_option("fp64_eval_form") in (0, 1)
     c2, f2 = ctx.fitness(h, arrs)
     assert all(same_bits(a, b) for a, b in zip(c1, c2))
     assert same_bits(f1["e0"], f2["e0"]) and same_bits(f1["e1"], f2["e1"])
     ms32 = ctx.autotune(h, arrs, precision="fp32")
     assert ms32 > 0 and ctx.get_option("fp32_chunk_rows") in (2048, 4096, 8192, 16384)
     ctx.set_option("fp64_eval_rows", 2)
+    ctx.set_option("fp64_eval_form", 1)
     ctx.set_option("fp32_rows", 2)
     ctx.set_option("fp32_chunk_rows", 4096)
     ctx.set_option("fp32_streams", 0)
