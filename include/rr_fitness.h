@@ -266,10 +266,16 @@ int rr_last_kernel_ms(rr_ctx* ctx, double* ms);
  * rr_last_kernel_ms. */
 #define RR_PHASE_PREPASS 0  /* k_nan0_fp64: factor 0 on the first 256 samples */
 #define RR_PHASE_EVAL 1     /* k_alive_fp64 + k_eval_fp64: the FP64 factor columns */
-#define RR_PHASE_QR 2       /* k_qr_*: the column-pivoted QR and the solve */
+#define RR_PHASE_QR 2       /* k_ts_* / k_qr_*: the column-pivoted QR and the solve */
 #define RR_PHASE_SCORE 3    /* k_score_fp64 + k_fit_fp64: prediction, SSR, R2, RMSE */
 #define RR_PHASE_FP32 4     /* the FP32 form's kernels (k_acc_fp32 + k_fin_fp32, or the raw pair) */
-#define RR_PHASES 5
+/* host wall-clock parts of the last rr_fitness_* call (recorded whatever
+ * "phase_timing" is): */
+#define RR_PHASE_HOST_CHECK 5  /* validating the batch (and hashing its programs) */
+#define RR_PHASE_HOST_PLAN 6   /* forms, work lists, staging and uploads, up to the first kernel */
+#define RR_PHASE_HOST_TAIL 7   /* from the first kernel to results in host memory, less the kernels'
+                                  own span (launch gaps, result copies, synchronisation) */
+#define RR_PHASES 8
 int rr_last_phase_ms(rr_ctx* ctx, int32_t phase, double* ms);
 
 #ifdef __cplusplus

@@ -256,7 +256,8 @@ class BatchRunner {
   double last_kernel_ms() { double ms = 0; rr_last_kernel_ms(ctx_, &ms); return ms; }
   // per-phase kernel time of the last run (option "phase_timing" must be 1)
   py::dict last_phase_ms() {
-    static const char* names[RR_PHASES] = {"prepass", "eval", "qr", "score", "fp32"};
+    static const char* names[RR_PHASES] = {"prepass", "eval", "qr", "score", "fp32", "host_check", "host_plan",
+                                           "host_tail"};
     py::dict d;
     for (int p = 0; p < RR_PHASES; ++p) {
       double ms = 0;

@@ -10,10 +10,13 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <condition_variable>
+#include <functional>
 #include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
+#include <chrono>
 
 #include "../../include/rr_fitness.h"
 #include "rr_kernels.h"
@@ -60,8 +63,10 @@ thread_local Arena* t_arena = nullptr;
 struct DevBuf {
   void* p = nullptr;
   size_t cap = 0;
+  bool view = false;   // p points into another buffer (staged inputs): never freed here
   int ensure(size_t bytes) {
     if (bytes <= cap) return RR_OK;
+    if (view) { p = nullptr; cap = 0; view = false; }
     if (t_arena) {
       const size_t off = (t_arena->used + 255) & ~size_t(255);
       if (off > t_arena->size || bytes > t_arena->size - off) {
@@ -84,7 +89,15 @@ struct DevBuf {
     cap = want;
     return RR_OK;
   }
-  void release() { if (p) (void)hipFree(p); p = nullptr; cap = 0; }
+  void release() {
+    if (p && !view) (void)hipFree(p);
+    p = nullptr; cap = 0; view = false;
+  }
+  // make this buffer a view of `bytes` at q (releasing what it owned)
+  void set_view(void* q, size_t bytes) {
+    if (!view) release();
+    p = q; cap = bytes; view = true;
+  }
 };
 
 struct HostBuf {
@@ -146,6 +159,8 @@ struct rr_ctx {
   DevBuf d_dead;                 // per candidate: decided by the factor-0 pre-pass (OLS mode)
   DevBuf d_blk_cand, d_blk_chunk, d_chunk_base, d_work, d_work_flags;   // FP32 chunk map and partials
   HostBuf h_stage, h_out;
+  DevBuf d_stage;                  // device mirror of h_stage: staged inputs arrive in one copy
+  size_t up_lo = SIZE_MAX, up_hi = 0;   // the staged bytes not yet copied
   int64_t slab_budget_mb = 32768;
   int64_t fp64_eval_rows = 2;
   int64_t fp64_eval_form = 1;    // the factor interpreter: 0 stack in registers, 1 stack in LDS
@@ -161,7 +176,7 @@ struct rr_ctx {
   std::vector<hipEvent_t> ev_pool;
   std::vector<PhaseMark> marks;
   size_t ev_used = 0;
-  double phase_ms[RR_PHASES] = {0, 0, 0, 0, 0};
+  double phase_ms[RR_PHASES] = {0, 0, 0, 0, 0, 0, 0, 0};
   std::vector<FacSlot> fac_table;   // kept across calls: never cleared, only re-stamped
   int32_t fac_gen = 0;
 };
@@ -199,7 +214,7 @@ void end_phase(rr_ctx* ctx, int mark, hipStream_t stream) {
 
 // after the call's streams have been synchronized
 void collect_phases(rr_ctx* ctx) {
-  for (double& v : ctx->phase_ms) v = 0.0;
+  for (int i = 0; i < RR_PHASE_HOST_CHECK; ++i) ctx->phase_ms[i] = 0.0;
   for (const rr_ctx::PhaseMark& m : ctx->marks) {
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, m.a, m.b) == hipSuccess) ctx->phase_ms[m.phase] += ms;
@@ -211,6 +226,73 @@ void collect_phases(rr_ctx* ctx) {
 Dataset* get_ds(rr_ctx* ctx, int32_t h) {
   if (h < 0 || h >= (int32_t)ctx->ds.size() || !ctx->ds[h].live) return nullptr;
   return &ctx->ds[h];
+}
+
+// Host worker threads for the batch check, kept for the life of the process
+// (a thread per call costs tens of microseconds to start).  The pool is
+// created once and never destroyed: its detached workers sleep on its
+// condition variable until the process ends, and destroying that variable
+// under them at exit would block the exit.  run(n, fn) calls
+// fn(0) here and fn(1) .. fn(n-1) on the workers, and returns when all are
+// done; a call that finds the pool busy (another context checking at the
+// same moment) returns false and the caller does the work another way.
+class WorkerPool {
+ public:
+  bool run(int n, const std::function<void(int)>& fn) {
+    std::unique_lock<std::mutex> busy(use_, std::try_to_lock);
+    if (!busy.owns_lock()) return false;
+    {
+      std::lock_guard<std::mutex> g(m_);
+      while ((int)workers_ + 1 < n) {
+        try {
+          std::thread(&WorkerPool::loop, this).detach();
+        } catch (...) {
+          break;
+        }
+        ++workers_;
+      }
+      if ((int)workers_ + 1 < n) return false;
+      job_ = &fn;
+      next_ = 1;
+      total_ = n;
+      left_ = n - 1;
+      ++gen_;
+    }
+    cv_.notify_all();
+    fn(0);
+    std::unique_lock<std::mutex> g(m_);
+    done_.wait(g, [&] { return left_ == 0; });
+    job_ = nullptr;
+    return true;
+  }
+
+ private:
+  void loop() {
+    uint64_t seen = 0;
+    std::unique_lock<std::mutex> g(m_);
+    for (;;) {
+      cv_.wait(g, [&] { return gen_ != seen; });
+      seen = gen_;
+      while (job_ && next_ < total_) {
+        const int i = next_++;
+        const std::function<void(int)>* f = job_;
+        g.unlock();
+        (*f)(i);
+        g.lock();
+        if (--left_ == 0) done_.notify_one();
+      }
+    }
+  }
+  std::mutex use_, m_;
+  std::condition_variable cv_, done_;
+  const std::function<void(int)>* job_ = nullptr;
+  int next_ = 0, total_ = 0, left_ = 0;
+  unsigned workers_ = 0;
+  uint64_t gen_ = 0;
+};
+WorkerPool& pool() {
+  static WorkerPool* p = new WorkerPool;
+  return *p;
 }
 
 // Validate one program: opcodes, operands and the stack discipline.
@@ -247,17 +329,6 @@ int check_program(const uint32_t* code, int32_t len, int32_t d, int32_t n_consts
   if (sp != 0 || !acc) return fail(RR_E_ARG, "program leaves the stack unbalanced");
   if (max_depth) *max_depth = std::max(*max_depth, depth);
   return RR_OK;
-}
-
-// The deepest operand stack of a (checked) program.
-int stack_depth(const uint32_t* code, int32_t len) {
-  int sp = 0, depth = 0;
-  for (int32_t pc = 0; pc < len; ++pc) {
-    const uint32_t op = code[pc] & 0xffu;
-    if (op == RR_OP_PUSH) depth = std::max(depth, ++sp);
-    else if (op >= RR_OP_BIN && (((op - RR_OP_BIN) & 7u) == RR_M_S || ((op - RR_OP_BIN) & 7u) == RR_M_SR)) --sp;
-  }
-  return depth;
 }
 
 // Does a (checked) program read a sample variable?  One that does not is a
@@ -333,11 +404,11 @@ const OpTables g_ops;
 // its value (bits), not its pool index, so equal programs hash equal.
 template <bool HASH>
 bool program_ok(const uint32_t* code, int32_t len, int32_t d, int32_t n_consts, const double* K, int64_t* w,
-                uint64_t* h) {
+                uint64_t* h, int32_t* depth) {
   if (len <= 0) return false;
   // branch-free: one table entry per instruction, the state in integers
   uint32_t bad = 0, acc = 0;
-  int32_t sp = 0;
+  int32_t sp = 0, maxsp = 0;
   int64_t cw = 0;
   uint64_t hh = 0x243F6A8885A308D3ULL;
   const uint32_t ud = d > 0 ? (uint32_t)d : 0u, uk = n_consts > 0 ? (uint32_t)n_consts : 0u;
@@ -346,6 +417,7 @@ bool program_ok(const uint32_t* code, int32_t len, int32_t d, int32_t n_consts, 
     const OpInfo o = g_ops.info[op];
     cw += o.cost;
     sp += o.dsp;
+    maxsp = sp > maxsp ? sp : maxsp;
     bad |= o.bad | (o.needacc & (acc ^ 1u)) | (uint32_t)((uint32_t)sp > (uint32_t)RR_MAX_STACK) |
            (o.var & (uint32_t)(arg >= ud)) | (o.cst & (uint32_t)(arg >= uk));
     acc = o.keep & (acc | o.set);
@@ -357,6 +429,7 @@ bool program_ok(const uint32_t* code, int32_t len, int32_t d, int32_t n_consts, 
     }
   }
   *w += cw;
+  *depth = maxsp > *depth ? maxsp : *depth;
   if (HASH) {   // splitmix64 finalizer: every output bit depends on every input bit
     hh ^= hh >> 30; hh *= 0xbf58476d1ce4e5b9ULL;
     hh ^= hh >> 27; hh *= 0x94d049bb133111ebULL;
@@ -374,7 +447,7 @@ bool program_ok(const uint32_t* code, int32_t len, int32_t d, int32_t n_consts, 
 // program_ok<true> (equal programs: equal hashes).  threads: host threads it
 // may use for a large batch.
 int check_batch(const rr_batch* b, int32_t mode, int32_t d, const int32_t* cand_d, std::vector<int32_t>* prog_cost,
-                std::vector<uint64_t>* prog_hash = nullptr, int threads = 1) {
+                std::vector<uint64_t>* prog_hash = nullptr, int threads = 1, int32_t* max_depth = nullptr) {
   if (!b || b->n_cands < 0) return fail(RR_E_ARG, "bad batch");
   if (b->n_cands == 0) return RR_OK;
   if (!b->cand_start || !b->prog_start || (b->n_consts > 0 && !b->consts)) return fail(RR_E_ARG, "null batch array");
@@ -395,15 +468,20 @@ int check_batch(const rr_batch* b, int32_t mode, int32_t d, const int32_t* cand_
   static const double zero = 0.0;   // constant reads of a batch without constants
   const double* K = b->n_consts > 0 ? b->consts : &zero;
   // candidates [c0, c1): the first invalid program, or -1
-  auto check_range = [&](int32_t c0, int32_t c1, int32_t* bad_prog, int32_t* bad_cand) {
+  // (depth: written once at the end -- the threads' slots share a cache line)
+  auto check_range = [&](int32_t c0, int32_t c1, int32_t* bad_prog, int32_t* bad_cand, int32_t* depth_out) {
+    int32_t depth_l = 0;
+    struct Put { int32_t* o; int32_t* v; ~Put() { *o = *v; } } put{depth_out, &depth_l};
+    int32_t* const depth = &depth_l;
     for (int32_t c = c0; c < c1; ++c) {
       const int32_t dc = cand_d ? cand_d[c] : d;
       for (int32_t p = b->cand_start[c]; p < b->cand_start[c + 1]; ++p) {
         const int32_t len = b->prog_start[p + 1] - b->prog_start[p];
         int64_t w = 0;
         const bool ok = prog_hash ? program_ok<true>(b->code + b->prog_start[p], len, dc, b->n_consts, K, &w,
-                                                     &(*prog_hash)[p])
-                                  : program_ok<false>(b->code + b->prog_start[p], len, dc, b->n_consts, K, &w, nullptr);
+                                                     &(*prog_hash)[p], depth)
+                                  : program_ok<false>(b->code + b->prog_start[p], len, dc, b->n_consts, K, &w, nullptr,
+                                                      depth);
         if (!ok) { *bad_prog = p; *bad_cand = c; return; }
         if (prog_cost) (*prog_cost)[p] = (int32_t)std::min<int64_t>(w, INT32_MAX);
       }
@@ -414,9 +492,9 @@ int check_batch(const rr_batch* b, int32_t mode, int32_t d, const int32_t* cand_
   // one reported, whatever the split
   const int64_t n_ins = b->prog_start[n_progs];
   const int nt = (int)std::max<int64_t>(1, std::min<int64_t>(threads, n_ins / 16384));
-  std::vector<int32_t> bad_prog(nt, -1), bad_cand(nt, -1);
+  std::vector<int32_t> bad_prog(nt, -1), bad_cand(nt, -1), depth(nt, 0);
   if (nt == 1) {
-    check_range(0, b->n_cands, &bad_prog[0], &bad_cand[0]);
+    check_range(0, b->n_cands, &bad_prog[0], &bad_cand[0], &depth[0]);
   } else {
     std::vector<int32_t> cut(nt + 1, b->n_cands);
     cut[0] = 0;
@@ -429,18 +507,24 @@ int check_batch(const rr_batch* b, int32_t mode, int32_t d, const int32_t* cand_
       }
       cut[t] = lo;
     }
-    std::vector<std::thread> th;
-    th.reserve(nt - 1);
-    for (int t = 1; t < nt; ++t) {
-      try {
-        th.emplace_back(check_range, cut[t], cut[t + 1], &bad_prog[t], &bad_cand[t]);
-      } catch (...) {   // no thread to be had: this range here
-        check_range(cut[t], cut[t + 1], &bad_prog[t], &bad_cand[t]);
+    const std::function<void(int)> part = [&](int t) {
+      check_range(cut[t], cut[t + 1], &bad_prog[t], &bad_cand[t], &depth[t]);
+    };
+    if (!pool().run(nt, part)) {   // the pool is busy: threads of this call's own
+      std::vector<std::thread> th;
+      th.reserve(nt - 1);
+      for (int t = 1; t < nt; ++t) {
+        try {
+          th.emplace_back(part, t);
+        } catch (...) {   // no thread to be had: this range here
+          part(t);
+        }
       }
+      part(0);
+      for (auto& x : th) x.join();
     }
-    check_range(cut[0], cut[1], &bad_prog[0], &bad_cand[0]);
-    for (auto& x : th) x.join();
   }
+  if (max_depth) *max_depth = *std::max_element(depth.begin(), depth.end());
   for (int t = 0; t < nt; ++t) {
     if (bad_prog[t] < 0) continue;
     const int32_t p = bad_prog[t], c = bad_cand[t];
@@ -455,17 +539,38 @@ int check_batch(const rr_batch* b, int32_t mode, int32_t d, const int32_t* cand_
   return RR_OK;
 }
 
-// Copies `bytes` from host `src` into device buffer `dst` through pinned staging.
+// Puts `bytes` from host `src` into device buffer `dst`.  Through pinned
+// staging: the bytes go to h_stage at stage_off now, dst becomes a view of
+// the same offset in the device mirror d_stage, and flush_uploads copies
+// everything staged since the last flush in one transfer -- it must run
+// before the first kernel that reads dst.  No-alloc calls (no staging): a
+// copy of its own, from the source itself.
 int upload(rr_ctx* ctx, DevBuf& dst, const void* src, size_t bytes, size_t stage_off) {
-  if (int rc = dst.ensure(bytes)) return rc;
-  if (bytes == 0) return RR_OK;
-  if (t_arena) {   // no pinned staging: copy from the source itself, which may go out of scope after
+  if (t_arena) {
+    if (int rc = dst.ensure(bytes)) return rc;
+    if (bytes == 0) return RR_OK;
     RR_HIP(hipMemcpyAsync(dst.p, src, bytes, hipMemcpyHostToDevice, ctx->stream));
     RR_HIP(hipStreamSynchronize(ctx->stream));
     return RR_OK;
   }
-  memcpy((char*)ctx->h_stage.p + stage_off, src, bytes);
-  RR_HIP(hipMemcpyAsync(dst.p, (char*)ctx->h_stage.p + stage_off, bytes, hipMemcpyHostToDevice, ctx->stream));
+  if (stage_off + bytes > ctx->h_stage.cap) return fail(RR_E_ARG, "internal: staging offset past the staging buffer");
+  if (int rc = ctx->d_stage.ensure(ctx->h_stage.cap)) return rc;
+  if (bytes > 0) memcpy((char*)ctx->h_stage.p + stage_off, src, bytes);
+  dst.set_view((char*)ctx->d_stage.p + stage_off, bytes);
+  if (bytes > 0) {
+    ctx->up_lo = std::min(ctx->up_lo, stage_off);
+    ctx->up_hi = std::max(ctx->up_hi, stage_off + bytes);
+  }
+  return RR_OK;
+}
+
+int flush_uploads(rr_ctx* ctx) {
+  if (ctx->up_hi > ctx->up_lo) {
+    RR_HIP(hipMemcpyAsync((char*)ctx->d_stage.p + ctx->up_lo, (char*)ctx->h_stage.p + ctx->up_lo,
+                          ctx->up_hi - ctx->up_lo, hipMemcpyHostToDevice, ctx->stream));
+  }
+  ctx->up_lo = SIZE_MAX;
+  ctx->up_hi = 0;
   return RR_OK;
 }
 
@@ -543,6 +648,7 @@ int rr_destroy(rr_ctx* ctx) {
   for (DevBuf* b : bufs) b->release();
   ctx->h_stage.release();
   ctx->h_out.release();
+  ctx->d_stage.release();
   for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
   (void)hipEventDestroy(ctx->ev0);
   (void)hipEventDestroy(ctx->ev1);
@@ -644,6 +750,9 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
   if (precision == RR_FP32_CHECKED && mode != RR_MODE_OLS) return fail(RR_E_ARG, "RR_FP32_CHECKED is an OLS-mode form");
   if (!b || b->n_cands < 0) return fail(RR_E_ARG, "bad batch");
   const int32_t nc = b->n_cands;
+  using clk = std::chrono::steady_clock;
+  const clk::time_point t_enter = clk::now();
+  auto ms_since = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
 
   // the datasets of this call: table entries and per-candidate indices
   std::vector<int32_t> uniq;          // dataset handles in table order
@@ -669,6 +778,7 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
   }
   std::vector<int32_t> pcost;   // per program: evaluation cost (filled by the check)
   std::vector<uint64_t> phash;  // per program: hash by value (filled by the check, OLS mode)
+  int32_t batch_depth = 0;      // the deepest operand stack of any program (filled by the check)
   {
     std::vector<int32_t> cand_d;
     if (handles) {
@@ -676,11 +786,12 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
       for (int32_t c = 0; c < nc; ++c) cand_d[c] = ctx->ds[uniq[cand_ds[c]]].d;
     }
     if ((rc = check_batch(b, mode, min_d, handles ? cand_d.data() : nullptr, &pcost,
-                          mode == RR_MODE_OLS ? &phash : nullptr, (int)ctx->host_threads)))
+                          mode == RR_MODE_OLS ? &phash : nullptr, (int)ctx->host_threads, &batch_depth)))
       return rc;
   }
   if (!fit_out) return fail(RR_E_ARG, "null fit_out");
   if (mode == RR_MODE_OLS && !coefs_out) return fail(RR_E_ARG, "null coefs_out");
+  const clk::time_point t_checked = clk::now();
   if (nc == 0) { ctx->last_ms = 0.0; return RR_OK; }
   const int32_t np = b->cand_start[nc];
   const int32_t ncode = b->prog_start[np];
@@ -775,8 +886,11 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
   if (ncode > 0) { if ((rc = upload(ctx, ctx->d_code, b->code, (size_t)ncode * 4, so))) return rc; }
   else if (ctx->d_code.ensure(4) != RR_OK) return RR_E_NOMEM;
   so += s_code;
-  if (b->n_consts > 0) { if ((rc = upload(ctx, ctx->d_K, b->consts, (size_t)b->n_consts * 8, so))) return rc; }
-  else if (ctx->d_K.ensure(8) != RR_OK) return RR_E_NOMEM;
+  {
+    static const double zero = 0.0;   // a batch without constants: one zero, never read
+    const double* kp = b->n_consts > 0 ? b->consts : &zero;
+    if ((rc = upload(ctx, ctx->d_K, kp, (size_t)std::max(b->n_consts, 1) * 8, so))) return rc;
+  }
   so += s_K;
   if ((rc = upload(ctx, ctx->d_dstab, table.data(), table.size() * sizeof(rr::DsDesc), so))) return rc;
   so += s_tab;
@@ -1059,10 +1173,7 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
       }
       if (items > INT32_MAX) return fail(RR_E_LIMIT, "too many factor-evaluation work items in one launch");
       ch.nitem = (int)items;
-      ch.depth = 0;   // the deepest operand stack of the chunk's factor programs
-      for (int u = ch.p0; u < ch.p0 + ch.npair; ++u)
-        ch.depth = std::max(ch.depth, stack_depth(b->code + b->prog_start[u_prog[u]],
-                                                   b->prog_start[u_prog[u] + 1] - b->prog_start[u_prog[u]]));
+      ch.depth = batch_depth;   // bounds the operand stack of the chunk's factor programs
       max_items = std::max(max_items, items);
       ch.per = per > 0 ? (int)per : 0;
     }
@@ -1163,6 +1274,7 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
   };
   auto launch64 = [&](const std::vector<Chunk64>& chunks, hipStream_t stream) -> int {
     if (chunks.empty()) return RR_OK;
+    if ((rc = flush_uploads(ctx))) return rc;   // a plan made after the first flush
     if (stream != ctx->stream) {   // after the work already queued on the context's stream
       RR_HIP(hipEventRecord(ctx->fork64, ctx->stream));
       RR_HIP(hipStreamWaitEvent(stream, ctx->fork64, 0));
@@ -1291,6 +1403,8 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
   // the timed region starts with every input uploaded
   ctx->marks.clear();
   ctx->ev_used = 0;
+  const clk::time_point t_launch = clk::now();
+  if ((rc = flush_uploads(ctx))) return rc;
   RR_HIP(hipEventRecord(ctx->ev0, ctx->stream));
   if (mode == RR_MODE_OLS && nc > 0) {
     // candidates whose factor 0 is NaN at an early row >= 1: outcome written here, skipped below
@@ -1330,16 +1444,31 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
   }
   RR_HIP(hipEventRecord(ctx->ev1, ctx->stream));
 
-  RR_HIP(hipMemcpyAsync(fit_out, dfit, (size_t)nc * sizeof(rr_fit), hipMemcpyDeviceToHost, ctx->stream));
-  if (mode == RR_MODE_OLS)
-    RR_HIP(hipMemcpyAsync(coefs_out, dcoef, (size_t)ncoef * 8, hipMemcpyDeviceToHost, ctx->stream));
-  RR_HIP(hipStreamSynchronize(ctx->stream));
+  // results: through the pinned buffer h_out (no-alloc calls: straight into
+  // the caller's arrays, no host allocation)
+  const size_t fb = (size_t)nc * sizeof(rr_fit), cb = mode == RR_MODE_OLS ? (size_t)ncoef * 8 : 0;
+  if (!t_arena && ctx->h_out.ensure(align_up(fb) + cb) == RR_OK) {
+    char* ho = (char*)ctx->h_out.p;
+    RR_HIP(hipMemcpyAsync(ho, dfit, fb, hipMemcpyDeviceToHost, ctx->stream));
+    if (cb) RR_HIP(hipMemcpyAsync(ho + align_up(fb), dcoef, cb, hipMemcpyDeviceToHost, ctx->stream));
+    RR_HIP(hipStreamSynchronize(ctx->stream));
+    memcpy(fit_out, ho, fb);
+    if (cb) memcpy(coefs_out, ho + align_up(fb), cb);
+  } else {
+    RR_HIP(hipMemcpyAsync(fit_out, dfit, fb, hipMemcpyDeviceToHost, ctx->stream));
+    if (cb) RR_HIP(hipMemcpyAsync(coefs_out, dcoef, cb, hipMemcpyDeviceToHost, ctx->stream));
+    RR_HIP(hipStreamSynchronize(ctx->stream));
+  }
   for (int32_t c : rescored) fit_out[c].flags |= RR_F_FP64;
   float ms = 0.f;
   RR_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
   ctx->last_ms = ms;
+  const clk::time_point t_done = clk::now();
   if (!ctx->marks.empty()) RR_HIP(hipStreamSynchronize(ctx->side64));
   collect_phases(ctx);
+  ctx->phase_ms[RR_PHASE_HOST_CHECK] = ms_since(t_enter, t_checked);
+  ctx->phase_ms[RR_PHASE_HOST_PLAN] = ms_since(t_checked, t_launch);
+  ctx->phase_ms[RR_PHASE_HOST_TAIL] = std::max(0.0, ms_since(t_launch, t_done) - (double)ms);
   return RR_OK;
 }
 
@@ -1412,9 +1541,12 @@ int rr_predict(rr_ctx* ctx, int32_t handle, const uint32_t* code, int32_t len, c
   const size_t s_code = align_up((size_t)len * 4);
   if (ctx->h_stage.ensure(s_code + align_up((size_t)std::max(n_consts, 1) * 8)) != RR_OK) return RR_E_NOMEM;
   if ((rc = upload(ctx, ctx->d_code, code, (size_t)len * 4, 0))) return rc;
-  if (n_consts > 0) { if ((rc = upload(ctx, ctx->d_K, consts, (size_t)n_consts * 8, s_code))) return rc; }
-  else if (ctx->d_K.ensure(8) != RR_OK) return RR_E_NOMEM;
+  {
+    static const double zero = 0.0;   // a program without constants: one zero, never read
+    if ((rc = upload(ctx, ctx->d_K, n_consts > 0 ? consts : &zero, (size_t)std::max(n_consts, 1) * 8, s_code))) return rc;
+  }
   if (ctx->d_misc.ensure((size_t)D->n * 8) != RR_OK) return RR_E_NOMEM;
+  if ((rc = flush_uploads(ctx))) return rc;
   RR_HIP(hipEventRecord(ctx->ev0, ctx->stream));
   RR_HIP(rr::launch_predict_fp64((const uint32_t*)ctx->d_code.p, len, (const double*)ctx->d_K.p, D->X, D->n, D->ldx,
                                  (double*)ctx->d_misc.p, ctx->stream));
@@ -1438,9 +1570,12 @@ int rr_predict_fp32(rr_ctx* ctx, int32_t handle, const uint32_t* code, int32_t l
   const size_t s_code = align_up((size_t)len * 4);
   if (ctx->h_stage.ensure(s_code + align_up((size_t)std::max(n_consts, 1) * 8)) != RR_OK) return RR_E_NOMEM;
   if ((rc = upload(ctx, ctx->d_code, code, (size_t)len * 4, 0))) return rc;
-  if (n_consts > 0) { if ((rc = upload(ctx, ctx->d_K, consts, (size_t)n_consts * 8, s_code))) return rc; }
-  else if (ctx->d_K.ensure(8) != RR_OK) return RR_E_NOMEM;
+  {
+    static const double zero = 0.0;   // a program without constants: one zero, never read
+    if ((rc = upload(ctx, ctx->d_K, n_consts > 0 ? consts : &zero, (size_t)std::max(n_consts, 1) * 8, s_code))) return rc;
+  }
   if (ctx->d_misc.ensure((size_t)D->n * 4) != RR_OK || ctx->d_pred_e.ensure((size_t)D->n * 4) != RR_OK) return RR_E_NOMEM;
+  if ((rc = flush_uploads(ctx))) return rc;
   RR_HIP(hipEventRecord(ctx->ev0, ctx->stream));
   RR_HIP(rr::launch_predict_fp32((const uint32_t*)ctx->d_code.p, len, (const double*)ctx->d_K.p, D->X32, D->n, D->ldx,
                                  (float*)ctx->d_misc.p, (float*)ctx->d_pred_e.p, ctx->stream));

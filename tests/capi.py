@@ -274,7 +274,7 @@ class Context:
     def phase_ms(self):
         """rr_last_phase_ms for every phase, by name."""
         out = {}
-        for name in ("PREPASS", "EVAL", "QR", "SCORE", "FP32"):
+        for name in ("PREPASS", "EVAL", "QR", "SCORE", "FP32", "HOST_CHECK", "HOST_PLAN", "HOST_TAIL"):
             v = ctypes.c_double()
             check(lib().rr_last_phase_ms(self.p, D["RR_PHASE_" + name], ctypes.byref(v)))
             out[name.lower()] = v.value

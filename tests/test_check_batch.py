@@ -2,6 +2,8 @@
 against an independent statement of the bytecode rules of include/rr_fitness.h,
 on valid programs compiled from random trees and on random corruptions of
 them.  CPU only: the check needs no device."""
+import os
+
 import numpy as np
 import pytest
 
@@ -145,3 +147,18 @@ def test_large_batch_reports_first_invalid_program():
     code[arrs["prog_start"][early]] = arrs["code"][arrs["prog_start"][early]]
     rc, msg = capi.check_batch(dict(arrs, code=code), "ols", 4)
     assert rc == D["RR_E_ARG"] and msg.endswith("(program %d)" % late), msg
+
+
+def test_process_exits_after_the_host_pool_was_used():
+    """A large batch check runs on the library's host worker threads, which
+    live until the process ends: a process that used them exits promptly."""
+    import subprocess
+    import sys
+    code = (
+        "import sys; sys.path.insert(0, %r)\n"
+        "import capi, treegen\n"
+        "arrs = capi.build_batch([[t, 'x0'] for t in treegen.random_trees(9, 4000, 10, 60, 4)])\n"
+        "assert capi.check_batch(arrs, 'ols', 4) == (0, '')\n"
+        "print('checked', flush=True)\n" % os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and "checked" in r.stdout, r.stderr[-2000:]
