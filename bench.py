@@ -406,6 +406,7 @@ def search_main(args):
     # the JSON line stays the last line
     rr = rils_rols_cpp.rils_rols(False, args.search_fit_calls, 10 ** 7, 0.001, 30, 1.0, True, 12345)
     rr.set_precision(args.search_precision)
+    rr.set_context_option("graph", args.graph)
     t = time.perf_counter()
     rr.fit(np.ascontiguousarray(X).reshape(-1), y, n, d)
     wall = time.perf_counter() - t
@@ -446,6 +447,7 @@ def small_main(args):
     X, y = d[:, :2], d[:, 2]
     n, f = X.shape
     rr = rils_rols_cpp.rils_rols(False, 100000, 10 ** 7, 0.001, 50, 1.0, False, 12345)
+    rr.set_context_option("graph", args.graph)
     t = time.perf_counter()
     rr.fit(np.ascontiguousarray(X).reshape(-1), y, n, f)
     gpu_s = time.perf_counter() - t
@@ -453,7 +455,8 @@ def small_main(args):
         "metric": "plain method, small: full search wall time (150 x 2, 100000 fit calls)",
         "value": rr.get_fit_calls() / gpu_s, "unit": "fitness evals/s", "n_gpus": 1, "higher_is_better": True,
         "dtype": "f64", "data": "reference test_example recipe (committed fixture)",
-        "config": {"workload": "plain method, small: 150 x 2, max_fit_calls=100000, random_state=12345"},
+        "config": {"workload": "plain method, small: 150 x 2, max_fit_calls=100000, random_state=12345",
+                   "graph": args.graph},
         "gpu_wall_s": gpu_s, "fit_calls": rr.get_fit_calls(), "model": rr.get_model_string(), "stats": rr.get_stats(),
     }
     if not args.no_cpu:
@@ -480,6 +483,8 @@ def main():
     ap.add_argument("--fp64-eval-rows", type=int, default=None, help="samples per thread of the FP64 evaluation (1, 2, 4)")
     ap.add_argument("--fp64-eval-form", type=int, default=None,
                     help="FP64 factor interpreter: 0 stack in registers, 1 stack in LDS with inlined math")
+    ap.add_argument("--graph", type=int, default=0, choices=[0, 1],
+                    help="search workloads: replay each plain FP64 fitness call as an executable graph")
     ap.add_argument("--fp32-rows", type=int, default=None, help="samples per lane per tile, FP32 kernels (1, 2)")
     ap.add_argument("--fp32-chunk", type=int, default=None, help="samples per FP32 workgroup (multiple of 512)")
     ap.add_argument("--fp32-streams", type=int, default=None, help="1: FP32 buckets on concurrent streams")

@@ -26,6 +26,7 @@ py::dict stats_dict(const rrh::Stats& s) {
   d["sweeps"] = s.sweeps; d["batched_cands"] = s.batched_cands; d["raw_batches"] = s.raw_batches;
   d["raw_cands"] = s.raw_cands; d["single_calls"] = s.single_calls; d["rescored_fp64"] = s.rescored_fp64;
   d["host_gen_s"] = s.host_gen_s; d["host_prep_s"] = s.host_prep_s; d["gpu_s"] = s.gpu_s;
+  d["host_main_s"] = s.host_main_s; d["setup_s"] = s.setup_s;
   d["kernel_ms"] = s.kernel_ms; d["replay_s"] = s.replay_s;
   return d;
 }
@@ -355,6 +356,7 @@ PYBIND11_MODULE(rils_rols_cpp, m) {
            })
       .def("set_device", &RilsRols::set_device)
       .def("set_slab_budget_mb", &RilsRols::set_slab_budget_mb)
+      .def("set_context_option", &RilsRols::set_context_option, py::arg("name"), py::arg("value"))
       .def("set_autotune", &RilsRols::set_autotune)
       .def("set_host_threads", &RilsRols::set_host_threads,
            "host threads for the per-candidate rewrites of a sweep (0: min(hardware, 8))");

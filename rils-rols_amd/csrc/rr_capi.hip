@@ -160,6 +160,18 @@ struct rr_ctx {
   DevBuf d_blk_cand, d_blk_chunk, d_chunk_base, d_work, d_work_flags;   // FP32 chunk map and partials
   HostBuf h_stage, h_out;
   DevBuf d_stage;                  // device mirror of h_stage: staged inputs arrive in one copy
+  // option "graph": the plain FP64 form's launches of a call are captured
+  // into a graph and replayed through an executable graph kept per launch
+  // sequence (the kernels in order), updated with the call's arguments
+  int64_t use_graph = 0;
+  struct GraphSlot {
+    std::vector<const void*> key;
+    hipGraphExec_t exec = nullptr;
+    uint64_t used = 0;
+  };
+  std::vector<GraphSlot> graphs;
+  uint64_t graph_tick = 0;
+  int64_t graph_made = 0, graph_updated = 0;   // executable graphs instantiated / reused
   size_t up_lo = SIZE_MAX, up_hi = 0;   // the staged bytes not yet copied
   int64_t slab_budget_mb = 32768;
   int64_t fp64_eval_rows = 2;
@@ -649,6 +661,8 @@ int rr_destroy(rr_ctx* ctx) {
   ctx->h_stage.release();
   ctx->h_out.release();
   ctx->d_stage.release();
+  for (rr_ctx::GraphSlot& g : ctx->graphs) if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  ctx->graphs.clear();
   for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
   (void)hipEventDestroy(ctx->ev0);
   (void)hipEventDestroy(ctx->ev1);
@@ -740,6 +754,80 @@ namespace {
 
 // One batched fitness call.  handles == nullptr: every candidate against
 // dataset `handle`; otherwise candidate c against dataset handles[c].
+// Capture what `issue` launches on the context's stream into a graph, find
+// or make the executable graph for that launch sequence, and launch it
+// between ev0 and ev1.  At most GRAPH_SLOTS executable graphs are kept per
+// context (least recently used goes).
+constexpr size_t GRAPH_SLOTS = 16;
+template <class F>
+int run_graph(rr_ctx* ctx, F issue) {
+  RR_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  const int rc_issue = issue();
+  hipGraph_t g = nullptr;
+  const hipError_t e_end = hipStreamEndCapture(ctx->stream, &g);
+  if (rc_issue != RR_OK || e_end != hipSuccess || !g) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc_issue != RR_OK ? rc_issue : fail(RR_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e_end));
+  }
+  // the launch sequence: the kernel of every node, in capture order
+  size_t nn = 0;
+  int rc = RR_OK;
+  std::vector<const void*> key;
+  if (hipGraphGetNodes(g, nullptr, &nn) == hipSuccess) {
+    std::vector<hipGraphNode_t> nodes(nn);
+    if (nn == 0 || hipGraphGetNodes(g, nodes.data(), &nn) == hipSuccess) {
+      key.reserve(nn);
+      for (hipGraphNode_t nd : nodes) {
+        hipGraphNodeType ty;
+        hipKernelNodeParams kp{};
+        if (hipGraphNodeGetType(nd, &ty) == hipSuccess && ty == hipGraphNodeTypeKernel &&
+            hipGraphKernelNodeGetParams(nd, &kp) == hipSuccess)
+          key.push_back(kp.func);
+        else
+          key.push_back((const void*)(uintptr_t)(0x1000 + (int)ty));
+      }
+    }
+  }
+  rr_ctx::GraphSlot* slot = nullptr;
+  for (rr_ctx::GraphSlot& s : ctx->graphs)
+    if (s.exec && s.key == key) { slot = &s; break; }
+  bool ready = false;
+  if (slot) {
+    hipGraphNode_t err_node = nullptr;
+    hipGraphExecUpdateResult res;
+    ready = hipGraphExecUpdate(slot->exec, g, &err_node, &res) == hipSuccess && res == hipGraphExecUpdateSuccess;
+    if (ready) ++ctx->graph_updated;
+    else { (void)hipGraphExecDestroy(slot->exec); slot->exec = nullptr; }
+  } else {
+    if (ctx->graphs.size() < GRAPH_SLOTS) {
+      ctx->graphs.emplace_back();
+      slot = &ctx->graphs.back();
+    } else {
+      slot = &ctx->graphs[0];
+      for (rr_ctx::GraphSlot& s : ctx->graphs) if (s.used < slot->used) slot = &s;
+      if (slot->exec) (void)hipGraphExecDestroy(slot->exec);
+      slot->exec = nullptr;
+    }
+  }
+  if (!ready) {
+    const hipError_t e = hipGraphInstantiate(&slot->exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      slot->exec = nullptr;
+      rc = fail(RR_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    } else {
+      slot->key = key;
+      ++ctx->graph_made;
+    }
+  }
+  (void)hipGraphDestroy(g);
+  if (rc) return rc;
+  slot->used = ++ctx->graph_tick;
+  RR_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+  RR_HIP(hipGraphLaunch(slot->exec, ctx->stream));
+  RR_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+  return RR_OK;
+}
+
 int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_batch* b, int32_t mode,
                  int32_t precision, double* coefs_out, rr_fit* fit_out) {
   int rc = set_device(ctx);
@@ -1405,25 +1493,42 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
   ctx->ev_used = 0;
   const clk::time_point t_launch = clk::now();
   if ((rc = flush_uploads(ctx))) return rc;
-  RR_HIP(hipEventRecord(ctx->ev0, ctx->stream));
-  if (mode == RR_MODE_OLS && nc > 0) {
-    // candidates whose factor 0 is NaN at an early row >= 1: outcome written here, skipped below
-    rr::Nan0Launch L0{dc, dp, dcode, dK, dtab, dcds, dcoef, dfit, (int32_t*)ctx->d_dead.p, nc};
-    const int pm = begin_phase(ctx, RR_PHASE_PREPASS, ctx->stream);
-    RR_HIP(rr::launch_nan0_fp64(L0, ctx->stream));
-    end_phase(ctx, pm, ctx->stream);
-  }
-  if (mode == RR_MODE_RAW && precision == RR_FP64) {
-    const int pm = begin_phase(ctx, RR_PHASE_SCORE, ctx->stream);
-    RR_HIP(rr::launch_score_fp64(raw64, ctx->stream));
-    end_phase(ctx, pm, ctx->stream);
+  auto prepass = [&]() -> int {
+    if (mode == RR_MODE_OLS && nc > 0) {
+      // candidates whose factor 0 is NaN at an early row >= 1: outcome written here, skipped below
+      rr::Nan0Launch L0{dc, dp, dcode, dK, dtab, dcds, dcoef, dfit, (int32_t*)ctx->d_dead.p, nc};
+      const int pm = begin_phase(ctx, RR_PHASE_PREPASS, ctx->stream);
+      RR_HIP(rr::launch_nan0_fp64(L0, ctx->stream));
+      end_phase(ctx, pm, ctx->stream);
+    }
+    return RR_OK;
+  };
+  // the plain FP64 form: every launch on the context's stream
+  auto plain64 = [&]() -> int {
+    if ((rc = prepass())) return rc;
+    if (mode == RR_MODE_RAW) {
+      const int pm = begin_phase(ctx, RR_PHASE_SCORE, ctx->stream);
+      RR_HIP(rr::launch_score_fp64(raw64, ctx->stream));
+      end_phase(ctx, pm, ctx->stream);
+      return RR_OK;
+    }
+    return launch64(plan, ctx->stream);
+  };
+  const bool graph = precision == RR_FP64 && ctx->use_graph && !ctx->phase_timing && !t_arena;
+  if (graph) {   // records ev0 and ev1 around the graph's launch
+    if ((rc = run_graph(ctx, plain64))) return rc;
   } else if (precision == RR_FP64) {
-    if ((rc = launch64(plan, ctx->stream))) return rc;
+    RR_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    if ((rc = plain64())) return rc;
   } else if (!checked) {
+    RR_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    if ((rc = prepass())) return rc;
     if ((rc = fp32_run())) return rc;
   } else {
     // FP64 for the predicted candidates beside the FP32 form, then FP64 for
     // the ones the FP32 form could not certify
+    RR_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    if ((rc = prepass())) return rc;
     if ((rc = launch64(plan, ctx->side64))) return rc;
     if ((rc = fp32_run())) return rc;
     if (!list64.empty()) {
@@ -1442,7 +1547,7 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
     rescored = list64;
     rescored.insert(rescored.end(), again.begin(), again.end());
   }
-  RR_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+  if (!graph) RR_HIP(hipEventRecord(ctx->ev1, ctx->stream));
 
   // results: through the pinned buffer h_out (no-alloc calls: straight into
   // the caller's arrays, no host allocation)
@@ -1638,6 +1743,9 @@ int rr_set_option(rr_ctx* ctx, const char* name, int64_t value) {
   else if (s == "fp64_eval_rows") {
     if (value != 1 && value != 2 && value != 4) return fail(RR_E_ARG, "fp64_eval_rows must be 1, 2 or 4");
     ctx->fp64_eval_rows = value;
+  } else if (s == "graph") {
+    if (value != 0 && value != 1) return fail(RR_E_ARG, "graph must be 0 or 1");
+    ctx->use_graph = value;
   } else if (s == "fp64_eval_form") {
     if (value != 0 && value != 1) return fail(RR_E_ARG, "fp64_eval_form must be 0 or 1");
     ctx->fp64_eval_form = value;
@@ -1666,6 +1774,9 @@ int rr_get_option(rr_ctx* ctx, const char* name, int64_t* value) {
   if (s == "slab_budget_mb") *value = ctx->slab_budget_mb;
   else if (s == "fp64_eval_rows") *value = ctx->fp64_eval_rows;
   else if (s == "fp64_eval_form") *value = ctx->fp64_eval_form;
+  else if (s == "graph") *value = ctx->use_graph;
+  else if (s == "graph_made") *value = ctx->graph_made;
+  else if (s == "graph_updated") *value = ctx->graph_updated;
   else if (s == "fp32_rows") *value = ctx->fp32_rows;
   else if (s == "fp32_chunk_rows") *value = ctx->fp32_chunk_rows;
   else if (s == "fp32_streams") *value = ctx->fp32_streams;

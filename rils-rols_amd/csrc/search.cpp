@@ -36,9 +36,10 @@ static void rr_check(int rc, const char* what) {
 class DirectBackend : public FitnessBackend {
  public:
   rr_ctx* ctx = nullptr;
-  DirectBackend(int device, int64_t slab_mb) {
+  DirectBackend(int device, int64_t slab_mb, const std::vector<std::pair<std::string, int64_t>>& options) {
     rr_check(rr_create(device, &ctx), "rr_create (the MI355X fitness library needs a HIP device)");
     if (slab_mb > 0) rr_check(rr_set_option(ctx, "slab_budget_mb", slab_mb), "rr_set_option");
+    for (const auto& o : options) rr_check(rr_set_option(ctx, o.first.c_str(), o.second), "rr_set_option");
   }
   ~DirectBackend() override { if (ctx) rr_destroy(ctx); }
   int32_t upload(const double* X_fm, const double* y, int64_t n, int32_t d) override {
@@ -67,11 +68,12 @@ class GpuSession {
   int64_t n = 0;
   int d = 0;
 
-  GpuSession(FitnessBackend* backend, int device, int64_t slab_mb) {
+  GpuSession(FitnessBackend* backend, int device, int64_t slab_mb,
+             const std::vector<std::pair<std::string, int64_t>>& options) {
     if (backend) {
       be = backend;
     } else {
-      be = new DirectBackend(device, slab_mb);
+      be = new DirectBackend(device, slab_mb, options);
       own = true;
     }
   }
@@ -679,6 +681,7 @@ void RilsRols::fit_inner(const std::vector<std::vector<double>>& X_all, const st
   stats_ = Stats();
   sweep_ms_.clear();
   tuned_ = false;
+  const auto t_setup = std::chrono::steady_clock::now();
   const int64_t N = (int64_t)y_all.size();
   const int sample_cnt = int(sample_size_ * (double)N);
   std::vector<int> selected(N);
@@ -694,8 +697,9 @@ void RilsRols::fit_inner(const std::vector<std::vector<double>>& X_all, const st
   if (sample_cnt <= 0) throw std::invalid_argument("sample_size * n must be at least one sample");
   std::vector<int> rel = relevant_features(X, y);
   setup_nodes(rel);
-  if (!gpu_) gpu_ = new GpuSession(backend_, device_, slab_budget_mb_);
+  if (!gpu_) gpu_ = new GpuSession(backend_, device_, slab_budget_mb_, ctx_options_);
   gpu_->upload(X, y);
+  stats_.setup_s = secs_since(t_setup);
 
   final_solution_ = {tok_const(0.0)};
   final_fitness_ = fitness_raw(final_solution_);
@@ -703,6 +707,7 @@ void RilsRols::fit_inner(const std::vector<std::vector<double>>& X_all, const st
   std::vector<rr_fit> res;
   while (!finished()) {
     main_it_ += 1;
+    auto tm = std::chrono::steady_clock::now();
     Flat start = final_solution_;
     if (!improved) {
       std::vector<Flat> all_perts = all_candidates(final_solution_, false);
@@ -728,14 +733,17 @@ void RilsRols::fit_inner(const std::vector<std::vector<double>>& X_all, const st
         ++virt;
       }
     }
+    stats_.host_main_s += secs_since(tm);
     gpu_->raw(sel, res, stats_);
     fit_calls_ += (int)sel.size();
+    tm = std::chrono::steady_clock::now();
     std::vector<std::pair<double, const Flat*>> by;
     by.reserve(sel.size());
     for (size_t i = 0; i < sel.size(); ++i) by.emplace_back(to_fit(res[i], (int)sel[i]->size()).e0, sel[i]);
     std::sort(by.begin(), by.end(), [](const std::pair<double, const Flat*>& a, const std::pair<double, const Flat*>& b) {
       return a.first < b.first;
     });
+    stats_.host_main_s += secs_since(tm);
     for (size_t i = 0; i < by.size(); ++i) {   // 833-855
       if (finished()) break;
       const Flat& lp = *by[i].second;
@@ -788,7 +796,7 @@ void RilsRols::local_search_from(const double* Xp, int64_t x_size, const double*
   std::vector<int> rel(feat_cnt);
   std::iota(rel.begin(), rel.end(), 0);
   setup_nodes(rel);
-  if (!gpu_) gpu_ = new GpuSession(backend_, device_, slab_budget_mb_);
+  if (!gpu_) gpu_ = new GpuSession(backend_, device_, slab_budget_mb_, ctx_options_);
   gpu_->upload(X, y);
   Fit f;
   final_solution_ = local_search(start, &f);
@@ -801,7 +809,7 @@ std::vector<double> RilsRols::predict(const double* Xp, int64_t x_size, int data
                                 " is not the same as the product of data count and feature count " +
                                 std::to_string((int64_t)data_cnt * feat_cnt));
   if (final_solution_.empty()) throw std::runtime_error("predict called before fit");
-  if (!gpu_) gpu_ = new GpuSession(backend_, device_, slab_budget_mb_);
+  if (!gpu_) gpu_ = new GpuSession(backend_, device_, slab_budget_mb_, ctx_options_);
   rr_ctx* ctx = gpu_->be->context();
   if (!ctx) throw std::runtime_error("predict needs a fitness backend with a device context");
   std::vector<double> fm((size_t)data_cnt * std::max(feat_cnt, 1));

@@ -39,6 +39,8 @@ struct Stats {
   int64_t sweeps = 0, batched_cands = 0, raw_batches = 0, raw_cands = 0, single_calls = 0;
   int64_t rescored_fp64 = 0;
   double host_gen_s = 0, host_prep_s = 0, gpu_s = 0, kernel_ms = 0, replay_s = 0;
+  double host_main_s = 0;   // the main loop's perturbation lists, skip checks and sort
+  double setup_s = 0;       // sampling, feature filter, device context and dataset upload
 };
 
 // glibc's rand() sequence with per-object state (random_r on a TYPE_3 state,
@@ -105,6 +107,9 @@ class RilsRols {
   int precision() const { return precision_; }
   void set_device(int d) { device_ = d; }
   void set_slab_budget_mb(int64_t mb) { slab_budget_mb_ = mb; }
+  // an rr_set_option for the context the search creates (its own backend
+  // only; applied when the context is made, in call order)
+  void set_context_option(const std::string& name, int64_t value) { ctx_options_.emplace_back(name, value); }
   void set_host_threads(int t);
   // route fitness calls through `be` (not owned; must outlive the fit)
   void set_backend(FitnessBackend* be) { backend_ = be; }
@@ -140,6 +145,7 @@ class RilsRols {
   int precision_ = RR_FP64;
   int device_ = 0;
   int64_t slab_budget_mb_ = 0;
+  std::vector<std::pair<std::string, int64_t>> ctx_options_;
   GpuSession* gpu_ = nullptr;
   FitnessBackend* backend_ = nullptr;
   bool autotune_ = false, tuned_ = false;

@@ -1,6 +1,8 @@
 // tree.cpp -- host-side expression engine (see tree.h).
 #include "tree.h"
 
+#include <charconv>
+
 #include <climits>
 #include <cmath>
 #include <cstdio>
@@ -36,19 +38,18 @@ void subtree_sizes(const Flat& f, std::vector<int32_t>& size) {
 // std::to_string(int) / std::to_string(double) as used by node::to_string;
 // (int)round(c) outside the int range is what x86 cvttsd2si returns (INT_MIN).
 void append_const(double c, std::string& out) {
-  char buf[512];
+  // std::to_chars in fixed notation with precision 6 writes what
+  // printf("%f") writes (the standard defines it so), without printf's cost
+  char buf[400];   // "-" + 309 integer digits + "." + 6 decimals fits
   const double rc = std::round(c);
   if (std::fabs(rc - c) < EPS) {
     const int iv = (rc >= -2147483648.0 && rc < 2147483648.0) ? (int)rc : INT_MIN;
-    const int k = snprintf(buf, sizeof buf, "%d", iv);
-    out.append(buf, k);
+    const std::to_chars_result r = std::to_chars(buf, buf + sizeof buf, iv);
+    out.append(buf, r.ptr);
     return;
   }
-  const int k = snprintf(buf, sizeof buf, "%f", c);
-  if (k < (int)sizeof buf) { out.append(buf, k); return; }
-  std::vector<char> big(k + 1);
-  snprintf(big.data(), big.size(), "%f", c);
-  out.append(big.data(), k);
+  const std::to_chars_result r = std::to_chars(buf, buf + sizeof buf, c, std::chars_format::fixed, 6);
+  out.append(buf, r.ptr);
 }
 
 // node.h:255-301

@@ -380,3 +380,44 @@ def test_phase_timing_and_early_flag(ctx):
     assert early[ln_x0]     # ln of a negative x0 within the first rows
     for t, tu, c, f in zip(trees, tuned, coefs, fit):   # the flag does not change any result
         assert same_bits(c, tu["coefs"]) and same_bits([f["e0"], f["e1"]], [tu["fit"][0], tu["fit"][1]]), t
+
+
+def test_graph_replays_same_bits(ctx):
+    """Option graph: the plain FP64 form's launches go out as an executable
+    graph, kept per launch sequence and updated with each call's arguments.
+    Calls of different shapes (OLS batches of different sizes, a RAW batch)
+    with eager calls between the replays: every replay gives the eager bits,
+    and repeated shapes reuse their executable graph."""
+    print("case: graph replays", flush=True)
+    X, y = make_data(91, 20000, 4)
+    od = O.Dataset(X, y)
+    h = ctx.upload(X, y)
+    tuned_a = [od.tune(t) for t in treegen.random_trees(91, 300, 3, 40, 4)]
+    tuned_b = [od.tune(t) for t in treegen.random_trees(92, 80, 3, 20, 4)]
+    arr_a = capi.build_batch([tu["factors"] for tu in tuned_a])
+    arr_b = capi.build_batch([tu["factors"] for tu in tuned_b])
+    raw = capi.build_batch([[t] for t in treegen.random_trees(93, 200, 3, 30, 4)])
+    eager = {"a": ctx.fitness(h, arr_a), "b": ctx.fitness(h, arr_b), "raw": ctx.fitness(h, raw, mode="raw")}
+    arrays = {"a": arr_a, "b": arr_b, "raw": raw}
+    ctx.set_option("graph", 1)
+    try:
+        made0, upd0 = ctx.get_option("graph_made"), ctx.get_option("graph_updated")
+        for step, name in enumerate(["a", "b", "a", "raw", "a", "b", "raw", "a"]):
+            mode = "raw" if name == "raw" else "ols"
+            c, f = ctx.fitness(h, arrays[name], mode=mode)
+            ce, fe = eager[name]
+            assert same_bits(f["e0"], fe["e0"]) and same_bits(f["e1"], fe["e1"]), (step, name)
+            if mode == "ols":
+                assert all(same_bits(p, q) for p, q in zip(c, ce)), (step, name)
+            ctx.set_option("graph", 0)                 # eager work between the replays
+            c2, f2 = ctx.fitness(h, arrays[name], mode=mode)
+            assert same_bits(f2["e0"], fe["e0"]), (step, name)
+            ctx.set_option("graph", 1)
+        made, upd = ctx.get_option("graph_made") - made0, ctx.get_option("graph_updated") - upd0
+        print("graphs made", made, "updated", upd, flush=True)
+        assert made >= 1 and upd >= 1 and made + upd == 8
+    finally:
+        ctx.set_option("graph", 0)
+    for tu, c, f in zip(tuned_a, *eager["a"]):            # and the eager bits are the oracle's
+        assert same_bits(c, tu["coefs"]) and same_bits([f["e0"], f["e1"]], [tu["fit"][0], tu["fit"][1]])
+    ctx.free(h)

@@ -26,12 +26,14 @@ def load_config1(which="train"):
 
 
 def run_product(X, y, classification=False, max_fit_calls=100000, complexity_penalty=0.001, max_complexity=50,
-                sample_size=1.0, random_state=0, precision="fp64"):
+                sample_size=1.0, random_state=0, precision="fp64", options=()):
     import rils_rols_cpp
     n, d = X.shape
     rr = rils_rols_cpp.rils_rols(classification, max_fit_calls, BIG_TIME, complexity_penalty, max_complexity,
                                  sample_size, False, random_state)
     rr.set_precision(precision)
+    for name, value in options:
+        rr.set_context_option(name, value)
     rr.fit(np.ascontiguousarray(X, dtype=np.float64).reshape(-1), np.ascontiguousarray(y, dtype=np.float64), n, d)
     return rr
 
@@ -45,12 +47,14 @@ def assert_same_run(rr, want):
     assert rr.get_model_string() == want["model"]
 
 
-@pytest.mark.parametrize("max_fit_calls", [100000])
-def test_config1_move_sequence_identical(max_fit_calls):
-    print("case: config1 max_fit_calls=%d seed=12345" % max_fit_calls, flush=True)
+@pytest.mark.parametrize("max_fit_calls,graph", [(100000, 0), (100000, 1)])
+def test_config1_move_sequence_identical(max_fit_calls, graph):
+    """The reference's own small case, the whole search; with graph=1 every
+    plain FP64 fitness call goes out as a replayed executable graph."""
+    print("case: config1 max_fit_calls=%d seed=12345 graph=%d" % (max_fit_calls, graph), flush=True)
     X, y = load_config1()
     want = O.fit(X, y, max_fit_calls=max_fit_calls, max_time=BIG_TIME, random_state=12345)
-    rr = run_product(X, y, max_fit_calls=max_fit_calls, random_state=12345)
+    rr = run_product(X, y, max_fit_calls=max_fit_calls, random_state=12345, options=[("graph", graph)])
     assert_same_run(rr, want)
     print("config1 model:", rr.get_model_string(), "fit_calls:", rr.get_fit_calls(), "stats:", rr.get_stats(),
           flush=True)

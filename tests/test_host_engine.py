@@ -102,3 +102,22 @@ def test_crand_is_glibc_rand():
         libc.srand(seed)
         want = [libc.rand() for _ in range(200)]
         assert rrc._crand(seed, 200) == want, seed
+
+
+def test_constant_strings_are_printf_f():
+    """A constant's text in the reference's strings (node.h:255-301:
+    std::to_string, i.e. printf "%f", for a non-integer; the integer as
+    "%d" when it is within 1e-12 of one): the product's formatter against
+    Python's "%f" (correctly rounded, as C's) over random magnitudes, ties of
+    the sixth decimal, huge, tiny and signed-zero values."""
+    rng = np.random.default_rng(7)
+    vals = list(rng.normal(0, 1, 3000)) + list(rng.normal(0, 1, 3000) * 10.0 ** rng.integers(-12, 40, 3000))
+    vals += [(k + 0.5) * 1e-6 for k in range(-50, 50)] + [0.0000015, 2.5e-7, 1e300, -1e300, 5e-324,
+                                                          123456.0000005, 0.1, -0.1, 1.0 / 3.0]
+    for v in vals:
+        v = float(v)
+        r = round(v)
+        want = ("%d" % (int(r) if -2147483648.0 <= r < 2147483648.0 else -2147483648)
+                if abs(r - v) < 1e-12 else "%f" % v)
+        got = rrc._rewrite("c" + v.hex(), "to_string")
+        assert got == want, (v.hex(), got, want)
