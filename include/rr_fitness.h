@@ -237,7 +237,12 @@ int rr_predict_fp32(rr_ctx* ctx, int32_t handle, const uint32_t* code, int32_t l
  * "fp32_streams" (1: the FP32 factor-count buckets run on concurrent streams;
  * default 0, back to back on the context's stream), "host_threads" (host
  * threads a call may use to check a large batch, 1..64; default 4),
- * "phase_timing" (1: record rr_last_phase_ms; default 0).
+ * "phase_timing" (1: record rr_last_phase_ms; default 0), "graph" (1: the
+ * plain FP64 form's launches of a call are captured into a HIP graph and
+ * replayed through an executable graph kept per launch sequence, up to 16,
+ * updated in place with the call's arguments; same bits; default 0).
+ * rr_get_option also reads "graph_made" and "graph_updated" (executable
+ * graphs instantiated / reused so far).
  * Unknown names return RR_E_ARG. */
 int rr_set_option(rr_ctx* ctx, const char* name, int64_t value);
 

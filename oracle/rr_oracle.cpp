@@ -720,9 +720,13 @@ static std::vector<double> ts_qr_solve(const std::vector<std::vector<double>>& A
 // chunked form above; more: Eigen's algorithm on the data itself, below.
 static std::vector<double> colpiv_qr_data(std::vector<std::vector<double>> A, const std::vector<double>& b,
                                           int64_t n, int* rank_out);
+// Set for timing only (orc_set_qr_direct): every solve takes Eigen's order on
+// the data, the reference's own algorithm and cost, instead of the canonical
+// chunked form the GPU matches bit for bit.
+static std::atomic<int> g_qr_direct{0};
 static std::vector<double> colpiv_qr_solve(std::vector<std::vector<double>> A, const std::vector<double>& b,
                                            int64_t n, int* rank_out) {
-  if ((int)A.size() <= TS_MAX_COLS) return ts_qr_solve(A, b, n, rank_out);
+  if ((int)A.size() <= TS_MAX_COLS && !g_qr_direct.load(std::memory_order_relaxed)) return ts_qr_solve(A, b, n, rank_out);
   return colpiv_qr_data(std::move(A), b, n, rank_out);
 }
 
@@ -1330,6 +1334,8 @@ int orc_tune(int h, const char* pre, char** ols_out, char** factors_out, double*
 }
 
 // A column-major n x k
+void orc_set_qr_direct(int on) { g_qr_direct.store(on ? 1 : 0); }
+
 // direct: 1 to take Eigen's order on the data itself whatever k is (the
 // chunked form's cross-check), 0 for the canonical choice by k
 int orc_qr_solve(const double* A, const double* b, int64_t n, int k, double* coefs, int* rank, int direct) {

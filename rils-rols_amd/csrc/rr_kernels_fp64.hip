@@ -1,49 +1,45 @@
-// rr_kernels_fp64.hip -- the plain FP64 form of the candidate-fitness path.
+// rr_kernels_fp64.hip -- the plain FP64 form of the candidate-fitness path
+// (tune_constants + fitness, rils_rols_cpp.cpp:445-541), bit for bit the
+// oracle's results.
 //
-// One workgroup of 16 wavefronts owns one candidate and walks all n samples
-// in tiles of 1024*R rows: thread t handles rows t, t+1024, t+2048, ..., R of
-// them per tile, in increasing order.  Every sum over samples therefore
-// follows the canonical order of rr_math.h / rr_device.h whatever R is, and
-// the column-pivoted Householder QR
-// follows Eigen's ColPivHouseholderQR (eigen/Eigen/src/QR/
-// ColPivHouseholderQR.h:482-607, Householder.h:67-135) operation for
-// operation, so the results are the same bits as the CPU oracle's.
-//
-// Per candidate, k_eval_fp64 first evaluates each factor program
-// (tune_constants, rils_rols_cpp.cpp:477-482) into a kept column of a work
-// slab in HBM, spread over many small workgroups (row chunks), so that a
-// long program does not hold one compute unit for the whole column.  A
-// factor that is a bare variable keeps no copy: its column is the dataset's.
-// Then the QR kernels (k_qr_*)
-//   1. reads the factor columns once for their norms;
-//   2. runs the pivoted QR on the slab: per step one pass computing the
-//      Householder dot products and one pass applying the reflector, which
-//      also accumulates the tail norms the next step and the LAWN-176 norm
-//      recomputation need; the reflectors are applied to y as they are made
-//      (Q^T b of _solve_impl).  The first step reads every column from where
-//      its values already are (kept column, dataset column, the dataset's
-//      column of ones for the free term, y) and its update pass writes the
-//      working copies; the essential part of a reflector is recomputed
-//      (v / (c0 - beta)) where it is used rather than stored, since nothing
-//      reads it after its step; the last step, with only y left, needs no
-//      update pass.  The factorisation stops at the first step under Eigen's
-//      rank threshold, after which nothing changes the solution, and at the
-//      first reflector whose beta is NaN, after which the outcome is fixed
-//      (see nan_exit below);
-//   3. back-substitutes on the leading r x r block and scatters the
-//      coefficients through the column permutation (zeros for the rest);
-// Then k_score_fp64, one workgroup per (candidate, 4096-row chunk),
-//   4. reads the kept factor values (or the dataset's columns, which the QR
-//      does not touch) to form the prediction of the OLS tree that tune_constants
-//      builds (rils_rols_cpp.cpp:488-517): left-deep sum of
-//      c*f (or f when |c-1| < 1e-12), skipping |c| < 1e-12, free term last,
-//      and forms the chunk's sum of squared residuals in the chunked order
-//      (rrm::chunk_sum); k_fit_fp64 adds a candidate's chunk sums in order
-//      into 1-R2 and RMSE (fitness, 520-541).  A kept NaN coefficient makes
-//      every prediction NaN, so k_qr_solve writes that case as the sentinel
-//      and it is not scored.
-// A tree scored as given (RAW mode) takes the same two kernels, evaluating
-// the tree in place of the OLS prediction.
+// Every sum over samples follows an order fixed in advance (the chunked order
+// of rr_math.h: 4096-row chunks, 256 partials each, xor butterflies, chunk
+// sums in chunk order; the chunked QR's own order below), so each pass splits
+// into independent 4096-row work items while the results stay bit-identical
+// to the CPU oracle's, which sums the same way.  One call over a batch runs:
+//   k_nan0_fp64   one workgroup per candidate: factor 0 on rows 0..255; a
+//                 NaN at a row >= 1 fixes the outcome (the sentinel), which
+//                 is written here, and later kernels skip the candidate;
+//   k_alive_fp64, k_eval_fp64
+//                 each unique factor program (equal programs over one dataset
+//                 shared by the candidates that use them) that a candidate
+//                 still to be scored uses, evaluated (rils_rols_cpp.cpp:
+//                 477-482) per (factor, 4096-row chunk) work item into a kept
+//                 column, with the chunk's sums of squares for the norms; a
+//                 factor that is a bare variable keeps nothing (its column is
+//                 the dataset's);
+//   k_ts_init, k_ts_chunk<NC>, k_ts_solve (candidates of at most 9 columns)
+//                 the column-pivoted QR (Eigen's ColPivHouseholderQR,
+//                 ColPivHouseholderQR.h:482-607) carried out on the
+//                 triangular factor of the data: per 4096-row chunk an
+//                 unpivoted Householder QR of [finite columns | y] held in
+//                 registers, then one wave per candidate folds the chunk
+//                 factors together in chunk order and runs Eigen's pivot
+//                 choice, rank cut, reflectors and back substitution on the
+//                 small factor;
+//   k_qr_* (candidates of more than 9 columns)
+//                 the same factorisation step by step over the columns in
+//                 HBM, every candidate of the launch in lockstep (described
+//                 below);
+//   k_score_fp64 + k_fit_fp64
+//                 per (candidate, chunk) the OLS tree's prediction from the
+//                 coefficients and the kept columns (rils_rols_cpp.cpp:
+//                 488-517: left-deep sum of c*f, or f when |c-1| < 1e-12,
+//                 skipping |c| < 1e-12, free term last) and the chunk's sum of
+//                 squared residuals; per candidate the chunk sums in order
+//                 into 1-R2 and RMSE (fitness, 520-541).
+// A tree scored as given (RAW mode) takes only the two scoring kernels, the
+// tree evaluated in place of the OLS prediction.
 #include "rr_device.h"
 #include "rr_kernels.h"
 
@@ -1223,13 +1219,14 @@ __device__ inline const double* ts_col(const TsArgs& a, const DsDesc& D, int pbe
   return var >= 0 ? D.X + (int64_t)var * D.ldx : a.slab + a.prog_kept[pbeg + col];
 }
 
-__global__ __launch_bounds__(64) void k_ts_init(TsArgs a) {
-  const int pos = a.pos0 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  if (pos >= a.pos1) return;
+// One wave per position: lane 0 does the position's work (norms, outcome of
+// a non-finite step-0 pivot, finite columns, threshold, list slot); then the
+// wave writes the chunk work items' descriptors, a lane per chunk.
+__device__ inline bool ts_init_position(const TsArgs& a, int pos, int* slot_out) {
   TsState& s = a.st[pos];
   s.live = 0;
   const int cand = a.order[pos];
-  if (a.dead && a.dead[cand]) return;
+  if (a.dead && a.dead[cand]) return false;
   const DsDesc& D = a.ds[a.cand_ds ? a.cand_ds[cand] : 0];
   const int pbeg = a.cand_start[cand];
   const int nf = a.cand_start[cand + 1] - pbeg;
@@ -1264,7 +1261,7 @@ __global__ __launch_bounds__(64) void k_ts_init(TsArgs a) {
     for (int i = 0; i < size; ++i) co[i == 0 ? bi : (i == bi ? 0 : i)] = rrm::qnan();
     write_fit(a.fit + cand, rrm::qnan(), D.sst, n, size, 0);
     a.info[cand] = -1;
-    return;
+    return false;
   }
   int kf = 0;
   for (int j = 0; j < k; ++j)
@@ -1282,22 +1279,48 @@ __global__ __launch_bounds__(64) void k_ts_init(TsArgs a) {
   s.nch = nch;
   s.n = n;
   s.live = 1;
+  const int nb = ts_bucket(k);
+  const int b = nb == 10 ? 0 : nb == 5 ? 1 : 2;
+  *slot_out = atomicAdd(a.ts_cnt + b, nch);
+  return true;
+}
+
+__global__ __launch_bounds__(WAVE) void k_ts_init(TsArgs a) {
+  const int pos = a.pos0 + (int)blockIdx.x;
+  // what the other lanes need of lane 0's work, through LDS
+  __shared__ int live, slot, kf_s, nch_s, k_s;
+  __shared__ int64_t n_s;
+  __shared__ const double* col_s[TS_MAX_COLS + 1];
+  if (threadIdx.x == 0) {
+    int sl = 0;
+    live = ts_init_position(a, pos, &sl) ? 1 : 0;
+    slot = sl;
+    if (live) {
+      const TsState& s = a.st[pos];
+      kf_s = s.kf; nch_s = s.nch; k_s = s.k; n_s = s.n;
+      for (int f = 0; f <= s.kf; ++f) col_s[f] = s.col[f];
+    }
+  }
+  __syncthreads();
+  if (!live) return;   // uniform
   // the position's chunk work items, appended to its bucket's list (the
   // order of the list does not matter: every item writes its own factor) as
   // self-contained descriptors: where its factor goes, its rows, its finite
   // column count, its columns from the chunk's first row
-  const int nb = ts_bucket(k);
+  const int kf = kf_s, nch = nch_s;
+  const int64_t n = n_s;
+  const int nb = ts_bucket(k_s);
   const int b = nb == 10 ? 0 : nb == 5 ? 1 : 2;
   const int dw = ts_desc_words(nb);
-  const int slot = atomicAdd(a.ts_cnt + b, nch);
-  int64_t* dst = a.ts_desc + a.list_base[b] + (int64_t)slot * dw;
+  int64_t* const base = a.ts_desc + a.list_base[b] + (int64_t)slot * dw;
   double* const out = a.rpart + a.rp_off[pos];
-  for (int c = 0; c < nch; ++c, dst += dw) {
+  for (int c = (int)threadIdx.x; c < nch; c += WAVE) {
+    int64_t* const dst = base + (int64_t)c * dw;
     const int64_t r0 = (int64_t)c * TS_ROWS;
     dst[0] = (int64_t)(out + (int64_t)c * nb * (nb - 1));
     dst[1] = (n - r0 < TS_ROWS) ? n - r0 : TS_ROWS;
     dst[2] = kf;
-    for (int f = 0; f <= kf; ++f) dst[3 + f] = (int64_t)(s.col[f] + r0);
+    for (int f = 0; f <= kf; ++f) dst[3 + f] = (int64_t)(col_s[f] + r0);
   }
 }
 
@@ -1736,7 +1759,7 @@ static TsArgs ts_args(const TsLaunch& L, int pos0, int pos1, int bucket) {
 hipError_t launch_ts_init(const TsLaunch& L, int pos0, int pos1, hipStream_t s) {
   if (pos1 <= pos0) return hipSuccess;
   hipLaunchKernelGGL(k_ts_zero, dim3(1), dim3(WAVE), 0, s, L.cnt);
-  hipLaunchKernelGGL(k_ts_init, dim3((unsigned)((pos1 - pos0 + 63) / 64)), dim3(64), 0, s, ts_args(L, pos0, pos1, 0));
+  hipLaunchKernelGGL(k_ts_init, dim3((unsigned)(pos1 - pos0)), dim3(WAVE), 0, s, ts_args(L, pos0, pos1, 0));
   return hipGetLastError();
 }
 

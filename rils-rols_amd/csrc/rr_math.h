@@ -364,12 +364,11 @@ RR_HD inline double sin(double x) {
   if (__builtin_fabs(x) < 0x1p-27) return x;
   double rh, rl; int q;
   reduce<L>(x, &rh, &rl, &q);
-  switch (q) {
-    case 0: return ksin(rh, rl);
-    case 1: return kcos(rh, rl);
-    case 2: return -ksin(rh, rl);
-    default: return -kcos(rh, rl);
-  }
+  // quadrant q: sin, cos, -sin, -cos.  One kernel per parity and the sign
+  // apart, so lanes of a wavefront in different quadrants run each of the
+  // two polynomials once (a four-way switch runs each twice).
+  const double v = (q & 1) ? kcos(rh, rl) : ksin(rh, rl);
+  return (q & 2) ? -v : v;
 }
 
 template <class L = LargeInline>
@@ -378,12 +377,9 @@ RR_HD inline double cos(double x) {
   if (__builtin_fabs(x) < 0x1p-27) return 1.0;
   double rh, rl; int q;
   reduce<L>(x, &rh, &rl, &q);
-  switch (q) {
-    case 0: return kcos(rh, rl);
-    case 1: return -ksin(rh, rl);
-    case 2: return -kcos(rh, rl);
-    default: return ksin(rh, rl);
-  }
+  // quadrant q: cos, -sin, -cos, sin
+  const double v = (q & 1) ? ksin(rh, rl) : kcos(rh, rl);
+  return ((q + 1) & 2) ? -v : v;
 }
 
 // IEEE-754 correctly rounded square root on both compilers.

@@ -38,6 +38,7 @@ def lib():
         L.orc_tune.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.POINTER(ctypes.c_void_p),
                                ctypes.POINTER(ctypes.c_void_p), dp, ctypes.c_int, ip, ip, dp]
         L.orc_qr_solve.argtypes = [dp, dp, ctypes.c_int64, ctypes.c_int, dp, ip, ctypes.c_int]
+        L.orc_set_qr_direct.argtypes = [ctypes.c_int]
         L.orc_rewrite.argtypes = [ctypes.c_char_p, ctypes.c_int]
         L.orc_rewrite.restype = ctypes.c_void_p
         L.orc_all_candidates.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int]
@@ -167,6 +168,13 @@ class Dataset:
         fit = np.empty(3 * len(prefixes))
         secs = lib().orc_batch_tune(self.h, "\n".join(prefixes).encode(), threads, _dp(fit))
         return secs, fit.reshape(-1, 3)
+
+
+def set_qr_direct(on):
+    """Timing only: every least-squares solve of the oracle takes Eigen's
+    order on the data (the reference's algorithm) instead of the canonical
+    chunked form; results are then not the GPU's bits."""
+    lib().orc_set_qr_direct(int(bool(on)))
 
 
 def qr_solve(A, b, direct=False):

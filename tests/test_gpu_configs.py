@@ -135,7 +135,7 @@ def test_serving_mix_runs_same_alone_and_in_the_mix():
     runs = bench.serving_workload(keep=set(SERVING_RUNS))
     assert sum(r["classification"] for r in runs) >= 3 and sum(r["X"].shape[0] >= 500_000 for r in runs) >= 2
     arrivals = [p[1] for p in plan[:len(runs)]]   # the mix's first 12 arrival times
-    budget = 20000
+    budget = 100000   # the reference's default max_fit_calls (rils_rols.py:156): runs last long enough to meet
 
     def solo(r):
         n, d = r["X"].shape
@@ -145,7 +145,9 @@ def test_serving_mix_runs_same_alone_and_in_the_mix():
 
     want = [solo(r) for r in runs]
     print("solo runs done", flush=True)
-    srv = rils_rols_cpp.Server(0, 0, 64, 1, 0)
+    # a 300 us gather window, as in the serving benchmark: a launch waits that long for the other
+    # resident runs' requests
+    srv = rils_rols_cpp.Server(0, 0, 64, 1, 0, 300)
     start = time.perf_counter()
     ids = []
     for r, t in zip(runs, arrivals):
