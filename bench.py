@@ -163,6 +163,35 @@ def timed_steps(runner, rescorer, precision, warmup, steps):
     return total, np.array(lat), np.array(kms), coefs, fit, n64, fp32_fit, phases
 
 
+def timed_steps_multi(runners, precision, warmup, steps):
+    """K steps of one batched call spread over len(runners) contexts, one host
+    thread each: every context has its own stream and buffers, so one thread
+    checks and plans its next batch while another context's kernels run.
+    Returns the wall time of all K steps and each call's latency."""
+    import threading
+    for r in runners:
+        for _ in range(warmup):
+            r.run(precision)
+    share = [steps // len(runners) + (1 if i < steps % len(runners) else 0) for i in range(len(runners))]
+    lat = [[] for _ in runners]
+    gate = threading.Barrier(len(runners) + 1)
+
+    def work(i):
+        gate.wait()
+        for _ in range(share[i]):
+            s = time.perf_counter()
+            runners[i].run(precision)
+            lat[i].append(time.perf_counter() - s)
+    th = [threading.Thread(target=work, args=(i,)) for i in range(len(runners))]
+    for t in th:
+        t.start()
+    gate.wait()
+    t0 = time.perf_counter()
+    for t in th:
+        t.join()
+    return time.perf_counter() - t0, np.array(sum(lat, []))
+
+
 def cpu_baseline(X, y, trees, seconds_target=15.0, threads=1):
     """The oracle (CPU restatement of the reference path, g++ -O3) on a bounded
     prefix of the same candidates: tune_constants + fitness per candidate,
@@ -513,6 +542,8 @@ def main():
     ap.add_argument("--fp32-chunk", type=int, default=None, help="samples per FP32 workgroup (multiple of 512)")
     ap.add_argument("--fp32-streams", type=int, default=None, help="1: FP32 buckets on concurrent streams")
     ap.add_argument("--host-threads", type=int, default=None, help="host threads that check a large batch (1..64)")
+    ap.add_argument("--contexts", type=int, default=1,
+                    help="batched workload: spread the timed steps over this many contexts, one host thread each")
     ap.add_argument("--slab-budget-mb", type=int, default=None,
                     help="device memory of one FP64 launch's work columns (candidates are launched in chunks that fit)")
     ap.add_argument("--no-autotune", action="store_true",
@@ -635,6 +666,21 @@ def batched_main(args):
         elif form == "fp32c":
             f32["fit"] = fit
         res[prec] = dict(total=total, lat=lat, kms=kms, coefs=coefs, fit=fit, n64=n64, f32=f32, phases=phases)
+        if args.contexts > 1 and form != "fp32":
+            # the same steps over several contexts driven by their own host threads
+            others = []
+            for _ in range(args.contexts - 1):
+                o = rils_rols_cpp.BatchRunner(np.ascontiguousarray(X.T), y, 0)
+                for k in ("fp64_eval_rows", "fp64_eval_form", "fp32_rows", "fp32_chunk_rows", "fp32_streams",
+                          "slab_budget_mb", "host_threads"):
+                    o.set_option(k, runner.get_option(k))
+                o.prepare_ols(trees)
+                others.append(o)
+            res[prec]["multi"] = timed_steps_multi([runner] + others, form, args.warmup, args.steps)
+            print("[bench] %s, %d contexts: %.3f ms/step wall, %.0f evals/s" %
+                  (prec, args.contexts, 1e3 * res[prec]["multi"][0] / args.steps,
+                   N_TREES * args.steps / res[prec]["multi"][0]), file=sys.stderr, flush=True)
+            del others
         print("[bench] %s: %.3f ms/step wall, %.3f ms kernel, %.0f evals/s, phases %s" %
               (prec, 1e3 * total / args.steps, kms.mean(), N_TREES * args.steps / total,
                {k: round(v, 4) for k, v in phases.items()}), file=sys.stderr, flush=True)
@@ -707,6 +753,16 @@ def batched_main(args):
             out["fp32"]["r2_abs_err_max_certified"] = \
                 float(np.max(np.abs(a[certified, 0] - b[certified, 0]))) if certified.any() else None
             out["fp32"]["sentinels_agree"] = bool(np.array_equal(a[:, 4].astype(int) & 1, b[:, 4].astype(int) & 1))
+
+    for prec, o in out.items():
+        if "multi" in res[prec]:
+            # the step loop over several contexts: its wall time and per-call latencies are the form's
+            # figures, the single-context loop's beside them
+            tot, lat = res[prec]["multi"]
+            o["single_context"] = {k: o[k] for k in ("evals_per_s", "ms_per_step", "p50_step_ms", "p99_step_ms")}
+            o.update(evals_per_s=N_TREES * args.steps / tot, ms_per_step=1e3 * tot / args.steps,
+                     p50_step_ms=float(np.percentile(lat, 50) * 1e3),
+                     p99_step_ms=float(np.percentile(lat, 99) * 1e3), contexts=args.contexts)
 
     cpu = None
     cpu1 = None

@@ -53,25 +53,42 @@
 //                             project's convention (Eigen's is unreadable):
 //                             last-bit agreement with the reference is
 //                             parity unpinned
-//   ColPivHouseholderQR ..... numpy lstsq on full-rank designs, exact integer
-//                             coefficients, duplicated / constant / zero
-//                             columns, n < k, pivot order of LAPACK geqp3
-//                             (test_qr_*)
+//   ColPivHouseholderQR ..... Eigen's order on the data (more than 9
+//                             columns, and the direct mode): numpy lstsq on
+//                             full-rank designs, exact integer coefficients,
+//                             duplicated / constant / zero columns, n < k,
+//                             pivot order of LAPACK geqp3 (test_qr_*)
+//   chunked QR (<= 9 cols) .. lstsq and exact integer fits over 1 to 49
+//                             chunks (1 to 4 groups) in every column bucket;
+//                             the same rank, zero pattern and coefficients as
+//                             Eigen's order on designs whose rank cut is clear
+//                             (columns living in one chunk or one group,
+//                             zero columns, wide designs); NaN / Inf outcomes.
+//                             Exactly collinear columns leave a residual at
+//                             the size of the cut itself, decided by rounding
+//                             in every order, Eigen's included
+//                             (test_exact_collinearity_rank_is_an_order_decision):
+//                             agreement with the reference there is parity
+//                             unpinned
 //   score / R2 / RMSE ....... closed forms: exact fits, formulas, NaN sentinel,
 //                             Inf passthrough
 //   rewrites, to_string ..... known answers derived by hand from node.cpp,
 //                             faithful bugs included
-//   all_candidates .......... known answer for the perturbations of x0; the
-//                             local-search lists beyond that: parity unpinned
-//                             (checked for agreement with the product only)
+//   all_candidates .......... known answers: the main loop's perturbations of
+//                             x0, the local-search lists of (x0+2) and
+//                             ln(sin(x0)) (tests/test_search_pins.py)
 //   acceptance primitives ... logic tables (test_acceptance_primitives_*)
 //   glibc rand sequence ..... the same library call (test_crand_is_glibc_rand
 //                             in tests/test_host_engine.py)
-//   local_search / fit_inner  determinism and fit-call budget accounting;
+//   local_search ............ a search hand-traced on a tiny data set:
+//                             acceptance, pareto, budget cuts, the extra fit
+//                             call (tests/test_search_pins.py)
+//   fit_inner ............... determinism and fit-call budget accounting;
 //                             exact recovery of the generating formula on seven
 //                             of the reference's own noise-free datasets, where
 //                             the reference reports R2 = 1 (tests/test_recovery.py);
-//   move sequence ........... the end-to-end sequence itself is parity unpinned
+//                             the move sequence of longer runs is checked for
+//                             agreement with the product only
 #define RR_HD
 #include "../rils-rols_amd/csrc/rr_math.h"
 
@@ -506,9 +523,14 @@ static std::vector<double> evaluate(const Node& nd, const Data& D) {
 //      is applied to a column t as w = t_j + (x . t) / den, t_j -= tau w,
 //      t_i -= x_i g with g = (tau w) / den for the rows below: one division
 //      per column, and every product-sum and update a fused multiply-add.
-//   3. The chunk factors combined in chunk order: R = R_0, then R = the
-//      triangular factor of [R; R_c] (Householder on each column of the
-//      stacked pair, sums in row order, the same update).
+//   3. The chunk factors combined in two levels: the chunks fall into groups
+//      of TS_GROUP consecutive chunks; within a group, in chunk order, G = the
+//      group's first factor, then G = the triangular factor of [G; R_c]
+//      (Householder on each column of the stacked pair, sums in row order,
+//      the same update); then R = G_0 and the group factors folded into it
+//      the same way in group order.  The groups fold independently of each
+//      other, so the sequential depth is TS_GROUP + groups instead of the
+//      chunk count.
 //   4. Eigen's pivoted steps on R: pivot = largest current norm (left to
 //      right, strict >), rank cut at biggest^2 < (max initial norm * eps)^2 /
 //      n * (n - p), makeHouseholder and its application to the remaining
@@ -519,6 +541,7 @@ static std::vector<double> evaluate(const Node& nd, const Data& D) {
 // pivoted QR on the data; rounding is this order's.
 constexpr int64_t TS_ROWS = 4096;      // rows per chunk
 constexpr int TS_MAX_COLS = 9;         // columns (free term included) handled this way
+constexpr int64_t TS_GROUP = 16;       // chunks per group of the two-level combination
 
 // The column bucket of a candidate with k columns: columns with y of its
 // chunk factors (3, 5 or 10), and the threads of its chunk sums: 256 threads
@@ -607,7 +630,29 @@ static std::vector<double> ts_qr_solve(const std::vector<std::vector<double>>& A
   const int nthr = ts_threads(ts_bucket(k));
   // 2. chunk factors; 3. their combination.  R: kf rows x nc columns, R[i][c].
   std::vector<double> R((size_t)kf * nc, 0.0), T((size_t)kf * nc);
+  // [Rm; Tm] -> Rm, Tm's column j has rows 0..j (Tm is overwritten)
+  auto fold = [&](std::vector<double>& Rm, std::vector<double>& Tm) {
+    for (int j = 0; j < kf; ++j) {
+      double sxx = 0.0;
+      for (int i = 0; i <= j; ++i) sxx = std::fma(Tm[(size_t)i * nc + j], Tm[(size_t)i * nc + j], sxx);
+      double beta, tau, den;
+      householder(Rm[(size_t)j * nc + j], sxx, &beta, &tau, &den);
+      if (tau != 0.0) {
+        for (int c = j + 1; c < nc; ++c) {
+          double sx = 0.0;
+          for (int i = 0; i <= j; ++i) sx = std::fma(Tm[(size_t)i * nc + j], Tm[(size_t)i * nc + c], sx);
+          const double w = Rm[(size_t)j * nc + c] + sx / den;
+          const double g = (tau * w) / den;
+          Rm[(size_t)j * nc + c] = Rm[(size_t)j * nc + c] - tau * w;
+          for (int i = 0; i <= j; ++i)
+            Tm[(size_t)i * nc + c] = std::fma(-Tm[(size_t)i * nc + j], g, Tm[(size_t)i * nc + c]);
+        }
+      }
+      Rm[(size_t)j * nc + j] = beta;
+    }
+  };
   const int64_t nch = (n + TS_ROWS - 1) / TS_ROWS;
+  std::vector<std::vector<double>> G((size_t)((nch + TS_GROUP - 1) / TS_GROUP));
   for (int64_t ch = 0; ch < nch; ++ch) {
     const int64_t r0 = ch * TS_ROWS, m = std::min<int64_t>(TS_ROWS, n - r0);
     std::vector<std::vector<double>> P(nc, std::vector<double>((size_t)m));
@@ -635,27 +680,15 @@ static std::vector<double> ts_qr_solve(const std::vector<std::vector<double>>& A
       }
       P[j][j] = beta;
     }
-    double* dst = ch == 0 ? R.data() : T.data();
     for (int i = 0; i < kf; ++i)
-      for (int c = 0; c < nc; ++c) dst[(size_t)i * nc + c] = (i < m && c >= i) ? P[c][i] : 0.0;
-    if (ch == 0) continue;
-    for (int j = 0; j < kf; ++j) {   // [R; T] -> R, T's column j has rows 0..j
-      double sxx = 0.0;
-      for (int i = 0; i <= j; ++i) sxx = std::fma(T[(size_t)i * nc + j], T[(size_t)i * nc + j], sxx);
-      double beta, tau, den;
-      householder(R[(size_t)j * nc + j], sxx, &beta, &tau, &den);
-      if (tau != 0.0) {
-        for (int c = j + 1; c < nc; ++c) {
-          double sx = 0.0;
-          for (int i = 0; i <= j; ++i) sx = std::fma(T[(size_t)i * nc + j], T[(size_t)i * nc + c], sx);
-          const double w = R[(size_t)j * nc + c] + sx / den;
-          const double g = (tau * w) / den;
-          R[(size_t)j * nc + c] = R[(size_t)j * nc + c] - tau * w;
-          for (int i = 0; i <= j; ++i) T[(size_t)i * nc + c] = std::fma(-T[(size_t)i * nc + j], g, T[(size_t)i * nc + c]);
-        }
-      }
-      R[(size_t)j * nc + j] = beta;
-    }
+      for (int c = 0; c < nc; ++c) T[(size_t)i * nc + c] = (i < m && c >= i) ? P[c][i] : 0.0;
+    std::vector<double>& Gg = G[(size_t)(ch / TS_GROUP)];
+    if (ch % TS_GROUP == 0) Gg = T;
+    else fold(Gg, T);
+  }
+  if (!G.empty()) {
+    R = G[0];
+    for (size_t g = 1; g < G.size(); ++g) fold(R, G[g]);
   }
   // 4. pivoted steps on R
   std::vector<int> fidx(k, -1);   // column -> its column in R

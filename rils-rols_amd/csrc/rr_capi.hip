@@ -1050,6 +1050,7 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
   struct Chunk64 {
     int c0, c1, maxk, p0, npair, nitem, per, sitem, sper, ne, depth;
     int ts_pos[4], ts_it[4];
+    int ts_maxch;       // the most chunk work items of a chunked-QR position
     int64_t ts_lw[4];   // the buckets' work-item descriptors: first word, and the words in all
     std::vector<int32_t> step_pos, step_items;
   };
@@ -1340,6 +1341,8 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
       }
       if (ch.ts_pos[3] != mc) return fail(RR_E_ARG, "internal: chunked QR positions out of column order");
       for (int i = 0; i < 4; ++i) ch.ts_it[i] = item_at(ch.ts_pos[i]);
+      ch.ts_maxch = 0;
+      for (int r = ch.ne; r < mc; ++r) ch.ts_maxch = std::max(ch.ts_maxch, item_at(r + 1) - item_at(r));
       ch.ts_lw[0] = 0;
       for (int i = 0; i < 3; ++i)
         ch.ts_lw[i + 1] = ch.ts_lw[i] + (int64_t)(ch.ts_it[i + 1] - ch.ts_it[i]) * rr::ts_desc_words(TS_NC[i]);
@@ -1422,7 +1425,7 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
         for (int i = 0; i < 3; ++i) T.list_base[i] = ch.ts_lw[i];
         RR_HIP(rr::launch_ts_init(T, ch.ne, mc, stream));
         for (int i = 0; i < 3; ++i) RR_HIP(rr::launch_ts_chunk(T, i, ch.ts_it[i + 1] - ch.ts_it[i], stream));
-        RR_HIP(rr::launch_ts_solve(T, ch.ne, mc, stream));
+        RR_HIP(rr::launch_ts_solve(T, ch.ne, mc, ch.ts_maxch, stream));
       }
       end_phase(ctx, pm, stream);
       rr::ScoreLaunch Sc{};

@@ -72,9 +72,14 @@ struct QrLaunch {
 //   launch_ts_chunk  per bucket, a grid of workgroups over the bucket's list:
 //                    unpivoted Householder QR of a chunk of TS_ROWS rows of
 //                    [finite columns | y], its triangular factor to rpart;
-//   launch_ts_solve  one wave per position: the chunk factors combined in
-//                    chunk order, the pivoted steps and the solve on the
-//                    result; coefficients, info (as the step-synchronous QR).
+//   launch_ts_solve  the chunk factors combined in two levels -- one wave
+//                    per (position, group of TS_GROUP consecutive chunks)
+//                    folds the group's factors in chunk order into the slot
+//                    of its first chunk, then one wave per position folds
+//                    the group factors in group order -- and on the result
+//                    the pivoted steps and the solve; coefficients, info (as
+//                    the step-synchronous QR).  max_chunks: the most chunks
+//                    of any position in the range.
 // Positions pos0..pos1-1 of the arrays (pointers not offset).  Column buckets
 // 0, 1, 2: 10, 5, 3 columns with y (ts_bucket).  Position p's chunk factors
 // sit at rpart + rp_off[p] + chunk * W, W = nc * (nc - 1) words for its
@@ -85,6 +90,7 @@ struct QrLaunch {
 constexpr int TS_MAX_COLS = 9;
 constexpr int64_t TS_ROWS = 4096;
 constexpr int TS_GRID = 2048;   // workgroups of a chunk launch at most
+constexpr int TS_GROUP = 16;    // chunks per group of the two-level combination
 struct TsState {
   int32_t k, kf, size, live;            // live: 1 when the chunk and solve kernels have work
   int32_t fin[TS_MAX_COLS];             // the finite columns, in column order
@@ -120,7 +126,7 @@ hipError_t launch_ts_init(const TsLaunch& L, int pos0, int pos1, hipStream_t s);
 // bucket 0, 1, 2; max_items: the bucket's chunk work items if every position
 // were live
 hipError_t launch_ts_chunk(const TsLaunch& L, int bucket, int max_items, hipStream_t s);
-hipError_t launch_ts_solve(const TsLaunch& L, int pos0, int pos1, hipStream_t s);
+hipError_t launch_ts_solve(const TsLaunch& L, int pos0, int pos1, int64_t max_chunks, hipStream_t s);
 // the column count with y of the bucket of a candidate with k columns (free term included, k <= TS_MAX_COLS)
 __host__ __device__ inline int ts_bucket(int k) { return k <= 2 ? 3 : k <= 4 ? 5 : 10; }
 // words of a chunk work item's descriptor in the bucket of nc columns with y

@@ -1482,41 +1482,27 @@ __global__ __launch_bounds__(ts_threads(NC)) void k_ts_chunk(TsArgs a) {
   }
 }
 
-// One wave per position: the chunk factors combined in chunk order, Eigen's
-// pivoted steps on the result, the solve.  The factor R (kf rows, nc columns
-// with y) lives in LDS; in every step lane c carries column c, its sums in
-// row order as in the oracle.  The next chunk factor is loaded while the
-// current one is folded in.
 constexpr int TS_RW = TS_MAX_COLS * (TS_MAX_COLS + 1);   // words of the largest factor (bucket 10)
-__global__ __launch_bounds__(WAVE) void k_ts_solve(TsArgs a) {
-  const int pos = a.pos0 + (int)blockIdx.x;
-  const TsState& s = a.st[pos];
-  if (!s.live) return;   // uniform
-  __shared__ double R[TS_RW], T[TS_RW];
-  __shared__ int perm[TS_MAX_COLS], fidx[TS_MAX_COLS];
-  __shared__ double nu[TS_MAX_COLS];
-  const int l = (int)threadIdx.x;
-  const int cand = a.order[pos];
-  const DsDesc& D = a.ds[a.cand_ds ? a.cand_ds[cand] : 0];
-  const int pbeg = a.cand_start[cand];
-  const int k = s.k, kf = s.kf, nc = kf + 1, size = s.size;
-  const int NC = ts_bucket(k);
-  const int W = NC * (NC - 1);   // words per chunk factor
-  const int64_t n = D.n;
-  const double* const src = a.rpart + a.rp_off[pos];
-  const int nch = s.nch;
-  double pre0 = 0.0, pre1 = 0.0;   // words l and l + 64 of the next chunk factor
-  if (l < W) R[l] = src[l];
-  if (l + WAVE < W) R[l + WAVE] = src[l + WAVE];
-  if (nch > 1) {
-    if (l < W) pre0 = src[W + l];
-    if (l + WAVE < W) pre1 = src[W + l + WAVE];
+
+// Fold count - 1 factors into R, in order: the ones at src + i * stride, i =
+// 1 .. count - 1 (R already holds the one at src).  Each fold replaces R by
+// the triangular factor of [R; T] (T's column j has rows 0..j): per column j
+// a Householder reflector from R(j, j) and T's column j, applied to the
+// columns right of j, lane c carrying column c, its sums in row order as in
+// the oracle.  The next factor is loaded while the current one is folded in.
+// One wave; R and T in LDS (kf rows of NC words).
+__device__ inline void ts_fold_chain(double* R, double* T, const double* src, int64_t stride, int count, int W,
+                                     int kf, int nc, int NC, int l) {
+  double pre0 = 0.0, pre1 = 0.0;   // words l and l + 64 of the next factor
+  if (count > 1) {
+    if (l < W) pre0 = src[stride + l];
+    if (l + WAVE < W) pre1 = src[stride + l + WAVE];
   }
-  for (int ch = 1; ch < nch; ++ch) {
+  for (int f = 1; f < count; ++f) {
     if (l < W) T[l] = pre0;
     if (l + WAVE < W) T[l + WAVE] = pre1;
-    if (ch + 1 < nch) {
-      const double* nx = src + (int64_t)(ch + 1) * W;
+    if (f + 1 < count) {
+      const double* nx = src + (int64_t)(f + 1) * stride;
       if (l < W) pre0 = nx[l];
       if (l + WAVE < W) pre1 = nx[l + WAVE];
     }
@@ -1540,6 +1526,56 @@ __global__ __launch_bounds__(WAVE) void k_ts_solve(TsArgs a) {
       __syncthreads();
     }
   }
+}
+
+// The first level of the combination: one wave per (position, group of
+// TS_GROUP consecutive chunks) folds the group's chunk factors in chunk order
+// and leaves the result in the slot of the group's first chunk.
+__global__ __launch_bounds__(WAVE) void k_ts_group(TsArgs a) {
+  const int pos = a.pos0 + (int)blockIdx.x;
+  const TsState& s = a.st[pos];
+  const int first = (int)blockIdx.y * TS_GROUP;
+  if (!s.live || first >= s.nch) return;   // uniform
+  const int count = min(TS_GROUP, s.nch - first);
+  if (count < 2) return;                    // a group of one chunk is its own factor
+  __shared__ double R[TS_RW], T[TS_RW];
+  const int l = (int)threadIdx.x;
+  const int kf = s.kf, nc = kf + 1;
+  const int NC = ts_bucket(s.k);
+  const int W = NC * (NC - 1);   // words per chunk factor
+  double* const src = a.rpart + a.rp_off[pos] + (int64_t)first * W;
+  if (l < W) R[l] = src[l];
+  if (l + WAVE < W) R[l + WAVE] = src[l + WAVE];
+  ts_fold_chain(R, T, src, W, count, W, kf, nc, NC, l);
+  __syncthreads();
+  if (l < W) src[l] = R[l];
+  if (l + WAVE < W) src[l + WAVE] = R[l + WAVE];
+}
+
+// One wave per position: the group factors (k_ts_group) combined in group
+// order, Eigen's pivoted steps on the result, the solve.  The factor R (kf
+// rows, nc columns with y) lives in LDS; in every step lane c carries column
+// c, its sums in row order as in the oracle.
+__global__ __launch_bounds__(WAVE) void k_ts_solve(TsArgs a) {
+  const int pos = a.pos0 + (int)blockIdx.x;
+  const TsState& s = a.st[pos];
+  if (!s.live) return;   // uniform
+  __shared__ double R[TS_RW], T[TS_RW];
+  __shared__ int perm[TS_MAX_COLS], fidx[TS_MAX_COLS];
+  __shared__ double nu[TS_MAX_COLS];
+  const int l = (int)threadIdx.x;
+  const int cand = a.order[pos];
+  const DsDesc& D = a.ds[a.cand_ds ? a.cand_ds[cand] : 0];
+  const int pbeg = a.cand_start[cand];
+  const int k = s.k, kf = s.kf, nc = kf + 1, size = s.size;
+  const int NC = ts_bucket(k);
+  const int W = NC * (NC - 1);   // words per chunk factor
+  const int64_t n = D.n;
+  const double* const src = a.rpart + a.rp_off[pos];
+  const int groups = (s.nch + TS_GROUP - 1) / TS_GROUP;
+  if (l < W) R[l] = src[l];
+  if (l + WAVE < W) R[l + WAVE] = src[l + WAVE];
+  ts_fold_chain(R, T, src, (int64_t)TS_GROUP * W, groups, W, kf, nc, NC, l);
   if (l < k) { perm[l] = l; fidx[l] = -1; nu[l] = s.nd[l]; }
   __syncthreads();
   if (l < kf) fidx[s.fin[l]] = l;
@@ -1773,9 +1809,14 @@ hipError_t launch_ts_chunk(const TsLaunch& L, int bucket, int max_items, hipStre
   return hipGetLastError();
 }
 
-hipError_t launch_ts_solve(const TsLaunch& L, int pos0, int pos1, hipStream_t s) {
+hipError_t launch_ts_solve(const TsLaunch& L, int pos0, int pos1, int64_t max_chunks, hipStream_t s) {
   if (pos1 <= pos0) return hipSuccess;
-  hipLaunchKernelGGL(k_ts_solve, dim3((unsigned)(pos1 - pos0)), dim3(WAVE), 0, s, ts_args(L, pos0, pos1, 0));
+  const TsArgs a = ts_args(L, pos0, pos1, 0);
+  const int64_t groups = (max_chunks + TS_GROUP - 1) / TS_GROUP;
+  if (groups > 65535) return hipErrorInvalidValue;
+  if (max_chunks > 1)
+    hipLaunchKernelGGL(k_ts_group, dim3((unsigned)(pos1 - pos0), (unsigned)groups), dim3(WAVE), 0, s, a);
+  hipLaunchKernelGGL(k_ts_solve, dim3((unsigned)(pos1 - pos0)), dim3(WAVE), 0, s, a);
   return hipGetLastError();
 }
 
