@@ -182,7 +182,7 @@ struct Nan0Launch {
   int32_t* dead;
   int n_cands;
 };
-constexpr int NAN0_ROWS = 256;
+constexpr int NAN0_ROWS = 64;
 
 // Factor evaluation of the FP64 OLS form into the kept slab columns: one
 // workgroup per (unique factor, chunk of EVAL_CHUNK_ROWS rows).  A unique

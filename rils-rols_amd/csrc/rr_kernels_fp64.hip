@@ -7,7 +7,7 @@
 // sums in chunk order; the chunked QR's own order below), so each pass splits
 // into independent 4096-row work items while the results stay bit-identical
 // to the CPU oracle's, which sums the same way.  One call over a batch runs:
-//   k_nan0_fp64   one workgroup per candidate: factor 0 on rows 0..255; a
+//   k_nan0_fp64   one wave per candidate: factor 0 on rows 0..63; a
 //                 NaN at a row >= 1 fixes the outcome (the sentinel), which
 //                 is written here, and later kernels skip the candidate;
 //   k_alive_fp64, k_eval_fp64
@@ -937,7 +937,7 @@ __global__ __launch_bounds__(EV_THREADS) void k_xnorm_fp64(const double* X, int6
   if (threadIdx.x == 0) { out[2 * blockIdx.x] = tot0; out[2 * blockIdx.x + 1] = tot1; }
 }
 
-// The factor-0 pre-pass: one workgroup of NAN0_ROWS threads per candidate,
+// The factor-0 pre-pass: one workgroup of NAN0_ROWS threads (one wave) per candidate,
 // one row each, so it costs one evaluation of latency.
 struct Nan0Args {
   const int32_t* cand_start;

@@ -76,7 +76,10 @@ def test_raw_fitness_bitexact(ctx, n, d, seed):
     ctx.free(h)
 
 
-@pytest.mark.parametrize("n,d,seed", [(150, 2, 11), (777, 4, 12), (5000, 10, 13), (64, 3, 14), (7, 3, 15)])
+# 70001 and 135000 rows: 18 and 33 chunks, folded in two and three groups of
+# 16 chunks (the last group of one chunk or a few)
+@pytest.mark.parametrize("n,d,seed", [(150, 2, 11), (777, 4, 12), (5000, 10, 13), (64, 3, 14), (7, 3, 15),
+                                      (70001, 6, 16), (135000, 3, 17)])
 def test_ols_bitexact(ctx, n, d, seed):
     print("case: ols n=%d d=%d seed=%d" % (n, d, seed), flush=True)
     X, y = make_data(seed, n, d)
