@@ -174,14 +174,18 @@ def timed_steps_multi(runners, precision, warmup, steps):
             r.run(precision)
     share = [steps // len(runners) + (1 if i < steps % len(runners) else 0) for i in range(len(runners))]
     lat = [[] for _ in runners]
+    errors = []
     gate = threading.Barrier(len(runners) + 1)
 
     def work(i):
         gate.wait()
-        for _ in range(share[i]):
-            s = time.perf_counter()
-            runners[i].run(precision)
-            lat[i].append(time.perf_counter() - s)
+        try:
+            for _ in range(share[i]):
+                s = time.perf_counter()
+                runners[i].run(precision)
+                lat[i].append(time.perf_counter() - s)
+        except Exception as e:   # reported by the main thread
+            errors.append(e)
     th = [threading.Thread(target=work, args=(i,)) for i in range(len(runners))]
     for t in th:
         t.start()
@@ -189,7 +193,10 @@ def timed_steps_multi(runners, precision, warmup, steps):
     t0 = time.perf_counter()
     for t in th:
         t.join()
-    return time.perf_counter() - t0, np.array(sum(lat, []))
+    total = time.perf_counter() - t0
+    if errors:
+        raise errors[0]
+    return total, np.array(sum(lat, []))
 
 
 def cpu_baseline(X, y, trees, seconds_target=15.0, threads=1):
@@ -237,6 +244,16 @@ def pmc_traffic(kernel_tag):
     with open(path) as f:
         d = json.load(f)
     return 1024.0 * (d["fetch_kib"] * d["fetch_scale"] + d["write_kib"] * d["write_scale"])
+
+
+def pmc_counters(kernel_tag):
+    """The dominant kernel's VALU counter figures from a committed PMC summary
+    (profiles/pmc_<tag>.json: eval_kernel_counters), or None."""
+    path = os.path.join(ROOT, "profiles", "pmc_%s.json" % kernel_tag)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("eval_kernel_counters")
 
 
 def copy_probe():
@@ -717,7 +734,8 @@ def batched_main(args):
                          "traffic": traffic,
                          "kernel": "k_alive_fp64 + k_eval_fp64<%d, %d>" % (runner.get_option("fp64_eval_rows"),
                                                                           runner.get_option("fp64_eval_form")),
-                         "launch_ms": ph["eval"], "algorithmic_flops": eval_flops, "factors_evaluated": n_eval},
+                         "launch_ms": ph["eval"], "algorithmic_flops": eval_flops, "factors_evaluated": n_eval,
+                         "counters": pmc_counters("fp64_step")},
             "step_roofline": {"bound": "valu", "achieved": step_flops / kernel_s / 1e12,
                               "peak": PEAK_FP64_VALU_TFLOPS, "unit": "TFLOP/s",
                               "frac": step_flops / kernel_s / 1e12 / PEAK_FP64_VALU_TFLOPS,
