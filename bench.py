@@ -568,6 +568,9 @@ def main():
     ap.add_argument("--fp32-chunk", type=int, default=None, help="samples per FP32 workgroup (multiple of 512)")
     ap.add_argument("--fp32-streams", type=int, default=None, help="1: FP32 buckets on concurrent streams")
     ap.add_argument("--host-threads", type=int, default=None, help="host threads that check a large batch (1..64)")
+    ap.add_argument("--pipeline", type=int, default=1, choices=[0, 1],
+                    help="batched workload, plain FP64 form: time the steps pipelined (submit the next step "
+                         "before waiting for the current one); the one-call-at-a-time figures stay beside them")
     ap.add_argument("--contexts", type=int, default=1,
                     help="batched workload: spread the timed steps over this many contexts, one host thread each")
     ap.add_argument("--slab-budget-mb", type=int, default=None,
@@ -576,7 +579,8 @@ def main():
                     help="keep the default kernel schedule instead of the autotuner's pick")
     ap.add_argument("--copy-probe", action="store_true", help="run only a plain 1 GiB device copy (PMC scaling)")
     ap.add_argument("--timed-only", action="store_true",
-                    help="launch nothing on the GPU beyond the warmup and timed steps (for kernel traces)")
+                    help="launch nothing on the GPU beyond the warmup and the one-call-at-a-time timed steps "
+                         "(for kernel traces and counters: no pipelined leg, no extra untimed calls)")
     ap.add_argument("--workload", default="batched", choices=["batched", "serving", "precision", "search", "small"],
                     help="batched: the headline batched candidate eval; serving: the serving mix; precision: 16384 "
                          "trees at n=1e6 x 20 in FP32; search: a full search at n=1e6 x 5")
@@ -692,7 +696,17 @@ def batched_main(args):
         elif form == "fp32c":
             f32["fit"] = fit
         res[prec] = dict(total=total, lat=lat, kms=kms, coefs=coefs, fit=fit, n64=n64, f32=f32, phases=phases)
-        if args.contexts > 1 and form != "fp32":
+        if form == "fp64" and args.pipeline and not args.timed_only:
+            # the same steps pipelined on one context: step i + 1 is checked, planned and launched
+            # while step i runs (rr_fitness_submit / rr_fitness_wait)
+            if args.warmup:
+                runner.run_pipelined(args.warmup)
+            _, _, plat, ptot = runner.run_pipelined(args.steps)
+            res[prec]["multi"] = (ptot, np.array(plat))
+            res[prec]["overlap"] = "pipelined: each step submitted before the previous one is waited for"
+            print("[bench] %s pipelined: %.3f ms/step wall, %.0f evals/s" %
+                  (prec, 1e3 * ptot / args.steps, N_TREES * args.steps / ptot), file=sys.stderr, flush=True)
+        elif args.contexts > 1 and form != "fp32":
             # the same steps over several contexts driven by their own host threads
             others = []
             for _ in range(args.contexts - 1):
@@ -703,6 +717,7 @@ def batched_main(args):
                 o.prepare_ols(trees)
                 others.append(o)
             res[prec]["multi"] = timed_steps_multi([runner] + others, form, args.warmup, args.steps)
+            res[prec]["overlap"] = "%d contexts, one host thread each" % args.contexts
             print("[bench] %s, %d contexts: %.3f ms/step wall, %.0f evals/s" %
                   (prec, args.contexts, 1e3 * res[prec]["multi"][0] / args.steps,
                    N_TREES * args.steps / res[prec]["multi"][0]), file=sys.stderr, flush=True)
@@ -786,10 +801,10 @@ def batched_main(args):
             # the step loop over several contexts: its wall time and per-call latencies are the form's
             # figures, the single-context loop's beside them
             tot, lat = res[prec]["multi"]
-            o["single_context"] = {k: o[k] for k in ("evals_per_s", "ms_per_step", "p50_step_ms", "p99_step_ms")}
+            o["one_call_at_a_time"] = {k: o[k] for k in ("evals_per_s", "ms_per_step", "p50_step_ms", "p99_step_ms")}
             o.update(evals_per_s=N_TREES * args.steps / tot, ms_per_step=1e3 * tot / args.steps,
                      p50_step_ms=float(np.percentile(lat, 50) * 1e3),
-                     p99_step_ms=float(np.percentile(lat, 99) * 1e3), contexts=args.contexts)
+                     p99_step_ms=float(np.percentile(lat, 99) * 1e3), overlap=res[prec]["overlap"])
 
     cpu = None
     cpu1 = None

@@ -122,7 +122,7 @@ typedef struct rr_batch {
                             and coefficients (NaN) must come from RR_FP64 */
 #define RR_F_FP64 4      /* RR_FP32_CHECKED: this result came from the FP64 form */
 #define RR_F_EARLY 8     /* OLS mode: decided by the factor-0 pre-pass (factor 0
-                            NaN at one of the first 256 samples, sample 0 aside):
+                            NaN at one of the first 64 samples, sample 0 aside):
                             the sentinel, no other pass over the samples */
 /* FP64 OLS form: bits 16..31 of flags hold the number of Householder steps
  * that ran passes over the samples (for traffic accounting) */
@@ -191,6 +191,24 @@ int rr_check_batch(const rr_batch* batch, int32_t mode, int32_t n_features);
  * serves the request loop that keeps many regression runs resident. */
 int rr_fitness_multi(rr_ctx* ctx, const int32_t* handles, const rr_batch* batch, int32_t mode, int32_t precision,
                      double* coefs_out, rr_fit* fit_out);
+
+/* Pipelined form of rr_fitness_batch(RR_FP64) for callers with independent
+ * batches: rr_fitness_submit checks, plans and launches the batch and returns
+ * without waiting for it; rr_fitness_wait completes the oldest pending call,
+ * writing its coefs_out / fit_out.  At most two calls are pending per context
+ * (a third submit fails with RR_E_ARG), so the host can check and plan the
+ * next batch while the current one runs, its kernels queued behind on the
+ * context's stream.  The batch arrays may be reused once submit returns; the
+ * output arrays must stay valid until the matching wait.  Any other call on
+ * the context (fitness, predict, dataset upload or free, autotune) completes
+ * the pending calls first; every submit is still retired by its own wait,
+ * which then returns at once.  With option "phase_timing" or "graph" set,
+ * submit runs the call to completion the same way.  Results are
+ * bit-for-bit those of rr_fitness_batch.  No counterpart in the reference,
+ * whose fitness calls are synchronous member functions. */
+int rr_fitness_submit(rr_ctx* ctx, int32_t handle, const rr_batch* batch, int32_t mode, double* coefs_out,
+                      rr_fit* fit_out);
+int rr_fitness_wait(rr_ctx* ctx);
 
 /* The plain FP64 form of rr_fitness_batch (RR_MODE_OLS or RR_MODE_RAW, one
  * dataset) with no device or pinned-host allocation: no counterpart in the
@@ -269,7 +287,7 @@ int rr_last_kernel_ms(rr_ctx* ctx, double* ms);
  * is 1 (default 0); a phase that did not run reads 0.  With the FP64 form on
  * a side stream (RR_FP32_CHECKED) phases overlap and their sum can exceed
  * rr_last_kernel_ms. */
-#define RR_PHASE_PREPASS 0  /* k_nan0_fp64: factor 0 on the first 256 samples */
+#define RR_PHASE_PREPASS 0  /* k_nan0_fp64: factor 0 on the first 64 samples */
 #define RR_PHASE_EVAL 1     /* k_alive_fp64 + k_eval_fp64: the FP64 factor columns */
 #define RR_PHASE_QR 2       /* k_ts_* / k_qr_*: the column-pivoted QR and the solve */
 #define RR_PHASE_SCORE 3    /* k_score_fp64 + k_fit_fp64: prediction, SSR, R2, RMSE */

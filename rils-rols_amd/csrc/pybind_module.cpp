@@ -11,6 +11,8 @@
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
 
+#include <chrono>
+
 #include "search.h"
 #include "serve.h"
 #include "tree.h"
@@ -219,6 +221,54 @@ class BatchRunner {
     }
     return py::make_tuple(coefs, fit);
   }
+  // k calls of the prepared batch (plain FP64 form), pipelined: call i + 1 is
+  // submitted (checked, planned, launched) before call i is waited for, so
+  // the host work of one call overlaps the kernels of the previous one.
+  // Returns the last call's (coefs, fit), each call's latency from its submit
+  // to the end of its wait (s), and the wall time of all k calls (s).
+  py::tuple run_pipelined(int k) {
+    if (k < 1) throw std::invalid_argument("k must be at least 1");
+    const int nc = (int)cand_.size() - 1;
+    const int64_t ncoef = (int64_t)cand_.back() + nc;
+    std::vector<double> cbuf[2];
+    std::vector<rr_fit> fbuf[2];
+    for (int i = 0; i < 2; ++i) {
+      cbuf[i].resize((size_t)std::max<int64_t>(ncoef, 1));
+      fbuf[i].resize((size_t)std::max(nc, 1));
+    }
+    rr_batch b{nc, cand_.data(), prog_.data(), code_.data(), consts_.data(), (int32_t)consts_.size()};
+    std::vector<double> lat((size_t)k);
+    std::vector<std::chrono::steady_clock::time_point> ts((size_t)k);
+    double total = 0;
+    {
+      py::gil_scoped_release nogil;
+      const auto t0 = std::chrono::steady_clock::now();
+      auto since = [](std::chrono::steady_clock::time_point a) {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
+      };
+      for (int i = 0; i < k; ++i) {
+        ts[(size_t)i] = std::chrono::steady_clock::now();
+        check(rr_fitness_submit(ctx_, h_, &b, mode_, cbuf[i % 2].data(), fbuf[i % 2].data()), "rr_fitness_submit");
+        if (i > 0) {
+          check(rr_fitness_wait(ctx_), "rr_fitness_wait");
+          lat[(size_t)i - 1] = since(ts[(size_t)i - 1]);
+        }
+      }
+      check(rr_fitness_wait(ctx_), "rr_fitness_wait");
+      lat[(size_t)k - 1] = since(ts[(size_t)k - 1]);
+      total = since(t0);
+    }
+    const int last = (k - 1) % 2;
+    py::array_t<double> coefs(std::max<int64_t>(ncoef, 1));
+    py::array_t<double> fit({(py::ssize_t)std::max(nc, 1), (py::ssize_t)5});
+    std::copy(cbuf[last].begin(), cbuf[last].end(), (double*)coefs.request().ptr);
+    double* fp = (double*)fit.request().ptr;
+    for (int c = 0; c < nc; ++c) {
+      const rr_fit& f = fbuf[last][(size_t)c];
+      fp[5 * c] = f.e0; fp[5 * c + 1] = f.e1; fp[5 * c + 2] = f.err; fp[5 * c + 3] = f.rank; fp[5 * c + 4] = f.flags;
+    }
+    return py::make_tuple(coefs, fit, lat, total);
+  }
   // Sub-batch of already prepared candidates (same constant pool): used to
   // re-score in FP64 the candidates the FP32 form flags.
   void select(const std::vector<int64_t>& idx) {
@@ -369,6 +419,7 @@ PYBIND11_MODULE(rils_rols_cpp, m) {
       .def("prepare_factors", &BatchRunner::prepare_factors)
       .def("prepare_raw", &BatchRunner::prepare_raw)
       .def("run", &BatchRunner::run, py::arg("precision") = "fp64")
+      .def("run_pipelined", &BatchRunner::run_pipelined, py::arg("k"))
       .def("select", &BatchRunner::select)
       .def("run_selected", &BatchRunner::run_selected, py::arg("precision") = "fp64")
       .def("last_kernel_ms", &BatchRunner::last_kernel_ms)

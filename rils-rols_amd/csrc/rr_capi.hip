@@ -60,6 +60,17 @@ struct Arena {
 };
 thread_local Arena* t_arena = nullptr;
 
+// Set while a call is planned behind a pending pipelined call
+// (rr_fitness_submit): that call's kernels and copies may still read the
+// context's buffers, so a buffer that has to grow first waits for them.
+thread_local hipStream_t t_busy_stream = nullptr;
+void wait_if_busy() {
+  if (t_busy_stream) {
+    (void)hipStreamSynchronize(t_busy_stream);
+    t_busy_stream = nullptr;
+  }
+}
+
 struct DevBuf {
   void* p = nullptr;
   size_t cap = 0;
@@ -78,7 +89,7 @@ struct DevBuf {
       t_arena->used = off + bytes;
       return RR_OK;
     }
-    if (p) { (void)hipFree(p); p = nullptr; cap = 0; }
+    if (p) { wait_if_busy(); (void)hipFree(p); p = nullptr; cap = 0; }
     size_t want = std::max<size_t>(bytes + bytes / 2, 4096);
     hipError_t e = hipMalloc(&p, want);
     if (e != hipSuccess) {
@@ -105,7 +116,7 @@ struct HostBuf {
   size_t cap = 0;
   int ensure(size_t bytes) {
     if (bytes <= cap) return RR_OK;
-    if (p) { (void)hipHostFree(p); p = nullptr; cap = 0; }
+    if (p) { wait_if_busy(); (void)hipHostFree(p); p = nullptr; cap = 0; }
     size_t want = std::max<size_t>(bytes + bytes / 2, 4096);
     if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess) { p = nullptr; return fail(RR_E_NOMEM, "hipHostMalloc failed"); }
     cap = want;
@@ -191,9 +202,28 @@ struct rr_ctx {
   double phase_ms[RR_PHASES] = {0, 0, 0, 0, 0, 0, 0, 0};
   std::vector<FacSlot> fac_table;   // kept across calls: never cleared, only re-stamped
   int32_t fac_gen = 0;
+  // pipelined calls (rr_fitness_submit / rr_fitness_wait): at most two in
+  // flight, completed in submission order; each has its own pinned result
+  // buffer and events.  h2d_done marks the last submitted call's staged
+  // copy: the staging buffer is not rewritten before it has completed.
+  struct Pending {
+    rr_fit* fit_out = nullptr;
+    double* coefs_out = nullptr;
+    size_t fb = 0, cb = 0;
+    bool ran = false;   // completed inside the submit (a form or option that runs synchronously)
+    double ms = 0.0;
+  };
+  Pending pend[2];
+  int n_pend = 0, pend_head = 0;
+  HostBuf h_res[2];
+  hipEvent_t pev0[2] = {}, pev1[2] = {}, pdone[2] = {};
+  hipEvent_t h2d_done = nullptr;
+  bool h2d_pending = false;
 };
 
 namespace {
+
+int drain(rr_ctx* ctx);   // completes the pending pipelined calls (below)
 
 int set_device(rr_ctx* ctx) {
   if (!ctx) return fail(RR_E_ARG, "null context");
@@ -633,6 +663,10 @@ int rr_create(int device, rr_ctx** out) {
   for (int i = 0; ok && i < rr::FP32_BUCKETS - 1; ++i)
     ok = hipStreamCreateWithFlags(&c->side[i], hipStreamNonBlocking) == hipSuccess &&
          hipEventCreateWithFlags(&c->join[i], hipEventDisableTiming) == hipSuccess;
+  for (int i = 0; ok && i < 2; ++i)
+    ok = hipEventCreate(&c->pev0[i]) == hipSuccess && hipEventCreate(&c->pev1[i]) == hipSuccess &&
+         hipEventCreateWithFlags(&c->pdone[i], hipEventDisableTiming) == hipSuccess;
+  ok = ok && hipEventCreateWithFlags(&c->h2d_done, hipEventDisableTiming) == hipSuccess;
   if (!ok) {
     delete c;
     return fail(RR_E_HIP, "stream/event creation failed");
@@ -645,6 +679,7 @@ int rr_destroy(rr_ctx* ctx) {
   if (!ctx) return RR_OK;
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
+  ctx->n_pend = 0;   // pending pipelined calls are dropped: their outputs are not written
   for (Dataset& d : ctx->ds) {
     if (!d.live) continue;
     (void)hipFree(d.X); (void)hipFree(d.y); (void)hipFree(d.X32); (void)hipFree(d.one); (void)hipFree(d.xnorm);
@@ -660,7 +695,14 @@ int rr_destroy(rr_ctx* ctx) {
   for (DevBuf* b : bufs) b->release();
   ctx->h_stage.release();
   ctx->h_out.release();
+  for (HostBuf& h : ctx->h_res) h.release();
   ctx->d_stage.release();
+  for (int i = 0; i < 2; ++i) {
+    (void)hipEventDestroy(ctx->pev0[i]);
+    (void)hipEventDestroy(ctx->pev1[i]);
+    (void)hipEventDestroy(ctx->pdone[i]);
+  }
+  (void)hipEventDestroy(ctx->h2d_done);
   for (rr_ctx::GraphSlot& g : ctx->graphs) if (g.exec) (void)hipGraphExecDestroy(g.exec);
   ctx->graphs.clear();
   for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
@@ -684,6 +726,7 @@ int rr_destroy(rr_ctx* ctx) {
 int rr_dataset_upload(rr_ctx* ctx, const double* X_fm, const double* y, int64_t n, int32_t d, int32_t* handle) {
   int rc = set_device(ctx);
   if (rc) return rc;
+  if ((rc = drain(ctx))) return rc;
   if (!handle || !y || n <= 0 || d < 0 || (d > 0 && !X_fm)) return fail(RR_E_ARG, "bad dataset arguments");
   if (n >= ((int64_t)1 << 40)) return fail(RR_E_ARG, "too many samples");
   Dataset D;
@@ -731,6 +774,7 @@ int rr_dataset_upload(rr_ctx* ctx, const double* X_fm, const double* y, int64_t 
 int rr_dataset_free(rr_ctx* ctx, int32_t handle) {
   int rc = set_device(ctx);
   if (rc) return rc;
+  if ((rc = drain(ctx))) return rc;
   Dataset* D = get_ds(ctx, handle);
   if (!D) return fail(RR_E_ARG, "bad dataset handle");
   RR_HIP(hipStreamSynchronize(ctx->stream));
@@ -828,10 +872,57 @@ int run_graph(rr_ctx* ctx, F issue) {
   return RR_OK;
 }
 
+// Completes pipelined call `slot` if it has not completed yet: waits for its
+// copies and writes its results to the caller's arrays.
+int complete_slot(rr_ctx* ctx, int slot) {
+  rr_ctx::Pending& P = ctx->pend[slot];
+  if (P.ran) return RR_OK;
+  P.ran = true;
+  RR_HIP(hipEventSynchronize(ctx->pdone[slot]));
+  const char* ho = (const char*)ctx->h_res[slot].p;
+  memcpy(P.fit_out, ho, P.fb);
+  if (P.cb) memcpy(P.coefs_out, ho + ((P.fb + 255) & ~size_t(255)), P.cb);
+  float ms = 0.f;
+  RR_HIP(hipEventElapsedTime(&ms, ctx->pev0[slot], ctx->pev1[slot]));
+  P.ms = ms;
+  return RR_OK;
+}
+
+// rr_fitness_wait: completes the oldest pending call and retires it.
+int wait_oldest(rr_ctx* ctx) {
+  if (ctx->n_pend == 0) return fail(RR_E_ARG, "no pipelined call is pending");
+  const int slot = ctx->pend_head;
+  const int rc = complete_slot(ctx, slot);
+  ctx->pend_head = (ctx->pend_head + 1) % 2;
+  --ctx->n_pend;
+  ctx->last_ms = ctx->pend[slot].ms;
+  return rc;
+}
+
+// Completes every pending pipelined call, before any other use of the
+// context's stream, buffers or datasets; the calls stay queued until their
+// rr_fitness_wait, which then returns at once.
+int drain(rr_ctx* ctx) {
+  int rc = RR_OK;
+  for (int i = 0; i < ctx->n_pend; ++i) {
+    const int r = complete_slot(ctx, (ctx->pend_head + i) % 2);
+    if (r && !rc) rc = r;
+  }
+  ctx->h2d_pending = false;
+  return rc;
+}
+
+// async: a pipelined call (rr_fitness_submit; plain FP64 form, no phase
+// timing, no graph): launched behind any pending call and left pending, its
+// results written by wait_oldest.  Otherwise pending calls complete first.
 int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_batch* b, int32_t mode,
-                 int32_t precision, double* coefs_out, rr_fit* fit_out) {
+                 int32_t precision, double* coefs_out, rr_fit* fit_out, bool async = false) {
   int rc = set_device(ctx);
   if (rc) return rc;
+  if (!async && ctx->n_pend > 0 && (rc = drain(ctx))) return rc;
+  struct BusyGuard { ~BusyGuard() { t_busy_stream = nullptr; } } busy_guard;
+  if (async && ctx->n_pend > 0) t_busy_stream = ctx->stream;
+  const int slot = (ctx->pend_head + ctx->n_pend) % 2;   // async: the call's result slot
   if (mode != RR_MODE_OLS && mode != RR_MODE_RAW) return fail(RR_E_ARG, "bad mode");
   if (precision != RR_FP64 && precision != RR_FP32_REFINED && precision != RR_FP32_CHECKED)
     return fail(RR_E_ARG, "bad precision");
@@ -964,6 +1055,10 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
                        2 * align_up((size_t)(np + 1) * 8) +   // kept-column offsets per program and per unique factor
                        2 * align_up((size_t)(np + 1) * 4) +   // the users of each unique factor
                        align_up((size_t)nc * 8);              // chunk-factor offsets of the chunked QR
+  if (ctx->h2d_pending) {   // the pending call's staged copy still reads the staging buffer
+    RR_HIP(hipEventSynchronize(ctx->h2d_done));
+    ctx->h2d_pending = false;
+  }
   if (!t_arena && ctx->h_stage.ensure(s_cand + s_prog + s_code + s_K + s_off + s_tab + s_cds + s_chunks + 2 * s_f64) != RR_OK)
     return RR_E_NOMEM;
   size_t so = 0;
@@ -1496,6 +1591,11 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
   ctx->ev_used = 0;
   const clk::time_point t_launch = clk::now();
   if ((rc = flush_uploads(ctx))) return rc;
+  if (async) {   // every input of the plain FP64 form is staged by now
+    RR_HIP(hipEventRecord(ctx->h2d_done, ctx->stream));
+    ctx->h2d_pending = true;
+  }
+  hipEvent_t const ev0 = async ? ctx->pev0[slot] : ctx->ev0, ev1 = async ? ctx->pev1[slot] : ctx->ev1;
   auto prepass = [&]() -> int {
     if (mode == RR_MODE_OLS && nc > 0) {
       // candidates whose factor 0 is NaN at an early row >= 1: outcome written here, skipped below
@@ -1521,7 +1621,7 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
   if (graph) {   // records ev0 and ev1 around the graph's launch
     if ((rc = run_graph(ctx, plain64))) return rc;
   } else if (precision == RR_FP64) {
-    RR_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    RR_HIP(hipEventRecord(ev0, ctx->stream));
     if ((rc = plain64())) return rc;
   } else if (!checked) {
     RR_HIP(hipEventRecord(ctx->ev0, ctx->stream));
@@ -1550,11 +1650,25 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
     rescored = list64;
     rescored.insert(rescored.end(), again.begin(), again.end());
   }
-  if (!graph) RR_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+  if (!graph) RR_HIP(hipEventRecord(ev1, ctx->stream));
 
   // results: through the pinned buffer h_out (no-alloc calls: straight into
-  // the caller's arrays, no host allocation)
+  // the caller's arrays, no host allocation); a pipelined call's through its
+  // slot's buffer, completed by wait_oldest
   const size_t fb = (size_t)nc * sizeof(rr_fit), cb = mode == RR_MODE_OLS ? (size_t)ncoef * 8 : 0;
+  if (async) {
+    if (ctx->h_res[slot].ensure(align_up(fb) + cb) != RR_OK) return RR_E_NOMEM;
+    char* ho = (char*)ctx->h_res[slot].p;
+    RR_HIP(hipMemcpyAsync(ho, dfit, fb, hipMemcpyDeviceToHost, ctx->stream));
+    if (cb) RR_HIP(hipMemcpyAsync(ho + align_up(fb), dcoef, cb, hipMemcpyDeviceToHost, ctx->stream));
+    RR_HIP(hipEventRecord(ctx->pdone[slot], ctx->stream));
+    rr_ctx::Pending& P = ctx->pend[slot];
+    P.fit_out = fit_out; P.coefs_out = coefs_out; P.fb = fb; P.cb = cb; P.ran = false; P.ms = 0.0;
+    ++ctx->n_pend;
+    ctx->phase_ms[RR_PHASE_HOST_CHECK] = ms_since(t_enter, t_checked);
+    ctx->phase_ms[RR_PHASE_HOST_PLAN] = ms_since(t_checked, t_launch);
+    return RR_OK;
+  }
   if (!t_arena && ctx->h_out.ensure(align_up(fb) + cb) == RR_OK) {
     char* ho = (char*)ctx->h_out.p;
     RR_HIP(hipMemcpyAsync(ho, dfit, fb, hipMemcpyDeviceToHost, ctx->stream));
@@ -1589,6 +1703,29 @@ int rr_fitness_batch(rr_ctx* ctx, int32_t handle, const rr_batch* b, int32_t mod
   return fitness_impl(ctx, handle, nullptr, b, mode, precision, coefs_out, fit_out);
 }
 
+int rr_fitness_submit(rr_ctx* ctx, int32_t handle, const rr_batch* b, int32_t mode, double* coefs_out,
+                      rr_fit* fit_out) {
+  if (!ctx) return fail(RR_E_ARG, "null context");
+  if (ctx->n_pend >= 2) return fail(RR_E_ARG, "two calls are pending: complete one with rr_fitness_wait first");
+  const bool async = !ctx->phase_timing && !ctx->use_graph && b && b->n_cands > 0;
+  if (async) return fitness_impl(ctx, handle, nullptr, b, mode, RR_FP64, coefs_out, fit_out, true);
+  // a call that runs synchronously (empty batch, phase timing or graph replay on): its
+  // results are written now, and its wait completes at once
+  if (int rc = fitness_impl(ctx, handle, nullptr, b, mode, RR_FP64, coefs_out, fit_out)) return rc;
+  const int slot = (ctx->pend_head + ctx->n_pend) % 2;
+  ctx->pend[slot] = rr_ctx::Pending{};
+  ctx->pend[slot].ran = true;
+  ctx->pend[slot].ms = ctx->last_ms;
+  ++ctx->n_pend;
+  return RR_OK;
+}
+
+int rr_fitness_wait(rr_ctx* ctx) {
+  if (!ctx) return fail(RR_E_ARG, "null context");
+  (void)hipSetDevice(ctx->device);
+  return wait_oldest(ctx);
+}
+
 int rr_fitness_multi(rr_ctx* ctx, const int32_t* handles, const rr_batch* b, int32_t mode, int32_t precision,
                      double* coefs_out, rr_fit* fit_out) {
   if (b && b->n_cands > 0 && !handles) return fail(RR_E_ARG, "null dataset handles");
@@ -1604,6 +1741,10 @@ int rr_fitness_multi(rr_ctx* ctx, const int32_t* handles, const rr_batch* b, int
 int rr_fitness_batch_noalloc(rr_ctx* ctx, int32_t handle, const rr_batch* b, int32_t mode, double* coefs_out,
                              rr_fit* fit_out) {
   if (!ctx) return fail(RR_E_ARG, "null context");
+  if (ctx->n_pend > 0) {
+    (void)hipSetDevice(ctx->device);
+    if (int rc = drain(ctx)) return rc;
+  }
   if (!ctx->scratch) return fail(RR_E_HIP, "no scratch array address for this device");
   // The call runs with every work buffer carved from the scratch array; the
   // context's own buffers are set aside and put back afterwards, untouched.
@@ -1642,6 +1783,7 @@ int rr_predict(rr_ctx* ctx, int32_t handle, const uint32_t* code, int32_t len, c
                double* out) {
   int rc = set_device(ctx);
   if (rc) return rc;
+  if ((rc = drain(ctx))) return rc;
   Dataset* D = get_ds(ctx, handle);
   if (!D) return fail(RR_E_ARG, "bad dataset handle");
   if (!code || !out || n_consts < 0 || (n_consts > 0 && !consts)) return fail(RR_E_ARG, "bad predict arguments");
@@ -1671,6 +1813,7 @@ int rr_predict_fp32(rr_ctx* ctx, int32_t handle, const uint32_t* code, int32_t l
                     int32_t n_consts, float* out_v, float* out_e) {
   int rc = set_device(ctx);
   if (rc) return rc;
+  if ((rc = drain(ctx))) return rc;
   Dataset* D = get_ds(ctx, handle);
   if (!D) return fail(RR_E_ARG, "bad dataset handle");
   if (!code || !out_v || !out_e || n_consts < 0 || (n_consts > 0 && !consts)) return fail(RR_E_ARG, "bad predict arguments");

@@ -77,6 +77,8 @@ def lib():
         L.rr_last_phase_ms.argtypes = [vp, ctypes.c_int32, dp]
         L.rr_fitness_batch_noalloc.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(Batch), ctypes.c_int32, dp, vp]
         L.rr_check_batch.argtypes = [ctypes.POINTER(Batch), ctypes.c_int32, ctypes.c_int32]
+        L.rr_fitness_submit.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(Batch), ctypes.c_int32, dp, vp]
+        L.rr_fitness_wait.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -308,6 +310,20 @@ class Context:
         check(lib().rr_fitness_batch_noalloc(self.p, h, ctypes.byref(_as_batch(arrs)), m, _dp(coefs),
                                              fit.ctypes.data_as(ctypes.c_void_p)))
         return self._split(arrs, coefs, fit)
+
+    def submit(self, h, arrs, mode="ols"):
+        """rr_fitness_submit (plain FP64, pipelined): returns a handle whose
+        result() is valid after the matching wait()."""
+        m = D["RR_MODE_OLS"] if mode == "ols" else D["RR_MODE_RAW"]
+        nc = arrs["n_cands"]
+        coefs = np.zeros(max(int(arrs["cand_start"][-1]) + nc, 1))
+        fit = np.zeros(max(nc, 1), dtype=FIT_DTYPE)
+        check(lib().rr_fitness_submit(self.p, h, ctypes.byref(_as_batch(arrs)), m, _dp(coefs),
+                                      fit.ctypes.data_as(ctypes.c_void_p)))
+        return lambda: self._split(arrs, coefs, fit)
+
+    def wait(self):
+        check(lib().rr_fitness_wait(self.p))
 
     def _split(self, arrs, coefs, fit):
         nc = arrs["n_cands"]

@@ -424,3 +424,59 @@ def test_graph_replays_same_bits(ctx):
     for tu, c, f in zip(tuned_a, *eager["a"]):            # and the eager bits are the oracle's
         assert same_bits(c, tu["coefs"]) and same_bits([f["e0"], f["e1"]], [tu["fit"][0], tu["fit"][1]])
     ctx.free(h)
+
+
+def test_pipelined_calls_same_bits(ctx):
+    """rr_fitness_submit / rr_fitness_wait: OLS and RAW batches of different
+    sizes, two in flight at a time, give rr_fitness_batch's bits; a third
+    submit while two are pending is refused; a synchronous call completes the
+    pending ones first; with phase timing on, submit runs the call at once."""
+    print("case: pipelined calls n=20000 d=4", flush=True)
+    X, y = make_data(91, 20000, 4)
+    trees = treegen.random_trees(91, 300, 3, 40, 4)
+    od = O.Dataset(X, y)
+    facs = [od.tune(t)["factors"] for t in trees]
+    h = ctx.upload(X, y)
+    big = capi.build_batch(facs)
+    small = capi.build_batch(facs[::3][:70])
+    raw = capi.build_batch([[t] for t in trees[:100]])
+    want = {"big": ctx.fitness(h, big), "small": ctx.fitness(h, small), "raw": ctx.fitness(h, raw, mode="raw")}
+
+    def assert_same(got, exp):
+        (cg, fg), (cw, fw) = got, exp
+        assert len(cg) == len(cw)
+        for a, b in zip(cg, cw):
+            assert same_bits(a, b)
+        assert fg.tobytes() == fw.tobytes()
+
+    seq = [("big", big, "ols"), ("small", small, "ols"), ("raw", raw, "raw"), ("big", big, "ols"),
+           ("small", small, "ols")]
+    pending = []
+    for name, arrs, mode in seq:
+        pending.append((ctx.submit(h, arrs, mode), name))
+        if len(pending) == 2:
+            ctx.wait()
+            r, name0 = pending.pop(0)
+            assert_same(r(), want[name0])
+    ctx.wait()
+    r, name0 = pending.pop(0)
+    assert_same(r(), want[name0])
+    a = ctx.submit(h, small)
+    b = ctx.submit(h, big)
+    with pytest.raises(RuntimeError, match="two calls are pending"):
+        ctx.submit(h, small)
+    assert_same(ctx.fitness(h, small), want["small"])   # completes both pending calls first
+    assert_same(a(), want["small"])
+    assert_same(b(), want["big"])
+    ctx.wait()                                           # each submit is still retired by its wait
+    ctx.wait()
+    with pytest.raises(RuntimeError, match="no pipelined call"):
+        ctx.wait()
+    ctx.set_option("phase_timing", 1)
+    try:
+        r = ctx.submit(h, big)
+        assert_same(r(), want["big"])                    # written by the submit itself
+        ctx.wait()
+    finally:
+        ctx.set_option("phase_timing", 0)
+    ctx.free(h)
