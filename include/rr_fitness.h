@@ -208,6 +208,10 @@ int rr_fitness_multi(rr_ctx* ctx, const int32_t* handles, const rr_batch* batch,
  * whose fitness calls are synchronous member functions. */
 int rr_fitness_submit(rr_ctx* ctx, int32_t handle, const rr_batch* batch, int32_t mode, double* coefs_out,
                       rr_fit* fit_out);
+/* The same for candidates of several runs (as rr_fitness_multi): the serving
+ * loop submits the next merged launch while the current one runs. */
+int rr_fitness_submit_multi(rr_ctx* ctx, const int32_t* handles, const rr_batch* batch, int32_t mode,
+                            double* coefs_out, rr_fit* fit_out);
 int rr_fitness_wait(rr_ctx* ctx);
 
 /* The plain FP64 form of rr_fitness_batch (RR_MODE_OLS or RR_MODE_RAW, one

@@ -1694,6 +1694,23 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
   return RR_OK;
 }
 
+int submit_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_batch* b, int32_t mode,
+                double* coefs_out, rr_fit* fit_out) {
+  if (!ctx) return fail(RR_E_ARG, "null context");
+  if (ctx->n_pend >= 2) return fail(RR_E_ARG, "two calls are pending: complete one with rr_fitness_wait first");
+  const bool async = !ctx->phase_timing && !ctx->use_graph && b && b->n_cands > 0;
+  if (async) return fitness_impl(ctx, handle, handles, b, mode, RR_FP64, coefs_out, fit_out, true);
+  // a call that runs synchronously (empty batch, phase timing or graph replay on): its
+  // results are written now, and its wait completes at once
+  if (int rc = fitness_impl(ctx, handle, handles, b, mode, RR_FP64, coefs_out, fit_out)) return rc;
+  const int slot = (ctx->pend_head + ctx->n_pend) % 2;
+  ctx->pend[slot] = rr_ctx::Pending{};
+  ctx->pend[slot].ran = true;
+  ctx->pend[slot].ms = ctx->last_ms;
+  ++ctx->n_pend;
+  return RR_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1705,19 +1722,13 @@ int rr_fitness_batch(rr_ctx* ctx, int32_t handle, const rr_batch* b, int32_t mod
 
 int rr_fitness_submit(rr_ctx* ctx, int32_t handle, const rr_batch* b, int32_t mode, double* coefs_out,
                       rr_fit* fit_out) {
-  if (!ctx) return fail(RR_E_ARG, "null context");
-  if (ctx->n_pend >= 2) return fail(RR_E_ARG, "two calls are pending: complete one with rr_fitness_wait first");
-  const bool async = !ctx->phase_timing && !ctx->use_graph && b && b->n_cands > 0;
-  if (async) return fitness_impl(ctx, handle, nullptr, b, mode, RR_FP64, coefs_out, fit_out, true);
-  // a call that runs synchronously (empty batch, phase timing or graph replay on): its
-  // results are written now, and its wait completes at once
-  if (int rc = fitness_impl(ctx, handle, nullptr, b, mode, RR_FP64, coefs_out, fit_out)) return rc;
-  const int slot = (ctx->pend_head + ctx->n_pend) % 2;
-  ctx->pend[slot] = rr_ctx::Pending{};
-  ctx->pend[slot].ran = true;
-  ctx->pend[slot].ms = ctx->last_ms;
-  ++ctx->n_pend;
-  return RR_OK;
+  return submit_impl(ctx, handle, nullptr, b, mode, coefs_out, fit_out);
+}
+
+int rr_fitness_submit_multi(rr_ctx* ctx, const int32_t* handles, const rr_batch* b, int32_t mode, double* coefs_out,
+                            rr_fit* fit_out) {
+  if (b && b->n_cands > 0 && !handles) return fail(RR_E_ARG, "null dataset handles");
+  return submit_impl(ctx, -1, handles, b, mode, coefs_out, fit_out);
 }
 
 int rr_fitness_wait(rr_ctx* ctx) {

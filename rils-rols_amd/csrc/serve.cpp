@@ -2,6 +2,7 @@
 #include "serve.h"
 
 #include <algorithm>
+#include <memory>
 #include <stdexcept>
 
 namespace rrh {
@@ -101,45 +102,146 @@ Dispatcher::Counters Dispatcher::counters() {
   return ctr_;
 }
 
+// A merged launch: the requests it carries and the arrays it needs until
+// its results are handed out.
+struct Dispatcher::Flight {
+  std::vector<Req*> group;
+  std::vector<int32_t> cs, ps, hs;
+  std::vector<uint32_t> code;
+  std::vector<double> consts, coefs;
+  std::vector<rr_fit> fit;
+  rr_batch mb{};
+  int64_t ncand = 0;
+};
+
+// Merges the group's batches into one launch; false (with every request's
+// error set) when they do not fit one.
+bool Dispatcher::merge(Flight& f) {
+  std::vector<BatchView> parts;
+  for (Req* r : f.group) {
+    parts.push_back({r->b, r->handle});
+    f.ncand += r->b->n_cands;
+  }
+  try {
+    merge_batches(parts, f.cs, f.ps, f.code, f.consts, f.hs);
+  } catch (const std::exception& e) {
+    for (Req* r : f.group) r->error = e.what();
+    return false;
+  }
+  f.mb.n_cands = (int32_t)f.ncand;
+  f.mb.cand_start = f.cs.data();
+  f.mb.prog_start = f.ps.data();
+  f.mb.code = f.code.data();
+  f.mb.consts = f.consts.data();
+  f.mb.n_consts = (int32_t)f.consts.size();
+  f.coefs.assign((size_t)f.cs.back() + (size_t)f.ncand + 1, 0.0);
+  f.fit.resize((size_t)std::max<int64_t>(f.ncand, 1));
+  return true;
+}
+
+// Hands a finished launch's results (or its error) to its requests and wakes them.
+void Dispatcher::finish(Flight& f, int rc, double ms) {
+  if (rc != RR_OK) {
+    const std::string e = rr_last_error();
+    for (Req* r : f.group) if (r->error.empty()) r->error = e;
+  } else {
+    int64_t cbase = 0;
+    for (Req* r : f.group) {
+      const rr_batch& b = *r->b;
+      for (int32_t c = 0; c < b.n_cands; ++c) {
+        r->fit[c] = f.fit[(size_t)(cbase + c)];
+        if (r->coefs) {
+          const int64_t src = (int64_t)f.cs[(size_t)(cbase + c)] + cbase + c;
+          const int64_t len = (int64_t)(b.cand_start[c + 1] - b.cand_start[c]) + 1;
+          std::copy(f.coefs.begin() + src, f.coefs.begin() + src + len, r->coefs + b.cand_start[c] + c);
+        }
+      }
+      r->kernel_ms = ms;
+      cbase += b.n_cands;
+    }
+  }
+  {
+    std::lock_guard<std::mutex> g(m_);
+    if (rc == RR_OK && f.ncand > 0) {
+      ctr_.launches++;
+      ctr_.requests += (int64_t)f.group.size();
+      ctr_.candidates += f.ncand;
+      ctr_.max_requests_per_launch = std::max<int64_t>(ctr_.max_requests_per_launch, (int64_t)f.group.size());
+      ctr_.kernel_ms += ms;
+    }
+    for (Req* r : f.group) r->done = true;
+  }
+  cv_.notify_all();
+}
+
+// Waits for the oldest launch in flight and hands out its results.
+void Dispatcher::retire_oldest() {
+  std::unique_ptr<Flight> f = std::move(flights_.front());
+  flights_.pop_front();
+  const int rc = rr_fitness_wait(ctx_);
+  double ms = 0;
+  rr_last_kernel_ms(ctx_, &ms);
+  finish(*f, rc, ms);
+}
+
+// The plain FP64 launches are pipelined: a merged launch is submitted
+// (checked, planned and queued on the context's stream) while the previous
+// one runs, and only then is the previous one waited for, so the host merge
+// and check of one launch overlap the kernels of the other.  Other forms run
+// one call at a time.
 void Dispatcher::loop() {
   for (;;) {
     std::vector<Req*> batch;
     double gathered = 0;
     {
       std::unique_lock<std::mutex> lk(m_);
-      cv_.wait(lk, [this] { return stop_ || !q_.empty(); });
-      if (q_.empty() && stop_) return;
-      // Continuous batching: a run whose sweep is not pending yet is on its
-      // host work between sweeps; when the pending fitness requests are fewer
-      // than the runs searching, give the others up to gather_us to join this
-      // launch (uploads and releases never wait).
-      auto fitness_pending = [this] {
-        int f = 0;
-        for (const Req* r : q_) {
-          if (r->kind != Req::FITNESS) return -1;   // an upload or release: go now
-          ++f;
+      if (flights_.empty()) cv_.wait(lk, [this] { return stop_ || !q_.empty(); });
+      if (q_.empty() && stop_ && flights_.empty()) return;
+      if (!q_.empty()) {
+        // Continuous batching: a run whose sweep is not pending yet is on its
+        // host work between sweeps; when the pending fitness requests are
+        // fewer than the runs searching (those with a launch in flight
+        // aside), give the others up to gather_us to join this launch
+        // (uploads and releases never wait).
+        auto fitness_pending = [this] {
+          int f = 0;
+          for (const Req* r : q_) {
+            if (r->kind != Req::FITNESS) return -1;   // an upload or release: go now
+            ++f;
+          }
+          return f;
+        };
+        int inflight = 0;
+        for (const std::unique_ptr<Flight>& f : flights_) inflight += (int)f->group.size();
+        const int searching = live_ - inflight;
+        const int f0 = fitness_pending();
+        if (gather_us_ > 0 && f0 > 0 && f0 < searching && !stop_) {
+          const auto t0 = std::chrono::steady_clock::now();
+          cv_.wait_until(lk, t0 + std::chrono::microseconds(gather_us_), [&] {
+            const int f = fitness_pending();
+            return stop_ || f < 0 || f >= searching;
+          });
+          gathered = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         }
-        return f;
-      };
-      const int f0 = fitness_pending();
-      if (gather_us_ > 0 && f0 > 0 && f0 < live_ && !stop_) {
-        const auto t0 = std::chrono::steady_clock::now();
-        cv_.wait_until(lk, t0 + std::chrono::microseconds(gather_us_), [&] {
-          const int f = fitness_pending();
-          return stop_ || f < 0 || f >= live_;
-        });
-        gathered = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        batch.assign(q_.begin(), q_.end());   // everything pending now goes into this round
+        q_.clear();
       }
-      batch.assign(q_.begin(), q_.end());   // everything pending now goes into this round
-      q_.clear();
     }
     const auto t0 = std::chrono::steady_clock::now();
+    if (batch.empty()) {   // nothing new while a launch runs: retire it
+      retire_oldest();
+      continue;
+    }
     int dlive = 0;
-    for (Req* r : batch) {
+    std::vector<Req*> admin;
+    for (Req* r : batch) if (r->kind != Req::FITNESS) admin.push_back(r);
+    if (!admin.empty())
+      while (!flights_.empty()) retire_oldest();   // uploads and releases need the context idle
+    for (Req* r : admin) {
       if (r->kind == Req::UPLOAD) {
         if (rr_dataset_upload(ctx_, r->X, r->y, r->n, r->d, &r->handle) != RR_OK) r->error = rr_last_error();
         else ++dlive;
-      } else if (r->kind == Req::RELEASE) {
+      } else {
         rr_dataset_free(ctx_, r->handle);
         --dlive;
       }
@@ -151,8 +253,29 @@ void Dispatcher::loop() {
       const int32_t mode = fits[0]->mode, prec = fits[0]->precision;
       std::vector<Req*> group, rest;
       for (Req* r : fits) (r->mode == mode && r->precision == prec ? group : rest).push_back(r);
-      run_fitness_group(group);
       fits.swap(rest);
+      std::unique_ptr<Flight> f(new Flight);
+      f->group = std::move(group);
+      if (!merge(*f)) {   // every request of the group already carries the error
+        finish(*f, RR_E_LIMIT, 0.0);
+        continue;
+      }
+      if (prec != RR_FP64) {
+        while (!flights_.empty()) retire_oldest();
+        const int rc = rr_fitness_multi(ctx_, f->hs.data(), &f->mb, mode, prec, f->coefs.data(), f->fit.data());
+        double ms = 0;
+        rr_last_kernel_ms(ctx_, &ms);
+        finish(*f, rc, ms);
+        continue;
+      }
+      if (flights_.size() == 2) retire_oldest();
+      const int rc = rr_fitness_submit_multi(ctx_, f->hs.data(), &f->mb, mode, f->coefs.data(), f->fit.data());
+      if (rc != RR_OK) {
+        finish(*f, rc, 0.0);
+        continue;
+      }
+      flights_.push_back(std::move(f));
+      if (flights_.size() == 2) retire_oldest();   // the one that ran while this one was prepared
     }
     const double busy = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     {
@@ -160,66 +283,10 @@ void Dispatcher::loop() {
       ctr_.busy_s += busy;
       ctr_.gather_s += gathered;
       live_ += dlive;
-      for (Req* r : batch) r->done = true;
+      for (Req* r : admin) r->done = true;
     }
     cv_.notify_all();
   }
-}
-
-void Dispatcher::run_fitness_group(std::vector<Req*>& group) {
-  std::vector<BatchView> parts;
-  int64_t ncand = 0;
-  for (Req* r : group) {
-    parts.push_back({r->b, r->handle});
-    ncand += r->b->n_cands;
-  }
-  if (ncand == 0) return;
-  std::vector<int32_t> cs, ps, hs;
-  std::vector<uint32_t> code;
-  std::vector<double> consts;
-  try {
-    merge_batches(parts, cs, ps, code, consts, hs);
-  } catch (const std::exception& e) {
-    for (Req* r : group) r->error = e.what();
-    return;
-  }
-  rr_batch mb;
-  mb.n_cands = (int32_t)ncand;
-  mb.cand_start = cs.data();
-  mb.prog_start = ps.data();
-  mb.code = code.data();
-  mb.consts = consts.data();
-  mb.n_consts = (int32_t)consts.size();
-  std::vector<double> coefs((size_t)cs.back() + (size_t)ncand + 1);
-  std::vector<rr_fit> fit((size_t)ncand);
-  const int rc = rr_fitness_multi(ctx_, hs.data(), &mb, group[0]->mode, group[0]->precision, coefs.data(), fit.data());
-  double ms = 0;
-  rr_last_kernel_ms(ctx_, &ms);
-  if (rc != RR_OK) {
-    const std::string e = rr_last_error();
-    for (Req* r : group) r->error = e;
-    return;
-  }
-  int64_t cbase = 0;
-  for (Req* r : group) {
-    const rr_batch& b = *r->b;
-    for (int32_t c = 0; c < b.n_cands; ++c) {
-      r->fit[c] = fit[(size_t)(cbase + c)];
-      if (r->coefs) {
-        const int64_t src = (int64_t)cs[(size_t)(cbase + c)] + cbase + c;
-        const int64_t len = (int64_t)(b.cand_start[c + 1] - b.cand_start[c]) + 1;
-        std::copy(coefs.begin() + src, coefs.begin() + src + len, r->coefs + b.cand_start[c] + c);
-      }
-    }
-    r->kernel_ms = ms;
-    cbase += b.n_cands;
-  }
-  std::lock_guard<std::mutex> g(m_);
-  ctr_.launches++;
-  ctr_.requests += (int64_t)group.size();
-  ctr_.candidates += ncand;
-  ctr_.max_requests_per_launch = std::max<int64_t>(ctr_.max_requests_per_launch, (int64_t)group.size());
-  ctr_.kernel_ms += ms;
 }
 
 // Requests from the search threads: queue and block until the round that

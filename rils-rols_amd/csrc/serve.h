@@ -64,8 +64,11 @@ class Dispatcher : public FitnessBackend {
 
  private:
   struct Req;
+  struct Flight;
   void loop();
-  void run_fitness_group(std::vector<Req*>& group);
+  bool merge(Flight& f);
+  void finish(Flight& f, int rc, double ms);
+  void retire_oldest();
   rr_ctx* ctx_ = nullptr;
   std::mutex m_;
   std::condition_variable cv_;
@@ -76,6 +79,7 @@ class Dispatcher : public FitnessBackend {
   int gather_us_ = 0;
   int live_ = 0;   // resident datasets: one per run that is searching
   Counters ctr_;
+  std::deque<std::unique_ptr<Flight>> flights_;   // submitted launches not yet waited for (dispatcher thread only)
   std::thread th_;
 };
 
