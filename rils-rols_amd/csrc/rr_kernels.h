@@ -155,7 +155,7 @@ struct ScoreLaunch {
   const int64_t* prog_kept;  // OLS, per non-bare program
   const int32_t* order;      // launched position -> candidate (null: identity)
   const int32_t* dead;       // or null
-  const int32_t* info;       // OLS: written by the QR (k_qr_init / k_qr_solve)
+  const int32_t* info;       // OLS: written by the QR (k_ts_init / k_ts_solve, or k_qr_init / k_qr_solve)
   const int32_t* pos_item;
   int n_pos;
   int n_items;

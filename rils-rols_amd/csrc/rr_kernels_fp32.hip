@@ -3,11 +3,13 @@
 // On this hardware plain FP32 and FP64 vector arithmetic issue at the same
 // rate; FP32 is cheaper only through the hardware transcendental, reciprocal
 // and square-root instructions and through moving half the bytes.  The form
-// is built around that: one wavefront owns one candidate and makes ONE pass
-// over the samples, evaluating the factor programs in FP32 with the hardware
-// instructions for R rows per lane, while every value carries a running bound
-// e >= |v32 - v64| on its distance from the FP64 form's value of the same
-// expression.  The pass accumulates, per candidate,
+// is built around that: ONE pass over the samples, split into row chunks --
+// k_acc_fp32 gives each (candidate, chunk) a 4-wave workgroup that evaluates
+// the factor programs in FP32 with the hardware instructions for R rows per
+// lane, while every value carries a running bound e >= |v32 - v64| on its
+// distance from the FP64 form's value of the same expression, and k_fin_fp32
+// (one wave per candidate) adds the chunk partials in chunk order.  The pass
+// accumulates, per candidate,
 //     G  = A^T A   (factors; FP64 -- a product of two FP32 numbers is exact)
 //     S1 = column sums, SY = A^T y, Bd = sum_i e_ij^2       (FP64)
 // and then, on one lane:
@@ -36,7 +38,7 @@
 // domain violation that is certain (sqrt or ln of an argument certainly
 // negative) makes the FP64 value NaN at that sample; the row state carries
 // that.  The QR outcome with certain-NaN columns is decided as in the FP64
-// form (see k_ols_fp32).  MIN and MAX can absorb a NaN: under them such a row
+// form (see k_fin_fp32).  MIN and MAX can absorb a NaN: under them such a row
 // becomes uncertain; comparisons give 0 or 1 either way (bound 1).
 //
 // Operator bounds (u = 2^-23, twice the FP32 unit roundoff; T = 2^-125 covers

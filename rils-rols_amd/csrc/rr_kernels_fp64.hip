@@ -972,7 +972,7 @@ __global__ __launch_bounds__(NAN0_ROWS) void k_nan0_fp64(Nan0Args a) {
   if (tid == 0) {
     a.dead[cand] = found;
     if (found) {
-      // the outcome k_qr_init writes when it sees the same NaN
+      // the outcome k_ts_init / k_qr_init write when they see the same NaN
       const int k = nf + 1;
       const int size0 = (int)((n < (int64_t)k) ? n : (int64_t)k);
       double* co = a.coefs + (int64_t)pbeg + cand;
