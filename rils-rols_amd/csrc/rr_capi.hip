@@ -212,6 +212,7 @@ struct rr_ctx {
     size_t fb = 0, cb = 0;
     bool ran = false;   // completed inside the submit (a form or option that runs synchronously)
     double ms = 0.0;
+    std::vector<int64_t> dup_copy;   // candidates that take another's results (copy_duplicates)
   };
   Pending pend[2];
   int n_pend = 0, pend_head = 0;
@@ -872,6 +873,37 @@ int run_graph(rr_ctx* ctx, F issue) {
   return RR_OK;
 }
 
+// Two programs of a batch are the same factor: same opcodes and operands,
+// constants compared by value (whatever their pool index).
+bool same_program(const rr_batch* b, int32_t p1, int32_t p2) {
+  const int32_t l1 = b->prog_start[p1 + 1] - b->prog_start[p1], l2 = b->prog_start[p2 + 1] - b->prog_start[p2];
+  if (l1 != l2) return false;
+  for (int32_t i = 0; i < l1; ++i) {
+    const uint32_t a1 = b->code[b->prog_start[p1] + i], a2 = b->code[b->prog_start[p2] + i];
+    const uint32_t op = a1 & 0xffu;
+    if (op != (a2 & 0xffu)) return false;
+    const bool kop = op == RR_OP_LDC ||
+                     (op >= RR_OP_BIN && (((op - RR_OP_BIN) & 7u) == RR_M_C || ((op - RR_OP_BIN) & 7u) == RR_M_CR));
+    if (kop) {
+      if (memcmp(&b->consts[a1 >> 8], &b->consts[a2 >> 8], 8) != 0) return false;
+    } else if (a1 != a2) {
+      return false;
+    }
+  }
+  return true;
+}
+
+// Candidates of a call whose factor programs equal another's were not
+// launched: they take that candidate's results.  dup_copy holds, per such
+// candidate, 5 words: the candidate, the one it copies, their first
+// coefficient offsets, and the coefficient count (the same for both).
+void copy_duplicates(const std::vector<int64_t>& dup_copy, rr_fit* fit_out, double* coefs_out) {
+  for (size_t i = 0; i + 4 < dup_copy.size(); i += 5) {
+    fit_out[dup_copy[i]] = fit_out[dup_copy[i + 1]];
+    if (coefs_out) memcpy(coefs_out + dup_copy[i + 2], coefs_out + dup_copy[i + 3], (size_t)dup_copy[i + 4] * 8);
+  }
+}
+
 // Completes pipelined call `slot` if it has not completed yet: waits for its
 // copies and writes its results to the caller's arrays.
 int complete_slot(rr_ctx* ctx, int slot) {
@@ -882,6 +914,7 @@ int complete_slot(rr_ctx* ctx, int slot) {
   const char* ho = (const char*)ctx->h_res[slot].p;
   memcpy(P.fit_out, ho, P.fb);
   if (P.cb) memcpy(P.coefs_out, ho + ((P.fb + 255) & ~size_t(255)), P.cb);
+  copy_duplicates(P.dup_copy, P.fit_out, P.cb ? P.coefs_out : nullptr);
   float ms = 0.f;
   RR_HIP(hipEventElapsedTime(&ms, ctx->pev0[slot], ctx->pev1[slot]));
   P.ms = ms;
@@ -1563,7 +1596,44 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
 
   std::vector<int32_t> rescored;   // candidates whose result came from the FP64 form (checked mode)
   std::vector<Chunk64> plan;
-  if (mode == RR_MODE_OLS && precision == RR_FP64) { if ((rc = plan64(nullptr, 0, plan))) return rc; }
+  // Candidates with the same factor programs in the same order over the same
+  // dataset have the same coefficients and fitness (a local-search sweep
+  // holds many: perturbations that only change constants the OLS absorbs).
+  // Only the first of each is launched; the others copy its results.
+  std::vector<int64_t> dup_copy;
+  if (mode == RR_MODE_OLS && precision == RR_FP64) {
+    std::vector<int32_t> reps;
+    size_t tsz = 64;
+    while (tsz < (size_t)2 * (size_t)nc) tsz <<= 1;
+    std::vector<int32_t> slot_of(tsz, -1);
+    std::vector<uint64_t> ch((size_t)nc);
+    for (int32_t c = 0; c < nc; ++c) {
+      uint64_t h = 0x9E3779B97F4A7C15ULL * (uint64_t)(uint32_t)((handles ? cand_ds[c] : 0) + 1);
+      for (int32_t p = b->cand_start[c]; p < b->cand_start[c + 1]; ++p)
+        h = (h ^ phash[p]) * 0xff51afd7ed558ccdULL + (uint64_t)(p - b->cand_start[c]);
+      ch[(size_t)c] = h ^ (h >> 29);
+      int32_t same = -1;
+      size_t t = ch[(size_t)c] & (tsz - 1);
+      for (; slot_of[t] >= 0; t = (t + 1) & (tsz - 1)) {
+        const int32_t r = slot_of[t];
+        if (ch[(size_t)r] != ch[(size_t)c]) continue;
+        const int32_t nr = b->cand_start[r + 1] - b->cand_start[r], ncn = b->cand_start[c + 1] - b->cand_start[c];
+        if (nr != ncn || (handles && cand_ds[r] != cand_ds[c])) continue;
+        bool eq = true;
+        for (int32_t i = 0; i < ncn && eq; ++i) eq = same_program(b, b->cand_start[r] + i, b->cand_start[c] + i);
+        if (eq) { same = r; break; }
+      }
+      if (same >= 0) {
+        dup_copy.insert(dup_copy.end(), {(int64_t)c, (int64_t)same, (int64_t)b->cand_start[c] + c,
+                                         (int64_t)b->cand_start[same] + same,
+                                         (int64_t)(b->cand_start[c + 1] - b->cand_start[c]) + 1});
+      } else {
+        slot_of[t] = c;
+        reps.push_back(c);
+      }
+    }
+    if ((rc = plan64(dup_copy.empty() ? nullptr : &reps, 0, plan))) return rc;
+  }
   rr::ScoreLaunch raw64{};   // RAW mode, plain FP64: every candidate scored in chunks
   if (mode == RR_MODE_RAW && precision == RR_FP64) {
     std::vector<int32_t> pos_item(nc + 1);
@@ -1664,6 +1734,7 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
     RR_HIP(hipEventRecord(ctx->pdone[slot], ctx->stream));
     rr_ctx::Pending& P = ctx->pend[slot];
     P.fit_out = fit_out; P.coefs_out = coefs_out; P.fb = fb; P.cb = cb; P.ran = false; P.ms = 0.0;
+    P.dup_copy.swap(dup_copy);
     ++ctx->n_pend;
     ctx->phase_ms[RR_PHASE_HOST_CHECK] = ms_since(t_enter, t_checked);
     ctx->phase_ms[RR_PHASE_HOST_PLAN] = ms_since(t_checked, t_launch);
@@ -1681,6 +1752,7 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
     if (cb) RR_HIP(hipMemcpyAsync(coefs_out, dcoef, cb, hipMemcpyDeviceToHost, ctx->stream));
     RR_HIP(hipStreamSynchronize(ctx->stream));
   }
+  copy_duplicates(dup_copy, fit_out, cb ? coefs_out : nullptr);
   for (int32_t c : rescored) fit_out[c].flags |= RR_F_FP64;
   float ms = 0.f;
   RR_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));

@@ -480,3 +480,32 @@ def test_pipelined_calls_same_bits(ctx):
     finally:
         ctx.set_option("phase_timing", 0)
     ctx.free(h)
+
+
+def test_repeated_candidates_take_the_same_results(ctx):
+    """Candidates whose factor programs are the same (constants compared by
+    value) are launched once and the others take the results: every result is
+    still the oracle's, synchronously and pipelined; a constant one bit apart
+    is a different factor."""
+    print("case: repeated candidates n=9000 d=3", flush=True)
+    X, y = make_data(77, 9000, 3)
+    trees = ["+ sin x0 * x1 x2", "exp * x0 c0x1p-2", "+ + sin x0 * x1 x2 c0x1.8p+1", "+ sin x0 * x1 x2",
+             "exp * x0 c0x1.0000000000001p-2", "* c0x1.4p+1 exp * x0 c0x1p-2", "ln x0", "ln x0",
+             "+ sqrt x2 x0", "+ x0 sqrt x2", "+ sqrt x2 x0"]
+    od = O.Dataset(X, y)
+    tuned = [od.tune(t) for t in trees]
+    arrs = capi.build_batch([tu["factors"] for tu in tuned])
+    h = ctx.upload(X, y)
+    got = [ctx.fitness(h, arrs)]
+    r = ctx.submit(h, arrs)
+    ctx.wait()
+    got.append(r())
+    for coefs, fit in got:
+        for t, tu, c, f in zip(trees, tuned, coefs, fit):
+            assert same_bits(c, tu["coefs"]), t
+            assert same_bits([f["e0"], f["e1"]], [tu["fit"][0], tu["fit"][1]]), t
+        assert fit[0].tobytes() == fit[2].tobytes() == fit[3].tobytes()
+        assert fit[1].tobytes() == fit[5].tobytes()
+        assert fit[8].tobytes() == fit[10].tobytes()
+    assert tuned[4]["factors"] != tuned[1]["factors"]
+    ctx.free(h)
