@@ -1356,11 +1356,13 @@ __device__ inline double ts_wave_sum(double v) {
 // rows t + THR u, u = 0 .. RPT-1.  Every sum is in the oracle's order for THR
 // threads: per thread its rows in order, groups of 64 by ts_wave_sum, the
 // group sums in order.  FULL: m == TS_ROWS, so only rows 0..j of step j
-// (thread t <= j, u = 0) need a condition.  One barrier per step: the group
-// sums go to LDS (double-buffered by step parity, buf) and every thread adds
-// them up itself.
+// (thread t <= j, u = 0) need a condition.  The group sums go to LDS
+// (double-buffered by step parity, buf); with up to 4 groups every thread adds
+// them up itself (one barrier per step), with more one lane per column of
+// wave 0 adds them and the others read its totals (two barriers).
 template <int NC, int THR, bool FULL>
-__device__ inline void ts_chunk_item(const int64_t* cur, double (*prow)[NC], double (*red)[THR / WAVE][NC], int& buf) {
+__device__ inline void ts_chunk_item(const int64_t* cur, double (*prow)[NC], double (*red)[THR / WAVE][NC],
+                                     double (*totl)[NC], int& buf) {
   constexpr int RPT = (int)(TS_ROWS / THR);
   constexpr int NG = THR / WAVE;
   const int t = (int)threadIdx.x;
@@ -1412,13 +1414,28 @@ __device__ inline void ts_chunk_item(const int64_t* cur, double (*prow)[NC], dou
     }
     __syncthreads();
     double tot[NC];
+    if constexpr (NG > 4) {
+      // many group sums: lane c of wave 0 adds column c's in group order and
+      // every thread reads the totals, instead of every thread adding them all
+      if (wave == 0 && lane >= j && lane < nc) {
+        double acc = red[buf][0][lane];
 #pragma unroll
-    for (int c = j; c < NC; ++c) {
-      if (c >= nc) continue;
-      double acc = red[buf][0][c];
+        for (int g = 1; g < NG; ++g) acc = acc + red[buf][g][lane];
+        totl[buf][lane] = acc;
+      }
+      __syncthreads();
 #pragma unroll
-      for (int g = 1; g < NG; ++g) acc = acc + red[buf][g][c];
-      tot[c] = acc;
+      for (int c = j; c < NC; ++c)
+        if (c < nc) tot[c] = totl[buf][c];
+    } else {
+#pragma unroll
+      for (int c = j; c < NC; ++c) {
+        if (c >= nc) continue;
+        double acc = red[buf][0][c];
+#pragma unroll
+        for (int g = 1; g < NG; ++g) acc = acc + red[buf][g][c];
+        tot[c] = acc;
+      }
     }
     double beta, tau, den;
     ts_householder(prow[buf][j], tot[j], beta, tau, den);
@@ -1460,6 +1477,7 @@ __global__ __launch_bounds__(ts_threads(NC)) void k_ts_chunk(TsArgs a) {
   constexpr bool PREFETCH = NC <= 5;
   __shared__ double prow[2][NC];
   __shared__ double red[2][THR / WAVE][NC];
+  __shared__ double totl[2][NC];
   const int cnt = a.ts_cnt[a.bucket];   // uniform
   const int64_t* const desc = a.ts_desc + a.list_base[a.bucket];
   int buf = 0;   // LDS half of the next step
@@ -1473,8 +1491,8 @@ __global__ __launch_bounds__(ts_threads(NC)) void k_ts_chunk(TsArgs a) {
     int64_t nxt[DW];
 #pragma unroll
     for (int w = 0; w < DW; ++w) nxt[w] = (PREFETCH && next < cnt) ? desc[(int64_t)next * DW + w] : 0;
-    if (cur[1] == TS_ROWS) ts_chunk_item<NC, THR, true>(cur, prow, red, buf);
-    else ts_chunk_item<NC, THR, false>(cur, prow, red, buf);
+    if (cur[1] == TS_ROWS) ts_chunk_item<NC, THR, true>(cur, prow, red, totl, buf);
+    else ts_chunk_item<NC, THR, false>(cur, prow, red, totl, buf);
     if (next >= cnt) break;
     item = next;
 #pragma unroll
@@ -1488,42 +1506,46 @@ constexpr int TS_RW = TS_MAX_COLS * (TS_MAX_COLS + 1);   // words of the largest
 // 1 .. count - 1 (R already holds the one at src).  Each fold replaces R by
 // the triangular factor of [R; T] (T's column j has rows 0..j): per column j
 // a Householder reflector from R(j, j) and T's column j, applied to the
-// columns right of j, lane c carrying column c, its sums in row order as in
-// the oracle.  The next factor is loaded while the current one is folded in.
-// One wave; R and T in LDS (kf rows of NC words).
-__device__ inline void ts_fold_chain(double* R, double* T, const double* src, int64_t stride, int count, int W,
+// columns right of j, sums in row order as in the oracle.  One wave, no
+// memory between the steps: lane c (< nc) holds column c of R (r[i] =
+// R(i, c), i < kf) and of T in registers, and the values of column j that
+// every lane needs are broadcast from lane j by readlane.  The next factor is
+// loaded while the current one is folded in.
+__device__ inline void ts_load_cols(const double* p, int kf, int nc, int NC, int l, double (&dst)[TS_MAX_COLS]) {
+#pragma unroll
+  for (int i = 0; i < TS_MAX_COLS; ++i) dst[i] = (i < kf && l < nc) ? p[i * NC + l] : 0.0;
+}
+__device__ inline void ts_fold_chain(double (&r)[TS_MAX_COLS], const double* src, int64_t stride, int count,
                                      int kf, int nc, int NC, int l) {
-  double pre0 = 0.0, pre1 = 0.0;   // words l and l + 64 of the next factor
-  if (count > 1) {
-    if (l < W) pre0 = src[stride + l];
-    if (l + WAVE < W) pre1 = src[stride + l + WAVE];
-  }
+  double nx[TS_MAX_COLS];
+  if (count > 1) ts_load_cols(src + stride, kf, nc, NC, l, nx);
   for (int f = 1; f < count; ++f) {
-    if (l < W) T[l] = pre0;
-    if (l + WAVE < W) T[l + WAVE] = pre1;
-    if (f + 1 < count) {
-      const double* nx = src + (int64_t)(f + 1) * stride;
-      if (l < W) pre0 = nx[l];
-      if (l + WAVE < W) pre1 = nx[l + WAVE];
-    }
-    __syncthreads();
-    for (int j = 0; j < kf; ++j) {   // [R; T] -> R; T's column j has rows 0..j
+    double t[TS_MAX_COLS];
+#pragma unroll
+    for (int i = 0; i < TS_MAX_COLS; ++i) t[i] = nx[i];
+    if (f + 1 < count) ts_load_cols(src + (int64_t)(f + 1) * stride, kf, nc, NC, l, nx);
+#pragma unroll
+    for (int j = 0; j < TS_MAX_COLS; ++j) {   // [R; T] -> R
+      if (j >= kf) continue;   // uniform
+      double tj[TS_MAX_COLS];   // T's column j, rows 0..j
+#pragma unroll
+      for (int i = 0; i <= j; ++i) tj[i] = ts_readlane(t[i], j);
       double sxx = 0.0;
-      for (int i = 0; i <= j; ++i) sxx = __builtin_fma(T[i * NC + j], T[i * NC + j], sxx);
+#pragma unroll
+      for (int i = 0; i <= j; ++i) sxx = __builtin_fma(tj[i], tj[i], sxx);
       double beta, tau, den;
-      ts_householder(R[j * NC + j], sxx, beta, tau, den);
+      ts_householder(ts_readlane(r[j], j), sxx, beta, tau, den);
       if (tau != 0.0 && l > j && l < nc) {
-        const int c = l;
         double sx = 0.0;
-        for (int i = 0; i <= j; ++i) sx = __builtin_fma(T[i * NC + j], T[i * NC + c], sx);
-        const double w = R[j * NC + c] + sx / den;
+#pragma unroll
+        for (int i = 0; i <= j; ++i) sx = __builtin_fma(tj[i], t[i], sx);
+        const double w = r[j] + sx / den;
         const double g = (tau * w) / den;
-        R[j * NC + c] = R[j * NC + c] - tau * w;
-        for (int i = 0; i <= j; ++i) T[i * NC + c] = __builtin_fma(-T[i * NC + j], g, T[i * NC + c]);
+        r[j] = r[j] - tau * w;
+#pragma unroll
+        for (int i = 0; i <= j; ++i) t[i] = __builtin_fma(-tj[i], g, t[i]);
       }
-      __syncthreads();
-      if (l == 0) R[j * NC + j] = beta;
-      __syncthreads();
+      if (l == j) r[j] = beta;
     }
   }
 }
@@ -1538,18 +1560,19 @@ __global__ __launch_bounds__(WAVE) void k_ts_group(TsArgs a) {
   if (!s.live || first >= s.nch) return;   // uniform
   const int count = min(TS_GROUP, s.nch - first);
   if (count < 2) return;                    // a group of one chunk is its own factor
-  __shared__ double R[TS_RW], T[TS_RW];
   const int l = (int)threadIdx.x;
   const int kf = s.kf, nc = kf + 1;
   const int NC = ts_bucket(s.k);
   const int W = NC * (NC - 1);   // words per chunk factor
   double* const src = a.rpart + a.rp_off[pos] + (int64_t)first * W;
-  if (l < W) R[l] = src[l];
-  if (l + WAVE < W) R[l + WAVE] = src[l + WAVE];
-  ts_fold_chain(R, T, src, W, count, W, kf, nc, NC, l);
-  __syncthreads();
-  if (l < W) src[l] = R[l];
-  if (l + WAVE < W) src[l + WAVE] = R[l + WAVE];
+  double r[TS_MAX_COLS];
+  ts_load_cols(src, kf, nc, NC, l, r);
+  ts_fold_chain(r, src, W, count, kf, nc, NC, l);
+  if (l < nc) {
+#pragma unroll
+    for (int i = 0; i < TS_MAX_COLS; ++i)
+      if (i < kf) src[i * NC + l] = r[i];
+  }
 }
 
 // One wave per position: the group factors (k_ts_group) combined in group
@@ -1560,7 +1583,7 @@ __global__ __launch_bounds__(WAVE) void k_ts_solve(TsArgs a) {
   const int pos = a.pos0 + (int)blockIdx.x;
   const TsState& s = a.st[pos];
   if (!s.live) return;   // uniform
-  __shared__ double R[TS_RW], T[TS_RW];
+  __shared__ double R[TS_RW];
   __shared__ int perm[TS_MAX_COLS], fidx[TS_MAX_COLS];
   __shared__ double nu[TS_MAX_COLS];
   const int l = (int)threadIdx.x;
@@ -1573,9 +1596,16 @@ __global__ __launch_bounds__(WAVE) void k_ts_solve(TsArgs a) {
   const int64_t n = D.n;
   const double* const src = a.rpart + a.rp_off[pos];
   const int groups = (s.nch + TS_GROUP - 1) / TS_GROUP;
-  if (l < W) R[l] = src[l];
-  if (l + WAVE < W) R[l + WAVE] = src[l + WAVE];
-  ts_fold_chain(R, T, src, (int64_t)TS_GROUP * W, groups, W, kf, nc, NC, l);
+  {
+    double r[TS_MAX_COLS];
+    ts_load_cols(src, kf, nc, NC, l, r);
+    ts_fold_chain(r, src, (int64_t)TS_GROUP * W, groups, kf, nc, NC, l);
+    if (l < nc) {
+#pragma unroll
+      for (int i = 0; i < TS_MAX_COLS; ++i)
+        if (i < kf) R[i * NC + l] = r[i];
+    }
+  }
   if (l < k) { perm[l] = l; fidx[l] = -1; nu[l] = s.nd[l]; }
   __syncthreads();
   if (l < kf) fidx[s.fin[l]] = l;
