@@ -11,6 +11,7 @@
 #include <numeric>
 #include <random>
 #include <stdexcept>
+#include <string_view>
 #include <utility>
 
 namespace rrh {
@@ -196,6 +197,7 @@ Workers& RilsRols::workers() {
     }
     workers_ = new Workers(t);
     pools_.assign((size_t)workers_->size(), Pool());
+    gen_.assign((size_t)workers_->size(), GenScratch());
   }
   return *workers_;
 }
@@ -415,9 +417,46 @@ void RilsRols::gen_change(const Flat& o, std::vector<Flat>& c) {   // 332-346
   add_change_binary_to_another(o, sz, c);
 }
 
+// Which strings are the first of their value, in index order: open
+// addressing over precomputed hashes, equal hashes settled by comparing the
+// strings, so the outcome is that of inserting them into a set of strings.
+// The table is at least twice the number of strings, so a probe ends.
+class FirstSeen {
+ public:
+  FirstSeen(const std::vector<uint64_t>& hashes, const std::vector<std::string_view>& keys)
+      : h_(hashes), k_(keys) {
+    size_t cap = 16;
+    while (cap < 2 * hashes.size()) cap <<= 1;
+    slot_.assign(cap, -1);
+    mask_ = cap - 1;
+  }
+  // true when no earlier insert had the string keys[i]
+  bool insert(int32_t i) {
+    for (size_t s = (size_t)h_[(size_t)i] & mask_;; s = (s + 1) & mask_) {
+      const int32_t o = slot_[s];
+      if (o < 0) {
+        slot_[s] = i;
+        return true;
+      }
+      if (h_[(size_t)o] == h_[(size_t)i] && k_[(size_t)o] == k_[(size_t)i]) return false;
+    }
+  }
+
+ private:
+  const std::vector<uint64_t>& h_;
+  const std::vector<std::string_view>& k_;
+  std::vector<int32_t> slot_;
+  size_t mask_ = 0;
+};
+
 // rils_rols_cpp.cpp:348-443.  Candidate strings for the dedup are assembled
 // from the solution's string around the changed site instead of re-printing
-// the whole tree.
+// the whole tree.  Three passes, the two dedups in the reference's order
+// between them: the sites (every node, breadth first) generate their
+// candidates and strings in parallel; the first of each string is built into
+// a whole tree, rewritten and printed in parallel; the first of each
+// rewritten string is returned.  Strings and trees live in per-site and
+// per-thread buffers, so a pass allocates nothing per candidate.
 std::vector<Flat> RilsRols::all_candidates(const Flat& sol, bool local_search) {
   std::vector<int32_t> sz;
   subtree_sizes(sol, sz);
@@ -425,54 +464,103 @@ std::vector<Flat> RilsRols::all_candidates(const Flat& sol, bool local_search) {
   std::vector<int32_t> be;
   append_string(sol, 0, full, &be);
 
-  std::vector<Flat> all;
-  all.reserve(3000);
-  std::unordered_set<std::string> seen;
-  seen.reserve(6000);
-  std::vector<Flat> cands;
-  std::string key, cs;
-  auto get = [&](const Flat& t) {
-    cands.clear();
-    if (local_search) gen_change(t, cands); else gen_perturb(t, cands);
-  };
-  auto site = [&](int p) {   // candidates replacing the subtree at position p
-    const Flat old(sol.begin() + p, sol.begin() + p + sz[p]);
-    get(old);
-    for (const Flat& c : cands) {
-      key.assign(full, 0, be[2 * p]);
-      cs.clear();
-      append_string(c, 0, cs);
-      key += cs;
-      key.append(full, be[2 * p + 1], std::string::npos);
-      if (seen.insert(key).second) {
-        Flat f;
-        f.reserve(sol.size() - sz[p] + c.size());
-        f.insert(f.end(), sol.begin(), sol.begin() + p);
-        f.insert(f.end(), c.begin(), c.end());
-        f.insert(f.end(), sol.begin() + p + sz[p], sol.end());
-        all.push_back(std::move(f));
-      }
-    }
-  };
-  std::vector<int32_t> q{0};
-  for (size_t h = 0; h < q.size(); ++h) {
-    const int p = q[h];
-    if (h == 0) site(0);
+  std::vector<int32_t> sites{0};   // every node, breadth first
+  for (size_t h = 0; h < sites.size(); ++h) {
+    const int p = sites[h];
     const int a = arity(sol[p].t);
-    if (a >= 1) { site(p + 1); q.push_back(p + 1); }
-    if (a >= 2) { const int r = p + 1 + sz[p + 1]; site(r); q.push_back(r); }
+    if (a >= 1) sites.push_back(p + 1);
+    if (a >= 2) sites.push_back(p + 1 + sz[p + 1]);
   }
-  std::vector<std::string> strs(all.size());
-  workers().parallel_for(all.size(), [&](size_t i, int tid) {
-    rewrite_rounds(all[i], true, pools_[(size_t)tid]);
-    strs[i] = to_string(all[i]);
+  struct Site {
+    Flat toks;                      // the replacement subtrees, back to back
+    std::vector<uint32_t> tend;     // end of each in toks
+    std::string keys;               // the whole tree's string with each, back to back
+    std::vector<uint32_t> kend;
+    std::vector<uint64_t> hash;
+  };
+  std::vector<Site> per(sites.size());
+  workers().parallel_for(sites.size(), [&](size_t i, int) {
+    const int p = sites[i];
+    const Flat old(sol.begin() + p, sol.begin() + p + sz[p]);
+    std::vector<Flat> cands;
+    if (local_search) gen_change(old, cands); else gen_perturb(old, cands);
+    Site& S = per[i];
+    S.tend.reserve(cands.size());
+    S.kend.reserve(cands.size());
+    S.hash.reserve(cands.size());
+    S.keys.reserve(cands.size() * (full.size() + 16));
+    for (const Flat& c : cands) {
+      S.toks.insert(S.toks.end(), c.begin(), c.end());
+      S.tend.push_back((uint32_t)S.toks.size());
+      const size_t k0 = S.keys.size();
+      S.keys.append(full, 0, be[2 * p]);
+      append_string(c, 0, S.keys);
+      S.keys.append(full, be[2 * p + 1], std::string::npos);
+      S.kend.push_back((uint32_t)S.keys.size());
+      S.hash.push_back(std::hash<std::string_view>{}(std::string_view(S.keys).substr(k0)));
+    }
+  }, 2, 1);
+
+  struct Ref { int32_t site, j; };
+  std::vector<std::string_view> keys;   // in the reference's order
+  std::vector<uint64_t> hashes;
+  std::vector<Ref> refs;
+  for (size_t i = 0; i < per.size(); ++i) {
+    const Site& S = per[i];
+    for (size_t j = 0; j < S.kend.size(); ++j) {
+      const uint32_t k0 = j ? S.kend[j - 1] : 0;
+      keys.emplace_back(S.keys.data() + k0, S.kend[j] - k0);
+      hashes.push_back(S.hash[j]);
+      refs.push_back(Ref{(int32_t)i, (int32_t)j});
+    }
+  }
+  std::vector<Ref> kept;
+  kept.reserve(refs.size());
+  {
+    FirstSeen first(hashes, keys);
+    for (size_t i = 0; i < refs.size(); ++i)
+      if (first.insert((int32_t)i)) kept.push_back(refs[i]);
+  }
+
+  struct Done { int32_t tid; uint32_t t0, t1, s0, s1; uint64_t hash; };
+  std::vector<Done> done(kept.size());
+  for (GenScratch& g : gen_) { g.toks.clear(); g.strs.clear(); }
+  workers().parallel_for(kept.size(), [&](size_t i, int tid) {
+    const Site& S = per[(size_t)kept[i].site];
+    const int p = sites[(size_t)kept[i].site];
+    const int j = kept[i].j;
+    GenScratch& g = gen_[(size_t)tid];
+    Flat& f = g.tree;
+    f.clear();
+    f.insert(f.end(), sol.begin(), sol.begin() + p);
+    f.insert(f.end(), S.toks.begin() + (j ? S.tend[j - 1] : 0), S.toks.begin() + S.tend[j]);
+    f.insert(f.end(), sol.begin() + p + sz[p], sol.end());
+    rewrite_rounds(f, true, pools_[(size_t)tid]);
+    Done& d = done[i];
+    d.tid = tid;
+    d.t0 = (uint32_t)g.toks.size();
+    g.toks.insert(g.toks.end(), f.begin(), f.end());
+    d.t1 = (uint32_t)g.toks.size();
+    d.s0 = (uint32_t)g.strs.size();
+    append_string(f, 0, g.strs);
+    d.s1 = (uint32_t)g.strs.size();
+    d.hash = std::hash<std::string_view>{}(std::string_view(g.strs).substr(d.s0));
   });
-  std::unordered_set<std::string> seen2;
-  seen2.reserve(all.size() * 2);
+
+  std::vector<std::string_view> strs(done.size());
+  std::vector<uint64_t> h2(done.size());
+  for (size_t i = 0; i < done.size(); ++i) {
+    const Done& d = done[i];
+    strs[i] = std::string_view(gen_[(size_t)d.tid].strs.data() + d.s0, d.s1 - d.s0);
+    h2[i] = d.hash;
+  }
+  FirstSeen first2(h2, strs);
   std::vector<Flat> out;
-  out.reserve(all.size());
-  for (size_t i = 0; i < all.size(); ++i) {
-    if (seen2.insert(std::move(strs[i])).second) out.push_back(std::move(all[i]));
+  out.reserve(done.size());
+  for (size_t i = 0; i < done.size(); ++i) {
+    if (!first2.insert((int32_t)i)) continue;
+    const Flat& t = gen_[(size_t)done[i].tid].toks;
+    out.emplace_back(t.begin() + done[i].t0, t.begin() + done[i].t1);
   }
   return out;
 }

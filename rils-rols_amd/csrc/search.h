@@ -154,6 +154,12 @@ class RilsRols {
   int host_threads_ = 0;                  // 0: min(hardware threads, 8)
   Workers* workers_ = nullptr;
   std::vector<Pool> pools_;               // one per worker thread
+  struct alignas(64) GenScratch {         // per worker thread: all_candidates' reusable buffers
+    Flat tree;                            // the candidate being rewritten
+    Flat toks;                            // rewritten candidates, back to back
+    std::string strs;                     // their strings, back to back
+  };
+  std::vector<GenScratch> gen_;           // one per worker thread
   Workers& workers();
 
   void reset();

@@ -58,7 +58,7 @@ struct PNode {
   int32_t l, r;
 };
 
-class Pool {
+class alignas(64) Pool {   // a cache line of its own: each worker thread owns one
  public:
   std::vector<PNode> n;
   void clear() { n.clear(); }

@@ -32,8 +32,10 @@ class Workers {
   int size() const { return n_; }
 
   // f(i, tid) for every i in [0, count), tid in [0, size()); returns when all
-  // items are done.  Small batches run on the calling thread.
-  void parallel_for(size_t count, const std::function<void(size_t, int)>& f, size_t min_items = 32) {
+  // items are done.  Small batches run on the calling thread.  Threads take
+  // chunk consecutive items at a time.
+  void parallel_for(size_t count, const std::function<void(size_t, int)>& f, size_t min_items = 32,
+                    size_t chunk = 4) {
     if (n_ == 1 || count < min_items) {
       for (size_t i = 0; i < count; ++i) f(i, 0);
       return;
@@ -42,6 +44,7 @@ class Workers {
       std::lock_guard<std::mutex> g(m_);
       fn_ = &f;
       count_ = count;
+      chunk_ = chunk < 1 ? 1 : chunk;
       next_.store(0);
       busy_ = n_ - 1;
       ++gen_;
@@ -54,12 +57,11 @@ class Workers {
   }
 
  private:
-  static constexpr size_t CHUNK = 4;
   void work(int tid) {
     for (;;) {
-      const size_t b = next_.fetch_add(CHUNK);
+      const size_t b = next_.fetch_add(chunk_);
       if (b >= count_) break;
-      const size_t e = b + CHUNK < count_ ? b + CHUNK : count_;
+      const size_t e = b + chunk_ < count_ ? b + chunk_ : count_;
       for (size_t i = b; i < e; ++i) (*fn_)(i, tid);
     }
   }
@@ -86,6 +88,7 @@ class Workers {
   std::condition_variable cv_, done_cv_;
   const std::function<void(size_t, int)>* fn_ = nullptr;
   size_t count_ = 0;
+  size_t chunk_ = 4;
   std::atomic<size_t> next_{0};
   int busy_ = 0;
   uint64_t gen_ = 0;
