@@ -1534,9 +1534,14 @@ int fitness_impl(rr_ctx* ctx, int32_t handle, const int32_t* handles, const rr_b
       pm = begin_phase(ctx, RR_PHASE_QR, stream);
       if (ch.ne > 0) {
         Q.n_pos = ch.ne;
-        RR_HIP(rr::launch_qr_init(Q, stream));
-        for (size_t st = 0; st < ch.step_pos.size(); ++st) RR_HIP(rr::launch_qr_step(Q, ch.step_pos[st], ch.step_items[st], stream));
-        RR_HIP(rr::launch_qr_solve(Q, stream));
+        if (ch.step_items[0] == ch.ne) {   // one chunk work item per position
+          RR_HIP(rr::launch_qr_all1(Q, stream));
+        } else {
+          RR_HIP(rr::launch_qr_init(Q, stream));
+          for (size_t st = 0; st < ch.step_pos.size(); ++st)
+            RR_HIP(rr::launch_qr_step(Q, ch.step_pos[st], ch.step_items[st], stream));
+          RR_HIP(rr::launch_qr_solve(Q, stream));
+        }
       }
       const int mc = ch.c1 - ch.c0;
       if (ch.ne < mc) {

@@ -218,6 +218,9 @@ constexpr int64_t EVAL_CHUNK_ROWS = 4096;   // the chunk of the norms' order (rr
 hipError_t launch_qr_init(const QrLaunch& L, hipStream_t s);
 hipError_t launch_qr_step(const QrLaunch& L, int n_act, int n_items, hipStream_t s);
 hipError_t launch_qr_solve(const QrLaunch& L, hipStream_t s);
+// The same QR (init, every step, solve) in one launch when every position has
+// one chunk work item (n <= EVAL_CHUNK_ROWS).
+hipError_t launch_qr_all1(const QrLaunch& L, hipStream_t s);
 hipError_t launch_eval_fp64(const EvalLaunch& L, hipStream_t s);
 // alive[u] = 1 if some candidate users[ustart[u] .. ustart[u+1]-1] has
 // dead == 0, else 0, for u < n

@@ -235,9 +235,7 @@ __device__ inline void ordered_sums(const double* base, int64_t stride, int64_t 
     for (int v = 0; v < NV; ++v) out[v] = out[v] + base[q * stride + v];
 }
 
-__global__ __launch_bounds__(256) void k_qr_init(QrArgs a) {
-  const int pos = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  if (pos >= a.n_pos) return;
+__device__ inline void qr_init_pos(const QrArgs& a, int pos) {
   QrState& s = a.st[pos];
   s.flags = QF_DONE;
   s.nan_exit = -2;   // nothing for k_qr_solve
@@ -306,9 +304,7 @@ __device__ inline void lawn_downdate(const QrArgs& a, int pos, const Cand& c, in
   }
 }
 
-__global__ __launch_bounds__(256) void k_qr_begin(QrArgs a) {
-  const int pos = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  if (pos >= a.n_pos) return;
+__device__ inline void qr_begin_pos(const QrArgs& a, int pos) {
   QrState& s = a.st[pos];
   if (s.flags & QF_DONE) return;
   const int p = s.p;
@@ -558,12 +554,8 @@ __device__ inline void rows_q1(const StepQ& Q, int p, int64_t r0, int64_t r1, F&
   }
 }
 
-__global__ __launch_bounds__(QR_THREADS) void k_qr_dot(QrArgs a) {
-  __shared__ double red[QR_THREADS / WAVE][QR_G];
-  __shared__ double red1[QR_THREADS / WAVE][1];
-  const int item = (int)blockIdx.x;
-  int64_t first;
-  const int pos = chunk_pos(a, item, a.n_pos, &first);
+__device__ inline void qr_dot_item(const QrArgs& a, int pos, int64_t first, int item, double (*red)[QR_G],
+                                   double (*red1)[1]) {
   const QrState& s = a.st[pos];
   if ((s.flags & (QF_PASS | QF_APPLY)) != (QF_PASS | QF_APPLY)) return;   // uniform
   const Cand c = cand_of(a, pos);
@@ -633,9 +625,7 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr_dot(QrArgs a) {
   }
 }
 
-__global__ __launch_bounds__(256) void k_qr_mid(QrArgs a) {
-  const int pos = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  if (pos >= a.n_pos) return;
+__device__ inline void qr_mid_pos(const QrArgs& a, int pos) {
   QrState& s = a.st[pos];
   if (!(s.flags & QF_PASS)) return;
   const Cand c = cand_of(a, pos);
@@ -676,11 +666,7 @@ __global__ __launch_bounds__(256) void k_qr_mid(QrArgs a) {
   }
 }
 
-__global__ __launch_bounds__(QR_THREADS) void k_qr_update(QrArgs a) {
-  __shared__ double red[QR_THREADS / WAVE][2 * QR_G];
-  const int item = (int)blockIdx.x;
-  int64_t first;
-  const int pos = chunk_pos(a, item, a.n_pos, &first);
+__device__ inline void qr_update_item(const QrArgs& a, int pos, int64_t first, int item, double (*red)[2 * QR_G]) {
   const QrState& s = a.st[pos];
   if (!(s.flags & QF_PASS) || (s.flags & QF_NOUPD)) return;   // uniform
   const Cand c = cand_of(a, pos);
@@ -744,9 +730,7 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr_update(QrArgs a) {
   }
 }
 
-__global__ __launch_bounds__(256) void k_qr_end(QrArgs a) {
-  const int pos = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  if (pos >= a.n_pos) return;
+__device__ inline void qr_end_pos(const QrArgs& a, int pos) {
   QrState& s = a.st[pos];
   if (!(s.flags & QF_PASS)) return;
   const Cand c = cand_of(a, pos);
@@ -765,9 +749,7 @@ __global__ __launch_bounds__(256) void k_qr_end(QrArgs a) {
 
 // back substitution on the leading r x r block, coefficients through the
 // column permutation (zeros for the rest); the outcome for the scoring
-__global__ __launch_bounds__(256) void k_qr_solve(QrArgs a) {
-  const int pos = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  if (pos >= a.n_pos) return;
+__device__ inline void qr_solve_pos(const QrArgs& a, int pos) {
   const QrState& s = a.st[pos];
   if (s.nan_exit == -2) return;   // skipped, or decided by factor 0
   const Cand c = cand_of(a, pos);
@@ -806,6 +788,77 @@ __global__ __launch_bounds__(256) void k_qr_solve(QrArgs a) {
 #undef CCS
 #undef TD
 #undef PHYS
+
+__global__ __launch_bounds__(256) void k_qr_init(QrArgs a) {
+  const int pos = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (pos < a.n_pos) qr_init_pos(a, pos);
+}
+__global__ __launch_bounds__(256) void k_qr_begin(QrArgs a) {
+  const int pos = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (pos < a.n_pos) qr_begin_pos(a, pos);
+}
+__global__ __launch_bounds__(QR_THREADS) void k_qr_dot(QrArgs a) {
+  __shared__ double red[QR_THREADS / WAVE][QR_G];
+  __shared__ double red1[QR_THREADS / WAVE][1];
+  const int item = (int)blockIdx.x;
+  int64_t first;
+  const int pos = chunk_pos(a, item, a.n_pos, &first);
+  qr_dot_item(a, pos, first, item, red, red1);
+}
+__global__ __launch_bounds__(256) void k_qr_mid(QrArgs a) {
+  const int pos = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (pos < a.n_pos) qr_mid_pos(a, pos);
+}
+__global__ __launch_bounds__(QR_THREADS) void k_qr_update(QrArgs a) {
+  __shared__ double red[QR_THREADS / WAVE][2 * QR_G];
+  const int item = (int)blockIdx.x;
+  int64_t first;
+  const int pos = chunk_pos(a, item, a.n_pos, &first);
+  qr_update_item(a, pos, first, item, red);
+}
+__global__ __launch_bounds__(256) void k_qr_end(QrArgs a) {
+  const int pos = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (pos < a.n_pos) qr_end_pos(a, pos);
+}
+__global__ __launch_bounds__(256) void k_qr_solve(QrArgs a) {
+  const int pos = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (pos < a.n_pos) qr_solve_pos(a, pos);
+}
+
+// The whole step-synchronous QR of positions that have one chunk each (n <=
+// rrm::NORM_CHUNK): one workgroup per position runs k_qr_init's work, then
+// for each of its k steps (a position with k columns takes part in steps 0 ..
+// k-1) the work of k_qr_begin, k_qr_dot, k_qr_mid, k_qr_update and k_qr_end
+// on its one chunk work item, then k_qr_solve's: the same device code in the
+// same order, one launch instead of five per step.  The per-position parts run
+// on thread 0; the barriers between the parts make its global-memory writes
+// visible to the workgroup.
+__global__ __launch_bounds__(QR_THREADS) void k_qr_all1(QrArgs a) {
+  __shared__ double red[QR_THREADS / WAVE][QR_G];
+  __shared__ double red1[QR_THREADS / WAVE][1];
+  __shared__ double red2[QR_THREADS / WAVE][2 * QR_G];
+  const int pos = (int)blockIdx.x;
+  const int64_t first = first_item(a, pos);
+  const int item = (int)first;
+  const bool t0 = threadIdx.x == 0;
+  if (t0) qr_init_pos(a, pos);
+  __syncthreads();
+  const int k = a.cand_start[a.order[pos] + 1] - a.cand_start[a.order[pos]] + 1;
+  for (int st = 0; st < k; ++st) {
+    if (t0) qr_begin_pos(a, pos);
+    __syncthreads();
+    if (a.st[pos].flags & QF_DONE) break;   // uniform; every later part returns at once
+    qr_dot_item(a, pos, first, item, red, red1);
+    __syncthreads();
+    if (t0) qr_mid_pos(a, pos);
+    __syncthreads();
+    qr_update_item(a, pos, first, item, red2);
+    __syncthreads();
+    if (t0) qr_end_pos(a, pos);
+    __syncthreads();
+  }
+  if (t0) qr_solve_pos(a, pos);
+}
 
 // Factor evaluation for the QR: one workgroup of EV_THREADS threads per
 // (unique factor, chunk of rrm::NORM_CHUNK rows).  A unique factor is one
@@ -1724,6 +1777,12 @@ hipError_t launch_qr_step(const QrLaunch& L, int n_act, int n_items, hipStream_t
   hipLaunchKernelGGL(k_qr_mid, pgrid, pblock, 0, s, a);
   hipLaunchKernelGGL(k_qr_update, cgrid, cblock, 0, s, a);
   hipLaunchKernelGGL(k_qr_end, pgrid, pblock, 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_qr_all1(const QrLaunch& L, hipStream_t s) {
+  if (L.n_pos <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_qr_all1, dim3((unsigned)L.n_pos), dim3(QR_THREADS), 0, s, qr_args(L, L.n_pos));
   return hipGetLastError();
 }
 

@@ -348,6 +348,43 @@ def test_sparse_columns_and_tiny_n_bitexact(ctx, n):
         assert same_bits([f["e0"], f["e1"]], [tu["fit"][0], tu["fit"][1]]), (t, f, tu["fit"])
 
 
+def wide_trees(rng, d, count, lo=10, hi=20):
+    """Sums of lo..hi-1 distinct terms over d variables: candidates of more
+    than ten columns (with the free term), which take the step-synchronous QR."""
+    pool = []
+    for i in range(d):
+        pool += ["x%d" % i, "sin x%d" % i, "cos x%d" % i, "sqr x%d" % i, "exp * c0x1p-2 x%d" % i]
+        pool += ["* x%d x%d" % (i, j) for j in range(i + 1, d)]
+    trees = []
+    for _ in range(count):
+        m = int(rng.integers(lo, hi))
+        terms = [pool[i] for i in rng.choice(len(pool), m, replace=False)]
+        trees.append("+ " * (m - 1) + " ".join(terms))
+    return trees
+
+
+@pytest.mark.parametrize("n", [7, 12, 150, 4096, 4097, 9000])
+def test_wide_candidates_bitexact(ctx, n):
+    """Candidates of 11 to 20 columns: the whole QR in one launch when every
+    candidate's rows fit one chunk (n <= 4096), launches per step beyond;
+    n below the column count stops after n steps."""
+    print("case: wide candidates n=%d" % n, flush=True)
+    rng = np.random.default_rng(90 + n)
+    X, y = make_data(90 + n, n, 4)
+    trees = wide_trees(rng, 4, 40)
+    od = O.Dataset(X, y)
+    tuned = [od.tune(t) for t in trees]
+    assert all(len(tu["factors"]) >= 10 for tu in tuned)
+    arrs = capi.build_batch([tu["factors"] for tu in tuned])
+    h = ctx.upload(X, y)
+    coefs, fit = ctx.fitness(h, arrs, mode="ols")
+    ctx.free(h)
+    for t, tu, c, f in zip(trees, tuned, coefs, fit):
+        assert same_bits(c, tu["coefs"]), (t, c, tu["coefs"])
+        assert f["rank"] == tu["rank"], t
+        assert same_bits([f["e0"], f["e1"]], [tu["fit"][0], tu["fit"][1]]), (t, f, tu["fit"])
+
+
 def test_phase_timing_and_early_flag(ctx):
     """Option phase_timing splits the kernel time of a call by phase; the
     phases of the plain FP64 form (one stream) add up to no more than the
