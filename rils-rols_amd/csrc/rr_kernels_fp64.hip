@@ -830,34 +830,48 @@ __global__ __launch_bounds__(256) void k_qr_solve(QrArgs a) {
 // for each of its k steps (a position with k columns takes part in steps 0 ..
 // k-1) the work of k_qr_begin, k_qr_dot, k_qr_mid, k_qr_update and k_qr_end
 // on its one chunk work item, then k_qr_solve's: the same device code in the
-// same order, one launch instead of five per step.  The per-position parts run
-// on thread 0; the barriers between the parts make its global-memory writes
-// visible to the workgroup.
+// same order, one launch instead of five per step.  The position's state,
+// per-column arrays and chunk sums live in LDS for the launch (nothing reads
+// them afterwards): the serial per-position parts, on thread 0, then wait on
+// LDS rather than on global memory.  The barriers between the parts make each
+// part's writes visible to the next.  Dynamic LDS: 7 ks doubles + ks ints.
 __global__ __launch_bounds__(QR_THREADS) void k_qr_all1(QrArgs a) {
   __shared__ double red[QR_THREADS / WAVE][QR_G];
   __shared__ double red1[QR_THREADS / WAVE][1];
   __shared__ double red2[QR_THREADS / WAVE][2 * QR_G];
+  __shared__ QrState lst;
+  extern __shared__ double qr_lds[];
   const int pos = (int)blockIdx.x;
   const int64_t first = first_item(a, pos);
   const int item = (int)first;
+  const int64_t ks = a.ks;
+  // the same addressing as the global arrays, one position and one item deep
+  // (bases moved back by integer arithmetic: the parts add the offsets again)
+  auto rebase = [](const void* p, int64_t bytes) { return (uintptr_t)p - (uintptr_t)bytes; };
+  QrArgs b = a;
+  b.st = (QrState*)rebase(&lst, pos * (int64_t)sizeof(QrState));
+  b.cold = (double*)rebase(qr_lds, pos * ks * 8);
+  b.col_block = ks;
+  b.part = (double*)rebase(qr_lds + 5 * ks, first * 2 * ks * 8);
+  b.phys = (int32_t*)rebase(qr_lds + 7 * ks, pos * ks * 4);
   const bool t0 = threadIdx.x == 0;
-  if (t0) qr_init_pos(a, pos);
+  if (t0) qr_init_pos(b, pos);
   __syncthreads();
   const int k = a.cand_start[a.order[pos] + 1] - a.cand_start[a.order[pos]] + 1;
   for (int st = 0; st < k; ++st) {
-    if (t0) qr_begin_pos(a, pos);
+    if (t0) qr_begin_pos(b, pos);
     __syncthreads();
-    if (a.st[pos].flags & QF_DONE) break;   // uniform; every later part returns at once
-    qr_dot_item(a, pos, first, item, red, red1);
+    if (lst.flags & QF_DONE) break;   // uniform; every later part returns at once
+    qr_dot_item(b, pos, first, item, red, red1);
     __syncthreads();
-    if (t0) qr_mid_pos(a, pos);
+    if (t0) qr_mid_pos(b, pos);
     __syncthreads();
-    qr_update_item(a, pos, first, item, red2);
+    qr_update_item(b, pos, first, item, red2);
     __syncthreads();
-    if (t0) qr_end_pos(a, pos);
+    if (t0) qr_end_pos(b, pos);
     __syncthreads();
   }
-  if (t0) qr_solve_pos(a, pos);
+  if (t0) qr_solve_pos(b, pos);
 }
 
 // Factor evaluation for the QR: one workgroup of EV_THREADS threads per
@@ -1782,7 +1796,8 @@ hipError_t launch_qr_step(const QrLaunch& L, int n_act, int n_items, hipStream_t
 
 hipError_t launch_qr_all1(const QrLaunch& L, hipStream_t s) {
   if (L.n_pos <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_qr_all1, dim3((unsigned)L.n_pos), dim3(QR_THREADS), 0, s, qr_args(L, L.n_pos));
+  const size_t lds = (size_t)L.ks * (7 * sizeof(double) + sizeof(int32_t));
+  hipLaunchKernelGGL(k_qr_all1, dim3((unsigned)L.n_pos), dim3(QR_THREADS), lds, s, qr_args(L, L.n_pos));
   return hipGetLastError();
 }
 
