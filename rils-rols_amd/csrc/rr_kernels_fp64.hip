@@ -1005,7 +1005,8 @@ __global__ __launch_bounds__(EV_THREADS) void k_xnorm_fp64(const double* X, int6
 }
 
 // The factor-0 pre-pass: one workgroup of NAN0_ROWS threads (one wave) per candidate,
-// one row each, so it costs one evaluation of latency.
+// one row each, so it costs one evaluation of latency (the LDS-stack
+// interpreter: transcendentals inlined, no calls).
 struct Nan0Args {
   const int32_t* cand_start;
   const int32_t* prog_start;
@@ -1020,6 +1021,7 @@ struct Nan0Args {
 
 __global__ __launch_bounds__(NAN0_ROWS) void k_nan0_fp64(Nan0Args a) {
   __shared__ int found;
+  __shared__ double stk[RR_MAX_STACK * NAN0_ROWS];   // eval_lds_f64's operand stack
   const int cand = (int)blockIdx.x;
   const int tid = (int)threadIdx.x;
   const DsDesc& D = a.ds[a.cand_ds ? a.cand_ds[cand] : 0];
@@ -1032,7 +1034,7 @@ __global__ __launch_bounds__(NAN0_ROWS) void k_nan0_fp64(Nan0Args a) {
     const uint32_t* code = a.code + a.prog_start[pbeg];
     const int len = a.prog_start[pbeg + 1] - a.prog_start[pbeg];
     double v[1];
-    eval_f64<1>(code, len, a.K, D.X, D.ldx, (int64_t)tid, v);
+    eval_lds_f64<1, NAN0_ROWS>(code, len, a.K, D.X, D.ldx, (int64_t)tid, v, stk);
     if (__any((int)(tid >= 1 && (int64_t)tid < n && v[0] != v[0])) && (tid & 63) == 0) found = 1;
   }
   __syncthreads();
