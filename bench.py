@@ -626,8 +626,9 @@ def factor_reads_variable(prefix):
 
 
 def executed_flops(factors, ks, E, flags, n, subset=None):
-    """Algorithmic flops F = n (E + k^2 + 5k + 3) (SURVEY of the reference's
-    tune_constants + fitness, each operator counting 1) over the candidates a
+    """Algorithmic flops F = n (E + k^2 + 5k + 3) (the reference's
+    tune_constants + fitness, rils_rols_cpp.cpp:445-541: E operator nodes
+    over the factors, k columns, each operator counting 1) over the candidates a
     launch evaluated in full; a candidate the factor-0 pre-pass settled
     (RR_F_EARLY) is credited only factor 0's operators on the pre-pass's 64
     rows."""

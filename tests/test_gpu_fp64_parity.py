@@ -385,6 +385,35 @@ def test_wide_candidates_bitexact(ctx, n):
         assert same_bits([f["e0"], f["e1"]], [tu["fit"][0], tu["fit"][1]]), (t, f, tu["fit"])
 
 
+@pytest.mark.parametrize("sizes", [(150, 3000), (150, 5000)])
+def test_wide_candidates_of_several_datasets_bitexact(ctx, sizes):
+    """Wide candidates of two datasets interleaved in one launch: each its own
+    dataset's rows, the whole QR in one launch when both fit one chunk, per
+    step when one does not."""
+    print("case: wide candidates of datasets %s" % (sizes,), flush=True)
+    hs, tuned, owner = [], [], []
+    for s, n in enumerate(sizes):
+        rng = np.random.default_rng(190 + n)
+        X, y = make_data(190 + n, n, 3)
+        od = O.Dataset(X, y)
+        tuned.append([od.tune(t) for t in wide_trees(rng, 3, 20, 10, 16)])
+        hs.append(ctx.upload(X, y))
+    mixed = []
+    for i in range(20):
+        for s in range(len(sizes)):
+            mixed.append(tuned[s][i]["factors"])
+            owner.append((s, i))
+    assert all(len(f) >= 10 for f in mixed)
+    coefs, fit = ctx.fitness(None, capi.build_batch(mixed), mode="ols", handles=[hs[s] for s, _ in owner])
+    for h in hs:
+        ctx.free(h)
+    for q, (s, i) in enumerate(owner):
+        tu = tuned[s][i]
+        assert same_bits(coefs[q], tu["coefs"]), (s, i)
+        assert fit[q]["rank"] == tu["rank"], (s, i)
+        assert same_bits([fit[q]["e0"], fit[q]["e1"]], [tu["fit"][0], tu["fit"][1]]), (s, i)
+
+
 def test_phase_timing_and_early_flag(ctx):
     """Option phase_timing splits the kernel time of a call by phase; the
     phases of the plain FP64 form (one stream) add up to no more than the
