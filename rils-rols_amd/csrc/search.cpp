@@ -784,6 +784,13 @@ void RilsRols::fit_inner(const std::vector<std::vector<double>>& X_all, const st
   }
   if (sample_cnt <= 0) throw std::invalid_argument("sample_size * n must be at least one sample");
   std::vector<int> rel = relevant_features(X, y);
+  // The reference appends to its node list on every fit of one object (its
+  // reset() leaves the list alone).  The list is only iterated by the
+  // candidate generators, and every candidate passes the first-occurrence
+  // string dedup, so the copies added only repeat candidates that dedup
+  // drops: clearing it gives the same candidates in the same order, without
+  // the growing generation time of later fits.
+  allowed_.clear();
   setup_nodes(rel);
   if (!gpu_) gpu_ = new GpuSession(backend_, device_, slab_budget_mb_, ctx_options_);
   gpu_->upload(X, y);

@@ -65,6 +65,23 @@ def test_config1_move_sequence_identical(max_fit_calls, graph):
     assert np.array_equal(got, od.eval(rr.get_model_prefix()))
 
 
+def test_refits_of_one_object_repeat_the_moves():
+    """Fitting the same object again starts from the same state: each fit
+    makes the oracle's moves and launches the same candidates."""
+    print("case: three fits of one object, config1, 20000 fit calls", flush=True)
+    import rils_rols_cpp
+    X, y = load_config1()
+    want = O.fit(X, y, max_fit_calls=20000, max_time=BIG_TIME, random_state=12345)
+    n, d = X.shape
+    rr = rils_rols_cpp.rils_rols(False, 20000, BIG_TIME, 0.001, 50, 1.0, False, 12345)
+    cands = []
+    for _ in range(3):
+        rr.fit(np.ascontiguousarray(X).reshape(-1), np.ascontiguousarray(y), n, d)
+        assert_same_run(rr, want)
+        cands.append(rr.get_stats()["batched_cands"])
+    assert cands[0] == cands[1] == cands[2]
+
+
 @pytest.mark.parametrize("seed,n,d,budget", [(1, 300, 3, 8000), (7, 1000, 5, 5000), (23654, 2000, 2, 4000)])
 def test_random_problem_move_sequence_identical(seed, n, d, budget):
     print("case: random problem seed=%d n=%d d=%d budget=%d" % (seed, n, d, budget), flush=True)
