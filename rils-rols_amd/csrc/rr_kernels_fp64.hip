@@ -1288,33 +1288,40 @@ __device__ inline const double* ts_col(const TsArgs& a, const DsDesc& D, int pbe
   return var >= 0 ? D.X + (int64_t)var * D.ldx : a.slab + a.prog_kept[pbeg + col];
 }
 
-// One wave per position: lane 0 does the position's work (norms, outcome of
-// a non-finite step-0 pivot, finite columns, threshold, list slot); then the
-// wave writes the chunk work items' descriptors, a lane per chunk.
-__device__ inline bool ts_init_position(const TsArgs& a, int pos, int* slot_out) {
+// Factor j's column norm (from the evaluation's chunk sums, or the
+// dataset's feature norms for a bare variable) and where its column is.
+__device__ inline void ts_factor(const TsArgs& a, const DsDesc& D, int pbeg, int nf, int j, double* nd,
+                                 const double** col) {
+  const int var = bare_variable(a.code + a.prog_start[pbeg + j], a.prog_start[pbeg + j + 1] - a.prog_start[pbeg + j]);
+  double s0;
+  if (var >= 0) {
+    s0 = D.xnorm[2 * var];
+  } else {
+    const int64_t nch_eval = (D.n + rrm::NORM_CHUNK - 1) / rrm::NORM_CHUNK;
+    double s01[2];
+    ordered_sums<2>(a.norm_part + 2 * (int64_t)a.prog_item[pbeg + j], 2, nch_eval, s01);
+    s0 = s01[0];
+  }
+  *nd = __builtin_sqrt(s0);
+  *col = ts_col(a, D, pbeg, nf, j);
+}
+
+// One wave per position: a lane per factor finds its norm and column
+// (ts_factor, into fnd / fcol); lane 0 then does the position's work (outcome
+// of a non-finite step-0 pivot, finite columns, threshold, list slot); then
+// the wave writes the chunk work items' descriptors, a lane per chunk.
+__device__ inline bool ts_init_position(const TsArgs& a, int pos, int* slot_out, const double* fnd,
+                                        const double* const* fcol) {
   TsState& s = a.st[pos];
   s.live = 0;
   const int cand = a.order[pos];
-  if (a.dead && a.dead[cand]) return false;
   const DsDesc& D = a.ds[a.cand_ds ? a.cand_ds[cand] : 0];
   const int pbeg = a.cand_start[cand];
   const int nf = a.cand_start[cand + 1] - pbeg;
   const int k = nf + 1;
   const int64_t n = D.n;
-  const int64_t nch_eval = (n + rrm::NORM_CHUNK - 1) / rrm::NORM_CHUNK;
   double nd[TS_MAX_COLS];
-  for (int j = 0; j < nf; ++j) {
-    const int var = bare_variable(a.code + a.prog_start[pbeg + j], a.prog_start[pbeg + j + 1] - a.prog_start[pbeg + j]);
-    double s0;
-    if (var >= 0) {
-      s0 = D.xnorm[2 * var];
-    } else {
-      double s01[2];
-      ordered_sums<2>(a.norm_part + 2 * (int64_t)a.prog_item[pbeg + j], 2, nch_eval, s01);
-      s0 = s01[0];
-    }
-    nd[j] = __builtin_sqrt(s0);
-  }
+  for (int j = 0; j < nf; ++j) nd[j] = fnd[j];
   nd[nf] = __builtin_sqrt((double)n);
   const int size = (int)((n < (int64_t)k) ? n : (int64_t)k);
   int bi = 0;
@@ -1339,7 +1346,7 @@ __device__ inline bool ts_init_position(const TsArgs& a, int pos, int* slot_out)
   for (int j = 1; j < k; ++j) mx = (mx < nd[j]) ? nd[j] : mx;
   s.thr = ((mx * DBL_EPS) * (mx * DBL_EPS)) / (double)n;
   for (int j = 0; j < k; ++j) s.nd[j] = nd[j];
-  for (int f = 0; f < kf; ++f) s.col[f] = ts_col(a, D, pbeg, nf, s.fin[f]);
+  for (int f = 0; f < kf; ++f) s.col[f] = s.fin[f] == nf ? D.one : fcol[s.fin[f]];
   s.col[kf] = D.y;
   s.k = k;
   s.kf = kf;
@@ -1360,9 +1367,23 @@ __global__ __launch_bounds__(WAVE) void k_ts_init(TsArgs a) {
   __shared__ int live, slot, kf_s, nch_s, k_s;
   __shared__ int64_t n_s;
   __shared__ const double* col_s[TS_MAX_COLS + 1];
+  __shared__ double fnd[TS_MAX_COLS];
+  __shared__ const double* fcol[TS_MAX_COLS];
+  const int cand = a.order[pos];
+  if (a.dead && a.dead[cand]) {   // uniform
+    if (threadIdx.x == 0) a.st[pos].live = 0;
+    return;
+  }
+  {
+    const DsDesc& D = a.ds[a.cand_ds ? a.cand_ds[cand] : 0];
+    const int pbeg = a.cand_start[cand];
+    const int nf = a.cand_start[cand + 1] - pbeg;   // < TS_MAX_COLS
+    if ((int)threadIdx.x < nf) ts_factor(a, D, pbeg, nf, (int)threadIdx.x, &fnd[threadIdx.x], &fcol[threadIdx.x]);
+  }
+  __syncthreads();
   if (threadIdx.x == 0) {
     int sl = 0;
-    live = ts_init_position(a, pos, &sl) ? 1 : 0;
+    live = ts_init_position(a, pos, &sl, fnd, fcol) ? 1 : 0;
     slot = sl;
     if (live) {
       const TsState& s = a.st[pos];
