@@ -1144,6 +1144,9 @@ __global__ __launch_bounds__(EV_THREADS, 8) void k_score_fp64(ScoreArgs a) {   /
         const int64_t i = base + (int64_t)threadIdx.x + (int64_t)EV_THREADS * r;
         ii[r] = i < r1 ? i : r0;
       }
+      double yy[QR_R];   // issued with the first factor's loads: one wait per pass
+#pragma unroll
+      for (int r = 0; r < QR_R; ++r) yy[r] = yv[ii[r]];
       double yp[QR_R], f[QR_R];
       bool any = false;   // the same for every row
       for (int j = 0; j < nf; ++j) {
@@ -1166,9 +1169,6 @@ __global__ __launch_bounds__(EV_THREADS, 8) void k_score_fp64(ScoreArgs a) {   /
         for (int r = 0; r < QR_R; ++r) yp[r] = any ? yp[r] + term : term;
         any = true;
       }
-      double yy[QR_R];
-#pragma unroll
-      for (int r = 0; r < QR_R; ++r) yy[r] = yv[ii[r]];
 #pragma unroll
       for (int r = 0; r < QR_R; ++r) {
         if (base + (int64_t)threadIdx.x + (int64_t)EV_THREADS * r < r1) {
