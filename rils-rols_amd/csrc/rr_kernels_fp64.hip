@@ -286,22 +286,24 @@ __device__ inline void qr_init_pos(const QrArgs& a, int pos) {
   s.flags = 0;
 }
 
-__device__ inline void lawn_downdate(const QrArgs& a, int pos, const Cand& c, int p, bool recompute_empty) {
+// the norm downdate of column j after step p (each column on its own)
+__device__ inline void lawn_downdate_col(const QrArgs& a, int pos, const Cand& c, int p, int j, bool recompute_empty) {
   const double dd_thr = __builtin_sqrt(DBL_EPS);
-  for (int j = p + 1; j < c.k; ++j) {
-    const double nu = NU(j);
-    if (nu != 0.0) {
-      const double apj = c.S[(int64_t)PHYS(j) * c.ldx + p];
-      double temp = __builtin_fabs(apj) / nu;
-      temp = (1.0 + temp) * (1.0 - temp);
-      temp = temp < 0.0 ? 0.0 : temp;
-      const double ratio = nu / ND(j);
-      const double temp2 = temp * (ratio * ratio);
-      const double rec = recompute_empty ? 0.0 : __builtin_sqrt(CCS(j));
-      if (temp2 <= dd_thr) { ND(j) = rec; NU(j) = rec; }
-      else NU(j) = nu * __builtin_sqrt(temp);
-    }
+  const double nu = NU(j);
+  if (nu != 0.0) {
+    const double apj = c.S[(int64_t)PHYS(j) * c.ldx + p];
+    double temp = __builtin_fabs(apj) / nu;
+    temp = (1.0 + temp) * (1.0 - temp);
+    temp = temp < 0.0 ? 0.0 : temp;
+    const double ratio = nu / ND(j);
+    const double temp2 = temp * (ratio * ratio);
+    const double rec = recompute_empty ? 0.0 : __builtin_sqrt(CCS(j));
+    if (temp2 <= dd_thr) { ND(j) = rec; NU(j) = rec; }
+    else NU(j) = nu * __builtin_sqrt(temp);
   }
+}
+__device__ inline void lawn_downdate(const QrArgs& a, int pos, const Cand& c, int p, bool recompute_empty) {
+  for (int j = p + 1; j < c.k; ++j) lawn_downdate_col(a, pos, c, p, j, recompute_empty);
 }
 
 __device__ inline void qr_begin_pos(const QrArgs& a, int pos) {
@@ -747,6 +749,74 @@ __device__ inline void qr_end_pos(const QrArgs& a, int pos) {
   s.flags = 0;
 }
 
+// k_qr_mid's and k_qr_end's work with a thread per target column, for a
+// workgroup that owns its position (k_qr_all1): each column's arithmetic is
+// that of qr_mid_pos / qr_end_pos, and the columns do not touch each other's
+// values; the position's bookkeeping follows on thread 0 after a barrier.
+// Every thread of the workgroup calls them (the early return is uniform).
+__device__ inline void qr_mid_block(const QrArgs& a, int pos) {
+  QrState& s = a.st[pos];
+  if (!(s.flags & QF_PASS)) return;
+  const Cand c = cand_of(a, pos);
+  const int p = s.p, nrem = c.k - 1 - p, ntarget = nrem + 1;
+  const bool apply = (s.flags & QF_APPLY) != 0;
+  const int64_t first = first_item(a, pos), nch = (c.n + rrm::NORM_CHUNK - 1) / rrm::NORM_CHUNK;
+  for (int t = (int)threadIdx.x; t < ntarget; t += (int)blockDim.x) {
+    if (apply) {
+      double sum;
+      ordered_sums<1>(a.part + first * 2 * a.ks + t, 2 * a.ks, nch, &sum);
+      double* const col = target_col(a, pos, c, p, t, nrem);
+      const double td = sum + col[p];
+      TD(t) = td;
+      col[p] = col[p] - s.tau * td;
+    } else {
+      TD(t) = 0.0;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  if (nrem == 2) {
+    s.qapply = apply ? 1 : 0;
+    s.qtau = s.tau;
+    s.qden = s.den;
+    s.qtd0 = TD(0);
+    s.qtd1 = TD(1);
+    s.qtd2 = TD(2);
+    s.qphys0 = PHYS(p + 1);
+    s.qphys1 = PHYS(p + 2);
+  }
+  if (nrem == 1) {
+    s.papply = apply ? 1 : 0;
+    s.ptau = s.tau;
+    s.pden = s.den;
+    s.ptd0 = TD(0);
+    s.ptd1 = TD(1);
+  }
+  if (s.flags & QF_NOUPD) {
+    s.p = p + 1;
+    s.flags = 0;
+  }
+}
+__device__ inline void qr_end_block(const QrArgs& a, int pos) {
+  QrState& s = a.st[pos];
+  if (!(s.flags & QF_PASS)) return;
+  const Cand c = cand_of(a, pos);
+  const int p = s.p, nrem = c.k - 1 - p;
+  const int64_t first = first_item(a, pos), nch = (c.n + rrm::NORM_CHUNK - 1) / rrm::NORM_CHUNK;
+  for (int t = (int)threadIdx.x; t < nrem; t += (int)blockDim.x) {
+    double s12[2];
+    ordered_sums<2>(a.part + first * 2 * a.ks + 2 * t, 2 * a.ks, nch, s12);
+    CCS(p + 1 + t) = s12[0];
+    TS(p + 1 + t) = s12[1];
+    lawn_downdate_col(a, pos, c, p, p + 1 + t, false);   // column p + 1 + t: its own values only
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    s.p = p + 1;
+    s.flags = 0;
+  }
+}
+
 // back substitution on the leading r x r block, coefficients through the
 // column permutation (zeros for the rest); the outcome for the scoring
 __device__ inline void qr_solve_pos(const QrArgs& a, int pos) {
@@ -830,7 +900,8 @@ __global__ __launch_bounds__(256) void k_qr_solve(QrArgs a) {
 // for each of its k steps (a position with k columns takes part in steps 0 ..
 // k-1) the work of k_qr_begin, k_qr_dot, k_qr_mid, k_qr_update and k_qr_end
 // on its one chunk work item, then k_qr_solve's: the same device code in the
-// same order, one launch instead of five per step.  The position's state,
+// same order, one launch instead of five per step (the mid and end parts with
+// a thread per target column: qr_mid_block, qr_end_block).  The position's state,
 // per-column arrays and chunk sums live in LDS for the launch (nothing reads
 // them afterwards): the serial per-position parts, on thread 0, then wait on
 // LDS rather than on global memory.  The barriers between the parts make each
@@ -864,11 +935,11 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr_all1(QrArgs a) {
     if (lst.flags & QF_DONE) break;   // uniform; every later part returns at once
     qr_dot_item(b, pos, first, item, red, red1);
     __syncthreads();
-    if (t0) qr_mid_pos(b, pos);
+    qr_mid_block(b, pos);
     __syncthreads();
     qr_update_item(b, pos, first, item, red2);
     __syncthreads();
-    if (t0) qr_end_pos(b, pos);
+    qr_end_block(b, pos);
     __syncthreads();
   }
   if (t0) qr_solve_pos(b, pos);
